@@ -1,0 +1,210 @@
+/*
+ * vedas_hip.h -- C ABI of the MI355X (gfx950) VEDAS query-path library
+ * (libvedas_hip.so).
+ *
+ * Plain C: opaque handles, raw device pointers and sizes, int status codes.
+ * Every entry point returns VD_OK (0) or a negative VD_E* code; the message
+ * of the last failure on a context is returned by vd_ctx_last_error().
+ * All device work of a context is issued on that context's single HIP
+ * stream, in call order.  Calls that return a size to the host (join,
+ * select, unique) synchronise the stream once to read it back.
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the reference tree Remixman/Vedas).  The reference
+ * runs these steps through Thrust and ModernGPU on CUDA; here they are
+ * hand-written HIP kernels.
+ */
+#ifndef VEDAS_HIP_H
+#define VEDAS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define VD_OK       0
+#define VD_EINVAL  (-1)   /* bad argument, shape or handle                 */
+#define VD_ENOMEM  (-2)   /* device or host allocation failed              */
+#define VD_EHIP    (-3)   /* a HIP runtime call failed                     */
+#define VD_ERANGE  (-4)   /* size beyond what the kernels support          */
+
+#define VD_MAX_COLS 16    /* columns per relation (the reference caps at 9) */
+#define VD_UNBOUND  0xFFFFFFFFu  /* "no constant" marker in batched lookups */
+
+typedef struct vd_ctx   vd_ctx;    /* device + stream + allocator + profiler */
+typedef struct vd_rel   vd_rel;    /* column-store relation of uint32 IDs    */
+typedef struct vd_index vd_index;  /* HBM-resident triple permutations       */
+
+/* Permutations.  XYZ = sort order; X is implicit through the L1 index, Y and
+ * Z are stored as columns (the reference's xy_data / xyz_data arrays,
+ * VedasStorage.h:87-95). */
+enum {
+    VD_PERM_SPO = 0, VD_PERM_SOP = 1, VD_PERM_PSO = 2,
+    VD_PERM_POS = 3, VD_PERM_OSP = 4, VD_PERM_OPS = 5,
+    VD_NUM_PERMS = 6
+};
+#define VD_PERM_BIT(p) (1u << (p))
+
+/* Device view of one permutation.  Sizes are element counts. */
+typedef struct {
+    uint64_t n;               /* triples                                   */
+    uint64_t n_l1;            /* distinct X                                */
+    uint64_t n_l2;            /* distinct (X,Y)                            */
+    const uint32_t *l1_values, *l1_offsets;   /* distinct X, first row     */
+    const uint32_t *l2_values, *l2_offsets;   /* Y of each (X,Y), first row */
+    const uint32_t *col_y, *col_z;            /* Y and Z per row, sorted   */
+} vd_perm_view_t;
+
+/* Inclusive value interval [lo, hi] removed from a select (the reference's
+ * empty-interval filter, EmptyIntervalDict.cu:108-170). */
+typedef struct { uint32_t lo, hi; } vd_interval_t;
+
+/* ---- context ----------------------------------------------------------- */
+/* Replaces mgpu::standard_context_t (VedasQuery.cu:86) plus the implicit
+ * Thrust device state: one per host thread / request slot. */
+int  vd_ctx_create(int device, vd_ctx **out);
+int  vd_ctx_destroy(vd_ctx *ctx);
+const char *vd_ctx_last_error(const vd_ctx *ctx);
+int  vd_ctx_sync(vd_ctx *ctx);
+void *vd_ctx_stream(vd_ctx *ctx);             /* the hipStream_t          */
+int  vd_ctx_trim(vd_ctx *ctx);                /* release cached blocks    */
+/* Device memory in use / cached by the context's allocator, in bytes. */
+int  vd_ctx_mem(vd_ctx *ctx, uint64_t *in_use, uint64_t *cached);
+/* Tunables: "radix_items" (keys per thread in a radix tile: 8/16/24),
+ * "join_nv" (merge-path diagonal per join expansion block), "join_rcap"
+ * (right keys staged in LDS per bounds block).  Returns VD_EINVAL for an
+ * unknown key or an unsupported value. */
+int  vd_ctx_tune(vd_ctx *ctx, const char *key, int value);
+
+/* ---- device buffers ---------------------------------------------------- */
+int  vd_dev_alloc(vd_ctx *ctx, size_t bytes, void **dptr);
+int  vd_dev_free(vd_ctx *ctx, void *dptr);
+int  vd_copy_h2d(vd_ctx *ctx, void *dst, const void *src, size_t bytes);
+int  vd_copy_d2h(vd_ctx *ctx, void *dst, const void *src, size_t bytes);
+int  vd_copy_d2d(vd_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+/* ---- index build (VedasStorage) ---------------------------------------- */
+/* Replaces VedasStorage::VedasStorage(RdfData&, preload, fullIndex)
+ * (VedasStorage.cu:25-73) with createIndex1Level / createIndex
+ * (VedasStorage.cu:270-358): for each permutation in perm_mask, sort the
+ * (X,Y,Z) triples lexicographically (duplicates kept) and build
+ * L1 = distinct X + first row, L2 = distinct (X,Y) + Y + first row.
+ * d_s/d_p/d_o are device arrays of n uint32 IDs; they are not modified.
+ * The index stays resident in HBM until vd_index_free. */
+int  vd_index_build(vd_ctx *ctx, const uint32_t *d_s, const uint32_t *d_p,
+                    const uint32_t *d_o, uint64_t n, uint32_t perm_mask,
+                    vd_index **out);
+int  vd_index_free(vd_ctx *ctx, vd_index *idx);
+int  vd_index_view(const vd_index *idx, int perm, vd_perm_view_t *out);
+uint32_t vd_index_perms(const vd_index *idx);   /* mask of built perms */
+
+/* Row range [begin, end) of the rows whose X == k1 (and Y == k2 when
+ * has_k2).  Replaces QueryExecutor::findL2OffsetFromL1 and
+ * findDataOffsetFromL2 (QueryExecutor.cu:596-646), with an absent constant
+ * giving an empty range (the reference returns the next key's range). */
+int  vd_index_lookup(vd_ctx *ctx, const vd_index *idx, int perm, uint32_t k1,
+                     int has_k2, uint32_t k2, uint64_t *begin, uint64_t *end);
+/* Batched form on device arrays: d_k2 may be NULL (one constant for every
+ * query) or hold VD_UNBOUND for queries with one constant. */
+int  vd_index_lookup_batch(vd_ctx *ctx, const vd_index *idx, int perm,
+                           const uint32_t *d_k1, const uint32_t *d_k2,
+                           uint64_t nq, uint64_t *d_begin, uint64_t *d_end);
+
+/* ---- relations (FullRelationIR) ---------------------------------------- */
+/* Replaces FullRelationIR(columnNum, relationSize) and its column accessors
+ * (FullRelationIR.cu:19-41, 524-592, 715-745).  Columns are separate device
+ * arrays of nrows uint32; column 0 is the sort key. */
+int  vd_rel_create(vd_ctx *ctx, int ncols, uint64_t nrows, vd_rel **out);
+int  vd_rel_destroy(vd_ctx *ctx, vd_rel *rel);
+int  vd_rel_ncols(const vd_rel *rel);
+uint64_t vd_rel_nrows(const vd_rel *rel);
+uint32_t *vd_rel_col(vd_rel *rel, int c);
+int  vd_rel_swap_cols(vd_rel *rel, int i, int j);   /* swapColumn :715  */
+int  vd_rel_drop_col(vd_ctx *ctx, vd_rel *rel, int i); /* removeColumn :729 */
+
+/* ---- select (SelectQueryJob) -------------------------------------------- */
+/* Materialise rows [begin, end) of a permutation.  cols: 1 = Z only (the
+ * 1-variable select, SelectQueryJob.cu:92-121), 2 = Y only,
+ * 3 = Y then Z (2-variable select, :122-264).  With clip, rows whose Y is
+ * outside [lo, hi] are dropped (variable bound, :130-141); rows whose Y lies
+ * in one of the nskip intervals are dropped (:170-246); with distinct and
+ * cols == 2 adjacent duplicates of Y are removed (:151-158).  clip, skip and
+ * distinct need Y as column 0 (cols 2 or 3) and a range inside one X group. */
+int  vd_select(vd_ctx *ctx, const vd_index *idx, int perm,
+               uint64_t begin, uint64_t end, int cols,
+               int clip, uint32_t lo, uint32_t hi,
+               const vd_interval_t *skip, int nskip, int distinct,
+               vd_rel **out);
+/* Batched range select: lookups for nq queries, then every matching row of
+ * every query written back to back as (Y, Z) pairs into d_out_y / d_out_z
+ * with d_row_offsets[q] = first output row of query q (nq + 1 entries).
+ * When the rows exceed max_rows nothing is written, *total_rows holds the
+ * size needed and VD_ERANGE is returned.  max_rows == 0 only counts. */
+int  vd_select_batch(vd_ctx *ctx, const vd_index *idx, int perm,
+                     const uint32_t *d_k1, const uint32_t *d_k2, uint64_t nq,
+                     uint64_t *d_row_offsets, uint32_t *d_out_y,
+                     uint32_t *d_out_z, uint64_t max_rows,
+                     uint64_t *total_rows);
+
+/* ---- join (JoinQueryJob) ------------------------------------------------ */
+/* Bag inner sort-merge join of L and R on their first nkeys (1 or 2)
+ * columns, both sorted ascending by those columns.  For each left row in
+ * order, the matching right rows in order (left-major, as in
+ * mgpu::inner_join, JoinQueryJob.cu:68-101, 262-269).  Output columns:
+ * [key columns, L columns after the keys, R columns after the keys]
+ * (JoinQueryJob.cu:287-292, 322-380).  Output column 0 stays sorted.
+ * Input relations are not modified. */
+int  vd_join(vd_ctx *ctx, const vd_rel *L, const vd_rel *R, int nkeys,
+             vd_rel **out);
+
+/* ---- re-sort and finalise (FullRelationIR / IndexSwapJob / QueryPlan) --- */
+/* Stable sort of all rows by column 0 only.  Replaces
+ * FullRelationIR::sortByFirstColumn (FullRelationIR.cu:594-713) as used by
+ * IndexSwapJob::startJob (IndexSwapJob.cu:28-95). */
+int  vd_sort_col0(vd_ctx *ctx, vd_rel *rel);
+/* Lexicographic sort of all rows over all columns.  Replaces
+ * FullRelationIR::sort (FullRelationIR.cu:218-336). */
+int  vd_sort_lex(vd_ctx *ctx, vd_rel *rel);
+/* Drop rows equal (in every column) to the row before them; no sort.
+ * Replaces FullRelationIR::removeDuplicate (FullRelationIR.cu:338-522). */
+int  vd_unique_adjacent(vd_ctx *ctx, vd_rel *rel);
+
+/* ---- synthetic triples (host) ------------------------------------------ */
+/* Seeded generator of (S,P,O) IDs: subjects over [1, n_entities] uniform
+ * (subject_zipf == 0) or Zipf(subject_zipf) by rank; predicates over
+ * [1, n_predicates] Zipf(pred_zipf) with ID = rank; objects an entity
+ * (uniform over [1, n_entities]) with probability 1 - literal_frac, else a
+ * literal literal_base + U[0, n_literals).  With distinct, repeated triples
+ * are redrawn until n distinct triples exist.  Host arrays of n entries. */
+typedef struct {
+    uint64_t seed;
+    uint64_t n;
+    uint32_t n_entities;
+    double   subject_zipf;     /* 0 = uniform                               */
+    uint32_t n_predicates;
+    double   pred_zipf;
+    double   literal_frac;
+    uint32_t literal_base;
+    uint32_t n_literals;
+    int      distinct;
+} vd_gen_params_t;
+int  vd_gen_triples(const vd_gen_params_t *prm, uint32_t *s, uint32_t *p,
+                    uint32_t *o);
+
+/* ---- profiling --------------------------------------------------------- */
+/* When enabled, every kernel launch is bracketed by HIP events on the
+ * context stream; per-kernel launch counts, device time and algorithmic
+ * HBM bytes are folded in at the next vd_ctx_sync. */
+int  vd_prof_enable(vd_ctx *ctx, int on);
+int  vd_prof_reset(vd_ctx *ctx);
+int  vd_prof_count(vd_ctx *ctx);
+int  vd_prof_get(vd_ctx *ctx, int i, const char **name, uint64_t *launches,
+                 double *ms, double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VEDAS_HIP_H */

@@ -1,0 +1,64 @@
+/*
+ * vedas_host.h -- C entry points of libvedas.so, the C++ operator API
+ * (VedasStorage, SelectQueryJob, JoinQueryJob, IndexSwapJob, QueryPlan,
+ * QueryExecutor).  They run one query through that API the way the
+ * reference's vdQuery does (VedasQuery.cu:136-205 -> QueryExecutor::query),
+ * for callers that cannot link C++ (Python via ctypes, other FFIs).
+ *
+ * Query text:  SELECT <* | ?v ...> WHERE { s p o . s p o ... } PLAN <node>
+ *   terms: ?name for a variable, a decimal ID for a constant;
+ *   node:  (U <pattern index> <var>)        select (UPLOAD), var = the
+ *                                           variable the index is chosen for
+ *          (J <var> <node> <node>)          join on var
+ *          (S <var> <node>)                 index swap to var
+ *   PLAN may be left out for a single pattern.
+ */
+#ifndef VEDAS_HOST_H
+#define VEDAS_HOST_H
+
+#include <stdint.h>
+
+#include "vedas_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vdq_engine vdq_engine;   /* context + resident storage */
+typedef struct vdq_result vdq_result;   /* final relation + statistics */
+
+typedef struct {
+    uint64_t rows;          /* final rows                               */
+    uint64_t join_rows;     /* sum of join output rows                  */
+    uint64_t select_rows;   /* sum of select output rows                */
+    uint64_t swap_rows;     /* rows re-sorted by index swaps            */
+    double elapsed_ms;      /* host wall time of the query              */
+    double join_ms, select_ms, swap_ms;
+    int jobs;               /* operators executed                       */
+} vdq_stats_t;
+
+int vdq_engine_create(int device, vdq_engine **out);
+int vdq_engine_free(vdq_engine *e);
+/* Build the store from host arrays (perm_mask of VD_PERM_BIT). */
+int vdq_engine_load(vdq_engine *e, const uint32_t *s, const uint32_t *p,
+                    const uint32_t *o, uint64_t n, uint32_t perm_mask);
+/* Build the store from device arrays of the engine's context. */
+int vdq_engine_load_device(vdq_engine *e, const uint32_t *d_s,
+                           const uint32_t *d_p, const uint32_t *d_o,
+                           uint64_t n, uint32_t perm_mask);
+vd_ctx *vdq_engine_ctx(vdq_engine *e);
+vd_index *vdq_engine_index(vdq_engine *e);
+
+int vdq_query(vdq_engine *e, const char *text, vdq_result **out);
+int vdq_result_shape(const vdq_result *r, int *ncols, uint64_t *nrows);
+const char *vdq_result_header(const vdq_result *r, int col);
+int vdq_result_column(const vdq_result *r, int col, uint32_t *host_out);
+int vdq_result_stats(const vdq_result *r, vdq_stats_t *out);
+int vdq_result_free(vdq_result *r);
+/* Message of the calling thread's last failure. */
+const char *vdq_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

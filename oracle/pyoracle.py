@@ -1,0 +1,189 @@
+"""ctypes front of oracle/liboracle.so (the C restatement) plus a naive
+nested-loop BGP evaluator.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg, never by the vedas_amd package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import itertools
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+PERMS = {"SPO": 0, "SOP": 1, "PSO": 2, "POS": 3, "OSP": 4, "OPS": 5}
+PERM_COLS = {"SPO": (0, 1, 2), "SOP": (0, 2, 1), "PSO": (1, 0, 2),
+             "POS": (1, 2, 0), "OSP": (2, 0, 1), "OPS": (2, 1, 0)}
+
+_lib = None
+U32P = C.POINTER(C.c_uint32)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} not built: run python -m vedas_amd.build oracle")
+        l = C.CDLL(LIB_PATH)
+        vp, u64, i = C.c_void_p, C.c_uint64, C.c_int
+        l.or_build_perm.restype = i
+        l.or_build_perm.argtypes = [vp, vp, vp, u64, i, vp, vp, C.POINTER(u64), vp, vp,
+                                    C.POINTER(u64), vp, vp]
+        l.or_lookup.restype = None
+        l.or_lookup.argtypes = [vp, vp, u64, vp, vp, u64, u64, C.c_uint32, i,
+                                C.c_uint32, C.POINTER(u64), C.POINTER(u64)]
+        l.or_select.restype = u64
+        l.or_select.argtypes = [vp, vp, u64, u64, i, i, C.c_uint32, C.c_uint32, vp, i,
+                                i, vp, vp]
+        l.or_join.restype = u64
+        l.or_join.argtypes = [C.POINTER(U32P), u64, i, C.POINTER(U32P), u64, i, i,
+                              C.POINTER(U32P)]
+        l.or_sort_col0.restype = i
+        l.or_sort_col0.argtypes = [C.POINTER(U32P), i, u64]
+        l.or_sort_lex.restype = i
+        l.or_sort_lex.argtypes = [C.POINTER(U32P), i, u64]
+        l.or_unique_adjacent.restype = u64
+        l.or_unique_adjacent.argtypes = [C.POINTER(U32P), i, u64]
+        _lib = l
+    return _lib
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+def _colptrs(cols):
+    arr = (U32P * max(1, len(cols)))()
+    for i, c in enumerate(cols):
+        arr[i] = c.ctypes.data_as(U32P)
+    return arr
+
+
+def build_perm(s, p, o, perm: str) -> dict:
+    n = len(s)
+    m = max(n, 1)
+    out = {k: np.zeros(m, np.uint32) for k in ("l1v", "l1o", "l2v", "l2o", "y", "z")}
+    n1, n2 = C.c_uint64(), C.c_uint64()
+    s, p, o = (np.ascontiguousarray(x, np.uint32) for x in (s, p, o))
+    rc = lib().or_build_perm(_p(s), _p(p), _p(o), n, PERMS[perm], _p(out["l1v"]),
+                             _p(out["l1o"]), C.byref(n1), _p(out["l2v"]), _p(out["l2o"]),
+                             C.byref(n2), _p(out["y"]), _p(out["z"]))
+    assert rc == 0
+    for k in ("l1v", "l1o"):
+        out[k] = out[k][: n1.value].copy()
+    for k in ("l2v", "l2o"):
+        out[k] = out[k][: n2.value].copy()
+    out["y"] = out["y"][:n].copy()
+    out["z"] = out["z"][:n].copy()
+    return out
+
+
+def lookup(ix: dict, k1: int, k2: int | None = None):
+    b, e = C.c_uint64(), C.c_uint64()
+    lib().or_lookup(_p(ix["l1v"]), _p(ix["l1o"]), len(ix["l1v"]), _p(ix["l2v"]),
+                    _p(ix["l2o"]), len(ix["l2v"]), len(ix["y"]), k1,
+                    int(k2 is not None), 0 if k2 is None else k2, C.byref(b), C.byref(e))
+    return b.value, e.value
+
+
+def select(ix: dict, b: int, e: int, cols: int = 3, clip=None, skip=(),
+           distinct: bool = False):
+    m = max(e - b, 1)
+    o0 = np.zeros(m, np.uint32)
+    o1 = np.zeros(m, np.uint32)
+    sk = np.array([v for iv in sorted(skip) for v in iv] or [0], np.uint32)
+    lo, hi = clip if clip is not None else (0, 0)
+    r = lib().or_select(_p(ix["y"]), _p(ix["z"]), b, e, cols, int(clip is not None),
+                        lo, hi, _p(sk), len(skip), int(distinct), _p(o0), _p(o1))
+    if cols == 3:
+        return [o0[:r].copy(), o1[:r].copy()]
+    return [o0[:r].copy()]
+
+
+def join(lcols, rcols, nk: int = 1):
+    lcols = [np.ascontiguousarray(c, np.uint32) for c in lcols]
+    rcols = [np.ascontiguousarray(c, np.uint32) for c in rcols]
+    nl, nr = len(lcols[0]), len(rcols[0])
+    lp, rp = _colptrs(lcols), _colptrs(rcols)
+    n = lib().or_join(lp, nl, len(lcols), rp, nr, len(rcols), nk, None)
+    ncols = len(lcols) + len(rcols) - nk
+    out = [np.zeros(max(n, 1), np.uint32) for _ in range(ncols)]
+    lib().or_join(lp, nl, len(lcols), rp, nr, len(rcols), nk, _colptrs(out))
+    return [c[:n].copy() for c in out]
+
+
+def sort_col0(cols):
+    cols = [np.array(c, np.uint32) for c in cols]
+    assert lib().or_sort_col0(_colptrs(cols), len(cols), len(cols[0])) == 0
+    return cols
+
+
+def sort_lex(cols):
+    cols = [np.array(c, np.uint32) for c in cols]
+    assert lib().or_sort_lex(_colptrs(cols), len(cols), len(cols[0])) == 0
+    return cols
+
+
+def unique_adjacent(cols):
+    cols = [np.array(c, np.uint32) for c in cols]
+    n = lib().or_unique_adjacent(_colptrs(cols), len(cols), len(cols[0]))
+    return [c[:n].copy() for c in cols]
+
+
+# ---- naive evaluator -----------------------------------------------------
+def naive_bgp(s, p, o, patterns):
+    """Bag of bindings of a basic graph pattern by nested loops over the
+    triples (a different algorithm from the plan of selects and joins).
+    patterns: list of (S, P, O) with ints for constants and '?x' strings for
+    variables.  Returns (sorted variable names, sorted list of row tuples)."""
+    triples = list(zip(s.tolist(), p.tolist(), o.tolist()))
+    vars_ = sorted({t for pat in patterns for t in pat if isinstance(t, str)})
+    rows = []
+
+    def rec(k, env):
+        if k == len(patterns):
+            rows.append(tuple(env[v] for v in vars_))
+            return
+        pat = patterns[k]
+        for tr in triples:
+            e2 = dict(env)
+            ok = True
+            for term, val in zip(pat, tr):
+                if isinstance(term, str):
+                    if term in e2 and e2[term] != val:
+                        ok = False
+                        break
+                    e2[term] = val
+                elif term != val:
+                    ok = False
+                    break
+            if ok:
+                rec(k + 1, e2)
+
+    rec(0, {})
+    rows.sort()
+    return vars_, rows
+
+
+def canonical(headers, cols):
+    """Rows of a relation with columns ordered by variable name, sorted."""
+    order = sorted(range(len(headers)), key=lambda i: headers[i])
+    names = [headers[i] for i in order]
+    if len(cols) == 0 or len(cols[0]) == 0:
+        return names, []
+    mat = np.stack([np.asarray(cols[i], np.uint32) for i in order], axis=1)
+    rows = sorted(map(tuple, mat.tolist()))
+    return names, rows
+
+
+def groupby_starts(keys):
+    """(value, first index) of each run of equal consecutive keys."""
+    out = []
+    i = 0
+    for k, g in itertools.groupby(keys):
+        out.append((k, i))
+        i += sum(1 for _ in g)
+    return out

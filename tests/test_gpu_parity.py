@@ -1,0 +1,338 @@
+"""GPU parity: every C-ABI operator against the oracle on the same seeded
+inputs, bit-exact (integer work).  Needs an MI355X."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+from tests import plans
+
+pytestmark = pytest.mark.gpu
+
+ALL_PERMS = ["SPO", "SOP", "PSO", "POS", "OSP", "OPS"]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from vedas_amd._native import Ctx
+    c = Ctx(0)
+    yield c
+    c.close()
+
+
+def cfg0_triples(n=200_000, seed=42, distinct=True):
+    from vedas_amd._native import gen_triples
+    return gen_triples(n, seed=seed, n_entities=100_000, n_predicates=64, pred_zipf=1.1,
+                       literal_frac=0.5, literal_base=50_000_000, n_literals=500_000,
+                       distinct=distinct)
+
+
+def rand_triples(n, seed, ns=300, npred=7, no=400):
+    rng = np.random.default_rng(seed)
+    return (rng.integers(1, ns, n).astype(np.uint32), rng.integers(1, npred, n).astype(np.uint32),
+            rng.integers(1, no, n).astype(np.uint32))
+
+
+def check_index(ctx, s, p, o, perms=ALL_PERMS):
+    from vedas_amd._native import Index
+    idx = Index(ctx, s, p, o, perms)
+    try:
+        for perm in perms:
+            got = idx.arrays(perm)
+            want = O.build_perm(s, p, o, perm)
+            for k in ("l1v", "l1o", "l2v", "l2o", "y", "z"):
+                assert np.array_equal(got[k], want[k]), (perm, k, len(got[k]), len(want[k]))
+    finally:
+        idx.free()
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 4095, 4096, 4097, 65537, 300_001])
+def test_index_build_sizes(ctx, n):
+    print(f"index build n={n}", flush=True)
+    s, p, o = rand_triples(n, n + 1)
+    check_index(ctx, s, p, o)
+
+
+def test_index_build_cfg0(ctx):
+    s, p, o = cfg0_triples()
+    check_index(ctx, s, p, o, ["SPO", "PSO", "POS", "OSP"])
+
+
+def test_index_build_duplicates_and_runs(ctx):
+    n = 50_000
+    s = np.full(n, 7, np.uint32)
+    p = np.full(n, 3, np.uint32)
+    o = (np.arange(n) % 5 + 1).astype(np.uint32)  # heavy duplicates
+    check_index(ctx, s, p, o)
+
+
+def test_index_build_wide_ids(ctx):
+    rng = np.random.default_rng(3)
+    n = 20_000
+    s = rng.integers(1, 1 << 24, n).astype(np.uint32)
+    p = rng.integers(1, 1 << 9, n).astype(np.uint32)
+    o = rng.integers(1, 1 << 31, n).astype(np.uint32)  # 24 + 9 + 31 = 64 bits
+    check_index(ctx, s, p, o, ["PSO", "OSP"])
+
+
+def test_index_build_too_wide_is_an_error(ctx):
+    from vedas_amd._native import Index, VedasError
+    n = 1000
+    big = np.full(n, 0xFFFFFFF0, np.uint32)
+    with pytest.raises(VedasError) as e:
+        Index(ctx, big, big, big, ["SPO"])
+    assert e.value.code == -4
+
+
+@pytest.fixture(scope="module")
+def store(ctx):
+    from vedas_amd._native import Index
+    s, p, o = cfg0_triples()
+    idx = Index(ctx, s, p, o, ["SPO", "PSO", "POS", "OSP"])
+    orc = {perm: O.build_perm(s, p, o, perm) for perm in ["SPO", "PSO", "POS", "OSP"]}
+    yield s, p, o, idx, orc
+    idx.free()
+
+
+def test_lookup_single_and_batch(ctx, store):
+    s, p, o, idx, orc = store
+    rng = np.random.default_rng(5)
+    import ctypes as C
+    from vedas_amd._native import PERMS, VD_UNBOUND
+    for perm in ["SPO", "PSO", "POS", "OSP"]:
+        ix = orc[perm]
+        xs = ix["l1v"]
+        k1 = np.concatenate([rng.choice(xs, 300), np.array([0, 1, xs.max() + 1, 2**32 - 2], np.uint32)]).astype(np.uint32)
+        ys = ix["y"]
+        k2 = np.where(rng.random(len(k1)) < 0.5, VD_UNBOUND, rng.choice(ys, len(k1))).astype(np.uint32)
+        want = [O.lookup(ix, int(a)) if b == VD_UNBOUND else O.lookup(ix, int(a), int(b))
+                for a, b in zip(k1, k2)]
+        for q in range(0, len(k1), 37):
+            kk = None if k2[q] == VD_UNBOUND else int(k2[q])
+            gb, ge = idx.lookup(perm, int(k1[q]), kk)
+            assert (ge - gb) == (want[q][1] - want[q][0])
+            if ge > gb:
+                assert (gb, ge) == want[q]
+        dk1, dk2 = ctx.upload(k1), ctx.upload(k2)
+        db, de = ctx.alloc(8 * len(k1)), ctx.alloc(8 * len(k1))
+        ctx.check(ctx.L.vd_index_lookup_batch(ctx.h, idx.h, PERMS[perm], C.c_void_p(dk1),
+                                              C.c_void_p(dk2), len(k1), C.c_void_p(db),
+                                              C.c_void_p(de)))
+        gb = ctx.download(db, len(k1), np.uint64)
+        ge = ctx.download(de, len(k1), np.uint64)
+        for q in range(len(k1)):
+            assert ge[q] - gb[q] == want[q][1] - want[q][0], q
+            if want[q][1] > want[q][0]:
+                assert (gb[q], ge[q]) == want[q]
+        for d in (dk1, dk2, db, de):
+            ctx.free(d)
+
+
+def test_select_modes(ctx, store):
+    s, p, o, idx, orc = store
+    ix = orc["PSO"]
+    for pred in [1, 2, 5, 17, 64]:
+        b, e = O.lookup(ix, pred)
+        for cols in (1, 2, 3):
+            rel = idx.select("PSO", b, e, cols)
+            want = O.select(ix, b, e, cols)
+            got = rel.columns()
+            rel.free()
+            assert all(np.array_equal(g, w) for g, w in zip(got, want)), (pred, cols)
+        rel = idx.select("PSO", b, e, 2, distinct=True)
+        got = rel.columns()
+        rel.free()
+        assert np.array_equal(got[0], O.select(ix, b, e, 2, distinct=True)[0])
+        ys = ix["y"][b:e]
+        if len(ys):
+            lo, hi = int(np.percentile(ys, 20)), int(np.percentile(ys, 70))
+            sk = [(lo + 10, lo + 500), (hi - 300, hi - 100)]
+            for clip, skip in [((lo, hi), ()), ((lo, hi), sk), (None, sk), ((hi, lo), ())]:
+                rel = idx.select("PSO", b, e, 3, clip=clip, skip=skip)
+                got = rel.columns()
+                rel.free()
+                want = O.select(ix, b, e, 3, clip=clip, skip=skip)
+                assert all(np.array_equal(g, w) for g, w in zip(got, want)), (pred, clip, skip)
+
+
+def test_select_batch(ctx, store):
+    import ctypes as C
+    from vedas_amd._native import PERMS, VD_UNBOUND
+    s, p, o, idx, orc = store
+    rng = np.random.default_rng(9)
+    nq = 1000
+    pick = rng.integers(0, len(s), nq)
+    perm = "SPO"
+    ix = orc[perm]
+    k1 = s[pick].astype(np.uint32)
+    k2 = np.where(rng.random(nq) < 0.5, VD_UNBOUND, p[pick]).astype(np.uint32)
+    want_y, want_z, sizes = [], [], []
+    for a, bb in zip(k1, k2):
+        b, e = O.lookup(ix, int(a)) if bb == VD_UNBOUND else O.lookup(ix, int(a), int(bb))
+        want_y.append(ix["y"][b:e])
+        want_z.append(ix["z"][b:e])
+        sizes.append(e - b)
+    total = int(sum(sizes))
+    dk1, dk2 = ctx.upload(k1), ctx.upload(k2)
+    doff = ctx.alloc(8 * (nq + 1))
+    dy, dz = ctx.alloc(4 * total), ctx.alloc(4 * total)
+    tot = C.c_uint64()
+    ctx.check(ctx.L.vd_select_batch(ctx.h, idx.h, PERMS[perm], C.c_void_p(dk1), C.c_void_p(dk2), nq,
+                                    C.c_void_p(doff), C.c_void_p(dy), C.c_void_p(dz), total,
+                                    C.byref(tot)))
+    assert tot.value == total
+    off = ctx.download(doff, nq + 1, np.uint64)
+    assert np.array_equal(off, np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64))
+    assert np.array_equal(ctx.download(dy, total), np.concatenate(want_y))
+    assert np.array_equal(ctx.download(dz, total), np.concatenate(want_z))
+    # too small an output -> VD_ERANGE and the needed size
+    rc = ctx.L.vd_select_batch(ctx.h, idx.h, PERMS[perm], C.c_void_p(dk1), C.c_void_p(dk2), nq,
+                               C.c_void_p(doff), C.c_void_p(dy), C.c_void_p(dz), max(1, total - 1),
+                               C.byref(tot))
+    assert rc == -4 and tot.value == total
+    for d in (dk1, dk2, doff, dy, dz):
+        ctx.free(d)
+
+
+def make_rel(rng, n, ncols, keymax, nk=1):
+    cols = [rng.integers(0, keymax, n).astype(np.uint32) for _ in range(ncols)]
+    return O.sort_lex(cols) if nk == 2 else O.sort_col0(cols)
+
+
+@pytest.mark.parametrize("nk", [1, 2])
+@pytest.mark.parametrize("shape", [(1000, 1500, 50), (20_000, 30_000, 3000), (5000, 7000, 3),
+                                   (100_000, 80_000, 200_000), (1, 1, 1)])
+def test_join_parity(ctx, nk, shape):
+    from vedas_amd._native import Rel
+    nl, nr, keymax = shape
+    print(f"join nk={nk} shape={shape}", flush=True)
+    rng = np.random.default_rng(nl + nr + nk)
+    L = make_rel(rng, nl, 3, keymax, nk)
+    R = make_rel(rng, nr, 2 + (nk - 1), keymax, nk)
+    want = O.join(L, R, nk)
+    gl, gr = Rel.from_columns(ctx, L), Rel.from_columns(ctx, R)
+    out = gl.join(gr, nk)
+    got = out.columns()
+    assert out.nrows == len(want[0])
+    for c, (g, w) in enumerate(zip(got, want)):
+        assert np.array_equal(g, w), c
+    for r in (gl, gr, out):
+        r.free()
+
+
+@pytest.mark.parametrize("nv,rcap", [(1024, 2048), (2048, 4096), (4096, 8192)])
+def test_join_tunables_and_skew(ctx, nv, rcap):
+    from vedas_amd._native import Rel
+    ctx.tune("join_nv", nv)
+    ctx.tune("join_rcap", rcap)
+    try:
+        rng = np.random.default_rng(nv)
+        # one hot key on both sides + a uniform tail, plus left keys absent on the right
+        lk = np.concatenate([np.full(3000, 5), rng.integers(10, 100_000, 40_000)])
+        rk = np.concatenate([np.full(2500, 5), rng.integers(50, 100_000, 60_000)])
+        L = O.sort_col0([lk.astype(np.uint32), rng.integers(0, 9, len(lk)).astype(np.uint32)])
+        R = O.sort_col0([rk.astype(np.uint32), rng.integers(0, 9, len(rk)).astype(np.uint32)])
+        want = O.join(L, R, 1)
+        gl, gr = Rel.from_columns(ctx, L), Rel.from_columns(ctx, R)
+        out = gl.join(gr, 1)
+        got = out.columns()
+        assert out.nrows == len(want[0]) and out.nrows > 7_500_000
+        for g, w in zip(got, want):
+            assert np.array_equal(g, w)
+        for r in (gl, gr, out):
+            r.free()
+    finally:
+        ctx.tune("join_nv", 2048)
+        ctx.tune("join_rcap", 4096)
+
+
+def test_join_empty_sides(ctx):
+    from vedas_amd._native import Rel
+    rng = np.random.default_rng(1)
+    R = make_rel(rng, 100, 2, 10)
+    E = [np.zeros(0, np.uint32), np.zeros(0, np.uint32)]
+    gl, gr = Rel.from_columns(ctx, E), Rel.from_columns(ctx, R)
+    out = gl.join(gr, 1)
+    assert out.nrows == 0 and out.ncols == 3
+    out2 = gr.join(gl, 1)
+    assert out2.nrows == 0
+    # disjoint keys
+    A = O.sort_col0([np.arange(0, 100, 2, dtype=np.uint32)])
+    B = O.sort_col0([np.arange(1, 100, 2, dtype=np.uint32)])
+    ga, gb = Rel.from_columns(ctx, A), Rel.from_columns(ctx, B)
+    out3 = ga.join(gb, 1)
+    assert out3.nrows == 0
+    for r in (gl, gr, out, out2, ga, gb, out3):
+        r.free()
+
+
+@pytest.mark.parametrize("n,ncols,keymax", [(0, 2, 5), (1, 3, 5), (1000, 1, 50), (70_001, 3, 1000),
+                                            (300_000, 5, 2**31), (4096, 2, 1)])
+def test_sorts_and_unique(ctx, n, ncols, keymax):
+    from vedas_amd._native import Rel
+    rng = np.random.default_rng(n + ncols)
+    cols = [rng.integers(0, keymax, n).astype(np.uint32) for _ in range(ncols)]
+    r = Rel.from_columns(ctx, cols)
+    r.sort_col0()
+    want = O.sort_col0(cols)
+    for g, w in zip(r.columns(), want):
+        assert np.array_equal(g, w)
+    r.free()
+    r = Rel.from_columns(ctx, cols)
+    r.sort_lex()
+    want = O.sort_lex(cols)
+    for g, w in zip(r.columns(), want):
+        assert np.array_equal(g, w)
+    r.unique_adjacent()
+    want = O.unique_adjacent(want)
+    assert r.nrows == len(want[0])
+    for g, w in zip(r.columns(), want):
+        assert np.array_equal(g, w)
+    r.free()
+    r = Rel.from_columns(ctx, cols)
+    r.unique_adjacent()
+    want = O.unique_adjacent(cols)
+    for g, w in zip(r.columns(), want):
+        assert np.array_equal(g, w)
+    r.free()
+
+
+@pytest.mark.parametrize("shape", ["star2", "star4", "chain4", "snow"])
+def test_plans_cfg0_store(ctx, store, shape):
+    s, p, o, idx, orc = store
+    if shape == "star2":
+        pats, node = plans.star_plan([1, 2])
+    elif shape == "star4":
+        pats, node = plans.star_plan([3, 4, 5, 6])
+    elif shape == "chain4":
+        pats, node = plans.chain_plan([10, 11, 12, 13])
+    else:
+        pats, node = plans.snowflake_plan([8, 9], [10, 11])
+    gex = plans.GpuExecutor(ctx, idx)
+    rel = gex.run(node)
+    gnames, grows = O.canonical(rel.headers, gex.columns(rel))
+    gex.free()
+    oex = plans.OracleExecutor(s, p, o)
+    orel = oex.run(node)
+    onames, orows = O.canonical(orel.headers, oex.columns(orel))
+    assert gnames == onames
+    assert grows == orows
+    assert len(grows) > 0
+
+
+def test_plan_small_store_naive(ctx):
+    from vedas_amd._native import Index
+    rng = np.random.default_rng(4)
+    trip = sorted(set(zip(rng.integers(1, 25, 300).tolist(), rng.integers(1, 5, 300).tolist(),
+                          rng.integers(1, 25, 300).tolist())))
+    s, p, o = (np.array(c, np.uint32) for c in zip(*trip))
+    idx = Index(ctx, s, p, o, ["PSO", "POS"])
+    try:
+        for pats, node in [plans.star_plan([1, 2, 3]), plans.chain_plan([1, 2, 3]),
+                           plans.snowflake_plan([1, 2], [3, 4])]:
+            gex = plans.GpuExecutor(ctx, idx)
+            rel = gex.run(node)
+            got = O.canonical(rel.headers, gex.columns(rel))
+            gex.free()
+            assert got == O.naive_bgp(s, p, o, pats)
+    finally:
+        idx.free()

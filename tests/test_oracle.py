@@ -1,0 +1,204 @@
+"""Pins the oracle (C restatement) with checks that do not share its code:
+Python sorting, linear scans, nested-loop joins, a naive BGP evaluator and
+the committed golden fixtures.  CPU only."""
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+from tests import plans
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+PERM_NAMES = ["SPO", "SOP", "PSO", "POS", "OSP", "OPS"]
+
+
+def rand_triples(n, seed, ns=30, npred=5, no=40):
+    rng = np.random.default_rng(seed)
+    s = rng.integers(1, ns + 1, n).astype(np.uint32)
+    p = rng.integers(1, npred + 1, n).astype(np.uint32)
+    o = rng.integers(1, no + 1, n).astype(np.uint32)
+    return s, p, o
+
+
+@pytest.mark.parametrize("perm", PERM_NAMES)
+@pytest.mark.parametrize("seed", [0, 1])
+def test_build_perm_is_sorted_tuples(perm, seed):
+    s, p, o = rand_triples(500, seed)  # duplicates included
+    ix = O.build_perm(s, p, o, perm)
+    cols = (s, p, o)
+    xi, yi, zi = O.PERM_COLS[perm]
+    tup = sorted(zip(cols[xi].tolist(), cols[yi].tolist(), cols[zi].tolist()))
+    assert ix["y"].tolist() == [t[1] for t in tup]
+    assert ix["z"].tolist() == [t[2] for t in tup]
+    l1 = O.groupby_starts([t[0] for t in tup])
+    assert ix["l1v"].tolist() == [k for k, _ in l1]
+    assert ix["l1o"].tolist() == [i for _, i in l1]
+    l2 = O.groupby_starts([(t[0], t[1]) for t in tup])
+    assert ix["l2v"].tolist() == [k[1] for k, _ in l2]
+    assert ix["l2o"].tolist() == [i for _, i in l2]
+
+
+def test_build_perm_empty():
+    e = np.zeros(0, np.uint32)
+    ix = O.build_perm(e, e, e, "PSO")
+    assert all(len(v) == 0 for v in ix.values())
+
+
+@pytest.mark.parametrize("perm", ["PSO", "POS", "SPO", "OSP"])
+def test_lookup_matches_linear_scan(perm):
+    s, p, o = rand_triples(400, 7)
+    ix = O.build_perm(s, p, o, perm)
+    cols = (s, p, o)
+    xi, yi, _ = O.PERM_COLS[perm]
+    tup = sorted(zip(cols[xi].tolist(), cols[yi].tolist(), cols[O.PERM_COLS[perm][2]].tolist()))
+    xs = sorted(set(cols[xi].tolist()))
+    probes = xs + [0, max(xs) + 1, 10_000]
+    for k1 in probes:
+        rows = [i for i, t in enumerate(tup) if t[0] == k1]
+        b, e = O.lookup(ix, k1)
+        if rows:
+            assert (b, e) == (rows[0], rows[-1] + 1)
+        else:
+            assert b == e
+        for k2 in list(range(0, 45, 3)):
+            rows2 = [i for i, t in enumerate(tup) if t[0] == k1 and t[1] == k2]
+            b2, e2 = O.lookup(ix, k1, k2)
+            if rows2:
+                assert (b2, e2) == (rows2[0], rows2[-1] + 1)
+            else:
+                assert b2 == e2
+
+
+def test_select_matches_filter():
+    s, p, o = rand_triples(600, 3)
+    ix = O.build_perm(s, p, o, "PSO")
+    tup = sorted(zip(p.tolist(), s.tolist(), o.tolist()))
+    for pred in range(1, 6):
+        b, e = O.lookup(ix, pred)
+        rng = tup[b:e]
+        y, z = O.select(ix, b, e, 3)
+        assert list(zip(y.tolist(), z.tolist())) == [(t[1], t[2]) for t in rng]
+        (zz,) = O.select(ix, b, e, 1)
+        assert zz.tolist() == [t[2] for t in rng]
+        (d,) = O.select(ix, b, e, 2, distinct=True)
+        assert d.tolist() == [k for k, _ in itertools.groupby(t[1] for t in rng)]
+        lo, hi = 5, 20
+        y, z = O.select(ix, b, e, 3, clip=(lo, hi))
+        assert list(zip(y.tolist(), z.tolist())) == [(t[1], t[2]) for t in rng if lo <= t[1] <= hi]
+        skip = [(3, 6), (12, 12), (25, 27)]
+        y, z = O.select(ix, b, e, 3, clip=(2, 28), skip=skip)
+        keep = [(t[1], t[2]) for t in rng if 2 <= t[1] <= 28
+                and not any(a <= t[1] <= c for a, c in skip)]
+        assert list(zip(y.tolist(), z.tolist())) == keep
+        y, z = O.select(ix, b, e, 3, clip=(9, 4))
+        assert len(y) == 0
+
+
+def nested_loop_join(L, R, nk):
+    """Left-major bag join by nested loops; R rows in their order."""
+    out = []
+    nl, nr = len(L[0]), len(R[0])
+    for i in range(nl):
+        kl = tuple(int(L[c][i]) for c in range(nk))
+        for j in range(nr):
+            if tuple(int(R[c][j]) for c in range(nk)) == kl:
+                out.append(tuple(int(L[c][i]) for c in range(len(L)))
+                           + tuple(int(R[c][j]) for c in range(nk, len(R))))
+    return out
+
+
+@pytest.mark.parametrize("nk", [1, 2])
+@pytest.mark.parametrize("seed", [0, 5, 9])
+def test_join_matches_nested_loop(nk, seed):
+    rng = np.random.default_rng(seed)
+    def rel(n, ncols):
+        cols = [rng.integers(0, 12, n).astype(np.uint32) for _ in range(ncols)]
+        return O.sort_lex(cols) if nk == 2 else O.sort_col0(cols)
+    L, R = rel(80, 3), rel(120, 2 + nk - 1)
+    got = O.join(L, R, nk)
+    want = nested_loop_join(L, R, nk)
+    assert list(zip(*[c.tolist() for c in got])) == want
+    assert O.join([np.zeros(0, np.uint32)] * 2, R, nk)[0].size == 0
+
+
+def test_sorts_and_unique():
+    rng = np.random.default_rng(11)
+    cols = [rng.integers(0, 6, 300).astype(np.uint32) for _ in range(3)]
+    got = O.sort_col0(cols)
+    rows = list(zip(*[c.tolist() for c in cols]))
+    assert list(zip(*[c.tolist() for c in got])) == sorted(rows, key=lambda r: r[0])
+    got = O.sort_lex(cols)
+    assert list(zip(*[c.tolist() for c in got])) == sorted(rows)
+    srt = O.sort_lex(cols)
+    got = O.unique_adjacent(srt)
+    assert list(zip(*[c.tolist() for c in got])) == [k for k, _ in itertools.groupby(sorted(rows))]
+    got = O.unique_adjacent(cols)  # unsorted: adjacent only
+    assert list(zip(*[c.tolist() for c in got])) == [k for k, _ in itertools.groupby(rows)]
+
+
+def small_store(seed=4):
+    s, p, o = rand_triples(300, seed, ns=25, npred=4, no=25)
+    # distinct triples, as the parity contract asks
+    uniq = sorted(set(zip(s.tolist(), p.tolist(), o.tolist())))
+    s, p, o = (np.array(c, np.uint32) for c in zip(*uniq))
+    return s, p, o
+
+
+@pytest.mark.parametrize("shape", ["star2", "star3", "chain3", "snow"])
+def test_plan_equals_naive_bgp(shape):
+    s, p, o = small_store()
+    if shape == "star2":
+        pats, node = plans.star_plan([1, 2])
+    elif shape == "star3":
+        pats, node = plans.star_plan([1, 2, 3])
+    elif shape == "chain3":
+        pats, node = plans.chain_plan([1, 2, 3])
+    else:
+        pats, node = plans.snowflake_plan([1, 2], [3, 4])
+    ex = plans.OracleExecutor(s, p, o)
+    rel = ex.run(node)
+    names, rows = O.canonical(rel.headers, ex.columns(rel))
+    vars_, want = O.naive_bgp(s, p, o, pats)
+    assert names == vars_
+    assert rows == want
+    assert len(rows) > 0
+
+
+def test_bound_constant_patterns():
+    s, p, o = small_store()
+    obj = int(o[0])
+    pats = [("?s", int(p[0]), obj), ("?s", 2, "?y")]
+    node = ("JOIN", ("UPLOAD", pats[0], "?s"), ("UPLOAD", pats[1], "?s"), "?s")
+    ex = plans.OracleExecutor(s, p, o)
+    rel = ex.run(node)
+    names, rows = O.canonical(rel.headers, ex.columns(rel))
+    assert (names, rows) == O.naive_bgp(s, p, o, pats)
+
+
+def test_golden_fixtures():
+    path = os.path.join(GOLDEN, "index_small.npz")
+    g = np.load(path, allow_pickle=False)
+    for perm in ["SPO", "PSO", "POS", "OSP"]:
+        ix = O.build_perm(g["s"], g["p"], g["o"], perm)
+        for k in ("l1v", "l1o", "l2v", "l2o", "y", "z"):
+            assert np.array_equal(ix[k], g[f"{perm}_{k}"]), (perm, k)
+    j = np.load(os.path.join(GOLDEN, "join_small.npz"), allow_pickle=False)
+    got = O.join([j["l0"], j["l1"]], [j["r0"], j["r1"]], 1)
+    for c in range(3):
+        assert np.array_equal(got[c], j[f"out{c}"])
+
+
+def test_planted_mistake_is_caught():
+    """A wrong join (right rows in reverse order) must fail the comparison
+    the GPU tests use."""
+    rng = np.random.default_rng(2)
+    L = O.sort_col0([rng.integers(0, 5, 40).astype(np.uint32), rng.integers(0, 9, 40).astype(np.uint32)])
+    R = O.sort_col0([rng.integers(0, 5, 40).astype(np.uint32), rng.integers(0, 9, 40).astype(np.uint32)])
+    good = O.join(L, R, 1)
+    bad = [c.copy() for c in good]
+    bad[2] = bad[2][::-1].copy()
+    want = nested_loop_join(L, R, 1)
+    assert list(zip(*[c.tolist() for c in good])) == want
+    assert list(zip(*[c.tolist() for c in bad])) != want

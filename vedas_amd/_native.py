@@ -1,0 +1,350 @@
+"""ctypes binding of the C ABI in include/vedas_hip.h (libvedas_hip.so).
+
+Thin and strict: a missing library raises at import of the binding, every
+non-zero status raises VedasError with the context's last-error message.
+There is no CPU fallback anywhere in this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvedas_hip.so")
+
+VD_OK, VD_EINVAL, VD_ENOMEM, VD_EHIP, VD_ERANGE = 0, -1, -2, -3, -4
+VD_MAX_COLS = 16
+VD_UNBOUND = 0xFFFFFFFF
+PERMS = {"SPO": 0, "SOP": 1, "PSO": 2, "POS": 3, "OSP": 4, "OPS": 5}
+PERM_NAMES = {v: k for k, v in PERMS.items()}
+
+
+class VedasError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"vedas error {code}: {msg}")
+        self.code = code
+
+
+class PermView(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("n_l1", C.c_uint64), ("n_l2", C.c_uint64),
+                ("l1_values", C.c_void_p), ("l1_offsets", C.c_void_p),
+                ("l2_values", C.c_void_p), ("l2_offsets", C.c_void_p),
+                ("col_y", C.c_void_p), ("col_z", C.c_void_p)]
+
+
+class Interval(C.Structure):
+    _fields_ = [("lo", C.c_uint32), ("hi", C.c_uint32)]
+
+
+class GenParams(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("n", C.c_uint64),
+                ("n_entities", C.c_uint32), ("subject_zipf", C.c_double),
+                ("n_predicates", C.c_uint32), ("pred_zipf", C.c_double),
+                ("literal_frac", C.c_double), ("literal_base", C.c_uint32),
+                ("n_literals", C.c_uint32), ("distinct", C.c_int)]
+
+
+EXPORTS = {
+    # name: (restype, argtypes)
+    "vd_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "vd_ctx_destroy": (C.c_int, [C.c_void_p]),
+    "vd_ctx_last_error": (C.c_char_p, [C.c_void_p]),
+    "vd_ctx_sync": (C.c_int, [C.c_void_p]),
+    "vd_ctx_stream": (C.c_void_p, [C.c_void_p]),
+    "vd_ctx_trim": (C.c_int, [C.c_void_p]),
+    "vd_ctx_mem": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "vd_ctx_tune": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
+    "vd_dev_alloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
+    "vd_dev_free": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "vd_copy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    "vd_copy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    "vd_copy_d2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    "vd_index_build": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "vd_index_free": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "vd_index_view": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(PermView)]),
+    "vd_index_perms": (C.c_uint32, [C.c_void_p]),
+    "vd_index_lookup": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_uint32,
+                                  C.c_int, C.c_uint32, C.POINTER(C.c_uint64),
+                                  C.POINTER(C.c_uint64)]),
+    "vd_index_lookup_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                        C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]),
+    "vd_rel_create": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "vd_rel_destroy": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "vd_rel_ncols": (C.c_int, [C.c_void_p]),
+    "vd_rel_nrows": (C.c_uint64, [C.c_void_p]),
+    "vd_rel_col": (C.c_void_p, [C.c_void_p, C.c_int]),
+    "vd_rel_swap_cols": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "vd_rel_drop_col": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "vd_select": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_uint64, C.c_uint64,
+                            C.c_int, C.c_int, C.c_uint32, C.c_uint32,
+                            C.POINTER(Interval), C.c_int, C.c_int,
+                            C.POINTER(C.c_void_p)]),
+    "vd_select_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                  C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "vd_join": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                          C.POINTER(C.c_void_p)]),
+    "vd_sort_col0": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "vd_sort_lex": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "vd_unique_adjacent": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "vd_gen_triples": (C.c_int, [C.POINTER(GenParams), C.c_void_p, C.c_void_p,
+                                 C.c_void_p]),
+    "vd_prof_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "vd_prof_reset": (C.c_int, [C.c_void_p]),
+    "vd_prof_count": (C.c_int, [C.c_void_p]),
+    "vd_prof_get": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_char_p),
+                              C.POINTER(C.c_uint64), C.POINTER(C.c_double),
+                              C.POINTER(C.c_double)]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libvedas_hip.so (built in-tree); raise if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} not built: run python -m vedas_amd.build")
+        l = C.CDLL(LIB_PATH)
+        for name, (res, args) in EXPORTS.items():
+            f = getattr(l, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = l
+    return _lib
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    assert a.flags["C_CONTIGUOUS"]
+    return C.c_void_p(a.ctypes.data)
+
+
+def gen_triples(n: int, seed: int = 42, n_entities: int = 100_000,
+                subject_zipf: float = 0.0, n_predicates: int = 64,
+                pred_zipf: float = 1.1, literal_frac: float = 0.5,
+                literal_base: int = 50_000_000, n_literals: int = 500_000,
+                distinct: bool = True):
+    """Seeded synthetic triples (host arrays)."""
+    prm = GenParams(seed, n, n_entities, subject_zipf, n_predicates, pred_zipf,
+                    literal_frac, literal_base, n_literals, int(distinct))
+    s = np.empty(n, np.uint32)
+    p = np.empty(n, np.uint32)
+    o = np.empty(n, np.uint32)
+    rc = lib().vd_gen_triples(C.byref(prm), _ptr(s), _ptr(p), _ptr(o))
+    if rc != VD_OK:
+        raise VedasError(rc, "vd_gen_triples failed")
+    return s, p, o
+
+
+class Ctx:
+    """A vd_ctx: one device, one stream."""
+
+    def __init__(self, device: int = 0):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.vd_ctx_create(device, C.byref(h))
+        if rc != VD_OK:
+            raise VedasError(rc, "vd_ctx_create failed (no GPU?)")
+        self.h = h
+
+    def check(self, rc: int) -> None:
+        if rc != VD_OK:
+            msg = self.L.vd_ctx_last_error(self.h)
+            raise VedasError(rc, msg.decode() if msg else "")
+
+    def close(self) -> None:
+        if self.h:
+            self.L.vd_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sync(self) -> None:
+        self.check(self.L.vd_ctx_sync(self.h))
+
+    def tune(self, key: str, value: int) -> None:
+        self.check(self.L.vd_ctx_tune(self.h, key.encode(), value))
+
+    def mem(self):
+        a, b = C.c_uint64(), C.c_uint64()
+        self.check(self.L.vd_ctx_mem(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    # -- device buffers
+    def alloc(self, nbytes: int) -> int:
+        p = C.c_void_p()
+        self.check(self.L.vd_dev_alloc(self.h, nbytes, C.byref(p)))
+        return p.value
+
+    def free(self, dptr: int) -> None:
+        self.check(self.L.vd_dev_free(self.h, C.c_void_p(dptr)))
+
+    def upload(self, a: np.ndarray) -> int:
+        a = np.ascontiguousarray(a)
+        d = self.alloc(max(a.nbytes, 4))
+        self.check(self.L.vd_copy_h2d(self.h, C.c_void_p(d), _ptr(a), a.nbytes))
+        return d
+
+    def download(self, dptr: int, n: int, dtype=np.uint32) -> np.ndarray:
+        out = np.empty(n, dtype)
+        if n:
+            self.check(self.L.vd_copy_d2h(self.h, _ptr(out), C.c_void_p(dptr), out.nbytes))
+        return out
+
+    # -- profiling
+    def prof(self, on: bool = True) -> None:
+        self.check(self.L.vd_prof_enable(self.h, int(on)))
+
+    def prof_reset(self) -> None:
+        self.check(self.L.vd_prof_reset(self.h))
+
+    def prof_stats(self) -> dict:
+        n = self.L.vd_prof_count(self.h)
+        if n < 0:
+            self.check(n)
+        out = {}
+        for i in range(n):
+            nm, la, ms, by = C.c_char_p(), C.c_uint64(), C.c_double(), C.c_double()
+            self.check(self.L.vd_prof_get(self.h, i, C.byref(nm), C.byref(la),
+                                          C.byref(ms), C.byref(by)))
+            out[nm.value.decode()] = {"launches": la.value, "ms": ms.value,
+                                      "bytes": by.value}
+        return out
+
+
+class Rel:
+    """A vd_rel handle owned by Python."""
+
+    def __init__(self, ctx: Ctx, handle: C.c_void_p):
+        self.ctx, self.h = ctx, handle
+
+    @classmethod
+    def from_columns(cls, ctx: Ctx, cols) -> "Rel":
+        cols = [np.ascontiguousarray(c, dtype=np.uint32) for c in cols]
+        n = len(cols[0])
+        h = C.c_void_p()
+        ctx.check(ctx.L.vd_rel_create(ctx.h, len(cols), n, C.byref(h)))
+        r = cls(ctx, h)
+        for i, c in enumerate(cols):
+            assert len(c) == n
+            if n:
+                ctx.check(ctx.L.vd_copy_h2d(ctx.h, C.c_void_p(r.col_ptr(i)), _ptr(c), c.nbytes))
+        return r
+
+    @property
+    def ncols(self) -> int:
+        return self.ctx.L.vd_rel_ncols(self.h)
+
+    @property
+    def nrows(self) -> int:
+        return self.ctx.L.vd_rel_nrows(self.h)
+
+    def col_ptr(self, i: int) -> int:
+        p = self.ctx.L.vd_rel_col(self.h, i)
+        if not p:
+            raise VedasError(VD_EINVAL, f"no column {i}")
+        return p
+
+    def columns(self) -> list:
+        return [self.ctx.download(self.col_ptr(i), self.nrows) for i in range(self.ncols)]
+
+    def swap_cols(self, i: int, j: int) -> None:
+        self.ctx.check(self.ctx.L.vd_rel_swap_cols(self.h, i, j))
+
+    def drop_col(self, i: int) -> None:
+        self.ctx.check(self.ctx.L.vd_rel_drop_col(self.ctx.h, self.h, i))
+
+    def sort_col0(self) -> None:
+        self.ctx.check(self.ctx.L.vd_sort_col0(self.ctx.h, self.h))
+
+    def sort_lex(self) -> None:
+        self.ctx.check(self.ctx.L.vd_sort_lex(self.ctx.h, self.h))
+
+    def unique_adjacent(self) -> None:
+        self.ctx.check(self.ctx.L.vd_unique_adjacent(self.ctx.h, self.h))
+
+    def join(self, other: "Rel", nkeys: int = 1) -> "Rel":
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.L.vd_join(self.ctx.h, self.h, other.h, nkeys, C.byref(h)))
+        return Rel(self.ctx, h)
+
+    def free(self) -> None:
+        if self.h:
+            self.ctx.L.vd_rel_destroy(self.ctx.h, self.h)
+            self.h = None
+
+
+class Index:
+    """HBM-resident permutations built from device triples."""
+
+    def __init__(self, ctx: Ctx, s, p, o, perms=("SPO", "PSO", "POS", "OSP")):
+        self.ctx = ctx
+        n = len(s)
+        self.n = n
+        ds, dp, do = ctx.upload(s), ctx.upload(p), ctx.upload(o)
+        mask = 0
+        for name in perms:
+            mask |= 1 << PERMS[name]
+        h = C.c_void_p()
+        try:
+            ctx.check(ctx.L.vd_index_build(ctx.h, C.c_void_p(ds), C.c_void_p(dp),
+                                           C.c_void_p(do), n, mask, C.byref(h)))
+        finally:
+            for d in (ds, dp, do):
+                ctx.free(d)
+        self.h = h
+
+    @classmethod
+    def from_device(cls, ctx: Ctx, ds: int, dp: int, do: int, n: int, perms):
+        self = cls.__new__(cls)
+        self.ctx, self.n = ctx, n
+        mask = 0
+        for name in perms:
+            mask |= 1 << PERMS[name]
+        h = C.c_void_p()
+        ctx.check(ctx.L.vd_index_build(ctx.h, C.c_void_p(ds), C.c_void_p(dp),
+                                       C.c_void_p(do), n, mask, C.byref(h)))
+        self.h = h
+        return self
+
+    def view(self, perm: str) -> PermView:
+        v = PermView()
+        self.ctx.check(self.ctx.L.vd_index_view(self.h, PERMS[perm], C.byref(v)))
+        return v
+
+    def arrays(self, perm: str) -> dict:
+        v = self.view(perm)
+        d = self.ctx.download
+        return {"l1v": d(v.l1_values, v.n_l1), "l1o": d(v.l1_offsets, v.n_l1),
+                "l2v": d(v.l2_values, v.n_l2), "l2o": d(v.l2_offsets, v.n_l2),
+                "y": d(v.col_y, v.n), "z": d(v.col_z, v.n)}
+
+    def lookup(self, perm: str, k1: int, k2: int | None = None):
+        b, e = C.c_uint64(), C.c_uint64()
+        self.ctx.check(self.ctx.L.vd_index_lookup(
+            self.ctx.h, self.h, PERMS[perm], k1, int(k2 is not None),
+            0 if k2 is None else k2, C.byref(b), C.byref(e)))
+        return b.value, e.value
+
+    def select(self, perm: str, begin: int, end: int, cols: int = 3, clip=None,
+               skip=(), distinct: bool = False) -> Rel:
+        sk = (Interval * max(1, len(skip)))(*[Interval(a, b) for a, b in skip])
+        lo, hi = clip if clip is not None else (0, 0)
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.L.vd_select(
+            self.ctx.h, self.h, PERMS[perm], begin, end, cols,
+            int(clip is not None), lo, hi, sk, len(skip), int(distinct), C.byref(h)))
+        return Rel(self.ctx, h)
+
+    def free(self) -> None:
+        if self.h:
+            self.ctx.L.vd_index_free(self.ctx.h, self.h)
+            self.h = None

@@ -1,0 +1,355 @@
+// Context, allocator, profiler, buffers and relation handles of libvedas_hip.
+
+#include <algorithm>
+#include <cstring>
+
+#include "vd_internal.hpp"
+
+namespace vd {
+
+// ---- allocator ---------------------------------------------------------------
+// Size classes: powers of two up to 64 MiB, then 32 MiB multiples.  Blocks
+// are kept per context and handed out again in stream order.
+static size_t size_class(size_t bytes) {
+    if (bytes < 256) bytes = 256;
+    const size_t big = size_t(64) << 20;
+    if (bytes <= big) {
+        size_t c = 256;
+        while (c < bytes) c <<= 1;
+        return c;
+    }
+    const size_t step = size_t(32) << 20;
+    return (bytes + step - 1) / step * step;
+}
+
+static void release_cached(vd_ctx *ctx) {
+    if (ctx->free_blocks.empty()) return;
+    VD_HIP(hipStreamSynchronize(ctx->stream));
+    for (auto &kv : ctx->free_blocks) (void)hipFree(kv.second);
+    ctx->free_blocks.clear();
+    ctx->bytes_cached = 0;
+}
+
+void *dalloc(vd_ctx *ctx, size_t bytes) {
+    const size_t c = size_class(bytes);
+    auto it = ctx->free_blocks.find(c);
+    void *p = nullptr;
+    if (it != ctx->free_blocks.end()) {
+        p = it->second;
+        ctx->free_blocks.erase(it);
+        ctx->bytes_cached -= c;
+    } else {
+        hipError_t e = hipMalloc(&p, c);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            release_cached(ctx);
+            e = hipMalloc(&p, c);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                throw Error(VD_ENOMEM, "device allocation of " +
+                                           std::to_string(c) + " bytes failed");
+            }
+        }
+    }
+    ctx->live_blocks[p] = c;
+    ctx->bytes_live += c;
+    return p;
+}
+
+void dfree(vd_ctx *ctx, void *p) {
+    if (!p) return;
+    auto it = ctx->live_blocks.find(p);
+    if (it == ctx->live_blocks.end()) return;  // not ours: ignore
+    const size_t c = it->second;
+    ctx->live_blocks.erase(it);
+    ctx->bytes_live -= c;
+    ctx->free_blocks.emplace(c, p);
+    ctx->bytes_cached += c;
+}
+
+// ---- profiling -----------------------------------------------------------------
+hipEvent_t take_event(vd_ctx *ctx) {
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    VD_HIP(hipEventCreate(&e));
+    return e;
+}
+
+void prof_begin(vd_ctx *ctx, hipEvent_t *ev) {
+    *ev = take_event(ctx);
+    VD_HIP(hipEventRecord(*ev, ctx->stream));
+}
+
+void prof_end(vd_ctx *ctx, const char *name, double bytes, hipEvent_t ev0) {
+    hipEvent_t ev1 = take_event(ctx);
+    VD_HIP(hipEventRecord(ev1, ctx->stream));
+    ctx->pending.push_back({name, bytes, ev0, ev1});
+}
+
+void prof_collect(vd_ctx *ctx) {
+    for (auto &p : ctx->pending) {
+        float ms = 0.f;
+        VD_HIP(hipEventElapsedTime(&ms, p.start, p.stop));
+        auto &st = ctx->prof_stats[p.name];
+        st.launches += 1;
+        st.ms += ms;
+        st.bytes += p.bytes;
+        ctx->event_pool.push_back(p.start);
+        ctx->event_pool.push_back(p.stop);
+    }
+    ctx->pending.clear();
+}
+
+void sync(vd_ctx *ctx) {
+    VD_HIP(hipStreamSynchronize(ctx->stream));
+    if (!ctx->pending.empty()) prof_collect(ctx);
+}
+
+uint64_t read_u64(vd_ctx *ctx, const uint64_t *dptr) {
+    VD_HIP(hipMemcpyAsync(ctx->h_pinned, dptr, sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+    return ctx->h_pinned[0];
+}
+
+void read_words(vd_ctx *ctx, uint64_t *dst, const uint64_t *dptr, int n) {
+    VD_REQUIRE(n <= 64, VD_EINVAL, "read_words: too many words");
+    VD_HIP(hipMemcpyAsync(ctx->h_pinned, dptr, n * sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+    std::memcpy(dst, ctx->h_pinned, n * sizeof(uint64_t));
+}
+
+void copy_d2d(vd_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return;
+    VD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+}
+
+// ---- relations -----------------------------------------------------------------
+vd_rel *rel_new(vd_ctx *ctx, int ncols, uint64_t nrows) {
+    VD_REQUIRE(ncols >= 1 && ncols <= VD_MAX_COLS, VD_EINVAL,
+               "relation column count out of range");
+    vd_rel *r = new vd_rel();
+    r->ncols = ncols;
+    r->nrows = nrows;
+    try {
+        for (int c = 0; c < ncols; ++c)
+            r->cols[c] = static_cast<uint32_t *>(
+                dalloc(ctx, (nrows ? nrows : 1) * sizeof(uint32_t)));
+    } catch (...) {
+        rel_free(ctx, r);
+        throw;
+    }
+    return r;
+}
+
+void rel_free(vd_ctx *ctx, vd_rel *r) {
+    if (!r) return;
+    for (int c = 0; c < r->ncols; ++c) dfree(ctx, r->cols[c]);
+    delete r;
+}
+
+}  // namespace vd
+
+// ---- C ABI ---------------------------------------------------------------------
+using vd::Error;
+
+extern "C" {
+
+int vd_ctx_create(int device, vd_ctx **out) {
+    if (!out) return VD_EINVAL;
+    *out = nullptr;
+    vd_ctx *ctx = new (std::nothrow) vd_ctx();
+    if (!ctx) return VD_ENOMEM;
+    try {
+        int ndev = 0;
+        VD_HIP(hipGetDeviceCount(&ndev));
+        VD_REQUIRE(device >= 0 && device < ndev, VD_EINVAL, "no such device");
+        ctx->device = device;
+        VD_HIP(hipSetDevice(device));
+        VD_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        VD_HIP(hipHostMalloc((void **)&ctx->h_pinned, 64 * sizeof(uint64_t),
+                             hipHostMallocDefault));
+    } catch (const Error &e) {
+        std::fprintf(stderr, "vd_ctx_create: %s\n", e.what());
+        int code = e.code;
+        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return code;
+    }
+    *out = ctx;
+    return VD_OK;
+}
+
+int vd_ctx_destroy(vd_ctx *ctx) {
+    if (!ctx) return VD_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &p : ctx->pending) {
+        (void)hipEventDestroy(p.start);
+        (void)hipEventDestroy(p.stop);
+    }
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    for (auto &kv : ctx->free_blocks) (void)hipFree(kv.second);
+    for (auto &kv : ctx->live_blocks) (void)hipFree(kv.first);
+    if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return VD_OK;
+}
+
+const char *vd_ctx_last_error(const vd_ctx *ctx) {
+    return ctx ? ctx->last_error.c_str() : "null context";
+}
+
+int vd_ctx_sync(vd_ctx *ctx) {
+    if (!ctx) return VD_EINVAL;
+    VD_TRY(ctx, vd::sync(ctx));
+}
+
+void *vd_ctx_stream(vd_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int vd_ctx_trim(vd_ctx *ctx) {
+    if (!ctx) return VD_EINVAL;
+    VD_TRY(ctx, vd::release_cached(ctx));
+}
+
+int vd_ctx_mem(vd_ctx *ctx, uint64_t *in_use, uint64_t *cached) {
+    if (!ctx) return VD_EINVAL;
+    if (in_use) *in_use = ctx->bytes_live;
+    if (cached) *cached = ctx->bytes_cached;
+    return VD_OK;
+}
+
+int vd_ctx_tune(vd_ctx *ctx, const char *key, int value) {
+    if (!ctx || !key) return VD_EINVAL;
+    std::string k(key);
+    if (k == "radix_items") {
+        if (value != 8 && value != 16 && value != 24) return VD_EINVAL;
+        ctx->radix_items = value;
+    } else if (k == "join_nv") {
+        if (value != 1024 && value != 2048 && value != 4096) return VD_EINVAL;
+        ctx->join_nv = value;
+    } else if (k == "join_rcap") {
+        if (value != 2048 && value != 4096 && value != 8192) return VD_EINVAL;
+        ctx->join_rcap = value;
+    } else {
+        ctx->last_error = "unknown tunable " + k;
+        return VD_EINVAL;
+    }
+    return VD_OK;
+}
+
+int vd_dev_alloc(vd_ctx *ctx, size_t bytes, void **dptr) {
+    if (!ctx || !dptr) return VD_EINVAL;
+    VD_TRY(ctx, *dptr = vd::dalloc(ctx, bytes));
+}
+
+int vd_dev_free(vd_ctx *ctx, void *dptr) {
+    if (!ctx) return VD_EINVAL;
+    VD_TRY(ctx, vd::dfree(ctx, dptr));
+}
+
+int vd_copy_h2d(vd_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx) return VD_EINVAL;
+    if (bytes == 0) return VD_OK;
+    VD_TRY(ctx, {
+        VD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        vd::sync(ctx);
+    });
+}
+
+int vd_copy_d2h(vd_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx) return VD_EINVAL;
+    if (bytes == 0) return VD_OK;
+    VD_TRY(ctx, {
+        VD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        vd::sync(ctx);
+    });
+}
+
+int vd_copy_d2d(vd_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx) return VD_EINVAL;
+    VD_TRY(ctx, vd::copy_d2d(ctx, dst, src, bytes));
+}
+
+int vd_rel_create(vd_ctx *ctx, int ncols, uint64_t nrows, vd_rel **out) {
+    if (!ctx || !out) return VD_EINVAL;
+    VD_TRY(ctx, *out = vd::rel_new(ctx, ncols, nrows));
+}
+
+int vd_rel_destroy(vd_ctx *ctx, vd_rel *rel) {
+    if (!ctx) return VD_EINVAL;
+    VD_TRY(ctx, vd::rel_free(ctx, rel));
+}
+
+int vd_rel_ncols(const vd_rel *rel) { return rel ? rel->ncols : 0; }
+uint64_t vd_rel_nrows(const vd_rel *rel) { return rel ? rel->nrows : 0; }
+
+uint32_t *vd_rel_col(vd_rel *rel, int c) {
+    if (!rel || c < 0 || c >= rel->ncols) return nullptr;
+    return rel->cols[c];
+}
+
+int vd_rel_swap_cols(vd_rel *rel, int i, int j) {
+    if (!rel || i < 0 || j < 0 || i >= rel->ncols || j >= rel->ncols)
+        return VD_EINVAL;
+    std::swap(rel->cols[i], rel->cols[j]);
+    return VD_OK;
+}
+
+int vd_rel_drop_col(vd_ctx *ctx, vd_rel *rel, int i) {
+    if (!ctx || !rel || i < 0 || i >= rel->ncols || rel->ncols == 1)
+        return VD_EINVAL;
+    vd::dfree(ctx, rel->cols[i]);
+    for (int c = i; c + 1 < rel->ncols; ++c) rel->cols[c] = rel->cols[c + 1];
+    rel->cols[rel->ncols - 1] = nullptr;
+    rel->ncols -= 1;
+    return VD_OK;
+}
+
+int vd_prof_enable(vd_ctx *ctx, int on) {
+    if (!ctx) return VD_EINVAL;
+    ctx->prof = on != 0;
+    return VD_OK;
+}
+
+int vd_prof_reset(vd_ctx *ctx) {
+    if (!ctx) return VD_EINVAL;
+    VD_TRY(ctx, {
+        vd::sync(ctx);
+        ctx->prof_stats.clear();
+        ctx->prof_names.clear();
+    });
+}
+
+int vd_prof_count(vd_ctx *ctx) {
+    if (!ctx) return VD_EINVAL;
+    try {
+        vd::sync(ctx);
+    } catch (const Error &e) {
+        ctx->last_error = e.what();
+        return e.code;
+    }
+    ctx->prof_names.clear();
+    for (auto &kv : ctx->prof_stats) ctx->prof_names.push_back(kv.first);
+    return (int)ctx->prof_names.size();
+}
+
+int vd_prof_get(vd_ctx *ctx, int i, const char **name, uint64_t *launches,
+                double *ms, double *bytes) {
+    if (!ctx || i < 0 || i >= (int)ctx->prof_names.size()) return VD_EINVAL;
+    const auto &nm = ctx->prof_names[i];
+    const auto &st = ctx->prof_stats[nm];
+    if (name) *name = nm.c_str();
+    if (launches) *launches = st.launches;
+    if (ms) *ms = st.ms;
+    if (bytes) *bytes = st.bytes;
+    return VD_OK;
+}
+
+}  // extern "C"

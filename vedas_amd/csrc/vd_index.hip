@@ -1,0 +1,583 @@
+// Index build (triple permutations with L1/L2 offset indexes), range
+// lookup and range select.
+//
+// Build of one permutation XYZ:
+//   1. k_pack_xyz      : key = X << (bY+bZ) | Y << bZ | Z   (the whole
+//                        triple in one 64-bit word; bX+bY+bZ <= 64) and the
+//                        digit histograms of every radix pass.
+//   2. radix_sort_u64  : LSD onesweep passes over bits [0, bX+bY+bZ).
+//   3. k_unpack_count  : Y and Z columns, plus per-block counts of L1 heads
+//                        (X changes) and L2 heads ((X,Y) changes).
+//   4. scan of the block counts.
+//   5. k_compact_heads : L1 = (X, first row), L2 = (Y, first row) entries.
+// The resulting arrays match the reference's createIndex1Level /
+// createIndex output (VedasStorage.cu:270-358): L1 values/offsets are the
+// distinct leading IDs and their first rows, L2 values/offsets the second
+// IDs of every distinct (X,Y) pair and their first rows, and the Y and Z
+// columns are the reference's xy_data / xyz_data arrays.
+
+#include <algorithm>
+
+#include "vd_radix_dev.hpp"
+
+struct vd_perm_arrays {
+    bool built = false;
+    uint64_t n = 0, n_l1 = 0, n_l2 = 0;
+    uint32_t *l1v = nullptr, *l1o = nullptr, *l2v = nullptr, *l2o = nullptr;
+    uint32_t *y = nullptr, *z = nullptr;
+};
+
+struct vd_index {
+    vd_ctx *owner = nullptr;
+    uint64_t n = 0;
+    uint32_t mask = 0;
+    vd_perm_arrays perm[VD_NUM_PERMS];
+};
+
+namespace vd {
+namespace {
+
+constexpr int PK_BLOCK = radix::HIST_BLOCK;
+constexpr int PK_ITEMS = 16;
+constexpr int UN_BLOCK = 256;
+constexpr int UN_ITEMS = 8;
+constexpr int UN_TILE = UN_BLOCK * UN_ITEMS;
+
+// Which of (s, p, o) is X, Y, Z for each permutation.
+constexpr int PERM_COLS[VD_NUM_PERMS][3] = {
+    {0, 1, 2},  // SPO
+    {0, 2, 1},  // SOP
+    {1, 0, 2},  // PSO
+    {1, 2, 0},  // POS
+    {2, 0, 1},  // OSP
+    {2, 1, 0},  // OPS
+};
+
+__global__ __launch_bounds__(PK_BLOCK) void k_pack_xyz(
+    const uint32_t *__restrict__ x, const uint32_t *__restrict__ y,
+    const uint32_t *__restrict__ z, uint64_t n, int sy, int sx,
+    uint64_t *__restrict__ keys, int kbits, int npass, uint32_t *ghist) {
+    __shared__ radix::LdsHist s;
+    radix::hist_zero(&s);
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * (PK_BLOCK * PK_ITEMS);
+#pragma unroll 4
+    for (int j = 0; j < PK_ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)j * PK_BLOCK + threadIdx.x;
+        if (i < n) {
+            const uint64_t key = ((uint64_t)x[i] << sx) | ((uint64_t)y[i] << sy) |
+                                 (uint64_t)z[i];
+            keys[i] = key;
+            radix::hist_add(&s, key, 0, kbits, npass);
+        }
+    }
+    __syncthreads();
+    radix::hist_flush(&s, ghist, npass);
+}
+
+// Columns Y and Z out of the sorted keys; per-block head counts.
+__global__ __launch_bounds__(UN_BLOCK) void k_unpack_count(
+    const uint64_t *__restrict__ keys, uint64_t n, int sy, int sx,
+    uint32_t ymask, uint32_t zmask, uint32_t *__restrict__ ycol,
+    uint32_t *__restrict__ zcol, uint32_t *__restrict__ cnt_l1,
+    uint32_t *__restrict__ cnt_l2) {
+    __shared__ uint32_t s_tmp[UN_BLOCK / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * UN_TILE;
+    uint32_t c1 = 0, c2 = 0;
+#pragma unroll
+    for (int j = 0; j < UN_ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)j * UN_BLOCK + threadIdx.x;
+        if (i < n) {
+            const uint64_t k = keys[i];
+            ycol[i] = (uint32_t)(k >> sy) & ymask;
+            zcol[i] = (uint32_t)k & zmask;
+            if (i == 0) {
+                c1 += 1; c2 += 1;
+            } else {
+                const uint64_t pk = keys[i - 1];
+                c1 += (k >> sx) != (pk >> sx);
+                c2 += (k >> sy) != (pk >> sy);
+            }
+        }
+    }
+    const uint32_t t1 = dev::block_reduce<UN_BLOCK>(c1, s_tmp);
+    const uint32_t t2 = dev::block_reduce<UN_BLOCK>(c2, s_tmp);
+    if (threadIdx.x == 0) {
+        cnt_l1[blockIdx.x] = t1;
+        cnt_l2[blockIdx.x] = t2;
+    }
+}
+
+__global__ __launch_bounds__(UN_BLOCK) void k_compact_heads(
+    const uint64_t *__restrict__ keys, uint64_t n, int sy, int sx,
+    uint32_t ymask, const uint64_t *__restrict__ off_l1,
+    const uint64_t *__restrict__ off_l2, uint32_t *__restrict__ l1v,
+    uint32_t *__restrict__ l1o, uint32_t *__restrict__ l2v,
+    uint32_t *__restrict__ l2o) {
+    __shared__ uint32_t s_tmp[UN_BLOCK / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * UN_TILE;
+    // Each thread owns UN_ITEMS consecutive rows so the block scan keeps row
+    // order.
+    bool h1[UN_ITEMS], h2[UN_ITEMS];
+    uint64_t kk[UN_ITEMS];
+    uint32_t c1 = 0, c2 = 0;
+#pragma unroll
+    for (int j = 0; j < UN_ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)threadIdx.x * UN_ITEMS + j;
+        h1[j] = h2[j] = false;
+        kk[j] = 0;
+        if (i < n) {
+            const uint64_t k = keys[i];
+            kk[j] = k;
+            if (i == 0) {
+                h1[j] = h2[j] = true;
+            } else {
+                const uint64_t pk = keys[i - 1];
+                h1[j] = (k >> sx) != (pk >> sx);
+                h2[j] = (k >> sy) != (pk >> sy);
+            }
+            c1 += h1[j];
+            c2 += h2[j];
+        }
+    }
+    uint32_t tot;
+    uint32_t p1 = dev::block_excl_scan<UN_BLOCK>(c1, s_tmp, &tot);
+    uint32_t p2 = dev::block_excl_scan<UN_BLOCK>(c2, s_tmp, &tot);
+    uint64_t o1 = off_l1[blockIdx.x] + p1, o2 = off_l2[blockIdx.x] + p2;
+#pragma unroll
+    for (int j = 0; j < UN_ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)threadIdx.x * UN_ITEMS + j;
+        if (h1[j]) {
+            l1v[o1] = (uint32_t)(kk[j] >> sx);
+            l1o[o1] = (uint32_t)i;
+            ++o1;
+        }
+        if (h2[j]) {
+            l2v[o2] = (uint32_t)(kk[j] >> sy) & ymask;
+            l2o[o2] = (uint32_t)i;
+            ++o2;
+        }
+    }
+}
+
+// ---- lookups -------------------------------------------------------------------
+struct PermDev {
+    uint64_t n, n_l1;
+    const uint32_t *l1v, *l1o, *y;
+};
+
+__device__ __forceinline__ void lookup_one(const PermDev &pd, uint32_t k1,
+                                           uint32_t k2, bool has_k2,
+                                           uint64_t *b, uint64_t *e) {
+    const uint64_t i = dev::lower_bound(pd.l1v, 0, pd.n_l1, k1);
+    if (i >= pd.n_l1 || pd.l1v[i] != k1) {
+        const uint64_t pos = i < pd.n_l1 ? pd.l1o[i] : pd.n;
+        *b = *e = pos;
+        return;
+    }
+    uint64_t lo = pd.l1o[i];
+    uint64_t hi = (i + 1 < pd.n_l1) ? pd.l1o[i + 1] : pd.n;
+    if (has_k2) {
+        lo = dev::lower_bound(pd.y, lo, hi, k2);
+        hi = dev::upper_bound(pd.y, lo, hi, k2);
+    }
+    *b = lo;
+    *e = hi;
+}
+
+__global__ __launch_bounds__(256) void k_lookup_batch(
+    PermDev pd, const uint32_t *__restrict__ k1, const uint32_t *__restrict__ k2,
+    uint64_t nq, uint64_t *__restrict__ begin, uint64_t *__restrict__ end,
+    uint64_t *__restrict__ sizes) {
+    const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const uint32_t c2 = k2 ? k2[q] : VD_UNBOUND;
+    uint64_t b, e;
+    lookup_one(pd, k1[q], c2, c2 != VD_UNBOUND, &b, &e);
+    if (begin) begin[q] = b;
+    if (end) end[q] = e;
+    if (sizes) sizes[q] = e - b;
+}
+
+// Rows of query q are output rows [off[q], off[q+1]); one thread per output
+// row finds its query by binary search over off (nq + 1 entries).
+__global__ __launch_bounds__(256) void k_gather_ranges(
+    const uint64_t *__restrict__ off, const uint64_t *__restrict__ begin,
+    uint64_t nq, uint64_t total, const uint32_t *__restrict__ y,
+    const uint32_t *__restrict__ z, uint32_t *__restrict__ oy,
+    uint32_t *__restrict__ oz) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total;
+         o += stride) {
+        const uint64_t q = dev::upper_bound(off, 0, nq + 1, o) - 1;
+        const uint64_t src = begin[q] + (o - off[q]);
+        if (oy) oy[o] = y[src];
+        if (oz) oz[o] = z[src];
+    }
+}
+
+// ---- select ------------------------------------------------------------------
+constexpr int MAX_PIECES = 8;
+
+struct Pieces {
+    int np;
+    uint64_t src[MAX_PIECES];
+    uint64_t dst[MAX_PIECES + 1];  // dst[np] = total rows
+};
+
+// One thread: turn [b, e) minus clip and skip intervals into pieces of
+// consecutive rows.  Intervals must be sorted by lo and disjoint.
+__global__ void k_select_pieces(const uint32_t *__restrict__ ycol, uint64_t b,
+                                uint64_t e, int clip, uint32_t lo, uint32_t hi,
+                                vd_interval_t s0, vd_interval_t s1,
+                                vd_interval_t s2, vd_interval_t s3, int nskip,
+                                Pieces *out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (clip) {
+        if (lo > hi) {
+            e = b;
+        } else {
+            const uint64_t nb = dev::lower_bound(ycol, b, e, lo);
+            e = dev::upper_bound(ycol, nb, e, hi);
+            b = nb;
+        }
+    }
+    vd_interval_t sk[4] = {s0, s1, s2, s3};
+    Pieces P;
+    P.np = 0;
+    uint64_t cur = b, acc = 0;
+    for (int i = 0; i < nskip; ++i) {
+        if (sk[i].lo > sk[i].hi) continue;
+        const uint64_t a = dev::lower_bound(ycol, cur, e, sk[i].lo);
+        const uint64_t c = dev::upper_bound(ycol, a, e, sk[i].hi);
+        if (a > cur) {
+            P.src[P.np] = cur;
+            P.dst[P.np] = acc;
+            acc += a - cur;
+            ++P.np;
+        }
+        cur = c;
+    }
+    if (e > cur) {
+        P.src[P.np] = cur;
+        P.dst[P.np] = acc;
+        acc += e - cur;
+        ++P.np;
+    }
+    P.dst[P.np] = acc;
+    *out = P;
+}
+
+__global__ __launch_bounds__(256) void k_copy_pieces(
+    const Pieces *__restrict__ P, const uint32_t *__restrict__ y,
+    const uint32_t *__restrict__ z, uint32_t *__restrict__ o0,
+    uint32_t *__restrict__ o1, int cols) {
+    const int np = P->np;
+    const uint64_t total = P->dst[np];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total;
+         o += stride) {
+        int k = 0;
+        while (k + 1 < np && P->dst[k + 1] <= o) ++k;
+        const uint64_t src = P->src[k] + (o - P->dst[k]);
+        if (cols == 1) {
+            o0[o] = z[src];
+        } else if (cols == 2) {
+            o0[o] = y[src];
+        } else {
+            o0[o] = y[src];
+            o1[o] = z[src];
+        }
+    }
+}
+
+unsigned stream_grid(uint64_t n) {
+    uint64_t g = ceil_div(n, 256 * 4);
+    if (g > 16384) g = 16384;
+    return g ? (unsigned)g : 1u;
+}
+
+PermDev perm_dev(const vd_perm_arrays &pa) {
+    return PermDev{pa.n, pa.n_l1, pa.l1v, pa.l1o, pa.y};
+}
+
+void build_perm(vd_ctx *ctx, vd_index *idx, int perm, const uint32_t *cols[3],
+                const int bits[3], uint64_t *keys, uint64_t *tmp) {
+    const uint64_t n = idx->n;
+    vd_perm_arrays &pa = idx->perm[perm];
+    const int xi = PERM_COLS[perm][0], yi = PERM_COLS[perm][1],
+              zi = PERM_COLS[perm][2];
+    const int bz = std::max(bits[zi], 1), by = std::max(bits[yi], 1),
+              bx = std::max(bits[xi], 1);
+    const int kbits = bx + by + bz;
+    VD_REQUIRE(kbits <= 64, VD_ERANGE,
+               "triple IDs need more than 64 bits packed; not supported");
+    const int sy = bz, sx = by + bz;
+    const int npass = radix_passes(0, kbits);
+
+    DBuf<uint32_t> hist(ctx, radix_hist_words(0, kbits));
+    fill_u32(ctx, hist.p, 0, radix_hist_words(0, kbits));
+    const uint64_t pk_tiles = ceil_div(n, PK_BLOCK * PK_ITEMS);
+    launch(ctx, "index_pack", (double)n * 20.0, dim3((unsigned)pk_tiles),
+           dim3(PK_BLOCK), 0, k_pack_xyz, cols[xi], cols[yi], cols[zi], n, sy,
+           sx, keys, kbits, npass, hist.p);
+    uint64_t *sorted = radix_sort_u64(ctx, keys, tmp, n, 0, kbits, hist.p, true);
+
+    pa.n = n;
+    pa.y = (uint32_t *)dalloc(ctx, n * 4);
+    pa.z = (uint32_t *)dalloc(ctx, n * 4);
+    const uint64_t tiles = ceil_div(n, UN_TILE);
+    DBuf<uint32_t> c1(ctx, tiles), c2(ctx, tiles);
+    DBuf<uint64_t> o1(ctx, tiles + 1), o2(ctx, tiles + 1);
+    const uint32_t ymask = by >= 32 ? 0xFFFFFFFFu : ((1u << by) - 1u);
+    const uint32_t zmask = bz >= 32 ? 0xFFFFFFFFu : ((1u << bz) - 1u);
+    launch(ctx, "index_unpack", (double)n * 16.0, dim3((unsigned)tiles),
+           dim3(UN_BLOCK), 0, k_unpack_count, (const uint64_t *)sorted, n, sy,
+           sx, ymask, zmask, pa.y, pa.z, c1.p, c2.p);
+    scan_u32_to_u64(ctx, c1.p, o1.p, tiles, o1.p + tiles);
+    scan_u32_to_u64(ctx, c2.p, o2.p, tiles, o2.p + tiles);
+    uint64_t tot[2];
+    {
+        DBuf<uint64_t> t(ctx, 2);
+        copy_d2d(ctx, t.p, o1.p + tiles, 8);
+        copy_d2d(ctx, t.p + 1, o2.p + tiles, 8);
+        read_words(ctx, tot, t.p, 2);
+    }
+    pa.n_l1 = tot[0];
+    pa.n_l2 = tot[1];
+    pa.l1v = (uint32_t *)dalloc(ctx, pa.n_l1 * 4);
+    pa.l1o = (uint32_t *)dalloc(ctx, pa.n_l1 * 4);
+    pa.l2v = (uint32_t *)dalloc(ctx, pa.n_l2 * 4);
+    pa.l2o = (uint32_t *)dalloc(ctx, pa.n_l2 * 4);
+    launch(ctx, "index_heads", (double)n * 8.0 + (pa.n_l1 + pa.n_l2) * 8.0,
+           dim3((unsigned)tiles), dim3(UN_BLOCK), 0, k_compact_heads,
+           (const uint64_t *)sorted, n, sy, sx, ymask, (const uint64_t *)o1.p,
+           (const uint64_t *)o2.p, pa.l1v, pa.l1o, pa.l2v, pa.l2o);
+    pa.built = true;
+}
+
+void free_perm(vd_ctx *ctx, vd_perm_arrays &pa) {
+    dfree(ctx, pa.l1v); dfree(ctx, pa.l1o);
+    dfree(ctx, pa.l2v); dfree(ctx, pa.l2o);
+    dfree(ctx, pa.y); dfree(ctx, pa.z);
+    pa = vd_perm_arrays();
+}
+
+const vd_perm_arrays &get_perm(const vd_index *idx, int perm) {
+    VD_REQUIRE(idx && perm >= 0 && perm < VD_NUM_PERMS, VD_EINVAL,
+               "bad index or permutation");
+    VD_REQUIRE(idx->perm[perm].built, VD_EINVAL, "permutation not built");
+    return idx->perm[perm];
+}
+
+}  // namespace
+}  // namespace vd
+
+using namespace vd;
+
+extern "C" {
+
+int vd_index_build(vd_ctx *ctx, const uint32_t *d_s, const uint32_t *d_p,
+                   const uint32_t *d_o, uint64_t n, uint32_t perm_mask,
+                   vd_index **out) {
+    if (!ctx || !out) return VD_EINVAL;
+    *out = nullptr;
+    vd_index *idx = new (std::nothrow) vd_index();
+    if (!idx) return VD_ENOMEM;
+    try {
+        VD_REQUIRE(perm_mask != 0 && perm_mask < (1u << VD_NUM_PERMS), VD_EINVAL,
+                   "empty or invalid permutation mask");
+        VD_REQUIRE(n == 0 || (d_s && d_p && d_o), VD_EINVAL, "null triples");
+        VD_REQUIRE(n < (1ull << 30), VD_ERANGE,
+                   "index build supports fewer than 2^30 triples per call");
+        idx->owner = ctx;
+        idx->n = n;
+        idx->mask = perm_mask;
+        const uint32_t *cols[3] = {d_s, d_p, d_o};
+        int bits[3] = {1, 1, 1};
+        if (n > 0)
+            for (int c = 0; c < 3; ++c) bits[c] = std::max(1, bits_for(max_u32(ctx, cols[c], n)));
+        DBuf<uint64_t> keys(ctx, n), tmp(ctx, n);
+        for (int p = 0; p < VD_NUM_PERMS; ++p) {
+            if (!(perm_mask & VD_PERM_BIT(p))) continue;
+            if (n == 0) {
+                vd_perm_arrays &pa = idx->perm[p];
+                pa.built = true;
+                pa.y = (uint32_t *)dalloc(ctx, 4);
+                pa.z = (uint32_t *)dalloc(ctx, 4);
+                pa.l1v = (uint32_t *)dalloc(ctx, 4);
+                pa.l1o = (uint32_t *)dalloc(ctx, 4);
+                pa.l2v = (uint32_t *)dalloc(ctx, 4);
+                pa.l2o = (uint32_t *)dalloc(ctx, 4);
+                continue;
+            }
+            build_perm(ctx, idx, p, cols, bits, keys.p, tmp.p);
+        }
+    } catch (const Error &e) {
+        ctx->last_error = e.what();
+        for (auto &pa : idx->perm) free_perm(ctx, pa);
+        delete idx;
+        return e.code;
+    } catch (const std::exception &e) {
+        ctx->last_error = e.what();
+        for (auto &pa : idx->perm) free_perm(ctx, pa);
+        delete idx;
+        return VD_EINVAL;
+    }
+    *out = idx;
+    return VD_OK;
+}
+
+int vd_index_free(vd_ctx *ctx, vd_index *idx) {
+    if (!idx) return VD_EINVAL;
+    vd_ctx *owner = idx->owner ? idx->owner : ctx;
+    VD_TRY(owner, {
+        for (auto &pa : idx->perm) free_perm(owner, pa);
+        delete idx;
+    });
+}
+
+uint32_t vd_index_perms(const vd_index *idx) { return idx ? idx->mask : 0; }
+
+int vd_index_view(const vd_index *idx, int perm, vd_perm_view_t *out) {
+    if (!idx || !out || perm < 0 || perm >= VD_NUM_PERMS || !idx->perm[perm].built)
+        return VD_EINVAL;
+    const vd_perm_arrays &pa = idx->perm[perm];
+    out->n = pa.n;
+    out->n_l1 = pa.n_l1;
+    out->n_l2 = pa.n_l2;
+    out->l1_values = pa.l1v;
+    out->l1_offsets = pa.l1o;
+    out->l2_values = pa.l2v;
+    out->l2_offsets = pa.l2o;
+    out->col_y = pa.y;
+    out->col_z = pa.z;
+    return VD_OK;
+}
+
+int vd_index_lookup_batch(vd_ctx *ctx, const vd_index *idx, int perm,
+                          const uint32_t *d_k1, const uint32_t *d_k2,
+                          uint64_t nq, uint64_t *d_begin, uint64_t *d_end) {
+    if (!ctx) return VD_EINVAL;
+    VD_TRY(ctx, {
+        const vd_perm_arrays &pa = get_perm(idx, perm);
+        VD_REQUIRE(nq == 0 || (d_k1 && d_begin && d_end), VD_EINVAL,
+                   "null lookup arrays");
+        launch(ctx, "index_lookup", (double)nq * 24.0,
+               dim3((unsigned)ceil_div(nq, 256)), dim3(256), 0, k_lookup_batch,
+               perm_dev(pa), d_k1, d_k2, nq, d_begin, d_end, (uint64_t *)nullptr);
+    });
+}
+
+int vd_index_lookup(vd_ctx *ctx, const vd_index *idx, int perm, uint32_t k1,
+                    int has_k2, uint32_t k2, uint64_t *begin, uint64_t *end) {
+    if (!ctx || !begin || !end) return VD_EINVAL;
+    VD_TRY(ctx, {
+        const vd_perm_arrays &pa = get_perm(idx, perm);
+        DBuf<uint32_t> k(ctx, 2);
+        DBuf<uint64_t> be(ctx, 2);
+        uint32_t hk[2] = {k1, has_k2 ? k2 : VD_UNBOUND};
+        VD_HIP(hipMemcpyAsync(k.p, hk, 8, hipMemcpyHostToDevice, ctx->stream));
+        launch(ctx, "index_lookup", 24.0, dim3(1), dim3(64), 0, k_lookup_batch,
+               perm_dev(pa), (const uint32_t *)k.p, (const uint32_t *)(k.p + 1), (uint64_t)1,
+               be.p, be.p + 1, (uint64_t *)nullptr);
+        uint64_t r[2];
+        read_words(ctx, r, be.p, 2);
+        *begin = r[0];
+        *end = r[1];
+    });
+}
+
+int vd_select(vd_ctx *ctx, const vd_index *idx, int perm, uint64_t begin,
+              uint64_t end, int cols, int clip, uint32_t lo, uint32_t hi,
+              const vd_interval_t *skip, int nskip, int distinct, vd_rel **out) {
+    if (!ctx || !out) return VD_EINVAL;
+    *out = nullptr;
+    VD_TRY(ctx, {
+        const vd_perm_arrays &pa = get_perm(idx, perm);
+        VD_REQUIRE(cols >= 1 && cols <= 3, VD_EINVAL, "cols must be 1, 2 or 3");
+        VD_REQUIRE(begin <= end && end <= pa.n, VD_EINVAL, "row range out of bounds");
+        VD_REQUIRE(nskip >= 0 && nskip <= 4 && (nskip == 0 || skip), VD_EINVAL,
+                   "at most 4 skip intervals");
+        VD_REQUIRE(!(clip || nskip || distinct) || cols != 1, VD_EINVAL,
+                   "clip, skip and distinct need Y as column 0");
+        VD_REQUIRE(!distinct || cols == 2, VD_EINVAL, "distinct needs cols == 2");
+        vd_interval_t sk[4] = {{1, 0}, {1, 0}, {1, 0}, {1, 0}};
+        for (int i = 0; i < nskip; ++i) sk[i] = skip[i];
+        std::sort(sk, sk + nskip, [](const vd_interval_t &a, const vd_interval_t &b) {
+            return a.lo < b.lo;
+        });
+        for (int i = 1; i < nskip; ++i)
+            VD_REQUIRE(sk[i].lo > sk[i - 1].hi || sk[i - 1].lo > sk[i - 1].hi,
+                       VD_EINVAL, "skip intervals overlap");
+        DBuf<Pieces> dp(ctx, 1);
+        launch(ctx, "select_pieces", 0.0, dim3(1), dim3(64), 0, k_select_pieces,
+               (const uint32_t *)pa.y, begin, end, clip, lo, hi, sk[0], sk[1],
+               sk[2], sk[3], nskip, dp.p);
+        Pieces hp;
+        VD_HIP(hipMemcpyAsync(&hp, dp.p, sizeof(Pieces), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        sync(ctx);
+        const uint64_t rows = hp.dst[hp.np];
+        const int ocols = cols == 3 ? 2 : 1;
+        vd_rel *r = rel_new(ctx, ocols, rows);
+        try {
+            launch(ctx, "select_copy", (double)rows * 8.0 * ocols,
+                   dim3(stream_grid(rows)), dim3(256), 0, k_copy_pieces,
+                   (const Pieces *)dp.p, (const uint32_t *)pa.y,
+                   (const uint32_t *)pa.z, r->cols[0],
+                   ocols == 2 ? r->cols[1] : (uint32_t *)nullptr, cols);
+            if (distinct && rows > 1) {
+                int rc = vd_unique_adjacent(ctx, r);
+                if (rc != VD_OK) throw Error(rc, ctx->last_error);
+            }
+        } catch (...) {
+            rel_free(ctx, r);
+            throw;
+        }
+        *out = r;
+    });
+}
+
+int vd_select_batch(vd_ctx *ctx, const vd_index *idx, int perm,
+                    const uint32_t *d_k1, const uint32_t *d_k2, uint64_t nq,
+                    uint64_t *d_row_offsets, uint32_t *d_out_y,
+                    uint32_t *d_out_z, uint64_t max_rows, uint64_t *total_rows) {
+    if (!ctx || !total_rows) return VD_EINVAL;
+    *total_rows = 0;
+    try {
+        const vd_perm_arrays &pa = get_perm(idx, perm);
+        VD_REQUIRE(nq == 0 || (d_k1 && d_row_offsets), VD_EINVAL,
+                   "null select arrays");
+        if (nq == 0) return VD_OK;
+        DBuf<uint64_t> begin(ctx, nq), sizes(ctx, nq);
+        launch(ctx, "select_lookup", (double)nq * 24.0,
+               dim3((unsigned)ceil_div(nq, 256)), dim3(256), 0, k_lookup_batch,
+               perm_dev(pa), d_k1, d_k2, nq, begin.p, (uint64_t *)nullptr,
+               sizes.p);
+        scan_u64(ctx, sizes.p, d_row_offsets, nq, d_row_offsets + nq);
+        const uint64_t total = read_u64(ctx, d_row_offsets + nq);
+        *total_rows = total;
+        if (max_rows == 0) return VD_OK;
+        if (total > max_rows) {
+            ctx->last_error = "select batch output exceeds max_rows";
+            return VD_ERANGE;
+        }
+        VD_REQUIRE(total == 0 || d_out_y || d_out_z, VD_EINVAL, "no output columns");
+        launch(ctx, "select_gather",
+               (double)total * 4.0 * ((d_out_y ? 2 : 0) + (d_out_z ? 2 : 0)),
+               dim3(stream_grid(total)), dim3(256), 0, k_gather_ranges,
+               (const uint64_t *)d_row_offsets, (const uint64_t *)begin.p, nq,
+               total, (const uint32_t *)pa.y, (const uint32_t *)pa.z, d_out_y,
+               d_out_z);
+    } catch (const Error &e) {
+        ctx->last_error = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        ctx->last_error = e.what();
+        return VD_EINVAL;
+    }
+    return VD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,285 @@
+// Internal definitions shared by the HIP translation units of libvedas_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "vedas_hip.h"
+
+namespace vd {
+
+// Error carried from the point of failure to the C-ABI boundary, where it
+// becomes a status code plus the context's last-error message.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define VD_HIP(call)                                                          \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess)                                                 \
+            throw ::vd::Error(VD_EHIP, std::string(#call) + ": " +           \
+                                           hipGetErrorString(e_));           \
+    } while (0)
+
+#define VD_REQUIRE(cond, code, msg)                                           \
+    do {                                                                      \
+        if (!(cond)) throw ::vd::Error((code), (msg));                        \
+    } while (0)
+
+// Body of a C-ABI entry point: maps exceptions to status codes and records
+// the message on the context.
+#define VD_TRY(ctx, ...)                                                      \
+    try {                                                                     \
+        __VA_ARGS__;                                                          \
+        return VD_OK;                                                         \
+    } catch (const ::vd::Error &e_) {                                         \
+        if (ctx) (ctx)->last_error = e_.what();                               \
+        return e_.code;                                                       \
+    } catch (const std::bad_alloc &) {                                        \
+        if (ctx) (ctx)->last_error = "host allocation failed";               \
+        return VD_ENOMEM;                                                     \
+    } catch (const std::exception &e_) {                                      \
+        if (ctx) (ctx)->last_error = e_.what();                               \
+        return VD_EINVAL;                                                     \
+    }
+
+struct ProfStat {
+    uint64_t launches = 0;
+    double ms = 0.0;
+    double bytes = 0.0;
+};
+
+struct PendingLaunch {
+    const char *name;
+    double bytes;
+    hipEvent_t start, stop;
+};
+
+}  // namespace vd
+
+struct vd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+
+    // Caching allocator: blocks are rounded to size classes and reused in
+    // stream order (every kernel of the context runs on `stream`, so a block
+    // freed after its last use may be handed to the next allocation at once).
+    std::multimap<size_t, void *> free_blocks;
+    std::unordered_map<void *, size_t> live_blocks;
+    uint64_t bytes_live = 0, bytes_cached = 0;
+
+    // Pinned host staging for small read-backs.
+    uint64_t *h_pinned = nullptr;  // 64 words
+
+    // Profiling.
+    bool prof = false;
+    std::vector<vd::PendingLaunch> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::vector<std::string> prof_names;
+    std::map<std::string, vd::ProfStat> prof_stats;
+
+    // Tunables.
+    int radix_items = 16;   // keys per thread in a radix tile
+    int join_nv = 2048;     // merge-path diagonal length per expansion block
+    int join_rcap = 4096;   // right keys staged in LDS per bounds block
+};
+
+struct vd_rel {
+    int ncols = 0;
+    uint64_t nrows = 0;
+    uint32_t *cols[VD_MAX_COLS] = {};
+};
+
+namespace vd {
+
+// ---- allocator -------------------------------------------------------------
+void *dalloc(vd_ctx *ctx, size_t bytes);
+void dfree(vd_ctx *ctx, void *p);
+
+// RAII device buffer from the context pool.
+template <typename T>
+struct DBuf {
+    vd_ctx *ctx = nullptr;
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(vd_ctx *c, size_t count) : ctx(c), n(count) {
+        p = static_cast<T *>(dalloc(c, count ? count * sizeof(T) : sizeof(T)));
+    }
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : ctx(o.ctx), p(o.p), n(o.n) { o.p = nullptr; }
+    DBuf &operator=(DBuf &&o) noexcept {
+        reset();
+        ctx = o.ctx; p = o.p; n = o.n; o.p = nullptr;
+        return *this;
+    }
+    ~DBuf() { reset(); }
+    void reset() {
+        if (p) dfree(ctx, p);
+        p = nullptr;
+    }
+    T *release() { T *q = p; p = nullptr; return q; }
+};
+
+// ---- launches --------------------------------------------------------------
+hipEvent_t take_event(vd_ctx *ctx);
+void prof_begin(vd_ctx *ctx, hipEvent_t *ev);
+void prof_end(vd_ctx *ctx, const char *name, double bytes, hipEvent_t ev0);
+void prof_collect(vd_ctx *ctx);  // after a stream sync
+
+// Launch `kernel` on the context stream; with profiling on, the launch is
+// bracketed by events and charged `bytes` algorithmic HBM bytes.
+template <typename... KArgs, typename... Args>
+inline void launch(vd_ctx *ctx, const char *name, double bytes, dim3 grid,
+                   dim3 block, size_t shmem, void (*kernel)(KArgs...),
+                   Args &&...args) {
+    if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
+    hipEvent_t ev0 = nullptr;
+    if (ctx->prof) prof_begin(ctx, &ev0);
+    hipLaunchKernelGGL(kernel, grid, block, shmem, ctx->stream,
+                       std::forward<Args>(args)...);
+    VD_HIP(hipGetLastError());
+    if (ctx->prof) prof_end(ctx, name, bytes, ev0);
+}
+
+void sync(vd_ctx *ctx);
+uint64_t read_u64(vd_ctx *ctx, const uint64_t *dptr);          // syncs
+void read_words(vd_ctx *ctx, uint64_t *dst, const uint64_t *dptr, int n);
+
+// ---- primitives ------------------------------------------------------------
+// Exclusive scan of n uint32 counts into uint64 offsets; *d_total (device)
+// receives the sum.  Arbitrary n.
+void scan_u32_to_u64(vd_ctx *ctx, const uint32_t *in, uint64_t *out,
+                     uint64_t n, uint64_t *d_total);
+// Exclusive scan of n uint64 in place-capable (in may equal out).
+void scan_u64(vd_ctx *ctx, const uint64_t *in, uint64_t *out, uint64_t n,
+              uint64_t *d_total);
+void fill_u32(vd_ctx *ctx, uint32_t *p, uint32_t v, uint64_t n);
+void fill_u64(vd_ctx *ctx, uint64_t *p, uint64_t v, uint64_t n);
+uint32_t max_u32(vd_ctx *ctx, const uint32_t *p, uint64_t n);  // syncs
+void copy_d2d(vd_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+// Radix sort of 64-bit keys on bits [lo_bit, hi_bit), stable (LSD).
+// keys and tmp hold n keys each; the sorted keys end up in the returned
+// pointer (either keys or tmp).  If hist_ready, d_hist already holds the
+// per-pass digit histograms (8-bit digits from lo_bit; npass x 256 uint32),
+// counted by a fused producer kernel.
+uint64_t *radix_sort_u64(vd_ctx *ctx, uint64_t *keys, uint64_t *tmp,
+                         uint64_t n, int lo_bit, int hi_bit,
+                         uint32_t *d_hist, bool hist_ready);
+int radix_passes(int lo_bit, int hi_bit);
+// Device histogram buffer size (uint32 words) needed for radix_sort_u64.
+inline size_t radix_hist_words(int lo_bit, int hi_bit) {
+    return (size_t)radix_passes(lo_bit, hi_bit) * 256;
+}
+
+inline int bits_for(uint64_t maxv) {
+    int b = 0;
+    while (b < 64 && (maxv >> b) != 0) ++b;
+    return b;
+}
+
+inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+// Relation helpers.
+vd_rel *rel_new(vd_ctx *ctx, int ncols, uint64_t nrows);
+void rel_free(vd_ctx *ctx, vd_rel *r);
+
+}  // namespace vd
+
+// ---- device helpers ----------------------------------------------------------
+namespace vd {
+namespace dev {
+
+constexpr int WAVE = 64;
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (WAVE - 1); }
+
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        T u = __shfl_up(v, off, WAVE);
+        if (lane >= off) v += u;
+    }
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_reduce(T v) {
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+// Block-wide exclusive scan; returns the block total through *total.
+// s_tmp must hold BLOCK/64 entries.  All threads must call it.
+template <int BLOCK, typename T>
+__device__ __forceinline__ T block_excl_scan(T v, T *s_tmp, T *total) {
+    constexpr int NW = BLOCK / WAVE;
+    const int lane = lane_id();
+    const int w = threadIdx.x / WAVE;
+    T inc = wave_incl_scan(v);
+    if (lane == WAVE - 1) s_tmp[w] = inc;
+    __syncthreads();
+    T base = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+        T x = s_tmp[i];
+        if (i < w) base += x;
+        tot += x;
+    }
+    __syncthreads();
+    *total = tot;
+    return base + inc - v;
+}
+
+template <int BLOCK, typename T>
+__device__ __forceinline__ T block_reduce(T v, T *s_tmp) {
+    constexpr int NW = BLOCK / WAVE;
+    T r = wave_reduce(v);
+    if (lane_id() == 0) s_tmp[threadIdx.x / WAVE] = r;
+    __syncthreads();
+    T tot = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) tot += s_tmp[i];
+    __syncthreads();
+    return tot;
+}
+
+// First index in [lo, hi) with a[i] >= v (a sorted ascending).
+template <typename T, typename V>
+__device__ __forceinline__ uint64_t lower_bound(const T *a, uint64_t lo,
+                                                uint64_t hi, V v) {
+    while (lo < hi) {
+        uint64_t mid = lo + ((hi - lo) >> 1);
+        if (a[mid] < v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// First index in [lo, hi) with a[i] > v.
+template <typename T, typename V>
+__device__ __forceinline__ uint64_t upper_bound(const T *a, uint64_t lo,
+                                                uint64_t hi, V v) {
+    while (lo < hi) {
+        uint64_t mid = lo + ((hi - lo) >> 1);
+        if (v < a[mid]) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+}
+
+}  // namespace dev
+}  // namespace vd

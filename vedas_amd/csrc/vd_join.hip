@@ -1,0 +1,280 @@
+// Sort-merge bag inner join of two relations sorted on their key columns.
+//
+//  1. k_join_bounds : for every left row i, lower_i = first right row with
+//                     key >= L[i] and cnt_i = number of right rows with key
+//                     == L[i].  A block takes 2048 consecutive left rows,
+//                     finds the right range they can touch with two global
+//                     binary searches, and stages that range in LDS when it
+//                     fits (otherwise it searches in global memory).
+//  2. scan          : off_i = exclusive scan of cnt_i (64-bit); the total is
+//                     the output size (read back once to allocate).
+//  3. k_join_partition / k_join_expand : load-balanced expansion.  The merge
+//                     of the left rows (row i at merge position i + off_i)
+//                     with the output indices is cut into diagonals of NV
+//                     items, so every block gets at most NV rows + outputs
+//                     whatever the duplicate-run lengths.  A block stages
+//                     off/lower of its rows in LDS, maps each output index
+//                     to (left row, right row) by binary search in LDS, and
+//                     gathers the output columns with coalesced writes; the
+//                     (left, right) pair list is never written to HBM.
+//
+// Output rows are left-major (left row order, then right row order), with
+// columns [keys, left payload, right payload] -- the order the reference's
+// ModernGPU inner_join + gather produce (JoinQueryJob.cu:68-101, 262-380).
+
+#include "vd_internal.hpp"
+
+namespace vd {
+namespace {
+
+constexpr int JB_BLOCK = 256;
+constexpr int JB_ITEMS = 8;
+constexpr int JB_TILE = JB_BLOCK * JB_ITEMS;
+constexpr int JX_BLOCK = 256;
+constexpr int JB_RCAP_BYTES = 32768;  // LDS for staged right keys
+
+struct KeyCols {
+    const uint32_t *c0, *c1;
+};
+
+template <int NK>
+struct KeyT;
+template <>
+struct KeyT<1> {
+    using T = uint32_t;
+    static __device__ __forceinline__ T at(const KeyCols &k, uint64_t i) {
+        return k.c0[i];
+    }
+};
+template <>
+struct KeyT<2> {
+    using T = uint64_t;
+    static __device__ __forceinline__ T at(const KeyCols &k, uint64_t i) {
+        return ((uint64_t)k.c0[i] << 32) | k.c1[i];
+    }
+};
+
+template <int NK>
+__device__ __forceinline__ uint64_t g_lower(const KeyCols &k, uint64_t lo,
+                                            uint64_t hi,
+                                            typename KeyT<NK>::T v) {
+    while (lo < hi) {
+        uint64_t mid = lo + ((hi - lo) >> 1);
+        if (KeyT<NK>::at(k, mid) < v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+template <int NK>
+__device__ __forceinline__ uint64_t g_upper(const KeyCols &k, uint64_t lo,
+                                            uint64_t hi,
+                                            typename KeyT<NK>::T v) {
+    while (lo < hi) {
+        uint64_t mid = lo + ((hi - lo) >> 1);
+        if (v < KeyT<NK>::at(k, mid)) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+}
+
+template <int NK>
+__global__ __launch_bounds__(JB_BLOCK) void k_join_bounds(
+    KeyCols L, uint64_t nl, KeyCols R, uint64_t nr, int rcap,
+    uint32_t *__restrict__ lower, uint32_t *__restrict__ cnt) {
+    using T = typename KeyT<NK>::T;
+    __shared__ T s_r[JB_RCAP_BYTES / sizeof(T)];
+    __shared__ uint64_t s_range[2];
+    const uint64_t base = (uint64_t)blockIdx.x * JB_TILE;
+    const uint64_t last = min(base + JB_TILE, nl) - 1;
+    if (threadIdx.x == 0) {
+        const T kf = KeyT<NK>::at(L, base), kl = KeyT<NK>::at(L, last);
+        const uint64_t lo = g_lower<NK>(R, 0, nr, kf);
+        const uint64_t hi = g_upper<NK>(R, lo, nr, kl);
+        s_range[0] = lo;
+        s_range[1] = hi;
+    }
+    __syncthreads();
+    const uint64_t rlo = s_range[0], rhi = s_range[1];
+    const uint64_t m = rhi - rlo;
+    const bool staged = m <= (uint64_t)rcap && m <= JB_RCAP_BYTES / sizeof(T);
+    if (staged) {
+        for (uint64_t t = threadIdx.x; t < m; t += JB_BLOCK)
+            s_r[t] = KeyT<NK>::at(R, rlo + t);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < JB_ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)j * JB_BLOCK + threadIdx.x;
+        if (i >= nl) break;
+        const T key = KeyT<NK>::at(L, i);
+        uint64_t lo, hi;
+        if (staged) {
+            lo = dev::lower_bound(s_r, 0, m, key);
+            hi = dev::upper_bound(s_r, lo, m, key);
+            lo += rlo;
+            hi += rlo;
+        } else {
+            lo = g_lower<NK>(R, rlo, rhi, key);
+            hi = g_upper<NK>(R, lo, rhi, key);
+        }
+        lower[i] = (uint32_t)lo;
+        cnt[i] = (uint32_t)(hi - lo);
+    }
+}
+
+// mp[b] = number of left rows i with i + off[i] < b * nv (clamped to the
+// end of the diagonal), for b = 0 .. nblocks.
+__global__ __launch_bounds__(256) void k_join_partition(
+    const uint64_t *__restrict__ off, uint64_t nl, uint64_t total, int nv,
+    uint64_t nblocks, uint64_t *__restrict__ mp) {
+    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > nblocks) return;
+    const uint64_t diag = min(b * (uint64_t)nv, nl + total);
+    uint64_t lo = 0, hi = nl;
+    while (lo < hi) {
+        const uint64_t mid = lo + ((hi - lo) >> 1);
+        if (mid + off[mid] < diag) lo = mid + 1; else hi = mid;
+    }
+    mp[b] = lo;
+}
+
+struct OutCols {
+    const uint32_t *l[VD_MAX_COLS];
+    const uint32_t *r[VD_MAX_COLS];
+    uint32_t *o[VD_MAX_COLS];
+    int nl, nr;  // total columns of L and R
+};
+
+template <int NK, int NV>
+__global__ __launch_bounds__(JX_BLOCK) void k_join_expand(
+    const uint64_t *__restrict__ off, const uint32_t *__restrict__ lower,
+    const uint64_t *__restrict__ mp, uint64_t nl, uint64_t total, OutCols oc) {
+    constexpr int ITEMS = NV / JX_BLOCK;
+    __shared__ uint64_t s_off[NV + 1];
+    __shared__ uint32_t s_low[NV + 1];
+    const uint64_t blk = blockIdx.x;
+    const uint64_t d0 = blk * NV;
+    const uint64_t d1 = min(d0 + NV, nl + total);
+    const uint64_t a0 = mp[blk], a1 = mp[blk + 1];
+    const uint64_t b0 = d0 - a0, b1 = d1 - a1;
+    if (b1 <= b0) return;  // no outputs on this diagonal
+    const uint64_t sa = a0 > 0 ? a0 - 1 : 0;
+    const int m = (int)(a1 - sa);
+    for (int t = threadIdx.x; t < m; t += JX_BLOCK) {
+        s_off[t] = off[sa + t];
+        s_low[t] = lower[sa + t];
+    }
+    __syncthreads();
+
+    uint32_t lrow[ITEMS], rrow[ITEMS];
+    const int nout = (int)(b1 - b0);
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const int q = j * JX_BLOCK + threadIdx.x;
+        lrow[j] = rrow[j] = 0;
+        if (q < nout) {
+            const uint64_t o = b0 + q;
+            // last k with s_off[k] <= o
+            int lo = 0, hi = m;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (s_off[mid] <= o) lo = mid + 1; else hi = mid;
+            }
+            const int k = lo - 1;
+            lrow[j] = (uint32_t)(sa + k);
+            rrow[j] = s_low[k] + (uint32_t)(o - s_off[k]);
+        }
+    }
+    // Key columns from the left rows.
+    int oc_i = 0;
+    for (int c = 0; c < oc.nl; ++c, ++oc_i) {
+        const uint32_t *src = oc.l[c];
+        uint32_t *dst = oc.o[oc_i];
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            const int q = j * JX_BLOCK + threadIdx.x;
+            if (q < nout) dst[b0 + q] = src[lrow[j]];
+        }
+    }
+    for (int c = NK; c < oc.nr; ++c, ++oc_i) {
+        const uint32_t *src = oc.r[c];
+        uint32_t *dst = oc.o[oc_i];
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            const int q = j * JX_BLOCK + threadIdx.x;
+            if (q < nout) dst[b0 + q] = src[rrow[j]];
+        }
+    }
+}
+
+template <int NK>
+vd_rel *join_impl(vd_ctx *ctx, const vd_rel *L, const vd_rel *R) {
+    const uint64_t nl = L->nrows, nr = R->nrows;
+    const int out_cols = L->ncols + R->ncols - NK;
+    VD_REQUIRE(out_cols <= VD_MAX_COLS, VD_EINVAL, "join output has too many columns");
+    if (nl == 0 || nr == 0) return rel_new(ctx, out_cols, 0);
+    VD_REQUIRE(nl < (1ull << 32) && nr < (1ull << 32), VD_ERANGE,
+               "join inputs limited to 2^32-1 rows");
+
+    KeyCols lk{L->cols[0], NK == 2 ? L->cols[1] : nullptr};
+    KeyCols rk{R->cols[0], NK == 2 ? R->cols[1] : nullptr};
+    DBuf<uint32_t> lower(ctx, nl), cnt(ctx, nl);
+    DBuf<uint64_t> off(ctx, nl + 1);
+    const uint64_t jb_tiles = ceil_div(nl, JB_TILE);
+    const int rcap = std::min<int>(ctx->join_rcap, JB_RCAP_BYTES / (NK == 2 ? 8 : 4));
+    launch(ctx, "join_bounds", 4.0 * NK * (double)(nl + nr) + 8.0 * nl,
+           dim3((unsigned)jb_tiles), dim3(JB_BLOCK), 0, k_join_bounds<NK>, lk,
+           nl, rk, nr, rcap, lower.p, cnt.p);
+    scan_u32_to_u64(ctx, cnt.p, off.p, nl, off.p + nl);
+    const uint64_t total = read_u64(ctx, off.p + nl);
+    VD_REQUIRE(total < (1ull << 40), VD_ERANGE, "join output too large");
+
+    vd_rel *out = rel_new(ctx, out_cols, total);
+    if (total == 0) return out;
+    try {
+        OutCols oc{};
+        for (int c = 0; c < L->ncols; ++c) oc.l[c] = L->cols[c];
+        for (int c = 0; c < R->ncols; ++c) oc.r[c] = R->cols[c];
+        for (int c = 0; c < out_cols; ++c) oc.o[c] = out->cols[c];
+        oc.nl = L->ncols;
+        oc.nr = R->ncols;
+        const int nv = ctx->join_nv;
+        const uint64_t nblocks = ceil_div(nl + total, (uint64_t)nv);
+        VD_REQUIRE(nblocks < (1ull << 31), VD_ERANGE, "join expansion grid too large");
+        DBuf<uint64_t> mp(ctx, nblocks + 1);
+        launch(ctx, "join_partition", 0.0, dim3((unsigned)ceil_div(nblocks + 1, 256)),
+               dim3(256), 0, k_join_partition, (const uint64_t *)off.p, nl, total,
+               nv, nblocks, mp.p);
+        const double xbytes = 4.0 * (double)total * out_cols + 12.0 * nl +
+                              4.0 * (double)nl * (L->ncols) + 4.0 * (double)nr * (R->ncols - NK);
+        auto go = [&](auto kern) {
+            launch(ctx, "join_expand", xbytes, dim3((unsigned)nblocks), dim3(JX_BLOCK),
+                   0, kern, (const uint64_t *)off.p, (const uint32_t *)lower.p,
+                   (const uint64_t *)mp.p, nl, total, oc);
+        };
+        if (nv == 1024) go(k_join_expand<NK, 1024>);
+        else if (nv == 4096) go(k_join_expand<NK, 4096>);
+        else go(k_join_expand<NK, 2048>);
+    } catch (...) {
+        rel_free(ctx, out);
+        throw;
+    }
+    return out;
+}
+
+}  // namespace
+}  // namespace vd
+
+using namespace vd;
+
+extern "C" int vd_join(vd_ctx *ctx, const vd_rel *L, const vd_rel *R, int nkeys,
+                       vd_rel **out) {
+    if (!ctx || !out) return VD_EINVAL;
+    *out = nullptr;
+    VD_TRY(ctx, {
+        VD_REQUIRE(L && R, VD_EINVAL, "null relation");
+        VD_REQUIRE(nkeys == 1 || nkeys == 2, VD_EINVAL, "nkeys must be 1 or 2");
+        VD_REQUIRE(L->ncols >= nkeys && R->ncols >= nkeys, VD_EINVAL,
+                   "relation has fewer columns than keys");
+        *out = nkeys == 1 ? join_impl<1>(ctx, L, R) : join_impl<2>(ctx, L, R);
+    });
+}

@@ -1,0 +1,240 @@
+// LSD radix sort of 64-bit keys, one kernel launch per 8-bit digit.
+//
+// Each pass is a single read and a single write of the keys ("onesweep"):
+//  * a block takes the next tile from an atomic counter, so tiles are
+//    handed out in the order blocks start;
+//  * it ranks its tile's keys with 64-lane ballots (peers with the same
+//    digit) and per-wave digit counters in LDS, which keeps the sort stable;
+//  * it publishes its per-digit counts and walks back over earlier tiles'
+//    published counts (decoupled look-back) to find its global digit
+//    offsets;
+//  * it stages the tile in LDS in digit order and writes it out, so
+//    consecutive lanes write consecutive addresses of a digit bucket.
+// The global digit histograms of all passes are counted once up front
+// (fused into the producer of the keys where possible).
+//
+// Look-back termination: tile t only waits on tiles < t.  Tile ids come
+// from the atomic counter as blocks start, so every tile < t belongs to a
+// block that is already resident, and a block publishes its aggregate
+// before it waits on anyone.  Tile 0 publishes an inclusive value without
+// waiting.  Hence every wait ends.
+
+#include "vd_radix_dev.hpp"
+
+namespace vd {
+namespace {
+
+using radix::RBINS;
+using radix::RBITS;
+
+constexpr int RS_BLOCK = 256;
+constexpr int RS_WAVES = RS_BLOCK / 64;
+
+constexpr uint32_t LB_AGG = 1u << 30;   // tile aggregate published
+constexpr uint32_t LB_INC = 2u << 30;   // inclusive prefix published
+constexpr uint32_t LB_VAL = (1u << 30) - 1u;
+
+__device__ __forceinline__ void lb_store(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t lb_load(const uint32_t *p) {
+    return __hip_atomic_load(const_cast<uint32_t *>(p), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(radix::HIST_BLOCK) void k_rs_hist(
+    const uint64_t *__restrict__ keys, uint64_t n, int lo_bit, int hi_bit,
+    int npass, uint32_t *ghist) {
+    __shared__ radix::LdsHist s;
+    radix::hist_zero(&s);
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += stride)
+        radix::hist_add(&s, keys[i], lo_bit, hi_bit, npass);
+    __syncthreads();
+    radix::hist_flush(&s, ghist, npass);
+}
+
+// Exclusive scan of each pass's 256-bin histogram: one block per pass.
+__global__ __launch_bounds__(RBINS) void k_rs_scan_hist(const uint32_t *ghist,
+                                                        uint32_t *gofs) {
+    __shared__ uint32_t s_tmp[RBINS / 64];
+    const uint32_t v = ghist[blockIdx.x * RBINS + threadIdx.x];
+    uint32_t tot;
+    const uint32_t ex = dev::block_excl_scan<RBINS>(v, s_tmp, &tot);
+    gofs[blockIdx.x * RBINS + threadIdx.x] = ex;
+}
+
+template <int ITEMS>
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_onesweep(
+    const uint64_t *__restrict__ in, uint64_t *__restrict__ out, uint64_t n,
+    int shift, int width, const uint32_t *__restrict__ gofs,
+    uint32_t *lookback, uint32_t *tile_counter) {
+    constexpr int TILE = RS_BLOCK * ITEMS;
+    __shared__ uint64_t s_keys[TILE];
+    __shared__ uint32_t s_wcnt[RS_WAVES][RBINS];
+    __shared__ uint32_t s_lstart[RBINS];
+    __shared__ uint32_t s_gbase[RBINS];
+    __shared__ uint32_t s_tmp[RS_WAVES];
+    __shared__ uint32_t s_tile;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int w = tid >> 6;
+
+    for (int i = tid; i < RS_WAVES * RBINS; i += RS_BLOCK) (&s_wcnt[0][0])[i] = 0;
+    if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
+    __syncthreads();
+
+    const uint32_t tile = s_tile;
+    const uint64_t base = (uint64_t)tile * TILE;
+    const uint64_t left = n - base;  // > 0: grid has exactly ceil(n/TILE) tiles
+    const uint32_t nvalid = left < (uint64_t)TILE ? (uint32_t)left : (uint32_t)TILE;
+    const uint32_t dmask = (1u << width) - 1u;
+
+    // Load: position within the tile = (w * ITEMS + j) * 64 + lane, which is
+    // also the ranking order below.  Missing keys are all-ones: they carry
+    // the largest digit and rank after every real key of the tile.
+    uint64_t k[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint64_t idx = base + (uint64_t)((w * ITEMS + j) * 64 + lane);
+        k[j] = idx < n ? in[idx] : ~0ull;
+    }
+
+    // Rank within the wave: peers = lanes holding the same digit.
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint32_t rk[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t d = (uint32_t)(k[j] >> shift) & dmask;
+        uint64_t peers = ~0ull;
+        for (int b = 0; b < width; ++b) {
+            const int bit = (d >> b) & 1;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const uint32_t before = s_wcnt[w][d];
+        const uint32_t r = (uint32_t)__popcll(peers & lt);
+        const uint32_t cnt = (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+        if (r == cnt - 1) s_wcnt[w][d] = before + cnt;
+        __builtin_amdgcn_wave_barrier();
+        rk[j] = before + r;
+    }
+    __syncthreads();
+
+    // Per digit (thread tid = digit): wave-exclusive offsets and tile count.
+    uint32_t tcount = 0;
+    {
+        uint32_t sum = 0;
+#pragma unroll
+        for (int ww = 0; ww < RS_WAVES; ++ww) {
+            const uint32_t c = s_wcnt[ww][tid];
+            s_wcnt[ww][tid] = sum;
+            sum += c;
+        }
+        tcount = sum;
+    }
+    uint32_t pub = tcount;
+    if ((uint32_t)tid == dmask) pub -= (uint32_t)TILE - nvalid;  // padding
+
+    uint32_t excl = 0;
+    uint32_t *mine = lookback + (uint64_t)tile * RBINS + tid;
+    if (tile == 0) {
+        lb_store(mine, LB_INC | pub);
+    } else {
+        lb_store(mine, LB_AGG | pub);
+        int64_t j = (int64_t)tile - 1;
+        for (;;) {
+            const uint32_t v = lb_load(lookback + (uint64_t)j * RBINS + tid);
+            const uint32_t st = v & ~LB_VAL;
+            if (st == 0) {
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            excl += v & LB_VAL;
+            if (st == LB_INC) break;
+            --j;
+        }
+        lb_store(mine, LB_INC | (excl + pub));
+    }
+
+    uint32_t tot;
+    const uint32_t lstart = dev::block_excl_scan<RS_BLOCK>(tcount, s_tmp, &tot);
+    s_lstart[tid] = lstart;
+    s_gbase[tid] = gofs[tid] + excl - lstart;
+    __syncthreads();
+
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t d = (uint32_t)(k[j] >> shift) & dmask;
+        s_keys[s_lstart[d] + s_wcnt[w][d] + rk[j]] = k[j];
+    }
+    __syncthreads();
+
+    for (uint32_t i = tid; i < nvalid; i += RS_BLOCK) {
+        const uint64_t key = s_keys[i];
+        const uint32_t d = (uint32_t)(key >> shift) & dmask;
+        out[(uint64_t)(s_gbase[d] + i)] = key;
+    }
+}
+
+template <int ITEMS>
+void run_passes(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, uint64_t n,
+                int lo_bit, int hi_bit, int npass, const uint32_t *gofs) {
+    constexpr uint64_t TILE = RS_BLOCK * ITEMS;
+    const uint64_t ntiles = ceil_div(n, TILE);
+    const uint64_t lb_words = (uint64_t)npass * ntiles * RBINS;
+    DBuf<uint32_t> lb(ctx, lb_words + npass);
+    fill_u32(ctx, lb.p, 0, lb_words + npass);
+    uint32_t *counters = lb.p + lb_words;
+    for (int p = 0; p < npass; ++p) {
+        const int shift = lo_bit + p * RBITS;
+        const int width = std::min(RBITS, hi_bit - shift);
+        launch(ctx, "radix_onesweep", (double)n * 16.0, dim3((unsigned)ntiles),
+               dim3(RS_BLOCK), 0, k_rs_onesweep<ITEMS>, (const uint64_t *)a, b,
+               n, shift, width, gofs + (uint64_t)p * RBINS,
+               lb.p + (uint64_t)p * ntiles * RBINS, counters + p);
+        std::swap(a, b);
+    }
+}
+
+}  // namespace
+
+int radix_passes(int lo_bit, int hi_bit) {
+    if (hi_bit <= lo_bit) return 0;
+    return (hi_bit - lo_bit + RBITS - 1) / RBITS;
+}
+
+uint64_t *radix_sort_u64(vd_ctx *ctx, uint64_t *keys, uint64_t *tmp,
+                         uint64_t n, int lo_bit, int hi_bit, uint32_t *d_hist,
+                         bool hist_ready) {
+    const int npass = radix_passes(lo_bit, hi_bit);
+    VD_REQUIRE(lo_bit >= 0 && hi_bit <= 64, VD_EINVAL, "radix bit range");
+    VD_REQUIRE(npass <= radix::MAX_PASSES, VD_EINVAL, "too many radix passes");
+    if (n <= 1 || npass == 0) return keys;
+    VD_REQUIRE(n <= (uint64_t)LB_VAL, VD_ERANGE,
+               "radix sort supports at most 2^30-1 keys per call");
+    if (!hist_ready) {
+        fill_u32(ctx, d_hist, 0, (uint64_t)npass * RBINS);
+        uint64_t g = ceil_div(n, 256 * 16);
+        if (g > 2048) g = 2048;
+        launch(ctx, "radix_hist", (double)n * 8.0, dim3((unsigned)g),
+               dim3(radix::HIST_BLOCK), 0, k_rs_hist, (const uint64_t *)keys, n,
+               lo_bit, hi_bit, npass, d_hist);
+    }
+    DBuf<uint32_t> gofs(ctx, (uint64_t)npass * RBINS);
+    launch(ctx, "radix_scan_hist", 0.0, dim3(npass), dim3(RBINS), 0,
+           k_rs_scan_hist, (const uint32_t *)d_hist, gofs.p);
+    uint64_t *a = keys, *b = tmp;
+    switch (ctx->radix_items) {
+        case 8: run_passes<8>(ctx, a, b, n, lo_bit, hi_bit, npass, gofs.p); break;
+        case 24: run_passes<24>(ctx, a, b, n, lo_bit, hi_bit, npass, gofs.p); break;
+        default: run_passes<16>(ctx, a, b, n, lo_bit, hi_bit, npass, gofs.p); break;
+    }
+    return a;
+}
+
+}  // namespace vd

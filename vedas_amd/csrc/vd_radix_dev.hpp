@@ -1,0 +1,57 @@
+// Device-side digit histogram used by the radix sort and fused into the
+// kernels that produce sort keys (index packing, row-id packing), so the
+// keys are read once for all passes' histograms.
+#pragma once
+
+#include "vd_internal.hpp"
+
+namespace vd {
+namespace radix {
+
+constexpr int RBITS = 8;
+constexpr int RBINS = 1 << RBITS;
+constexpr int MAX_PASSES = 8;
+// Threads of every kernel that uses the LDS histogram below.
+constexpr int HIST_BLOCK = 256;
+constexpr int HIST_WAVES = HIST_BLOCK / 64;
+
+// LDS histogram: one copy per wave (fewer same-address atomics across
+// waves), MAX_PASSES x 256 bins each.
+struct LdsHist {
+    uint32_t h[HIST_WAVES][MAX_PASSES][RBINS];
+};
+
+__device__ __forceinline__ void hist_zero(LdsHist *s) {
+    uint32_t *p = &s->h[0][0][0];
+    for (int i = threadIdx.x; i < HIST_WAVES * MAX_PASSES * RBINS; i += HIST_BLOCK)
+        p[i] = 0;
+}
+
+// Count the digits of one key for passes [0, npass), digit p covering bits
+// [lo_bit + 8p, min(lo_bit + 8p + 8, hi_bit)).
+__device__ __forceinline__ void hist_add(LdsHist *s, uint64_t key, int lo_bit,
+                                         int hi_bit, int npass) {
+    const int w = threadIdx.x / 64;
+    for (int p = 0; p < npass; ++p) {
+        const int sh = lo_bit + p * RBITS;
+        const int width = min(RBITS, hi_bit - sh);
+        const uint32_t d = (uint32_t)(key >> sh) & ((1u << width) - 1u);
+        atomicAdd(&s->h[w][p][d], 1u);
+    }
+}
+
+// Fold the block's LDS histogram into the global one (npass x 256 words).
+// Must be preceded by a __syncthreads().
+__device__ __forceinline__ void hist_flush(LdsHist *s, uint32_t *ghist,
+                                           int npass) {
+    for (int i = threadIdx.x; i < npass * RBINS; i += HIST_BLOCK) {
+        const int p = i / RBINS, d = i % RBINS;
+        uint32_t c = 0;
+#pragma unroll
+        for (int w = 0; w < HIST_WAVES; ++w) c += s->h[w][p][d];
+        if (c) atomicAdd(&ghist[i], c);
+    }
+}
+
+}  // namespace radix
+}  // namespace vd

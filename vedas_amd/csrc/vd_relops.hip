@@ -1,0 +1,273 @@
+// Relation re-sorts and finalisation: sort by column 0 (IndexSwap),
+// lexicographic sort, and removal of adjacent duplicate rows.
+//
+// sort_col0: key = col0 << 32 | row, radix-sorted on the bits of col0 only
+// (the row id in the low word keeps the original order of equal keys, so
+// the result is the stable sort), then one gather pass writes column 0 from
+// the keys and the other columns through the row ids.
+
+#include "vd_radix_dev.hpp"
+
+namespace vd {
+namespace {
+
+constexpr int RB = radix::HIST_BLOCK;
+constexpr int R_ITEMS = 16;
+constexpr int U_BLOCK = 256;
+constexpr int U_ITEMS = 8;
+constexpr int U_TILE = U_BLOCK * U_ITEMS;
+
+struct Cols {
+    const uint32_t *in[VD_MAX_COLS];
+    uint32_t *out[VD_MAX_COLS];
+    int n;
+};
+
+// key = col[perm[i]] << 32 | i (perm may be null = identity).
+__global__ __launch_bounds__(RB) void k_pack_rowid(
+    const uint32_t *__restrict__ col, const uint32_t *__restrict__ perm,
+    uint64_t n, uint64_t *__restrict__ keys, int hi_bit, int npass,
+    uint32_t *ghist) {
+    __shared__ radix::LdsHist s;
+    radix::hist_zero(&s);
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * (RB * R_ITEMS);
+#pragma unroll 4
+    for (int j = 0; j < R_ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)j * RB + threadIdx.x;
+        if (i < n) {
+            const uint32_t v = perm ? col[perm[i]] : col[i];
+            const uint64_t key = ((uint64_t)v << 32) | (uint64_t)i;
+            keys[i] = key;
+            radix::hist_add(&s, key, 32, hi_bit, npass);
+        }
+    }
+    __syncthreads();
+    radix::hist_flush(&s, ghist, npass);
+}
+
+// out0[i] = key >> 32; out_c[i] = in_c[row] for c >= 1.
+__global__ __launch_bounds__(256) void k_gather_sorted(
+    const uint64_t *__restrict__ keys, uint64_t n, Cols cols) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += stride) {
+        const uint64_t k = keys[i];
+        const uint32_t row = (uint32_t)k;
+        cols.out[0][i] = (uint32_t)(k >> 32);
+        for (int c = 1; c < cols.n; ++c) cols.out[c][i] = cols.in[c][row];
+    }
+}
+
+// perm2[i] = perm[key_i & 0xffffffff] (perm null = identity).
+__global__ __launch_bounds__(256) void k_compose_perm(
+    const uint64_t *__restrict__ keys, const uint32_t *__restrict__ perm,
+    uint64_t n, uint32_t *__restrict__ perm2) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += stride) {
+        const uint32_t r = (uint32_t)keys[i];
+        perm2[i] = perm ? perm[r] : r;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gather_perm(
+    const uint32_t *__restrict__ perm, uint64_t n, Cols cols) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += stride) {
+        const uint32_t r = perm[i];
+        for (int c = 0; c < cols.n; ++c) cols.out[c][i] = cols.in[c][r];
+    }
+}
+
+__device__ __forceinline__ bool row_differs(const Cols &cols, uint64_t i) {
+    if (i == 0) return true;
+    for (int c = 0; c < cols.n; ++c)
+        if (cols.in[c][i] != cols.in[c][i - 1]) return true;
+    return false;
+}
+
+__global__ __launch_bounds__(U_BLOCK) void k_unique_count(Cols cols, uint64_t n,
+                                                          uint32_t *cnt) {
+    __shared__ uint32_t s_tmp[U_BLOCK / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * U_TILE;
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < U_ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)j * U_BLOCK + threadIdx.x;
+        if (i < n) c += row_differs(cols, i);
+    }
+    const uint32_t t = dev::block_reduce<U_BLOCK>(c, s_tmp);
+    if (threadIdx.x == 0) cnt[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(U_BLOCK) void k_unique_scatter(
+    Cols cols, uint64_t n, const uint64_t *__restrict__ off) {
+    __shared__ uint32_t s_tmp[U_BLOCK / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * U_TILE;
+    bool keep[U_ITEMS];
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < U_ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)threadIdx.x * U_ITEMS + j;
+        keep[j] = i < n && row_differs(cols, i);
+        c += keep[j];
+    }
+    uint32_t tot;
+    uint64_t o = off[blockIdx.x] + dev::block_excl_scan<U_BLOCK>(c, s_tmp, &tot);
+#pragma unroll
+    for (int j = 0; j < U_ITEMS; ++j) {
+        if (!keep[j]) continue;
+        const uint64_t i = base + (uint64_t)threadIdx.x * U_ITEMS + j;
+        for (int cc = 0; cc < cols.n; ++cc) cols.out[cc][o] = cols.in[cc][i];
+        ++o;
+    }
+}
+
+unsigned sgrid(uint64_t n) {
+    uint64_t g = ceil_div(n, 256 * 4);
+    if (g > 16384) g = 16384;
+    return g ? (unsigned)g : 1u;
+}
+
+// Sort keys (col[perm[i]] << 32 | i) on the bits of col; returns sorted keys
+// (owned by one of the two buffers).
+uint64_t *sort_by_col(vd_ctx *ctx, const uint32_t *col, const uint32_t *perm,
+                      uint64_t n, DBuf<uint64_t> &keys, DBuf<uint64_t> &tmp) {
+    const int b = std::max(1, bits_for(max_u32(ctx, col, n)));
+    const int hi = 32 + b;
+    const int npass = radix_passes(32, hi);
+    DBuf<uint32_t> hist(ctx, radix_hist_words(32, hi));
+    fill_u32(ctx, hist.p, 0, radix_hist_words(32, hi));
+    launch(ctx, "sort_pack", (double)n * (perm ? 16.0 : 12.0),
+           dim3((unsigned)ceil_div(n, RB * R_ITEMS)), dim3(RB), 0, k_pack_rowid,
+           col, perm, n, keys.p, hi, npass, hist.p);
+    return radix_sort_u64(ctx, keys.p, tmp.p, n, 32, hi, hist.p, true);
+}
+
+void sort_col0(vd_ctx *ctx, vd_rel *rel) {
+    const uint64_t n = rel->nrows;
+    if (n <= 1) return;
+    VD_REQUIRE(n < (1ull << 30), VD_ERANGE, "sort supports fewer than 2^30 rows");
+    DBuf<uint64_t> keys(ctx, n), tmp(ctx, n);
+    uint64_t *sorted = sort_by_col(ctx, rel->cols[0], nullptr, n, keys, tmp);
+    Cols cols{};
+    cols.n = rel->ncols;
+    uint32_t *fresh[VD_MAX_COLS] = {};
+    try {
+        cols.out[0] = rel->cols[0];  // column 0 comes from the keys: in place
+        for (int c = 1; c < rel->ncols; ++c) {
+            fresh[c] = (uint32_t *)dalloc(ctx, n * 4);
+            cols.in[c] = rel->cols[c];
+            cols.out[c] = fresh[c];
+        }
+    } catch (...) {
+        for (auto p : fresh) dfree(ctx, p);
+        throw;
+    }
+    launch(ctx, "sort_gather", (double)n * (8.0 + 4.0 + 8.0 * (rel->ncols - 1)),
+           dim3(sgrid(n)), dim3(256), 0, k_gather_sorted,
+           (const uint64_t *)sorted, n, cols);
+    for (int c = 1; c < rel->ncols; ++c) {
+        dfree(ctx, rel->cols[c]);
+        rel->cols[c] = fresh[c];
+    }
+}
+
+void sort_lex(vd_ctx *ctx, vd_rel *rel) {
+    const uint64_t n = rel->nrows;
+    if (n <= 1) return;
+    VD_REQUIRE(n < (1ull << 30), VD_ERANGE, "sort supports fewer than 2^30 rows");
+    DBuf<uint64_t> keys(ctx, n), tmp(ctx, n);
+    DBuf<uint32_t> perm(ctx, n), perm2(ctx, n);
+    bool have_perm = false;
+    for (int c = rel->ncols - 1; c >= 0; --c) {
+        uint64_t *sorted = sort_by_col(ctx, rel->cols[c],
+                                       have_perm ? perm.p : nullptr, n, keys, tmp);
+        launch(ctx, "sort_compose", (double)n * 16.0, dim3(sgrid(n)), dim3(256),
+               0, k_compose_perm, (const uint64_t *)sorted,
+               have_perm ? (const uint32_t *)perm.p : (const uint32_t *)nullptr,
+               n, perm2.p);
+        std::swap(perm.p, perm2.p);
+        have_perm = true;
+    }
+    Cols cols{};
+    cols.n = rel->ncols;
+    uint32_t *fresh[VD_MAX_COLS] = {};
+    try {
+        for (int c = 0; c < rel->ncols; ++c) {
+            fresh[c] = (uint32_t *)dalloc(ctx, n * 4);
+            cols.in[c] = rel->cols[c];
+            cols.out[c] = fresh[c];
+        }
+    } catch (...) {
+        for (auto p : fresh) dfree(ctx, p);
+        throw;
+    }
+    launch(ctx, "sort_gather", (double)n * (4.0 + 8.0 * rel->ncols),
+           dim3(sgrid(n)), dim3(256), 0, k_gather_perm, (const uint32_t *)perm.p,
+           n, cols);
+    for (int c = 0; c < rel->ncols; ++c) {
+        dfree(ctx, rel->cols[c]);
+        rel->cols[c] = fresh[c];
+    }
+}
+
+void unique_adjacent(vd_ctx *ctx, vd_rel *rel) {
+    const uint64_t n = rel->nrows;
+    if (n <= 1) return;
+    Cols cols{};
+    cols.n = rel->ncols;
+    for (int c = 0; c < rel->ncols; ++c) cols.in[c] = rel->cols[c];
+    const uint64_t tiles = ceil_div(n, U_TILE);
+    DBuf<uint32_t> cnt(ctx, tiles);
+    DBuf<uint64_t> off(ctx, tiles + 1);
+    launch(ctx, "unique_count", (double)n * 4.0 * rel->ncols, dim3((unsigned)tiles),
+           dim3(U_BLOCK), 0, k_unique_count, cols, n, cnt.p);
+    scan_u32_to_u64(ctx, cnt.p, off.p, tiles, off.p + tiles);
+    const uint64_t kept = read_u64(ctx, off.p + tiles);
+    if (kept == n) return;
+    uint32_t *fresh[VD_MAX_COLS] = {};
+    try {
+        for (int c = 0; c < rel->ncols; ++c) {
+            fresh[c] = (uint32_t *)dalloc(ctx, (kept ? kept : 1) * 4);
+            cols.out[c] = fresh[c];
+        }
+    } catch (...) {
+        for (auto p : fresh) dfree(ctx, p);
+        throw;
+    }
+    launch(ctx, "unique_scatter", 4.0 * rel->ncols * (double)(n + kept),
+           dim3((unsigned)tiles), dim3(U_BLOCK), 0, k_unique_scatter, cols, n,
+           (const uint64_t *)off.p);
+    for (int c = 0; c < rel->ncols; ++c) {
+        dfree(ctx, rel->cols[c]);
+        rel->cols[c] = fresh[c];
+    }
+    rel->nrows = kept;
+}
+
+}  // namespace
+}  // namespace vd
+
+using namespace vd;
+
+extern "C" {
+
+int vd_sort_col0(vd_ctx *ctx, vd_rel *rel) {
+    if (!ctx || !rel) return VD_EINVAL;
+    VD_TRY(ctx, sort_col0(ctx, rel));
+}
+
+int vd_sort_lex(vd_ctx *ctx, vd_rel *rel) {
+    if (!ctx || !rel) return VD_EINVAL;
+    VD_TRY(ctx, sort_lex(ctx, rel));
+}
+
+int vd_unique_adjacent(vd_ctx *ctx, vd_rel *rel) {
+    if (!ctx || !rel) return VD_EINVAL;
+    VD_TRY(ctx, unique_adjacent(ctx, rel));
+}
+
+}  // extern "C"

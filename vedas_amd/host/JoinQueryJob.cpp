@@ -1,0 +1,184 @@
+#include "JoinQueryJob.h"
+
+#include <chrono>
+#include <iostream>
+
+#include "QueryExecutor.h"
+
+JoinQueryJob::JoinQueryJob(QueryJob *leftJob, QueryJob *rightJob, std::string joinVariable,
+                           VedasContext *context, VariableBoundMap *variables_bound,
+                           EmptyIntervalDict *ei_dict, bool lastJoinForVar)
+    : variables_bound(variables_bound), ei_dict(ei_dict), joinVariable(joinVariable),
+      context(context), leftJob(leftJob), rightJob(rightJob), lastJoinForVar(lastJoinForVar) {}
+
+JoinQueryJob::~JoinQueryJob() { delete intermediateResult; }
+
+IR *JoinQueryJob::getIR() { return intermediateResult; }
+
+IR *JoinQueryJob::releaseIR() {
+    IR *r = intermediateResult;
+    intermediateResult = nullptr;
+    return r;
+}
+
+int JoinQueryJob::startJob(int gpuId) {
+    this->gpuId = gpuId;
+    ContextGuard guard(context);
+    auto t0 = std::chrono::steady_clock::now();
+    auto *l = dynamic_cast<FullRelationIR *>(leftJob->getIR());
+    auto *r = dynamic_cast<FullRelationIR *>(rightJob->getIR());
+    if (!l || !r) throw VedasError(VD_EINVAL, "join operand has no relation");
+    intermediateResult = join(l, r);
+    const long long ns = std::chrono::duration_cast<std::chrono::nanoseconds>(
+        std::chrono::steady_clock::now() - t0).count();
+    context->join_ns += ns;
+    if (planTreeNode) {
+        planTreeNode->resultSize = dynamic_cast<FullRelationIR *>(intermediateResult)->size();
+        planTreeNode->nanosecTime = ns;
+    }
+    return 0;
+}
+
+static void split(const std::string &str, char delim, std::vector<std::string> &out) {
+    size_t start, end = 0;
+    while ((start = str.find_first_not_of(delim, end)) != std::string::npos) {
+        end = str.find(delim, start);
+        out.push_back(str.substr(start, end - start));
+    }
+}
+
+// JoinQueryJob::join, JoinQueryJob.cu:125-453.
+IR *JoinQueryJob::join(FullRelationIR *lir, FullRelationIR *rir) {
+    if (lir->size() > rir->size()) std::swap(lir, rir);
+
+    std::vector<std::string> joinVars;
+    split(joinVariable, ',', joinVars);
+    const bool join1Var = joinVars.size() == 1;
+    if (joinVars.empty() || joinVars.size() > 2)
+        throw VedasError(VD_EINVAL, "join variable list must hold 1 or 2 variables");
+
+    // Projection push-down (:137-155): drop columns whose variable no later
+    // operator needs (counter 0) and that are not join variables.  A column
+    // is dropped at most once.
+    auto counter = [&](const std::string &h) -> size_t {
+        return query_variable_counter ? (*query_variable_counter)[h] : 1;
+    };
+    auto isJoinVar = [&](const std::string &h) {
+        for (auto &v : joinVars) if (v == h) return true;
+        return false;
+    };
+    for (int i = (int)lir->getColumnNum() - 1; i >= 0; --i) {
+        const std::string h = lir->getHeader(i);
+        if (counter(h) == 0 && !isJoinVar(h)) lir->removeColumn(i, joinVars[0]);
+    }
+    for (int i = (int)rir->getColumnNum() - 1; i >= 0; --i) {
+        const std::string h = rir->getHeader(i);
+        if (counter(h) == 0 && !isJoinVar(h)) rir->removeColumn(i, joinVars[0]);
+    }
+    if (query_variable_counter)
+        for (auto &v : joinVars) (*query_variable_counter)[v] -= 1;
+
+    auto swap_start = std::chrono::steady_clock::now();
+    if (join1Var) {
+        // Implicit index swap when the key is not column 0 (:162-180).
+        const size_t lIdx = lir->getColumnId(joinVars[0]);
+        if (lIdx != 0) {
+            lir->swapColumn(0, lIdx);
+            lir->sortByFirstColumn();
+            context->exe_log.emplace_back(gpuId, SWAP_OP, "", lir->size(), lir->getColumnNum());
+            context->swap_rows += lir->size();
+        }
+        const size_t rIdx = rir->getColumnId(joinVars[0]);
+        if (rIdx != 0) {
+            rir->swapColumn(0, rIdx);
+            rir->sortByFirstColumn();
+            context->exe_log.emplace_back(gpuId, SWAP_OP, "", rir->size(), rir->getColumnNum());
+            context->swap_rows += rir->size();
+        }
+    } else {
+        // Two join variables: both relations ordered on (v0, v1) (:181-225).
+        const bool aligned =
+            lir->getColumnId(joinVars[0]) == rir->getColumnId(joinVars[0]) &&
+            lir->getColumnId(joinVars[1]) == rir->getColumnId(joinVars[1]) &&
+            ((lir->getColumnId(joinVars[0]) == 0 && lir->getColumnId(joinVars[1]) == 1) ||
+             (lir->getColumnId(joinVars[0]) == 1 && lir->getColumnId(joinVars[1]) == 0));
+        if (!aligned) {
+            rir->swapColumn(0, rir->getColumnId(joinVars[0]));
+            rir->swapColumn(1, rir->getColumnId(joinVars[1]));
+            lir->swapColumn(0, lir->getColumnId(joinVars[0]));
+            lir->swapColumn(1, lir->getColumnId(joinVars[1]));
+            rir->sort();
+            lir->sort();
+        }
+    }
+    context->swap_index_ns += std::chrono::duration_cast<std::chrono::nanoseconds>(
+        std::chrono::steady_clock::now() - swap_start).count();
+
+    const size_t nk = joinVars.size();
+    vd_rel *out = nullptr;
+    if (lir->getColumnNum() == 0 || rir->getColumnNum() == 0)
+        throw VedasError(VD_EINVAL, "join operand without columns");
+    context->check(vd_join(context->raw(), lir->raw(), rir->raw(), (int)nk, &out), "vd_join");
+    FullRelationIR *newIr = new FullRelationIR(context, out);
+    size_t h = 0;
+    for (size_t i = 0; i < nk; ++i) newIr->setHeader(h++, lir->getHeader(i), lir->getIsPredicate(i));
+    for (size_t i = nk; i < lir->getColumnNum(); ++i)
+        newIr->setHeader(h++, lir->getHeader(i), lir->getIsPredicate(i));
+    for (size_t i = nk; i < rir->getColumnNum(); ++i)
+        newIr->setHeader(h++, rir->getHeader(i), rir->getIsPredicate(i));
+    const size_t result_size = newIr->size();
+    context->join_rows += result_size;
+    context->exe_log.emplace_back(gpuId, JOIN_OP, lir->getHeaders("") + " x " + rir->getHeaders(""),
+                                  lir->size(), rir->size(), result_size);
+
+    // Bound and empty-interval updates after the join (:388-433); pruning
+    // only, the result is not affected.
+    if (QueryExecutor::ENABLE_UPDATE_BOUND_AFTER_JOIN && variables_bound) {
+        if (result_size > 0) {
+            TYPEID ends[2];
+            context->check(vd_copy_d2h(context->raw(), &ends[0], newIr->getRelationRawPointer(0), 4),
+                           "vd_copy_d2h");
+            context->check(vd_copy_d2h(context->raw(), &ends[1],
+                                       newIr->getRelationRawPointer(0) + result_size - 1, 4),
+                           "vd_copy_d2h");
+            QueryExecutor::updateBoundDict(variables_bound, joinVariable, ends[0], ends[1]);
+            if (!lastJoinForVar && result_size > 2 && result_size < EIF_ALPHA && ei_dict) {
+                auto e0 = std::chrono::steady_clock::now();
+                size_t b[4];
+                eifFromSortedColumn(newIr->download(0), b);
+                context->eif_count++;
+                ei_dict->updateBound(joinVariable, b[0], b[1], b[2], b[3]);
+                context->eif_ns += std::chrono::duration_cast<std::chrono::nanoseconds>(
+                    std::chrono::steady_clock::now() - e0).count();
+            }
+        } else {
+            QueryExecutor::updateBoundDict(variables_bound, joinVariable, 0, 0);
+        }
+    }
+    return newIr;
+}
+
+void JoinQueryJob::setOperands(QueryJob *l, QueryJob *r) {
+    leftJob = l;
+    rightJob = r;
+}
+
+void JoinQueryJob::print() const {
+    std::cout << "\tJOIN JOB - on " << joinVariable << "\n";
+}
+
+void JoinQueryJob::setQueryVarCounter(std::map<std::string, size_t> *c) { query_variable_counter = c; }
+
+unsigned JoinQueryJob::getLeftIRSize() const {
+    auto *ir = dynamic_cast<FullRelationIR *>(leftJob->getIR());
+    return ir ? (unsigned)ir->size() : 0;
+}
+
+unsigned JoinQueryJob::getRightIRSize() const {
+    auto *ir = dynamic_cast<FullRelationIR *>(rightJob->getIR());
+    return ir ? (unsigned)ir->size() : 0;
+}
+
+std::string JoinQueryJob::getJoinVariable() const { return joinVariable; }
+
+std::string JoinQueryJob::jobTypeName() const { return "Join Job"; }

@@ -1,0 +1,55 @@
+// Select operator (reference SelectQueryJob.h:11-59).
+//
+// The reference constructors take the host L1/L2 arrays and data columns of
+// one permutation; here an IndexRef (storage + permutation) names the
+// HBM-resident arrays, and the lookups run on the device.
+#ifndef VEDAS_HOST_SELECTQUERYJOB_H
+#define VEDAS_HOST_SELECTQUERYJOB_H
+
+#include <string>
+
+#include "EmptyIntervalDict.h"
+#include "FullRelationIR.h"
+#include "QueryJob.h"
+#include "VedasStorage.h"
+
+struct IndexRef {
+    VedasStorage *storage;
+    int perm;  // VD_PERM_*
+};
+
+class SelectQueryJob : public QueryJob {
+  public:
+    // 2-variable pattern: rows with X == id1; columns [v1 = Y, v2 = Z]
+    // (SelectQueryJob.cu:20-44).
+    SelectQueryJob(IndexRef index, std::string v1, std::string v2, TYPEID id1,
+                   VariableBoundMap *variables_bound, EmptyIntervalDict *ei_dict,
+                   bool *is_predicates, bool is_second_var_used, VedasContext *context);
+    // 1-variable pattern: rows with X == id1 and Y == id2; column [v1 = Z]
+    // (SelectQueryJob.cu:46-66).
+    SelectQueryJob(IndexRef index, std::string v1, TYPEID id1, TYPEID id2,
+                   VariableBoundMap *variables_bound, EmptyIntervalDict *ei_dict,
+                   bool *is_predicates, VedasContext *context);
+    ~SelectQueryJob() override;
+    int startJob(int gpuId) override;
+    IR *getIR() override;
+    IR *releaseIR() override;
+    std::string getVariable(size_t i) const;
+    size_t getVariableNum() const;
+    TYPEID getId(size_t i) const;
+    void print() const override;
+    std::string jobTypeName() const override;
+
+  private:
+    IndexRef index;
+    std::string variables[3];
+    size_t variable_num;
+    TYPEID dataid[2] = {0, 0};
+    bool is_predicates[2] = {false, false};
+    bool is_second_var_used = true;
+    VariableBoundMap *variables_bound{nullptr};
+    EmptyIntervalDict *ei_dict{nullptr};
+    VedasContext *context;
+};
+
+#endif

@@ -1,0 +1,265 @@
+// C entry points of libvedas.so over the C++ operator API.
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <cctype>
+#include <set>
+#include <sstream>
+
+#include "QueryExecutor.h"
+#include "vedas_host.h"
+
+struct vdq_engine {
+    std::unique_ptr<VedasContext> ctx;
+    std::unique_ptr<VedasStorage> storage;
+};
+
+struct vdq_result {
+    SparqlResult result;
+    std::vector<std::string> headers;
+    vdq_stats_t stats{};
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+template <typename F>
+int guarded(F &&f) {
+    try {
+        f();
+        return VD_OK;
+    } catch (const VedasError &e) {
+        return fail(e.code(), e.what());
+    } catch (const std::bad_alloc &) {
+        return fail(VD_ENOMEM, "host allocation failed");
+    } catch (const std::exception &e) {
+        return fail(VD_EINVAL, e.what());
+    }
+}
+
+// ---- query text -------------------------------------------------------------------
+struct Parser {
+    std::vector<std::string> tok;
+    size_t pos = 0;
+    explicit Parser(const std::string &s) {
+        std::string cur;
+        auto flush = [&]() { if (!cur.empty()) { tok.push_back(cur); cur.clear(); } };
+        for (char c : s) {
+            if (std::isspace((unsigned char)c)) { flush(); continue; }
+            if (c == '{' || c == '}' || c == '(' || c == ')' || c == '.') {
+                flush();
+                tok.push_back(std::string(1, c));
+                continue;
+            }
+            cur += c;
+        }
+        flush();
+    }
+    bool more() const { return pos < tok.size(); }
+    const std::string &peek() const {
+        if (!more()) throw VedasError(VD_EINVAL, "query text ends early");
+        return tok[pos];
+    }
+    std::string next() { std::string t = peek(); ++pos; return t; }
+    void expect(const std::string &t) {
+        if (next() != t) throw VedasError(VD_EINVAL, "query text: expected " + t);
+    }
+};
+
+struct PlanArena {
+    std::vector<std::unique_ptr<PlanTreeNode>> nodes;
+    PlanTreeNode *make() {
+        nodes.emplace_back(new PlanTreeNode());
+        return nodes.back().get();
+    }
+};
+
+PlanTreeNode *parse_node(Parser &p, PlanArena &a, std::vector<TriplePattern> &pats) {
+    p.expect("(");
+    const std::string op = p.next();
+    PlanTreeNode *n = a.make();
+    if (op == "U") {
+        const size_t i = std::stoul(p.next());
+        if (i >= pats.size()) throw VedasError(VD_EINVAL, "plan names a missing pattern");
+        n->op = UPLOAD;
+        n->tp = &pats[i];
+        n->var = p.next();
+    } else if (op == "J") {
+        n->op = JOIN;
+        n->var = p.next();
+        n->child1 = parse_node(p, a, pats);
+        n->child2 = parse_node(p, a, pats);
+    } else if (op == "S") {
+        n->op = INDEXSWAP;
+        n->var = p.next();
+        n->child1 = parse_node(p, a, pats);
+    } else {
+        throw VedasError(VD_EINVAL, "unknown plan operator " + op);
+    }
+    p.expect(")");
+    return n;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vdq_engine_create(int device, vdq_engine **out) {
+    if (!out) return fail(VD_EINVAL, "null out");
+    return guarded([&] {
+        auto e = std::make_unique<vdq_engine>();
+        e->ctx.reset(new VedasContext(device));
+        *out = e.release();
+    });
+}
+
+int vdq_engine_free(vdq_engine *e) {
+    if (!e) return fail(VD_EINVAL, "null engine");
+    return guarded([&] {
+        e->storage.reset();
+        e->ctx.reset();
+        delete e;
+    });
+}
+
+int vdq_engine_load(vdq_engine *e, const uint32_t *s, const uint32_t *p, const uint32_t *o,
+                    uint64_t n, uint32_t perm_mask) {
+    if (!e) return fail(VD_EINVAL, "null engine");
+    return guarded([&] {
+        vd_ctx *c = e->ctx->raw();
+        void *d[3] = {nullptr, nullptr, nullptr};
+        const uint32_t *h[3] = {s, p, o};
+        try {
+            for (int i = 0; i < 3; ++i) {
+                e->ctx->check(vd_dev_alloc(c, (n ? n : 1) * 4, &d[i]), "vd_dev_alloc");
+                if (n) e->ctx->check(vd_copy_h2d(c, d[i], h[i], n * 4), "vd_copy_h2d");
+            }
+            e->storage.reset();
+            e->storage.reset(new VedasStorage(e->ctx.get(), (uint32_t *)d[0], (uint32_t *)d[1],
+                                              (uint32_t *)d[2], n, perm_mask));
+        } catch (...) {
+            for (auto x : d) if (x) vd_dev_free(c, x);
+            throw;
+        }
+        for (auto x : d) vd_dev_free(c, x);
+    });
+}
+
+int vdq_engine_load_device(vdq_engine *e, const uint32_t *d_s, const uint32_t *d_p,
+                           const uint32_t *d_o, uint64_t n, uint32_t perm_mask) {
+    if (!e) return fail(VD_EINVAL, "null engine");
+    return guarded([&] {
+        e->storage.reset();
+        e->storage.reset(new VedasStorage(e->ctx.get(), d_s, d_p, d_o, n, perm_mask));
+    });
+}
+
+vd_ctx *vdq_engine_ctx(vdq_engine *e) { return e ? e->ctx->raw() : nullptr; }
+vd_index *vdq_engine_index(vdq_engine *e) {
+    return (e && e->storage) ? e->storage->index() : nullptr;
+}
+
+int vdq_query(vdq_engine *e, const char *text, vdq_result **out) {
+    if (!e || !text || !out) return fail(VD_EINVAL, "null argument");
+    if (!e->storage) return fail(VD_EINVAL, "engine has no store loaded");
+    *out = nullptr;
+    return guarded([&] {
+        auto t0 = std::chrono::steady_clock::now();
+        Parser p(text);
+        p.expect("SELECT");
+        std::set<std::string> selected;
+        bool star = false;
+        while (p.peek() != "WHERE") {
+            std::string v = p.next();
+            if (v == "*") star = true; else selected.insert(v);
+        }
+        p.expect("WHERE");
+        p.expect("{");
+        std::vector<TriplePattern> pats;
+        pats.reserve(64);
+        while (p.peek() != "}") {
+            std::string s = p.next(), pr = p.next(), o = p.next();
+            pats.emplace_back(pats.size(), s, pr, o);
+            if (p.peek() == ".") p.next();
+        }
+        p.expect("}");
+        if (pats.empty()) throw VedasError(VD_EINVAL, "no patterns");
+        if (star)
+            for (auto &tp : pats)
+                for (auto t : {tp.getSubject(), tp.getPredicate(), tp.getObject()})
+                    if (!t.empty() && t[0] == '?') selected.insert(t);
+        PlanArena arena;
+        PlanTreeNode *root = nullptr;
+        if (p.more()) {
+            p.expect("PLAN");
+            root = parse_node(p, arena, pats);
+        } else if (pats.size() != 1) {
+            throw VedasError(VD_EINVAL, "PLAN required for more than one pattern");
+        }
+        auto res = std::make_unique<vdq_result>();
+        VedasContext *ctx = e->ctx.get();
+        ctx->resetStats();
+        QueryExecutor ex(e->storage.get(), ctx);
+        ex.query(pats, selected, root, res->result);
+        ctx->sync();
+        FullRelationIR *ir = res->result.getResultIR();
+        for (size_t i = 0; ir && i < ir->getColumnNum(); ++i) res->headers.push_back(ir->getHeader(i));
+        auto &st = res->stats;
+        st.rows = ir ? ir->size() : 0;
+        st.join_rows = ctx->join_rows;
+        st.select_rows = ctx->select_rows;
+        st.swap_rows = ctx->swap_rows;
+        st.join_ms = ctx->join_ns / 1e6;
+        st.select_ms = ctx->upload_ns / 1e6;
+        st.swap_ms = ctx->swap_index_ns / 1e6;
+        st.jobs = (int)ctx->exe_log.size();
+        st.elapsed_ms = std::chrono::duration<double, std::milli>(
+            std::chrono::steady_clock::now() - t0).count();
+        *out = res.release();
+    });
+}
+
+int vdq_result_shape(const vdq_result *r, int *ncols, uint64_t *nrows) {
+    if (!r) return fail(VD_EINVAL, "null result");
+    FullRelationIR *ir = r->result.getResultIR();
+    if (ncols) *ncols = ir ? (int)ir->getColumnNum() : 0;
+    if (nrows) *nrows = ir ? ir->size() : 0;
+    return VD_OK;
+}
+
+const char *vdq_result_header(const vdq_result *r, int col) {
+    if (!r || col < 0 || col >= (int)r->headers.size()) return nullptr;
+    return r->headers[col].c_str();
+}
+
+int vdq_result_column(const vdq_result *r, int col, uint32_t *host_out) {
+    if (!r || !host_out) return fail(VD_EINVAL, "null argument");
+    return guarded([&] {
+        FullRelationIR *ir = r->result.getResultIR();
+        if (!ir || col < 0 || col >= (int)ir->getColumnNum())
+            throw VedasError(VD_EINVAL, "no such column");
+        TYPEID_HOST_VEC v = ir->download(col);
+        if (!v.empty()) std::memcpy(host_out, v.data(), v.size() * 4);
+    });
+}
+
+int vdq_result_stats(const vdq_result *r, vdq_stats_t *out) {
+    if (!r || !out) return fail(VD_EINVAL, "null argument");
+    *out = r->stats;
+    return VD_OK;
+}
+
+int vdq_result_free(vdq_result *r) {
+    if (!r) return fail(VD_EINVAL, "null result");
+    return guarded([&] { delete r; });
+}
+
+const char *vdq_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"
