@@ -150,6 +150,16 @@ class Ctx:
         if rc != VD_OK:
             raise VedasError(rc, "vd_ctx_create failed (no GPU?)")
         self.h = h
+        self.owned = True
+
+    @classmethod
+    def wrap(cls, handle) -> "Ctx":
+        """A non-owning view of a context created elsewhere (libvedas.so)."""
+        self = cls.__new__(cls)
+        self.L = lib()
+        self.h = C.c_void_p(handle)
+        self.owned = False
+        return self
 
     def check(self, rc: int) -> None:
         if rc != VD_OK:
@@ -157,9 +167,9 @@ class Ctx:
             raise VedasError(rc, msg.decode() if msg else "")
 
     def close(self) -> None:
-        if self.h:
+        if self.h and self.owned:
             self.L.vd_ctx_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __enter__(self):
         return self
@@ -303,6 +313,14 @@ class Index:
         self.h = h
 
     @classmethod
+    def wrap(cls, ctx: Ctx, handle, n: int) -> "Index":
+        """A non-owning view of an index built elsewhere (libvedas.so)."""
+        self = cls.__new__(cls)
+        self.ctx, self.n, self.h = ctx, n, C.c_void_p(handle)
+        self.owned = False
+        return self
+
+    @classmethod
     def from_device(cls, ctx: Ctx, ds: int, dp: int, do: int, n: int, perms):
         self = cls.__new__(cls)
         self.ctx, self.n = ctx, n
@@ -345,6 +363,6 @@ class Index:
         return Rel(self.ctx, h)
 
     def free(self) -> None:
-        if self.h:
+        if self.h and getattr(self, "owned", True):
             self.ctx.L.vd_index_free(self.ctx.h, self.h)
-            self.h = None
+        self.h = None

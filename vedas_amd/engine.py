@@ -1,0 +1,164 @@
+"""ctypes binding of include/vedas_host.h (libvedas.so): queries through the
+C++ operator API (QueryExecutor -> SelectQueryJob / JoinQueryJob /
+IndexSwapJob -> QueryPlan), the drop-in path of the reference's vdQuery."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _native
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvedas.so")
+
+
+class Stats(C.Structure):
+    _fields_ = [("rows", C.c_uint64), ("join_rows", C.c_uint64),
+                ("select_rows", C.c_uint64), ("swap_rows", C.c_uint64),
+                ("elapsed_ms", C.c_double), ("join_ms", C.c_double),
+                ("select_ms", C.c_double), ("swap_ms", C.c_double), ("jobs", C.c_int)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} not built: run python -m vedas_amd.build")
+        _native.lib()  # libvedas_hip.so first (same directory, rpath $ORIGIN)
+        l = C.CDLL(LIB_PATH)
+        vp, u64, i = C.c_void_p, C.c_uint64, C.c_int
+        sig = {
+            "vdq_engine_create": (i, [i, C.POINTER(vp)]),
+            "vdq_engine_free": (i, [vp]),
+            "vdq_engine_load": (i, [vp, vp, vp, vp, u64, C.c_uint32]),
+            "vdq_engine_load_device": (i, [vp, vp, vp, vp, u64, C.c_uint32]),
+            "vdq_engine_ctx": (vp, [vp]),
+            "vdq_engine_index": (vp, [vp]),
+            "vdq_query": (i, [vp, C.c_char_p, C.POINTER(vp)]),
+            "vdq_result_shape": (i, [vp, C.POINTER(i), C.POINTER(u64)]),
+            "vdq_result_header": (C.c_char_p, [vp, i]),
+            "vdq_result_column": (i, [vp, i, vp]),
+            "vdq_result_stats": (i, [vp, C.POINTER(Stats)]),
+            "vdq_result_free": (i, [vp]),
+            "vdq_last_error": (C.c_char_p, []),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(l, name)
+            f.restype, f.argtypes = res, args
+        _lib = l
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise _native.VedasError(rc, (lib().vdq_last_error() or b"").decode())
+
+
+class Result:
+    def __init__(self, h):
+        self.h = h
+
+    def shape(self):
+        nc, nr = C.c_int(), C.c_uint64()
+        _check(lib().vdq_result_shape(self.h, C.byref(nc), C.byref(nr)))
+        return nc.value, nr.value
+
+    def headers(self):
+        nc, _ = self.shape()
+        return [lib().vdq_result_header(self.h, i).decode() for i in range(nc)]
+
+    def columns(self):
+        nc, nr = self.shape()
+        out = []
+        for i in range(nc):
+            a = np.empty(nr, np.uint32)
+            _check(lib().vdq_result_column(self.h, i, C.c_void_p(a.ctypes.data)))
+            out.append(a)
+        return out
+
+    def stats(self) -> dict:
+        s = Stats()
+        _check(lib().vdq_result_stats(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+    def free(self):
+        if self.h:
+            lib().vdq_result_free(self.h)
+            self.h = None
+
+
+class Engine:
+    """A device context plus an HBM-resident triple store."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(lib().vdq_engine_create(device, C.byref(h)))
+        self.h = h
+
+    def load(self, s, p, o, perms=("PSO", "POS")):
+        mask = 0
+        for name in perms:
+            mask |= 1 << _native.PERMS[name]
+        s, p, o = (np.ascontiguousarray(x, np.uint32) for x in (s, p, o))
+        _check(lib().vdq_engine_load(self.h, C.c_void_p(s.ctypes.data), C.c_void_p(p.ctypes.data),
+                                     C.c_void_p(o.ctypes.data), len(s), mask))
+
+    def load_device(self, ds: int, dp: int, do: int, n: int, perms):
+        mask = 0
+        for name in perms:
+            mask |= 1 << _native.PERMS[name]
+        _check(lib().vdq_engine_load_device(self.h, C.c_void_p(ds), C.c_void_p(dp),
+                                            C.c_void_p(do), n, mask))
+
+    def ctx_handle(self):
+        return lib().vdq_engine_ctx(self.h)
+
+    def index_handle(self):
+        return lib().vdq_engine_index(self.h)
+
+    def query(self, text: str) -> Result:
+        r = C.c_void_p()
+        _check(lib().vdq_query(self.h, text.encode(), C.byref(r)))
+        return Result(r)
+
+    def close(self):
+        if self.h:
+            lib().vdq_engine_free(self.h)
+            self.h = None
+
+
+# ---- query text helpers ---------------------------------------------------------
+def plan_text(node) -> str:
+    """Plan tree (tests/plans.py form) -> the PLAN s-expression."""
+    op = node[0]
+    if op == "UPLOAD":
+        return f"(U {node[1]} {node[2]})"
+    if op == "JOIN":
+        return f"(J {node[3]} {plan_text(node[1])} {plan_text(node[2])})"
+    if op == "INDEXSWAP":
+        return f"(S {node[2]} {plan_text(node[1])})"
+    raise ValueError(op)
+
+
+def query_text(patterns, node, select="*") -> str:
+    """patterns: list of (s, p, o) terms; node: plan tree whose UPLOAD
+    leaves hold the pattern tuple itself or its index."""
+    idx = {tuple(p): i for i, p in enumerate(patterns)}
+
+    def fix(n):
+        if n[0] == "UPLOAD":
+            pat = n[1]
+            i = pat if isinstance(pat, int) else idx[tuple(pat)]
+            return ("UPLOAD", i, n[2])
+        if n[0] == "JOIN":
+            return ("JOIN", fix(n[1]), fix(n[2]), n[3])
+        return ("INDEXSWAP", fix(n[1]), n[2])
+
+    body = " . ".join(" ".join(str(t) for t in pat) for pat in patterns)
+    sel = select if isinstance(select, str) else " ".join(select)
+    return f"SELECT {sel} WHERE {{ {body} }} PLAN {plan_text(fix(node))}"
