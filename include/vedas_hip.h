@@ -172,6 +172,29 @@ int  vd_sort_lex(vd_ctx *ctx, vd_rel *rel);
  * Replaces FullRelationIR::removeDuplicate (FullRelationIR.cu:338-522). */
 int  vd_unique_adjacent(vd_ctx *ctx, vd_rel *rel);
 
+/* ---- batched operators (many queries per launch) ------------------------ */
+/* No counterpart in the reference, which runs one query and one operator at
+ * a time (QueryPlan.cu:160-205).  Each call runs one operator for many
+ * independent inputs with a fixed number of kernel launches and one stream
+ * synchronisation, whatever the number of inputs; results equal those of
+ * the single-input entry points above, input by input. */
+typedef struct {
+    int      perm;   /* VD_PERM_*                                          */
+    uint32_t k1;     /* X constant                                          */
+    uint32_t k2;     /* Y constant, or VD_UNBOUND                           */
+    int      cols;   /* 1 = Z only, 3 = (Y, Z)   (as vd_select)            */
+} vd_select_job_t;
+/* outs[i] = vd_select of job i over its lookup range (no clip/skip). */
+int  vd_select_multi(vd_ctx *ctx, const vd_index *idx,
+                     const vd_select_job_t *jobs, int njobs, vd_rel **outs);
+/* outs[i] = vd_join(L[i], R[i], 1). */
+int  vd_join_multi(vd_ctx *ctx, const vd_rel *const *L, const vd_rel *const *R,
+                   int npairs, vd_rel **outs);
+/* vd_sort_col0 on every relation. */
+int  vd_sort_col0_multi(vd_ctx *ctx, vd_rel *const *rels, int n);
+/* vd_unique_adjacent on every relation. */
+int  vd_unique_adjacent_multi(vd_ctx *ctx, vd_rel *const *rels, int n);
+
 /* ---- synthetic triples (host) ------------------------------------------ */
 /* Seeded generator of (S,P,O) IDs: subjects over [1, n_entities] uniform
  * (subject_zipf == 0) or Zipf(subject_zipf) by rank; predicates over

@@ -50,6 +50,19 @@ vd_ctx *vdq_engine_ctx(vdq_engine *e);
 vd_index *vdq_engine_index(vdq_engine *e);
 
 int vdq_query(vdq_engine *e, const char *text, vdq_result **out);
+
+/* Many queries at once through the batched operators (no counterpart in
+ * the reference): all plans advance together in rounds, one batched call
+ * per operator kind per round.  outs[i] is query i's result, equal to
+ * vdq_query's after ordering rows and columns canonically. */
+typedef struct {
+    int rounds;             /* operator rounds                          */
+    int calls;              /* batched C-ABI operator calls             */
+    uint64_t join_rows, select_rows, swap_rows;
+    double elapsed_ms;      /* host wall time of the whole batch        */
+} vdq_batch_stats_t;
+int vdq_batch(vdq_engine *e, const char *const *texts, int n, vdq_result **outs,
+              vdq_batch_stats_t *stats);
 int vdq_result_shape(const vdq_result *r, int *ncols, uint64_t *nrows);
 const char *vdq_result_header(const vdq_result *r, int col);
 int vdq_result_column(const vdq_result *r, int col, uint32_t *host_out);

@@ -304,7 +304,7 @@ def test_plans_cfg0_store(ctx, store, shape):
     elif shape == "star4":
         pats, node = plans.star_plan([3, 4, 5, 6])
     elif shape == "chain4":
-        pats, node = plans.chain_plan([10, 11, 12, 13])
+        pats, node = plans.chain_plan([1, 1, 1, 2])
     else:
         pats, node = plans.snowflake_plan([8, 9], [10, 11])
     gex = plans.GpuExecutor(ctx, idx)

@@ -38,6 +38,10 @@ class Interval(C.Structure):
     _fields_ = [("lo", C.c_uint32), ("hi", C.c_uint32)]
 
 
+class SelectJob(C.Structure):
+    _fields_ = [("perm", C.c_int), ("k1", C.c_uint32), ("k2", C.c_uint32), ("cols", C.c_int)]
+
+
 class GenParams(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("n", C.c_uint64),
                 ("n_entities", C.c_uint32), ("subject_zipf", C.c_double),
@@ -87,6 +91,12 @@ EXPORTS = {
                                   C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "vd_join": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                           C.POINTER(C.c_void_p)]),
+    "vd_select_multi": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                  C.POINTER(C.c_void_p)]),
+    "vd_join_multi": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                C.c_int, C.POINTER(C.c_void_p)]),
+    "vd_sort_col0_multi": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int]),
+    "vd_unique_adjacent_multi": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int]),
     "vd_sort_col0": (C.c_int, [C.c_void_p, C.c_void_p]),
     "vd_sort_lex": (C.c_int, [C.c_void_p, C.c_void_p]),
     "vd_unique_adjacent": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -366,3 +376,34 @@ class Index:
         if self.h and getattr(self, "owned", True):
             self.ctx.L.vd_index_free(self.ctx.h, self.h)
         self.h = None
+
+
+# ---- batched operators ---------------------------------------------------------
+def _handles(rels):
+    arr = (C.c_void_p * max(1, len(rels)))()
+    for i, r in enumerate(rels):
+        arr[i] = r.h.value if isinstance(r.h, C.c_void_p) else r.h
+    return arr
+
+
+def select_multi(ctx: Ctx, index: "Index", jobs) -> list:
+    """jobs: [(perm name, k1, k2 or None, cols)] -> [Rel]"""
+    arr = (SelectJob * max(1, len(jobs)))(*[
+        SelectJob(PERMS[p], k1, VD_UNBOUND if k2 is None else k2, cols) for p, k1, k2, cols in jobs])
+    outs = (C.c_void_p * max(1, len(jobs)))()
+    ctx.check(ctx.L.vd_select_multi(ctx.h, index.h, C.cast(arr, C.c_void_p), len(jobs), outs))
+    return [Rel(ctx, C.c_void_p(outs[i])) for i in range(len(jobs))]
+
+
+def join_multi(ctx: Ctx, lefts, rights) -> list:
+    outs = (C.c_void_p * max(1, len(lefts)))()
+    ctx.check(ctx.L.vd_join_multi(ctx.h, _handles(lefts), _handles(rights), len(lefts), outs))
+    return [Rel(ctx, C.c_void_p(outs[i])) for i in range(len(lefts))]
+
+
+def sort_col0_multi(ctx: Ctx, rels) -> None:
+    ctx.check(ctx.L.vd_sort_col0_multi(ctx.h, _handles(rels), len(rels)))
+
+
+def unique_adjacent_multi(ctx: Ctx, rels) -> None:
+    ctx.check(ctx.L.vd_unique_adjacent_multi(ctx.h, _handles(rels), len(rels)))

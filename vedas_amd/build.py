@@ -34,7 +34,7 @@ HIP_FLAGS = [
     "-I", INCLUDE, "-I", CSRC,
 ]
 HIP_SOURCES = ["vd_ctx.hip", "vd_prim.hip", "vd_radix.hip", "vd_index.hip",
-               "vd_join.hip", "vd_relops.hip"]
+               "vd_join.hip", "vd_relops.hip", "vd_batch.hip"]
 HOST_SOURCES_HIPLIB = ["vd_gen.cpp"]
 
 

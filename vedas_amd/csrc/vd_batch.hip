@@ -1,0 +1,627 @@
+// Batched operators: one operator applied to many independent inputs (the
+// queries of a batch) with a fixed number of launches and one stream
+// synchronisation.  Inputs are laid end to end in a virtual index space:
+//  * per-input tile ranges (tbase) map a block to its input, so a tile never
+//    spans two inputs (bounds, max, pack, unique);
+//  * per-input row bases (lbase / rbase) map a global row or output index
+//    back to its input (join expansion, sort gather, select gather).
+// The kernels are the single-input ones of vd_join / vd_relops / vd_index
+// with that extra indirection.
+
+#include <algorithm>
+#include <vector>
+
+#include "vd_index.hpp"
+#include "vd_radix_dev.hpp"
+
+namespace vd {
+namespace {
+
+constexpr int MB_BLOCK = 256;
+constexpr int MB_ITEMS = 8;
+constexpr int MB_TILE = MB_BLOCK * MB_ITEMS;  // rows per tile
+constexpr int MJ_BLOCK = 256;
+constexpr int MJ_NV = 2048;
+constexpr int MJ_RCAP = 8192;  // staged right keys (32 KB)
+
+__device__ __forceinline__ int find_seg(const uint64_t *base, int n, uint64_t x) {
+    // last j in [0, n) with base[j] <= x (base[0] == 0, non-decreasing)
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (base[mid] <= x) lo = mid + 1; else hi = mid;
+    }
+    return lo - 1;
+}
+
+unsigned grid_cap(uint64_t n) {
+    uint64_t g = ceil_div(n, 256 * 4);
+    if (g > 16384) g = 16384;
+    return g ? (unsigned)g : 1u;
+}
+
+// ---- select ------------------------------------------------------------------
+struct SelJobDev {
+    PermDev pd;
+    const uint32_t *y, *z;
+    uint32_t k1, k2;
+    int cols;
+    uint32_t *o0, *o1;
+};
+
+__global__ __launch_bounds__(256) void k_sel_lookup_multi(const SelJobDev *jobs, int nj,
+                                                          uint64_t *begin, uint64_t *sizes) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nj) return;
+    const SelJobDev &J = jobs[j];
+    uint64_t b, e;
+    lookup_one(J.pd, J.k1, J.k2, J.k2 != VD_UNBOUND, &b, &e);
+    begin[j] = b;
+    sizes[j] = e - b;
+}
+
+__global__ __launch_bounds__(256) void k_sel_gather_multi(const SelJobDev *jobs, int nj,
+                                                          const uint64_t *begin,
+                                                          const uint64_t *prefix,
+                                                          uint64_t total) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total; o += stride) {
+        const int j = find_seg(prefix, nj, o);
+        const SelJobDev &J = jobs[j];
+        const uint64_t loc = o - prefix[j];
+        const uint64_t src = begin[j] + loc;
+        if (J.cols == 1) {
+            J.o0[loc] = J.z[src];
+        } else {
+            J.o0[loc] = J.y[src];
+            J.o1[loc] = J.z[src];
+        }
+    }
+}
+
+// ---- join --------------------------------------------------------------------
+struct JPairDev {
+    const uint32_t *lc[VD_MAX_COLS];
+    const uint32_t *rc[VD_MAX_COLS];
+    uint32_t *oc[VD_MAX_COLS];
+    uint64_t nl, nr;
+    uint64_t lbase;  // first global left row
+    uint64_t obase;  // first global output row
+    int ncl, ncr;
+};
+
+__global__ __launch_bounds__(MB_BLOCK) void k_join_bounds_multi(
+    const JPairDev *P, int np, const uint64_t *tbase, uint32_t *lower, uint32_t *cnt) {
+    __shared__ uint32_t s_r[MJ_RCAP];
+    __shared__ uint64_t s_range[2];
+    __shared__ int s_j;
+    if (threadIdx.x == 0) s_j = find_seg(tbase, np, blockIdx.x);
+    __syncthreads();
+    const int j = s_j;
+    const JPairDev &D = P[j];
+    const uint32_t *lk = D.lc[0], *rk = D.rc[0];
+    const uint64_t base = (blockIdx.x - tbase[j]) * (uint64_t)MB_TILE;
+    const uint64_t nl = D.nl, nr = D.nr;
+    const uint64_t last = min(base + MB_TILE, nl) - 1;
+    if (threadIdx.x == 0) {
+        const uint64_t lo = dev::lower_bound(rk, 0, nr, lk[base]);
+        s_range[0] = lo;
+        s_range[1] = dev::upper_bound(rk, lo, nr, lk[last]);
+    }
+    __syncthreads();
+    const uint64_t rlo = s_range[0], rhi = s_range[1], m = rhi - rlo;
+    const bool staged = m <= (uint64_t)MJ_RCAP;
+    if (staged)
+        for (uint64_t t = threadIdx.x; t < m; t += MB_BLOCK) s_r[t] = rk[rlo + t];
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < MB_ITEMS; ++it) {
+        const uint64_t i = base + (uint64_t)it * MB_BLOCK + threadIdx.x;
+        if (i >= nl) break;
+        const uint32_t key = lk[i];
+        uint64_t lo, hi;
+        if (staged) {
+            lo = dev::lower_bound(s_r, 0, m, key);
+            hi = dev::upper_bound(s_r, lo, m, key) + rlo;
+            lo += rlo;
+        } else {
+            lo = dev::lower_bound(rk, rlo, rhi, key);
+            hi = dev::upper_bound(rk, lo, rhi, key);
+        }
+        lower[D.lbase + i] = (uint32_t)lo;
+        cnt[D.lbase + i] = (uint32_t)(hi - lo);
+    }
+}
+
+// Output size of every pair from the global offsets.
+__global__ void k_pair_totals(const JPairDev *P, int np, const uint64_t *off, uint64_t *tot) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= np) return;
+    tot[j] = off[P[j].lbase + P[j].nl] - off[P[j].lbase];
+}
+
+__global__ __launch_bounds__(256) void k_join_partition_multi(const uint64_t *off, uint64_t nl,
+                                                              uint64_t total, uint64_t nblocks,
+                                                              uint64_t *mp) {
+    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > nblocks) return;
+    const uint64_t diag = min(b * (uint64_t)MJ_NV, nl + total);
+    uint64_t lo = 0, hi = nl;
+    while (lo < hi) {
+        const uint64_t mid = lo + ((hi - lo) >> 1);
+        if (mid + off[mid] < diag) lo = mid + 1; else hi = mid;
+    }
+    mp[b] = lo;
+}
+
+__global__ __launch_bounds__(MJ_BLOCK) void k_join_expand_multi(
+    const uint64_t *off, const uint32_t *lower, const uint64_t *mp, uint64_t nl, uint64_t total,
+    const JPairDev *P, int np, const uint64_t *lbase) {
+    constexpr int ITEMS = MJ_NV / MJ_BLOCK;
+    __shared__ uint64_t s_off[MJ_NV + 1];
+    __shared__ uint32_t s_low[MJ_NV + 1];
+    const uint64_t blk = blockIdx.x;
+    const uint64_t d0 = blk * MJ_NV;
+    const uint64_t d1 = min(d0 + MJ_NV, nl + total);
+    const uint64_t a0 = mp[blk], a1 = mp[blk + 1];
+    const uint64_t b0 = d0 - a0, b1 = d1 - a1;
+    if (b1 <= b0) return;
+    const uint64_t sa = a0 > 0 ? a0 - 1 : 0;
+    const int m = (int)(a1 - sa);
+    for (int t = threadIdx.x; t < m; t += MJ_BLOCK) {
+        s_off[t] = off[sa + t];
+        s_low[t] = lower[sa + t];
+    }
+    __syncthreads();
+    const int nout = (int)(b1 - b0);
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+        const int q = it * MJ_BLOCK + threadIdx.x;
+        if (q >= nout) continue;
+        const uint64_t o = b0 + q;
+        int lo = 0, hi = m;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (s_off[mid] <= o) lo = mid + 1; else hi = mid;
+        }
+        const int k = lo - 1;
+        const uint64_t grow = sa + k;  // global left row
+        const int j = find_seg(lbase, np, grow);
+        const JPairDev &D = P[j];
+        const uint64_t lrow = grow - D.lbase;
+        const uint64_t rrow = s_low[k] + (o - s_off[k]);
+        const uint64_t loc = o - D.obase;
+        int oc = 0;
+        for (int c = 0; c < D.ncl; ++c) D.oc[oc++][loc] = D.lc[c][lrow];
+        for (int c = 1; c < D.ncr; ++c) D.oc[oc++][loc] = D.rc[c][rrow];
+    }
+}
+
+// ---- sort by column 0 -----------------------------------------------------------
+struct SortRelDev {
+    const uint32_t *in[VD_MAX_COLS];
+    uint32_t *out[VD_MAX_COLS];
+    uint64_t n, base;
+    int ncols;
+};
+
+__global__ __launch_bounds__(MB_BLOCK) void k_max_col0_multi(const SortRelDev *S, int ns,
+                                                             const uint64_t *tbase,
+                                                             uint32_t *out) {
+    __shared__ int s_j;
+    __shared__ uint32_t s_m[MB_BLOCK / 64];
+    if (threadIdx.x == 0) s_j = find_seg(tbase, ns, blockIdx.x);
+    __syncthreads();
+    const SortRelDev &D = S[s_j];
+    const uint64_t base = (blockIdx.x - tbase[s_j]) * (uint64_t)MB_TILE;
+    uint32_t m = 0;
+    for (int it = 0; it < MB_ITEMS; ++it) {
+        const uint64_t i = base + (uint64_t)it * MB_BLOCK + threadIdx.x;
+        if (i < D.n) m = max(m, D.in[0][i]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor(m, off, 64));
+    if (dev::lane_id() == 0) s_m[threadIdx.x / 64] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t r = 0;
+        for (int w = 0; w < MB_BLOCK / 64; ++w) r = max(r, s_m[w]);
+        atomicMax(out, r);
+    }
+}
+
+// key = seg << (b + R) | col0 << R | global row
+__global__ __launch_bounds__(MB_BLOCK) void k_pack_multi(const SortRelDev *S, int ns,
+                                                         const uint64_t *tbase, int rbits,
+                                                         int cbits, uint64_t *keys, int lo_bit,
+                                                         int hi_bit, int npass, uint32_t *ghist) {
+    __shared__ radix::LdsHist h;
+    __shared__ int s_j;
+    radix::hist_zero(&h);
+    if (threadIdx.x == 0) s_j = find_seg(tbase, ns, blockIdx.x);
+    __syncthreads();
+    const int j = s_j;
+    const SortRelDev &D = S[j];
+    const uint64_t base = (blockIdx.x - tbase[j]) * (uint64_t)MB_TILE;
+    for (int it = 0; it < MB_ITEMS; ++it) {
+        const uint64_t i = base + (uint64_t)it * MB_BLOCK + threadIdx.x;
+        if (i < D.n) {
+            const uint64_t r = D.base + i;
+            const uint64_t key = ((uint64_t)j << (cbits + rbits)) |
+                                 ((uint64_t)D.in[0][i] << rbits) | r;
+            keys[r] = key;
+            radix::hist_add(&h, key, lo_bit, hi_bit, npass);
+        }
+    }
+    __syncthreads();
+    radix::hist_flush(&h, ghist, npass);
+}
+
+__global__ __launch_bounds__(256) void k_gather_sort_multi(const uint64_t *keys, uint64_t total,
+                                                           const SortRelDev *S, int rbits,
+                                                           int cbits) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t rmask = (rbits >= 64) ? ~0ull : ((1ull << rbits) - 1);
+    const uint64_t cmask = (1ull << cbits) - 1;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const uint64_t k = keys[i];
+        const int j = (int)(k >> (cbits + rbits));
+        const SortRelDev &D = S[j];
+        const uint64_t src = (k & rmask) - D.base;
+        const uint64_t dst = i - D.base;
+        D.out[0][dst] = (uint32_t)((k >> rbits) & cmask);
+        for (int c = 1; c < D.ncols; ++c) D.out[c][dst] = D.in[c][src];
+    }
+}
+
+// ---- unique -------------------------------------------------------------------
+__device__ __forceinline__ bool differs(const SortRelDev &D, uint64_t i) {
+    if (i == 0) return true;
+    for (int c = 0; c < D.ncols; ++c)
+        if (D.in[c][i] != D.in[c][i - 1]) return true;
+    return false;
+}
+
+__global__ __launch_bounds__(MB_BLOCK) void k_unique_count_multi(const SortRelDev *S, int ns,
+                                                                 const uint64_t *tbase,
+                                                                 uint32_t *cnt) {
+    __shared__ int s_j;
+    __shared__ uint32_t s_tmp[MB_BLOCK / 64];
+    if (threadIdx.x == 0) s_j = find_seg(tbase, ns, blockIdx.x);
+    __syncthreads();
+    const SortRelDev &D = S[s_j];
+    const uint64_t base = (blockIdx.x - tbase[s_j]) * (uint64_t)MB_TILE;
+    uint32_t c = 0;
+    for (int it = 0; it < MB_ITEMS; ++it) {
+        const uint64_t i = base + (uint64_t)it * MB_BLOCK + threadIdx.x;
+        if (i < D.n) c += differs(D, i);
+    }
+    const uint32_t t = dev::block_reduce<MB_BLOCK>(c, s_tmp);
+    if (threadIdx.x == 0) cnt[blockIdx.x] = t;
+}
+
+__global__ void k_unique_totals(const uint64_t *tbase, int ns, const uint64_t *off,
+                                uint64_t *kept) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ns) return;
+    kept[j] = off[tbase[j + 1]] - off[tbase[j]];
+}
+
+__global__ __launch_bounds__(MB_BLOCK) void k_unique_scatter_multi(const SortRelDev *S, int ns,
+                                                                   const uint64_t *tbase,
+                                                                   const uint64_t *off) {
+    __shared__ int s_j;
+    __shared__ uint32_t s_tmp[MB_BLOCK / 64];
+    if (threadIdx.x == 0) s_j = find_seg(tbase, ns, blockIdx.x);
+    __syncthreads();
+    const int j = s_j;
+    const SortRelDev &D = S[j];
+    const uint64_t base = (blockIdx.x - tbase[j]) * (uint64_t)MB_TILE;
+    bool keep[MB_ITEMS];
+    uint32_t c = 0;
+#pragma unroll
+    for (int it = 0; it < MB_ITEMS; ++it) {
+        const uint64_t i = base + (uint64_t)threadIdx.x * MB_ITEMS + it;
+        keep[it] = i < D.n && differs(D, i);
+        c += keep[it];
+    }
+    uint32_t tot;
+    uint64_t o = off[blockIdx.x] - off[tbase[j]] +
+                 dev::block_excl_scan<MB_BLOCK>(c, s_tmp, &tot);
+#pragma unroll
+    for (int it = 0; it < MB_ITEMS; ++it) {
+        if (!keep[it]) continue;
+        const uint64_t i = base + (uint64_t)threadIdx.x * MB_ITEMS + it;
+        for (int cc = 0; cc < D.ncols; ++cc) D.out[cc][o] = D.in[cc][i];
+        ++o;
+    }
+}
+
+// ---- host drivers ---------------------------------------------------------------
+template <typename T>
+DBuf<T> to_device(vd_ctx *ctx, const std::vector<T> &h) {
+    DBuf<T> d(ctx, h.size());
+    upload(ctx, d.p, h.data(), h.size() * sizeof(T));
+    return d;
+}
+
+// Tile base prefix (n + 1 entries) for row counts.
+std::vector<uint64_t> tile_prefix(const std::vector<uint64_t> &rows) {
+    std::vector<uint64_t> t(rows.size() + 1, 0);
+    for (size_t j = 0; j < rows.size(); ++j) t[j + 1] = t[j] + ceil_div(rows[j], MB_TILE);
+    return t;
+}
+
+void select_multi(vd_ctx *ctx, const vd_index *idx, const vd_select_job_t *jobs, int nj,
+                  vd_rel **outs) {
+    std::vector<SelJobDev> hj(nj);
+    for (int j = 0; j < nj; ++j) {
+        const vd_perm_arrays &pa = get_perm(idx, jobs[j].perm);
+        VD_REQUIRE(jobs[j].cols == 1 || jobs[j].cols == 3, VD_EINVAL,
+                   "batched select supports cols 1 and 3");
+        hj[j] = SelJobDev{perm_dev(pa), pa.y, pa.z, jobs[j].k1, jobs[j].k2, jobs[j].cols,
+                          nullptr, nullptr};
+    }
+    DBuf<SelJobDev> dj = to_device(ctx, hj);
+    DBuf<uint64_t> begin(ctx, nj), sizes(ctx, nj), prefix(ctx, nj + 1);
+    launch(ctx, "select_lookup", 24.0 * nj, dim3((unsigned)ceil_div(nj, 256)), dim3(256), 0,
+           k_sel_lookup_multi, (const SelJobDev *)dj.p, nj, begin.p, sizes.p);
+    scan_u64(ctx, sizes.p, prefix.p, nj, prefix.p + nj);
+    std::vector<uint64_t> hp(nj + 1);
+    download(ctx, hp.data(), prefix.p, (nj + 1) * sizeof(uint64_t));
+    for (int j = 0; j < nj; ++j) outs[j] = nullptr;
+    try {
+        for (int j = 0; j < nj; ++j) {
+            outs[j] = rel_new(ctx, jobs[j].cols == 3 ? 2 : 1, hp[j + 1] - hp[j]);
+            hj[j].o0 = outs[j]->cols[0];
+            hj[j].o1 = jobs[j].cols == 3 ? outs[j]->cols[1] : nullptr;
+        }
+        upload(ctx, dj.p, hj.data(), nj * sizeof(SelJobDev));
+        const uint64_t total = hp[nj];
+        double bytes = 0;
+        for (int j = 0; j < nj; ++j) bytes += 8.0 * (hp[j + 1] - hp[j]) * (jobs[j].cols == 3 ? 2 : 1);
+        if (total)
+            launch(ctx, "select_gather", bytes, dim3(grid_cap(total)), dim3(256), 0,
+                   k_sel_gather_multi, (const SelJobDev *)dj.p, nj, (const uint64_t *)begin.p,
+                   (const uint64_t *)prefix.p, total);
+    } catch (...) {
+        for (int j = 0; j < nj; ++j) rel_free(ctx, outs[j]), outs[j] = nullptr;
+        throw;
+    }
+}
+
+void join_multi(vd_ctx *ctx, const vd_rel *const *L, const vd_rel *const *R, int np,
+                vd_rel **outs) {
+    std::vector<JPairDev> hp(np);
+    std::vector<uint64_t> lrows(np), lbase(np + 1, 0);
+    for (int j = 0; j < np; ++j) {
+        VD_REQUIRE(L[j] && R[j] && L[j]->ncols >= 1 && R[j]->ncols >= 1, VD_EINVAL,
+                   "bad join input");
+        VD_REQUIRE(L[j]->ncols + R[j]->ncols - 1 <= VD_MAX_COLS, VD_EINVAL,
+                   "join output has too many columns");
+        JPairDev d{};
+        for (int c = 0; c < L[j]->ncols; ++c) d.lc[c] = L[j]->cols[c];
+        for (int c = 0; c < R[j]->ncols; ++c) d.rc[c] = R[j]->cols[c];
+        d.ncl = L[j]->ncols;
+        d.ncr = R[j]->ncols;
+        // an empty right side contributes no left rows (all counts 0)
+        d.nl = R[j]->nrows ? L[j]->nrows : 0;
+        d.nr = R[j]->nrows;
+        d.lbase = lbase[j];
+        lrows[j] = d.nl;
+        lbase[j + 1] = lbase[j] + d.nl;
+        hp[j] = d;
+    }
+    const uint64_t nl = lbase[np];
+    VD_REQUIRE(nl < (1ull << 32), VD_ERANGE, "batched join inputs too large");
+    for (int j = 0; j < np; ++j) outs[j] = nullptr;
+    std::vector<uint64_t> tot(np, 0);
+    DBuf<JPairDev> dp = to_device(ctx, hp);
+    DBuf<uint64_t> dlbase = to_device(ctx, lbase);
+    DBuf<uint32_t> lower(ctx, nl), cnt(ctx, nl);
+    DBuf<uint64_t> off(ctx, nl + 1), dtot(ctx, np);
+    if (nl > 0) {
+        std::vector<uint64_t> tb = tile_prefix(lrows);
+        DBuf<uint64_t> dtb = to_device(ctx, tb);
+        double bbytes = 0;
+        for (int j = 0; j < np; ++j) bbytes += 4.0 * (hp[j].nl + hp[j].nr) + 8.0 * hp[j].nl;
+        launch(ctx, "join_bounds", bbytes, dim3((unsigned)tb[np]), dim3(MB_BLOCK), 0,
+               k_join_bounds_multi, (const JPairDev *)dp.p, np, (const uint64_t *)dtb.p,
+               lower.p, cnt.p);
+        scan_u32_to_u64(ctx, cnt.p, off.p, nl, off.p + nl);
+        launch(ctx, "join_totals", 0.0, dim3((unsigned)ceil_div(np, 256)), dim3(256), 0,
+               k_pair_totals, (const JPairDev *)dp.p, np, (const uint64_t *)off.p, dtot.p);
+        download(ctx, tot.data(), dtot.p, np * sizeof(uint64_t));
+    }
+    uint64_t total = 0;
+    try {
+        for (int j = 0; j < np; ++j) {
+            outs[j] = rel_new(ctx, L[j]->ncols + R[j]->ncols - 1, tot[j]);
+            for (int c = 0; c < outs[j]->ncols; ++c) hp[j].oc[c] = outs[j]->cols[c];
+            hp[j].obase = total;  // outputs of the pairs are laid end to end
+            total += tot[j];
+        }
+        VD_REQUIRE(total < (1ull << 40), VD_ERANGE, "join output too large");
+        if (total > 0) {
+            upload(ctx, dp.p, hp.data(), np * sizeof(JPairDev));
+            const uint64_t nblocks = ceil_div(nl + total, (uint64_t)MJ_NV);
+            DBuf<uint64_t> mp(ctx, nblocks + 1);
+            launch(ctx, "join_partition", 0.0, dim3((unsigned)ceil_div(nblocks + 1, 256)),
+                   dim3(256), 0, k_join_partition_multi, (const uint64_t *)off.p, nl, total,
+                   nblocks, mp.p);
+            double xbytes = 0;
+            for (int j = 0; j < np; ++j)
+                xbytes += 4.0 * tot[j] * outs[j]->ncols + 4.0 * hp[j].nl * hp[j].ncl +
+                          4.0 * hp[j].nr * (hp[j].ncr - 1);
+            launch(ctx, "join_expand", xbytes, dim3((unsigned)nblocks), dim3(MJ_BLOCK), 0,
+                   k_join_expand_multi, (const uint64_t *)off.p, (const uint32_t *)lower.p,
+                   (const uint64_t *)mp.p, nl, total, (const JPairDev *)dp.p, np,
+                   (const uint64_t *)dlbase.p);
+        }
+    } catch (...) {
+        for (int j = 0; j < np; ++j) rel_free(ctx, outs[j]), outs[j] = nullptr;
+        throw;
+    }
+}
+
+void sort_col0_range(vd_ctx *ctx, vd_rel *const *rels, int n);
+
+void sort_col0_multi(vd_ctx *ctx, vd_rel *const *rels, int n) {
+    std::vector<vd_rel *> work;
+    for (int j = 0; j < n; ++j)
+        if (rels[j] && rels[j]->nrows > 1) work.push_back(rels[j]);
+    if (!work.empty()) sort_col0_range(ctx, work.data(), (int)work.size());
+}
+
+void sort_col0_range(vd_ctx *ctx, vd_rel *const *rels, int n) {
+    std::vector<SortRelDev> hs(n);
+    std::vector<uint64_t> rows(n);
+    uint64_t total = 0;
+    for (int j = 0; j < n; ++j) {
+        SortRelDev d{};
+        d.n = rels[j]->nrows;
+        d.base = total;
+        d.ncols = rels[j]->ncols;
+        for (int c = 0; c < d.ncols; ++c) d.in[c] = rels[j]->cols[c];
+        hs[j] = d;
+        rows[j] = d.n;
+        total += d.n;
+    }
+    VD_REQUIRE(total < (1ull << 30), VD_ERANGE, "batched sort supports < 2^30 rows");
+    const std::vector<uint64_t> tb = tile_prefix(rows);
+    DBuf<SortRelDev> ds = to_device(ctx, hs);
+    DBuf<uint64_t> dtb = to_device(ctx, tb);
+    DBuf<uint64_t> dmax(ctx, 1);
+    fill_u64(ctx, dmax.p, 0, 1);
+    launch(ctx, "sort_max", 4.0 * total, dim3((unsigned)tb[n]), dim3(MB_BLOCK), 0,
+           k_max_col0_multi, (const SortRelDev *)ds.p, n, (const uint64_t *)dtb.p,
+           (uint32_t *)dmax.p);
+    const uint32_t mx = (uint32_t)read_u64(ctx, dmax.p);
+    const int rbits = std::max(1, bits_for(total - 1));
+    const int cbits = std::max(1, bits_for(mx));
+    const int sbits = bits_for((uint64_t)n - 1);
+    if (rbits + cbits + sbits > 64) {
+        VD_REQUIRE(n > 1, VD_ERANGE, "sort key does not fit 64 bits");
+        sort_col0_range(ctx, rels, n / 2);
+        sort_col0_range(ctx, rels + n / 2, n - n / 2);
+        return;
+    }
+    const int lo = rbits, hi = rbits + cbits + sbits;
+    const int npass = radix_passes(lo, hi);
+    DBuf<uint64_t> keys(ctx, total), tmp(ctx, total);
+    DBuf<uint32_t> hist(ctx, radix_hist_words(lo, hi));
+    fill_u32(ctx, hist.p, 0, radix_hist_words(lo, hi));
+    launch(ctx, "sort_pack", 12.0 * total, dim3((unsigned)tb[n]), dim3(MB_BLOCK), 0,
+           k_pack_multi, (const SortRelDev *)ds.p, n, (const uint64_t *)dtb.p, rbits, cbits,
+           keys.p, lo, hi, npass, hist.p);
+    uint64_t *sorted = radix_sort_u64(ctx, keys.p, tmp.p, total, lo, hi, hist.p, true);
+    std::vector<uint32_t *> fresh;
+    try {
+        for (int j = 0; j < n; ++j)
+            for (int c = 0; c < hs[j].ncols; ++c) {
+                fresh.push_back((uint32_t *)dalloc(ctx, hs[j].n * 4));
+                hs[j].out[c] = fresh.back();
+            }
+    } catch (...) {
+        for (auto p : fresh) dfree(ctx, p);
+        throw;
+    }
+    upload(ctx, ds.p, hs.data(), n * sizeof(SortRelDev));
+    double gbytes = 0;
+    for (int j = 0; j < n; ++j) gbytes += (8.0 + 4.0 + 8.0 * (hs[j].ncols - 1)) * hs[j].n;
+    launch(ctx, "sort_gather", gbytes, dim3(grid_cap(total)), dim3(256), 0, k_gather_sort_multi,
+           (const uint64_t *)sorted, total, (const SortRelDev *)ds.p, rbits, cbits);
+    for (int j = 0; j < n; ++j)
+        for (int c = 0; c < hs[j].ncols; ++c) {
+            dfree(ctx, rels[j]->cols[c]);
+            rels[j]->cols[c] = hs[j].out[c];
+        }
+}
+
+void unique_multi(vd_ctx *ctx, vd_rel *const *rels_in, int n_in) {
+    std::vector<vd_rel *> rels;
+    for (int j = 0; j < n_in; ++j)
+        if (rels_in[j] && rels_in[j]->nrows > 1) rels.push_back(rels_in[j]);
+    const int n = (int)rels.size();
+    if (n == 0) return;
+    std::vector<SortRelDev> hs(n);
+    std::vector<uint64_t> rows(n);
+    double bytes = 0;
+    for (int j = 0; j < n; ++j) {
+        SortRelDev d{};
+        d.n = rels[j]->nrows;
+        d.ncols = rels[j]->ncols;
+        for (int c = 0; c < d.ncols; ++c) d.in[c] = rels[j]->cols[c];
+        hs[j] = d;
+        rows[j] = d.n;
+        bytes += 4.0 * d.n * d.ncols;
+    }
+    const std::vector<uint64_t> tb = tile_prefix(rows);
+    const uint64_t tiles = tb[n];
+    DBuf<SortRelDev> ds = to_device(ctx, hs);
+    DBuf<uint64_t> dtb = to_device(ctx, tb);
+    DBuf<uint32_t> cnt(ctx, tiles);
+    DBuf<uint64_t> off(ctx, tiles + 1), kept(ctx, n);
+    launch(ctx, "unique_count", bytes, dim3((unsigned)tiles), dim3(MB_BLOCK), 0,
+           k_unique_count_multi, (const SortRelDev *)ds.p, n, (const uint64_t *)dtb.p, cnt.p);
+    scan_u32_to_u64(ctx, cnt.p, off.p, tiles, off.p + tiles);
+    launch(ctx, "unique_totals", 0.0, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0,
+           k_unique_totals, (const uint64_t *)dtb.p, n, (const uint64_t *)off.p, kept.p);
+    std::vector<uint64_t> hk(n);
+    download(ctx, hk.data(), kept.p, n * sizeof(uint64_t));
+    std::vector<uint32_t *> fresh;
+    try {
+        for (int j = 0; j < n; ++j)
+            for (int c = 0; c < hs[j].ncols; ++c) {
+                fresh.push_back((uint32_t *)dalloc(ctx, std::max<uint64_t>(hk[j], 1) * 4));
+                hs[j].out[c] = fresh.back();
+            }
+    } catch (...) {
+        for (auto p : fresh) dfree(ctx, p);
+        throw;
+    }
+    upload(ctx, ds.p, hs.data(), n * sizeof(SortRelDev));
+    launch(ctx, "unique_scatter", 2.0 * bytes, dim3((unsigned)tiles), dim3(MB_BLOCK), 0,
+           k_unique_scatter_multi, (const SortRelDev *)ds.p, n, (const uint64_t *)dtb.p,
+           (const uint64_t *)off.p);
+    for (int j = 0; j < n; ++j) {
+        for (int c = 0; c < hs[j].ncols; ++c) {
+            dfree(ctx, rels[j]->cols[c]);
+            rels[j]->cols[c] = hs[j].out[c];
+        }
+        rels[j]->nrows = hk[j];
+    }
+}
+
+}  // namespace
+}  // namespace vd
+
+using namespace vd;
+
+extern "C" {
+
+int vd_select_multi(vd_ctx *ctx, const vd_index *idx, const vd_select_job_t *jobs, int njobs,
+                    vd_rel **outs) {
+    if (!ctx || njobs < 0 || (njobs && (!jobs || !outs))) return VD_EINVAL;
+    if (njobs == 0) return VD_OK;
+    VD_TRY(ctx, select_multi(ctx, idx, jobs, njobs, outs));
+}
+
+int vd_join_multi(vd_ctx *ctx, const vd_rel *const *L, const vd_rel *const *R, int npairs,
+                  vd_rel **outs) {
+    if (!ctx || npairs < 0 || (npairs && (!L || !R || !outs))) return VD_EINVAL;
+    if (npairs == 0) return VD_OK;
+    VD_TRY(ctx, join_multi(ctx, L, R, npairs, outs));
+}
+
+int vd_sort_col0_multi(vd_ctx *ctx, vd_rel *const *rels, int n) {
+    if (!ctx || n < 0 || (n && !rels)) return VD_EINVAL;
+    VD_TRY(ctx, sort_col0_multi(ctx, rels, n));
+}
+
+int vd_unique_adjacent_multi(vd_ctx *ctx, vd_rel *const *rels, int n) {
+    if (!ctx || n < 0 || (n && !rels)) return VD_EINVAL;
+    VD_TRY(ctx, unique_multi(ctx, rels, n));
+}
+
+}  // extern "C"

@@ -124,6 +124,33 @@ void read_words(vd_ctx *ctx, uint64_t *dst, const uint64_t *dptr, int n) {
     std::memcpy(dst, ctx->h_pinned, n * sizeof(uint64_t));
 }
 
+void upload(vd_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return;
+    if (ctx->stage_pending) {
+        VD_HIP(hipEventSynchronize(ctx->stage_done));
+        ctx->stage_pending = false;
+    }
+    if (bytes > ctx->h_stage_bytes) {
+        if (ctx->h_stage) VD_HIP(hipHostFree(ctx->h_stage));
+        ctx->h_stage = nullptr;
+        size_t cap = 1 << 16;
+        while (cap < bytes) cap <<= 1;
+        VD_HIP(hipHostMalloc(&ctx->h_stage, cap, hipHostMallocDefault));
+        ctx->h_stage_bytes = cap;
+    }
+    if (!ctx->stage_done) VD_HIP(hipEventCreateWithFlags(&ctx->stage_done, hipEventDisableTiming));
+    std::memcpy(ctx->h_stage, src, bytes);
+    VD_HIP(hipMemcpyAsync(dst, ctx->h_stage, bytes, hipMemcpyHostToDevice, ctx->stream));
+    VD_HIP(hipEventRecord(ctx->stage_done, ctx->stream));
+    ctx->stage_pending = true;
+}
+
+void download(vd_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return;
+    VD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+}
+
 void copy_d2d(vd_ctx *ctx, void *dst, const void *src, size_t bytes) {
     if (bytes == 0) return;
     VD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
@@ -194,6 +221,8 @@ int vd_ctx_destroy(vd_ctx *ctx) {
         (void)hipEventDestroy(p.stop);
     }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    if (ctx->stage_done) (void)hipEventDestroy(ctx->stage_done);
+    if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
     for (auto &kv : ctx->free_blocks) (void)hipFree(kv.second);
     for (auto &kv : ctx->live_blocks) (void)hipFree(kv.first);
     if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);

@@ -20,19 +20,7 @@
 
 #include "vd_radix_dev.hpp"
 
-struct vd_perm_arrays {
-    bool built = false;
-    uint64_t n = 0, n_l1 = 0, n_l2 = 0;
-    uint32_t *l1v = nullptr, *l1o = nullptr, *l2v = nullptr, *l2o = nullptr;
-    uint32_t *y = nullptr, *z = nullptr;
-};
-
-struct vd_index {
-    vd_ctx *owner = nullptr;
-    uint64_t n = 0;
-    uint32_t mask = 0;
-    vd_perm_arrays perm[VD_NUM_PERMS];
-};
+#include "vd_index.hpp"
 
 namespace vd {
 namespace {
@@ -161,30 +149,6 @@ __global__ __launch_bounds__(UN_BLOCK) void k_compact_heads(
 }
 
 // ---- lookups -------------------------------------------------------------------
-struct PermDev {
-    uint64_t n, n_l1;
-    const uint32_t *l1v, *l1o, *y;
-};
-
-__device__ __forceinline__ void lookup_one(const PermDev &pd, uint32_t k1,
-                                           uint32_t k2, bool has_k2,
-                                           uint64_t *b, uint64_t *e) {
-    const uint64_t i = dev::lower_bound(pd.l1v, 0, pd.n_l1, k1);
-    if (i >= pd.n_l1 || pd.l1v[i] != k1) {
-        const uint64_t pos = i < pd.n_l1 ? pd.l1o[i] : pd.n;
-        *b = *e = pos;
-        return;
-    }
-    uint64_t lo = pd.l1o[i];
-    uint64_t hi = (i + 1 < pd.n_l1) ? pd.l1o[i + 1] : pd.n;
-    if (has_k2) {
-        lo = dev::lower_bound(pd.y, lo, hi, k2);
-        hi = dev::upper_bound(pd.y, lo, hi, k2);
-    }
-    *b = lo;
-    *e = hi;
-}
-
 __global__ __launch_bounds__(256) void k_lookup_batch(
     PermDev pd, const uint32_t *__restrict__ k1, const uint32_t *__restrict__ k2,
     uint64_t nq, uint64_t *__restrict__ begin, uint64_t *__restrict__ end,
@@ -297,10 +261,6 @@ unsigned stream_grid(uint64_t n) {
     return g ? (unsigned)g : 1u;
 }
 
-PermDev perm_dev(const vd_perm_arrays &pa) {
-    return PermDev{pa.n, pa.n_l1, pa.l1v, pa.l1o, pa.y};
-}
-
 void build_perm(vd_ctx *ctx, vd_index *idx, int perm, const uint32_t *cols[3],
                 const int bits[3], uint64_t *keys, uint64_t *tmp) {
     const uint64_t n = idx->n;
@@ -363,6 +323,12 @@ void free_perm(vd_ctx *ctx, vd_perm_arrays &pa) {
     pa = vd_perm_arrays();
 }
 
+}  // namespace
+
+PermDev perm_dev(const vd_perm_arrays &pa) {
+    return PermDev{pa.n, pa.n_l1, pa.l1v, pa.l1o, pa.y};
+}
+
 const vd_perm_arrays &get_perm(const vd_index *idx, int perm) {
     VD_REQUIRE(idx && perm >= 0 && perm < VD_NUM_PERMS, VD_EINVAL,
                "bad index or permutation");
@@ -370,7 +336,6 @@ const vd_perm_arrays &get_perm(const vd_index *idx, int perm) {
     return idx->perm[perm];
 }
 
-}  // namespace
 }  // namespace vd
 
 using namespace vd;

@@ -1,0 +1,49 @@
+// Index structures shared by the index, select and batched-operator units.
+#pragma once
+
+#include "vd_internal.hpp"
+
+struct vd_perm_arrays {
+    bool built = false;
+    uint64_t n = 0, n_l1 = 0, n_l2 = 0;
+    uint32_t *l1v = nullptr, *l1o = nullptr, *l2v = nullptr, *l2o = nullptr;
+    uint32_t *y = nullptr, *z = nullptr;
+};
+
+struct vd_index {
+    vd_ctx *owner = nullptr;
+    uint64_t n = 0;
+    uint32_t mask = 0;
+    vd_perm_arrays perm[VD_NUM_PERMS];
+};
+
+namespace vd {
+
+struct PermDev {
+    uint64_t n, n_l1;
+    const uint32_t *l1v, *l1o, *y;
+};
+
+__device__ __forceinline__ void lookup_one(const PermDev &pd, uint32_t k1,
+                                           uint32_t k2, bool has_k2,
+                                           uint64_t *b, uint64_t *e) {
+    const uint64_t i = dev::lower_bound(pd.l1v, 0, pd.n_l1, k1);
+    if (i >= pd.n_l1 || pd.l1v[i] != k1) {
+        const uint64_t pos = i < pd.n_l1 ? pd.l1o[i] : pd.n;
+        *b = *e = pos;
+        return;
+    }
+    uint64_t lo = pd.l1o[i];
+    uint64_t hi = (i + 1 < pd.n_l1) ? pd.l1o[i + 1] : pd.n;
+    if (has_k2) {
+        lo = dev::lower_bound(pd.y, lo, hi, k2);
+        hi = dev::upper_bound(pd.y, lo, hi, k2);
+    }
+    *b = lo;
+    *e = hi;
+}
+
+PermDev perm_dev(const vd_perm_arrays &pa);
+const vd_perm_arrays &get_perm(const vd_index *idx, int perm);
+
+}  // namespace vd

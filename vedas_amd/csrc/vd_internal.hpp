@@ -80,6 +80,12 @@ struct vd_ctx {
 
     // Pinned host staging for small read-backs.
     uint64_t *h_pinned = nullptr;  // 64 words
+    // Pinned staging for descriptor uploads; reused once the previous
+    // upload's event has completed.
+    void *h_stage = nullptr;
+    size_t h_stage_bytes = 0;
+    hipEvent_t stage_done = nullptr;
+    bool stage_pending = false;
 
     // Profiling.
     bool prof = false;
@@ -155,6 +161,11 @@ inline void launch(vd_ctx *ctx, const char *name, double bytes, dim3 grid,
 
 void sync(vd_ctx *ctx);
 uint64_t read_u64(vd_ctx *ctx, const uint64_t *dptr);          // syncs
+// Stream-ordered host -> device copy through pinned staging (the host
+// buffer may be reused as soon as this returns).
+void upload(vd_ctx *ctx, void *dst, const void *src, size_t bytes);
+// Device -> host copy of `bytes`, synchronising the stream.
+void download(vd_ctx *ctx, void *dst, const void *src, size_t bytes);
 void read_words(vd_ctx *ctx, uint64_t *dst, const uint64_t *dptr, int n);
 
 // ---- primitives ------------------------------------------------------------

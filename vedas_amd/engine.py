@@ -21,6 +21,12 @@ class Stats(C.Structure):
                 ("select_ms", C.c_double), ("swap_ms", C.c_double), ("jobs", C.c_int)]
 
 
+class BatchStats(C.Structure):
+    _fields_ = [("rounds", C.c_int), ("calls", C.c_int), ("join_rows", C.c_uint64),
+                ("select_rows", C.c_uint64), ("swap_rows", C.c_uint64),
+                ("elapsed_ms", C.c_double)]
+
+
 _lib = None
 
 
@@ -46,6 +52,8 @@ def lib() -> C.CDLL:
             "vdq_result_stats": (i, [vp, C.POINTER(Stats)]),
             "vdq_result_free": (i, [vp]),
             "vdq_last_error": (C.c_char_p, []),
+            "vdq_batch": (i, [vp, C.POINTER(C.c_char_p), i, C.POINTER(vp),
+                              C.POINTER(BatchStats)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(l, name)
@@ -125,6 +133,16 @@ class Engine:
         r = C.c_void_p()
         _check(lib().vdq_query(self.h, text.encode(), C.byref(r)))
         return Result(r)
+
+    def batch(self, texts):
+        """Run many queries together; returns ([Result], stats dict)."""
+        n = len(texts)
+        arr = (C.c_char_p * max(1, n))(*[t.encode() for t in texts])
+        outs = (C.c_void_p * max(1, n))()
+        st = BatchStats()
+        _check(lib().vdq_batch(self.h, arr, n, outs, C.byref(st)))
+        return ([Result(C.c_void_p(outs[i])) for i in range(n)],
+                {k: getattr(st, k) for k, _ in BatchStats._fields_})
 
     def close(self):
         if self.h:

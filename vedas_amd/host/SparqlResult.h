@@ -10,6 +10,7 @@ class SparqlResult {
   public:
     void setResult(FullRelationIR *ir) { result.reset(ir); }
     FullRelationIR *getResultIR() const { return result.get(); }
+    FullRelationIR *release() { return result.release(); }
   private:
     std::unique_ptr<FullRelationIR> result;
 };

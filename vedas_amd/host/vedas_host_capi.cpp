@@ -6,6 +6,7 @@
 #include <set>
 #include <sstream>
 
+#include "BatchExecutor.h"
 #include "QueryExecutor.h"
 #include "vedas_host.h"
 
@@ -106,6 +107,60 @@ PlanTreeNode *parse_node(Parser &p, PlanArena &a, std::vector<TriplePattern> &pa
     return n;
 }
 
+// One parsed query: patterns (stable storage), SELECT set, plan tree.
+struct ParsedQuery {
+    std::vector<TriplePattern> pats;
+    std::set<std::string> selected;
+    PlanArena arena;
+    PlanTreeNode *root = nullptr;
+};
+
+void parse_query(const char *text, ParsedQuery &pq) {
+    Parser p(text);
+    p.expect("SELECT");
+    bool star = false;
+    while (p.peek() != "WHERE") {
+        std::string v = p.next();
+        if (v == "*") star = true; else pq.selected.insert(v);
+    }
+    p.expect("WHERE");
+    p.expect("{");
+    pq.pats.reserve(64);
+    while (p.peek() != "}") {
+        std::string s = p.next(), pr = p.next(), o = p.next();
+        if (pq.pats.size() == 64) throw VedasError(VD_EINVAL, "at most 64 patterns");
+        pq.pats.emplace_back(pq.pats.size(), s, pr, o);
+        if (p.peek() == ".") p.next();
+    }
+    p.expect("}");
+    if (pq.pats.empty()) throw VedasError(VD_EINVAL, "no patterns");
+    if (star)
+        for (auto &tp : pq.pats)
+            for (auto t : {tp.getSubject(), tp.getPredicate(), tp.getObject()})
+                if (!t.empty() && t[0] == '?') pq.selected.insert(t);
+    if (p.more()) {
+        p.expect("PLAN");
+        pq.root = parse_node(p, pq.arena, pq.pats);
+        if (p.more()) throw VedasError(VD_EINVAL, "text after the plan");
+    } else if (pq.pats.size() != 1) {
+        throw VedasError(VD_EINVAL, "PLAN required for more than one pattern");
+    }
+}
+
+void fill_stats(vdq_result *res, VedasContext *ctx) {
+    FullRelationIR *ir = res->result.getResultIR();
+    for (size_t i = 0; ir && i < ir->getColumnNum(); ++i) res->headers.push_back(ir->getHeader(i));
+    res->stats.rows = ir ? ir->size() : 0;
+    if (!ctx) return;
+    res->stats.join_rows = ctx->join_rows;
+    res->stats.select_rows = ctx->select_rows;
+    res->stats.swap_rows = ctx->swap_rows;
+    res->stats.join_ms = ctx->join_ns / 1e6;
+    res->stats.select_ms = ctx->upload_ns / 1e6;
+    res->stats.swap_ms = ctx->swap_index_ns / 1e6;
+    res->stats.jobs = (int)ctx->exe_log.size();
+}
+
 }  // namespace
 
 extern "C" {
@@ -171,57 +226,62 @@ int vdq_query(vdq_engine *e, const char *text, vdq_result **out) {
     *out = nullptr;
     return guarded([&] {
         auto t0 = std::chrono::steady_clock::now();
-        Parser p(text);
-        p.expect("SELECT");
-        std::set<std::string> selected;
-        bool star = false;
-        while (p.peek() != "WHERE") {
-            std::string v = p.next();
-            if (v == "*") star = true; else selected.insert(v);
-        }
-        p.expect("WHERE");
-        p.expect("{");
-        std::vector<TriplePattern> pats;
-        pats.reserve(64);
-        while (p.peek() != "}") {
-            std::string s = p.next(), pr = p.next(), o = p.next();
-            pats.emplace_back(pats.size(), s, pr, o);
-            if (p.peek() == ".") p.next();
-        }
-        p.expect("}");
-        if (pats.empty()) throw VedasError(VD_EINVAL, "no patterns");
-        if (star)
-            for (auto &tp : pats)
-                for (auto t : {tp.getSubject(), tp.getPredicate(), tp.getObject()})
-                    if (!t.empty() && t[0] == '?') selected.insert(t);
-        PlanArena arena;
-        PlanTreeNode *root = nullptr;
-        if (p.more()) {
-            p.expect("PLAN");
-            root = parse_node(p, arena, pats);
-        } else if (pats.size() != 1) {
-            throw VedasError(VD_EINVAL, "PLAN required for more than one pattern");
-        }
+        ParsedQuery pq;
+        parse_query(text, pq);
         auto res = std::make_unique<vdq_result>();
         VedasContext *ctx = e->ctx.get();
         ctx->resetStats();
         QueryExecutor ex(e->storage.get(), ctx);
-        ex.query(pats, selected, root, res->result);
+        ex.query(pq.pats, pq.selected, pq.root, res->result);
         ctx->sync();
-        FullRelationIR *ir = res->result.getResultIR();
-        for (size_t i = 0; ir && i < ir->getColumnNum(); ++i) res->headers.push_back(ir->getHeader(i));
-        auto &st = res->stats;
-        st.rows = ir ? ir->size() : 0;
-        st.join_rows = ctx->join_rows;
-        st.select_rows = ctx->select_rows;
-        st.swap_rows = ctx->swap_rows;
-        st.join_ms = ctx->join_ns / 1e6;
-        st.select_ms = ctx->upload_ns / 1e6;
-        st.swap_ms = ctx->swap_index_ns / 1e6;
-        st.jobs = (int)ctx->exe_log.size();
-        st.elapsed_ms = std::chrono::duration<double, std::milli>(
+        fill_stats(res.get(), ctx);
+        res->stats.elapsed_ms = std::chrono::duration<double, std::milli>(
             std::chrono::steady_clock::now() - t0).count();
         *out = res.release();
+    });
+}
+
+int vdq_batch(vdq_engine *e, const char *const *texts, int n, vdq_result **outs,
+              vdq_batch_stats_t *stats) {
+    if (!e || n < 0 || (n && (!texts || !outs))) return fail(VD_EINVAL, "null argument");
+    if (!e->storage) return fail(VD_EINVAL, "engine has no store loaded");
+    for (int i = 0; i < n; ++i) outs[i] = nullptr;
+    return guarded([&] {
+        auto t0 = std::chrono::steady_clock::now();
+        std::vector<std::unique_ptr<ParsedQuery>> pqs;
+        std::vector<BatchQuery> bq(n);
+        for (int i = 0; i < n; ++i) {
+            pqs.emplace_back(new ParsedQuery());
+            parse_query(texts[i], *pqs.back());
+            bq[i].patterns = std::move(pqs.back()->pats);
+            bq[i].selected = pqs.back()->selected;
+            bq[i].root = pqs.back()->root;
+        }
+        // plan nodes point at the patterns: re-point them at the moved copies
+        for (int i = 0; i < n; ++i)
+            for (auto &node : pqs[i]->arena.nodes)
+                if (node->tp) node->tp = &bq[i].patterns[node->tp->getId()];
+        std::vector<SparqlResult> results;
+        BatchStats st;
+        BatchExecutor bx(e->storage.get(), e->ctx.get());
+        bx.run(bq, results, &st);
+        e->ctx->sync();
+        std::vector<std::unique_ptr<vdq_result>> rs;
+        for (int i = 0; i < n; ++i) {
+            rs.emplace_back(new vdq_result());
+            rs.back()->result.setResult(results[i].release());
+            fill_stats(rs.back().get(), nullptr);
+        }
+        for (int i = 0; i < n; ++i) outs[i] = rs[i].release();
+        if (stats) {
+            stats->rounds = st.rounds;
+            stats->calls = st.launches_calls;
+            stats->join_rows = st.join_rows;
+            stats->select_rows = st.select_rows;
+            stats->swap_rows = st.swap_rows;
+            stats->elapsed_ms = std::chrono::duration<double, std::milli>(
+                std::chrono::steady_clock::now() - t0).count();
+        }
     });
 }
 
