@@ -1,21 +1,23 @@
 #!/usr/bin/env python3
 """Benchmark of the VEDAS query path on one MI355X.
 
-Workload (synthetic, seeded): the 100M-triple store -- 10M entities with
-Zipf(0.8) subjects, 256 predicates Zipf(1.1), objects 60% entities / 40%
-literals -- indexed into SPO, PSO, POS and OSP in HBM, then the single-query
-join set run through the drop-in operator API (QueryExecutor ->
-Select/Join/IndexSwap jobs -> QueryPlan):
+Store (synthetic, seeded): 100M triples -- 10M entities with Zipf(0.8)
+subjects, 256 predicates Zipf(1.1), objects 60% entities / 40% literals --
+indexed into SPO, PSO, POS and OSP, all resident in HBM.
+
+Headline (`value`): join output rows per second of the single-query join
+set run through the drop-in operator API (QueryExecutor -> Select / Join /
+IndexSwap jobs -> QueryPlan), one step = the three queries once:
 
   star4      ?s 30 ?a . ?s 31 ?b . ?x 1 ?s . ?y 2 ?s
   chain4     ?a 1 ?b . ?b 1 ?c . ?c 1 ?d . ?d 1 ?e      (index swaps between joins)
   snowflake6 ?x 50 ?y . ?x 51 ?u . ?w 1 ?x . ?y 30 ?v . ?y 31 ?t . ?z 1 ?y
 
-A step runs the three queries once (triples and permutations already
-resident in HBM).  `value` = join output rows per second over the timed
-steps.  The index build (4 permutations) is timed once beside it.
+Also reported: the index build (4 permutations), 10k range selects, a
+256-query mixed batch run one query at a time vs batched, and an open-loop
+Poisson arrival stream served plain vs with continuous batching (p50/p99).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--triples N]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 from __future__ import annotations
 
@@ -30,29 +32,10 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from vedas_amd.workload import JOIN_SET, STORE_100M, query_mix  # noqa: E402
+
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
-
-QUERIES = {
-    "star4": ([("?s", 30, "?a"), ("?s", 31, "?b"), ("?x", 1, "?s"), ("?y", 2, "?s")],
-              ("JOIN", ("JOIN", ("JOIN", ("UPLOAD", 0, "?s"), ("UPLOAD", 1, "?s"), "?s"),
-                        ("UPLOAD", 2, "?s"), "?s"), ("UPLOAD", 3, "?s"), "?s")),
-    "chain4": ([("?a", 1, "?b"), ("?b", 1, "?c"), ("?c", 1, "?d"), ("?d", 1, "?e")],
-               ("JOIN", ("INDEXSWAP", ("JOIN", ("INDEXSWAP", ("JOIN", ("UPLOAD", 0, "?b"),
-                                                              ("UPLOAD", 1, "?b"), "?b"), "?c"),
-                                                 ("UPLOAD", 2, "?c"), "?c"), "?d"),
-                ("UPLOAD", 3, "?d"), "?d")),
-    "snowflake6": ([("?x", 50, "?y"), ("?x", 51, "?u"), ("?w", 1, "?x"),
-                    ("?y", 30, "?v"), ("?y", 31, "?t"), ("?z", 1, "?y")],
-                   ("JOIN",
-                    ("INDEXSWAP", ("JOIN", ("JOIN", ("UPLOAD", 0, "?x"), ("UPLOAD", 1, "?x"), "?x"),
-                                   ("UPLOAD", 2, "?x"), "?x"), "?y"),
-                    ("JOIN", ("JOIN", ("UPLOAD", 3, "?y"), ("UPLOAD", 4, "?y"), "?y"),
-                     ("UPLOAD", 5, "?y"), "?y"),
-                    "?y")),
-}
-
-GEN = dict(n_entities=10_000_000, subject_zipf=0.8, n_predicates=256, pred_zipf=1.1,
-           literal_frac=0.4, literal_base=50_000_000, n_literals=50_000_000, distinct=False)
+GEN = {k: v for k, v in STORE_100M.items() if k != "n"}
 
 
 def log(*a):
@@ -67,12 +50,18 @@ def index_bytes(n: int, kbits: int) -> float:
     return passes * 2 * 8 * n + 20 * n + 16 * n
 
 
+def kernel_table(prof):
+    return {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
+                "GB/s": round(v["bytes"] / (v["ms"] / 1e3) / 1e9, 1) if v["ms"] > 0 else None}
+            for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])}
+
+
 def select_bench(ctx, idx, s, p, o, nq=10_000, seed=7, max_chunk_rows=400_000_000):
     """10k range selects with one or two constants taken from uniformly drawn
     existing triples: S -> SPO, P -> PSO, O -> OSP, (S,P) -> SPO, (P,O) -> POS,
     (O,S) -> OSP.  Each permutation's queries run as batched selects whose
     output (Y, Z) rows are materialised in HBM, in chunks of at most
-    max_chunk_rows rows.  Returns selects/s, rows/s and GB/s."""
+    max_chunk_rows rows."""
     import ctypes as C
     from vedas_amd import _native
     rng = np.random.default_rng(seed)
@@ -95,7 +84,6 @@ def select_bench(ctx, idx, s, p, o, nq=10_000, seed=7, max_chunk_rows=400_000_00
             continue
         k1 = np.array(k1, np.uint32)
         k2 = np.array(k2, np.uint32)
-        # sizes on the host to cut the batch into chunks that fit
         d1, d2 = ctx.upload(k1), ctx.upload(k2)
         db, de = ctx.alloc(8 * len(k1)), ctx.alloc(8 * len(k1))
         ctx.check(ctx.L.vd_index_lookup_batch(ctx.h, idx.h, _native.PERMS[perm], C.c_void_p(d1),
@@ -134,17 +122,91 @@ def select_bench(ctx, idx, s, p, o, nq=10_000, seed=7, max_chunk_rows=400_000_00
     ctx.free(dy)
     ctx.free(dz)
     return {"selects": nq, "ms": dt * 1e3, "selects_per_s": nq / dt, "rows": rows,
-            "rows_per_s": rows / dt, "GB/s": 16.0 * rows / dt / 1e9,
-            "chunks": len(staged)}
+            "rows_per_s": rows / dt, "GB/s": 16.0 * rows / dt / 1e9, "chunks": len(staged)}
 
 
-def cpu_baseline(engine_ctx, sample_seconds: float):
+def chunk_by_memory(eng, texts, budget):
+    """Consecutive groups of queries whose estimated peak HBM bytes (store
+    statistics) sum to at most `budget`."""
+    groups, cur, acc = [], [], 0.0
+    for t in texts:
+        need = eng.estimate(t)[1]
+        if cur and acc + need > budget:
+            groups.append(cur)
+            cur, acc = [], 0.0
+        cur.append(t)
+        acc += need
+    if cur:
+        groups.append(cur)
+    return groups
+
+
+def batch_bench(eng, texts, reps=2, budget=96 << 30):
+    """The same query mix one query at a time (plain drop-in path) and in
+    batches (as many queries per batch as the memory estimate allows);
+    results compared canonically."""
+    from oracle.pyoracle import canonical  # checker only
+    groups = chunk_by_memory(eng, texts, budget)
+    plain_t = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        for t in texts:
+            eng.query(t).free()
+        plain_t.append(time.perf_counter() - t0)
+    batch_t = []
+    rounds = calls = join_rows = 0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        rounds = calls = join_rows = 0
+        for g in groups:
+            res, st = eng.batch(g)
+            rounds += st["rounds"]
+            calls += st["calls"]
+            join_rows += st["join_rows"]
+            for r in res:
+                r.free()
+        batch_t.append(time.perf_counter() - t0)
+    # equality of results, checked outside the timed runs
+    same = 0
+    for g in groups:
+        res, _ = eng.batch(g)
+        for t, r in zip(g, res):
+            a = eng.query(t)
+            same += canonical(a.headers(), a.columns()) == canonical(r.headers(), r.columns())
+            a.free()
+            r.free()
+    pt, bt = min(plain_t), min(batch_t)
+    return {"queries": len(texts), "batches": len(groups), "plain_ms": pt * 1e3,
+            "batched_ms": bt * 1e3, "plain_qps": len(texts) / pt, "batched_qps": len(texts) / bt,
+            "speedup": pt / bt, "rounds": rounds, "calls": calls, "join_rows": join_rows,
+            "results_equal": same}
+
+
+def serve_bench(eng, texts, qps, seed=1, slots=4, max_batch=64, budget=96 << 30):
+    """Open-loop Poisson arrivals at `qps` over the mix, served plain (one
+    query at a time) and with continuous batching; p50/p99 latency."""
+    rng = np.random.default_rng(seed)
+    arr = list(np.cumsum(rng.exponential(1000.0 / qps, len(texts))))
+
+    def summarise(recs):
+        lat = np.array([r["done_ms"] - r["arrival_ms"] for r in recs])
+        span = max(r["done_ms"] for r in recs) - min(r["arrival_ms"] for r in recs)
+        return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "qps": len(recs) / (span / 1e3), "batches": len({r["batch"] for r in recs})}
+
+    plain, _ = eng.serve(texts, arr, plain=True)
+    served, _ = eng.serve(texts, arr, slots=slots, max_batch=max_batch, mem_budget=budget)
+    out = {"offered_qps": qps, "requests": len(texts), "slots": slots, "max_batch": max_batch,
+           "mem_budget": budget, "plain": summarise(plain), "served": summarise(served)}
+    out["qps_speedup"] = out["served"]["qps"] / out["plain"]["qps"]
+    return out
+
+
+def cpu_baseline(idx, sample_seconds: float):
     """The oracle's join (C, one thread) on the first left rows of the
-    star4 query's first join (?s 30 ?a ⋈ ?s 31 ?b on the same store), so the
-    CPU does the same kind of work; rows/s extrapolate linearly in left rows."""
+    star4 query's first join (?s 30 ?a join ?s 31 ?b on the same store), so
+    the CPU does the same kind of work; rows/s over that prefix."""
     from oracle import pyoracle as O
-    from vedas_amd import _native
-    ctx, idx = engine_ctx
     rels = []
     for pred in (30, 31):
         b, e = idx.lookup("PSO", pred)
@@ -153,23 +215,17 @@ def cpu_baseline(engine_ctx, sample_seconds: float):
         r.free()
     L, R = rels if len(rels[0][0]) <= len(rels[1][0]) else rels[::-1]
     m = min(len(L[0]), 2000)
-    rows = 0
-    t0 = time.perf_counter()
-    elapsed = 0.0
-    while elapsed < sample_seconds and m <= len(L[0]):
+    while True:
+        t0 = time.perf_counter()
         out = O.join([c[:m] for c in L], R, 1)
-        rows = len(out[0])
-        elapsed = time.perf_counter() - t0
-        if elapsed < sample_seconds / 4 and m < len(L[0]):
-            m = min(len(L[0]), m * 4)
-            t0 = time.perf_counter()
-            elapsed = 0.0
-            continue
-        break
-    return {"value": rows / max(elapsed, 1e-9), "unit": "join rows/s", "cores": 1,
-            "kind": "port",
-            "sample": f"oracle or_join (1 thread) of the first {m} left rows of "
-                      f"star4's first join ({len(L[0])} x {len(R[0])} rows); {rows} output rows"}
+        el = time.perf_counter() - t0
+        if el >= sample_seconds / 4 or m >= len(L[0]):
+            break
+        m = min(len(L[0]), m * 4)
+    rows = len(out[0])
+    return {"value": rows / max(el, 1e-9), "unit": "join rows/s", "cores": 1, "kind": "port",
+            "sample": f"oracle or_join (1 thread) of the first {m} left rows of star4's first "
+                      f"join ({len(L[0])} x {len(R[0])} rows): {rows} output rows in {el:.2f} s"}
 
 
 def main():
@@ -177,12 +233,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--triples", type=int, default=100_000_000)
+    ap.add_argument("--triples", type=int, default=STORE_100M["n"])
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--queries", default="star4,chain4,snowflake6")
     ap.add_argument("--selects", type=int, default=10_000)
+    ap.add_argument("--mix", type=int, default=256, help="queries in the batched mix (0: skip)")
+    ap.add_argument("--serve-qps", type=float, default=2000.0, help="0: skip serving")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -195,7 +253,7 @@ def main():
 
     log(f"[bench] generating {args.triples} triples")
     t = time.perf_counter()
-    s, p, o = _native.gen_triples(args.triples, seed=args.seed, **GEN)
+    s, p, o = _native.gen_triples(args.triples, seed=args.seed, distinct=False, **GEN)
     log(f"[bench] generated in {time.perf_counter() - t:.1f}s")
 
     eng = Engine(0)
@@ -203,9 +261,8 @@ def main():
     ds, dp, do = ctx.upload(s), ctx.upload(p), ctx.upload(o)
     n = len(s)
     perms = ("SPO", "PSO", "POS", "OSP")
-    # index build: one untimed warm-up build, then a timed one
     log("[bench] index build")
-    eng.load_device(ds, dp, do, n, perms)
+    eng.load_device(ds, dp, do, n, perms)  # warm-up build
     ctx.sync()
     ctx.prof(True)
     ctx.prof_reset()
@@ -231,7 +288,7 @@ def main():
         log(f"[bench] selects: {sel['selects_per_s']:.0f}/s, {sel['GB/s']:.0f} GB/s")
 
     names = [q for q in args.queries.split(",") if q]
-    texts = {q: query_text(QUERIES[q][0], QUERIES[q][1]) for q in names}
+    texts = {q: query_text(JOIN_SET[q][0], JOIN_SET[q][1]) for q in names}
 
     def step(collect=None):
         rows = 0
@@ -270,10 +327,26 @@ def main():
                 "avg_launch_ms": dom["ms"] / max(dom["launches"], 1),
                 "alg_bytes_per_launch": dom["bytes"] / max(dom["launches"], 1)}
 
+    batch = serve = None
+    if args.mix > 0:
+        mix = query_mix(args.mix, 3, STORE_100M, s, p, o)
+        mtexts = [query_text(pats, node) for _, pats, node in mix]
+        log(f"[bench] batched mix of {len(mtexts)} queries")
+        for r in eng.batch(mtexts[:16])[0]:  # warm-up
+            r.free()
+        batch = batch_bench(eng, mtexts)
+        log(f"[bench] mix: plain {batch['plain_qps']:.0f} q/s, batched {batch['batched_qps']:.0f} q/s")
+        if args.serve_qps > 0:
+            log("[bench] serving")
+            serve = serve_bench(eng, mtexts, args.serve_qps)
+            log(f"[bench] serving: plain {serve['plain']['qps']:.0f} q/s p99 "
+                f"{serve['plain']['p99_ms']:.1f} ms; served {serve['served']['qps']:.0f} q/s p99 "
+                f"{serve['served']['p99_ms']:.1f} ms")
+
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline")
-        cpu = cpu_baseline((ctx, idx_view), args.cpu_seconds)
+        cpu = cpu_baseline(idx_view, args.cpu_seconds)
 
     queries = {}
     for q, sts in per_q.items():
@@ -281,11 +354,8 @@ def main():
         jr = sts[0]["join_rows"]
         queries[q] = {"ms_median": ms, "join_rows": jr, "rows": sts[0]["rows"],
                       "join_rows_per_s": jr / (ms / 1e3), "swap_rows": sts[0]["swap_rows"]}
-    kernels = {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
-                   "GB/s": round(v["bytes"] / (v["ms"] / 1e3) / 1e9, 1) if v["ms"] > 0 else None}
-               for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])}
     out = {
-        "metric": "join rows/s (config-2 query set on the 100M-triple store)",
+        "metric": "join rows/s (single-query join set on the 100M-triple store)",
         "value": total_rows / dt,
         "unit": "join rows/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -299,17 +369,14 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "queries": queries,
-        "index_build": {"ms": build_s * 1e3, "permutations": list(perms),
-                        "key_bits": kb, "alg_bytes": build_alg,
-                        "GB/s": build_alg / build_s / 1e9,
-                        "frac": build_alg / build_s / HBM_PEAK,
-                        "keys_per_s": 4 * n / build_s,
-                        "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
-                                        "GB/s": round(v["bytes"] / (v["ms"] / 1e3) / 1e9, 1)
-                                        if v["ms"] > 0 else None}
-                                    for k, v in sorted(build_prof.items(), key=lambda kv: -kv[1]["ms"])}},
-        "kernels": kernels,
+        "index_build": {"ms": build_s * 1e3, "permutations": list(perms), "key_bits": kb,
+                        "alg_bytes": build_alg, "GB/s": build_alg / build_s / 1e9,
+                        "frac": build_alg / build_s / HBM_PEAK, "keys_per_s": 4 * n / build_s,
+                        "kernels": kernel_table(build_prof)},
         "range_selects": sel,
+        "batched_mix": batch,
+        "serving": serve,
+        "kernels": kernel_table(prof),
     }
     print(json.dumps(out), flush=True)
     eng.close()

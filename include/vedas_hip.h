@@ -107,6 +107,13 @@ uint32_t vd_index_perms(const vd_index *idx);   /* mask of built perms */
  * giving an empty range (the reference returns the next key's range). */
 int  vd_index_lookup(vd_ctx *ctx, const vd_index *idx, int perm, uint32_t k1,
                      int has_k2, uint32_t k2, uint64_t *begin, uint64_t *end);
+/* Statistics of a permutation for size estimates: for every L1 entry i
+ * (leading key X = l1_values[i]) the number of rows and the number of
+ * distinct (X,Y) pairs, into device arrays of n_l1 entries.  Plays the role
+ * of the reference's per-key run lengths and Histogram / DataMetaInfo
+ * statistics (VedasStorage.cu:278-288,342-355; Histogram.cu:170-231). */
+int  vd_index_stats(vd_ctx *ctx, const vd_index *idx, int perm,
+                    uint64_t *d_rows, uint64_t *d_pairs);
 /* Batched form on device arrays: d_k2 may be NULL (one constant for every
  * query) or hold VD_UNBOUND for queries with one constant. */
 int  vd_index_lookup_batch(vd_ctx *ctx, const vd_index *idx, int perm,

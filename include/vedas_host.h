@@ -63,6 +63,30 @@ typedef struct {
 } vdq_batch_stats_t;
 int vdq_batch(vdq_engine *e, const char *const *texts, int n, vdq_result **outs,
               vdq_batch_stats_t *stats);
+/* Serving (no counterpart in the reference): requests arrive open loop at
+ * arrival_ms (relative to the call), are admitted while their estimated
+ * peak memory fits mem_budget, and run in continuous batches on `slots`
+ * concurrent streams.  plain != 0 runs each request alone through the
+ * single-query path on one stream (the baseline).  recs[i] reports
+ * request i; outs (may be NULL) receives the results. */
+typedef struct {
+    int slots;
+    int max_batch;
+    uint64_t mem_budget;
+    int plain;
+} vdq_serve_cfg_t;
+typedef struct {
+    double arrival_ms, start_ms, done_ms;
+    double est_bytes;
+    uint64_t rows;
+    int batch, slot;
+} vdq_serve_rec_t;
+int vdq_serve(vdq_engine *e, const char *const *texts, const double *arrival_ms, int n,
+              const vdq_serve_cfg_t *cfg, vdq_serve_rec_t *recs, vdq_result **outs);
+/* Estimated final rows and peak HBM bytes of a query from the store
+ * statistics (what serving admission uses). */
+int vdq_estimate(vdq_engine *e, const char *text, double *rows, double *peak_bytes);
+
 int vdq_result_shape(const vdq_result *r, int *ncols, uint64_t *nrows);
 const char *vdq_result_header(const vdq_result *r, int col);
 int vdq_result_column(const vdq_result *r, int col, uint32_t *host_out);

@@ -68,6 +68,19 @@ def test_generator_deterministic_and_distinct():
     assert abs((p == 1).mean() - 1 / h) < 0.01
 
 
+def test_generator_chunks_are_independent():
+    """Triples are drawn in 2^20-triple chunks: later chunks must not
+    repeat earlier ones (a shifted stream would)."""
+    from vedas_amd._native import gen_triples
+    n = 3 << 20
+    s, p, o = gen_triples(n, seed=3, n_entities=10_000_000, n_predicates=256,
+                          literal_frac=0.4, n_literals=50_000_000, distinct=False)
+    key = (s.astype(np.uint64) << np.uint64(32)) | o
+    assert len(np.unique(key)) > 0.99 * n
+    c = 1 << 20
+    assert not np.array_equal(s[1:c], s[c:2 * c - 1])
+
+
 def test_generator_zipf_subjects():
     from vedas_amd._native import gen_triples
     s, p, o = gen_triples(200_000, seed=1, n_entities=10_000, subject_zipf=0.8,

@@ -74,13 +74,19 @@ def test_index_build_wide_ids(ctx):
     check_index(ctx, s, p, o, ["PSO", "OSP"])
 
 
-def test_index_build_too_wide_is_an_error(ctx):
-    from vedas_amd._native import Index, VedasError
-    n = 1000
+@pytest.mark.parametrize("n", [1, 3, 5000, 70_001])
+def test_index_build_over_64_bits(ctx, n):
+    """IDs needing more than 64 bits per packed triple take the two-phase
+    path (sort (Y,Z) carrying X, then stable by X)."""
+    rng = np.random.default_rng(n)
+    s = rng.integers(1, 1 << 26, n).astype(np.uint32)
+    p = rng.integers(1, 1 << 11, n).astype(np.uint32)
+    o = rng.integers(1, 1 << 29, n).astype(np.uint32)
+    s[: n // 2] = s[0]  # long X runs
+    check_index(ctx, s, p, o)
     big = np.full(n, 0xFFFFFFF0, np.uint32)
-    with pytest.raises(VedasError) as e:
-        Index(ctx, big, big, big, ["SPO"])
-    assert e.value.code == -4
+    big[::3] = 0xFFFFFFFF
+    check_index(ctx, big, big[::-1].copy(), big, ["SPO", "OSP"])
 
 
 @pytest.fixture(scope="module")

@@ -1,0 +1,66 @@
+"""GPU tests of the serving loop: every request's result is the same served
+in the mix (continuous batches on several streams) as run alone."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def served():
+    from vedas_amd._native import gen_triples
+    from vedas_amd.engine import Engine, query_text
+    from vedas_amd.workload import query_mix
+    prm = dict(n=1_000_000, n_entities=100_000, subject_zipf=0.8, n_predicates=64,
+               pred_zipf=1.1, literal_frac=0.4, literal_base=50_000_000, n_literals=500_000)
+    s, p, o = gen_triples(prm["n"], seed=21, **{k: v for k, v in prm.items() if k != "n"},
+                          distinct=True)
+    mix = query_mix(40, 9, prm, s, p, o, lo=10, hi=2e5)
+    texts = [query_text(pats, node) for _, pats, node in mix]
+    e = Engine(0)
+    e.load(s, p, o, ("SPO", "PSO", "POS", "OSP"))
+    alone = []
+    for t in texts:
+        r = e.query(t)
+        alone.append(O.canonical(r.headers(), r.columns()))
+        r.free()
+    yield e, texts, alone
+    e.close()
+
+
+def arrivals(n, qps, seed):
+    rng = np.random.default_rng(seed)
+    return list(np.cumsum(rng.exponential(1000.0 / qps, n)))
+
+
+@pytest.mark.parametrize("slots,max_batch,budget", [(3, 8, 64 << 30), (2, 32, 64 << 30),
+                                                    (4, 4, 1 << 20)])
+def test_served_results_equal_alone(served, slots, max_batch, budget):
+    e, texts, alone = served
+    recs, res = e.serve(texts, arrivals(len(texts), 2000.0, slots), slots=slots,
+                        max_batch=max_batch, mem_budget=budget, keep_results=True)
+    got = [O.canonical(r.headers(), r.columns()) for r in res]
+    for r in res:
+        r.free()
+    for i, (a, b) in enumerate(zip(alone, got)):
+        assert a == b, i
+    for rc in recs:
+        assert rc["done_ms"] >= rc["start_ms"] >= 0
+        assert rc["start_ms"] + 1e-3 >= rc["arrival_ms"] - 1.0  # started after arriving
+        assert rc["slot"] >= 0 and rc["batch"] >= 0
+    assert len({rc["batch"] for rc in recs}) < len(texts) or max_batch == 1
+    assert max(rc["slot"] for rc in recs) < slots
+
+
+def test_plain_serving_and_estimates(served):
+    e, texts, alone = served
+    recs, res = e.serve(texts, arrivals(len(texts), 500.0, 1), plain=True, keep_results=True)
+    got = [O.canonical(r.headers(), r.columns()) for r in res]
+    for r in res:
+        r.free()
+    assert got == alone
+    assert len({rc["batch"] for rc in recs}) == len(texts)  # one request per batch
+    rows, peak = e.estimate(texts[0])
+    assert rows >= 0 and peak > 0

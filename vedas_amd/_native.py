@@ -73,6 +73,7 @@ EXPORTS = {
     "vd_index_lookup": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_uint32,
                                   C.c_int, C.c_uint32, C.POINTER(C.c_uint64),
                                   C.POINTER(C.c_uint64)]),
+    "vd_index_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "vd_index_lookup_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                         C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]),
     "vd_rel_create": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),

@@ -26,6 +26,16 @@ inline uint64_t splitmix64(uint64_t &x) {
     return z ^ (z >> 31);
 }
 
+// 64-bit finaliser of MurmurHash3 (a bijection that mixes all bits).
+inline uint64_t fmix64(uint64_t h) {
+    h ^= h >> 33;
+    h *= 0xFF51AFD7ED558CCDull;
+    h ^= h >> 33;
+    h *= 0xC4CEB9FE1A85EC53ull;
+    h ^= h >> 33;
+    return h;
+}
+
 inline double unit(uint64_t &st) {
     return (double)(splitmix64(st) >> 11) * (1.0 / 9007199254740992.0);
 }
@@ -72,8 +82,10 @@ struct Gen {
     }
     void chunk(uint64_t k, uint64_t count, uint32_t *s, uint32_t *pp,
                uint32_t *o) const {
-        uint64_t st = p.seed * 0x2545F4914F6CDD1Dull + k * 0x9E3779B97F4A7C15ull + 1;
-        splitmix64(st);
+        // Start of chunk k's stream: a hash of (seed, k).  (An offset of
+        // k times the splitmix increment would only shift chunk 0's stream
+        // by k draws.)
+        uint64_t st = fmix64(fmix64(p.seed + 0x632BE59BD9B4E019ull) ^ (k * 0xD1B54A32D192ED03ull + 1));
         for (uint64_t i = 0; i < count; ++i) {
             s[i] = subj.draw(st);
             pp[i] = pred.draw(st);

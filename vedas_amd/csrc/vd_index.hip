@@ -17,6 +17,7 @@
 // columns are the reference's xy_data / xyz_data arrays.
 
 #include <algorithm>
+#include <type_traits>
 
 #include "vd_radix_dev.hpp"
 
@@ -63,12 +64,42 @@ __global__ __launch_bounds__(PK_BLOCK) void k_pack_xyz(
     radix::hist_flush(&s, ghist, npass);
 }
 
-// Columns Y and Z out of the sorted keys; per-block head counts.
+// Sorted triples, one row = (X, Y, Z).  PackedRows: the whole triple in one
+// 64-bit word.  SplitRows: X in its own column and (Y, Z) packed in a 64-bit
+// word, for IDs that need more than 64 bits in total.
+struct Row {
+    uint32_t x, y, z;
+    uint64_t xy;  // identity of the (X, Y) pair
+};
+
+struct PackedRows {
+    const uint64_t *k;
+    int sx, sy;
+    uint32_t ymask, zmask;
+    __device__ __forceinline__ Row at(uint64_t i) const {
+        const uint64_t v = k[i];
+        return Row{(uint32_t)(v >> sx), (uint32_t)(v >> sy) & ymask, (uint32_t)v & zmask, v >> sy};
+    }
+};
+
+struct SplitRows {
+    const uint32_t *xs;
+    const uint64_t *lo;
+    int sy;
+    uint32_t ymask, zmask;
+    __device__ __forceinline__ Row at(uint64_t i) const {
+        const uint64_t v = lo[i];
+        const uint32_t x = xs[i], y = (uint32_t)(v >> sy) & ymask;
+        return Row{x, y, (uint32_t)v & zmask, ((uint64_t)x << 32) | y};
+    }
+};
+
+// Columns Y and Z; per-block counts of L1 heads (X changes) and L2 heads
+// ((X,Y) changes).
+template <typename R>
 __global__ __launch_bounds__(UN_BLOCK) void k_unpack_count(
-    const uint64_t *__restrict__ keys, uint64_t n, int sy, int sx,
-    uint32_t ymask, uint32_t zmask, uint32_t *__restrict__ ycol,
-    uint32_t *__restrict__ zcol, uint32_t *__restrict__ cnt_l1,
-    uint32_t *__restrict__ cnt_l2) {
+    R rows, uint64_t n, uint32_t *__restrict__ ycol, uint32_t *__restrict__ zcol,
+    uint32_t *__restrict__ cnt_l1, uint32_t *__restrict__ cnt_l2) {
     __shared__ uint32_t s_tmp[UN_BLOCK / 64];
     const uint64_t base = (uint64_t)blockIdx.x * UN_TILE;
     uint32_t c1 = 0, c2 = 0;
@@ -76,15 +107,15 @@ __global__ __launch_bounds__(UN_BLOCK) void k_unpack_count(
     for (int j = 0; j < UN_ITEMS; ++j) {
         const uint64_t i = base + (uint64_t)j * UN_BLOCK + threadIdx.x;
         if (i < n) {
-            const uint64_t k = keys[i];
-            ycol[i] = (uint32_t)(k >> sy) & ymask;
-            zcol[i] = (uint32_t)k & zmask;
+            const Row r = rows.at(i);
+            ycol[i] = r.y;
+            zcol[i] = r.z;
             if (i == 0) {
                 c1 += 1; c2 += 1;
             } else {
-                const uint64_t pk = keys[i - 1];
-                c1 += (k >> sx) != (pk >> sx);
-                c2 += (k >> sy) != (pk >> sy);
+                const Row q = rows.at(i - 1);
+                c1 += r.x != q.x;
+                c2 += r.xy != q.xy;
             }
         }
     }
@@ -96,9 +127,9 @@ __global__ __launch_bounds__(UN_BLOCK) void k_unpack_count(
     }
 }
 
+template <typename R>
 __global__ __launch_bounds__(UN_BLOCK) void k_compact_heads(
-    const uint64_t *__restrict__ keys, uint64_t n, int sy, int sx,
-    uint32_t ymask, const uint64_t *__restrict__ off_l1,
+    R rows, uint64_t n, const uint64_t *__restrict__ off_l1,
     const uint64_t *__restrict__ off_l2, uint32_t *__restrict__ l1v,
     uint32_t *__restrict__ l1o, uint32_t *__restrict__ l2v,
     uint32_t *__restrict__ l2o) {
@@ -107,22 +138,23 @@ __global__ __launch_bounds__(UN_BLOCK) void k_compact_heads(
     // Each thread owns UN_ITEMS consecutive rows so the block scan keeps row
     // order.
     bool h1[UN_ITEMS], h2[UN_ITEMS];
-    uint64_t kk[UN_ITEMS];
+    uint32_t xv[UN_ITEMS], yv[UN_ITEMS];
     uint32_t c1 = 0, c2 = 0;
 #pragma unroll
     for (int j = 0; j < UN_ITEMS; ++j) {
         const uint64_t i = base + (uint64_t)threadIdx.x * UN_ITEMS + j;
         h1[j] = h2[j] = false;
-        kk[j] = 0;
+        xv[j] = yv[j] = 0;
         if (i < n) {
-            const uint64_t k = keys[i];
-            kk[j] = k;
+            const Row r = rows.at(i);
+            xv[j] = r.x;
+            yv[j] = r.y;
             if (i == 0) {
                 h1[j] = h2[j] = true;
             } else {
-                const uint64_t pk = keys[i - 1];
-                h1[j] = (k >> sx) != (pk >> sx);
-                h2[j] = (k >> sy) != (pk >> sy);
+                const Row q = rows.at(i - 1);
+                h1[j] = r.x != q.x;
+                h2[j] = r.xy != q.xy;
             }
             c1 += h1[j];
             c2 += h2[j];
@@ -136,15 +168,72 @@ __global__ __launch_bounds__(UN_BLOCK) void k_compact_heads(
     for (int j = 0; j < UN_ITEMS; ++j) {
         const uint64_t i = base + (uint64_t)threadIdx.x * UN_ITEMS + j;
         if (h1[j]) {
-            l1v[o1] = (uint32_t)(kk[j] >> sx);
+            l1v[o1] = xv[j];
             l1o[o1] = (uint32_t)i;
             ++o1;
         }
         if (h2[j]) {
-            l2v[o2] = (uint32_t)(kk[j] >> sy) & ymask;
+            l2v[o2] = yv[j];
             l2o[o2] = (uint32_t)i;
             ++o2;
         }
+    }
+}
+
+// Split path, phase 1: key = Y << bZ | Z, value = X.
+__global__ __launch_bounds__(PK_BLOCK) void k_pack_split(
+    const uint32_t *__restrict__ x, const uint32_t *__restrict__ y,
+    const uint32_t *__restrict__ z, uint64_t n, int sy, uint64_t *__restrict__ keys,
+    uint32_t *__restrict__ vals, int kbits, int npass, uint32_t *ghist) {
+    __shared__ radix::LdsHist s;
+    radix::hist_zero(&s);
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * (PK_BLOCK * PK_ITEMS);
+#pragma unroll 4
+    for (int j = 0; j < PK_ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)j * PK_BLOCK + threadIdx.x;
+        if (i < n) {
+            const uint64_t key = ((uint64_t)y[i] << sy) | (uint64_t)z[i];
+            keys[i] = key;
+            vals[i] = x[i];
+            radix::hist_add(&s, key, 0, kbits, npass);
+        }
+    }
+    __syncthreads();
+    radix::hist_flush(&s, ghist, npass);
+}
+
+// Split path, phase 2: key = X << 32 | position, sorted on X's bits only.
+__global__ __launch_bounds__(PK_BLOCK) void k_pack_xpos(
+    const uint32_t *__restrict__ xs, uint64_t n, uint64_t *__restrict__ keys, int hi_bit,
+    int npass, uint32_t *ghist) {
+    __shared__ radix::LdsHist s;
+    radix::hist_zero(&s);
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * (PK_BLOCK * PK_ITEMS);
+#pragma unroll 4
+    for (int j = 0; j < PK_ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)j * PK_BLOCK + threadIdx.x;
+        if (i < n) {
+            const uint64_t key = ((uint64_t)xs[i] << 32) | i;
+            keys[i] = key;
+            radix::hist_add(&s, key, 32, hi_bit, npass);
+        }
+    }
+    __syncthreads();
+    radix::hist_flush(&s, ghist, npass);
+}
+
+// X column and (Y,Z) words in final order.
+__global__ __launch_bounds__(256) void k_split_gather(const uint64_t *__restrict__ k2,
+                                                      const uint64_t *__restrict__ lo, uint64_t n,
+                                                      uint32_t *__restrict__ xs,
+                                                      uint64_t *__restrict__ lo_out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t k = k2[i];
+        xs[i] = (uint32_t)(k >> 32);
+        lo_out[i] = lo[(uint32_t)k];
     }
 }
 
@@ -161,6 +250,16 @@ __global__ __launch_bounds__(256) void k_lookup_batch(
     if (begin) begin[q] = b;
     if (end) end[q] = e;
     if (sizes) sizes[q] = e - b;
+}
+
+__global__ __launch_bounds__(256) void k_index_stats(
+    const uint32_t *__restrict__ l1o, uint64_t n_l1, const uint32_t *__restrict__ l2o,
+    uint64_t n_l2, uint64_t n, uint64_t *__restrict__ rows, uint64_t *__restrict__ pairs) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_l1) return;
+    const uint64_t b = l1o[i], e = (i + 1 < n_l1) ? l1o[i + 1] : n;
+    rows[i] = e - b;
+    pairs[i] = dev::lower_bound(l2o, 0, n_l2, e) - dev::lower_bound(l2o, 0, n_l2, b);
 }
 
 // Rows of query q are output rows [off[q], off[q+1]); one thread per output
@@ -261,39 +360,18 @@ unsigned stream_grid(uint64_t n) {
     return g ? (unsigned)g : 1u;
 }
 
-void build_perm(vd_ctx *ctx, vd_index *idx, int perm, const uint32_t *cols[3],
-                const int bits[3], uint64_t *keys, uint64_t *tmp) {
-    const uint64_t n = idx->n;
-    vd_perm_arrays &pa = idx->perm[perm];
-    const int xi = PERM_COLS[perm][0], yi = PERM_COLS[perm][1],
-              zi = PERM_COLS[perm][2];
-    const int bz = std::max(bits[zi], 1), by = std::max(bits[yi], 1),
-              bx = std::max(bits[xi], 1);
-    const int kbits = bx + by + bz;
-    VD_REQUIRE(kbits <= 64, VD_ERANGE,
-               "triple IDs need more than 64 bits packed; not supported");
-    const int sy = bz, sx = by + bz;
-    const int npass = radix_passes(0, kbits);
-
-    DBuf<uint32_t> hist(ctx, radix_hist_words(0, kbits));
-    fill_u32(ctx, hist.p, 0, radix_hist_words(0, kbits));
-    const uint64_t pk_tiles = ceil_div(n, PK_BLOCK * PK_ITEMS);
-    launch(ctx, "index_pack", (double)n * 20.0, dim3((unsigned)pk_tiles),
-           dim3(PK_BLOCK), 0, k_pack_xyz, cols[xi], cols[yi], cols[zi], n, sy,
-           sx, keys, kbits, npass, hist.p);
-    uint64_t *sorted = radix_sort_u64(ctx, keys, tmp, n, 0, kbits, hist.p, true);
-
+// Y/Z columns and L1/L2 arrays from sorted rows.
+template <typename R>
+void finish_perm(vd_ctx *ctx, vd_perm_arrays &pa, uint64_t n, const R &rows) {
     pa.n = n;
     pa.y = (uint32_t *)dalloc(ctx, n * 4);
     pa.z = (uint32_t *)dalloc(ctx, n * 4);
     const uint64_t tiles = ceil_div(n, UN_TILE);
     DBuf<uint32_t> c1(ctx, tiles), c2(ctx, tiles);
     DBuf<uint64_t> o1(ctx, tiles + 1), o2(ctx, tiles + 1);
-    const uint32_t ymask = by >= 32 ? 0xFFFFFFFFu : ((1u << by) - 1u);
-    const uint32_t zmask = bz >= 32 ? 0xFFFFFFFFu : ((1u << bz) - 1u);
-    launch(ctx, "index_unpack", (double)n * 16.0, dim3((unsigned)tiles),
-           dim3(UN_BLOCK), 0, k_unpack_count, (const uint64_t *)sorted, n, sy,
-           sx, ymask, zmask, pa.y, pa.z, c1.p, c2.p);
+    constexpr bool split = std::is_same<R, SplitRows>::value;
+    launch(ctx, "index_unpack", (double)n * (split ? 20.0 : 16.0), dim3((unsigned)tiles),
+           dim3(UN_BLOCK), 0, k_unpack_count<R>, rows, n, pa.y, pa.z, c1.p, c2.p);
     scan_u32_to_u64(ctx, c1.p, o1.p, tiles, o1.p + tiles);
     scan_u32_to_u64(ctx, c2.p, o2.p, tiles, o2.p + tiles);
     uint64_t tot[2];
@@ -309,10 +387,61 @@ void build_perm(vd_ctx *ctx, vd_index *idx, int perm, const uint32_t *cols[3],
     pa.l1o = (uint32_t *)dalloc(ctx, pa.n_l1 * 4);
     pa.l2v = (uint32_t *)dalloc(ctx, pa.n_l2 * 4);
     pa.l2o = (uint32_t *)dalloc(ctx, pa.n_l2 * 4);
-    launch(ctx, "index_heads", (double)n * 8.0 + (pa.n_l1 + pa.n_l2) * 8.0,
-           dim3((unsigned)tiles), dim3(UN_BLOCK), 0, k_compact_heads,
-           (const uint64_t *)sorted, n, sy, sx, ymask, (const uint64_t *)o1.p,
-           (const uint64_t *)o2.p, pa.l1v, pa.l1o, pa.l2v, pa.l2o);
+    launch(ctx, "index_heads", (double)n * (split ? 12.0 : 8.0) + (pa.n_l1 + pa.n_l2) * 8.0,
+           dim3((unsigned)tiles), dim3(UN_BLOCK), 0, k_compact_heads<R>, rows, n,
+           (const uint64_t *)o1.p, (const uint64_t *)o2.p, pa.l1v, pa.l1o, pa.l2v, pa.l2o);
+}
+
+void build_perm(vd_ctx *ctx, vd_index *idx, int perm, const uint32_t *cols[3],
+                const int bits[3], uint64_t *keys, uint64_t *tmp) {
+    const uint64_t n = idx->n;
+    vd_perm_arrays &pa = idx->perm[perm];
+    const int xi = PERM_COLS[perm][0], yi = PERM_COLS[perm][1],
+              zi = PERM_COLS[perm][2];
+    const int bz = std::max(bits[zi], 1), by = std::max(bits[yi], 1),
+              bx = std::max(bits[xi], 1);
+    const int kbits = bx + by + bz;
+    const int sy = bz, sx = by + bz;
+    const uint32_t ymask = by >= 32 ? 0xFFFFFFFFu : ((1u << by) - 1u);
+    const uint32_t zmask = bz >= 32 ? 0xFFFFFFFFu : ((1u << bz) - 1u);
+    const uint64_t pk_tiles = ceil_div(n, PK_BLOCK * PK_ITEMS);
+
+    if (kbits <= 64) {
+        // The whole triple in one key: a key-only sort.
+        const int npass = radix_passes(0, kbits);
+        DBuf<uint32_t> hist(ctx, radix_hist_words(0, kbits));
+        fill_u32(ctx, hist.p, 0, radix_hist_words(0, kbits));
+        launch(ctx, "index_pack", (double)n * 20.0, dim3((unsigned)pk_tiles),
+               dim3(PK_BLOCK), 0, k_pack_xyz, cols[xi], cols[yi], cols[zi], n, sy,
+               sx, keys, kbits, npass, hist.p);
+        uint64_t *sorted = radix_sort_u64(ctx, keys, tmp, n, 0, kbits, hist.p, true);
+        finish_perm(ctx, pa, n, PackedRows{sorted, sx, sy, ymask, zmask});
+    } else {
+        // Two LSD phases: (Y,Z) with X carried as a value, then a stable
+        // sort on X (key X << 32 | position) and a gather of the (Y,Z) words.
+        const int lbits = by + bz;
+        DBuf<uint32_t> xa(ctx, n), xb(ctx, n);
+        DBuf<uint32_t> hist(ctx, radix_hist_words(0, lbits));
+        fill_u32(ctx, hist.p, 0, radix_hist_words(0, lbits));
+        launch(ctx, "index_pack", (double)n * 24.0, dim3((unsigned)pk_tiles), dim3(PK_BLOCK), 0,
+               k_pack_split, cols[xi], cols[yi], cols[zi], n, sy, keys, xa.p, lbits,
+               radix_passes(0, lbits), hist.p);
+        uint64_t *lo = keys, *lo_tmp = tmp;
+        uint32_t *xv = xa.p, *xv_tmp = xb.p;
+        radix_sort_u64_kv(ctx, lo, lo_tmp, xv, xv_tmp, n, 0, lbits, hist.p, true);
+        // lo/xv hold phase-1 order; lo_tmp / xv_tmp are free.
+        DBuf<uint64_t> k2(ctx, n), k2t(ctx, n);
+        const int hi = 32 + bx;
+        const int npass2 = radix_passes(32, hi);
+        DBuf<uint32_t> hist2(ctx, radix_hist_words(32, hi));
+        fill_u32(ctx, hist2.p, 0, radix_hist_words(32, hi));
+        launch(ctx, "index_pack", (double)n * 12.0, dim3((unsigned)pk_tiles), dim3(PK_BLOCK), 0,
+               k_pack_xpos, (const uint32_t *)xv, n, k2.p, hi, npass2, hist2.p);
+        uint64_t *s2 = radix_sort_u64(ctx, k2.p, k2t.p, n, 32, hi, hist2.p, true);
+        launch(ctx, "index_gather", (double)n * 24.0, dim3(stream_grid(n)), dim3(256), 0,
+               k_split_gather, (const uint64_t *)s2, (const uint64_t *)lo, n, xv_tmp, lo_tmp);
+        finish_perm(ctx, pa, n, SplitRows{xv_tmp, lo_tmp, sy, ymask, zmask});
+    }
     pa.built = true;
 }
 
@@ -431,6 +560,19 @@ int vd_index_lookup_batch(vd_ctx *ctx, const vd_index *idx, int perm,
         launch(ctx, "index_lookup", (double)nq * 24.0,
                dim3((unsigned)ceil_div(nq, 256)), dim3(256), 0, k_lookup_batch,
                perm_dev(pa), d_k1, d_k2, nq, d_begin, d_end, (uint64_t *)nullptr);
+    });
+}
+
+int vd_index_stats(vd_ctx *ctx, const vd_index *idx, int perm, uint64_t *d_rows,
+                   uint64_t *d_pairs) {
+    if (!ctx) return VD_EINVAL;
+    VD_TRY(ctx, {
+        const vd_perm_arrays &pa = get_perm(idx, perm);
+        VD_REQUIRE(pa.n_l1 == 0 || (d_rows && d_pairs), VD_EINVAL, "null output arrays");
+        launch(ctx, "index_stats", (double)pa.n_l1 * 24.0,
+               dim3((unsigned)ceil_div(pa.n_l1, 256)), dim3(256), 0, k_index_stats,
+               (const uint32_t *)pa.l1o, pa.n_l1, (const uint32_t *)pa.l2o, pa.n_l2, pa.n,
+               d_rows, d_pairs);
     });
 }
 

@@ -189,6 +189,11 @@ void copy_d2d(vd_ctx *ctx, void *dst, const void *src, size_t bytes);
 uint64_t *radix_sort_u64(vd_ctx *ctx, uint64_t *keys, uint64_t *tmp,
                          uint64_t n, int lo_bit, int hi_bit,
                          uint32_t *d_hist, bool hist_ready);
+// Same with a 32-bit value per key; on return keys/vals hold the sorted
+// data (the four pointers are swapped as the passes alternate buffers).
+void radix_sort_u64_kv(vd_ctx *ctx, uint64_t *&keys, uint64_t *&ktmp, uint32_t *&vals,
+                       uint32_t *&vtmp, uint64_t n, int lo_bit, int hi_bit,
+                       uint32_t *d_hist, bool hist_ready);
 int radix_passes(int lo_bit, int hi_bit);
 // Device histogram buffer size (uint32 words) needed for radix_sort_u64.
 inline size_t radix_hist_words(int lo_bit, int hi_bit) {

@@ -66,13 +66,16 @@ __global__ __launch_bounds__(RBINS) void k_rs_scan_hist(const uint32_t *ghist,
     gofs[blockIdx.x * RBINS + threadIdx.x] = ex;
 }
 
-template <int ITEMS>
+// VAL: a 32-bit value travels with every key (same source and destination).
+template <int ITEMS, bool VAL>
 __global__ __launch_bounds__(RS_BLOCK) void k_rs_onesweep(
-    const uint64_t *__restrict__ in, uint64_t *__restrict__ out, uint64_t n,
+    const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
+    const uint32_t *__restrict__ vin, uint32_t *__restrict__ vout, uint64_t n,
     int shift, int width, const uint32_t *__restrict__ gofs,
     uint32_t *lookback, uint32_t *tile_counter) {
     constexpr int TILE = RS_BLOCK * ITEMS;
     __shared__ uint64_t s_keys[TILE];
+    __shared__ uint32_t s_vals[VAL ? TILE : 1];
     __shared__ uint32_t s_wcnt[RS_WAVES][RBINS];
     __shared__ uint32_t s_lstart[RBINS];
     __shared__ uint32_t s_gbase[RBINS];
@@ -97,10 +100,12 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_onesweep(
     // also the ranking order below.  Missing keys are all-ones: they carry
     // the largest digit and rank after every real key of the tile.
     uint64_t k[ITEMS];
+    uint32_t v[VAL ? ITEMS : 1];
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
         const uint64_t idx = base + (uint64_t)((w * ITEMS + j) * 64 + lane);
         k[j] = idx < n ? in[idx] : ~0ull;
+        if (VAL) v[VAL ? j : 0] = idx < n ? vin[idx] : 0u;
     }
 
     // Rank within the wave: peers = lanes holding the same digit.
@@ -170,20 +175,24 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_onesweep(
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
         const uint32_t d = (uint32_t)(k[j] >> shift) & dmask;
-        s_keys[s_lstart[d] + s_wcnt[w][d] + rk[j]] = k[j];
+        const uint32_t pos = s_lstart[d] + s_wcnt[w][d] + rk[j];
+        s_keys[pos] = k[j];
+        if (VAL) s_vals[VAL ? pos : 0] = v[VAL ? j : 0];
     }
     __syncthreads();
 
     for (uint32_t i = tid; i < nvalid; i += RS_BLOCK) {
         const uint64_t key = s_keys[i];
         const uint32_t d = (uint32_t)(key >> shift) & dmask;
-        out[(uint64_t)(s_gbase[d] + i)] = key;
+        const uint64_t dst = (uint64_t)(s_gbase[d] + i);
+        out[dst] = key;
+        if (VAL) vout[dst] = s_vals[VAL ? i : 0];
     }
 }
 
-template <int ITEMS>
-void run_passes(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, uint64_t n,
-                int lo_bit, int hi_bit, int npass, const uint32_t *gofs) {
+template <int ITEMS, bool VAL>
+void run_passes(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, uint32_t *&va, uint32_t *&vb,
+                uint64_t n, int lo_bit, int hi_bit, int npass, const uint32_t *gofs) {
     constexpr uint64_t TILE = RS_BLOCK * ITEMS;
     const uint64_t ntiles = ceil_div(n, TILE);
     const uint64_t lb_words = (uint64_t)npass * ntiles * RBINS;
@@ -193,11 +202,13 @@ void run_passes(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, uint64_t n,
     for (int p = 0; p < npass; ++p) {
         const int shift = lo_bit + p * RBITS;
         const int width = std::min(RBITS, hi_bit - shift);
-        launch(ctx, "radix_onesweep", (double)n * 16.0, dim3((unsigned)ntiles),
-               dim3(RS_BLOCK), 0, k_rs_onesweep<ITEMS>, (const uint64_t *)a, b,
-               n, shift, width, gofs + (uint64_t)p * RBINS,
+        launch(ctx, VAL ? "radix_onesweep_kv" : "radix_onesweep",
+               (double)n * (VAL ? 24.0 : 16.0), dim3((unsigned)ntiles),
+               dim3(RS_BLOCK), 0, k_rs_onesweep<ITEMS, VAL>, (const uint64_t *)a, b,
+               (const uint32_t *)va, vb, n, shift, width, gofs + (uint64_t)p * RBINS,
                lb.p + (uint64_t)p * ntiles * RBINS, counters + p);
         std::swap(a, b);
+        if (VAL) std::swap(va, vb);
     }
 }
 
@@ -208,13 +219,33 @@ int radix_passes(int lo_bit, int hi_bit) {
     return (hi_bit - lo_bit + RBITS - 1) / RBITS;
 }
 
+static void sort_impl(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, uint32_t *&va,
+                      uint32_t *&vb, bool val, uint64_t n, int lo_bit, int hi_bit,
+                      uint32_t *d_hist, bool hist_ready);
+
 uint64_t *radix_sort_u64(vd_ctx *ctx, uint64_t *keys, uint64_t *tmp,
                          uint64_t n, int lo_bit, int hi_bit, uint32_t *d_hist,
                          bool hist_ready) {
+    uint64_t *a = keys, *b = tmp;
+    uint32_t *va = nullptr, *vb = nullptr;
+    sort_impl(ctx, a, b, va, vb, false, n, lo_bit, hi_bit, d_hist, hist_ready);
+    return a;
+}
+
+void radix_sort_u64_kv(vd_ctx *ctx, uint64_t *&keys, uint64_t *&ktmp, uint32_t *&vals,
+                       uint32_t *&vtmp, uint64_t n, int lo_bit, int hi_bit,
+                       uint32_t *d_hist, bool hist_ready) {
+    sort_impl(ctx, keys, ktmp, vals, vtmp, true, n, lo_bit, hi_bit, d_hist, hist_ready);
+}
+
+static void sort_impl(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, uint32_t *&va,
+                      uint32_t *&vb, bool val, uint64_t n, int lo_bit, int hi_bit,
+                      uint32_t *d_hist, bool hist_ready) {
+    uint64_t *keys = a;
     const int npass = radix_passes(lo_bit, hi_bit);
     VD_REQUIRE(lo_bit >= 0 && hi_bit <= 64, VD_EINVAL, "radix bit range");
     VD_REQUIRE(npass <= radix::MAX_PASSES, VD_EINVAL, "too many radix passes");
-    if (n <= 1 || npass == 0) return keys;
+    if (n <= 1 || npass == 0) return;
     VD_REQUIRE(n <= (uint64_t)LB_VAL, VD_ERANGE,
                "radix sort supports at most 2^30-1 keys per call");
     if (!hist_ready) {
@@ -228,13 +259,18 @@ uint64_t *radix_sort_u64(vd_ctx *ctx, uint64_t *keys, uint64_t *tmp,
     DBuf<uint32_t> gofs(ctx, (uint64_t)npass * RBINS);
     launch(ctx, "radix_scan_hist", 0.0, dim3(npass), dim3(RBINS), 0,
            k_rs_scan_hist, (const uint32_t *)d_hist, gofs.p);
-    uint64_t *a = keys, *b = tmp;
-    switch (ctx->radix_items) {
-        case 8: run_passes<8>(ctx, a, b, n, lo_bit, hi_bit, npass, gofs.p); break;
-        case 24: run_passes<24>(ctx, a, b, n, lo_bit, hi_bit, npass, gofs.p); break;
-        default: run_passes<16>(ctx, a, b, n, lo_bit, hi_bit, npass, gofs.p); break;
+    if (val) {  // values need LDS staging: at most 16 keys per thread
+        if (ctx->radix_items == 8)
+            run_passes<8, true>(ctx, a, b, va, vb, n, lo_bit, hi_bit, npass, gofs.p);
+        else
+            run_passes<16, true>(ctx, a, b, va, vb, n, lo_bit, hi_bit, npass, gofs.p);
+        return;
     }
-    return a;
+    switch (ctx->radix_items) {
+        case 8: run_passes<8, false>(ctx, a, b, va, vb, n, lo_bit, hi_bit, npass, gofs.p); break;
+        case 24: run_passes<24, false>(ctx, a, b, va, vb, n, lo_bit, hi_bit, npass, gofs.p); break;
+        default: run_passes<16, false>(ctx, a, b, va, vb, n, lo_bit, hi_bit, npass, gofs.p); break;
+    }
 }
 
 }  // namespace vd

@@ -27,6 +27,17 @@ class BatchStats(C.Structure):
                 ("elapsed_ms", C.c_double)]
 
 
+class ServeCfg(C.Structure):
+    _fields_ = [("slots", C.c_int), ("max_batch", C.c_int), ("mem_budget", C.c_uint64),
+                ("plain", C.c_int)]
+
+
+class ServeRec(C.Structure):
+    _fields_ = [("arrival_ms", C.c_double), ("start_ms", C.c_double), ("done_ms", C.c_double),
+                ("est_bytes", C.c_double), ("rows", C.c_uint64), ("batch", C.c_int),
+                ("slot", C.c_int)]
+
+
 _lib = None
 
 
@@ -54,6 +65,9 @@ def lib() -> C.CDLL:
             "vdq_last_error": (C.c_char_p, []),
             "vdq_batch": (i, [vp, C.POINTER(C.c_char_p), i, C.POINTER(vp),
                               C.POINTER(BatchStats)]),
+            "vdq_serve": (i, [vp, C.POINTER(C.c_char_p), C.POINTER(C.c_double), i,
+                              C.POINTER(ServeCfg), C.POINTER(ServeRec), C.POINTER(vp)]),
+            "vdq_estimate": (i, [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(l, name)
@@ -143,6 +157,25 @@ class Engine:
         _check(lib().vdq_batch(self.h, arr, n, outs, C.byref(st)))
         return ([Result(C.c_void_p(outs[i])) for i in range(n)],
                 {k: getattr(st, k) for k, _ in BatchStats._fields_})
+
+    def serve(self, texts, arrival_ms, slots=4, max_batch=64, mem_budget=64 << 30,
+              plain=False, keep_results=False):
+        """Serve an open-loop arrival stream; returns (records, [Result] or None)."""
+        n = len(texts)
+        arr = (C.c_char_p * max(1, n))(*[t.encode() for t in texts])
+        arrival = (C.c_double * max(1, n))(*arrival_ms)
+        recs = (ServeRec * max(1, n))()
+        outs = (C.c_void_p * max(1, n))() if keep_results else None
+        cfg = ServeCfg(slots, max_batch, mem_budget, int(plain))
+        _check(lib().vdq_serve(self.h, arr, arrival, n, C.byref(cfg), recs, outs))
+        out = [{k: getattr(recs[i], k) for k, _ in ServeRec._fields_} for i in range(n)]
+        res = [Result(C.c_void_p(outs[i])) for i in range(n)] if keep_results else None
+        return out, res
+
+    def estimate(self, text: str):
+        r, b = C.c_double(), C.c_double()
+        _check(lib().vdq_estimate(self.h, text.encode(), C.byref(r), C.byref(b)))
+        return r.value, b.value
 
     def close(self):
         if self.h:

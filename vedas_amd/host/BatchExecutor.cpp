@@ -277,16 +277,16 @@ void BatchExecutor::run(std::vector<BatchQuery> &queries, std::vector<SparqlResu
         if (Q.q->root) {
             Q.root = walk(Q.q->root);
         } else {
-            if (Q.q->patterns.size() != 1) throw VedasError(VD_EINVAL, "plan required");
+            if (Q.q->patterns->size() != 1) throw VedasError(VD_EINVAL, "plan required");
             Node n;
             n.op = UPLOAD;
-            n.tp = &Q.q->patterns[0];
+            n.tp = const_cast<TriplePattern *>(&(*Q.q->patterns)[0]);
             n.var = n.tp->getSubject();
             Q.nodes.push_back(n);
             Q.root = 0;
         }
         std::map<std::string, int> cnt;
-        for (auto &tp : Q.q->patterns)
+        for (auto &tp : *Q.q->patterns)
             for (auto t : {tp.getSubject(), tp.getPredicate(), tp.getObject()})
                 if (!t.empty() && t[0] == '?') cnt[t]++;
         for (auto &kv : cnt)

@@ -25,7 +25,7 @@
 #include "VedasStorage.h"
 
 struct BatchQuery {
-    std::vector<TriplePattern> patterns;
+    const std::vector<TriplePattern> *patterns = nullptr;  // not owned
     std::set<std::string> selected;
     PlanTreeNode *root = nullptr;  // may be null for one pattern
 };

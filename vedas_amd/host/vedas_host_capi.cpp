@@ -8,11 +8,18 @@
 
 #include "BatchExecutor.h"
 #include "QueryExecutor.h"
+#include "Server.h"
+#include "Statistics.h"
 #include "vedas_host.h"
 
 struct vdq_engine {
     std::unique_ptr<VedasContext> ctx;
     std::unique_ptr<VedasStorage> storage;
+    std::unique_ptr<Statistics> stats;  // built on first use
+    Statistics &statistics() {
+        if (!stats) stats.reset(new Statistics(storage.get()));
+        return *stats;
+    }
 };
 
 struct vdq_result {
@@ -253,14 +260,10 @@ int vdq_batch(vdq_engine *e, const char *const *texts, int n, vdq_result **outs,
         for (int i = 0; i < n; ++i) {
             pqs.emplace_back(new ParsedQuery());
             parse_query(texts[i], *pqs.back());
-            bq[i].patterns = std::move(pqs.back()->pats);
+            bq[i].patterns = &pqs.back()->pats;
             bq[i].selected = pqs.back()->selected;
             bq[i].root = pqs.back()->root;
         }
-        // plan nodes point at the patterns: re-point them at the moved copies
-        for (int i = 0; i < n; ++i)
-            for (auto &node : pqs[i]->arena.nodes)
-                if (node->tp) node->tp = &bq[i].patterns[node->tp->getId()];
         std::vector<SparqlResult> results;
         BatchStats st;
         BatchExecutor bx(e->storage.get(), e->ctx.get());
@@ -282,6 +285,73 @@ int vdq_batch(vdq_engine *e, const char *const *texts, int n, vdq_result **outs,
             stats->elapsed_ms = std::chrono::duration<double, std::milli>(
                 std::chrono::steady_clock::now() - t0).count();
         }
+    });
+}
+
+int vdq_serve(vdq_engine *e, const char *const *texts, const double *arrival_ms, int n,
+              const vdq_serve_cfg_t *cfg, vdq_serve_rec_t *recs, vdq_result **outs) {
+    if (!e || n < 0 || !cfg || (n && (!texts || !arrival_ms || !recs)))
+        return fail(VD_EINVAL, "null argument");
+    if (!e->storage) return fail(VD_EINVAL, "engine has no store loaded");
+    if (outs)
+        for (int i = 0; i < n; ++i) outs[i] = nullptr;
+    return guarded([&] {
+        std::vector<std::unique_ptr<ParsedQuery>> pqs;
+        std::vector<ServeRequest> reqs(n);
+        for (int i = 0; i < n; ++i) {
+            pqs.emplace_back(new ParsedQuery());
+            parse_query(texts[i], *pqs.back());
+            reqs[i].patterns = pqs.back()->pats;
+            reqs[i].selected = pqs.back()->selected;
+            reqs[i].root = pqs.back()->root;
+            reqs[i].arrival_ms = arrival_ms[i];
+        }
+        // plan nodes of request i must point at reqs[i].patterns
+        for (int i = 0; i < n; ++i)
+            for (auto &node : pqs[i]->arena.nodes)
+                if (node->tp) node->tp = &reqs[i].patterns[node->tp->getId()];
+        ServeConfig sc;
+        sc.slots = cfg->slots;
+        sc.max_batch = cfg->max_batch;
+        sc.mem_budget = cfg->mem_budget;
+        sc.plain = cfg->plain != 0;
+        sc.keep_results = outs != nullptr;
+        Statistics &st = e->statistics();
+        Server srv(e->storage.get(), &st, sc);
+        std::vector<ServeRecord> rr;
+        std::vector<SparqlResult> results;
+        srv.run(reqs, rr, outs ? &results : nullptr);
+        for (int i = 0; i < n; ++i) {
+            recs[i].arrival_ms = rr[i].arrival_ms;
+            recs[i].start_ms = rr[i].start_ms;
+            recs[i].done_ms = rr[i].done_ms;
+            recs[i].est_bytes = rr[i].est_bytes;
+            recs[i].rows = rr[i].rows;
+            recs[i].batch = rr[i].batch;
+            recs[i].slot = rr[i].slot;
+        }
+        if (outs) {
+            std::vector<std::unique_ptr<vdq_result>> rs;
+            for (int i = 0; i < n; ++i) {
+                rs.emplace_back(new vdq_result());
+                rs.back()->result.setResult(results[i].release());
+                fill_stats(rs.back().get(), nullptr);
+            }
+            for (int i = 0; i < n; ++i) outs[i] = rs[i].release();
+        }
+    });
+}
+
+int vdq_estimate(vdq_engine *e, const char *text, double *rows, double *peak_bytes) {
+    if (!e || !text) return fail(VD_EINVAL, "null argument");
+    if (!e->storage) return fail(VD_EINVAL, "engine has no store loaded");
+    return guarded([&] {
+        ParsedQuery pq;
+        parse_query(text, pq);
+        PlanEstimator est(e->statistics());
+        PlanEstimate r = est.estimate(pq.pats, pq.root);
+        if (rows) *rows = r.rows;
+        if (peak_bytes) *peak_bytes = r.peak_bytes;
     });
 }
 
