@@ -312,7 +312,7 @@ def test_plans_cfg0_store(ctx, store, shape):
     elif shape == "chain4":
         pats, node = plans.chain_plan([1, 1, 1, 2])
     else:
-        pats, node = plans.snowflake_plan([8, 9], [10, 11])
+        pats, node = plans.snowflake_plan([1, 2], [1, 3])
     gex = plans.GpuExecutor(ctx, idx)
     rel = gex.run(node)
     gnames, grows = O.canonical(rel.headers, gex.columns(rel))

@@ -90,10 +90,26 @@ struct JPairDev {
     int ncl, ncr;
 };
 
+// Right-row range of every bounds tile, all tiles in parallel.
+__global__ __launch_bounds__(256) void k_join_ranges_multi(const JPairDev *P, int np,
+                                                           const uint64_t *tbase, uint64_t ntiles,
+                                                           uint64_t *ranges) {
+    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= ntiles) return;
+    const int j = find_seg(tbase, np, b);
+    const JPairDev &D = P[j];
+    const uint32_t *lk = D.lc[0], *rk = D.rc[0];
+    const uint64_t base = (b - tbase[j]) * (uint64_t)MB_TILE;
+    const uint64_t last = min(base + MB_TILE, D.nl) - 1;
+    const uint64_t lo = dev::lower_bound(rk, 0, D.nr, lk[base]);
+    ranges[2 * b] = lo;
+    ranges[2 * b + 1] = dev::upper_bound(rk, lo, D.nr, lk[last]);
+}
+
 __global__ __launch_bounds__(MB_BLOCK) void k_join_bounds_multi(
-    const JPairDev *P, int np, const uint64_t *tbase, uint32_t *lower, uint32_t *cnt) {
+    const JPairDev *P, int np, const uint64_t *tbase, const uint64_t *ranges, uint32_t *lower,
+    uint32_t *cnt) {
     __shared__ uint32_t s_r[MJ_RCAP];
-    __shared__ uint64_t s_range[2];
     __shared__ int s_j;
     if (threadIdx.x == 0) s_j = find_seg(tbase, np, blockIdx.x);
     __syncthreads();
@@ -101,15 +117,8 @@ __global__ __launch_bounds__(MB_BLOCK) void k_join_bounds_multi(
     const JPairDev &D = P[j];
     const uint32_t *lk = D.lc[0], *rk = D.rc[0];
     const uint64_t base = (blockIdx.x - tbase[j]) * (uint64_t)MB_TILE;
-    const uint64_t nl = D.nl, nr = D.nr;
-    const uint64_t last = min(base + MB_TILE, nl) - 1;
-    if (threadIdx.x == 0) {
-        const uint64_t lo = dev::lower_bound(rk, 0, nr, lk[base]);
-        s_range[0] = lo;
-        s_range[1] = dev::upper_bound(rk, lo, nr, lk[last]);
-    }
-    __syncthreads();
-    const uint64_t rlo = s_range[0], rhi = s_range[1], m = rhi - rlo;
+    const uint64_t nl = D.nl;
+    const uint64_t rlo = ranges[2 * blockIdx.x], rhi = ranges[2 * blockIdx.x + 1], m = rhi - rlo;
     const bool staged = m <= (uint64_t)MJ_RCAP;
     if (staged)
         for (uint64_t t = threadIdx.x; t < m; t += MB_BLOCK) s_r[t] = rk[rlo + t];
@@ -425,9 +434,13 @@ void join_multi(vd_ctx *ctx, const vd_rel *const *L, const vd_rel *const *R, int
         DBuf<uint64_t> dtb = to_device(ctx, tb);
         double bbytes = 0;
         for (int j = 0; j < np; ++j) bbytes += 4.0 * (hp[j].nl + hp[j].nr) + 8.0 * hp[j].nl;
+        DBuf<uint64_t> ranges(ctx, 2 * tb[np]);
+        launch(ctx, "join_ranges", 0.0, dim3((unsigned)ceil_div(tb[np], 256)), dim3(256), 0,
+               k_join_ranges_multi, (const JPairDev *)dp.p, np, (const uint64_t *)dtb.p, tb[np],
+               ranges.p);
         launch(ctx, "join_bounds", bbytes, dim3((unsigned)tb[np]), dim3(MB_BLOCK), 0,
                k_join_bounds_multi, (const JPairDev *)dp.p, np, (const uint64_t *)dtb.p,
-               lower.p, cnt.p);
+               (const uint64_t *)ranges.p, lower.p, cnt.p);
         scan_u32_to_u64(ctx, cnt.p, off.p, nl, off.p + nl);
         launch(ctx, "join_totals", 0.0, dim3((unsigned)ceil_div(np, 256)), dim3(256), 0,
                k_pair_totals, (const JPairDev *)dp.p, np, (const uint64_t *)off.p, dtot.p);

@@ -23,6 +23,7 @@ static size_t size_class(size_t bytes) {
 }
 
 static void release_cached(vd_ctx *ctx) {
+    std::lock_guard<std::recursive_mutex> lk(ctx->alloc_mu);
     if (ctx->free_blocks.empty()) return;
     VD_HIP(hipStreamSynchronize(ctx->stream));
     for (auto &kv : ctx->free_blocks) (void)hipFree(kv.second);
@@ -31,6 +32,7 @@ static void release_cached(vd_ctx *ctx) {
 }
 
 void *dalloc(vd_ctx *ctx, size_t bytes) {
+    std::lock_guard<std::recursive_mutex> lk(ctx->alloc_mu);
     const size_t c = size_class(bytes);
     auto it = ctx->free_blocks.find(c);
     void *p = nullptr;
@@ -39,6 +41,8 @@ void *dalloc(vd_ctx *ctx, size_t bytes) {
         ctx->free_blocks.erase(it);
         ctx->bytes_cached -= c;
     } else {
+        // the calling host thread may not have selected the context's device
+        VD_HIP(hipSetDevice(ctx->device));
         hipError_t e = hipMalloc(&p, c);
         if (e != hipSuccess) {
             (void)hipGetLastError();
@@ -58,6 +62,7 @@ void *dalloc(vd_ctx *ctx, size_t bytes) {
 
 void dfree(vd_ctx *ctx, void *p) {
     if (!p) return;
+    std::lock_guard<std::recursive_mutex> lk(ctx->alloc_mu);
     auto it = ctx->live_blocks.find(p);
     if (it == ctx->live_blocks.end()) return;  // not ours: ignore
     const size_t c = it->second;

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -77,6 +78,9 @@ struct vd_ctx {
     std::multimap<size_t, void *> free_blocks;
     std::unordered_map<void *, size_t> live_blocks;
     uint64_t bytes_live = 0, bytes_cached = 0;
+    // Guards the pool: a relation may be freed from another host thread
+    // than the one issuing work on the context.
+    std::recursive_mutex alloc_mu;
 
     // Pinned host staging for small read-backs.
     uint64_t *h_pinned = nullptr;  // 64 words

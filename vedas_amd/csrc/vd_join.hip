@@ -76,24 +76,31 @@ __device__ __forceinline__ uint64_t g_upper(const KeyCols &k, uint64_t lo,
     return lo;
 }
 
+// Right-row range [lo, hi) each bounds block can touch: one thread per
+// block, all searches in parallel (a serial search inside every block
+// would put ~2 log2(|R|) dependent global loads in front of its work).
+template <int NK>
+__global__ __launch_bounds__(256) void k_join_ranges(KeyCols L, uint64_t nl, KeyCols R,
+                                                     uint64_t nr, uint64_t nblocks,
+                                                     uint64_t *__restrict__ ranges) {
+    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblocks) return;
+    const uint64_t base = b * JB_TILE;
+    const uint64_t last = min(base + JB_TILE, nl) - 1;
+    const uint64_t lo = g_lower<NK>(R, 0, nr, KeyT<NK>::at(L, base));
+    ranges[2 * b] = lo;
+    ranges[2 * b + 1] = g_upper<NK>(R, lo, nr, KeyT<NK>::at(L, last));
+}
+
 template <int NK>
 __global__ __launch_bounds__(JB_BLOCK) void k_join_bounds(
     KeyCols L, uint64_t nl, KeyCols R, uint64_t nr, int rcap,
-    uint32_t *__restrict__ lower, uint32_t *__restrict__ cnt) {
+    const uint64_t *__restrict__ ranges, uint32_t *__restrict__ lower,
+    uint32_t *__restrict__ cnt) {
     using T = typename KeyT<NK>::T;
     __shared__ T s_r[JB_RCAP_BYTES / sizeof(T)];
-    __shared__ uint64_t s_range[2];
     const uint64_t base = (uint64_t)blockIdx.x * JB_TILE;
-    const uint64_t last = min(base + JB_TILE, nl) - 1;
-    if (threadIdx.x == 0) {
-        const T kf = KeyT<NK>::at(L, base), kl = KeyT<NK>::at(L, last);
-        const uint64_t lo = g_lower<NK>(R, 0, nr, kf);
-        const uint64_t hi = g_upper<NK>(R, lo, nr, kl);
-        s_range[0] = lo;
-        s_range[1] = hi;
-    }
-    __syncthreads();
-    const uint64_t rlo = s_range[0], rhi = s_range[1];
+    const uint64_t rlo = ranges[2 * blockIdx.x], rhi = ranges[2 * blockIdx.x + 1];
     const uint64_t m = rhi - rlo;
     const bool staged = m <= (uint64_t)rcap && m <= JB_RCAP_BYTES / sizeof(T);
     if (staged) {
@@ -221,9 +228,12 @@ vd_rel *join_impl(vd_ctx *ctx, const vd_rel *L, const vd_rel *R) {
     DBuf<uint64_t> off(ctx, nl + 1);
     const uint64_t jb_tiles = ceil_div(nl, JB_TILE);
     const int rcap = std::min<int>(ctx->join_rcap, JB_RCAP_BYTES / (NK == 2 ? 8 : 4));
+    DBuf<uint64_t> ranges(ctx, 2 * jb_tiles);
+    launch(ctx, "join_ranges", 0.0, dim3((unsigned)ceil_div(jb_tiles, 256)), dim3(256), 0,
+           k_join_ranges<NK>, lk, nl, rk, nr, jb_tiles, ranges.p);
     launch(ctx, "join_bounds", 4.0 * NK * (double)(nl + nr) + 8.0 * nl,
            dim3((unsigned)jb_tiles), dim3(JB_BLOCK), 0, k_join_bounds<NK>, lk,
-           nl, rk, nr, rcap, lower.p, cnt.p);
+           nl, rk, nr, rcap, (const uint64_t *)ranges.p, lower.p, cnt.p);
     scan_u32_to_u64(ctx, cnt.p, off.p, nl, off.p + nl);
     const uint64_t total = read_u64(ctx, off.p + nl);
     VD_REQUIRE(total < (1ull << 40), VD_ERANGE, "join output too large");

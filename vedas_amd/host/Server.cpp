@@ -10,8 +10,9 @@
 #include "BatchExecutor.h"
 #include "QueryExecutor.h"
 
-Server::Server(VedasStorage *s, const Statistics *st, const ServeConfig &c)
-    : storage(s), stats(st), cfg(c) {}
+Server::Server(VedasStorage *s, const Statistics *st, const ServeConfig &c,
+               std::vector<VedasContext *> slots)
+    : storage(s), stats(st), cfg(c), slot_ctx(std::move(slots)) {}
 
 namespace {
 
@@ -37,6 +38,7 @@ void Server::run(std::vector<ServeRequest> &reqs, std::vector<ServeRecord> &recs
     if (n == 0) return;
     const int nslots = cfg.plain ? 1 : std::max(1, cfg.slots);
     const int max_batch = cfg.plain ? 1 : std::max(1, cfg.max_batch);
+    if ((int)slot_ctx.size() < nslots) throw VedasError(VD_EINVAL, "too few slot contexts");
 
     // Memory estimate of every request (admission).
     PlanEstimator est(*stats);
@@ -64,7 +66,7 @@ void Server::run(std::vector<ServeRequest> &reqs, std::vector<ServeRecord> &recs
     };
 
     auto worker = [&](int si) {
-        VedasContext ctx(storage->context()->device());
+        VedasContext &ctx = *slot_ctx[si];
         for (;;) {
             std::vector<int> batch;
             {

@@ -48,7 +48,11 @@ struct ServeRecord {
 
 class Server {
   public:
-    Server(VedasStorage *storage, const Statistics *stats, const ServeConfig &cfg);
+    // slot_ctx: one context per worker slot (at least cfg.slots, or one in
+    // plain mode); results live in these contexts, so they must outlive the
+    // results.
+    Server(VedasStorage *storage, const Statistics *stats, const ServeConfig &cfg,
+           std::vector<VedasContext *> slot_ctx);
     // Serves all requests; recs[i] / results[i] describe request i.
     void run(std::vector<ServeRequest> &reqs, std::vector<ServeRecord> &recs,
              std::vector<SparqlResult> *results);
@@ -57,6 +61,7 @@ class Server {
     VedasStorage *storage;
     const Statistics *stats;
     ServeConfig cfg;
+    std::vector<VedasContext *> slot_ctx;
 };
 
 #endif

@@ -16,9 +16,18 @@ struct vdq_engine {
     std::unique_ptr<VedasContext> ctx;
     std::unique_ptr<VedasStorage> storage;
     std::unique_ptr<Statistics> stats;  // built on first use
+    // Serving slots: contexts (streams + pools) that hold served results,
+    // kept for the engine's lifetime.
+    std::vector<std::unique_ptr<VedasContext>> slot_ctx;
     Statistics &statistics() {
         if (!stats) stats.reset(new Statistics(storage.get()));
         return *stats;
+    }
+    std::vector<VedasContext *> slots(int n) {
+        while ((int)slot_ctx.size() < n) slot_ctx.emplace_back(new VedasContext(ctx->device()));
+        std::vector<VedasContext *> v;
+        for (int i = 0; i < n; ++i) v.push_back(slot_ctx[i].get());
+        return v;
     }
 };
 
@@ -202,6 +211,7 @@ int vdq_engine_load(vdq_engine *e, const uint32_t *s, const uint32_t *p, const u
                 e->ctx->check(vd_dev_alloc(c, (n ? n : 1) * 4, &d[i]), "vd_dev_alloc");
                 if (n) e->ctx->check(vd_copy_h2d(c, d[i], h[i], n * 4), "vd_copy_h2d");
             }
+            e->stats.reset();
             e->storage.reset();
             e->storage.reset(new VedasStorage(e->ctx.get(), (uint32_t *)d[0], (uint32_t *)d[1],
                                               (uint32_t *)d[2], n, perm_mask));
@@ -217,6 +227,7 @@ int vdq_engine_load_device(vdq_engine *e, const uint32_t *d_s, const uint32_t *d
                            const uint32_t *d_o, uint64_t n, uint32_t perm_mask) {
     if (!e) return fail(VD_EINVAL, "null engine");
     return guarded([&] {
+        e->stats.reset();
         e->storage.reset();
         e->storage.reset(new VedasStorage(e->ctx.get(), d_s, d_p, d_o, n, perm_mask));
     });
@@ -317,7 +328,7 @@ int vdq_serve(vdq_engine *e, const char *const *texts, const double *arrival_ms,
         sc.plain = cfg->plain != 0;
         sc.keep_results = outs != nullptr;
         Statistics &st = e->statistics();
-        Server srv(e->storage.get(), &st, sc);
+        Server srv(e->storage.get(), &st, sc, e->slots(sc.plain ? 1 : std::max(1, sc.slots)));
         std::vector<ServeRecord> rr;
         std::vector<SparqlResult> results;
         srv.run(reqs, rr, outs ? &results : nullptr);
