@@ -155,7 +155,13 @@ def batch_bench(eng, texts, reps=2, budget=96 << 30):
         plain_t.append(time.perf_counter() - t0)
     batch_t = []
     rounds = calls = join_rows = 0
-    for _ in range(reps):
+    from vedas_amd import _native
+    ctx = _native.Ctx.wrap(eng.ctx_handle())
+    prof = None
+    for rep in range(reps + 1):
+        if rep == reps:  # one more run with per-kernel timing, not counted
+            ctx.prof(True)
+            ctx.prof_reset()
         t0 = time.perf_counter()
         rounds = calls = join_rows = 0
         for g in groups:
@@ -165,7 +171,11 @@ def batch_bench(eng, texts, reps=2, budget=96 << 30):
             join_rows += st["join_rows"]
             for r in res:
                 r.free()
-        batch_t.append(time.perf_counter() - t0)
+        if rep < reps:
+            batch_t.append(time.perf_counter() - t0)
+        else:
+            prof = kernel_table(ctx.prof_stats())
+            ctx.prof(False)
     # equality of results, checked outside the timed runs
     same = 0
     for g in groups:
@@ -179,7 +189,7 @@ def batch_bench(eng, texts, reps=2, budget=96 << 30):
     return {"queries": len(texts), "batches": len(groups), "plain_ms": pt * 1e3,
             "batched_ms": bt * 1e3, "plain_qps": len(texts) / pt, "batched_qps": len(texts) / bt,
             "speedup": pt / bt, "rounds": rounds, "calls": calls, "join_rows": join_rows,
-            "results_equal": same}
+            "results_equal": same, "kernels": prof}
 
 
 def serve_bench(eng, texts, qps, seed=1, slots=4, max_batch=64, budget=96 << 30):
@@ -240,7 +250,8 @@ def main():
     ap.add_argument("--queries", default="star4,chain4,snowflake6")
     ap.add_argument("--selects", type=int, default=10_000)
     ap.add_argument("--mix", type=int, default=256, help="queries in the batched mix (0: skip)")
-    ap.add_argument("--serve-qps", type=float, default=2000.0, help="0: skip serving")
+    ap.add_argument("--serve-qps", type=float, nargs="*", default=[2000.0, 20000.0],
+                    help="offered loads of the open-loop arrival stream (none: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -336,12 +347,15 @@ def main():
             r.free()
         batch = batch_bench(eng, mtexts)
         log(f"[bench] mix: plain {batch['plain_qps']:.0f} q/s, batched {batch['batched_qps']:.0f} q/s")
-        if args.serve_qps > 0:
-            log("[bench] serving")
-            serve = serve_bench(eng, mtexts, args.serve_qps)
-            log(f"[bench] serving: plain {serve['plain']['qps']:.0f} q/s p99 "
-                f"{serve['plain']['p99_ms']:.1f} ms; served {serve['served']['qps']:.0f} q/s p99 "
-                f"{serve['served']['p99_ms']:.1f} ms")
+        if args.serve_qps:
+            serve = []
+            for q in args.serve_qps:
+                log(f"[bench] serving at {q:.0f} offered q/s")
+                sv = serve_bench(eng, mtexts, q)
+                serve.append(sv)
+                log(f"[bench]   plain {sv['plain']['qps']:.0f} q/s p50 {sv['plain']['p50_ms']:.1f} "
+                    f"p99 {sv['plain']['p99_ms']:.1f} ms; served {sv['served']['qps']:.0f} q/s "
+                    f"p50 {sv['served']['p50_ms']:.1f} p99 {sv['served']['p99_ms']:.1f} ms")
 
     cpu = None
     if not args.no_cpu:

@@ -169,6 +169,7 @@ __global__ __launch_bounds__(MJ_BLOCK) void k_join_expand_multi(
     constexpr int ITEMS = MJ_NV / MJ_BLOCK;
     __shared__ uint64_t s_off[MJ_NV + 1];
     __shared__ uint32_t s_low[MJ_NV + 1];
+    __shared__ uint16_t s_pair[MJ_NV + 1];  // pair of each staged left row
     const uint64_t blk = blockIdx.x;
     const uint64_t d0 = blk * MJ_NV;
     const uint64_t d1 = min(d0 + MJ_NV, nl + total);
@@ -180,6 +181,7 @@ __global__ __launch_bounds__(MJ_BLOCK) void k_join_expand_multi(
     for (int t = threadIdx.x; t < m; t += MJ_BLOCK) {
         s_off[t] = off[sa + t];
         s_low[t] = lower[sa + t];
+        s_pair[t] = (uint16_t)find_seg(lbase, np, sa + t);
     }
     __syncthreads();
     const int nout = (int)(b1 - b0);
@@ -195,8 +197,7 @@ __global__ __launch_bounds__(MJ_BLOCK) void k_join_expand_multi(
         }
         const int k = lo - 1;
         const uint64_t grow = sa + k;  // global left row
-        const int j = find_seg(lbase, np, grow);
-        const JPairDev &D = P[j];
+        const JPairDev &D = P[s_pair[k]];
         const uint64_t lrow = grow - D.lbase;
         const uint64_t rrow = s_low[k] + (o - s_off[k]);
         const uint64_t loc = o - D.obase;
@@ -326,23 +327,17 @@ __global__ __launch_bounds__(MB_BLOCK) void k_unique_scatter_multi(const SortRel
     const int j = s_j;
     const SortRelDev &D = S[j];
     const uint64_t base = (blockIdx.x - tbase[j]) * (uint64_t)MB_TILE;
-    bool keep[MB_ITEMS];
-    uint32_t c = 0;
-#pragma unroll
+    // rounds of MB_BLOCK consecutive rows, compacted with a block scan each
+    uint64_t o = off[blockIdx.x] - off[tbase[j]];
     for (int it = 0; it < MB_ITEMS; ++it) {
-        const uint64_t i = base + (uint64_t)threadIdx.x * MB_ITEMS + it;
-        keep[it] = i < D.n && differs(D, i);
-        c += keep[it];
-    }
-    uint32_t tot;
-    uint64_t o = off[blockIdx.x] - off[tbase[j]] +
-                 dev::block_excl_scan<MB_BLOCK>(c, s_tmp, &tot);
-#pragma unroll
-    for (int it = 0; it < MB_ITEMS; ++it) {
-        if (!keep[it]) continue;
-        const uint64_t i = base + (uint64_t)threadIdx.x * MB_ITEMS + it;
-        for (int cc = 0; cc < D.ncols; ++cc) D.out[cc][o] = D.in[cc][i];
-        ++o;
+        const uint64_t i = base + (uint64_t)it * MB_BLOCK + threadIdx.x;
+        const uint32_t keep = (i < D.n && differs(D, i)) ? 1u : 0u;
+        uint32_t tot;
+        const uint32_t ex = dev::block_excl_scan<MB_BLOCK>(keep, s_tmp, &tot);
+        if (keep)
+            for (int cc = 0; cc < D.ncols; ++cc) D.out[cc][o + ex] = D.in[cc][i];
+        o += tot;
+        if (base + (uint64_t)(it + 1) * MB_BLOCK >= D.n) break;  // block-uniform
     }
 }
 
@@ -401,6 +396,7 @@ void select_multi(vd_ctx *ctx, const vd_index *idx, const vd_select_job_t *jobs,
 
 void join_multi(vd_ctx *ctx, const vd_rel *const *L, const vd_rel *const *R, int np,
                 vd_rel **outs) {
+    VD_REQUIRE(np <= 65535, VD_ERANGE, "at most 65535 joins per batched call");
     std::vector<JPairDev> hp(np);
     std::vector<uint64_t> lrows(np), lbase(np + 1, 0);
     for (int j = 0; j < np; ++j) {

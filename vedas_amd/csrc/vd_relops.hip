@@ -102,26 +102,23 @@ __global__ __launch_bounds__(U_BLOCK) void k_unique_count(Cols cols, uint64_t n,
     if (threadIdx.x == 0) cnt[blockIdx.x] = t;
 }
 
+// The tile is processed in U_ITEMS rounds of U_BLOCK consecutive rows
+// (row = base + round * U_BLOCK + thread), each round compacted with a block
+// scan, so loads and stores stay coalesced and row order is kept.
 __global__ __launch_bounds__(U_BLOCK) void k_unique_scatter(
     Cols cols, uint64_t n, const uint64_t *__restrict__ off) {
     __shared__ uint32_t s_tmp[U_BLOCK / 64];
     const uint64_t base = (uint64_t)blockIdx.x * U_TILE;
-    bool keep[U_ITEMS];
-    uint32_t c = 0;
-#pragma unroll
+    uint64_t o = off[blockIdx.x];
     for (int j = 0; j < U_ITEMS; ++j) {
-        const uint64_t i = base + (uint64_t)threadIdx.x * U_ITEMS + j;
-        keep[j] = i < n && row_differs(cols, i);
-        c += keep[j];
-    }
-    uint32_t tot;
-    uint64_t o = off[blockIdx.x] + dev::block_excl_scan<U_BLOCK>(c, s_tmp, &tot);
-#pragma unroll
-    for (int j = 0; j < U_ITEMS; ++j) {
-        if (!keep[j]) continue;
-        const uint64_t i = base + (uint64_t)threadIdx.x * U_ITEMS + j;
-        for (int cc = 0; cc < cols.n; ++cc) cols.out[cc][o] = cols.in[cc][i];
-        ++o;
+        const uint64_t i = base + (uint64_t)j * U_BLOCK + threadIdx.x;
+        const uint32_t keep = (i < n && row_differs(cols, i)) ? 1u : 0u;
+        uint32_t tot;
+        const uint32_t ex = dev::block_excl_scan<U_BLOCK>(keep, s_tmp, &tot);
+        if (keep)
+            for (int cc = 0; cc < cols.n; ++cc) cols.out[cc][o + ex] = cols.in[cc][i];
+        o += tot;
+        if (base + (uint64_t)(j + 1) * U_BLOCK >= n) break;  // block-uniform
     }
 }
 
