@@ -73,7 +73,8 @@ void *vd_ctx_stream(vd_ctx *ctx);             /* the hipStream_t          */
 int  vd_ctx_trim(vd_ctx *ctx);                /* release cached blocks    */
 /* Device memory in use / cached by the context's allocator, in bytes. */
 int  vd_ctx_mem(vd_ctx *ctx, uint64_t *in_use, uint64_t *cached);
-/* Tunables: "radix_items" (keys per thread in a radix tile: 8/16/24),
+/* Tunables: "radix_items" (keys per thread in a radix tile: 16 = 512
+ * threads x 16 keys, the default; 8 = 512 x 8; 24 = 256 x 24, keys only),
  * "join_nv" (merge-path diagonal per join expansion block), "join_rcap"
  * (right keys staged in LDS per bounds block).  Returns VD_EINVAL for an
  * unknown key or an unsupported value. */
@@ -131,9 +132,20 @@ uint64_t vd_rel_nrows(const vd_rel *rel);
 uint32_t *vd_rel_col(vd_rel *rel, int c);
 int  vd_rel_swap_cols(vd_rel *rel, int i, int j);   /* swapColumn :715  */
 int  vd_rel_drop_col(vd_ctx *ctx, vd_rel *rel, int i); /* removeColumn :729 */
+/* No counterpart in the reference, whose selects always copy.  A column of
+ * a relation returned by an unfiltered vd_select / vd_select_multi is a
+ * read-only view of the index permutation (no copy); it keeps the index
+ * arrays alive after vd_index_free until the relation is destroyed.  The
+ * operators never write into a view (they replace the column); a caller
+ * that wants to write through vd_rel_col first calls vd_rel_materialize,
+ * which gives every view column a private copy.  vd_rel_is_view(rel, c) is
+ * 1 for a view column, else 0. */
+int  vd_rel_materialize(vd_ctx *ctx, vd_rel *rel);
+int  vd_rel_is_view(const vd_rel *rel, int c);
 
 /* ---- select (SelectQueryJob) -------------------------------------------- */
-/* Materialise rows [begin, end) of a permutation.  cols: 1 = Z only (the
+/* Rows [begin, end) of a permutation (a view when nothing filters them,
+ * a compacted copy otherwise; see vd_rel_materialize).  cols: 1 = Z only (the
  * 1-variable select, SelectQueryJob.cu:92-121), 2 = Y only,
  * 3 = Y then Z (2-variable select, :122-264).  With clip, rows whose Y is
  * outside [lo, hi] are dropped (variable bound, :130-141); rows whose Y lies
@@ -191,7 +203,8 @@ typedef struct {
     uint32_t k2;     /* Y constant, or VD_UNBOUND                           */
     int      cols;   /* 1 = Z only, 3 = (Y, Z)   (as vd_select)            */
 } vd_select_job_t;
-/* outs[i] = vd_select of job i over its lookup range (no clip/skip). */
+/* outs[i] = vd_select of job i over its lookup range (no clip/skip): a
+ * view of the permutation's rows. */
 int  vd_select_multi(vd_ctx *ctx, const vd_index *idx,
                      const vd_select_job_t *jobs, int njobs, vd_rel **outs);
 /* outs[i] = vd_join(L[i], R[i], 1). */

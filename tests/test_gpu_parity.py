@@ -160,6 +160,45 @@ def test_select_modes(ctx, store):
                 assert all(np.array_equal(g, w) for g, w in zip(got, want)), (pred, clip, skip)
 
 
+def test_select_views_and_lifetime(ctx):
+    """Unfiltered selects read the permutation in place: operators replace
+    view columns instead of writing into them, and a view keeps the index
+    arrays alive after the index is freed."""
+    from vedas_amd._native import Index
+    s, p, o = rand_triples(50_000, 11)
+    idx = Index(ctx, s, p, o, ["PSO", "POS"])
+    ix = O.build_perm(s, p, o, "PSO")
+    b, e = O.lookup(ix, 3)
+    assert e - b > 100
+    want = O.select(ix, b, e, 3)
+    r1 = idx.select("PSO", b, e, 3)
+    assert r1.is_view(0) and r1.is_view(1)
+    r1.swap_cols(0, 1)
+    r1.sort_col0()
+    assert not r1.is_view(0) and not r1.is_view(1)
+    for g, w in zip(r1.columns(), O.sort_col0([want[1], want[0]])):
+        assert np.array_equal(g, w)
+    r2 = idx.select("PSO", b, e, 3)
+    r2.drop_col(0)
+    assert r2.ncols == 1 and r2.is_view(0)
+    r3 = idx.select("PSO", b, e, 1)
+    r3.materialize()
+    assert not r3.is_view(0)
+    r4 = idx.select("PSO", b, e, 3)  # the sort above left the index as it was
+    for g, w in zip(r4.columns(), want):
+        assert np.array_equal(g, w)
+    r5 = idx.select("PSO", b, e, 2)
+    r5.unique_adjacent()
+    assert np.array_equal(r5.columns()[0], O.unique_adjacent([want[0]])[0])
+    idx.free()
+    for g, w in zip(r4.columns(), want):  # storage kept alive by the views
+        assert np.array_equal(g, w)
+    assert np.array_equal(r2.columns()[0], want[1])
+    assert np.array_equal(r3.columns()[0], O.select(ix, b, e, 1)[0])
+    for r in (r1, r2, r3, r4, r5):
+        r.free()
+
+
 def test_select_batch(ctx, store):
     import ctypes as C
     from vedas_amd._native import PERMS, VD_UNBOUND
@@ -272,7 +311,9 @@ def test_join_empty_sides(ctx):
 
 
 @pytest.mark.parametrize("n,ncols,keymax", [(0, 2, 5), (1, 3, 5), (1000, 1, 50), (70_001, 3, 1000),
-                                            (300_000, 5, 2**31), (4096, 2, 1)])
+                                            (300_000, 5, 2**31), (4096, 2, 1),
+                                            (100_000, 2, 2**32 - 1), (50_000, 4, 2**20),
+                                            (3000, 16, 7)])
 def test_sorts_and_unique(ctx, n, ncols, keymax):
     from vedas_amd._native import Rel
     rng = np.random.default_rng(n + ncols)

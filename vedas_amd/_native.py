@@ -83,6 +83,8 @@ EXPORTS = {
     "vd_rel_col": (C.c_void_p, [C.c_void_p, C.c_int]),
     "vd_rel_swap_cols": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "vd_rel_drop_col": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "vd_rel_materialize": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "vd_rel_is_view": (C.c_int, [C.c_void_p, C.c_int]),
     "vd_select": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_uint64, C.c_uint64,
                             C.c_int, C.c_int, C.c_uint32, C.c_uint32,
                             C.POINTER(Interval), C.c_int, C.c_int,
@@ -276,6 +278,14 @@ class Rel:
 
     def columns(self) -> list:
         return [self.ctx.download(self.col_ptr(i), self.nrows) for i in range(self.ncols)]
+
+    def is_view(self, i: int) -> bool:
+        """Column i reads index storage in place (read-only)."""
+        return bool(self.ctx.L.vd_rel_is_view(self.h, i))
+
+    def materialize(self) -> None:
+        """Give every view column a private copy (before writing into it)."""
+        self.ctx.check(self.ctx.L.vd_rel_materialize(self.ctx.h, self.h))
 
     def swap_cols(self, i: int, j: int) -> None:
         self.ctx.check(self.ctx.L.vd_rel_swap_cols(self.h, i, j))

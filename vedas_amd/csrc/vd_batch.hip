@@ -43,40 +43,19 @@ unsigned grid_cap(uint64_t n) {
 // ---- select ------------------------------------------------------------------
 struct SelJobDev {
     PermDev pd;
-    const uint32_t *y, *z;
     uint32_t k1, k2;
-    int cols;
-    uint32_t *o0, *o1;
 };
 
+// range[2j], range[2j+1] = the rows [begin, end) matching job j.
 __global__ __launch_bounds__(256) void k_sel_lookup_multi(const SelJobDev *jobs, int nj,
-                                                          uint64_t *begin, uint64_t *sizes) {
+                                                          uint64_t *range) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nj) return;
     const SelJobDev &J = jobs[j];
     uint64_t b, e;
     lookup_one(J.pd, J.k1, J.k2, J.k2 != VD_UNBOUND, &b, &e);
-    begin[j] = b;
-    sizes[j] = e - b;
-}
-
-__global__ __launch_bounds__(256) void k_sel_gather_multi(const SelJobDev *jobs, int nj,
-                                                          const uint64_t *begin,
-                                                          const uint64_t *prefix,
-                                                          uint64_t total) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total; o += stride) {
-        const int j = find_seg(prefix, nj, o);
-        const SelJobDev &J = jobs[j];
-        const uint64_t loc = o - prefix[j];
-        const uint64_t src = begin[j] + loc;
-        if (J.cols == 1) {
-            J.o0[loc] = J.z[src];
-        } else {
-            J.o0[loc] = J.y[src];
-            J.o1[loc] = J.z[src];
-        }
-    }
+    range[2 * j] = b;
+    range[2 * j + 1] = e;
 }
 
 // ---- join --------------------------------------------------------------------
@@ -363,31 +342,26 @@ void select_multi(vd_ctx *ctx, const vd_index *idx, const vd_select_job_t *jobs,
         const vd_perm_arrays &pa = get_perm(idx, jobs[j].perm);
         VD_REQUIRE(jobs[j].cols == 1 || jobs[j].cols == 3, VD_EINVAL,
                    "batched select supports cols 1 and 3");
-        hj[j] = SelJobDev{perm_dev(pa), pa.y, pa.z, jobs[j].k1, jobs[j].k2, jobs[j].cols,
-                          nullptr, nullptr};
+        hj[j] = SelJobDev{perm_dev(pa), jobs[j].k1, jobs[j].k2};
     }
     DBuf<SelJobDev> dj = to_device(ctx, hj);
-    DBuf<uint64_t> begin(ctx, nj), sizes(ctx, nj), prefix(ctx, nj + 1);
+    DBuf<uint64_t> range(ctx, 2 * (uint64_t)nj);
     launch(ctx, "select_lookup", 24.0 * nj, dim3((unsigned)ceil_div(nj, 256)), dim3(256), 0,
-           k_sel_lookup_multi, (const SelJobDev *)dj.p, nj, begin.p, sizes.p);
-    scan_u64(ctx, sizes.p, prefix.p, nj, prefix.p + nj);
-    std::vector<uint64_t> hp(nj + 1);
-    download(ctx, hp.data(), prefix.p, (nj + 1) * sizeof(uint64_t));
+           k_sel_lookup_multi, (const SelJobDev *)dj.p, nj, range.p);
+    std::vector<uint64_t> hr(2 * (size_t)nj);
+    download(ctx, hr.data(), range.p, hr.size() * sizeof(uint64_t));
+    // Each result is the contiguous row range of its permutation: a view.
     for (int j = 0; j < nj; ++j) outs[j] = nullptr;
     try {
         for (int j = 0; j < nj; ++j) {
-            outs[j] = rel_new(ctx, jobs[j].cols == 3 ? 2 : 1, hp[j + 1] - hp[j]);
-            hj[j].o0 = outs[j]->cols[0];
-            hj[j].o1 = jobs[j].cols == 3 ? outs[j]->cols[1] : nullptr;
+            const vd_perm_arrays &pa = get_perm(idx, jobs[j].perm);
+            const uint64_t b = hr[2 * j], e = hr[2 * j + 1];
+            uint32_t *vc[2];
+            int nc = 0;
+            if (jobs[j].cols == 3) vc[nc++] = pa.y + b;
+            vc[nc++] = pa.z + b;
+            outs[j] = rel_view(nc, e - b, vc, idx->hold);
         }
-        upload(ctx, dj.p, hj.data(), nj * sizeof(SelJobDev));
-        const uint64_t total = hp[nj];
-        double bytes = 0;
-        for (int j = 0; j < nj; ++j) bytes += 8.0 * (hp[j + 1] - hp[j]) * (jobs[j].cols == 3 ? 2 : 1);
-        if (total)
-            launch(ctx, "select_gather", bytes, dim3(grid_cap(total)), dim3(256), 0,
-                   k_sel_gather_multi, (const SelJobDev *)dj.p, nj, (const uint64_t *)begin.p,
-                   (const uint64_t *)prefix.p, total);
     } catch (...) {
         for (int j = 0; j < nj; ++j) rel_free(ctx, outs[j]), outs[j] = nullptr;
         throw;
@@ -542,8 +516,7 @@ void sort_col0_range(vd_ctx *ctx, vd_rel *const *rels, int n) {
            (const uint64_t *)sorted, total, (const SortRelDev *)ds.p, rbits, cbits);
     for (int j = 0; j < n; ++j)
         for (int c = 0; c < hs[j].ncols; ++c) {
-            dfree(ctx, rels[j]->cols[c]);
-            rels[j]->cols[c] = hs[j].out[c];
+            rel_set_col(ctx, rels[j], c, hs[j].out[c]);
         }
 }
 
@@ -595,8 +568,7 @@ void unique_multi(vd_ctx *ctx, vd_rel *const *rels_in, int n_in) {
            (const uint64_t *)off.p);
     for (int j = 0; j < n; ++j) {
         for (int c = 0; c < hs[j].ncols; ++c) {
-            dfree(ctx, rels[j]->cols[c]);
-            rels[j]->cols[c] = hs[j].out[c];
+            rel_set_col(ctx, rels[j], c, hs[j].out[c]);
         }
         rels[j]->nrows = hk[j];
     }

@@ -179,10 +179,30 @@ vd_rel *rel_new(vd_ctx *ctx, int ncols, uint64_t nrows) {
     return r;
 }
 
+vd_rel *rel_view(int ncols, uint64_t nrows, uint32_t *const *cols, std::shared_ptr<void> hold) {
+    VD_REQUIRE(ncols >= 1 && ncols <= VD_MAX_COLS, VD_EINVAL,
+               "relation column count out of range");
+    vd_rel *r = new vd_rel();
+    r->ncols = ncols;
+    r->nrows = nrows;
+    for (int c = 0; c < ncols; ++c) r->cols[c] = cols[c];
+    r->borrowed = (1u << ncols) - 1u;
+    r->hold = std::move(hold);
+    return r;
+}
+
 void rel_free(vd_ctx *ctx, vd_rel *r) {
     if (!r) return;
-    for (int c = 0; c < r->ncols; ++c) dfree(ctx, r->cols[c]);
+    for (int c = 0; c < r->ncols; ++c)
+        if (!rel_borrowed(r, c)) dfree(ctx, r->cols[c]);
     delete r;
+}
+
+void rel_set_col(vd_ctx *ctx, vd_rel *r, int c, uint32_t *fresh) {
+    if (!rel_borrowed(r, c)) dfree(ctx, r->cols[c]);
+    r->cols[c] = fresh;
+    r->borrowed &= ~(1u << c);
+    if (!r->borrowed) r->hold.reset();
 }
 
 }  // namespace vd
@@ -333,17 +353,41 @@ int vd_rel_swap_cols(vd_rel *rel, int i, int j) {
     if (!rel || i < 0 || j < 0 || i >= rel->ncols || j >= rel->ncols)
         return VD_EINVAL;
     std::swap(rel->cols[i], rel->cols[j]);
+    const uint32_t bi = (rel->borrowed >> i) & 1u, bj = (rel->borrowed >> j) & 1u;
+    rel->borrowed &= ~((1u << i) | (1u << j));
+    rel->borrowed |= (bi << j) | (bj << i);
     return VD_OK;
 }
 
 int vd_rel_drop_col(vd_ctx *ctx, vd_rel *rel, int i) {
     if (!ctx || !rel || i < 0 || i >= rel->ncols || rel->ncols == 1)
         return VD_EINVAL;
-    vd::dfree(ctx, rel->cols[i]);
+    if (!vd::rel_borrowed(rel, i)) vd::dfree(ctx, rel->cols[i]);
     for (int c = i; c + 1 < rel->ncols; ++c) rel->cols[c] = rel->cols[c + 1];
     rel->cols[rel->ncols - 1] = nullptr;
+    const uint32_t low = rel->borrowed & ((1u << i) - 1u);
+    rel->borrowed = low | ((rel->borrowed >> (i + 1)) << i);
     rel->ncols -= 1;
+    if (!rel->borrowed) rel->hold.reset();
     return VD_OK;
+}
+
+int vd_rel_materialize(vd_ctx *ctx, vd_rel *rel) {
+    if (!ctx || !rel) return VD_EINVAL;
+    VD_TRY(ctx, {
+        for (int c = 0; c < rel->ncols; ++c) {
+            if (!vd::rel_borrowed(rel, c)) continue;
+            const size_t bytes = (rel->nrows ? rel->nrows : 1) * sizeof(uint32_t);
+            uint32_t *fresh = static_cast<uint32_t *>(vd::dalloc(ctx, bytes));
+            if (rel->nrows) vd::copy_d2d(ctx, fresh, rel->cols[c], rel->nrows * sizeof(uint32_t));
+            vd::rel_set_col(ctx, rel, c, fresh);
+        }
+    });
+}
+
+int vd_rel_is_view(const vd_rel *rel, int c) {
+    if (!rel || c < 0 || c >= rel->ncols) return 0;
+    return vd::rel_borrowed(rel, c) ? 1 : 0;
 }
 
 int vd_prof_enable(vd_ctx *ctx, int on) {

@@ -454,6 +454,17 @@ void free_perm(vd_ctx *ctx, vd_perm_arrays &pa) {
 
 }  // namespace
 
+// Frees the arrays of every built permutation when the last owner lets go.
+std::shared_ptr<void> make_hold(vd_ctx *owner, const vd_perm_arrays (&perm)[VD_NUM_PERMS]) {
+    std::vector<vd_perm_arrays> arrays(perm, perm + VD_NUM_PERMS);
+    return std::shared_ptr<void>(nullptr, [owner, arrays](void *) mutable {
+        try {
+            for (auto &pa : arrays) free_perm(owner, pa);
+        } catch (...) {
+        }
+    });
+}
+
 PermDev perm_dev(const vd_perm_arrays &pa) {
     return PermDev{pa.n, pa.n_l1, pa.l1v, pa.l1o, pa.y};
 }
@@ -518,6 +529,13 @@ int vd_index_build(vd_ctx *ctx, const uint32_t *d_s, const uint32_t *d_p,
         delete idx;
         return VD_EINVAL;
     }
+    try {
+        idx->hold = make_hold(ctx, idx->perm);
+    } catch (...) {
+        for (auto &pa : idx->perm) free_perm(ctx, pa);
+        delete idx;
+        return VD_ENOMEM;
+    }
     *out = idx;
     return VD_OK;
 }
@@ -526,7 +544,7 @@ int vd_index_free(vd_ctx *ctx, vd_index *idx) {
     if (!idx) return VD_EINVAL;
     vd_ctx *owner = idx->owner ? idx->owner : ctx;
     VD_TRY(owner, {
-        for (auto &pa : idx->perm) free_perm(owner, pa);
+        idx->hold.reset();  // arrays go when no select view still reads them
         delete idx;
     });
 }
@@ -617,6 +635,16 @@ int vd_select(vd_ctx *ctx, const vd_index *idx, int perm, uint64_t begin,
         for (int i = 1; i < nskip; ++i)
             VD_REQUIRE(sk[i].lo > sk[i - 1].hi || sk[i - 1].lo > sk[i - 1].hi,
                        VD_EINVAL, "skip intervals overlap");
+        if (!clip && !nskip && !distinct) {
+            // An unfiltered range is already contiguous in the permutation's
+            // columns: the relation reads it in place.
+            uint32_t *vc[2];
+            int nc = 0;
+            if (cols & 2) vc[nc++] = pa.y + begin;
+            if (cols & 1) vc[nc++] = pa.z + begin;
+            *out = rel_view(nc, end - begin, vc, idx->hold);
+            return VD_OK;
+        }
         DBuf<Pieces> dp(ctx, 1);
         launch(ctx, "select_pieces", 0.0, dim3(1), dim3(64), 0, k_select_pieces,
                (const uint32_t *)pa.y, begin, end, clip, lo, hi, sk[0], sk[1],

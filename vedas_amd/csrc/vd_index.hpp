@@ -15,6 +15,10 @@ struct vd_index {
     uint64_t n = 0;
     uint32_t mask = 0;
     vd_perm_arrays perm[VD_NUM_PERMS];
+    // Owns the device arrays once the build succeeded; relations that read
+    // ranges of them without a copy share it, so the arrays outlive
+    // vd_index_free until the last such relation is destroyed.
+    std::shared_ptr<void> hold;
 };
 
 namespace vd {

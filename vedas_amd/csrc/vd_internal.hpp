@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -104,10 +105,16 @@ struct vd_ctx {
     int join_rcap = 4096;   // right keys staged in LDS per bounds block
 };
 
+// A relation: ncols device columns of nrows uint32 each.  A column whose bit
+// is set in `borrowed` is a read-only window into storage the relation does
+// not own (a range of an index permutation); `hold` keeps that storage alive.
+// Operators never write into a borrowed column: they replace it.
 struct vd_rel {
     int ncols = 0;
     uint64_t nrows = 0;
     uint32_t *cols[VD_MAX_COLS] = {};
+    uint32_t borrowed = 0;
+    std::shared_ptr<void> hold;
 };
 
 namespace vd {
@@ -214,7 +221,13 @@ inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
 // Relation helpers.
 vd_rel *rel_new(vd_ctx *ctx, int ncols, uint64_t nrows);
+// A relation of borrowed columns (no device allocation).
+vd_rel *rel_view(int ncols, uint64_t nrows, uint32_t *const *cols, std::shared_ptr<void> hold);
 void rel_free(vd_ctx *ctx, vd_rel *r);
+// Replace column c by `fresh` (a pool allocation the relation now owns),
+// freeing the old column if the relation owned it.
+void rel_set_col(vd_ctx *ctx, vd_rel *r, int c, uint32_t *fresh);
+inline bool rel_borrowed(const vd_rel *r, int c) { return (r->borrowed >> c) & 1u; }
 
 }  // namespace vd
 

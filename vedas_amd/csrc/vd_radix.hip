@@ -17,7 +17,8 @@
 // from the atomic counter as blocks start, so every tile < t belongs to a
 // block that is already resident, and a block publishes its aggregate
 // before it waits on anyone.  Tile 0 publishes an inclusive value without
-// waiting.  Hence every wait ends.
+// waiting.  Hence every wait ends.  The look-back reads a window of
+// predecessors per round trip and stops at the first unpublished one.
 
 #include "vd_radix_dev.hpp"
 
@@ -27,8 +28,6 @@ namespace {
 using radix::RBINS;
 using radix::RBITS;
 
-constexpr int RS_BLOCK = 256;
-constexpr int RS_WAVES = RS_BLOCK / 64;
 
 constexpr uint32_t LB_AGG = 1u << 30;   // tile aggregate published
 constexpr uint32_t LB_INC = 2u << 30;   // inclusive prefix published
@@ -66,27 +65,36 @@ __global__ __launch_bounds__(RBINS) void k_rs_scan_hist(const uint32_t *ghist,
     gofs[blockIdx.x * RBINS + threadIdx.x] = ex;
 }
 
+// Look-back window: predecessor tiles read per round trip.  The published
+// values cross XCDs (agent scope), so each round trip is long; reading a
+// window of tiles at once cuts the number of trips.
+constexpr int LB_WINDOW = 8;
+
 // VAL: a 32-bit value travels with every key (same source and destination).
-template <int ITEMS, bool VAL>
-__global__ __launch_bounds__(RS_BLOCK) void k_rs_onesweep(
+// BLOCK threads (a multiple of 256: threads 0..255 own one digit each in the
+// count and look-back phases), ITEMS keys per thread.
+template <int BLOCK, int ITEMS, bool VAL>
+__global__ __launch_bounds__(BLOCK) void k_rs_onesweep(
     const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
     const uint32_t *__restrict__ vin, uint32_t *__restrict__ vout, uint64_t n,
     int shift, int width, const uint32_t *__restrict__ gofs,
     uint32_t *lookback, uint32_t *tile_counter) {
-    constexpr int TILE = RS_BLOCK * ITEMS;
+    constexpr int WAVES = BLOCK / 64;
+    constexpr int TILE = BLOCK * ITEMS;
+    static_assert(BLOCK % RBINS == 0, "one digit per thread in the count phase");
     __shared__ uint64_t s_keys[TILE];
     __shared__ uint32_t s_vals[VAL ? TILE : 1];
-    __shared__ uint32_t s_wcnt[RS_WAVES][RBINS];
+    __shared__ uint32_t s_wcnt[WAVES][RBINS];
     __shared__ uint32_t s_lstart[RBINS];
     __shared__ uint32_t s_gbase[RBINS];
-    __shared__ uint32_t s_tmp[RS_WAVES];
+    __shared__ uint32_t s_tmp[WAVES];
     __shared__ uint32_t s_tile;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int w = tid >> 6;
 
-    for (int i = tid; i < RS_WAVES * RBINS; i += RS_BLOCK) (&s_wcnt[0][0])[i] = 0;
+    for (int i = tid; i < WAVES * RBINS; i += BLOCK) (&s_wcnt[0][0])[i] = 0;
     if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
     __syncthreads();
 
@@ -130,46 +138,59 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_onesweep(
     }
     __syncthreads();
 
-    // Per digit (thread tid = digit): wave-exclusive offsets and tile count.
-    uint32_t tcount = 0;
-    {
+    // Threads 0..255, one digit each: wave-exclusive offsets, tile count,
+    // publish, look back.
+    uint32_t tcount = 0, excl = 0;
+    if (tid < RBINS) {
         uint32_t sum = 0;
 #pragma unroll
-        for (int ww = 0; ww < RS_WAVES; ++ww) {
+        for (int ww = 0; ww < WAVES; ++ww) {
             const uint32_t c = s_wcnt[ww][tid];
             s_wcnt[ww][tid] = sum;
             sum += c;
         }
         tcount = sum;
-    }
-    uint32_t pub = tcount;
-    if ((uint32_t)tid == dmask) pub -= (uint32_t)TILE - nvalid;  // padding
-
-    uint32_t excl = 0;
-    uint32_t *mine = lookback + (uint64_t)tile * RBINS + tid;
-    if (tile == 0) {
-        lb_store(mine, LB_INC | pub);
-    } else {
-        lb_store(mine, LB_AGG | pub);
-        int64_t j = (int64_t)tile - 1;
-        for (;;) {
-            const uint32_t v = lb_load(lookback + (uint64_t)j * RBINS + tid);
-            const uint32_t st = v & ~LB_VAL;
-            if (st == 0) {
-                __builtin_amdgcn_s_sleep(1);
-                continue;
+        uint32_t pub = tcount;
+        if ((uint32_t)tid == dmask) pub -= (uint32_t)TILE - nvalid;  // padding
+        uint32_t *mine = lookback + (uint64_t)tile * RBINS + tid;
+        if (tile == 0) {
+            lb_store(mine, LB_INC | pub);
+        } else {
+            lb_store(mine, LB_AGG | pub);
+            int64_t j = (int64_t)tile - 1;
+            for (;;) {
+                // Read tiles j, j-1, ..., j-LB_WINDOW+1 at once; take them in
+                // order up to the first inclusive value or unpublished tile.
+                uint32_t lv[LB_WINDOW];
+#pragma unroll
+                for (int q = 0; q < LB_WINDOW; ++q)
+                    lv[q] = j - q >= 0 ? lb_load(lookback + (uint64_t)(j - q) * RBINS + tid)
+                                       : LB_INC;
+                bool done = false, stop = false;
+                int used = 0;
+#pragma unroll
+                for (int q = 0; q < LB_WINDOW; ++q) {
+                    const uint32_t st = lv[q] & ~LB_VAL;
+                    const bool take = !done && !stop && st != 0;
+                    stop = stop || st == 0;
+                    excl += take ? (lv[q] & LB_VAL) : 0u;
+                    used += take ? 1 : 0;
+                    done = done || (take && st == LB_INC);
+                }
+                if (done) break;
+                j -= used;
+                if (used == 0) __builtin_amdgcn_s_sleep(1);
             }
-            excl += v & LB_VAL;
-            if (st == LB_INC) break;
-            --j;
+            lb_store(mine, LB_INC | (excl + pub));
         }
-        lb_store(mine, LB_INC | (excl + pub));
     }
 
     uint32_t tot;
-    const uint32_t lstart = dev::block_excl_scan<RS_BLOCK>(tcount, s_tmp, &tot);
-    s_lstart[tid] = lstart;
-    s_gbase[tid] = gofs[tid] + excl - lstart;
+    const uint32_t lstart = dev::block_excl_scan<BLOCK>(tid < RBINS ? tcount : 0u, s_tmp, &tot);
+    if (tid < RBINS) {
+        s_lstart[tid] = lstart;
+        s_gbase[tid] = gofs[tid] + excl - lstart;
+    }
     __syncthreads();
 
 #pragma unroll
@@ -181,7 +202,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_onesweep(
     }
     __syncthreads();
 
-    for (uint32_t i = tid; i < nvalid; i += RS_BLOCK) {
+    for (uint32_t i = tid; i < nvalid; i += BLOCK) {
         const uint64_t key = s_keys[i];
         const uint32_t d = (uint32_t)(key >> shift) & dmask;
         const uint64_t dst = (uint64_t)(s_gbase[d] + i);
@@ -190,10 +211,10 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_onesweep(
     }
 }
 
-template <int ITEMS, bool VAL>
+template <int BLOCK, int ITEMS, bool VAL>
 void run_passes(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, uint32_t *&va, uint32_t *&vb,
                 uint64_t n, int lo_bit, int hi_bit, int npass, const uint32_t *gofs) {
-    constexpr uint64_t TILE = RS_BLOCK * ITEMS;
+    constexpr uint64_t TILE = (uint64_t)BLOCK * ITEMS;
     const uint64_t ntiles = ceil_div(n, TILE);
     const uint64_t lb_words = (uint64_t)npass * ntiles * RBINS;
     DBuf<uint32_t> lb(ctx, lb_words + npass);
@@ -204,7 +225,7 @@ void run_passes(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, uint32_t *&va, uint32_t
         const int width = std::min(RBITS, hi_bit - shift);
         launch(ctx, VAL ? "radix_onesweep_kv" : "radix_onesweep",
                (double)n * (VAL ? 24.0 : 16.0), dim3((unsigned)ntiles),
-               dim3(RS_BLOCK), 0, k_rs_onesweep<ITEMS, VAL>, (const uint64_t *)a, b,
+               dim3(BLOCK), 0, k_rs_onesweep<BLOCK, ITEMS, VAL>, (const uint64_t *)a, b,
                (const uint32_t *)va, vb, n, shift, width, gofs + (uint64_t)p * RBINS,
                lb.p + (uint64_t)p * ntiles * RBINS, counters + p);
         std::swap(a, b);
@@ -259,17 +280,21 @@ static void sort_impl(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, uint32_t *&va,
     DBuf<uint32_t> gofs(ctx, (uint64_t)npass * RBINS);
     launch(ctx, "radix_scan_hist", 0.0, dim3(npass), dim3(RBINS), 0,
            k_rs_scan_hist, (const uint32_t *)d_hist, gofs.p);
-    if (val) {  // values need LDS staging: at most 16 keys per thread
-        if (ctx->radix_items == 8)
-            run_passes<8, true>(ctx, a, b, va, vb, n, lo_bit, hi_bit, npass, gofs.p);
+    // Tile shapes (threads x keys per thread), chosen by the radix_items
+    // tunable: 16 -> 512 x 16 (default), 8 -> 512 x 8, 24 -> 256 x 24 (keys
+    // only); values need LDS staging too, so at most 16 keys per thread.
+    const int items = ctx->radix_items;
+    if (val) {
+        if (items == 8)
+            run_passes<512, 8, true>(ctx, a, b, va, vb, n, lo_bit, hi_bit, npass, gofs.p);
         else
-            run_passes<16, true>(ctx, a, b, va, vb, n, lo_bit, hi_bit, npass, gofs.p);
+            run_passes<512, 16, true>(ctx, a, b, va, vb, n, lo_bit, hi_bit, npass, gofs.p);
         return;
     }
-    switch (ctx->radix_items) {
-        case 8: run_passes<8, false>(ctx, a, b, va, vb, n, lo_bit, hi_bit, npass, gofs.p); break;
-        case 24: run_passes<24, false>(ctx, a, b, va, vb, n, lo_bit, hi_bit, npass, gofs.p); break;
-        default: run_passes<16, false>(ctx, a, b, va, vb, n, lo_bit, hi_bit, npass, gofs.p); break;
+    switch (items) {
+        case 8: run_passes<512, 8, false>(ctx, a, b, va, vb, n, lo_bit, hi_bit, npass, gofs.p); break;
+        case 24: run_passes<256, 24, false>(ctx, a, b, va, vb, n, lo_bit, hi_bit, npass, gofs.p); break;
+        default: run_passes<512, 16, false>(ctx, a, b, va, vb, n, lo_bit, hi_bit, npass, gofs.p); break;
     }
 }
 

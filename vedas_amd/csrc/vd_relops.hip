@@ -1,10 +1,9 @@
 // Relation re-sorts and finalisation: sort by column 0 (IndexSwap),
 // lexicographic sort, and removal of adjacent duplicate rows.
 //
-// sort_col0: key = col0 << 32 | row, radix-sorted on the bits of col0 only
-// (the row id in the low word keeps the original order of equal keys, so
-// the result is the stable sort), then one gather pass writes column 0 from
-// the keys and the other columns through the row ids.
+// sort_col0: key = col0 << 32 | col1, radix-sorted on the bits of col0 only
+// (LSD passes are stable, so equal keys keep their original order), with a
+// 32-bit value for a third column or the row id of wider relations.
 
 #include "vd_radix_dev.hpp"
 
@@ -46,16 +45,47 @@ __global__ __launch_bounds__(RB) void k_pack_rowid(
     radix::hist_flush(&s, ghist, npass);
 }
 
-// out0[i] = key >> 32; out_c[i] = in_c[row] for c >= 1.
-__global__ __launch_bounds__(256) void k_gather_sorted(
-    const uint64_t *__restrict__ keys, uint64_t n, Cols cols) {
+// key = col0[i] << 32 | col1[i] (no col1: 0); with vals, vals[i] = vsrc[i]
+// (vsrc null: the row id i).
+__global__ __launch_bounds__(RB) void k_pack_c01(
+    const uint32_t *__restrict__ c0, const uint32_t *__restrict__ c1,
+    const uint32_t *__restrict__ vsrc, uint64_t n, uint64_t *__restrict__ keys,
+    uint32_t *__restrict__ vals, int hi_bit, int npass, uint32_t *ghist) {
+    __shared__ radix::LdsHist s;
+    radix::hist_zero(&s);
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * (RB * R_ITEMS);
+#pragma unroll 4
+    for (int j = 0; j < R_ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)j * RB + threadIdx.x;
+        if (i < n) {
+            const uint64_t key = ((uint64_t)c0[i] << 32) | (c1 ? c1[i] : 0u);
+            keys[i] = key;
+            if (vals) vals[i] = vsrc ? vsrc[i] : (uint32_t)i;
+            radix::hist_add(&s, key, 32, hi_bit, npass);
+        }
+    }
+    __syncthreads();
+    radix::hist_flush(&s, ghist, npass);
+}
+
+// out0 = key >> 32, out1 = low word of the key; three columns: out2 = val;
+// more: out_c = in_c[val] for c >= 2 (val is the source row).
+__global__ __launch_bounds__(256) void k_unpack_c01(const uint64_t *__restrict__ keys,
+                                                    const uint32_t *__restrict__ vals,
+                                                    uint64_t n, Cols cols) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += stride) {
         const uint64_t k = keys[i];
-        const uint32_t row = (uint32_t)k;
         cols.out[0][i] = (uint32_t)(k >> 32);
-        for (int c = 1; c < cols.n; ++c) cols.out[c][i] = cols.in[c][row];
+        if (cols.n >= 2) cols.out[1][i] = (uint32_t)k;
+        if (cols.n == 3) {
+            cols.out[2][i] = vals[i];
+        } else if (cols.n > 3) {
+            const uint32_t row = vals[i];
+            for (int c = 2; c < cols.n; ++c) cols.out[c][i] = cols.in[c][row];
+        }
     }
 }
 
@@ -143,18 +173,47 @@ uint64_t *sort_by_col(vd_ctx *ctx, const uint32_t *col, const uint32_t *perm,
     return radix_sort_u64(ctx, keys.p, tmp.p, n, 32, hi, hist.p, true);
 }
 
+// Stable sort by column 0.  Column 1 rides in the low word of the 64-bit key
+// (only the bits of column 0 are sorted, so equal keys keep their order);
+// column 2 of a three-column relation rides as the 32-bit value, and wider
+// relations carry the row id as the value and gather columns 2.. once.
+// Columns 0 and 1 (and 2 of a three-column relation) are rewritten in place
+// unless they are borrowed.
 void sort_col0(vd_ctx *ctx, vd_rel *rel) {
     const uint64_t n = rel->nrows;
+    const int nc = rel->ncols;
     if (n <= 1) return;
     VD_REQUIRE(n < (1ull << 30), VD_ERANGE, "sort supports fewer than 2^30 rows");
-    DBuf<uint64_t> keys(ctx, n), tmp(ctx, n);
-    uint64_t *sorted = sort_by_col(ctx, rel->cols[0], nullptr, n, keys, tmp);
+    const int b = std::max(1, bits_for(max_u32(ctx, rel->cols[0], n)));
+    const int hi = 32 + b;
+    const int npass = radix_passes(32, hi);
+    const bool kv = nc >= 3;
+    DBuf<uint64_t> keys(ctx, n), ktmp(ctx, n);
+    DBuf<uint32_t> vals(ctx, kv ? n : 0), vtmp(ctx, kv ? n : 0);
+    DBuf<uint32_t> hist(ctx, radix_hist_words(32, hi));
+    fill_u32(ctx, hist.p, 0, radix_hist_words(32, hi));
+    const uint32_t *vsrc = nc == 3 ? rel->cols[2] : nullptr;
+    launch(ctx, "sort_pack", (double)n * (8.0 + 4.0 * std::min(nc, 2) + (kv ? 4.0 : 0.0) +
+                                          (nc == 3 ? 4.0 : 0.0)),
+           dim3((unsigned)ceil_div(n, RB * R_ITEMS)), dim3(RB), 0, k_pack_c01,
+           (const uint32_t *)rel->cols[0], (const uint32_t *)(nc >= 2 ? rel->cols[1] : nullptr),
+           vsrc, n, keys.p, kv ? vals.p : nullptr, hi, npass, hist.p);
+    uint64_t *k = keys.p, *kt = ktmp.p;
+    uint32_t *v = vals.p, *vt = vtmp.p;
+    if (kv)
+        radix_sort_u64_kv(ctx, k, kt, v, vt, n, 32, hi, hist.p, true);
+    else
+        k = radix_sort_u64(ctx, k, kt, n, 32, hi, hist.p, true);
     Cols cols{};
-    cols.n = rel->ncols;
+    cols.n = nc;
     uint32_t *fresh[VD_MAX_COLS] = {};
     try {
-        cols.out[0] = rel->cols[0];  // column 0 comes from the keys: in place
-        for (int c = 1; c < rel->ncols; ++c) {
+        for (int c = 0; c < nc; ++c) {
+            const bool inplace = c < 2 || (c == 2 && nc == 3);
+            if (inplace && !rel_borrowed(rel, c)) {
+                cols.out[c] = rel->cols[c];
+                continue;
+            }
             fresh[c] = (uint32_t *)dalloc(ctx, n * 4);
             cols.in[c] = rel->cols[c];
             cols.out[c] = fresh[c];
@@ -163,13 +222,12 @@ void sort_col0(vd_ctx *ctx, vd_rel *rel) {
         for (auto p : fresh) dfree(ctx, p);
         throw;
     }
-    launch(ctx, "sort_gather", (double)n * (8.0 + 4.0 + 8.0 * (rel->ncols - 1)),
-           dim3(sgrid(n)), dim3(256), 0, k_gather_sorted,
-           (const uint64_t *)sorted, n, cols);
-    for (int c = 1; c < rel->ncols; ++c) {
-        dfree(ctx, rel->cols[c]);
-        rel->cols[c] = fresh[c];
-    }
+    const double gathered = nc > 3 ? 4.0 + 8.0 * (nc - 2) : (nc == 3 ? 8.0 : 0.0);
+    launch(ctx, "sort_unpack", (double)n * (8.0 + 4.0 * std::min(nc, 2) + gathered),
+           dim3(sgrid(n)), dim3(256), 0, k_unpack_c01, (const uint64_t *)k,
+           (const uint32_t *)v, n, cols);
+    for (int c = 0; c < nc; ++c)
+        if (fresh[c]) rel_set_col(ctx, rel, c, fresh[c]);
 }
 
 void sort_lex(vd_ctx *ctx, vd_rel *rel) {
@@ -206,8 +264,7 @@ void sort_lex(vd_ctx *ctx, vd_rel *rel) {
            dim3(sgrid(n)), dim3(256), 0, k_gather_perm, (const uint32_t *)perm.p,
            n, cols);
     for (int c = 0; c < rel->ncols; ++c) {
-        dfree(ctx, rel->cols[c]);
-        rel->cols[c] = fresh[c];
+        rel_set_col(ctx, rel, c, fresh[c]);
     }
 }
 
@@ -239,8 +296,7 @@ void unique_adjacent(vd_ctx *ctx, vd_rel *rel) {
            dim3((unsigned)tiles), dim3(U_BLOCK), 0, k_unique_scatter, cols, n,
            (const uint64_t *)off.p);
     for (int c = 0; c < rel->ncols; ++c) {
-        dfree(ctx, rel->cols[c]);
-        rel->cols[c] = fresh[c];
+        rel_set_col(ctx, rel, c, fresh[c]);
     }
     rel->nrows = kept;
 }

@@ -85,6 +85,7 @@ void FullRelationIR::setRelation(size_t i, size_t offset, TYPEID_HOST_VEC::itera
     const size_t n = eit - bit;
     if (n == 0) return;
     if (offset + n > size()) throw VedasError(VD_EINVAL, "setRelation out of range");
+    ctx_->check(vd_rel_materialize(ctx_->raw(), rel_), "vd_rel_materialize");
     ctx_->check(vd_copy_h2d(ctx_->raw(), getRelationRawPointer(i) + offset, &*bit,
                             n * sizeof(TYPEID)), "vd_copy_h2d");
 }
@@ -99,6 +100,7 @@ void FullRelationIR::setRelation(size_t i, size_t offset, const TYPEID *dbegin,
     const size_t n = dend - dbegin;
     if (n == 0) return;
     if (offset + n > size()) throw VedasError(VD_EINVAL, "setRelation out of range");
+    ctx_->check(vd_rel_materialize(ctx_->raw(), rel_), "vd_rel_materialize");
     ctx_->check(vd_copy_d2d(ctx_->raw(), getRelationRawPointer(i) + offset, dbegin,
                             n * sizeof(TYPEID)), "vd_copy_d2d");
 }
