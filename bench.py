@@ -332,11 +332,22 @@ def main():
     # dominant kernel by device time in the timed region
     dom_name, dom = max(prof.items(), key=lambda kv: kv[1]["ms"])
     achieved = dom["bytes"] / (dom["ms"] / 1e3) if dom["ms"] > 0 else 0.0
+    alg_per_launch = dom["bytes"] / max(dom["launches"], 1)
     roofline = {"bound": "hbm", "kernel": dom_name, "achieved": achieved / 1e9,
                 "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
                 "traffic": None, "launches": dom["launches"],
                 "avg_launch_ms": dom["ms"] / max(dom["launches"], 1),
-                "alg_bytes_per_launch": dom["bytes"] / max(dom["launches"], 1)}
+                "alg_bytes_per_launch": alg_per_launch}
+    # HBM bytes per launch of the same kernel on the same workload, measured
+    # by two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) and kept as text.
+    traffic_file = os.path.join(ROOT, "profiles", "hbm_traffic_joinset_100M.md")
+    if os.path.exists(traffic_file) and n == STORE_100M["n"]:
+        from vedas_amd.prof_summary import read_traffic
+        measured = read_traffic(traffic_file).get(dom_name)
+        if measured:
+            roofline["traffic"] = measured
+            roofline["traffic_over_alg"] = measured / alg_per_launch if alg_per_launch else None
+            roofline["traffic_source"] = os.path.relpath(traffic_file, ROOT)
 
     batch = serve = None
     if args.mix > 0:

@@ -205,6 +205,23 @@ uint64_t *radix_sort_u64(vd_ctx *ctx, uint64_t *keys, uint64_t *tmp,
 void radix_sort_u64_kv(vd_ctx *ctx, uint64_t *&keys, uint64_t *&ktmp, uint32_t *&vals,
                        uint32_t *&vtmp, uint64_t n, int lo_bit, int hi_bit,
                        uint32_t *d_hist, bool hist_ready);
+// Payload columns moved with the keys by radix_sort_u64_payload.
+constexpr int RADIX_MAX_PAYLOAD = 3;
+struct RadixCols {
+    uint32_t *p[RADIX_MAX_PAYLOAD];
+};
+struct RadixPayload {  // kernel argument of one pass
+    const uint32_t *in[RADIX_MAX_PAYLOAD];
+    uint32_t *out[RADIX_MAX_PAYLOAD];
+};
+// Same with m <= RADIX_MAX_PAYLOAD uint32 columns: the first pass reads
+// src, pass p writes pb (p even) or pa (p odd); *final gets the buffers
+// holding the sorted columns (src itself if no pass ran).  src may alias pa.
+// Returns the buffer holding the sorted keys (keys or tmp).
+uint64_t *radix_sort_u64_payload(vd_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n,
+                                 int lo_bit, int hi_bit, uint32_t *d_hist, bool hist_ready,
+                                 int m, const RadixCols &src, const RadixCols &pa,
+                                 const RadixCols &pb, RadixCols *final);
 int radix_passes(int lo_bit, int hi_bit);
 // Device histogram buffer size (uint32 words) needed for radix_sort_u64.
 inline size_t radix_hist_words(int lo_bit, int hi_bit) {

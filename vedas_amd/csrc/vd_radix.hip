@@ -70,20 +70,22 @@ __global__ __launch_bounds__(RBINS) void k_rs_scan_hist(const uint32_t *ghist,
 // window of tiles at once cuts the number of trips.
 constexpr int LB_WINDOW = 8;
 
-// VAL: a 32-bit value travels with every key (same source and destination).
+// M 32-bit payload columns travel with the keys (pay.in[c] -> pay.out[c]);
+// they are loaded with the keys and staged through one LDS buffer, column
+// after column, so every global access stays coalesced.
 // BLOCK threads (a multiple of 256: threads 0..255 own one digit each in the
 // count and look-back phases), ITEMS keys per thread.
-template <int BLOCK, int ITEMS, bool VAL>
+template <int BLOCK, int ITEMS, int M>
 __global__ __launch_bounds__(BLOCK) void k_rs_onesweep(
-    const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
-    const uint32_t *__restrict__ vin, uint32_t *__restrict__ vout, uint64_t n,
-    int shift, int width, const uint32_t *__restrict__ gofs,
+    const uint64_t *__restrict__ in, uint64_t *__restrict__ out, RadixPayload pay,
+    uint64_t n, int shift, int width, const uint32_t *__restrict__ gofs,
     uint32_t *lookback, uint32_t *tile_counter) {
     constexpr int WAVES = BLOCK / 64;
     constexpr int TILE = BLOCK * ITEMS;
     static_assert(BLOCK % RBINS == 0, "one digit per thread in the count phase");
+    static_assert(M >= 0 && M <= RADIX_MAX_PAYLOAD, "payload columns");
     __shared__ uint64_t s_keys[TILE];
-    __shared__ uint32_t s_vals[VAL ? TILE : 1];
+    __shared__ uint32_t s_vals[M ? TILE : 1];
     __shared__ uint32_t s_wcnt[WAVES][RBINS];
     __shared__ uint32_t s_lstart[RBINS];
     __shared__ uint32_t s_gbase[RBINS];
@@ -108,12 +110,13 @@ __global__ __launch_bounds__(BLOCK) void k_rs_onesweep(
     // also the ranking order below.  Missing keys are all-ones: they carry
     // the largest digit and rank after every real key of the tile.
     uint64_t k[ITEMS];
-    uint32_t v[VAL ? ITEMS : 1];
+    uint32_t v[M ? M : 1][ITEMS];
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
         const uint64_t idx = base + (uint64_t)((w * ITEMS + j) * 64 + lane);
         k[j] = idx < n ? in[idx] : ~0ull;
-        if (VAL) v[VAL ? j : 0] = idx < n ? vin[idx] : 0u;
+#pragma unroll
+        for (int c = 0; c < M; ++c) v[c][j] = idx < n ? pay.in[c][idx] : 0u;
     }
 
     // Rank within the wave: peers = lanes holding the same digit.
@@ -193,26 +196,49 @@ __global__ __launch_bounds__(BLOCK) void k_rs_onesweep(
     }
     __syncthreads();
 
+    uint32_t pos[ITEMS];
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
         const uint32_t d = (uint32_t)(k[j] >> shift) & dmask;
-        const uint32_t pos = s_lstart[d] + s_wcnt[w][d] + rk[j];
-        s_keys[pos] = k[j];
-        if (VAL) s_vals[VAL ? pos : 0] = v[VAL ? j : 0];
+        pos[j] = s_lstart[d] + s_wcnt[w][d] + rk[j];
+        s_keys[pos[j]] = k[j];
+        if (M) s_vals[pos[j]] = v[0][j];
     }
     __syncthreads();
 
-    for (uint32_t i = tid; i < nvalid; i += BLOCK) {
-        const uint64_t key = s_keys[i];
-        const uint32_t d = (uint32_t)(key >> shift) & dmask;
-        const uint64_t dst = (uint64_t)(s_gbase[d] + i);
-        out[dst] = key;
-        if (VAL) vout[dst] = s_vals[VAL ? i : 0];
+    // Staged position i (i = tid + BLOCK * r) goes to dst[r].
+    uint32_t dst[ITEMS];
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+        const uint32_t i = (uint32_t)(tid + r * BLOCK);
+        if (i < nvalid) {
+            const uint64_t key = s_keys[i];
+            const uint32_t d = (uint32_t)(key >> shift) & dmask;
+            dst[r] = s_gbase[d] + i;
+            out[dst[r]] = key;
+            if (M) pay.out[0][dst[r]] = s_vals[i];
+        }
+    }
+#pragma unroll
+    for (int c = 1; c < M; ++c) {
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) s_vals[pos[j]] = v[c][j];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < ITEMS; ++r) {
+            const uint32_t i = (uint32_t)(tid + r * BLOCK);
+            if (i < nvalid) pay.out[c][dst[r]] = s_vals[i];
+        }
     }
 }
 
-template <int BLOCK, int ITEMS, bool VAL>
-void run_passes(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, uint32_t *&va, uint32_t *&vb,
+// Pass p reads the keys from a and writes b (then swaps), and moves the
+// payload from src (first pass) or the buffer the previous pass wrote, into
+// pa (odd passes) or pb (even passes); *final receives the last buffers.
+template <int BLOCK, int ITEMS, int M>
+void run_passes(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, const RadixCols &src,
+                const RadixCols &pa, const RadixCols &pb, RadixCols *final,
                 uint64_t n, int lo_bit, int hi_bit, int npass, const uint32_t *gofs) {
     constexpr uint64_t TILE = (uint64_t)BLOCK * ITEMS;
     const uint64_t ntiles = ceil_div(n, TILE);
@@ -220,17 +246,25 @@ void run_passes(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, uint32_t *&va, uint32_t
     DBuf<uint32_t> lb(ctx, lb_words + npass);
     fill_u32(ctx, lb.p, 0, lb_words + npass);
     uint32_t *counters = lb.p + lb_words;
+    RadixCols cur = src;
     for (int p = 0; p < npass; ++p) {
         const int shift = lo_bit + p * RBITS;
         const int width = std::min(RBITS, hi_bit - shift);
-        launch(ctx, VAL ? "radix_onesweep_kv" : "radix_onesweep",
-               (double)n * (VAL ? 24.0 : 16.0), dim3((unsigned)ntiles),
-               dim3(BLOCK), 0, k_rs_onesweep<BLOCK, ITEMS, VAL>, (const uint64_t *)a, b,
-               (const uint32_t *)va, vb, n, shift, width, gofs + (uint64_t)p * RBINS,
+        const RadixCols &dstc = (p % 2 == 0) ? pb : pa;
+        RadixPayload pay{};
+        for (int c = 0; c < M; ++c) {
+            pay.in[c] = cur.p[c];
+            pay.out[c] = dstc.p[c];
+        }
+        launch(ctx, M ? "radix_onesweep_kv" : "radix_onesweep",
+               (double)n * (16.0 + 8.0 * M), dim3((unsigned)ntiles),
+               dim3(BLOCK), 0, k_rs_onesweep<BLOCK, ITEMS, M>, (const uint64_t *)a, b, pay,
+               n, shift, width, gofs + (uint64_t)p * RBINS,
                lb.p + (uint64_t)p * ntiles * RBINS, counters + p);
         std::swap(a, b);
-        if (VAL) std::swap(va, vb);
+        cur = dstc;
     }
+    if (final) *final = cur;
 }
 
 }  // namespace
@@ -240,33 +274,51 @@ int radix_passes(int lo_bit, int hi_bit) {
     return (hi_bit - lo_bit + RBITS - 1) / RBITS;
 }
 
-static void sort_impl(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, uint32_t *&va,
-                      uint32_t *&vb, bool val, uint64_t n, int lo_bit, int hi_bit,
-                      uint32_t *d_hist, bool hist_ready);
+static uint64_t *sort_impl(vd_ctx *ctx, uint64_t *a, uint64_t *b, int m,
+                           const RadixCols &src, const RadixCols &pa, const RadixCols &pb,
+                           RadixCols *final, uint64_t n, int lo_bit, int hi_bit,
+                           uint32_t *d_hist, bool hist_ready);
 
 uint64_t *radix_sort_u64(vd_ctx *ctx, uint64_t *keys, uint64_t *tmp,
                          uint64_t n, int lo_bit, int hi_bit, uint32_t *d_hist,
                          bool hist_ready) {
-    uint64_t *a = keys, *b = tmp;
-    uint32_t *va = nullptr, *vb = nullptr;
-    sort_impl(ctx, a, b, va, vb, false, n, lo_bit, hi_bit, d_hist, hist_ready);
-    return a;
+    RadixCols none{};
+    return sort_impl(ctx, keys, tmp, 0, none, none, none, nullptr, n, lo_bit, hi_bit, d_hist,
+                     hist_ready);
 }
 
 void radix_sort_u64_kv(vd_ctx *ctx, uint64_t *&keys, uint64_t *&ktmp, uint32_t *&vals,
                        uint32_t *&vtmp, uint64_t n, int lo_bit, int hi_bit,
                        uint32_t *d_hist, bool hist_ready) {
-    sort_impl(ctx, keys, ktmp, vals, vtmp, true, n, lo_bit, hi_bit, d_hist, hist_ready);
+    RadixCols src{}, pa{}, pb{}, fin{};
+    src.p[0] = pa.p[0] = vals;
+    pb.p[0] = vtmp;
+    uint64_t *k0 = keys, *k1 = ktmp;
+    uint64_t *sorted = sort_impl(ctx, k0, k1, 1, src, pa, pb, &fin, n, lo_bit, hi_bit, d_hist,
+                                 hist_ready);
+    if (sorted != keys) std::swap(keys, ktmp);
+    if (fin.p[0] && fin.p[0] != vals) std::swap(vals, vtmp);
 }
 
-static void sort_impl(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, uint32_t *&va,
-                      uint32_t *&vb, bool val, uint64_t n, int lo_bit, int hi_bit,
-                      uint32_t *d_hist, bool hist_ready) {
+uint64_t *radix_sort_u64_payload(vd_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n,
+                                 int lo_bit, int hi_bit, uint32_t *d_hist, bool hist_ready,
+                                 int m, const RadixCols &src, const RadixCols &pa,
+                                 const RadixCols &pb, RadixCols *final) {
+    VD_REQUIRE(m >= 0 && m <= RADIX_MAX_PAYLOAD, VD_EINVAL, "payload columns");
+    return sort_impl(ctx, keys, tmp, m, src, pa, pb, final, n, lo_bit, hi_bit, d_hist,
+                     hist_ready);
+}
+
+static uint64_t *sort_impl(vd_ctx *ctx, uint64_t *a, uint64_t *b, int m,
+                           const RadixCols &src, const RadixCols &pa, const RadixCols &pb,
+                           RadixCols *final, uint64_t n, int lo_bit, int hi_bit,
+                           uint32_t *d_hist, bool hist_ready) {
     uint64_t *keys = a;
     const int npass = radix_passes(lo_bit, hi_bit);
     VD_REQUIRE(lo_bit >= 0 && hi_bit <= 64, VD_EINVAL, "radix bit range");
     VD_REQUIRE(npass <= radix::MAX_PASSES, VD_EINVAL, "too many radix passes");
-    if (n <= 1 || npass == 0) return;
+    if (final) *final = src;
+    if (n <= 1 || npass == 0) return a;
     VD_REQUIRE(n <= (uint64_t)LB_VAL, VD_ERANGE,
                "radix sort supports at most 2^30-1 keys per call");
     if (!hist_ready) {
@@ -282,20 +334,34 @@ static void sort_impl(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, uint32_t *&va,
            k_rs_scan_hist, (const uint32_t *)d_hist, gofs.p);
     // Tile shapes (threads x keys per thread), chosen by the radix_items
     // tunable: 16 -> 512 x 16 (default), 8 -> 512 x 8, 24 -> 256 x 24 (keys
-    // only); values need LDS staging too, so at most 16 keys per thread.
+    // only); payload columns need LDS staging too, so at most 16 keys per
+    // thread with them.
     const int items = ctx->radix_items;
-    if (val) {
-        if (items == 8)
-            run_passes<512, 8, true>(ctx, a, b, va, vb, n, lo_bit, hi_bit, npass, gofs.p);
-        else
-            run_passes<512, 16, true>(ctx, a, b, va, vb, n, lo_bit, hi_bit, npass, gofs.p);
-        return;
+    const uint32_t *g = gofs.p;
+#define VD_RUN(BL, IT, MM) run_passes<BL, IT, MM>(ctx, a, b, src, pa, pb, final, n, lo_bit, hi_bit, npass, g)
+    if (m > 0) {
+        if (items == 8) {
+            switch (m) {
+                case 1: VD_RUN(512, 8, 1); break;
+                case 2: VD_RUN(512, 8, 2); break;
+                default: VD_RUN(512, 8, 3); break;
+            }
+        } else {
+            switch (m) {
+                case 1: VD_RUN(512, 16, 1); break;
+                case 2: VD_RUN(512, 16, 2); break;
+                default: VD_RUN(512, 16, 3); break;
+            }
+        }
+        return a;
     }
     switch (items) {
-        case 8: run_passes<512, 8, false>(ctx, a, b, va, vb, n, lo_bit, hi_bit, npass, gofs.p); break;
-        case 24: run_passes<256, 24, false>(ctx, a, b, va, vb, n, lo_bit, hi_bit, npass, gofs.p); break;
-        default: run_passes<512, 16, false>(ctx, a, b, va, vb, n, lo_bit, hi_bit, npass, gofs.p); break;
+        case 8: VD_RUN(512, 8, 0); break;
+        case 24: VD_RUN(256, 24, 0); break;
+        default: VD_RUN(512, 16, 0); break;
     }
+#undef VD_RUN
+    return a;
 }
 
 }  // namespace vd

@@ -5,6 +5,8 @@
 // (LSD passes are stable, so equal keys keep their original order), with a
 // 32-bit value for a third column or the row id of wider relations.
 
+#include <vector>
+
 #include "vd_radix_dev.hpp"
 
 namespace vd {
@@ -69,10 +71,10 @@ __global__ __launch_bounds__(RB) void k_pack_c01(
     radix::hist_flush(&s, ghist, npass);
 }
 
-// out0 = key >> 32, out1 = low word of the key; three columns: out2 = val;
-// more: out_c = in_c[val] for c >= 2 (val is the source row).
+// out0 = key >> 32, out1 = low word of the key; with rows, out_c =
+// in_c[rows[i]] for c >= 2 (rows = the source row of each sorted key).
 __global__ __launch_bounds__(256) void k_unpack_c01(const uint64_t *__restrict__ keys,
-                                                    const uint32_t *__restrict__ vals,
+                                                    const uint32_t *__restrict__ rows,
                                                     uint64_t n, Cols cols) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -80,11 +82,9 @@ __global__ __launch_bounds__(256) void k_unpack_c01(const uint64_t *__restrict__
         const uint64_t k = keys[i];
         cols.out[0][i] = (uint32_t)(k >> 32);
         if (cols.n >= 2) cols.out[1][i] = (uint32_t)k;
-        if (cols.n == 3) {
-            cols.out[2][i] = vals[i];
-        } else if (cols.n > 3) {
-            const uint32_t row = vals[i];
-            for (int c = 2; c < cols.n; ++c) cols.out[c][i] = cols.in[c][row];
+        if (rows) {
+            const uint32_t r = rows[i];
+            for (int c = 2; c < cols.n; ++c) cols.out[c][i] = cols.in[c][r];
         }
     }
 }
@@ -174,11 +174,11 @@ uint64_t *sort_by_col(vd_ctx *ctx, const uint32_t *col, const uint32_t *perm,
 }
 
 // Stable sort by column 0.  Column 1 rides in the low word of the 64-bit key
-// (only the bits of column 0 are sorted, so equal keys keep their order);
-// column 2 of a three-column relation rides as the 32-bit value, and wider
-// relations carry the row id as the value and gather columns 2.. once.
-// Columns 0 and 1 (and 2 of a three-column relation) are rewritten in place
-// unless they are borrowed.
+// (only the bits of column 0 are sorted, so equal keys keep their order).
+// Columns 2..4 travel through the radix passes as payload columns (read from
+// the relation by the first pass); wider relations carry the row id as the
+// payload and gather columns 2.. once at the end.  Owned columns are reused
+// as buffers; borrowed (view) columns are replaced, never written.
 void sort_col0(vd_ctx *ctx, vd_rel *rel) {
     const uint64_t n = rel->nrows;
     const int nc = rel->ncols;
@@ -187,47 +187,76 @@ void sort_col0(vd_ctx *ctx, vd_rel *rel) {
     const int b = std::max(1, bits_for(max_u32(ctx, rel->cols[0], n)));
     const int hi = 32 + b;
     const int npass = radix_passes(32, hi);
-    const bool kv = nc >= 3;
+    const bool by_row = nc > 2 + RADIX_MAX_PAYLOAD;
+    const int m = nc <= 2 ? 0 : (by_row ? 1 : nc - 2);
     DBuf<uint64_t> keys(ctx, n), ktmp(ctx, n);
-    DBuf<uint32_t> vals(ctx, kv ? n : 0), vtmp(ctx, kv ? n : 0);
+    DBuf<uint32_t> rows(ctx, by_row ? n : 0);
     DBuf<uint32_t> hist(ctx, radix_hist_words(32, hi));
     fill_u32(ctx, hist.p, 0, radix_hist_words(32, hi));
-    const uint32_t *vsrc = nc == 3 ? rel->cols[2] : nullptr;
-    launch(ctx, "sort_pack", (double)n * (8.0 + 4.0 * std::min(nc, 2) + (kv ? 4.0 : 0.0) +
-                                          (nc == 3 ? 4.0 : 0.0)),
+    launch(ctx, "sort_pack", (double)n * (8.0 + 4.0 * std::min(nc, 2) + (by_row ? 4.0 : 0.0)),
            dim3((unsigned)ceil_div(n, RB * R_ITEMS)), dim3(RB), 0, k_pack_c01,
            (const uint32_t *)rel->cols[0], (const uint32_t *)(nc >= 2 ? rel->cols[1] : nullptr),
-           vsrc, n, keys.p, kv ? vals.p : nullptr, hi, npass, hist.p);
-    uint64_t *k = keys.p, *kt = ktmp.p;
-    uint32_t *v = vals.p, *vt = vtmp.p;
-    if (kv)
-        radix_sort_u64_kv(ctx, k, kt, v, vt, n, 32, hi, hist.p, true);
-    else
-        k = radix_sort_u64(ctx, k, kt, n, 32, hi, hist.p, true);
-    Cols cols{};
-    cols.n = nc;
-    uint32_t *fresh[VD_MAX_COLS] = {};
+           (const uint32_t *)nullptr, n, keys.p, by_row ? rows.p : nullptr, hi, npass, hist.p);
+
+    // Payload buffers: src = the relation's columns (or the row ids); pa
+    // aliases src where the relation owns the column, pb is fresh.
+    std::vector<uint32_t *> fresh;
+    auto fresh_col = [&]() {
+        fresh.push_back(nullptr);
+        fresh.back() = (uint32_t *)dalloc(ctx, n * 4);
+        return fresh.back();
+    };
+    RadixCols src{}, pa{}, pb{}, fin{};
+    uint32_t *out01[2] = {nullptr, nullptr};
+    uint32_t *gathered[VD_MAX_COLS] = {};
+    uint64_t *sorted = nullptr;
     try {
-        for (int c = 0; c < nc; ++c) {
-            const bool inplace = c < 2 || (c == 2 && nc == 3);
-            if (inplace && !rel_borrowed(rel, c)) {
-                cols.out[c] = rel->cols[c];
-                continue;
+        for (int c = 0; c < m; ++c) {
+            if (by_row) {
+                src.p[0] = pa.p[0] = rows.p;
+            } else {
+                src.p[c] = rel->cols[2 + c];
+                pa.p[c] = rel_borrowed(rel, 2 + c) ? fresh_col() : rel->cols[2 + c];
             }
-            fresh[c] = (uint32_t *)dalloc(ctx, n * 4);
-            cols.in[c] = rel->cols[c];
-            cols.out[c] = fresh[c];
+            pb.p[c] = fresh_col();
         }
+        sorted = radix_sort_u64_payload(ctx, keys.p, ktmp.p, n, 32, hi, hist.p, true, m, src,
+                                        pa, pb, &fin);
+        for (int c = 0; c < std::min(nc, 2); ++c)
+            out01[c] = rel_borrowed(rel, c) ? fresh_col() : rel->cols[c];
+        if (by_row)
+            for (int c = 2; c < nc; ++c) gathered[c] = fresh_col();
     } catch (...) {
         for (auto p : fresh) dfree(ctx, p);
         throw;
     }
-    const double gathered = nc > 3 ? 4.0 + 8.0 * (nc - 2) : (nc == 3 ? 8.0 : 0.0);
-    launch(ctx, "sort_unpack", (double)n * (8.0 + 4.0 * std::min(nc, 2) + gathered),
-           dim3(sgrid(n)), dim3(256), 0, k_unpack_c01, (const uint64_t *)k,
-           (const uint32_t *)v, n, cols);
-    for (int c = 0; c < nc; ++c)
-        if (fresh[c]) rel_set_col(ctx, rel, c, fresh[c]);
+    Cols cols{};
+    cols.n = nc;
+    for (int c = 0; c < nc; ++c) cols.in[c] = rel->cols[c];
+    cols.out[0] = out01[0];
+    if (nc >= 2) cols.out[1] = out01[1];
+    if (by_row)
+        for (int c = 2; c < nc; ++c) cols.out[c] = gathered[c];
+    launch(ctx, "sort_unpack",
+           (double)n * (8.0 + 4.0 * std::min(nc, 2) + (by_row ? 4.0 + 8.0 * (nc - 2) : 0.0)),
+           dim3(sgrid(n)), dim3(256), 0, k_unpack_c01, (const uint64_t *)sorted,
+           (const uint32_t *)(by_row ? fin.p[0] : nullptr), n, cols);
+    // Install the new columns; free the scratch that is not installed.
+    auto installed = [&](uint32_t *p) {
+        for (int c = 0; c < nc; ++c)
+            if (rel->cols[c] == p) return true;
+        return false;
+    };
+    for (int c = 0; c < std::min(nc, 2); ++c)
+        if (out01[c] != rel->cols[c]) rel_set_col(ctx, rel, c, out01[c]);
+    if (by_row) {
+        for (int c = 2; c < nc; ++c) rel_set_col(ctx, rel, c, gathered[c]);
+    } else {
+        for (int c = 0; c < m; ++c)
+            if (fin.p[c] != rel->cols[2 + c]) rel_set_col(ctx, rel, 2 + c, fin.p[c]);
+    }
+    for (auto p : fresh)
+        if (!installed(p)) dfree(ctx, p);
 }
 
 void sort_lex(vd_ctx *ctx, vd_rel *rel) {
