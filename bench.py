@@ -32,7 +32,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from vedas_amd.workload import JOIN_SET, STORE_100M, query_mix  # noqa: E402
+from vedas_amd.workload import JOIN_SET, STORE_100M, STORE_500M, query_mix  # noqa: E402
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 GEN = {k: v for k, v in STORE_100M.items() if k != "n"}
@@ -212,30 +212,82 @@ def serve_bench(eng, texts, qps, seed=1, slots=4, max_batch=64, budget=96 << 30)
     return out
 
 
-def cpu_baseline(idx, sample_seconds: float):
-    """The oracle's join (C, one thread) on the first left rows of the
-    star4 query's first join (?s 30 ?a join ?s 31 ?b on the same store), so
-    the CPU does the same kind of work; rows/s over that prefix."""
-    from oracle import pyoracle as O
-    rels = []
-    for pred in (30, 31):
-        b, e = idx.lookup("PSO", pred)
-        r = idx.select("PSO", b, e, 3)
-        rels.append(r.columns())
+def serve_section(store: dict, loads, n_queries: int, seed: int, check: int = 32):
+    """The served form: a fresh engine over its own store (all permutations
+    resident), the seeded query mix drawn on that store, served at each
+    offered load; then `check` requests compared canonically between the
+    served mix and the same query run alone."""
+    from oracle.pyoracle import canonical  # checker only
+    from vedas_amd import _native
+    from vedas_amd.engine import Engine, query_text
+    prm = {k: v for k, v in store.items() if k != "n"}
+    log(f"[bench] serving store: generating {store['n']} triples")
+    s, p, o = _native.gen_triples(store["n"], seed=seed, distinct=False, **prm)
+    eng = Engine(0)
+    ctx = _native.Ctx.wrap(eng.ctx_handle())
+    ds, dp, do = ctx.upload(s), ctx.upload(p), ctx.upload(o)
+    t0 = time.perf_counter()
+    eng.load_device(ds, dp, do, len(s), ("SPO", "PSO", "POS", "OSP"))
+    ctx.sync()
+    build_ms = (time.perf_counter() - t0) * 1e3
+    for d in (ds, dp, do):
+        ctx.free(d)
+    kbits = int(s.max()).bit_length() + int(p.max()).bit_length() + int(o.max()).bit_length()
+    log(f"[bench] serving store built in {build_ms:.0f} ms ({kbits}-bit keys)")
+    mix = query_mix(n_queries, seed + 1, store, s, p, o)
+    del s, p, o
+    texts = [query_text(pats, node) for _, pats, node in mix]
+    for r in eng.batch(texts[:16])[0]:  # warm-up
         r.free()
-    L, R = rels if len(rels[0][0]) <= len(rels[1][0]) else rels[::-1]
-    m = min(len(L[0]), 2000)
-    while True:
+    loads_out = []
+    for q in loads:
+        log(f"[bench] serving at {q:.0f} offered q/s")
+        sv = serve_bench(eng, texts, q)
+        loads_out.append(sv)
+        log(f"[bench]   plain {sv['plain']['qps']:.0f} q/s p50 {sv['plain']['p50_ms']:.1f} "
+            f"p99 {sv['plain']['p99_ms']:.1f} ms; served {sv['served']['qps']:.0f} q/s "
+            f"p50 {sv['served']['p50_ms']:.1f} p99 {sv['served']['p99_ms']:.1f} ms")
+    # same result alone and in the served mix
+    rng = np.random.default_rng(seed + 2)
+    arr = list(np.cumsum(rng.exponential(1000.0 / max(loads), len(texts))))
+    _, outs = eng.serve(texts, arr, keep_results=True)
+    same = 0
+    for i in range(min(check, len(texts))):
+        a = eng.query(texts[i])
+        same += canonical(a.headers(), a.columns()) == canonical(outs[i].headers(),
+                                                                  outs[i].columns())
+        a.free()
+    for r in outs:
+        r.free()
+    eng.close()
+    return {"store": {**store, "seed": seed, "key_bits": kbits}, "build_ms": build_ms,
+            "queries": len(texts), "loads": loads_out,
+            "results_equal_alone": f"{same}/{min(check, len(texts))}"}
+
+
+def cpu_baseline(idx, names, sample_seconds: float):
+    """The join set on the host: the oracle's plan walk (C restatement of
+    select / join / IndexSwap, one thread) over the PSO and POS arrays of
+    the same store (downloaded from the device build, since the index build
+    is not part of the timed step).  Queries run in order until the sample
+    reaches sample_seconds (at least one query); join rows/s over them."""
+    from oracle import pyoracle as O
+    ix = {perm: idx.arrays(perm) for perm in ("PSO", "POS")}
+    done, rows, el = [], 0, 0.0
+    for q in names:
+        pats, node = JOIN_SET[q]
+        st = {}
         t0 = time.perf_counter()
-        out = O.join([c[:m] for c in L], R, 1)
-        el = time.perf_counter() - t0
-        if el >= sample_seconds / 4 or m >= len(L[0]):
+        O.run_plan(ix, pats, node, st)
+        el += time.perf_counter() - t0
+        rows += st["join_rows"]
+        done.append(f"{q} ({st['join_rows']} join rows)")
+        log(f"[bench] cpu baseline: {q} done, {el:.1f} s so far")
+        if el >= sample_seconds:
             break
-        m = min(len(L[0]), m * 4)
-    rows = len(out[0])
     return {"value": rows / max(el, 1e-9), "unit": "join rows/s", "cores": 1, "kind": "port",
-            "sample": f"oracle or_join (1 thread) of the first {m} left rows of star4's first "
-                      f"join ({len(L[0])} x {len(R[0])} rows): {rows} output rows in {el:.2f} s"}
+            "sample": "oracle plan walk (C restatement, 1 thread) of " + ", ".join(done) +
+                      f" on the same 100M-triple store: {rows} join rows in {el:.1f} s"}
 
 
 def main():
@@ -245,13 +297,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--triples", type=int, default=STORE_100M["n"])
     ap.add_argument("--seed", type=int, default=42)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--queries", default="star4,chain4,snowflake6")
     ap.add_argument("--selects", type=int, default=10_000)
     ap.add_argument("--mix", type=int, default=256, help="queries in the batched mix (0: skip)")
-    ap.add_argument("--serve-qps", type=float, nargs="*", default=[2000.0, 20000.0],
+    ap.add_argument("--serve-qps", type=float, nargs="*", default=[2000.0, 50000.0],
                     help="offered loads of the open-loop arrival stream (none: skip)")
+    ap.add_argument("--serve-requests", type=int, default=1024,
+                    help="requests in the served arrival stream")
+    ap.add_argument("--serve-store", choices=["500M", "100M"], default="500M",
+                    help="store of the served form")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -358,20 +414,16 @@ def main():
             r.free()
         batch = batch_bench(eng, mtexts)
         log(f"[bench] mix: plain {batch['plain_qps']:.0f} q/s, batched {batch['batched_qps']:.0f} q/s")
-        if args.serve_qps:
-            serve = []
-            for q in args.serve_qps:
-                log(f"[bench] serving at {q:.0f} offered q/s")
-                sv = serve_bench(eng, mtexts, q)
-                serve.append(sv)
-                log(f"[bench]   plain {sv['plain']['qps']:.0f} q/s p50 {sv['plain']['p50_ms']:.1f} "
-                    f"p99 {sv['plain']['p99_ms']:.1f} ms; served {sv['served']['qps']:.0f} q/s "
-                    f"p50 {sv['served']['p50_ms']:.1f} p99 {sv['served']['p99_ms']:.1f} ms")
 
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline")
-        cpu = cpu_baseline(idx_view, args.cpu_seconds)
+        cpu = cpu_baseline(idx_view, names, args.cpu_seconds)
+
+    if args.serve_qps:
+        eng.close()  # the served form runs on its own store
+        serve = serve_section(STORE_500M if args.serve_store == "500M" else STORE_100M,
+                              args.serve_qps, args.serve_requests, args.seed + 100)
 
     queries = {}
     for q, sts in per_q.items():

@@ -133,6 +133,69 @@ def unique_adjacent(cols):
     return [c[:n].copy() for c in cols]
 
 
+# ---- plan walk ----------------------------------------------------------------
+def run_plan(ix_by_perm: dict, patterns, node, stats: dict | None = None):
+    """Execute a plan tree over the oracle: QueryExecutor::postorderTraversal
+    (QueryExecutor.cu:106-135) with the index choice of
+    create1VarSelectQueryJob / create2VarSelectQueryJob
+    (QueryExecutor.cu:781-971: 2-variable pattern -> PSO [?s, ?o] when the
+    node's variable is the subject, else POS [?o, ?s]; s p ?o -> PSO (p, s);
+    ?s p o -> POS (p, o)) and the join rules of JoinQueryJob::join
+    (JoinQueryJob.cu:125-180: smaller relation left, key moved to column 0
+    with a re-sort, output [key, left rest, right rest]).  Nodes:
+    ("UPLOAD", pattern or index into patterns, var), ("JOIN", l, r, var),
+    ("INDEXSWAP", child, var).  ix_by_perm: {"PSO": arrays, "POS": arrays}
+    as build_perm returns.  stats (optional) accumulates "join_rows" and
+    "swap_rows".  Returns (headers, columns)."""
+    st = stats if stats is not None else {}
+    st.setdefault("join_rows", 0)
+    st.setdefault("swap_rows", 0)
+
+    def key_first(h, cols, var):
+        i = h.index(var)
+        if i == 0:
+            return h, cols
+        h, cols = list(h), list(cols)
+        h[0], h[i] = h[i], h[0]
+        cols[0], cols[i] = cols[i], cols[0]
+        st["swap_rows"] += len(cols[0])
+        return h, sort_col0(cols)
+
+    def walk(nd):
+        op = nd[0]
+        if op == "UPLOAD":
+            pat = patterns[nd[1]] if isinstance(nd[1], int) else nd[1]
+            sv, pv, ov = pat
+            var = nd[2]
+            if isinstance(sv, str) and isinstance(ov, str):
+                perm, hdr = ("PSO", [sv, ov]) if sv == var else ("POS", [ov, sv])
+                ix = ix_by_perm[perm]
+                b, e = lookup(ix, pv)
+                return hdr, select(ix, b, e, 3)
+            if isinstance(sv, str):
+                ix = ix_by_perm["POS"]
+                b, e = lookup(ix, pv, ov)
+                return [sv], select(ix, b, e, 1)
+            ix = ix_by_perm["PSO"]
+            b, e = lookup(ix, pv, sv)
+            return [ov], select(ix, b, e, 1)
+        if op == "JOIN":
+            a, b = walk(nd[1]), walk(nd[2])
+            if len(a[1][0]) > len(b[1][0]):
+                a, b = b, a
+            ha, ca = key_first(a[0], a[1], nd[3])
+            hb, cb = key_first(b[0], b[1], nd[3])
+            out = join(ca, cb, 1)
+            st["join_rows"] += len(out[0])
+            return [nd[3]] + ha[1:] + hb[1:], out
+        if op == "INDEXSWAP":
+            h, cols = walk(nd[1])
+            return key_first(h, cols, nd[2])
+        raise ValueError(op)
+
+    return walk(node)
+
+
 # ---- naive evaluator -----------------------------------------------------
 def naive_bgp(s, p, o, patterns):
     """Bag of bindings of a basic graph pattern by nested loops over the

@@ -166,6 +166,27 @@ def test_plan_equals_naive_bgp(shape):
     assert len(rows) > 0
 
 
+@pytest.mark.parametrize("shape", ["star3", "chain3", "snow", "bound"])
+def test_run_plan_equals_naive_bgp(shape):
+    """The oracle plan walk used by the benchmark's CPU baseline."""
+    s, p, o = small_store()
+    if shape == "star3":
+        pats, node = plans.star_plan([1, 2, 3])
+    elif shape == "chain3":
+        pats, node = plans.chain_plan([1, 2, 3])
+    elif shape == "snow":
+        pats, node = plans.snowflake_plan([1, 2], [3, 4])
+    else:
+        pats = [("?s", int(p[0]), int(o[0])), ("?s", 2, "?y"), (int(s[3]), 3, "?s")]
+        node = ("JOIN", ("JOIN", ("UPLOAD", 0, "?s"), ("UPLOAD", 1, "?s"), "?s"),
+                ("UPLOAD", 2, "?s"), "?s")
+    ix = {perm: O.build_perm(s, p, o, perm) for perm in ("PSO", "POS")}
+    st = {}
+    hdr, cols = O.run_plan(ix, pats, node, st)
+    assert O.canonical(hdr, cols) == O.naive_bgp(s, p, o, pats)
+    assert st["join_rows"] >= len(cols[0])
+
+
 def test_bound_constant_patterns():
     s, p, o = small_store()
     obj = int(o[0])

@@ -18,6 +18,13 @@ import numpy as np
 STORE_100M = dict(n=100_000_000, n_entities=10_000_000, subject_zipf=0.8, n_predicates=256,
                   pred_zipf=1.1, literal_frac=0.4, literal_base=50_000_000,
                   n_literals=50_000_000)
+# 500M-triple store of the served form: 50M entities, 1024 predicates.
+# Literal IDs start above every entity ID (> 5e7) and, 5:1 to entities as
+# in the other stores, reach 3.1e8: a packed (P, S, O) key then needs
+# 11 + 26 + 29 = 66 bits, so this store exercises the two-phase build.
+STORE_500M = dict(n=500_000_000, n_entities=50_000_000, subject_zipf=0.8, n_predicates=1024,
+                  pred_zipf=1.1, literal_frac=0.4, literal_base=60_000_000,
+                  n_literals=250_000_000)
 # 1M-triple store of the CPU-runnable case
 STORE_1M = dict(n=1_000_000, n_entities=100_000, subject_zipf=0.0, n_predicates=64,
                 pred_zipf=1.1, literal_frac=0.5, literal_base=50_000_000, n_literals=500_000)
@@ -176,10 +183,15 @@ def query_mix(n_queries: int, seed: int, prm: dict, s=None, p=None, o=None,
             preds = [int(x) for x in rng.integers(1, P + 1, L)]
             sizes = m.chain(preds)
             if bound:
-                cand = np.flatnonzero(p == preds[-1])
-                if len(cand) == 0 or max(sizes[:-1]) > hi:
+                if max(sizes[:-1]) > hi:
                     continue
-                oc = int(o[cand[int(rng.integers(0, len(cand)))]])
+                # an existing object of the last predicate (random probes;
+                # a rare predicate is redrawn)
+                t = rng.integers(0, len(p), 4096)
+                hit = t[p[t] == preds[-1]]
+                if len(hit) == 0:
+                    continue
+                oc = int(o[hit[0]])
                 pats, node = _chain_query(preds, oc)
                 accept((f"chain{L}b", pats, node))
                 continue
