@@ -299,6 +299,7 @@ def main():
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default kernel tunables")
     ap.add_argument("--queries", default="star4,chain4,snowflake6")
     ap.add_argument("--selects", type=int, default=10_000)
     ap.add_argument("--mix", type=int, default=256, help="queries in the batched mix (0: skip)")
@@ -368,6 +369,12 @@ def main():
                 collect.setdefault(q, []).append(st)
         return rows
 
+    tune = None
+    if not args.no_tune:
+        log("[bench] autotuning kernel tunables on the join set")
+        tune = eng.autotune([texts[q] for q in names], reps=2)
+        log(f"[bench] autotune: radix_items {tune['radix_items']} join_nv {tune['join_nv']} "
+            f"join_rcap {tune['join_rcap']} ({tune['base_ms']:.2f} -> {tune['tuned_ms']:.2f} ms)")
     for w in range(args.warmup):
         log(f"[bench] warmup {w}")
         step()
@@ -450,6 +457,7 @@ def main():
                         "alg_bytes": build_alg, "GB/s": build_alg / build_s / 1e9,
                         "frac": build_alg / build_s / HBM_PEAK, "keys_per_s": 4 * n / build_s,
                         "kernels": kernel_table(build_prof)},
+        "autotune": tune,
         "range_selects": sel,
         "batched_mix": batch,
         "serving": serve,

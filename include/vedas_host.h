@@ -51,6 +51,21 @@ vd_index *vdq_engine_index(vdq_engine *e);
 
 int vdq_query(vdq_engine *e, const char *text, vdq_result **out);
 
+/* Autotuner (no counterpart in the reference, whose kernel shapes are
+ * fixed by ModernGPU and Thrust): times the given queries under each
+ * candidate of the kernel tunables -- radix tile shape ("radix_items"),
+ * merge-path diagonal ("join_nv"), LDS right-key capacity ("join_rcap"),
+ * see vd_ctx_tune -- by coordinate descent, keeps the fastest and applies
+ * it to the engine's contexts (serving slots included).  Results do not
+ * depend on the tunables. */
+typedef struct {
+    int radix_items, join_nv, join_rcap;  /* chosen values               */
+    double base_ms;                       /* queries with the defaults    */
+    double tuned_ms;                      /* queries with the chosen set  */
+    int evaluations;
+} vdq_tune_t;
+int vdq_autotune(vdq_engine *e, const char *const *texts, int n, int reps, vdq_tune_t *out);
+
 /* Many queries at once through the batched operators (no counterpart in
  * the reference): all plans advance together in rounds, one batched call
  * per operator kind per round.  outs[i] is query i's result, equal to

@@ -102,3 +102,34 @@ def test_cfg0_star_query():
     want = O.canonical(rel.headers, ex.columns(rel))
     assert got == want
     assert st["join_rows"] == len(want[1]) and 1e5 < st["join_rows"] < 2e6
+
+
+def test_autotune_keeps_results(small):
+    """The autotuner only picks kernel shapes: chosen values are valid
+    candidates and results stay the same after tuning (and in a serving
+    slot created afterwards)."""
+    from vedas_amd.engine import query_text
+    s, p, o, e = small
+    shapes = [plans.star_plan([1, 2, 3]), plans.chain_plan([1, 2, 3]),
+              plans.snowflake_plan([1, 2], [3, 4])]
+    texts = [query_text(pats, node) for pats, node in shapes]
+    before = []
+    for t in texts:
+        r = e.query(t)
+        before.append(O.canonical(r.headers(), r.columns()))
+        r.free()
+    tr = e.autotune(texts, reps=1)
+    assert tr["radix_items"] in (8, 16, 24)
+    assert tr["join_nv"] in (1024, 2048, 4096)
+    assert tr["join_rcap"] in (2048, 4096, 8192)
+    assert tr["evaluations"] >= 7 and tr["tuned_ms"] <= tr["base_ms"] + 1e-9
+    for t, want in zip(texts, before):
+        r = e.query(t)
+        assert O.canonical(r.headers(), r.columns()) == want
+        r.free()
+    recs, outs = e.serve(texts, [0.0] * len(texts), slots=2, max_batch=2, keep_results=True)
+    for res, want in zip(outs, before):
+        assert O.canonical(res.headers(), res.columns()) == want
+        res.free()
+    for (pats, _), want in zip(shapes, before):
+        assert want == O.naive_bgp(s, p, o, pats)

@@ -187,6 +187,13 @@ def test_select_views_and_lifetime(ctx):
     r4 = idx.select("PSO", b, e, 3)  # the sort above left the index as it was
     for g, w in zip(r4.columns(), want):
         assert np.array_equal(g, w)
+    ys = want[0]
+    lo, hi = int(np.percentile(ys, 25)), int(np.percentile(ys, 75))
+    r6 = idx.select("PSO", b, e, 3, clip=(lo, hi))  # one run of rows: a view
+    assert r6.is_view(0) and r6.is_view(1)
+    for g, w in zip(r6.columns(), O.select(ix, b, e, 3, clip=(lo, hi))):
+        assert np.array_equal(g, w)
+    r6.free()
     r5 = idx.select("PSO", b, e, 2)
     r5.unique_adjacent()
     assert np.array_equal(r5.columns()[0], O.unique_adjacent([want[0]])[0])

@@ -12,7 +12,6 @@
 #include <vector>
 
 #include "vd_index.hpp"
-#include "vd_join_dev.hpp"
 #include "vd_radix_dev.hpp"
 
 namespace vd {
@@ -86,17 +85,10 @@ __global__ __launch_bounds__(256) void k_join_ranges_multi(const JPairDev *P, in
     ranges[2 * b + 1] = dev::upper_bound(rk, lo, D.nr, lk[last]);
 }
 
-__device__ __forceinline__ uint32_t mb_slot(uint32_t r) { return r + (r >> 3); }
-constexpr int MB_SLOTS = MB_TILE + MB_TILE / 8;
-
-// As k_join_bounds (vd_join.hip): each thread resolves MB_ITEMS consecutive
-// rows of the tile by one binary search and galloping.
 __global__ __launch_bounds__(MB_BLOCK) void k_join_bounds_multi(
     const JPairDev *P, int np, const uint64_t *tbase, const uint64_t *ranges, uint32_t *lower,
     uint32_t *cnt) {
-    static_assert(2 * MB_SLOTS <= MJ_RCAP, "results reuse the right-key buffer");
     __shared__ uint32_t s_r[MJ_RCAP];
-    __shared__ uint32_t s_l[MB_SLOTS];
     __shared__ int s_j;
     if (threadIdx.x == 0) s_j = find_seg(tbase, np, blockIdx.x);
     __syncthreads();
@@ -104,39 +96,28 @@ __global__ __launch_bounds__(MB_BLOCK) void k_join_bounds_multi(
     const JPairDev &D = P[j];
     const uint32_t *lk = D.lc[0], *rk = D.rc[0];
     const uint64_t base = (blockIdx.x - tbase[j]) * (uint64_t)MB_TILE;
-    const uint32_t nvalid = (uint32_t)min((uint64_t)MB_TILE, D.nl - base);
+    const uint64_t nl = D.nl;
     const uint64_t rlo = ranges[2 * blockIdx.x], rhi = ranges[2 * blockIdx.x + 1], m = rhi - rlo;
     const bool staged = m <= (uint64_t)MJ_RCAP;
     if (staged)
         for (uint64_t t = threadIdx.x; t < m; t += MB_BLOCK) s_r[t] = rk[rlo + t];
-    for (uint32_t t = threadIdx.x; t < nvalid; t += MB_BLOCK) s_l[mb_slot(t)] = lk[base + t];
     __syncthreads();
-    const uint32_t r0 = threadIdx.x * MB_ITEMS;
-    uint32_t lo[MB_ITEMS], ct[MB_ITEMS];
-    struct Padded {
-        const uint32_t *p;
-        __device__ uint32_t operator[](uint32_t r) const { return p[mb_slot(r)]; }
-    } pl{s_l};
-    if (staged) {
-        const uint32_t *sr = s_r;
-        jdev::tile_bounds<MB_ITEMS, uint32_t>(pl, nvalid, r0, [sr](uint64_t i) { return sr[i]; },
-                                              m, lo, ct);
-    } else {
-        const uint32_t *rr = rk + rlo;
-        jdev::tile_bounds<MB_ITEMS, uint32_t>(pl, nvalid, r0, [rr](uint64_t i) { return rr[i]; },
-                                              m, lo, ct);
-    }
-    __syncthreads();
-    uint32_t *s_lo = s_r, *s_ct = s_r + MB_SLOTS;
 #pragma unroll
     for (int it = 0; it < MB_ITEMS; ++it) {
-        s_lo[mb_slot(r0 + it)] = (uint32_t)rlo + lo[it];
-        s_ct[mb_slot(r0 + it)] = ct[it];
-    }
-    __syncthreads();
-    for (uint32_t t = threadIdx.x; t < nvalid; t += MB_BLOCK) {
-        lower[D.lbase + base + t] = s_lo[mb_slot(t)];
-        cnt[D.lbase + base + t] = s_ct[mb_slot(t)];
+        const uint64_t i = base + (uint64_t)it * MB_BLOCK + threadIdx.x;
+        if (i >= nl) break;
+        const uint32_t key = lk[i];
+        uint64_t lo, hi;
+        if (staged) {
+            lo = dev::lower_bound(s_r, 0, m, key);
+            hi = dev::upper_bound(s_r, lo, m, key) + rlo;
+            lo += rlo;
+        } else {
+            lo = dev::lower_bound(rk, rlo, rhi, key);
+            hi = dev::upper_bound(rk, lo, rhi, key);
+        }
+        lower[D.lbase + i] = (uint32_t)lo;
+        cnt[D.lbase + i] = (uint32_t)(hi - lo);
     }
 }
 

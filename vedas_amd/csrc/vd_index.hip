@@ -655,6 +655,16 @@ int vd_select(vd_ctx *ctx, const vd_index *idx, int perm, uint64_t begin,
         sync(ctx);
         const uint64_t rows = hp.dst[hp.np];
         const int ocols = cols == 3 ? 2 : 1;
+        if (hp.np <= 1 && !distinct) {
+            // Clip and skips left one run of consecutive rows: a view.
+            const uint64_t src = hp.np == 1 ? hp.src[0] : begin;
+            uint32_t *vc[2];
+            int nc = 0;
+            if (cols & 2) vc[nc++] = pa.y + src;
+            if (cols & 1) vc[nc++] = pa.z + src;
+            *out = rel_view(nc, rows, vc, idx->hold);
+            return VD_OK;
+        }
         vd_rel *r = rel_new(ctx, ocols, rows);
         try {
             launch(ctx, "select_copy", (double)rows * 8.0 * ocols,

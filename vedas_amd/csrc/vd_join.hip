@@ -2,12 +2,10 @@
 //
 //  1. k_join_bounds : for every left row i, lower_i = first right row with
 //                     key >= L[i] and cnt_i = number of right rows with key
-//                     == L[i].  A block takes 2048 consecutive left rows;
-//                     the right range they can touch comes from
-//                     k_join_ranges and is staged in LDS when it fits
-//                     (otherwise read from global memory).  Each thread
-//                     resolves 8 consecutive rows by one binary search and
-//                     galloping from the previous answer.
+//                     == L[i].  A block takes 2048 consecutive left rows,
+//                     finds the right range they can touch with two global
+//                     binary searches, and stages that range in LDS when it
+//                     fits (otherwise it searches in global memory).
 //  2. scan          : off_i = exclusive scan of cnt_i (64-bit); the total is
 //                     the output size (read back once to allocate).
 //  3. k_join_partition / k_join_expand : load-balanced expansion.  The merge
@@ -24,7 +22,7 @@
 // columns [keys, left payload, right payload] -- the order the reference's
 // ModernGPU inner_join + gather produce (JoinQueryJob.cu:68-101, 262-380).
 
-#include "vd_join_dev.hpp"
+#include "vd_internal.hpp"
 
 namespace vd {
 namespace {
@@ -94,58 +92,39 @@ __global__ __launch_bounds__(256) void k_join_ranges(KeyCols L, uint64_t nl, Key
     ranges[2 * b + 1] = g_upper<NK>(R, lo, nr, KeyT<NK>::at(L, last));
 }
 
-// LDS slot of tile row r: one pad word per 8 rows, so the 8 consecutive
-// rows of each thread fall in different banks.
-__device__ __forceinline__ uint32_t jb_slot(uint32_t r) { return r + (r >> 3); }
-constexpr int JB_SLOTS = JB_TILE + JB_TILE / 8;
-
 template <int NK>
 __global__ __launch_bounds__(JB_BLOCK) void k_join_bounds(
     KeyCols L, uint64_t nl, KeyCols R, uint64_t nr, int rcap,
     const uint64_t *__restrict__ ranges, uint32_t *__restrict__ lower,
     uint32_t *__restrict__ cnt) {
     using T = typename KeyT<NK>::T;
-    constexpr int RCAP = JB_RCAP_BYTES / sizeof(T);
-    static_assert(2 * JB_SLOTS * 4 <= JB_RCAP_BYTES, "results reuse the right-key buffer");
-    __shared__ T s_r[RCAP];
-    __shared__ T s_l[JB_SLOTS];
+    __shared__ T s_r[JB_RCAP_BYTES / sizeof(T)];
     const uint64_t base = (uint64_t)blockIdx.x * JB_TILE;
-    const uint32_t nvalid = (uint32_t)min((uint64_t)JB_TILE, nl - base);
     const uint64_t rlo = ranges[2 * blockIdx.x], rhi = ranges[2 * blockIdx.x + 1];
     const uint64_t m = rhi - rlo;
-    const bool staged = m <= (uint64_t)rcap && m <= (uint64_t)RCAP;
-    if (staged)
-        for (uint64_t t = threadIdx.x; t < m; t += JB_BLOCK) s_r[t] = KeyT<NK>::at(R, rlo + t);
-    for (uint32_t t = threadIdx.x; t < nvalid; t += JB_BLOCK)
-        s_l[jb_slot(t)] = KeyT<NK>::at(L, base + t);
-    __syncthreads();
-    const uint32_t r0 = threadIdx.x * JB_ITEMS;
-    uint32_t lo[JB_ITEMS], ct[JB_ITEMS];
-    const T *sl = s_l;
-    struct Padded {  // the thread's rows through the padded layout
-        const T *p;
-        __device__ T operator[](uint32_t r) const { return p[jb_slot(r)]; }
-    } pl{sl};
+    const bool staged = m <= (uint64_t)rcap && m <= JB_RCAP_BYTES / sizeof(T);
     if (staged) {
-        const T *sr = s_r;
-        jdev::tile_bounds<JB_ITEMS, T>(pl, nvalid, r0, [sr](uint64_t i) { return sr[i]; }, m, lo,
-                                    ct);
-    } else {
-        jdev::tile_bounds<JB_ITEMS, T>(
-            pl, nvalid, r0, [R, rlo](uint64_t i) { return KeyT<NK>::at(R, rlo + i); }, m, lo, ct);
+        for (uint64_t t = threadIdx.x; t < m; t += JB_BLOCK)
+            s_r[t] = KeyT<NK>::at(R, rlo + t);
     }
-    __syncthreads();  // s_r is free: results go through it
-    uint32_t *s_lo = reinterpret_cast<uint32_t *>(s_r);
-    uint32_t *s_ct = s_lo + JB_SLOTS;
+    __syncthreads();
 #pragma unroll
     for (int j = 0; j < JB_ITEMS; ++j) {
-        s_lo[jb_slot(r0 + j)] = (uint32_t)rlo + lo[j];
-        s_ct[jb_slot(r0 + j)] = ct[j];
-    }
-    __syncthreads();
-    for (uint32_t t = threadIdx.x; t < nvalid; t += JB_BLOCK) {
-        lower[base + t] = s_lo[jb_slot(t)];
-        cnt[base + t] = s_ct[jb_slot(t)];
+        const uint64_t i = base + (uint64_t)j * JB_BLOCK + threadIdx.x;
+        if (i >= nl) break;
+        const T key = KeyT<NK>::at(L, i);
+        uint64_t lo, hi;
+        if (staged) {
+            lo = dev::lower_bound(s_r, 0, m, key);
+            hi = dev::upper_bound(s_r, lo, m, key);
+            lo += rlo;
+            hi += rlo;
+        } else {
+            lo = g_lower<NK>(R, rlo, rhi, key);
+            hi = g_upper<NK>(R, lo, rhi, key);
+        }
+        lower[i] = (uint32_t)lo;
+        cnt[i] = (uint32_t)(hi - lo);
     }
 }
 

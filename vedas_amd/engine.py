@@ -38,6 +38,11 @@ class ServeRec(C.Structure):
                 ("slot", C.c_int)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("radix_items", C.c_int), ("join_nv", C.c_int), ("join_rcap", C.c_int),
+                ("base_ms", C.c_double), ("tuned_ms", C.c_double), ("evaluations", C.c_int)]
+
+
 _lib = None
 
 
@@ -68,6 +73,7 @@ def lib() -> C.CDLL:
             "vdq_serve": (i, [vp, C.POINTER(C.c_char_p), C.POINTER(C.c_double), i,
                               C.POINTER(ServeCfg), C.POINTER(ServeRec), C.POINTER(vp)]),
             "vdq_estimate": (i, [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+            "vdq_autotune": (i, [vp, C.POINTER(C.c_char_p), i, i, C.POINTER(TuneResult)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(l, name)
@@ -171,6 +177,14 @@ class Engine:
         out = [{k: getattr(recs[i], k) for k, _ in ServeRec._fields_} for i in range(n)]
         res = [Result(C.c_void_p(outs[i])) for i in range(n)] if keep_results else None
         return out, res
+
+    def autotune(self, texts, reps: int = 2) -> dict:
+        """Pick the kernel tunables by timing `texts` (coordinate descent);
+        returns the chosen values and the before/after times."""
+        arr = (C.c_char_p * max(1, len(texts)))(*[t.encode() for t in texts])
+        out = TuneResult()
+        _check(lib().vdq_autotune(self.h, arr, len(texts), reps, C.byref(out)))
+        return {k: getattr(out, k) for k, _ in TuneResult._fields_}
 
     def estimate(self, text: str):
         r, b = C.c_double(), C.c_double()

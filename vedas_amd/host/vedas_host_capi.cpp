@@ -23,8 +23,16 @@ struct vdq_engine {
         if (!stats) stats.reset(new Statistics(storage.get()));
         return *stats;
     }
+    // Tunable values chosen by vdq_autotune; every context gets them.
+    std::vector<std::pair<std::string, int>> tuned;
+    void apply_tuned(VedasContext *c) {
+        for (auto &kv : tuned) vd_ctx_tune(c->raw(), kv.first.c_str(), kv.second);
+    }
     std::vector<VedasContext *> slots(int n) {
-        while ((int)slot_ctx.size() < n) slot_ctx.emplace_back(new VedasContext(ctx->device()));
+        while ((int)slot_ctx.size() < n) {
+            slot_ctx.emplace_back(new VedasContext(ctx->device()));
+            apply_tuned(slot_ctx.back().get());
+        }
         std::vector<VedasContext *> v;
         for (int i = 0; i < n; ++i) v.push_back(slot_ctx[i].get());
         return v;
@@ -350,6 +358,78 @@ int vdq_serve(vdq_engine *e, const char *const *texts, const double *arrival_ms,
             }
             for (int i = 0; i < n; ++i) outs[i] = rs[i].release();
         }
+    });
+}
+
+// Coordinate descent over the kernel tunables: each candidate value of one
+// tunable is timed on the caller's queries (best of `reps` runs of all of
+// them) with the others held; a value is kept when it is at least 2%
+// faster.  Two sweeps; the defaults are the starting point.
+int vdq_autotune(vdq_engine *e, const char *const *texts, int n, int reps, vdq_tune_t *out) {
+    if (!e || !texts || n <= 0 || !out) return fail(VD_EINVAL, "bad autotune arguments");
+    if (!e->storage) return fail(VD_EINVAL, "engine has no store loaded");
+    return guarded([&] {
+        std::vector<std::unique_ptr<ParsedQuery>> qs;
+        for (int i = 0; i < n; ++i) {
+            qs.emplace_back(new ParsedQuery());
+            parse_query(texts[i], *qs.back());
+        }
+        VedasContext *ctx = e->ctx.get();
+        auto set = [&](const char *key, int v) {
+            ctx->check(vd_ctx_tune(ctx->raw(), key, v), "vd_ctx_tune");
+        };
+        auto run_all = [&]() {
+            double best = 1e300;
+            for (int r = 0; r < std::max(1, reps); ++r) {
+                auto t0 = std::chrono::steady_clock::now();
+                for (auto &q : qs) {
+                    SparqlResult res;
+                    QueryExecutor ex(e->storage.get(), ctx);
+                    ex.query(q->pats, q->selected, q->root, res);
+                }
+                ctx->sync();
+                best = std::min(best, std::chrono::duration<double, std::milli>(
+                                          std::chrono::steady_clock::now() - t0).count());
+            }
+            return best;
+        };
+        struct Knob {
+            const char *key;
+            std::vector<int> values;  // values[0] = default
+            int cur;
+        };
+        std::vector<Knob> knobs = {{"radix_items", {16, 8, 24}, 16},
+                                   {"join_nv", {2048, 1024, 4096}, 2048},
+                                   {"join_rcap", {4096, 2048, 8192}, 4096}};
+        for (auto &k : knobs) set(k.key, k.cur);
+        run_all();  // warm-up
+        const double base = run_all();
+        double best = base;
+        int evals = 1;
+        for (int sweep = 0; sweep < 2; ++sweep) {
+            for (auto &k : knobs) {
+                for (int v : k.values) {
+                    if (v == k.cur) continue;
+                    set(k.key, v);
+                    const double t = run_all();
+                    ++evals;
+                    if (t < 0.98 * best) {
+                        best = t;
+                        k.cur = v;
+                    }
+                }
+                set(k.key, k.cur);
+            }
+        }
+        e->tuned.clear();
+        for (auto &k : knobs) e->tuned.emplace_back(k.key, k.cur);
+        for (auto &sc : e->slot_ctx) e->apply_tuned(sc.get());
+        out->radix_items = knobs[0].cur;
+        out->join_nv = knobs[1].cur;
+        out->join_rcap = knobs[2].cur;
+        out->base_ms = base;
+        out->tuned_ms = best;
+        out->evaluations = evals;
     });
 }
 
