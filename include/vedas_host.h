@@ -11,7 +11,9 @@
  *                                           variable the index is chosen for
  *          (J <var> <node> <node>)          join on var
  *          (S <var> <node>)                 index swap to var
- *   PLAN may be left out for a single pattern.
+ *   Without PLAN, the query graph planner (QueryGraph::generateQueryPlan,
+ *   QueryGraph.cu:64-279) builds the plan from each pattern's index-range
+ *   rows (the reference's estimate_rows, QueryExecutor.cu:666,758).
  */
 #ifndef VEDAS_HOST_H
 #define VEDAS_HOST_H
@@ -50,6 +52,12 @@ vd_ctx *vdq_engine_ctx(vdq_engine *e);
 vd_index *vdq_engine_index(vdq_engine *e);
 
 int vdq_query(vdq_engine *e, const char *text, vdq_result **out);
+
+/* The plan of a query text as a PLAN clause: the text's own, or the query
+ * graph planner's with est_rows[i] as pattern i's range rows (no device
+ * work).  Replaces QueryGraph(...).generateQueryPlan() (QueryGraph.cu:
+ * 38-74).  VD_ERANGE if the plan does not fit cap bytes. */
+int vdq_plan(const char *text, const uint64_t *est_rows, char *out, size_t cap);
 
 /* Autotuner (no counterpart in the reference, whose kernel shapes are
  * fixed by ModernGPU and Thrust): times the given queries under each

@@ -133,3 +133,33 @@ def test_autotune_keeps_results(small):
         res.free()
     for (pats, _), want in zip(shapes, before):
         assert want == O.naive_bgp(s, p, o, pats)
+
+
+@pytest.mark.parametrize("shape", ["star3", "chain3", "snow", "bound"])
+def test_unplanned_queries_use_the_planner(small, shape):
+    """Queries without PLAN are planned by the query graph planner from the
+    device lookups' range sizes; single, batched and served runs all equal
+    the naive evaluator."""
+    from vedas_amd.engine import query_text
+    s, p, o, e = small
+    if shape == "star3":
+        pats = plans.star_plan([1, 2, 3])[0]
+    elif shape == "chain3":
+        pats = plans.chain_plan([1, 2, 3])[0]
+    elif shape == "snow":
+        pats = plans.snowflake_plan([1, 2], [3, 4])[0]
+    else:
+        obj = int(o[p == 1][0])
+        pats = [("?s", 1, obj), ("?s", 2, "?y"), ("?y", 3, "?z")]
+    text = query_text(pats, None)
+    want = O.naive_bgp(s, p, o, pats)
+    r = e.query(text)
+    assert O.canonical(r.headers(), r.columns()) == want
+    r.free()
+    res, _ = e.batch([text, text])
+    for r in res:
+        assert O.canonical(r.headers(), r.columns()) == want
+        r.free()
+    _, outs = e.serve([text], [0.0], slots=1, max_batch=1, keep_results=True)
+    assert O.canonical(outs[0].headers(), outs[0].columns()) == want
+    outs[0].free()

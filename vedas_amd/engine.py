@@ -74,6 +74,7 @@ def lib() -> C.CDLL:
                               C.POINTER(ServeCfg), C.POINTER(ServeRec), C.POINTER(vp)]),
             "vdq_estimate": (i, [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
             "vdq_autotune": (i, [vp, C.POINTER(C.c_char_p), i, i, C.POINTER(TuneResult)]),
+            "vdq_plan": (i, [C.c_char_p, C.POINTER(u64), C.c_char_p, C.c_size_t]),
         }
         for name, (res, args) in sig.items():
             f = getattr(l, name)
@@ -197,6 +198,50 @@ class Engine:
             self.h = None
 
 
+def plan(text: str, est_rows=None) -> str:
+    """PLAN clause of a query text: its own, or the query graph planner's
+    from est_rows (rows of each pattern's index range).  Host only."""
+    est = None
+    if est_rows is not None:
+        est = (C.c_uint64 * max(1, len(est_rows)))(*[int(x) for x in est_rows])
+    buf = C.create_string_buffer(1 << 16)
+    _check(lib().vdq_plan(text.encode(), est, buf, len(buf)))
+    return buf.value.decode()
+
+
+def parse_plan(clause: str):
+    """PLAN clause -> plan tree of ("UPLOAD", index, var) / ("JOIN", l, r,
+    var) / ("INDEXSWAP", child, var) nodes."""
+    toks = clause.replace("(", " ( ").replace(")", " ) ").split()
+    pos = 0
+
+    def node():
+        nonlocal pos
+        assert toks[pos] == "("
+        op = toks[pos + 1]
+        pos += 2
+        if op == "U":
+            out = ("UPLOAD", int(toks[pos]), toks[pos + 1])
+            pos += 2
+        elif op == "J":
+            var = toks[pos]
+            pos += 1
+            a = node()
+            b = node()
+            out = ("JOIN", a, b, var)
+        elif op == "S":
+            var = toks[pos]
+            pos += 1
+            out = ("INDEXSWAP", node(), var)
+        else:
+            raise ValueError(op)
+        assert toks[pos] == ")"
+        pos += 1
+        return out
+
+    return node()
+
+
 # ---- query text helpers ---------------------------------------------------------
 def plan_text(node) -> str:
     """Plan tree (tests/plans.py form) -> the PLAN s-expression."""
@@ -212,7 +257,7 @@ def plan_text(node) -> str:
 
 def query_text(patterns, node, select="*") -> str:
     """patterns: list of (s, p, o) terms; node: plan tree whose UPLOAD
-    leaves hold the pattern tuple itself or its index."""
+    leaves hold the pattern tuple itself or its index (None: no PLAN)."""
     idx = {tuple(p): i for i, p in enumerate(patterns)}
 
     def fix(n):
@@ -226,4 +271,6 @@ def query_text(patterns, node, select="*") -> str:
 
     body = " . ".join(" ".join(str(t) for t in pat) for pat in patterns)
     sel = select if isinstance(select, str) else " ".join(select)
+    if node is None:  # planned by the engine's query graph planner
+        return f"SELECT {sel} WHERE {{ {body} }}"
     return f"SELECT {sel} WHERE {{ {body} }} PLAN {plan_text(fix(node))}"

@@ -8,6 +8,7 @@
 
 #include "BatchExecutor.h"
 #include "QueryExecutor.h"
+#include "QueryGraph.h"
 #include "Server.h"
 #include "Statistics.h"
 #include "vedas_host.h"
@@ -131,11 +132,13 @@ PlanTreeNode *parse_node(Parser &p, PlanArena &a, std::vector<TriplePattern> &pa
     return n;
 }
 
-// One parsed query: patterns (stable storage), SELECT set, plan tree.
+// One parsed query: patterns (stable storage), SELECT set, plan tree
+// (from the text, or from the query graph planner).
 struct ParsedQuery {
     std::vector<TriplePattern> pats;
     std::set<std::string> selected;
     PlanArena arena;
+    std::unique_ptr<QueryGraph> graph;
     PlanTreeNode *root = nullptr;
 };
 
@@ -166,9 +169,53 @@ void parse_query(const char *text, ParsedQuery &pq) {
         p.expect("PLAN");
         pq.root = parse_node(p, pq.arena, pq.pats);
         if (p.more()) throw VedasError(VD_EINVAL, "text after the plan");
-    } else if (pq.pats.size() != 1) {
-        throw VedasError(VD_EINVAL, "PLAN required for more than one pattern");
     }
+}
+
+// Rows of each pattern's index range (the reference's estimate_rows,
+// QueryExecutor.cu:666,758): PSO for a bound or free subject, POS for a
+// bound object.
+void estimate_rows(vdq_engine *e, std::vector<TriplePattern> &pats) {
+    VedasContext *ctx = e->ctx.get();
+    vd_index *idx = e->storage->index();
+    for (auto &tp : pats) {
+        if (tp.predicateIsVariable()) {
+            tp.estimate_rows = e->storage->getTripleSize();
+            continue;
+        }
+        uint64_t b = 0, en = 0;
+        int perm = VD_PERM_PSO, has2 = 0;
+        uint32_t k2 = 0;
+        if (!tp.subjectIsVariable()) {
+            has2 = 1;
+            k2 = tp.getSubjectId();
+        } else if (!tp.objectIsVariable()) {
+            perm = VD_PERM_POS;
+            has2 = 1;
+            k2 = tp.getObjectId();
+        }
+        ctx->check(vd_index_lookup(ctx->raw(), idx, perm, tp.getPredicateId(), has2, k2, &b, &en),
+                   "vd_index_lookup");
+        tp.estimate_rows = en - b;
+    }
+}
+
+std::string plan_text(const PlanTreeNode *n) {
+    switch (n->op) {
+        case UPLOAD: return "(U " + std::to_string(n->tp->getId()) + " " + n->var + ")";
+        case JOIN: return "(J " + n->var + " " + plan_text(n->child1) + " " + plan_text(n->child2) + ")";
+        default: return "(S " + n->var + " " + plan_text(n->child1) + ")";
+    }
+}
+
+// A query without PLAN is planned by the query graph (QueryExecutor.cu:
+// 325-326), over the rows of each pattern's range.
+void ensure_plan(vdq_engine *e, ParsedQuery &pq) {
+    if (pq.root || pq.pats.size() == 1) return;
+    estimate_rows(e, pq.pats);
+    pq.graph.reset(new QueryGraph(pq.pats));
+    pq.root = pq.graph->generateQueryPlan();
+    if (!pq.root) throw VedasError(VD_EINVAL, "the planner produced no plan");
 }
 
 void fill_stats(vdq_result *res, VedasContext *ctx) {
@@ -254,6 +301,7 @@ int vdq_query(vdq_engine *e, const char *text, vdq_result **out) {
         auto t0 = std::chrono::steady_clock::now();
         ParsedQuery pq;
         parse_query(text, pq);
+        ensure_plan(e, pq);
         auto res = std::make_unique<vdq_result>();
         VedasContext *ctx = e->ctx.get();
         ctx->resetStats();
@@ -279,6 +327,7 @@ int vdq_batch(vdq_engine *e, const char *const *texts, int n, vdq_result **outs,
         for (int i = 0; i < n; ++i) {
             pqs.emplace_back(new ParsedQuery());
             parse_query(texts[i], *pqs.back());
+            ensure_plan(e, *pqs.back());
             bq[i].patterns = &pqs.back()->pats;
             bq[i].selected = pqs.back()->selected;
             bq[i].root = pqs.back()->root;
@@ -320,6 +369,7 @@ int vdq_serve(vdq_engine *e, const char *const *texts, const double *arrival_ms,
         for (int i = 0; i < n; ++i) {
             pqs.emplace_back(new ParsedQuery());
             parse_query(texts[i], *pqs.back());
+            ensure_plan(e, *pqs.back());
             reqs[i].patterns = pqs.back()->pats;
             reqs[i].selected = pqs.back()->selected;
             reqs[i].root = pqs.back()->root;
@@ -373,6 +423,7 @@ int vdq_autotune(vdq_engine *e, const char *const *texts, int n, int reps, vdq_t
         for (int i = 0; i < n; ++i) {
             qs.emplace_back(new ParsedQuery());
             parse_query(texts[i], *qs.back());
+            ensure_plan(e, *qs.back());
         }
         VedasContext *ctx = e->ctx.get();
         auto set = [&](const char *key, int v) {
@@ -433,12 +484,37 @@ int vdq_autotune(vdq_engine *e, const char *const *texts, int n, int reps, vdq_t
     });
 }
 
+int vdq_plan(const char *text, const uint64_t *est_rows, char *out, size_t cap) {
+    if (!text || !out || cap == 0) return fail(VD_EINVAL, "null argument");
+    return guarded([&] {
+        ParsedQuery pq;
+        parse_query(text, pq);
+        std::string plan;
+        if (pq.root) {
+            plan = plan_text(pq.root);
+        } else if (pq.pats.size() == 1) {
+            const TriplePattern &tp = pq.pats[0];
+            plan = "(U 0 " + (tp.subjectIsVariable() ? tp.getSubject() : tp.getObject()) + ")";
+        } else {
+            if (!est_rows) throw VedasError(VD_EINVAL, "estimates needed to plan");
+            for (size_t i = 0; i < pq.pats.size(); ++i) pq.pats[i].estimate_rows = est_rows[i];
+            QueryGraph g(pq.pats);
+            PlanTreeNode *root = g.generateQueryPlan();
+            if (!root) throw VedasError(VD_EINVAL, "the planner produced no plan");
+            plan = plan_text(root);
+        }
+        if (plan.size() + 1 > cap) throw VedasError(VD_ERANGE, "plan text longer than the buffer");
+        std::memcpy(out, plan.c_str(), plan.size() + 1);
+    });
+}
+
 int vdq_estimate(vdq_engine *e, const char *text, double *rows, double *peak_bytes) {
     if (!e || !text) return fail(VD_EINVAL, "null argument");
     if (!e->storage) return fail(VD_EINVAL, "engine has no store loaded");
     return guarded([&] {
         ParsedQuery pq;
         parse_query(text, pq);
+        ensure_plan(e, pq);
         PlanEstimator est(e->statistics());
         PlanEstimate r = est.estimate(pq.pats, pq.root);
         if (rows) *rows = r.rows;
