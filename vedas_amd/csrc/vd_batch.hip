@@ -34,6 +34,10 @@ __device__ __forceinline__ int find_seg(const uint64_t *base, int n, uint64_t x)
     return lo - 1;
 }
 
+// An LDS-broadcast block index made wave-uniform, so the per-input
+// descriptor behind it is read with scalar loads.
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
 unsigned grid_cap(uint64_t n) {
     uint64_t g = ceil_div(n, 256 * 4);
     if (g > 16384) g = 16384;
@@ -92,7 +96,7 @@ __global__ __launch_bounds__(MB_BLOCK) void k_join_bounds_multi(
     __shared__ int s_j;
     if (threadIdx.x == 0) s_j = find_seg(tbase, np, blockIdx.x);
     __syncthreads();
-    const int j = s_j;
+    const int j = uniform(s_j);
     const JPairDev &D = P[j];
     const uint32_t *lk = D.lc[0], *rk = D.rc[0];
     const uint64_t base = (blockIdx.x - tbase[j]) * (uint64_t)MB_TILE;
@@ -164,6 +168,49 @@ __global__ __launch_bounds__(MJ_BLOCK) void k_join_expand_multi(
     }
     __syncthreads();
     const int nout = (int)(b1 - b0);
+    if (s_pair[0] == s_pair[m - 1]) {
+        // One join for the whole block (the common case): its descriptor is
+        // uniform, columns are written one after the other, coalesced.
+        const JPairDev &D = P[uniform(s_pair[0])];
+        uint32_t lrow[ITEMS], rrow[ITEMS];
+#pragma unroll
+        for (int it = 0; it < ITEMS; ++it) {
+            const int q = it * MJ_BLOCK + threadIdx.x;
+            lrow[it] = rrow[it] = 0;
+            if (q < nout) {
+                const uint64_t o = b0 + q;
+                int lo = 0, hi = m;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_off[mid] <= o) lo = mid + 1; else hi = mid;
+                }
+                const int k = lo - 1;
+                lrow[it] = (uint32_t)(sa + k - D.lbase);
+                rrow[it] = s_low[k] + (uint32_t)(o - s_off[k]);
+            }
+        }
+        const uint64_t ob = b0 - D.obase;
+        int oc = 0;
+        for (int c = 0; c < D.ncl; ++c, ++oc) {
+            const uint32_t *src = D.lc[c];
+            uint32_t *dst = D.oc[oc] + ob;
+#pragma unroll
+            for (int it = 0; it < ITEMS; ++it) {
+                const int q = it * MJ_BLOCK + threadIdx.x;
+                if (q < nout) dst[q] = src[lrow[it]];
+            }
+        }
+        for (int c = 1; c < D.ncr; ++c, ++oc) {
+            const uint32_t *src = D.rc[c];
+            uint32_t *dst = D.oc[oc] + ob;
+#pragma unroll
+            for (int it = 0; it < ITEMS; ++it) {
+                const int q = it * MJ_BLOCK + threadIdx.x;
+                if (q < nout) dst[q] = src[rrow[it]];
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int it = 0; it < ITEMS; ++it) {
         const int q = it * MJ_BLOCK + threadIdx.x;
@@ -201,8 +248,9 @@ __global__ __launch_bounds__(MB_BLOCK) void k_max_col0_multi(const SortRelDev *S
     __shared__ uint32_t s_m[MB_BLOCK / 64];
     if (threadIdx.x == 0) s_j = find_seg(tbase, ns, blockIdx.x);
     __syncthreads();
-    const SortRelDev &D = S[s_j];
-    const uint64_t base = (blockIdx.x - tbase[s_j]) * (uint64_t)MB_TILE;
+    const int j = uniform(s_j);
+    const SortRelDev &D = S[j];
+    const uint64_t base = (blockIdx.x - tbase[j]) * (uint64_t)MB_TILE;
     uint32_t m = 0;
     for (int it = 0; it < MB_ITEMS; ++it) {
         const uint64_t i = base + (uint64_t)it * MB_BLOCK + threadIdx.x;
@@ -229,7 +277,7 @@ __global__ __launch_bounds__(MB_BLOCK) void k_pack_multi(const SortRelDev *S, in
     radix::hist_zero(&h);
     if (threadIdx.x == 0) s_j = find_seg(tbase, ns, blockIdx.x);
     __syncthreads();
-    const int j = s_j;
+    const int j = uniform(s_j);
     const SortRelDev &D = S[j];
     const uint64_t base = (blockIdx.x - tbase[j]) * (uint64_t)MB_TILE;
     for (int it = 0; it < MB_ITEMS; ++it) {
@@ -278,8 +326,9 @@ __global__ __launch_bounds__(MB_BLOCK) void k_unique_count_multi(const SortRelDe
     __shared__ uint32_t s_tmp[MB_BLOCK / 64];
     if (threadIdx.x == 0) s_j = find_seg(tbase, ns, blockIdx.x);
     __syncthreads();
-    const SortRelDev &D = S[s_j];
-    const uint64_t base = (blockIdx.x - tbase[s_j]) * (uint64_t)MB_TILE;
+    const int j = uniform(s_j);
+    const SortRelDev &D = S[j];
+    const uint64_t base = (blockIdx.x - tbase[j]) * (uint64_t)MB_TILE;
     uint32_t c = 0;
     for (int it = 0; it < MB_ITEMS; ++it) {
         const uint64_t i = base + (uint64_t)it * MB_BLOCK + threadIdx.x;
@@ -303,7 +352,7 @@ __global__ __launch_bounds__(MB_BLOCK) void k_unique_scatter_multi(const SortRel
     __shared__ uint32_t s_tmp[MB_BLOCK / 64];
     if (threadIdx.x == 0) s_j = find_seg(tbase, ns, blockIdx.x);
     __syncthreads();
-    const int j = s_j;
+    const int j = uniform(s_j);
     const SortRelDev &D = S[j];
     const uint64_t base = (blockIdx.x - tbase[j]) * (uint64_t)MB_TILE;
     // rounds of MB_BLOCK consecutive rows, compacted with a block scan each
