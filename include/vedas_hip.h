@@ -142,6 +142,12 @@ int  vd_rel_drop_col(vd_ctx *ctx, vd_rel *rel, int i); /* removeColumn :729 */
  * 1 for a view column, else 0. */
 int  vd_rel_materialize(vd_ctx *ctx, vd_rel *rel);
 int  vd_rel_is_view(const vd_rel *rel, int c);
+/* First and last value of column 0 as vd_join computed them with its size
+ * read-back: returns 1 and sets *lo/*hi, or 0 when unknown (another
+ * producer, column 0 changed since, or no rows).  Replaces the two device
+ * reads of the join result's ends in JoinQueryJob::join
+ * (JoinQueryJob.cu:388-397). */
+int  vd_rel_key_range(const vd_rel *rel, uint32_t *lo, uint32_t *hi);
 
 /* ---- select (SelectQueryJob) -------------------------------------------- */
 /* Rows [begin, end) of a permutation (a view when nothing filters them,
@@ -157,6 +163,15 @@ int  vd_select(vd_ctx *ctx, const vd_index *idx, int perm,
                int clip, uint32_t lo, uint32_t hi,
                const vd_interval_t *skip, int nskip, int distinct,
                vd_rel **out);
+/* vd_index_lookup + vd_select in one device step with one read-back: the
+ * rows of key (k1[, k2]) in the permutation, then filtered as vd_select.
+ * Replaces the findL2OffsetFromL1 / findDataOffsetFromL2 lookup followed
+ * by SelectQueryJob::startJob (QueryExecutor.cu:596-646,
+ * SelectQueryJob.cu:86-270).  begin/end (may be NULL) receive the range. */
+int  vd_select_key(vd_ctx *ctx, const vd_index *idx, int perm, uint32_t k1,
+                   int has_k2, uint32_t k2, int cols, int clip, uint32_t lo,
+                   uint32_t hi, const vd_interval_t *skip, int nskip,
+                   int distinct, vd_rel **out, uint64_t *begin, uint64_t *end);
 /* Batched range select: lookups for nq queries, then every matching row of
  * every query written back to back as (Y, Z) pairs into d_out_y / d_out_z
  * with d_row_offsets[q] = first output row of query q (nq + 1 entries).

@@ -73,8 +73,10 @@ def test_projection_and_finalise(small):
 def test_errors_are_reported(small):
     from vedas_amd._native import VedasError
     s, p, o, e = small
-    with pytest.raises(VedasError):
-        e.query("SELECT * WHERE { ?s 1 ?x . ?s 2 ?y }")  # no plan
+    with pytest.raises(VedasError):  # two patterns over the same two terms: needs a PLAN
+        e.query("SELECT * WHERE { ?s 1 ?x . ?s 2 ?x }")
+    with pytest.raises(VedasError):  # disconnected patterns: no plan joins them
+        e.query("SELECT * WHERE { ?s 1 ?x . ?a 2 ?b }")
     with pytest.raises(VedasError):
         e.query("SELECT * WHERE { ?s 1 ?x } PLAN (U 3 ?s)")
     with pytest.raises(VedasError):

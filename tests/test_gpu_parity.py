@@ -206,6 +206,33 @@ def test_select_views_and_lifetime(ctx):
         r.free()
 
 
+def test_select_key_equals_lookup_then_select(ctx, store):
+    s, p, o, idx, orc = store
+    for perm in ("PSO", "POS"):
+        ix = orc[perm]
+        for pred in [1, 2, 5, 17, 64, 10**6]:
+            b, e = O.lookup(ix, pred)
+            ys = ix["y"][b:e]
+            lo, hi = (int(np.percentile(ys, 20)), int(np.percentile(ys, 70))) if len(ys) else (0, 9)
+            sk = [(lo + 10, lo + 500), (hi - 300, hi - 100)]
+            for cols, clip, skip, distinct in [(3, None, (), False), (3, (lo, hi), (), False),
+                                               (3, (lo, hi), sk, False), (2, (lo, hi), (), True),
+                                               (2, None, (), True)]:
+                rel, gb, ge = idx.select_key(perm, pred, None, cols, clip, skip, distinct)
+                assert (gb, ge) == (b, e) or (b == e and gb == ge)
+                want = O.select(ix, b, e, cols, clip=clip, skip=skip, distinct=distinct)
+                got = rel.columns()
+                rel.free()
+                assert all(np.array_equal(g, w) for g, w in zip(got, want)), (perm, pred, cols)
+            if len(ys):  # two bound constants, Z only
+                y0 = int(ys[len(ys) // 2])
+                rel, gb, ge = idx.select_key(perm, pred, y0, 1)
+                bb, ee = O.lookup(ix, pred, y0)
+                assert (gb, ge) == (bb, ee)
+                assert np.array_equal(rel.columns()[0], O.select(ix, bb, ee, 1)[0])
+                rel.free()
+
+
 def test_select_batch(ctx, store):
     import ctypes as C
     from vedas_amd._native import PERMS, VD_UNBOUND

@@ -85,6 +85,11 @@ EXPORTS = {
     "vd_rel_drop_col": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "vd_rel_materialize": (C.c_int, [C.c_void_p, C.c_void_p]),
     "vd_rel_is_view": (C.c_int, [C.c_void_p, C.c_int]),
+    "vd_rel_key_range": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "vd_select_key": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_uint32, C.c_int,
+                                C.c_uint32, C.c_int, C.c_int, C.c_uint32, C.c_uint32,
+                                C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_void_p),
+                                C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "vd_select": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_uint64, C.c_uint64,
                             C.c_int, C.c_int, C.c_uint32, C.c_uint32,
                             C.POINTER(Interval), C.c_int, C.c_int,
@@ -279,6 +284,13 @@ class Rel:
     def columns(self) -> list:
         return [self.ctx.download(self.col_ptr(i), self.nrows) for i in range(self.ncols)]
 
+    def key_range(self):
+        """(first, last) of column 0 when the producing join knew them, else None."""
+        lo, hi = C.c_uint32(), C.c_uint32()
+        if self.ctx.L.vd_rel_key_range(self.h, C.byref(lo), C.byref(hi)):
+            return lo.value, hi.value
+        return None
+
     def is_view(self, i: int) -> bool:
         """Column i reads index storage in place (read-only)."""
         return bool(self.ctx.L.vd_rel_is_view(self.h, i))
@@ -382,6 +394,18 @@ class Index:
             self.ctx.h, self.h, PERMS[perm], begin, end, cols,
             int(clip is not None), lo, hi, sk, len(skip), int(distinct), C.byref(h)))
         return Rel(self.ctx, h)
+
+    def select_key(self, perm: str, k1: int, k2=None, cols: int = 3, clip=None, skip=(),
+                   distinct: bool = False):
+        """Lookup + select in one device step; returns (Rel, begin, end)."""
+        sk = (Interval * max(1, len(skip)))(*[Interval(a, b) for a, b in skip])
+        lo, hi = clip if clip is not None else (0, 0)
+        h, b, e = C.c_void_p(), C.c_uint64(), C.c_uint64()
+        self.ctx.check(self.ctx.L.vd_select_key(
+            self.ctx.h, self.h, PERMS[perm], k1, int(k2 is not None), 0 if k2 is None else k2,
+            cols, int(clip is not None), lo, hi, C.cast(sk, C.c_void_p), len(skip),
+            int(distinct), C.byref(h), C.byref(b), C.byref(e)))
+        return Rel(self.ctx, h), b.value, e.value
 
     def free(self) -> None:
         if self.h and getattr(self, "owned", True):

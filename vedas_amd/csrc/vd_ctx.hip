@@ -199,6 +199,7 @@ void rel_free(vd_ctx *ctx, vd_rel *r) {
 }
 
 void rel_set_col(vd_ctx *ctx, vd_rel *r, int c, uint32_t *fresh) {
+    if (c == 0) r->key_range = false;
     if (!rel_borrowed(r, c)) dfree(ctx, r->cols[c]);
     r->cols[c] = fresh;
     r->borrowed &= ~(1u << c);
@@ -353,6 +354,7 @@ int vd_rel_swap_cols(vd_rel *rel, int i, int j) {
     if (!rel || i < 0 || j < 0 || i >= rel->ncols || j >= rel->ncols)
         return VD_EINVAL;
     std::swap(rel->cols[i], rel->cols[j]);
+    if (i == 0 || j == 0) rel->key_range = false;
     const uint32_t bi = (rel->borrowed >> i) & 1u, bj = (rel->borrowed >> j) & 1u;
     rel->borrowed &= ~((1u << i) | (1u << j));
     rel->borrowed |= (bi << j) | (bj << i);
@@ -363,6 +365,7 @@ int vd_rel_drop_col(vd_ctx *ctx, vd_rel *rel, int i) {
     if (!ctx || !rel || i < 0 || i >= rel->ncols || rel->ncols == 1)
         return VD_EINVAL;
     if (!vd::rel_borrowed(rel, i)) vd::dfree(ctx, rel->cols[i]);
+    if (i == 0) rel->key_range = false;
     for (int c = i; c + 1 < rel->ncols; ++c) rel->cols[c] = rel->cols[c + 1];
     rel->cols[rel->ncols - 1] = nullptr;
     const uint32_t low = rel->borrowed & ((1u << i) - 1u);
@@ -383,6 +386,13 @@ int vd_rel_materialize(vd_ctx *ctx, vd_rel *rel) {
             vd::rel_set_col(ctx, rel, c, fresh);
         }
     });
+}
+
+int vd_rel_key_range(const vd_rel *rel, uint32_t *lo, uint32_t *hi) {
+    if (!rel || !rel->key_range || rel->nrows == 0) return 0;
+    if (lo) *lo = rel->key_lo;
+    if (hi) *hi = rel->key_hi;
+    return 1;
 }
 
 int vd_rel_is_view(const vd_rel *rel, int c) {

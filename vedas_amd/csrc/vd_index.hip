@@ -288,14 +288,11 @@ struct Pieces {
     uint64_t dst[MAX_PIECES + 1];  // dst[np] = total rows
 };
 
-// One thread: turn [b, e) minus clip and skip intervals into pieces of
-// consecutive rows.  Intervals must be sorted by lo and disjoint.
-__global__ void k_select_pieces(const uint32_t *__restrict__ ycol, uint64_t b,
-                                uint64_t e, int clip, uint32_t lo, uint32_t hi,
-                                vd_interval_t s0, vd_interval_t s1,
-                                vd_interval_t s2, vd_interval_t s3, int nskip,
-                                Pieces *out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// [b, e) minus clip and skip intervals as pieces of consecutive rows.
+// Intervals must be sorted by lo and disjoint.
+__device__ Pieces make_pieces(const uint32_t *__restrict__ ycol, uint64_t b, uint64_t e,
+                              int clip, uint32_t lo, uint32_t hi, const vd_interval_t *sk,
+                              int nskip) {
     if (clip) {
         if (lo > hi) {
             e = b;
@@ -305,7 +302,6 @@ __global__ void k_select_pieces(const uint32_t *__restrict__ ycol, uint64_t b,
             b = nb;
         }
     }
-    vd_interval_t sk[4] = {s0, s1, s2, s3};
     Pieces P;
     P.np = 0;
     uint64_t cur = b, acc = 0;
@@ -328,7 +324,37 @@ __global__ void k_select_pieces(const uint32_t *__restrict__ ycol, uint64_t b,
         ++P.np;
     }
     P.dst[P.np] = acc;
-    *out = P;
+    return P;
+}
+
+struct SkipSet {
+    vd_interval_t s[4];
+};
+
+// One thread: pieces of a given row range.
+__global__ void k_select_pieces(const uint32_t *__restrict__ ycol, uint64_t b, uint64_t e,
+                                int clip, uint32_t lo, uint32_t hi, SkipSet sk, int nskip,
+                                Pieces *out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    *out = make_pieces(ycol, b, e, clip, lo, hi, sk.s, nskip);
+}
+
+// Pieces plus the key's row range, so a lookup and a select cost one
+// read-back together.
+struct KeyPieces {
+    Pieces p;
+    uint64_t begin, end;
+};
+
+__global__ void k_lookup_pieces(PermDev pd, uint32_t k1, int has_k2, uint32_t k2, int clip,
+                                uint32_t lo, uint32_t hi, SkipSet sk, int nskip,
+                                KeyPieces *out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint64_t b, e;
+    lookup_one(pd, k1, k2, has_k2 != 0, &b, &e);
+    out->begin = b;
+    out->end = e;
+    out->p = make_pieces(pd.y, b, e, clip, lo, hi, sk.s, nskip);
 }
 
 __global__ __launch_bounds__(256) void k_copy_pieces(
@@ -478,6 +504,66 @@ const vd_perm_arrays &get_perm(const vd_index *idx, int perm) {
 
 }  // namespace vd
 
+namespace vd {
+namespace {
+
+// Argument checks shared by the select entry points; the skip intervals
+// sorted by lo.
+SkipSet check_select(int cols, int clip, const vd_interval_t *skip, int nskip, int distinct) {
+    VD_REQUIRE(cols >= 1 && cols <= 3, VD_EINVAL, "cols must be 1, 2 or 3");
+    VD_REQUIRE(nskip >= 0 && nskip <= 4 && (nskip == 0 || skip), VD_EINVAL,
+               "at most 4 skip intervals");
+    VD_REQUIRE(!(clip || nskip || distinct) || cols != 1, VD_EINVAL,
+               "clip, skip and distinct need Y as column 0");
+    VD_REQUIRE(!distinct || cols == 2, VD_EINVAL, "distinct needs cols == 2");
+    SkipSet sk{{{1, 0}, {1, 0}, {1, 0}, {1, 0}}};
+    for (int i = 0; i < nskip; ++i) sk.s[i] = skip[i];
+    std::sort(sk.s, sk.s + nskip, [](const vd_interval_t &a, const vd_interval_t &b) {
+        return a.lo < b.lo;
+    });
+    for (int i = 1; i < nskip; ++i)
+        VD_REQUIRE(sk.s[i].lo > sk.s[i - 1].hi || sk.s[i - 1].lo > sk.s[i - 1].hi, VD_EINVAL,
+                   "skip intervals overlap");
+    return sk;
+}
+
+// Rows [src, src + rows) of the permutation's columns as a view.
+vd_rel *range_view(const vd_index *idx, const vd_perm_arrays &pa, int cols, uint64_t src,
+                   uint64_t rows) {
+    uint32_t *vc[2];
+    int nc = 0;
+    if (cols & 2) vc[nc++] = pa.y + src;
+    if (cols & 1) vc[nc++] = pa.z + src;
+    return rel_view(nc, rows, vc, idx->hold);
+}
+
+// The relation of computed pieces: one run of rows is a view, several are
+// copied; distinct then removes adjacent duplicates of Y.
+vd_rel *select_pieces_rel(vd_ctx *ctx, const vd_index *idx, const vd_perm_arrays &pa, int cols,
+                          int distinct, const Pieces &hp, const Pieces *dp) {
+    const uint64_t rows = hp.dst[hp.np];
+    if (hp.np <= 1 && !distinct)
+        return range_view(idx, pa, cols, hp.np == 1 ? hp.src[0] : 0, rows);
+    const int ocols = cols == 3 ? 2 : 1;
+    vd_rel *r = rel_new(ctx, ocols, rows);
+    try {
+        launch(ctx, "select_copy", (double)rows * 8.0 * ocols, dim3(stream_grid(rows)),
+               dim3(256), 0, k_copy_pieces, dp, (const uint32_t *)pa.y, (const uint32_t *)pa.z,
+               r->cols[0], ocols == 2 ? r->cols[1] : (uint32_t *)nullptr, cols);
+        if (distinct && rows > 1) {
+            int rc = vd_unique_adjacent(ctx, r);
+            if (rc != VD_OK) throw Error(rc, ctx->last_error);
+        }
+    } catch (...) {
+        rel_free(ctx, r);
+        throw;
+    }
+    return r;
+}
+
+}  // namespace
+}  // namespace vd
+
 using namespace vd;
 
 extern "C" {
@@ -620,67 +706,43 @@ int vd_select(vd_ctx *ctx, const vd_index *idx, int perm, uint64_t begin,
     *out = nullptr;
     VD_TRY(ctx, {
         const vd_perm_arrays &pa = get_perm(idx, perm);
-        VD_REQUIRE(cols >= 1 && cols <= 3, VD_EINVAL, "cols must be 1, 2 or 3");
         VD_REQUIRE(begin <= end && end <= pa.n, VD_EINVAL, "row range out of bounds");
-        VD_REQUIRE(nskip >= 0 && nskip <= 4 && (nskip == 0 || skip), VD_EINVAL,
-                   "at most 4 skip intervals");
-        VD_REQUIRE(!(clip || nskip || distinct) || cols != 1, VD_EINVAL,
-                   "clip, skip and distinct need Y as column 0");
-        VD_REQUIRE(!distinct || cols == 2, VD_EINVAL, "distinct needs cols == 2");
-        vd_interval_t sk[4] = {{1, 0}, {1, 0}, {1, 0}, {1, 0}};
-        for (int i = 0; i < nskip; ++i) sk[i] = skip[i];
-        std::sort(sk, sk + nskip, [](const vd_interval_t &a, const vd_interval_t &b) {
-            return a.lo < b.lo;
-        });
-        for (int i = 1; i < nskip; ++i)
-            VD_REQUIRE(sk[i].lo > sk[i - 1].hi || sk[i - 1].lo > sk[i - 1].hi,
-                       VD_EINVAL, "skip intervals overlap");
+        const SkipSet sk = check_select(cols, clip, skip, nskip, distinct);
         if (!clip && !nskip && !distinct) {
             // An unfiltered range is already contiguous in the permutation's
             // columns: the relation reads it in place.
-            uint32_t *vc[2];
-            int nc = 0;
-            if (cols & 2) vc[nc++] = pa.y + begin;
-            if (cols & 1) vc[nc++] = pa.z + begin;
-            *out = rel_view(nc, end - begin, vc, idx->hold);
+            *out = range_view(idx, pa, cols, begin, end - begin);
             return VD_OK;
         }
         DBuf<Pieces> dp(ctx, 1);
         launch(ctx, "select_pieces", 0.0, dim3(1), dim3(64), 0, k_select_pieces,
-               (const uint32_t *)pa.y, begin, end, clip, lo, hi, sk[0], sk[1],
-               sk[2], sk[3], nskip, dp.p);
+               (const uint32_t *)pa.y, begin, end, clip, lo, hi, sk, nskip, dp.p);
         Pieces hp;
-        VD_HIP(hipMemcpyAsync(&hp, dp.p, sizeof(Pieces), hipMemcpyDeviceToHost,
+        VD_HIP(hipMemcpyAsync(&hp, dp.p, sizeof(Pieces), hipMemcpyDeviceToHost, ctx->stream));
+        sync(ctx);
+        *out = select_pieces_rel(ctx, idx, pa, cols, distinct, hp, dp.p);
+    });
+}
+
+int vd_select_key(vd_ctx *ctx, const vd_index *idx, int perm, uint32_t k1, int has_k2,
+                  uint32_t k2, int cols, int clip, uint32_t lo, uint32_t hi,
+                  const vd_interval_t *skip, int nskip, int distinct, vd_rel **out,
+                  uint64_t *begin, uint64_t *end) {
+    if (!ctx || !out) return VD_EINVAL;
+    *out = nullptr;
+    VD_TRY(ctx, {
+        const vd_perm_arrays &pa = get_perm(idx, perm);
+        const SkipSet sk = check_select(cols, clip, skip, nskip, distinct);
+        DBuf<KeyPieces> dk(ctx, 1);
+        launch(ctx, "select_pieces", 0.0, dim3(1), dim3(64), 0, k_lookup_pieces, perm_dev(pa),
+               k1, has_k2, k2, clip, lo, hi, sk, nskip, dk.p);
+        KeyPieces hk;
+        VD_HIP(hipMemcpyAsync(&hk, dk.p, sizeof(KeyPieces), hipMemcpyDeviceToHost,
                               ctx->stream));
         sync(ctx);
-        const uint64_t rows = hp.dst[hp.np];
-        const int ocols = cols == 3 ? 2 : 1;
-        if (hp.np <= 1 && !distinct) {
-            // Clip and skips left one run of consecutive rows: a view.
-            const uint64_t src = hp.np == 1 ? hp.src[0] : begin;
-            uint32_t *vc[2];
-            int nc = 0;
-            if (cols & 2) vc[nc++] = pa.y + src;
-            if (cols & 1) vc[nc++] = pa.z + src;
-            *out = rel_view(nc, rows, vc, idx->hold);
-            return VD_OK;
-        }
-        vd_rel *r = rel_new(ctx, ocols, rows);
-        try {
-            launch(ctx, "select_copy", (double)rows * 8.0 * ocols,
-                   dim3(stream_grid(rows)), dim3(256), 0, k_copy_pieces,
-                   (const Pieces *)dp.p, (const uint32_t *)pa.y,
-                   (const uint32_t *)pa.z, r->cols[0],
-                   ocols == 2 ? r->cols[1] : (uint32_t *)nullptr, cols);
-            if (distinct && rows > 1) {
-                int rc = vd_unique_adjacent(ctx, r);
-                if (rc != VD_OK) throw Error(rc, ctx->last_error);
-            }
-        } catch (...) {
-            rel_free(ctx, r);
-            throw;
-        }
-        *out = r;
+        if (begin) *begin = hk.begin;
+        if (end) *end = hk.end;
+        *out = select_pieces_rel(ctx, idx, pa, cols, distinct, hk.p, &dk.p->p);
     });
 }
 

@@ -115,6 +115,10 @@ struct vd_rel {
     uint32_t *cols[VD_MAX_COLS] = {};
     uint32_t borrowed = 0;
     std::shared_ptr<void> hold;
+    // First and last value of column 0 when the operator that produced the
+    // relation knew them (a join); cleared when column 0 changes.
+    bool key_range = false;
+    uint32_t key_lo = 0, key_hi = 0;
 };
 
 namespace vd {

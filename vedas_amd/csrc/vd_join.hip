@@ -144,6 +144,32 @@ __global__ __launch_bounds__(256) void k_join_partition(
     mp[b] = lo;
 }
 
+// out = {total, first output key, last output key}: the total is the last
+// scanned offset; the first output row comes from the first left row with
+// matches (first i with off[i+1] > 0), the last from the last one (last i
+// with off[i] < total).
+__global__ void k_join_total_keys(const uint64_t *__restrict__ off, uint64_t nl,
+                                  const uint32_t *__restrict__ lkey, uint64_t *out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint64_t total = off[nl];
+    out[0] = total;
+    out[1] = out[2] = 0;
+    if (total == 0) return;
+    uint64_t lo = 0, hi = nl;  // first i with off[i + 1] > 0
+    while (lo < hi) {
+        const uint64_t mid = lo + ((hi - lo) >> 1);
+        if (off[mid + 1] > 0) hi = mid; else lo = mid + 1;
+    }
+    out[1] = lkey[lo];
+    lo = 0;
+    hi = nl;  // first i with off[i] >= total, minus one
+    while (lo < hi) {
+        const uint64_t mid = lo + ((hi - lo) >> 1);
+        if (off[mid] >= total) hi = mid; else lo = mid + 1;
+    }
+    out[2] = lkey[lo - 1];
+}
+
 struct OutCols {
     const uint32_t *l[VD_MAX_COLS];
     const uint32_t *r[VD_MAX_COLS];
@@ -235,10 +261,18 @@ vd_rel *join_impl(vd_ctx *ctx, const vd_rel *L, const vd_rel *R) {
            dim3((unsigned)jb_tiles), dim3(JB_BLOCK), 0, k_join_bounds<NK>, lk,
            nl, rk, nr, rcap, (const uint64_t *)ranges.p, lower.p, cnt.p);
     scan_u32_to_u64(ctx, cnt.p, off.p, nl, off.p + nl);
-    const uint64_t total = read_u64(ctx, off.p + nl);
+    DBuf<uint64_t> tk(ctx, 3);
+    launch(ctx, "join_total", 0.0, dim3(1), dim3(64), 0, k_join_total_keys,
+           (const uint64_t *)off.p, nl, (const uint32_t *)L->cols[0], tk.p);
+    uint64_t h[3];
+    read_words(ctx, h, tk.p, 3);
+    const uint64_t total = h[0];
     VD_REQUIRE(total < (1ull << 40), VD_ERANGE, "join output too large");
 
     vd_rel *out = rel_new(ctx, out_cols, total);
+    out->key_range = true;  // output column 0 = left keys of matched rows
+    out->key_lo = (uint32_t)h[1];
+    out->key_hi = (uint32_t)h[2];
     if (total == 0) return out;
     try {
         OutCols oc{};

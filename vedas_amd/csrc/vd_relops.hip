@@ -183,6 +183,7 @@ void sort_col0(vd_ctx *ctx, vd_rel *rel) {
     const uint64_t n = rel->nrows;
     const int nc = rel->ncols;
     if (n <= 1) return;
+    rel->key_range = false;
     VD_REQUIRE(n < (1ull << 30), VD_ERANGE, "sort supports fewer than 2^30 rows");
     const int b = std::max(1, bits_for(max_u32(ctx, rel->cols[0], n)));
     const int hi = 32 + b;
@@ -262,6 +263,7 @@ void sort_col0(vd_ctx *ctx, vd_rel *rel) {
 void sort_lex(vd_ctx *ctx, vd_rel *rel) {
     const uint64_t n = rel->nrows;
     if (n <= 1) return;
+    rel->key_range = false;
     VD_REQUIRE(n < (1ull << 30), VD_ERANGE, "sort supports fewer than 2^30 rows");
     DBuf<uint64_t> keys(ctx, n), tmp(ctx, n);
     DBuf<uint32_t> perm(ctx, n), perm2(ctx, n);

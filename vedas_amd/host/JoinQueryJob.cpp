@@ -136,11 +136,14 @@ IR *JoinQueryJob::join(FullRelationIR *lir, FullRelationIR *rir) {
     if (QueryExecutor::ENABLE_UPDATE_BOUND_AFTER_JOIN && variables_bound) {
         if (result_size > 0) {
             TYPEID ends[2];
-            context->check(vd_copy_d2h(context->raw(), &ends[0], newIr->getRelationRawPointer(0), 4),
-                           "vd_copy_d2h");
-            context->check(vd_copy_d2h(context->raw(), &ends[1],
-                                       newIr->getRelationRawPointer(0) + result_size - 1, 4),
-                           "vd_copy_d2h");
+            // vd_join returns the ends of column 0 with its size read-back.
+            if (!vd_rel_key_range(out, &ends[0], &ends[1])) {
+                context->check(vd_copy_d2h(context->raw(), &ends[0],
+                                           newIr->getRelationRawPointer(0), 4), "vd_copy_d2h");
+                context->check(vd_copy_d2h(context->raw(), &ends[1],
+                                           newIr->getRelationRawPointer(0) + result_size - 1, 4),
+                               "vd_copy_d2h");
+            }
             QueryExecutor::updateBoundDict(variables_bound, joinVariable, ends[0], ends[1]);
             if (!lastJoinForVar && result_size > 2 && result_size < EIF_ALPHA && ei_dict) {
                 auto e0 = std::chrono::steady_clock::now();

@@ -41,11 +41,11 @@ int SelectQueryJob::startJob(int gpuId) {
     uint64_t b = 0, e = 0;
     vd_rel *rel = nullptr;
     FullRelationIR *ir = nullptr;
+    // The key lookup and the select run as one device step (vd_select_key).
     if (variable_num == 1) {
-        context->check(vd_index_lookup(c, idx, index.perm, dataid[0], 1, dataid[1], &b, &e),
-                       "vd_index_lookup");
-        context->check(vd_select(c, idx, index.perm, b, e, 1, 0, 0, 0, nullptr, 0, 0, &rel),
-                       "vd_select");
+        context->check(vd_select_key(c, idx, index.perm, dataid[0], 1, dataid[1], 1, 0, 0, 0,
+                                     nullptr, 0, 0, &rel, &b, &e),
+                       "vd_select_key");
         ir = new FullRelationIR(context, rel);
         ir->setHeader(0, variables[0], is_predicates[0]);
         context->exe_log.emplace_back(gpuId, UPLOAD_OP, ir->getHeaders(""), ir->size(), 1);
@@ -54,8 +54,6 @@ int SelectQueryJob::startJob(int gpuId) {
         if (variables_bound && variables_bound->count(variables[0]) > 0 && ir->size() == 0)
             QueryExecutor::updateBoundDict(variables_bound, variables[0], 0, 0);
     } else {
-        context->check(vd_index_lookup(c, idx, index.perm, dataid[0], 0, 0, &b, &e),
-                       "vd_index_lookup");
         int clip = 0;
         TYPEID lo = 0, hi = 0;
         if (variables_bound && variables_bound->count(variables[0]) > 0) {
@@ -65,17 +63,18 @@ int SelectQueryJob::startJob(int gpuId) {
             hi = bound.second;
         }
         if (!is_second_var_used) {
-            context->check(vd_select(c, idx, index.perm, b, e, 2, clip, lo, hi, nullptr, 0, 1, &rel),
-                           "vd_select");
+            context->check(vd_select_key(c, idx, index.perm, dataid[0], 0, 0, 2, clip, lo, hi,
+                                         nullptr, 0, 1, &rel, &b, &e),
+                           "vd_select_key");
             ir = new FullRelationIR(context, rel);
             ir->setHeader(0, variables[0], is_predicates[0]);
         } else {
             std::vector<vd_interval_t> skip;
             if (ei_dict) skip = ei_dict->skipIntervals(variables[0]);
-            context->check(vd_select(c, idx, index.perm, b, e, 3, clip, lo, hi,
-                                     skip.empty() ? nullptr : skip.data(), (int)skip.size(),
-                                     0, &rel),
-                           "vd_select");
+            context->check(vd_select_key(c, idx, index.perm, dataid[0], 0, 0, 3, clip, lo, hi,
+                                         skip.empty() ? nullptr : skip.data(), (int)skip.size(),
+                                         0, &rel, &b, &e),
+                           "vd_select_key");
             ir = new FullRelationIR(context, rel);
             ir->setHeader(0, variables[0], is_predicates[0]);
             ir->setHeader(1, variables[1], is_predicates[1]);
