@@ -98,6 +98,13 @@ int  vd_copy_d2d(vd_ctx *ctx, void *dst, const void *src, size_t bytes);
 int  vd_index_build(vd_ctx *ctx, const uint32_t *d_s, const uint32_t *d_p,
                     const uint32_t *d_o, uint64_t n, uint32_t perm_mask,
                     vd_index **out);
+/* An index over permutation arrays already in device memory (what a .vds
+ * file holds: VedasStorage::open, VedasStorage.cu:452-484): for every
+ * permutation p in perm_mask, views[p] gives n, n_l1, n_l2 and the six
+ * arrays with vd_index_view's meaning.  The arrays are copied (the caller
+ * keeps its buffers); only sizes are checked, not the sort order. */
+int  vd_index_from_arrays(vd_ctx *ctx, uint32_t perm_mask, const vd_perm_view_t *views,
+                          vd_index **out);
 int  vd_index_free(vd_ctx *ctx, vd_index *idx);
 int  vd_index_view(const vd_index *idx, int perm, vd_perm_view_t *out);
 uint32_t vd_index_perms(const vd_index *idx);   /* mask of built perms */

@@ -48,6 +48,11 @@ int vdq_engine_load(vdq_engine *e, const uint32_t *s, const uint32_t *p,
 int vdq_engine_load_device(vdq_engine *e, const uint32_t *d_s,
                            const uint32_t *d_p, const uint32_t *d_o,
                            uint64_t n, uint32_t perm_mask);
+/* The .vds file of the reference's default index (PSO and POS):
+ * VedasStorage::write / VedasStorage::open (VedasStorage.cu:452-515).
+ * open replaces the engine's store with the file's (no sort). */
+int vdq_engine_save(vdq_engine *e, const char *path);
+int vdq_engine_open(vdq_engine *e, const char *path);
 vd_ctx *vdq_engine_ctx(vdq_engine *e);
 vd_index *vdq_engine_index(vdq_engine *e);
 

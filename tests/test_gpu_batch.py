@@ -20,9 +20,11 @@ def rand_rel(rng, n, ncols, keymax):
     return O.sort_col0([rng.integers(0, keymax, n).astype(np.uint32) for _ in range(ncols)])
 
 
-def test_join_multi(ctx):
+@pytest.mark.parametrize("nv", [1024, 2048, 4096])
+def test_join_multi(ctx, nv):
     from vedas_amd._native import Rel, join_multi
-    rng = np.random.default_rng(0)
+    ctx.tune("join_nv", nv)
+    rng = np.random.default_rng(nv)
     shapes = [(0, 10, 5), (10, 0, 5), (1, 1, 1), (3000, 5000, 40), (20_000, 10, 9000), (5, 70_000, 3),
               (100_000, 90_000, 300_000), (17, 33, 4)] * 3
     Ls, Rs, want = [], [], []
@@ -40,6 +42,7 @@ def test_join_multi(ctx):
             assert np.array_equal(g, ww), j
     for r in Ls + Rs + outs:
         r.free()
+    ctx.tune("join_nv", 2048)
 
 
 def test_sort_and_unique_multi(ctx):

@@ -213,8 +213,8 @@ def test_select_key_equals_lookup_then_select(ctx, store):
         for pred in [1, 2, 5, 17, 64, 10**6]:
             b, e = O.lookup(ix, pred)
             ys = ix["y"][b:e]
-            lo, hi = (int(np.percentile(ys, 20)), int(np.percentile(ys, 70))) if len(ys) else (0, 9)
-            sk = [(lo + 10, lo + 500), (hi - 300, hi - 100)]
+            lo, hi = (int(np.percentile(ys, 20)), int(np.percentile(ys, 70))) if len(ys) else (0, 2000)
+            sk = [(lo + 10, lo + 500), (max(hi - 300, lo + 501), max(hi - 100, lo + 502))]
             for cols, clip, skip, distinct in [(3, None, (), False), (3, (lo, hi), (), False),
                                                (3, (lo, hi), sk, False), (2, (lo, hi), (), True),
                                                (2, None, (), True)]:

@@ -21,7 +21,6 @@ constexpr int MB_BLOCK = 256;
 constexpr int MB_ITEMS = 8;
 constexpr int MB_TILE = MB_BLOCK * MB_ITEMS;  // rows per tile
 constexpr int MJ_BLOCK = 256;
-constexpr int MJ_NV = 2048;
 constexpr int MJ_RCAP = 8192;  // staged right keys (32 KB)
 
 __device__ __forceinline__ int find_seg(const uint64_t *base, int n, uint64_t x) {
@@ -133,11 +132,11 @@ __global__ void k_pair_totals(const JPairDev *P, int np, const uint64_t *off, ui
 }
 
 __global__ __launch_bounds__(256) void k_join_partition_multi(const uint64_t *off, uint64_t nl,
-                                                              uint64_t total, uint64_t nblocks,
-                                                              uint64_t *mp) {
+                                                              uint64_t total, int nv,
+                                                              uint64_t nblocks, uint64_t *mp) {
     const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b > nblocks) return;
-    const uint64_t diag = min(b * (uint64_t)MJ_NV, nl + total);
+    const uint64_t diag = min(b * (uint64_t)nv, nl + total);
     uint64_t lo = 0, hi = nl;
     while (lo < hi) {
         const uint64_t mid = lo + ((hi - lo) >> 1);
@@ -146,6 +145,8 @@ __global__ __launch_bounds__(256) void k_join_partition_multi(const uint64_t *of
     mp[b] = lo;
 }
 
+// NV: merge-path diagonal per block (the join_nv tunable).
+template <int MJ_NV>
 __global__ __launch_bounds__(MJ_BLOCK) void k_join_expand_multi(
     const uint64_t *off, const uint32_t *lower, const uint64_t *mp, uint64_t nl, uint64_t total,
     const JPairDev *P, int np, const uint64_t *lbase) {
@@ -476,19 +477,26 @@ void join_multi(vd_ctx *ctx, const vd_rel *const *L, const vd_rel *const *R, int
         VD_REQUIRE(total < (1ull << 40), VD_ERANGE, "join output too large");
         if (total > 0) {
             upload(ctx, dp.p, hp.data(), np * sizeof(JPairDev));
-            const uint64_t nblocks = ceil_div(nl + total, (uint64_t)MJ_NV);
+            const int nv = ctx->join_nv;
+            const uint64_t nblocks = ceil_div(nl + total, (uint64_t)nv);
+            VD_REQUIRE(nblocks < (1ull << 31), VD_ERANGE, "join expansion grid too large");
             DBuf<uint64_t> mp(ctx, nblocks + 1);
             launch(ctx, "join_partition", 0.0, dim3((unsigned)ceil_div(nblocks + 1, 256)),
                    dim3(256), 0, k_join_partition_multi, (const uint64_t *)off.p, nl, total,
-                   nblocks, mp.p);
+                   nv, nblocks, mp.p);
             double xbytes = 0;
             for (int j = 0; j < np; ++j)
                 xbytes += 4.0 * tot[j] * outs[j]->ncols + 4.0 * hp[j].nl * hp[j].ncl +
                           4.0 * hp[j].nr * (hp[j].ncr - 1);
-            launch(ctx, "join_expand", xbytes, dim3((unsigned)nblocks), dim3(MJ_BLOCK), 0,
-                   k_join_expand_multi, (const uint64_t *)off.p, (const uint32_t *)lower.p,
-                   (const uint64_t *)mp.p, nl, total, (const JPairDev *)dp.p, np,
-                   (const uint64_t *)dlbase.p);
+            auto go = [&](auto kern) {
+                launch(ctx, "join_expand", xbytes, dim3((unsigned)nblocks), dim3(MJ_BLOCK), 0,
+                       kern, (const uint64_t *)off.p, (const uint32_t *)lower.p,
+                       (const uint64_t *)mp.p, nl, total, (const JPairDev *)dp.p, np,
+                       (const uint64_t *)dlbase.p);
+            };
+            if (nv == 1024) go(k_join_expand_multi<1024>);
+            else if (nv == 4096) go(k_join_expand_multi<4096>);
+            else go(k_join_expand_multi<2048>);
         }
     } catch (...) {
         for (int j = 0; j < np; ++j) rel_free(ctx, outs[j]), outs[j] = nullptr;

@@ -626,6 +626,62 @@ int vd_index_build(vd_ctx *ctx, const uint32_t *d_s, const uint32_t *d_p,
     return VD_OK;
 }
 
+int vd_index_from_arrays(vd_ctx *ctx, uint32_t perm_mask, const vd_perm_view_t *views,
+                         vd_index **out) {
+    if (!ctx || !out || !views) return VD_EINVAL;
+    *out = nullptr;
+    vd_index *idx = new (std::nothrow) vd_index();
+    if (!idx) return VD_ENOMEM;
+    try {
+        VD_REQUIRE(perm_mask != 0 && perm_mask < (1u << VD_NUM_PERMS), VD_EINVAL,
+                   "empty or invalid permutation mask");
+        idx->owner = ctx;
+        idx->mask = perm_mask;
+        bool first = true;
+        for (int p = 0; p < VD_NUM_PERMS; ++p) {
+            if (!(perm_mask & VD_PERM_BIT(p))) continue;
+            const vd_perm_view_t &v = views[p];
+            VD_REQUIRE(first || v.n == idx->n, VD_EINVAL, "permutations of different sizes");
+            VD_REQUIRE(v.n < (1ull << 32) && v.n_l1 <= v.n && v.n_l2 <= v.n && v.n_l1 <= v.n_l2,
+                       VD_EINVAL, "inconsistent permutation sizes");
+            VD_REQUIRE(v.n == 0 || (v.col_y && v.col_z && v.l1_values && v.l1_offsets &&
+                                    v.l2_values && v.l2_offsets),
+                       VD_EINVAL, "null permutation array");
+            first = false;
+            idx->n = v.n;
+            vd_perm_arrays &pa = idx->perm[p];
+            auto copy = [&](const uint32_t *src, uint64_t cnt) {
+                uint32_t *d = (uint32_t *)dalloc(ctx, (cnt ? cnt : 1) * 4);
+                if (cnt) copy_d2d(ctx, d, src, cnt * 4);
+                return d;
+            };
+            pa.n = v.n;
+            pa.n_l1 = v.n_l1;
+            pa.n_l2 = v.n_l2;
+            pa.y = copy(v.col_y, v.n);
+            pa.z = copy(v.col_z, v.n);
+            pa.l1v = copy(v.l1_values, v.n_l1);
+            pa.l1o = copy(v.l1_offsets, v.n_l1);
+            pa.l2v = copy(v.l2_values, v.n_l2);
+            pa.l2o = copy(v.l2_offsets, v.n_l2);
+            pa.built = true;
+        }
+        idx->hold = make_hold(ctx, idx->perm);
+    } catch (const Error &e) {
+        ctx->last_error = e.what();
+        for (auto &pa : idx->perm) free_perm(ctx, pa);
+        delete idx;
+        return e.code;
+    } catch (const std::exception &e) {
+        ctx->last_error = e.what();
+        for (auto &pa : idx->perm) free_perm(ctx, pa);
+        delete idx;
+        return VD_EINVAL;
+    }
+    *out = idx;
+    return VD_OK;
+}
+
 int vd_index_free(vd_ctx *ctx, vd_index *idx) {
     if (!idx) return VD_EINVAL;
     vd_ctx *owner = idx->owner ? idx->owner : ctx;

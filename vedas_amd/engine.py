@@ -60,6 +60,8 @@ def lib() -> C.CDLL:
             "vdq_engine_load": (i, [vp, vp, vp, vp, u64, C.c_uint32]),
             "vdq_engine_load_device": (i, [vp, vp, vp, vp, u64, C.c_uint32]),
             "vdq_engine_ctx": (vp, [vp]),
+            "vdq_engine_save": (i, [vp, C.c_char_p]),
+            "vdq_engine_open": (i, [vp, C.c_char_p]),
             "vdq_engine_index": (vp, [vp]),
             "vdq_query": (i, [vp, C.c_char_p, C.POINTER(vp)]),
             "vdq_result_shape": (i, [vp, C.POINTER(i), C.POINTER(u64)]),
@@ -143,6 +145,14 @@ class Engine:
             mask |= 1 << _native.PERMS[name]
         _check(lib().vdq_engine_load_device(self.h, C.c_void_p(ds), C.c_void_p(dp),
                                             C.c_void_p(do), n, mask))
+
+    def save(self, path: str) -> None:
+        """Write the store as a .vds file (the reference's default index)."""
+        _check(lib().vdq_engine_save(self.h, path.encode()))
+
+    def open(self, path: str) -> None:
+        """Replace the store with a .vds file's permutations."""
+        _check(lib().vdq_engine_open(self.h, path.encode()))
 
     def ctx_handle(self):
         return lib().vdq_engine_ctx(self.h)

@@ -24,7 +24,15 @@ class VedasStorage {
     // Explicit permutation set (mask of VD_PERM_BIT) from device triples.
     VedasStorage(VedasContext *ctx, const uint32_t *d_s, const uint32_t *d_p,
                  const uint32_t *d_o, uint64_t n, uint32_t perm_mask);
+    // A store read from a .vds file (VedasStorage::open, VedasStorage.cu:
+    // 452-484): the PSO and POS permutations of the reference's default
+    // index, uploaded as they are (no sort).
+    VedasStorage(VedasContext *ctx, const char *fname);
     ~VedasStorage();
+    // Write the default-index part of the .vds format (VedasStorage::write,
+    // VedasStorage.cu:486-515): ten arrays, each a size_t count followed by
+    // the uint32 values.  Needs the PSO and POS permutations.
+    void write(const char *fname);
     VedasStorage(const VedasStorage &) = delete;
     VedasStorage &operator=(const VedasStorage &) = delete;
 

@@ -278,6 +278,21 @@ int vdq_engine_load(vdq_engine *e, const uint32_t *s, const uint32_t *p, const u
     });
 }
 
+int vdq_engine_save(vdq_engine *e, const char *path) {
+    if (!e || !path) return fail(VD_EINVAL, "null argument");
+    if (!e->storage) return fail(VD_EINVAL, "engine has no store loaded");
+    return guarded([&] { e->storage->write(path); });
+}
+
+int vdq_engine_open(vdq_engine *e, const char *path) {
+    if (!e || !path) return fail(VD_EINVAL, "null argument");
+    return guarded([&] {
+        e->stats.reset();
+        e->storage.reset();
+        e->storage.reset(new VedasStorage(e->ctx.get(), path));
+    });
+}
+
 int vdq_engine_load_device(vdq_engine *e, const uint32_t *d_s, const uint32_t *d_p,
                            const uint32_t *d_o, uint64_t n, uint32_t perm_mask) {
     if (!e) return fail(VD_EINVAL, "null engine");
