@@ -412,6 +412,23 @@ def main():
             roofline["traffic_over_alg"] = measured / alg_per_launch if alg_per_launch else None
             roofline["traffic_source"] = os.path.relpath(traffic_file, ROOT)
 
+    # The same queries planned by the query graph planner (no PLAN clause):
+    # time and result size beside the given plans (outside the timed steps).
+    # Final rows can differ between plans when the store has duplicate
+    # triples: the finalise step removes adjacent duplicates only.
+    planned = {}
+    for q in names:
+        pats = JOIN_SET[q][0]
+        text = query_text(pats, None)
+        r = eng.query(text)  # warm
+        r.free()
+        r = eng.query(text)
+        st = r.stats()
+        r.free()
+        planned[q] = {"ms": st["elapsed_ms"], "rows": st["rows"], "join_rows": st["join_rows"],
+                      "rows_given_plan": per_q[q][0]["rows"]}
+    log("[bench] planner: " + ", ".join(f"{q} {v['ms']:.1f} ms" for q, v in planned.items()))
+
     batch = serve = None
     if args.mix > 0:
         mix = query_mix(args.mix, 3, STORE_100M, s, p, o)
@@ -458,6 +475,7 @@ def main():
                         "frac": build_alg / build_s / HBM_PEAK, "keys_per_s": 4 * n / build_s,
                         "kernels": kernel_table(build_prof)},
         "autotune": tune,
+        "planned_queries": planned,
         "range_selects": sel,
         "batched_mix": batch,
         "serving": serve,

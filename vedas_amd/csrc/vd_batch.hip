@@ -88,6 +88,8 @@ __global__ __launch_bounds__(256) void k_join_ranges_multi(const JPairDev *P, in
     ranges[2 * b + 1] = dev::upper_bound(rk, lo, D.nr, lk[last]);
 }
 
+// As k_join_bounds (vd_join.hip): the right range, or a directory of every
+// k-th key of it, in LDS.
 __global__ __launch_bounds__(MB_BLOCK) void k_join_bounds_multi(
     const JPairDev *P, int np, const uint64_t *tbase, const uint64_t *ranges, uint32_t *lower,
     uint32_t *cnt) {
@@ -101,9 +103,9 @@ __global__ __launch_bounds__(MB_BLOCK) void k_join_bounds_multi(
     const uint64_t base = (blockIdx.x - tbase[j]) * (uint64_t)MB_TILE;
     const uint64_t nl = D.nl;
     const uint64_t rlo = ranges[2 * blockIdx.x], rhi = ranges[2 * blockIdx.x + 1], m = rhi - rlo;
-    const bool staged = m <= (uint64_t)MJ_RCAP;
-    if (staged)
-        for (uint64_t t = threadIdx.x; t < m; t += MB_BLOCK) s_r[t] = rk[rlo + t];
+    const uint64_t k = m <= (uint64_t)MJ_RCAP ? 1 : (m + MJ_RCAP - 1) / MJ_RCAP;
+    const uint64_t nd = (m + k - 1) / k;
+    for (uint64_t t = threadIdx.x; t < nd; t += MB_BLOCK) s_r[t] = rk[rlo + t * k];
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < MB_ITEMS; ++it) {
@@ -111,15 +113,18 @@ __global__ __launch_bounds__(MB_BLOCK) void k_join_bounds_multi(
         if (i >= nl) break;
         const uint32_t key = lk[i];
         uint64_t lo, hi;
-        if (staged) {
+        if (k == 1) {
             lo = dev::lower_bound(s_r, 0, m, key);
-            hi = dev::upper_bound(s_r, lo, m, key) + rlo;
-            lo += rlo;
+            hi = dev::upper_bound(s_r, lo, m, key);
         } else {
-            lo = dev::lower_bound(rk, rlo, rhi, key);
-            hi = dev::upper_bound(rk, lo, rhi, key);
+            const uint64_t t0 = dev::lower_bound(s_r, 0, nd, key);
+            const uint64_t t1 = dev::upper_bound(s_r, t0, nd, key);
+            lo = dev::lower_bound(rk, rlo + (t0 ? (t0 - 1) * k + 1 : 0), rlo + min(t0 * k, m),
+                                  key) - rlo;
+            hi = dev::upper_bound(rk, rlo + (t1 ? (t1 - 1) * k + 1 : 0), rlo + min(t1 * k, m),
+                                  key) - rlo;
         }
-        lower[D.lbase + i] = (uint32_t)lo;
+        lower[D.lbase + i] = (uint32_t)(rlo + lo);
         cnt[D.lbase + i] = (uint32_t)(hi - lo);
     }
 }

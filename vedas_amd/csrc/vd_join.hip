@@ -2,10 +2,10 @@
 //
 //  1. k_join_bounds : for every left row i, lower_i = first right row with
 //                     key >= L[i] and cnt_i = number of right rows with key
-//                     == L[i].  A block takes 2048 consecutive left rows,
-//                     finds the right range they can touch with two global
-//                     binary searches, and stages that range in LDS when it
-//                     fits (otherwise it searches in global memory).
+//                     == L[i].  A block takes 2048 consecutive left rows;
+//                     the right range they can touch (k_join_ranges) is
+//                     staged in LDS, or a directory of every k-th key of it
+//                     when it is longer than the LDS capacity.
 //  2. scan          : off_i = exclusive scan of cnt_i (64-bit); the total is
 //                     the output size (read back once to allocate).
 //  3. k_join_partition / k_join_expand : load-balanced expansion.  The merge
@@ -92,6 +92,10 @@ __global__ __launch_bounds__(256) void k_join_ranges(KeyCols L, uint64_t nl, Key
     ranges[2 * b + 1] = g_upper<NK>(R, lo, nr, KeyT<NK>::at(L, last));
 }
 
+// The right range of a tile goes to LDS whole when it has at most rcap
+// keys.  A longer range (dense right side, hot keys) is staged as a
+// directory of every k-th key (k = ceil(m / rcap)): a search runs in LDS
+// over the directory, then over at most k - 1 keys in global memory.
 template <int NK>
 __global__ __launch_bounds__(JB_BLOCK) void k_join_bounds(
     KeyCols L, uint64_t nl, KeyCols R, uint64_t nr, int rcap,
@@ -102,11 +106,9 @@ __global__ __launch_bounds__(JB_BLOCK) void k_join_bounds(
     const uint64_t base = (uint64_t)blockIdx.x * JB_TILE;
     const uint64_t rlo = ranges[2 * blockIdx.x], rhi = ranges[2 * blockIdx.x + 1];
     const uint64_t m = rhi - rlo;
-    const bool staged = m <= (uint64_t)rcap && m <= JB_RCAP_BYTES / sizeof(T);
-    if (staged) {
-        for (uint64_t t = threadIdx.x; t < m; t += JB_BLOCK)
-            s_r[t] = KeyT<NK>::at(R, rlo + t);
-    }
+    const uint64_t k = m <= (uint64_t)rcap ? 1 : (m + rcap - 1) / rcap;  // directory stride
+    const uint64_t nd = (m + k - 1) / k;                                 // directory entries
+    for (uint64_t t = threadIdx.x; t < nd; t += JB_BLOCK) s_r[t] = KeyT<NK>::at(R, rlo + t * k);
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < JB_ITEMS; ++j) {
@@ -114,16 +116,19 @@ __global__ __launch_bounds__(JB_BLOCK) void k_join_bounds(
         if (i >= nl) break;
         const T key = KeyT<NK>::at(L, i);
         uint64_t lo, hi;
-        if (staged) {
+        if (k == 1) {
             lo = dev::lower_bound(s_r, 0, m, key);
             hi = dev::upper_bound(s_r, lo, m, key);
-            lo += rlo;
-            hi += rlo;
         } else {
-            lo = g_lower<NK>(R, rlo, rhi, key);
-            hi = g_upper<NK>(R, lo, rhi, key);
+            // first directory entry >= key / > key, then the k-1 keys before it
+            const uint64_t t0 = dev::lower_bound(s_r, 0, nd, key);
+            const uint64_t t1 = dev::upper_bound(s_r, t0, nd, key);
+            lo = g_lower<NK>(R, rlo + (t0 ? (t0 - 1) * k + 1 : 0), rlo + min(t0 * k, m), key) -
+                 rlo;
+            hi = g_upper<NK>(R, rlo + (t1 ? (t1 - 1) * k + 1 : 0), rlo + min(t1 * k, m), key) -
+                 rlo;
         }
-        lower[i] = (uint32_t)lo;
+        lower[i] = (uint32_t)(rlo + lo);
         cnt[i] = (uint32_t)(hi - lo);
     }
 }
