@@ -48,6 +48,30 @@ int vdq_engine_load(vdq_engine *e, const uint32_t *s, const uint32_t *p,
 int vdq_engine_load_device(vdq_engine *e, const uint32_t *d_s,
                            const uint32_t *d_p, const uint32_t *d_o,
                            uint64_t n, uint32_t perm_mask);
+/* Front-ends (no counterpart API in the reference, which reads N-Triples
+ * with raptor2 and SPARQL with rasqal: loader.cu:12-93, SparqlQuery.cu:
+ * 29-124).  A dictionary numbers the terms of an N-Triples file as the
+ * reference's loader does (subjects/objects from 1, predicates from 1,
+ * literal objects from 5e7 with literal_dict); vdq_engine_load_dict builds
+ * the engine's store from its triples and keeps a pointer to the
+ * dictionary (the caller keeps it alive).  vdq_sparql runs a SPARQL
+ * SELECT over a basic graph pattern (PREFIX, SELECT vars|*, WHERE {
+ * triples with ; and , }) by mapping its terms to IDs (unknown terms match
+ * nothing) and planning it with the query graph planner; vdq_dict_term
+ * turns result IDs back into terms (predicate columns:
+ * vdq_result_is_predicate). */
+typedef struct vdq_dict vdq_dict;
+int vdq_dict_create(int literal_dict, vdq_dict **out);
+int vdq_dict_free(vdq_dict *d);
+int vdq_dict_load_ntriples(vdq_dict *d, const char *path, uint64_t *n_triples);
+int vdq_dict_load_ntriples_text(vdq_dict *d, const char *text, uint64_t *n_triples);
+int vdq_dict_triples(const vdq_dict *d, uint32_t *s, uint32_t *p, uint32_t *o);
+int vdq_dict_term(const vdq_dict *d, int is_predicate, uint32_t id, char *out, size_t cap);
+int vdq_dict_translate(const vdq_dict *d, const char *sparql, char *out, size_t cap);
+int vdq_engine_load_dict(vdq_engine *e, const vdq_dict *d, uint32_t perm_mask);
+int vdq_sparql(vdq_engine *e, const char *sparql, vdq_result **out);
+int vdq_result_is_predicate(const vdq_result *r, int col);
+
 /* The .vds file of the reference's default index (PSO and POS):
  * VedasStorage::write / VedasStorage::open (VedasStorage.cu:452-515).
  * open replaces the engine's store with the file's (no sort). */

@@ -60,6 +60,16 @@ def lib() -> C.CDLL:
             "vdq_engine_load": (i, [vp, vp, vp, vp, u64, C.c_uint32]),
             "vdq_engine_load_device": (i, [vp, vp, vp, vp, u64, C.c_uint32]),
             "vdq_engine_ctx": (vp, [vp]),
+            "vdq_dict_create": (i, [i, C.POINTER(vp)]),
+            "vdq_dict_free": (i, [vp]),
+            "vdq_dict_load_ntriples": (i, [vp, C.c_char_p, C.POINTER(u64)]),
+            "vdq_dict_load_ntriples_text": (i, [vp, C.c_char_p, C.POINTER(u64)]),
+            "vdq_dict_triples": (i, [vp, vp, vp, vp]),
+            "vdq_dict_term": (i, [vp, i, C.c_uint32, C.c_char_p, C.c_size_t]),
+            "vdq_dict_translate": (i, [vp, C.c_char_p, C.c_char_p, C.c_size_t]),
+            "vdq_engine_load_dict": (i, [vp, vp, C.c_uint32]),
+            "vdq_sparql": (i, [vp, C.c_char_p, C.POINTER(vp)]),
+            "vdq_result_is_predicate": (i, [vp, i]),
             "vdq_engine_save": (i, [vp, C.c_char_p]),
             "vdq_engine_open": (i, [vp, C.c_char_p]),
             "vdq_engine_index": (vp, [vp]),
@@ -112,6 +122,17 @@ class Result:
             out.append(a)
         return out
 
+    def is_predicate(self, col: int) -> bool:
+        return bool(lib().vdq_result_is_predicate(self.h, col))
+
+    def terms(self, d: "Dict"):
+        """Result rows as term strings through the dictionary."""
+        cols = self.columns()
+        preds = [self.is_predicate(c) for c in range(len(cols))]
+        n = len(cols[0]) if cols else 0
+        return [tuple(d.term(int(cols[c][r]), preds[c]) for c in range(len(cols)))
+                for r in range(n)]
+
     def stats(self) -> dict:
         s = Stats()
         _check(lib().vdq_result_stats(self.h, C.byref(s)))
@@ -145,6 +166,19 @@ class Engine:
             mask |= 1 << _native.PERMS[name]
         _check(lib().vdq_engine_load_device(self.h, C.c_void_p(ds), C.c_void_p(dp),
                                             C.c_void_p(do), n, mask))
+
+    def load_dict(self, d: "Dict", perms=("PSO", "POS")) -> None:
+        """Build the store from a dictionary's triples; SPARQL then works."""
+        mask = 0
+        for name in perms:
+            mask |= 1 << _native.PERMS[name]
+        _check(lib().vdq_engine_load_dict(self.h, d.h, mask))
+        self._dict = d
+
+    def sparql(self, text: str) -> "Result":
+        h = C.c_void_p()
+        _check(lib().vdq_sparql(self.h, text.encode(), C.byref(h)))
+        return Result(h)
 
     def save(self, path: str) -> None:
         """Write the store as a .vds file (the reference's default index)."""
@@ -205,6 +239,49 @@ class Engine:
     def close(self):
         if self.h:
             lib().vdq_engine_free(self.h)
+            self.h = None
+
+
+class Dict:
+    """Term dictionary of an N-Triples store (host only)."""
+
+    def __init__(self, literal_dict: bool = False):
+        h = C.c_void_p()
+        _check(lib().vdq_dict_create(int(literal_dict), C.byref(h)))
+        self.h = h
+        self.n = 0
+
+    def load(self, path: str) -> int:
+        n = C.c_uint64()
+        _check(lib().vdq_dict_load_ntriples(self.h, path.encode(), C.byref(n)))
+        self.n = n.value
+        return self.n
+
+    def load_text(self, text: str) -> int:
+        n = C.c_uint64()
+        _check(lib().vdq_dict_load_ntriples_text(self.h, text.encode(), C.byref(n)))
+        self.n = n.value
+        return self.n
+
+    def triples(self):
+        import numpy as np
+        s, p, o = (np.empty(self.n, np.uint32) for _ in range(3))
+        _check(lib().vdq_dict_triples(self.h, s.ctypes.data, p.ctypes.data, o.ctypes.data))
+        return s, p, o
+
+    def term(self, ident: int, is_predicate: bool = False) -> str:
+        buf = C.create_string_buffer(1 << 16)
+        _check(lib().vdq_dict_term(self.h, int(is_predicate), ident, buf, len(buf)))
+        return buf.value.decode()
+
+    def translate(self, sparql: str) -> str:
+        buf = C.create_string_buffer(1 << 16)
+        _check(lib().vdq_dict_translate(self.h, sparql.encode(), buf, len(buf)))
+        return buf.value.decode()
+
+    def close(self):
+        if self.h:
+            lib().vdq_dict_free(self.h)
             self.h = None
 
 

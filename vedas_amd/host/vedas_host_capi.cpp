@@ -7,6 +7,7 @@
 #include <sstream>
 
 #include "BatchExecutor.h"
+#include "Dictionary.h"
 #include "QueryExecutor.h"
 #include "QueryGraph.h"
 #include "Server.h"
@@ -24,6 +25,7 @@ struct vdq_engine {
         if (!stats) stats.reset(new Statistics(storage.get()));
         return *stats;
     }
+    const vdq_dict *dict = nullptr;  // terms of the loaded store (caller owned)
     // Tunable values chosen by vdq_autotune; every context gets them.
     std::vector<std::pair<std::string, int>> tuned;
     void apply_tuned(VedasContext *c) {
@@ -43,7 +45,13 @@ struct vdq_engine {
 struct vdq_result {
     SparqlResult result;
     std::vector<std::string> headers;
+    std::vector<int> is_predicate;
     vdq_stats_t stats{};
+};
+
+struct vdq_dict {
+    Dictionary d;
+    explicit vdq_dict(bool ldict) : d(ldict) {}
 };
 
 namespace {
@@ -220,7 +228,10 @@ void ensure_plan(vdq_engine *e, ParsedQuery &pq) {
 
 void fill_stats(vdq_result *res, VedasContext *ctx) {
     FullRelationIR *ir = res->result.getResultIR();
-    for (size_t i = 0; ir && i < ir->getColumnNum(); ++i) res->headers.push_back(ir->getHeader(i));
+    for (size_t i = 0; ir && i < ir->getColumnNum(); ++i) {
+        res->headers.push_back(ir->getHeader(i));
+        res->is_predicate.push_back(ir->getIsPredicate(i) ? 1 : 0);
+    }
     res->stats.rows = ir ? ir->size() : 0;
     if (!ctx) return;
     res->stats.join_rows = ctx->join_rows;
@@ -276,6 +287,83 @@ int vdq_engine_load(vdq_engine *e, const uint32_t *s, const uint32_t *p, const u
         }
         for (auto x : d) vd_dev_free(c, x);
     });
+}
+
+int vdq_dict_create(int literal_dict, vdq_dict **out) {
+    if (!out) return fail(VD_EINVAL, "null argument");
+    *out = nullptr;
+    return guarded([&] { *out = new vdq_dict(literal_dict != 0); });
+}
+
+int vdq_dict_free(vdq_dict *d) {
+    delete d;
+    return VD_OK;
+}
+
+int vdq_dict_load_ntriples(vdq_dict *d, const char *path, uint64_t *n_triples) {
+    if (!d || !path) return fail(VD_EINVAL, "null argument");
+    return guarded([&] {
+        d->d.loadNTriples(path);
+        if (n_triples) *n_triples = d->d.subjects().size();
+    });
+}
+
+int vdq_dict_load_ntriples_text(vdq_dict *d, const char *text, uint64_t *n_triples) {
+    if (!d || !text) return fail(VD_EINVAL, "null argument");
+    return guarded([&] {
+        d->d.loadNTriplesText(text);
+        if (n_triples) *n_triples = d->d.subjects().size();
+    });
+}
+
+int vdq_dict_triples(const vdq_dict *d, uint32_t *s, uint32_t *p, uint32_t *o) {
+    if (!d || !s || !p || !o) return fail(VD_EINVAL, "null argument");
+    const size_t n = d->d.subjects().size();
+    std::memcpy(s, d->d.subjects().data(), n * 4);
+    std::memcpy(p, d->d.predicates().data(), n * 4);
+    std::memcpy(o, d->d.objects().data(), n * 4);
+    return VD_OK;
+}
+
+int vdq_dict_term(const vdq_dict *d, int is_predicate, uint32_t id, char *out, size_t cap) {
+    if (!d || !out || cap == 0) return fail(VD_EINVAL, "null argument");
+    const std::string t = d->d.term(id, is_predicate != 0);
+    if (t.empty()) return fail(VD_EINVAL, "unknown ID " + std::to_string(id));
+    if (t.size() + 1 > cap) return fail(VD_ERANGE, "term longer than the buffer");
+    std::memcpy(out, t.c_str(), t.size() + 1);
+    return VD_OK;
+}
+
+int vdq_dict_translate(const vdq_dict *d, const char *sparql, char *out, size_t cap) {
+    if (!d || !sparql || !out || cap == 0) return fail(VD_EINVAL, "null argument");
+    return guarded([&] {
+        const std::string t = translateSparql(parseSparql(sparql), d->d);
+        if (t.size() + 1 > cap) throw VedasError(VD_ERANGE, "query text longer than the buffer");
+        std::memcpy(out, t.c_str(), t.size() + 1);
+    });
+}
+
+int vdq_engine_load_dict(vdq_engine *e, const vdq_dict *d, uint32_t perm_mask) {
+    if (!e || !d) return fail(VD_EINVAL, "null argument");
+    const size_t n = d->d.subjects().size();
+    const int rc = vdq_engine_load(e, d->d.subjects().data(), d->d.predicates().data(),
+                                   d->d.objects().data(), n, perm_mask);
+    if (rc == VD_OK) e->dict = d;
+    return rc;
+}
+
+int vdq_sparql(vdq_engine *e, const char *sparql, vdq_result **out) {
+    if (!e || !sparql || !out) return fail(VD_EINVAL, "null argument");
+    if (!e->dict) return fail(VD_EINVAL, "engine store has no dictionary (load it with vdq_engine_load_dict)");
+    std::string text;
+    const int rc = guarded([&] { text = translateSparql(parseSparql(sparql), e->dict->d); });
+    if (rc != VD_OK) return rc;
+    return vdq_query(e, text.c_str(), out);
+}
+
+int vdq_result_is_predicate(const vdq_result *r, int col) {
+    if (!r || col < 0 || col >= (int)r->is_predicate.size()) return 0;
+    return r->is_predicate[col];
 }
 
 int vdq_engine_save(vdq_engine *e, const char *path) {
