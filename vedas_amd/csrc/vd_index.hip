@@ -288,17 +288,33 @@ struct Pieces {
     uint64_t dst[MAX_PIECES + 1];  // dst[np] = total rows
 };
 
+// Binary searches of one thread, or wave-cooperative ones (every lane of a
+// wave calls with the same arguments; see dev::wave_bound).
+template <bool WAVE>
+struct Search {
+    static __device__ __forceinline__ uint64_t lower(const uint32_t *a, uint64_t lo, uint64_t hi,
+                                                     uint32_t v) {
+        return WAVE ? dev::wave_bound<false>(a, lo, hi, v) : dev::lower_bound(a, lo, hi, v);
+    }
+    static __device__ __forceinline__ uint64_t upper(const uint32_t *a, uint64_t lo, uint64_t hi,
+                                                     uint32_t v) {
+        return WAVE ? dev::wave_bound<true>(a, lo, hi, v) : dev::upper_bound(a, lo, hi, v);
+    }
+};
+
 // [b, e) minus clip and skip intervals as pieces of consecutive rows.
 // Intervals must be sorted by lo and disjoint.
+template <bool WAVE>
 __device__ Pieces make_pieces(const uint32_t *__restrict__ ycol, uint64_t b, uint64_t e,
                               int clip, uint32_t lo, uint32_t hi, const vd_interval_t *sk,
                               int nskip) {
+    using S = Search<WAVE>;
     if (clip) {
         if (lo > hi) {
             e = b;
         } else {
-            const uint64_t nb = dev::lower_bound(ycol, b, e, lo);
-            e = dev::upper_bound(ycol, nb, e, hi);
+            const uint64_t nb = S::lower(ycol, b, e, lo);
+            e = S::upper(ycol, nb, e, hi);
             b = nb;
         }
     }
@@ -307,8 +323,8 @@ __device__ Pieces make_pieces(const uint32_t *__restrict__ ycol, uint64_t b, uin
     uint64_t cur = b, acc = 0;
     for (int i = 0; i < nskip; ++i) {
         if (sk[i].lo > sk[i].hi) continue;
-        const uint64_t a = dev::lower_bound(ycol, cur, e, sk[i].lo);
-        const uint64_t c = dev::upper_bound(ycol, a, e, sk[i].hi);
+        const uint64_t a = S::lower(ycol, cur, e, sk[i].lo);
+        const uint64_t c = S::upper(ycol, a, e, sk[i].hi);
         if (a > cur) {
             P.src[P.np] = cur;
             P.dst[P.np] = acc;
@@ -331,12 +347,13 @@ struct SkipSet {
     vd_interval_t s[4];
 };
 
-// One thread: pieces of a given row range.
-__global__ void k_select_pieces(const uint32_t *__restrict__ ycol, uint64_t b, uint64_t e,
-                                int clip, uint32_t lo, uint32_t hi, SkipSet sk, int nskip,
-                                Pieces *out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    *out = make_pieces(ycol, b, e, clip, lo, hi, sk.s, nskip);
+// One wave: pieces of a given row range.
+__global__ __launch_bounds__(64) void k_select_pieces(const uint32_t *__restrict__ ycol,
+                                                      uint64_t b, uint64_t e, int clip,
+                                                      uint32_t lo, uint32_t hi, SkipSet sk,
+                                                      int nskip, Pieces *out) {
+    const Pieces P = make_pieces<true>(ycol, b, e, clip, lo, hi, sk.s, nskip);
+    if (threadIdx.x == 0) *out = P;
 }
 
 // Pieces plus the key's row range, so a lookup and a select cost one
@@ -346,15 +363,37 @@ struct KeyPieces {
     uint64_t begin, end;
 };
 
-__global__ void k_lookup_pieces(PermDev pd, uint32_t k1, int has_k2, uint32_t k2, int clip,
-                                uint32_t lo, uint32_t hi, SkipSet sk, int nskip,
-                                KeyPieces *out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// The key lookup of lookup_one, by one wave.
+__device__ __forceinline__ void wave_lookup(const PermDev &pd, uint32_t k1, uint32_t k2,
+                                            bool has_k2, uint64_t *b, uint64_t *e) {
+    const uint64_t i = dev::wave_bound<false>(pd.l1v, 0, pd.n_l1, k1);
+    if (i >= pd.n_l1 || pd.l1v[i] != k1) {
+        const uint64_t pos = i < pd.n_l1 ? pd.l1o[i] : pd.n;
+        *b = *e = pos;
+        return;
+    }
+    uint64_t lo = pd.l1o[i];
+    uint64_t hi = (i + 1 < pd.n_l1) ? pd.l1o[i + 1] : pd.n;
+    if (has_k2) {
+        lo = dev::wave_bound<false>(pd.y, lo, hi, k2);
+        hi = dev::wave_bound<true>(pd.y, lo, hi, k2);
+    }
+    *b = lo;
+    *e = hi;
+}
+
+__global__ __launch_bounds__(64) void k_lookup_pieces(PermDev pd, uint32_t k1, int has_k2,
+                                                      uint32_t k2, int clip, uint32_t lo,
+                                                      uint32_t hi, SkipSet sk, int nskip,
+                                                      KeyPieces *out) {
     uint64_t b, e;
-    lookup_one(pd, k1, k2, has_k2 != 0, &b, &e);
-    out->begin = b;
-    out->end = e;
-    out->p = make_pieces(pd.y, b, e, clip, lo, hi, sk.s, nskip);
+    wave_lookup(pd, k1, k2, has_k2 != 0, &b, &e);
+    const Pieces P = make_pieces<true>(pd.y, b, e, clip, lo, hi, sk.s, nskip);
+    if (threadIdx.x == 0) {
+        out->begin = b;
+        out->end = e;
+        out->p = P;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_copy_pieces(

@@ -335,5 +335,32 @@ __device__ __forceinline__ uint64_t upper_bound(const T *a, uint64_t lo,
     return lo;
 }
 
+// Wave-cooperative search over [lo, hi): the first index whose element is
+// >= v (UPPER = false) or > v (UPPER = true).  Each round the 64 lanes probe
+// the last element of 64 equal segments and a ballot picks the segment the
+// bound lies in, so a search of n elements takes about log64(n) dependent
+// loads instead of log2(n).  Every lane of the wave must call it with the
+// same arguments; every lane gets the result.
+template <bool UPPER, typename T, typename V>
+__device__ __forceinline__ uint64_t wave_bound(const T *a, uint64_t lo, uint64_t hi, V v) {
+    const int lane = lane_id();
+    auto before = [&](uint64_t i) {  // the bound lies after index i
+        const T x = a[i];
+        return UPPER ? !(v < x) : (x < v);
+    };
+    while (hi - lo > (uint64_t)WAVE) {
+        const uint64_t step = (hi - lo + WAVE - 1) / WAVE;
+        const uint64_t q = lo + (uint64_t)(lane + 1) * step - 1;  // last of segment `lane`
+        const bool b = q < hi && before(q);
+        const int c = __popcll(__ballot(b));  // segments wholly before the bound
+        const uint64_t qc = lo + (uint64_t)(c + 1) * step - 1;  // last of segment c
+        hi = qc < hi ? qc + 1 : hi;
+        lo = min(lo + (uint64_t)c * step, hi);  // c == 64: every element before
+    }
+    const uint64_t q = lo + (uint64_t)lane;
+    const bool b = q < hi && before(q);
+    return lo + (uint64_t)__popcll(__ballot(b));
+}
+
 }  // namespace dev
 }  // namespace vd

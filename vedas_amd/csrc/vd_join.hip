@@ -153,26 +153,26 @@ __global__ __launch_bounds__(256) void k_join_partition(
 // scanned offset; the first output row comes from the first left row with
 // matches (first i with off[i+1] > 0), the last from the last one (last i
 // with off[i] < total).
-__global__ void k_join_total_keys(const uint64_t *__restrict__ off, uint64_t nl,
-                                  const uint32_t *__restrict__ lkey, uint64_t *out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One wave (wave-cooperative searches over the offsets).
+__global__ __launch_bounds__(64) void k_join_total_keys(const uint64_t *__restrict__ off,
+                                                        uint64_t nl,
+                                                        const uint32_t *__restrict__ lkey,
+                                                        uint64_t *out) {
     const uint64_t total = off[nl];
-    out[0] = total;
-    out[1] = out[2] = 0;
-    if (total == 0) return;
-    uint64_t lo = 0, hi = nl;  // first i with off[i + 1] > 0
-    while (lo < hi) {
-        const uint64_t mid = lo + ((hi - lo) >> 1);
-        if (off[mid + 1] > 0) hi = mid; else lo = mid + 1;
+    uint64_t first = 0, last = 0;
+    if (total > 0) {
+        // first i with off[i + 1] > 0 is the first index of off[1..nl] above 0
+        const uint64_t i0 = dev::wave_bound<true>(off + 1, 0, nl, (uint64_t)0);
+        // last i with off[i] < total: the first index with off >= total, minus one
+        const uint64_t i1 = dev::wave_bound<false>(off, 0, nl, total) - 1;
+        first = lkey[i0];
+        last = lkey[i1];
     }
-    out[1] = lkey[lo];
-    lo = 0;
-    hi = nl;  // first i with off[i] >= total, minus one
-    while (lo < hi) {
-        const uint64_t mid = lo + ((hi - lo) >> 1);
-        if (off[mid] >= total) hi = mid; else lo = mid + 1;
+    if (threadIdx.x == 0) {
+        out[0] = total;
+        out[1] = first;
+        out[2] = last;
     }
-    out[2] = lkey[lo - 1];
 }
 
 struct OutCols {
