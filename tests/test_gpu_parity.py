@@ -377,6 +377,41 @@ def test_sorts_and_unique(ctx, n, ncols, keymax):
     r.free()
 
 
+@pytest.mark.parametrize("items", [8, 16, 24])
+@pytest.mark.parametrize("ncols", [1, 2, 3, 4, 5, 6])
+def test_sort_col0_every_tile_shape(ctx, items, ncols):
+    """Every radix tile shape (keys only and with 1-3 payload columns, whose
+    tiles hold 8192 / 6144 / 4096 rows) at sizes one past a tile boundary,
+    with duplicate keys so stability shows."""
+    from vedas_amd._native import Rel
+    ctx.tune("radix_items", items)
+    try:
+        for n, keymax in ((4097, 2**31), (6145, 300), (8193, 2**20), (24_577, 7),
+                          (100_003, 2**32 - 1)):
+            print(f"sort_col0 items={items} ncols={ncols} n={n}", flush=True)
+            rng = np.random.default_rng(n * 31 + ncols)
+            cols = [rng.integers(0, keymax, n, dtype=np.uint64).astype(np.uint32)]
+            cols += [rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+                     for _ in range(ncols - 1)]
+            r = Rel.from_columns(ctx, cols)
+            r.sort_col0()
+            for g, w in zip(r.columns(), O.sort_col0(cols)):
+                assert np.array_equal(g, w)
+            r.free()
+    finally:
+        ctx.tune("radix_items", 16)
+
+
+@pytest.mark.parametrize("items", [8, 24])
+def test_index_build_tile_shapes(ctx, items):
+    ctx.tune("radix_items", items)
+    try:
+        s, p, o = rand_triples(70_001, 5 + items)
+        check_index(ctx, s, p, o, ["SPO", "PSO", "OSP"])
+    finally:
+        ctx.tune("radix_items", 16)
+
+
 @pytest.mark.parametrize("shape", ["star2", "star4", "chain4", "snow"])
 def test_plans_cfg0_store(ctx, store, shape):
     s, p, o, idx, orc = store

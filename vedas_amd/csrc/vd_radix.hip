@@ -66,17 +66,28 @@ __global__ __launch_bounds__(RBINS) void k_rs_scan_hist(const uint32_t *ghist,
 }
 
 // Look-back window: predecessor tiles read per round trip.  The published
-// values cross XCDs (agent scope), so each round trip is long; reading a
-// window of tiles at once cuts the number of trips.
-constexpr int LB_WINDOW = 8;
+// values cross XCDs (agent scope), so each round trip is long; reading two
+// tiles at once halves the trips, while wider windows cost more in extra
+// loads than they save (measured on 100M keys: windows of 1/2/4/8 tiles give
+// 2.99/3.19/3.18/3.10 TB/s per pass).
+constexpr int LB_WINDOW = 2;
+
+// Occupancy target (waves per SIMD) of each tile shape: two 512-thread
+// blocks per CU, which needs <= 128 VGPRs and <= 80 KB of LDS per block.
+template <int BLOCK, int ITEMS, int M>
+struct OsShape {
+    static constexpr int WAVES_PER_EU = (BLOCK == 512 && ITEMS * (2 + M) <= 48) ? 4 : 1;
+};
 
 // M 32-bit payload columns travel with the keys (pay.in[c] -> pay.out[c]);
-// they are loaded with the keys and staged through one LDS buffer, column
-// after column, so every global access stays coalesced.
+// they are loaded with the keys and staged, column after column, through the
+// same LDS buffer as the keys once those are written out, so every global
+// access stays coalesced and the block keeps two-per-CU occupancy.
 // BLOCK threads (a multiple of 256: threads 0..255 own one digit each in the
 // count and look-back phases), ITEMS keys per thread.
 template <int BLOCK, int ITEMS, int M>
-__global__ __launch_bounds__(BLOCK) void k_rs_onesweep(
+__global__ __launch_bounds__(BLOCK)
+__attribute__((amdgpu_waves_per_eu(OsShape<BLOCK, ITEMS, M>::WAVES_PER_EU, 8))) void k_rs_onesweep(
     const uint64_t *__restrict__ in, uint64_t *__restrict__ out, RadixPayload pay,
     uint64_t n, int shift, int width, const uint32_t *__restrict__ gofs,
     uint32_t *lookback, uint32_t *tile_counter) {
@@ -84,9 +95,9 @@ __global__ __launch_bounds__(BLOCK) void k_rs_onesweep(
     constexpr int TILE = BLOCK * ITEMS;
     static_assert(BLOCK % RBINS == 0, "one digit per thread in the count phase");
     static_assert(M >= 0 && M <= RADIX_MAX_PAYLOAD, "payload columns");
-    __shared__ uint64_t s_keys[TILE];
-    __shared__ uint32_t s_vals[M ? TILE : 1];
-    __shared__ uint32_t s_wcnt[WAVES][RBINS];
+    static_assert(TILE <= 65535, "16-bit wave counters");
+    __shared__ uint64_t s_keys[TILE];  // keys, then each payload column (as u32)
+    __shared__ uint16_t s_wcnt[WAVES][RBINS];
     __shared__ uint32_t s_lstart[RBINS];
     __shared__ uint32_t s_gbase[RBINS];
     __shared__ uint32_t s_tmp[WAVES];
@@ -96,7 +107,8 @@ __global__ __launch_bounds__(BLOCK) void k_rs_onesweep(
     const int lane = tid & 63;
     const int w = tid >> 6;
 
-    for (int i = tid; i < WAVES * RBINS; i += BLOCK) (&s_wcnt[0][0])[i] = 0;
+    for (int i = tid; i < WAVES * RBINS / 2; i += BLOCK)
+        reinterpret_cast<uint32_t *>(&s_wcnt[0][0])[i] = 0;
     if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
     __syncthreads();
 
@@ -119,14 +131,16 @@ __global__ __launch_bounds__(BLOCK) void k_rs_onesweep(
         for (int c = 0; c < M; ++c) v[c][j] = idx < n ? pay.in[c][idx] : 0u;
     }
 
-    // Rank within the wave: peers = lanes holding the same digit.
+    // Rank within the wave: peers = lanes holding the same digit (digit bits
+    // above `width` are zero for every lane, so their ballots change nothing).
     const uint64_t lt = (1ull << lane) - 1ull;
     uint32_t rk[ITEMS];
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
         const uint32_t d = (uint32_t)(k[j] >> shift) & dmask;
         uint64_t peers = ~0ull;
-        for (int b = 0; b < width; ++b) {
+#pragma unroll
+        for (int b = 0; b < RBITS; ++b) {
             const int bit = (d >> b) & 1;
             const uint64_t bal = __ballot(bit);
             peers &= bit ? bal : ~bal;
@@ -135,7 +149,7 @@ __global__ __launch_bounds__(BLOCK) void k_rs_onesweep(
         const uint32_t r = (uint32_t)__popcll(peers & lt);
         const uint32_t cnt = (uint32_t)__popcll(peers);
         __builtin_amdgcn_wave_barrier();
-        if (r == cnt - 1) s_wcnt[w][d] = before + cnt;
+        if (r == cnt - 1) s_wcnt[w][d] = (uint16_t)(before + cnt);
         __builtin_amdgcn_wave_barrier();
         rk[j] = before + r;
     }
@@ -149,7 +163,7 @@ __global__ __launch_bounds__(BLOCK) void k_rs_onesweep(
 #pragma unroll
         for (int ww = 0; ww < WAVES; ++ww) {
             const uint32_t c = s_wcnt[ww][tid];
-            s_wcnt[ww][tid] = sum;
+            s_wcnt[ww][tid] = (uint16_t)sum;
             sum += c;
         }
         tcount = sum;
@@ -202,33 +216,31 @@ __global__ __launch_bounds__(BLOCK) void k_rs_onesweep(
         const uint32_t d = (uint32_t)(k[j] >> shift) & dmask;
         pos[j] = s_lstart[d] + s_wcnt[w][d] + rk[j];
         s_keys[pos[j]] = k[j];
-        if (M) s_vals[pos[j]] = v[0][j];
     }
     __syncthreads();
 
-    // Staged position i (i = tid + BLOCK * r) goes to dst[r].
+    // Staged position i (i = tid + BLOCK * r) goes to dst[r].  Positions at
+    // or past nvalid hold the padding keys and are not written.
     uint32_t dst[ITEMS];
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
         const uint32_t i = (uint32_t)(tid + r * BLOCK);
-        if (i < nvalid) {
-            const uint64_t key = s_keys[i];
-            const uint32_t d = (uint32_t)(key >> shift) & dmask;
-            dst[r] = s_gbase[d] + i;
-            out[dst[r]] = key;
-            if (M) pay.out[0][dst[r]] = s_vals[i];
-        }
+        const uint64_t key = s_keys[i];
+        const uint32_t d = (uint32_t)(key >> shift) & dmask;
+        dst[r] = s_gbase[d] + i;
+        if (i < nvalid) out[dst[r]] = key;
     }
+    uint32_t *s_v = reinterpret_cast<uint32_t *>(s_keys);
 #pragma unroll
-    for (int c = 1; c < M; ++c) {
-        __syncthreads();
+    for (int c = 0; c < M; ++c) {
+        __syncthreads();  // the previous reads of the staging buffer are done
 #pragma unroll
-        for (int j = 0; j < ITEMS; ++j) s_vals[pos[j]] = v[c][j];
+        for (int j = 0; j < ITEMS; ++j) s_v[pos[j]] = v[c][j];
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < ITEMS; ++r) {
             const uint32_t i = (uint32_t)(tid + r * BLOCK);
-            if (i < nvalid) pay.out[c][dst[r]] = s_vals[i];
+            if (i < nvalid) pay.out[c][dst[r]] = s_v[i];
         }
     }
 }
@@ -334,8 +346,9 @@ static uint64_t *sort_impl(vd_ctx *ctx, uint64_t *a, uint64_t *b, int m,
            k_rs_scan_hist, (const uint32_t *)d_hist, gofs.p);
     // Tile shapes (threads x keys per thread), chosen by the radix_items
     // tunable: 16 -> 512 x 16 (default), 8 -> 512 x 8, 24 -> 256 x 24 (keys
-    // only); payload columns need LDS staging too, so at most 16 keys per
-    // thread with them.
+    // only).  With payload columns the default shape narrows as columns are
+    // added (512 x 16 / 12 / 8 for 1 / 2 / 3 columns), which keeps the
+    // registers of a thread within two blocks per CU.
     const int items = ctx->radix_items;
     const uint32_t *g = gofs.p;
 #define VD_RUN(BL, IT, MM) run_passes<BL, IT, MM>(ctx, a, b, src, pa, pb, final, n, lo_bit, hi_bit, npass, g)
@@ -349,8 +362,8 @@ static uint64_t *sort_impl(vd_ctx *ctx, uint64_t *a, uint64_t *b, int m,
         } else {
             switch (m) {
                 case 1: VD_RUN(512, 16, 1); break;
-                case 2: VD_RUN(512, 16, 2); break;
-                default: VD_RUN(512, 16, 3); break;
+                case 2: VD_RUN(512, 12, 2); break;
+                default: VD_RUN(512, 8, 3); break;
             }
         }
         return a;
