@@ -229,6 +229,14 @@ typedef struct {
  * view of the permutation's rows. */
 int  vd_select_multi(vd_ctx *ctx, const vd_index *idx,
                      const vd_select_job_t *jobs, int njobs, vd_rel **outs);
+/* begin[i]/end[i] = vd_index_lookup of job i (its perm, k1 and k2; cols is
+ * not used), for every job in one kernel launch and one read-back.  The
+ * plan executor resolves all of a plan's select ranges with it before the
+ * first operator runs; in the reference each SelectQueryJob does its own
+ * findL2OffsetFromL1 / findDataOffsetFromL2 (QueryExecutor.cu:596-646). */
+int  vd_index_lookup_multi(vd_ctx *ctx, const vd_index *idx,
+                           const vd_select_job_t *jobs, int njobs,
+                           uint64_t *begin, uint64_t *end);
 /* outs[i] = vd_join(L[i], R[i], 1). */
 int  vd_join_multi(vd_ctx *ctx, const vd_rel *const *L, const vd_rel *const *R,
                    int npairs, vd_rel **outs);

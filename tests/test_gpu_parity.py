@@ -131,6 +131,20 @@ def test_lookup_single_and_batch(ctx, store):
                 assert (gb[q], ge[q]) == want[q]
         for d in (dk1, dk2, db, de):
             ctx.free(d)
+    # every permutation in one batched call
+    from vedas_amd._native import lookup_multi
+    jobs, want = [], []
+    for perm in ["SPO", "PSO", "POS", "OSP"]:
+        ix = orc[perm]
+        for a in list(rng.choice(ix["l1v"], 20)) + [0, 2**32 - 2]:
+            b = None if rng.random() < 0.5 else int(rng.choice(ix["y"]))
+            jobs.append((perm, int(a), b))
+            want.append(O.lookup(ix, int(a)) if b is None else O.lookup(ix, int(a), b))
+    got = lookup_multi(ctx, idx, jobs)
+    for g, w in zip(got, want):
+        assert g[1] - g[0] == w[1] - w[0]
+        if w[1] > w[0]:
+            assert g == w
 
 
 def test_select_modes(ctx, store):

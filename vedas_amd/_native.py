@@ -103,6 +103,8 @@ EXPORTS = {
                           C.POINTER(C.c_void_p)]),
     "vd_select_multi": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                   C.POINTER(C.c_void_p)]),
+    "vd_index_lookup_multi": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                        C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "vd_join_multi": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                 C.c_int, C.POINTER(C.c_void_p)]),
     "vd_sort_col0_multi": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int]),
@@ -430,6 +432,17 @@ def select_multi(ctx: Ctx, index: "Index", jobs) -> list:
     outs = (C.c_void_p * max(1, len(jobs)))()
     ctx.check(ctx.L.vd_select_multi(ctx.h, index.h, C.cast(arr, C.c_void_p), len(jobs), outs))
     return [Rel(ctx, C.c_void_p(outs[i])) for i in range(len(jobs))]
+
+
+def lookup_multi(ctx: Ctx, index: "Index", jobs) -> list:
+    """jobs: [(perm name, k1, k2 or None)] -> [(begin, end)], one launch."""
+    arr = (SelectJob * max(1, len(jobs)))(*[
+        SelectJob(PERMS[p], k1, VD_UNBOUND if k2 is None else k2, 1) for p, k1, k2 in jobs])
+    b = (C.c_uint64 * max(1, len(jobs)))()
+    e = (C.c_uint64 * max(1, len(jobs)))()
+    ctx.check(ctx.L.vd_index_lookup_multi(ctx.h, index.h, C.cast(arr, C.c_void_p), len(jobs),
+                                          b, e))
+    return [(int(b[i]), int(e[i])) for i in range(len(jobs))]
 
 
 def join_multi(ctx: Ctx, lefts, rights) -> list:

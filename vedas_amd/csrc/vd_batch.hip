@@ -390,13 +390,12 @@ std::vector<uint64_t> tile_prefix(const std::vector<uint64_t> &rows) {
     return t;
 }
 
-void select_multi(vd_ctx *ctx, const vd_index *idx, const vd_select_job_t *jobs, int nj,
-                  vd_rel **outs) {
+// Row ranges of many lookups (any permutations): hr[2j], hr[2j+1].
+std::vector<uint64_t> lookup_multi(vd_ctx *ctx, const vd_index *idx, const vd_select_job_t *jobs,
+                                   int nj) {
     std::vector<SelJobDev> hj(nj);
     for (int j = 0; j < nj; ++j) {
         const vd_perm_arrays &pa = get_perm(idx, jobs[j].perm);
-        VD_REQUIRE(jobs[j].cols == 1 || jobs[j].cols == 3, VD_EINVAL,
-                   "batched select supports cols 1 and 3");
         hj[j] = SelJobDev{perm_dev(pa), jobs[j].k1, jobs[j].k2};
     }
     DBuf<SelJobDev> dj = to_device(ctx, hj);
@@ -405,6 +404,15 @@ void select_multi(vd_ctx *ctx, const vd_index *idx, const vd_select_job_t *jobs,
            k_sel_lookup_multi, (const SelJobDev *)dj.p, nj, range.p);
     std::vector<uint64_t> hr(2 * (size_t)nj);
     download(ctx, hr.data(), range.p, hr.size() * sizeof(uint64_t));
+    return hr;
+}
+
+void select_multi(vd_ctx *ctx, const vd_index *idx, const vd_select_job_t *jobs, int nj,
+                  vd_rel **outs) {
+    for (int j = 0; j < nj; ++j)
+        VD_REQUIRE(jobs[j].cols == 1 || jobs[j].cols == 3, VD_EINVAL,
+                   "batched select supports cols 1 and 3");
+    const std::vector<uint64_t> hr = lookup_multi(ctx, idx, jobs, nj);
     // Each result is the contiguous row range of its permutation: a view.
     for (int j = 0; j < nj; ++j) outs[j] = nullptr;
     try {
@@ -648,6 +656,19 @@ int vd_select_multi(vd_ctx *ctx, const vd_index *idx, const vd_select_job_t *job
     if (!ctx || njobs < 0 || (njobs && (!jobs || !outs))) return VD_EINVAL;
     if (njobs == 0) return VD_OK;
     VD_TRY(ctx, select_multi(ctx, idx, jobs, njobs, outs));
+}
+
+int vd_index_lookup_multi(vd_ctx *ctx, const vd_index *idx, const vd_select_job_t *jobs,
+                          int njobs, uint64_t *begin, uint64_t *end) {
+    if (!ctx || njobs < 0 || (njobs && (!jobs || !begin || !end))) return VD_EINVAL;
+    if (njobs == 0) return VD_OK;
+    VD_TRY(ctx, {
+        const std::vector<uint64_t> hr = lookup_multi(ctx, idx, jobs, njobs);
+        for (int j = 0; j < njobs; ++j) {
+            begin[j] = hr[2 * j];
+            end[j] = hr[2 * j + 1];
+        }
+    });
 }
 
 int vd_join_multi(vd_ctx *ctx, const vd_rel *const *L, const vd_rel *const *R, int npairs,

@@ -5,9 +5,13 @@
 //                     == L[i].  A block takes 2048 consecutive left rows;
 //                     the right range they can touch (k_join_ranges) is
 //                     staged in LDS, or a directory of every k-th key of it
-//                     when it is longer than the LDS capacity.
-//  2. scan          : off_i = exclusive scan of cnt_i (64-bit); the total is
-//                     the output size (read back once to allocate).
+//                     when it is longer than the LDS capacity.  The same
+//                     kernel scans the counts of its tile: loc_i = exclusive
+//                     scan of cnt_i within the tile, plus the tile total.
+//  2. k_join_tile_scan : one block scans the tile totals into tpre (tpre[t] =
+//                     rows before tile t; tpre[ntiles] = output size, read
+//                     back once to allocate).  The output offset of left row
+//                     i is off_i = tpre[i / 2048] + loc_i (join_off below).
 //  3. k_join_partition / k_join_expand : load-balanced expansion.  The merge
 //                     of the left rows (row i at merge position i + off_i)
 //                     with the output indices is cut into diagonals of NV
@@ -100,43 +104,107 @@ template <int NK>
 __global__ __launch_bounds__(JB_BLOCK) void k_join_bounds(
     KeyCols L, uint64_t nl, KeyCols R, uint64_t nr, int rcap,
     const uint64_t *__restrict__ ranges, uint32_t *__restrict__ lower,
-    uint32_t *__restrict__ cnt) {
+    uint64_t *__restrict__ loc, uint64_t *__restrict__ tile_total) {
     using T = typename KeyT<NK>::T;
-    __shared__ T s_r[JB_RCAP_BYTES / sizeof(T)];
-    const uint64_t base = (uint64_t)blockIdx.x * JB_TILE;
-    const uint64_t rlo = ranges[2 * blockIdx.x], rhi = ranges[2 * blockIdx.x + 1];
+    constexpr int SR_WORDS = JB_RCAP_BYTES / sizeof(T);
+    static_assert(JB_RCAP_BYTES >= JB_TILE * 12, "counts and offsets reuse the staging buffer");
+    // Right keys; once the searches are done, the tile's offsets (u64, row
+    // order) and counts (u32, row order) in the same space.
+    __shared__ T s_r[SR_WORDS];
+    __shared__ uint64_t s_tmp[JB_BLOCK / 64];
+    const uint64_t tile = blockIdx.x;
+    const uint64_t base = tile * JB_TILE;
+    const uint64_t rlo = ranges[2 * tile], rhi = ranges[2 * tile + 1];
     const uint64_t m = rhi - rlo;
     const uint64_t k = m <= (uint64_t)rcap ? 1 : (m + rcap - 1) / rcap;  // directory stride
     const uint64_t nd = (m + k - 1) / k;                                 // directory entries
     for (uint64_t t = threadIdx.x; t < nd; t += JB_BLOCK) s_r[t] = KeyT<NK>::at(R, rlo + t * k);
     __syncthreads();
+    uint32_t cnt[JB_ITEMS];
 #pragma unroll
     for (int j = 0; j < JB_ITEMS; ++j) {
         const uint64_t i = base + (uint64_t)j * JB_BLOCK + threadIdx.x;
-        if (i >= nl) break;
-        const T key = KeyT<NK>::at(L, i);
-        uint64_t lo, hi;
-        if (k == 1) {
-            lo = dev::lower_bound(s_r, 0, m, key);
-            hi = dev::upper_bound(s_r, lo, m, key);
-        } else {
-            // first directory entry >= key / > key, then the k-1 keys before it
-            const uint64_t t0 = dev::lower_bound(s_r, 0, nd, key);
-            const uint64_t t1 = dev::upper_bound(s_r, t0, nd, key);
-            lo = g_lower<NK>(R, rlo + (t0 ? (t0 - 1) * k + 1 : 0), rlo + min(t0 * k, m), key) -
-                 rlo;
-            hi = g_upper<NK>(R, rlo + (t1 ? (t1 - 1) * k + 1 : 0), rlo + min(t1 * k, m), key) -
-                 rlo;
+        uint32_t c = 0;
+        if (i < nl) {
+            const T key = KeyT<NK>::at(L, i);
+            uint64_t lo, hi;
+            if (k == 1) {
+                lo = dev::lower_bound(s_r, 0, m, key);
+                hi = dev::upper_bound(s_r, lo, m, key);
+            } else {
+                // first directory entry >= key / > key, then the k-1 keys before it
+                const uint64_t t0 = dev::lower_bound(s_r, 0, nd, key);
+                const uint64_t t1 = dev::upper_bound(s_r, t0, nd, key);
+                lo = g_lower<NK>(R, rlo + (t0 ? (t0 - 1) * k + 1 : 0), rlo + min(t0 * k, m),
+                                 key) - rlo;
+                hi = g_upper<NK>(R, rlo + (t1 ? (t1 - 1) * k + 1 : 0), rlo + min(t1 * k, m),
+                                 key) - rlo;
+            }
+            lower[i] = (uint32_t)(rlo + lo);
+            c = (uint32_t)(hi - lo);
         }
-        lower[i] = (uint32_t)(rlo + lo);
-        cnt[i] = (uint32_t)(hi - lo);
+        cnt[j] = c;
+    }
+    __syncthreads();  // the right keys are no longer read
+    uint64_t *s_off = reinterpret_cast<uint64_t *>(s_r);
+    uint32_t *s_cnt = reinterpret_cast<uint32_t *>(s_off + JB_TILE);
+#pragma unroll
+    for (int j = 0; j < JB_ITEMS; ++j) s_cnt[j * JB_BLOCK + threadIdx.x] = cnt[j];
+    __syncthreads();
+    // Thread t scans rows t*ITEMS .. t*ITEMS+ITEMS-1 of the tile.
+    uint64_t run = 0;
+    uint32_t mine[JB_ITEMS];
+#pragma unroll
+    for (int j = 0; j < JB_ITEMS; ++j) {
+        mine[j] = s_cnt[threadIdx.x * JB_ITEMS + j];
+        run += mine[j];
+    }
+    uint64_t tot;
+    uint64_t ex = dev::block_excl_scan<JB_BLOCK>(run, s_tmp, &tot);
+#pragma unroll
+    for (int j = 0; j < JB_ITEMS; ++j) {
+        s_off[threadIdx.x * JB_ITEMS + j] = ex;
+        ex += mine[j];
+    }
+    if (threadIdx.x == 0) tile_total[tile] = tot;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < JB_ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)j * JB_BLOCK + threadIdx.x;
+        if (i < nl) loc[i] = s_off[j * JB_BLOCK + threadIdx.x];
     }
 }
+
+// tpre[t] = sum of tile_total[0..t), t = 0..ntiles (one block, chunks of
+// 1024 tiles with a running carry).
+__global__ __launch_bounds__(1024) void k_join_tile_scan(const uint64_t *__restrict__ tile_total,
+                                                         uint64_t ntiles,
+                                                         uint64_t *__restrict__ tpre) {
+    __shared__ uint64_t s_tmp[1024 / 64];
+    uint64_t carry = 0;
+    for (uint64_t b = 0; b < ntiles; b += 1024) {
+        const uint64_t t = b + threadIdx.x;
+        const uint64_t v = t < ntiles ? tile_total[t] : 0;
+        uint64_t tot;
+        const uint64_t ex = dev::block_excl_scan<1024>(v, s_tmp, &tot);
+        if (t < ntiles) tpre[t] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) tpre[ntiles] = carry;
+}
+
+// Output offset of left row i (i < nl): tile prefix + offset within tile.
+struct JoinOff {
+    const uint64_t *loc, *tpre;
+    __device__ __forceinline__ uint64_t operator()(uint64_t i) const {
+        return tpre[i / JB_TILE] + loc[i];
+    }
+};
 
 // mp[b] = number of left rows i with i + off[i] < b * nv (clamped to the
 // end of the diagonal), for b = 0 .. nblocks.
 __global__ __launch_bounds__(256) void k_join_partition(
-    const uint64_t *__restrict__ off, uint64_t nl, uint64_t total, int nv,
+    JoinOff off, uint64_t nl, uint64_t total, int nv,
     uint64_t nblocks, uint64_t *__restrict__ mp) {
     const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b > nblocks) return;
@@ -144,27 +212,39 @@ __global__ __launch_bounds__(256) void k_join_partition(
     uint64_t lo = 0, hi = nl;
     while (lo < hi) {
         const uint64_t mid = lo + ((hi - lo) >> 1);
-        if (mid + off[mid] < diag) lo = mid + 1; else hi = mid;
+        if (mid + off(mid) < diag) lo = mid + 1; else hi = mid;
     }
     mp[b] = lo;
 }
 
-// out = {total, first output key, last output key}: the total is the last
-// scanned offset; the first output row comes from the first left row with
-// matches (first i with off[i+1] > 0), the last from the last one (last i
-// with off[i] < total).
-// One wave (wave-cooperative searches over the offsets).
-__global__ __launch_bounds__(64) void k_join_total_keys(const uint64_t *__restrict__ off,
-                                                        uint64_t nl,
+// out = {total, first output key, last output key}: the total is the
+// last tile prefix; the first output row comes from the first left row with
+// matches (first i with off[i+1] > 0, off[nl] = total), the last from the
+// last one (last i with off[i] < total).
+// One wave (64-ary searches over the offsets).
+__global__ __launch_bounds__(64) void k_join_total_keys(JoinOff off, uint64_t nl, uint64_t ntiles,
                                                         const uint32_t *__restrict__ lkey,
                                                         uint64_t *out) {
-    const uint64_t total = off[nl];
+    const uint64_t total = off.tpre[ntiles];
+    auto at = [&](uint64_t i) { return i < nl ? off(i) : total; };
+    // first index q in [lo, hi) with pred(q) true, for a monotone pred
+    auto search = [&](uint64_t lo, uint64_t hi, auto pred) {
+        const int lane = threadIdx.x & 63;
+        while (hi - lo > 64) {
+            const uint64_t step = (hi - lo + 63) / 64;
+            const uint64_t q = lo + (uint64_t)(lane + 1) * step - 1;
+            const int c = __popcll(__ballot(q < hi && !pred(q)));
+            const uint64_t qc = lo + (uint64_t)(c + 1) * step - 1;
+            hi = qc < hi ? qc + 1 : hi;
+            lo = min(lo + (uint64_t)c * step, hi);
+        }
+        const uint64_t q = lo + (uint64_t)lane;
+        return lo + (uint64_t)__popcll(__ballot(q < hi && !pred(q)));
+    };
     uint64_t first = 0, last = 0;
     if (total > 0) {
-        // first i with off[i + 1] > 0 is the first index of off[1..nl] above 0
-        const uint64_t i0 = dev::wave_bound<true>(off + 1, 0, nl, (uint64_t)0);
-        // last i with off[i] < total: the first index with off >= total, minus one
-        const uint64_t i1 = dev::wave_bound<false>(off, 0, nl, total) - 1;
+        const uint64_t i0 = search(0, nl, [&](uint64_t i) { return at(i + 1) > 0; });
+        const uint64_t i1 = search(0, nl, [&](uint64_t i) { return at(i) >= total; }) - 1;
         first = lkey[i0];
         last = lkey[i1];
     }
@@ -184,7 +264,7 @@ struct OutCols {
 
 template <int NK, int NV>
 __global__ __launch_bounds__(JX_BLOCK) void k_join_expand(
-    const uint64_t *__restrict__ off, const uint32_t *__restrict__ lower,
+    JoinOff off, const uint32_t *__restrict__ lower,
     const uint64_t *__restrict__ mp, uint64_t nl, uint64_t total, OutCols oc) {
     constexpr int ITEMS = NV / JX_BLOCK;
     __shared__ uint64_t s_off[NV + 1];
@@ -198,7 +278,7 @@ __global__ __launch_bounds__(JX_BLOCK) void k_join_expand(
     const uint64_t sa = a0 > 0 ? a0 - 1 : 0;
     const int m = (int)(a1 - sa);
     for (int t = threadIdx.x; t < m; t += JX_BLOCK) {
-        s_off[t] = off[sa + t];
+        s_off[t] = off(sa + t);
         s_low[t] = lower[sa + t];
     }
     __syncthreads();
@@ -255,20 +335,25 @@ vd_rel *join_impl(vd_ctx *ctx, const vd_rel *L, const vd_rel *R) {
 
     KeyCols lk{L->cols[0], NK == 2 ? L->cols[1] : nullptr};
     KeyCols rk{R->cols[0], NK == 2 ? R->cols[1] : nullptr};
-    DBuf<uint32_t> lower(ctx, nl), cnt(ctx, nl);
-    DBuf<uint64_t> off(ctx, nl + 1);
+    DBuf<uint32_t> lower(ctx, nl);
+    DBuf<uint64_t> off(ctx, nl);  // offsets within each bounds tile
     const uint64_t jb_tiles = ceil_div(nl, JB_TILE);
     const int rcap = std::min<int>(ctx->join_rcap, JB_RCAP_BYTES / (NK == 2 ? 8 : 4));
-    DBuf<uint64_t> ranges(ctx, 2 * jb_tiles);
+    // ranges (2 per tile), tile totals (1 per tile), tile prefixes (1 per
+    // tile + the total).
+    DBuf<uint64_t> ranges(ctx, 4 * jb_tiles + 1);
+    uint64_t *ttot = ranges.p + 2 * jb_tiles, *tpre = ranges.p + 3 * jb_tiles;
     launch(ctx, "join_ranges", 0.0, dim3((unsigned)ceil_div(jb_tiles, 256)), dim3(256), 0,
            k_join_ranges<NK>, lk, nl, rk, nr, jb_tiles, ranges.p);
-    launch(ctx, "join_bounds", 4.0 * NK * (double)(nl + nr) + 8.0 * nl,
+    launch(ctx, "join_bounds", 4.0 * NK * (double)(nl + nr) + 12.0 * nl,
            dim3((unsigned)jb_tiles), dim3(JB_BLOCK), 0, k_join_bounds<NK>, lk,
-           nl, rk, nr, rcap, (const uint64_t *)ranges.p, lower.p, cnt.p);
-    scan_u32_to_u64(ctx, cnt.p, off.p, nl, off.p + nl);
+           nl, rk, nr, rcap, (const uint64_t *)ranges.p, lower.p, off.p, ttot);
+    launch(ctx, "join_tile_scan", 8.0 * (double)jb_tiles, dim3(1), dim3(1024), 0,
+           k_join_tile_scan, (const uint64_t *)ttot, jb_tiles, tpre);
+    const JoinOff joff{off.p, tpre};
     DBuf<uint64_t> tk(ctx, 3);
-    launch(ctx, "join_total", 0.0, dim3(1), dim3(64), 0, k_join_total_keys,
-           (const uint64_t *)off.p, nl, (const uint32_t *)L->cols[0], tk.p);
+    launch(ctx, "join_total", 0.0, dim3(1), dim3(64), 0, k_join_total_keys, joff, nl, jb_tiles,
+           (const uint32_t *)L->cols[0], tk.p);
     uint64_t h[3];
     read_words(ctx, h, tk.p, 3);
     const uint64_t total = h[0];
@@ -291,13 +376,13 @@ vd_rel *join_impl(vd_ctx *ctx, const vd_rel *L, const vd_rel *R) {
         VD_REQUIRE(nblocks < (1ull << 31), VD_ERANGE, "join expansion grid too large");
         DBuf<uint64_t> mp(ctx, nblocks + 1);
         launch(ctx, "join_partition", 0.0, dim3((unsigned)ceil_div(nblocks + 1, 256)),
-               dim3(256), 0, k_join_partition, (const uint64_t *)off.p, nl, total,
+               dim3(256), 0, k_join_partition, joff, nl, total,
                nv, nblocks, mp.p);
         const double xbytes = 4.0 * (double)total * out_cols + 12.0 * nl +
                               4.0 * (double)nl * (L->ncols) + 4.0 * (double)nr * (R->ncols - NK);
         auto go = [&](auto kern) {
             launch(ctx, "join_expand", xbytes, dim3((unsigned)nblocks), dim3(JX_BLOCK),
-                   0, kern, (const uint64_t *)off.p, (const uint32_t *)lower.p,
+                   0, kern, joff, (const uint32_t *)lower.p,
                    (const uint64_t *)mp.p, nl, total, oc);
         };
         if (nv == 1024) go(k_join_expand<NK, 1024>);

@@ -65,6 +65,20 @@ __global__ __launch_bounds__(RBINS) void k_rs_scan_hist(const uint32_t *ghist,
     gofs[blockIdx.x * RBINS + threadIdx.x] = ex;
 }
 
+// Exclusive scan of each digit position's histogram (one block per
+// position) and a flag per position: 1 when the digit varies (no bin holds
+// all n rows), 0 when every row has the same digit there.
+__global__ __launch_bounds__(RBINS) void k_rs_scan_hist_flags(const uint32_t *ghist, uint64_t n,
+                                                              uint32_t *gofs, uint64_t *varies) {
+    __shared__ uint32_t s_tmp[RBINS / 64];
+    const uint32_t v = ghist[blockIdx.x * RBINS + threadIdx.x];
+    uint32_t tot;
+    const uint32_t ex = dev::block_excl_scan<RBINS>(v, s_tmp, &tot);
+    gofs[blockIdx.x * RBINS + threadIdx.x] = ex;
+    const int all_one_bin = __syncthreads_or((uint64_t)v == n);
+    if (threadIdx.x == 0) varies[blockIdx.x] = all_one_bin ? 0 : 1;
+}
+
 // Look-back window: predecessor tiles read per round trip.  The published
 // values cross XCDs (agent scope), so each round trip is long; reading two
 // tiles at once halves the trips, while wider windows cost more in extra
@@ -248,10 +262,13 @@ __attribute__((amdgpu_waves_per_eu(OsShape<BLOCK, ITEMS, M>::WAVES_PER_EU, 8))) 
 // Pass p reads the keys from a and writes b (then swaps), and moves the
 // payload from src (first pass) or the buffer the previous pass wrote, into
 // pa (odd passes) or pb (even passes); *final receives the last buffers.
+// Pass p sorts on the digit at shifts[p] (widths[p] bits) with the 256
+// exclusive digit offsets at gofs[p].
 template <int BLOCK, int ITEMS, int M>
 void run_passes(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, const RadixCols &src,
                 const RadixCols &pa, const RadixCols &pb, RadixCols *final,
-                uint64_t n, int lo_bit, int hi_bit, int npass, const uint32_t *gofs) {
+                uint64_t n, const int *shifts, const int *widths, int npass,
+                const uint32_t *const *gofs) {
     constexpr uint64_t TILE = (uint64_t)BLOCK * ITEMS;
     const uint64_t ntiles = ceil_div(n, TILE);
     const uint64_t lb_words = (uint64_t)npass * ntiles * RBINS;
@@ -260,8 +277,6 @@ void run_passes(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, const RadixCols &src,
     uint32_t *counters = lb.p + lb_words;
     RadixCols cur = src;
     for (int p = 0; p < npass; ++p) {
-        const int shift = lo_bit + p * RBITS;
-        const int width = std::min(RBITS, hi_bit - shift);
         const RadixCols &dstc = (p % 2 == 0) ? pb : pa;
         RadixPayload pay{};
         for (int c = 0; c < M; ++c) {
@@ -271,12 +286,48 @@ void run_passes(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, const RadixCols &src,
         launch(ctx, M ? "radix_onesweep_kv" : "radix_onesweep",
                (double)n * (16.0 + 8.0 * M), dim3((unsigned)ntiles),
                dim3(BLOCK), 0, k_rs_onesweep<BLOCK, ITEMS, M>, (const uint64_t *)a, b, pay,
-               n, shift, width, gofs + (uint64_t)p * RBINS,
+               n, shifts[p], widths[p], gofs[p],
                lb.p + (uint64_t)p * ntiles * RBINS, counters + p);
         std::swap(a, b);
         cur = dstc;
     }
     if (final) *final = cur;
+}
+
+// Runs the passes with the tile shape the radix_items tunable selects:
+// 16 -> 512 x 16 (default), 8 -> 512 x 8, 24 -> 256 x 24 (keys only).  With
+// payload columns the default shape narrows as columns are added (512 x 16 /
+// 12 / 8 for 1 / 2 / 3 columns), which keeps the registers of a thread
+// within two blocks per CU.
+void dispatch_passes(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, int m, const RadixCols &src,
+                     const RadixCols &pa, const RadixCols &pb, RadixCols *final, uint64_t n,
+                     const int *shifts, const int *widths, int npass,
+                     const uint32_t *const *g) {
+    const int items = ctx->radix_items;
+#define VD_RUN(BL, IT, MM) \
+    run_passes<BL, IT, MM>(ctx, a, b, src, pa, pb, final, n, shifts, widths, npass, g)
+    if (m > 0) {
+        if (items == 8) {
+            switch (m) {
+                case 1: VD_RUN(512, 8, 1); break;
+                case 2: VD_RUN(512, 8, 2); break;
+                default: VD_RUN(512, 8, 3); break;
+            }
+        } else {
+            switch (m) {
+                case 1: VD_RUN(512, 16, 1); break;
+                case 2: VD_RUN(512, 12, 2); break;
+                default: VD_RUN(512, 8, 3); break;
+            }
+        }
+        return;
+    }
+    switch (items) {
+        case 8: VD_RUN(512, 8, 0); break;
+        case 24: VD_RUN(256, 24, 0); break;
+        default: VD_RUN(512, 16, 0); break;
+    }
+#undef VD_RUN
 }
 
 }  // namespace
@@ -344,37 +395,38 @@ static uint64_t *sort_impl(vd_ctx *ctx, uint64_t *a, uint64_t *b, int m,
     DBuf<uint32_t> gofs(ctx, (uint64_t)npass * RBINS);
     launch(ctx, "radix_scan_hist", 0.0, dim3(npass), dim3(RBINS), 0,
            k_rs_scan_hist, (const uint32_t *)d_hist, gofs.p);
-    // Tile shapes (threads x keys per thread), chosen by the radix_items
-    // tunable: 16 -> 512 x 16 (default), 8 -> 512 x 8, 24 -> 256 x 24 (keys
-    // only).  With payload columns the default shape narrows as columns are
-    // added (512 x 16 / 12 / 8 for 1 / 2 / 3 columns), which keeps the
-    // registers of a thread within two blocks per CU.
-    const int items = ctx->radix_items;
-    const uint32_t *g = gofs.p;
-#define VD_RUN(BL, IT, MM) run_passes<BL, IT, MM>(ctx, a, b, src, pa, pb, final, n, lo_bit, hi_bit, npass, g)
-    if (m > 0) {
-        if (items == 8) {
-            switch (m) {
-                case 1: VD_RUN(512, 8, 1); break;
-                case 2: VD_RUN(512, 8, 2); break;
-                default: VD_RUN(512, 8, 3); break;
-            }
-        } else {
-            switch (m) {
-                case 1: VD_RUN(512, 16, 1); break;
-                case 2: VD_RUN(512, 12, 2); break;
-                default: VD_RUN(512, 8, 3); break;
-            }
-        }
-        return a;
+    int shifts[radix::MAX_PASSES], widths[radix::MAX_PASSES];
+    const uint32_t *g[radix::MAX_PASSES];
+    for (int p = 0; p < npass; ++p) {
+        shifts[p] = lo_bit + p * RBITS;
+        widths[p] = std::min(RBITS, hi_bit - shifts[p]);
+        g[p] = gofs.p + (uint64_t)p * RBINS;
     }
-    switch (items) {
-        case 8: VD_RUN(512, 8, 0); break;
-        case 24: VD_RUN(256, 24, 0); break;
-        default: VD_RUN(512, 16, 0); break;
-    }
-#undef VD_RUN
+    dispatch_passes(ctx, a, b, m, src, pa, pb, final, n, shifts, widths, npass, g);
     return a;
+}
+
+uint64_t *radix_sort_shifts(vd_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n,
+                            const int *shifts, int npass, const uint32_t *const *d_gofs, int m,
+                            const RadixCols &src, const RadixCols &pa, const RadixCols &pb,
+                            RadixCols *final) {
+    VD_REQUIRE(m >= 0 && m <= RADIX_MAX_PAYLOAD, VD_EINVAL, "payload columns");
+    VD_REQUIRE(npass >= 0 && npass <= radix::MAX_PASSES, VD_EINVAL, "radix pass count");
+    if (final) *final = src;
+    if (npass == 0 || n <= 1) return keys;
+    VD_REQUIRE(n <= (uint64_t)LB_VAL, VD_ERANGE,
+               "radix sort supports at most 2^30-1 keys per call");
+    int widths[radix::MAX_PASSES];
+    for (int p = 0; p < npass; ++p) widths[p] = std::min(RBITS, 64 - shifts[p]);
+    uint64_t *a = keys, *b = tmp;
+    dispatch_passes(ctx, a, b, m, src, pa, pb, final, n, shifts, widths, npass, d_gofs);
+    return a;
+}
+
+void radix_scan_hist_flags(vd_ctx *ctx, const uint32_t *d_hist, int npos, uint64_t n,
+                           uint32_t *d_gofs, uint64_t *d_varies) {
+    launch(ctx, "radix_scan_hist", 0.0, dim3(npos), dim3(RBINS), 0, k_rs_scan_hist_flags,
+           d_hist, n, d_gofs, d_varies);
 }
 
 }  // namespace vd

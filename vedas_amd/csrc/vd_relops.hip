@@ -173,31 +173,46 @@ uint64_t *sort_by_col(vd_ctx *ctx, const uint32_t *col, const uint32_t *perm,
     return radix_sort_u64(ctx, keys.p, tmp.p, n, 32, hi, hist.p, true);
 }
 
-// Stable sort by column 0.  Column 1 rides in the low word of the 64-bit key
-// (only the bits of column 0 are sorted, so equal keys keep their order).
-// Columns 2..4 travel through the radix passes as payload columns (read from
-// the relation by the first pass); wider relations carry the row id as the
-// payload and gather columns 2.. once at the end.  Owned columns are reused
-// as buffers; borrowed (view) columns are replaced, never written.
+// Stable sort by column 0 (the reference's sortByFirstColumn).  The sort
+// key is col0 << 32 | col1, sorted on the bytes of column 0 only (LSD
+// passes are stable, so equal keys keep their order).  The packing kernel
+// counts the digit histograms of column 0's four bytes; they are read back
+// as one flag per byte, and the bytes that are equal in every row (the top
+// bytes of small IDs) get no pass.  Columns 2..4 travel through the passes
+// as payload columns (read from the relation by the first pass); wider
+// relations carry the row id and gather columns 2.. once at the end.  Owned
+// columns are reused as buffers; borrowed (view) columns are replaced,
+// never written.
 void sort_col0(vd_ctx *ctx, vd_rel *rel) {
     const uint64_t n = rel->nrows;
     const int nc = rel->ncols;
     if (n <= 1) return;
     rel->key_range = false;
     VD_REQUIRE(n < (1ull << 30), VD_ERANGE, "sort supports fewer than 2^30 rows");
-    const int b = std::max(1, bits_for(max_u32(ctx, rel->cols[0], n)));
-    const int hi = 32 + b;
-    const int npass = radix_passes(32, hi);
     const bool by_row = nc > 2 + RADIX_MAX_PAYLOAD;
     const int m = nc <= 2 ? 0 : (by_row ? 1 : nc - 2);
     DBuf<uint64_t> keys(ctx, n), ktmp(ctx, n);
     DBuf<uint32_t> rows(ctx, by_row ? n : 0);
-    DBuf<uint32_t> hist(ctx, radix_hist_words(32, hi));
-    fill_u32(ctx, hist.p, 0, radix_hist_words(32, hi));
+    DBuf<uint32_t> hist(ctx, 4 * 256), gofs(ctx, 4 * 256);
+    DBuf<uint64_t> varies(ctx, 4);
+    fill_u32(ctx, hist.p, 0, 4 * 256);
     launch(ctx, "sort_pack", (double)n * (8.0 + 4.0 * std::min(nc, 2) + (by_row ? 4.0 : 0.0)),
            dim3((unsigned)ceil_div(n, RB * R_ITEMS)), dim3(RB), 0, k_pack_c01,
            (const uint32_t *)rel->cols[0], (const uint32_t *)(nc >= 2 ? rel->cols[1] : nullptr),
-           (const uint32_t *)nullptr, n, keys.p, by_row ? rows.p : nullptr, hi, npass, hist.p);
+           (const uint32_t *)nullptr, n, keys.p, by_row ? rows.p : nullptr, 64, 4, hist.p);
+    radix_scan_hist_flags(ctx, hist.p, 4, n, gofs.p, varies.p);
+    uint64_t hv[4];
+    read_words(ctx, hv, varies.p, 4);
+    int shifts[4];
+    const uint32_t *g[4];
+    int npass = 0;
+    for (int d = 0; d < 4; ++d)
+        if (hv[d]) {
+            shifts[npass] = 32 + 8 * d;
+            g[npass] = gofs.p + d * 256;
+            ++npass;
+        }
+    if (npass == 0) return;  // column 0 is constant: already in order
 
     // Payload buffers: src = the relation's columns (or the row ids); pa
     // aliases src where the relation owns the column, pb is fresh.
@@ -221,8 +236,8 @@ void sort_col0(vd_ctx *ctx, vd_rel *rel) {
             }
             pb.p[c] = fresh_col();
         }
-        sorted = radix_sort_u64_payload(ctx, keys.p, ktmp.p, n, 32, hi, hist.p, true, m, src,
-                                        pa, pb, &fin);
+        sorted = radix_sort_shifts(ctx, keys.p, ktmp.p, n, shifts, npass, g, m, src, pa, pb,
+                                   &fin);
         for (int c = 0; c < std::min(nc, 2); ++c)
             out01[c] = rel_borrowed(rel, c) ? fresh_col() : rel->cols[c];
         if (by_row)

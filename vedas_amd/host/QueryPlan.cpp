@@ -2,9 +2,12 @@
 
 #include <climits>
 #include <iostream>
+#include <map>
+#include <vector>
 
 #include "FullRelationIR.h"
 #include "JoinQueryJob.h"
+#include "SelectQueryJob.h"
 
 QueryPlan::QueryPlan(const std::set<std::string> &sel) {
     for (auto &v : sel) select_variables.push_back(v);
@@ -23,8 +26,31 @@ void QueryPlan::pushJob(QueryJob *job) {
 
 QueryJob *QueryPlan::getJob(size_t i) { return jobs.at(i); }
 
+// Resolves the row range of every select of the plan with one batched
+// lookup per store (the ranges depend only on the pattern constants), so
+// that each select job then only filters its range: an unfiltered range is
+// a view and needs no device round trip.
+static void resolveSelectRanges(const std::vector<QueryJob *> &jobs) {
+    std::map<vd_index *, std::vector<SelectQueryJob *>> groups;
+    for (auto *job : jobs)
+        if (auto *sj = dynamic_cast<SelectQueryJob *>(job)) groups[sj->indexHandle()].push_back(sj);
+    for (auto &g : groups) {
+        const auto &sel = g.second;
+        if (sel.size() < 2) continue;  // one lookup: vd_select_key costs the same
+        std::vector<vd_select_job_t> lk;
+        for (auto *sj : sel) lk.push_back(sj->lookupJob());
+        std::vector<uint64_t> b(sel.size()), e(sel.size());
+        VedasContext *ctx = sel[0]->getContext();
+        ctx->check(vd_index_lookup_multi(ctx->raw(), g.first, lk.data(), (int)lk.size(),
+                                         b.data(), e.data()),
+                   "vd_index_lookup_multi");
+        for (size_t i = 0; i < sel.size(); ++i) sel[i]->setRange(b[i], e[i]);
+    }
+}
+
 // QueryPlan::execute, single-GPU branch (QueryPlan.cu:160-205).
 void QueryPlan::execute(SparqlResult &result) {
+    resolveSelectRanges(jobs);
     for (auto &v : select_variables) query_variable_counter[v] = ULONG_MAX;
     for (auto *job : jobs) {
         if (auto *jj = dynamic_cast<JoinQueryJob *>(job)) jj->setQueryVarCounter(&query_variable_counter);

@@ -2,6 +2,7 @@
 
 #include <chrono>
 #include <iostream>
+#include <limits>
 
 #include "QueryExecutor.h"
 
@@ -41,11 +42,26 @@ int SelectQueryJob::startJob(int gpuId) {
     uint64_t b = 0, e = 0;
     vd_rel *rel = nullptr;
     FullRelationIR *ir = nullptr;
-    // The key lookup and the select run as one device step (vd_select_key).
+    // The key lookup and the select run as one device step (vd_select_key),
+    // or, with the range resolved ahead of time, the select alone
+    // (vd_select: no device work for an unfiltered range, which is a view).
+    auto select = [&](int cols, int clip, TYPEID lo, TYPEID hi, const vd_interval_t *skip,
+                      int nskip, int distinct) {
+        if (has_range) {
+            b = range_begin;
+            e = range_end;
+            context->check(vd_select(c, idx, index.perm, b, e, cols, clip, lo, hi, skip, nskip,
+                                     distinct, &rel),
+                           "vd_select");
+        } else {
+            context->check(vd_select_key(c, idx, index.perm, dataid[0], variable_num == 1,
+                                         variable_num == 1 ? dataid[1] : 0, cols, clip, lo, hi,
+                                         skip, nskip, distinct, &rel, &b, &e),
+                           "vd_select_key");
+        }
+    };
     if (variable_num == 1) {
-        context->check(vd_select_key(c, idx, index.perm, dataid[0], 1, dataid[1], 1, 0, 0, 0,
-                                     nullptr, 0, 0, &rel, &b, &e),
-                       "vd_select_key");
+        select(1, 0, 0, 0, nullptr, 0, 0);
         ir = new FullRelationIR(context, rel);
         ir->setHeader(0, variables[0], is_predicates[0]);
         context->exe_log.emplace_back(gpuId, UPLOAD_OP, ir->getHeaders(""), ir->size(), 1);
@@ -58,23 +74,21 @@ int SelectQueryJob::startJob(int gpuId) {
         TYPEID lo = 0, hi = 0;
         if (variables_bound && variables_bound->count(variables[0]) > 0) {
             auto bound = (*variables_bound)[variables[0]];
-            clip = 1;
             lo = bound.first;
             hi = bound.second;
+            // A bound still covering every ID (a join variable before its
+            // first join) removes no row: the range stays a view.
+            clip = (lo == std::numeric_limits<TYPEID>::min() &&
+                    hi == std::numeric_limits<TYPEID>::max()) ? 0 : 1;
         }
         if (!is_second_var_used) {
-            context->check(vd_select_key(c, idx, index.perm, dataid[0], 0, 0, 2, clip, lo, hi,
-                                         nullptr, 0, 1, &rel, &b, &e),
-                           "vd_select_key");
+            select(2, clip, lo, hi, nullptr, 0, 1);
             ir = new FullRelationIR(context, rel);
             ir->setHeader(0, variables[0], is_predicates[0]);
         } else {
             std::vector<vd_interval_t> skip;
             if (ei_dict) skip = ei_dict->skipIntervals(variables[0]);
-            context->check(vd_select_key(c, idx, index.perm, dataid[0], 0, 0, 3, clip, lo, hi,
-                                         skip.empty() ? nullptr : skip.data(), (int)skip.size(),
-                                         0, &rel, &b, &e),
-                           "vd_select_key");
+            select(3, clip, lo, hi, skip.empty() ? nullptr : skip.data(), (int)skip.size(), 0);
             ir = new FullRelationIR(context, rel);
             ir->setHeader(0, variables[0], is_predicates[0]);
             ir->setHeader(1, variables[1], is_predicates[1]);
@@ -92,6 +106,23 @@ int SelectQueryJob::startJob(int gpuId) {
         planTreeNode->nanosecTime = ns;
     }
     return 0;
+}
+
+vd_select_job_t SelectQueryJob::lookupJob() const {
+    vd_select_job_t j{};
+    j.perm = index.perm;
+    j.k1 = dataid[0];
+    j.k2 = variable_num == 1 ? dataid[1] : VD_UNBOUND;
+    j.cols = variable_num == 1 ? 1 : 3;
+    return j;
+}
+
+vd_index *SelectQueryJob::indexHandle() const { return index.storage->index(); }
+
+void SelectQueryJob::setRange(uint64_t begin, uint64_t end) {
+    has_range = true;
+    range_begin = begin;
+    range_end = end;
 }
 
 IR *SelectQueryJob::getIR() { return intermediateResult; }

@@ -40,7 +40,18 @@ class SelectQueryJob : public QueryJob {
     void print() const override;
     std::string jobTypeName() const override;
 
+    // The lookup this select starts with (permutation and constants; the
+    // cols field is not used) and the store it reads.
+    vd_select_job_t lookupJob() const;
+    vd_index *indexHandle() const;
+    VedasContext *getContext() const { return context; }
+    // Row range of the lookup, resolved ahead of time (QueryPlan::execute
+    // batches the lookups of a plan): startJob then only filters it.
+    void setRange(uint64_t begin, uint64_t end);
+
   private:
+    bool has_range = false;
+    uint64_t range_begin = 0, range_end = 0;
     IndexRef index;
     std::string variables[3];
     size_t variable_num;
