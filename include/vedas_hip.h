@@ -272,6 +272,11 @@ int  vd_gen_triples(const vd_gen_params_t *prm, uint32_t *s, uint32_t *p,
  * context stream; per-kernel launch counts, device time and algorithmic
  * HBM bytes are folded in at the next vd_ctx_sync. */
 int  vd_prof_enable(vd_ctx *ctx, int on);
+/* Restricts the timing to the launches whose label is in the
+ * comma-separated list (NULL or "" = every launch): each timed launch adds
+ * two events to the stream, so a caller timing one kernel class keeps the
+ * others free of that cost. */
+int  vd_prof_only(vd_ctx *ctx, const char *labels);
 int  vd_prof_reset(vd_ctx *ctx);
 int  vd_prof_count(vd_ctx *ctx);
 int  vd_prof_get(vd_ctx *ctx, int i, const char **name, uint64_t *launches,

@@ -116,6 +116,7 @@ EXPORTS = {
                                  C.c_void_p]),
     "vd_prof_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "vd_prof_reset": (C.c_int, [C.c_void_p]),
+    "vd_prof_only": (C.c_int, [C.c_void_p, C.c_char_p]),
     "vd_prof_count": (C.c_int, [C.c_void_p]),
     "vd_prof_get": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_char_p),
                               C.POINTER(C.c_uint64), C.POINTER(C.c_double),
@@ -234,6 +235,10 @@ class Ctx:
     # -- profiling
     def prof(self, on: bool = True) -> None:
         self.check(self.L.vd_prof_enable(self.h, int(on)))
+
+    def prof_only(self, labels=None) -> None:
+        """Time only the launches with these labels (None: all)."""
+        self.check(self.L.vd_prof_only(self.h, ",".join(labels).encode() if labels else None))
 
     def prof_reset(self) -> None:
         self.check(self.L.vd_prof_reset(self.h))

@@ -3,6 +3,8 @@
 #include <algorithm>
 #include <cstring>
 
+#include <chrono>
+
 #include "vd_internal.hpp"
 
 namespace vd {
@@ -109,8 +111,23 @@ void prof_collect(vd_ctx *ctx) {
     ctx->pending.clear();
 }
 
+// Most waits here are short: a size read-back right after the kernel that
+// computes it.  Polling the stream for a while before the blocking wait
+// keeps the host thread's wake-up latency out of each of them.
 void sync(vd_ctx *ctx) {
-    VD_HIP(hipStreamSynchronize(ctx->stream));
+    constexpr double kSpinMicros = 200.0;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+        const hipError_t e = hipStreamQuery(ctx->stream);
+        if (e == hipSuccess) break;
+        if (e != hipErrorNotReady) VD_HIP(e);
+        const double us = std::chrono::duration<double, std::micro>(
+                              std::chrono::steady_clock::now() - t0).count();
+        if (us > kSpinMicros) {
+            VD_HIP(hipStreamSynchronize(ctx->stream));
+            break;
+        }
+    }
     if (!ctx->pending.empty()) prof_collect(ctx);
 }
 
@@ -404,6 +421,24 @@ int vd_prof_enable(vd_ctx *ctx, int on) {
     if (!ctx) return VD_EINVAL;
     ctx->prof = on != 0;
     return VD_OK;
+}
+
+int vd_prof_only(vd_ctx *ctx, const char *labels) {
+    if (!ctx) return VD_EINVAL;
+    VD_TRY(ctx, {
+        std::vector<std::string> v;
+        std::string cur;
+        for (const char *c = labels ? labels : ""; ; ++c) {
+            if (*c == ',' || *c == 0) {
+                if (!cur.empty()) v.push_back(cur);
+                cur.clear();
+                if (*c == 0) break;
+            } else {
+                cur += *c;
+            }
+        }
+        ctx->prof_only = v;
+    });
 }
 
 int vd_prof_reset(vd_ctx *ctx) {

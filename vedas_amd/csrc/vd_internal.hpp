@@ -92,8 +92,10 @@ struct vd_ctx {
     hipEvent_t stage_done = nullptr;
     bool stage_pending = false;
 
-    // Profiling.
+    // Profiling (of the launches whose label is in prof_only, or of all
+    // launches when it is empty).
     bool prof = false;
+    std::vector<std::string> prof_only;
     std::vector<vd::PendingLaunch> pending;
     std::vector<hipEvent_t> event_pool;
     std::vector<std::string> prof_names;
@@ -158,6 +160,12 @@ hipEvent_t take_event(vd_ctx *ctx);
 void prof_begin(vd_ctx *ctx, hipEvent_t *ev);
 void prof_end(vd_ctx *ctx, const char *name, double bytes, hipEvent_t ev0);
 void prof_collect(vd_ctx *ctx);  // after a stream sync
+inline bool prof_selected(const vd_ctx *ctx, const char *name) {
+    if (ctx->prof_only.empty()) return true;
+    for (const auto &l : ctx->prof_only)
+        if (l == name) return true;
+    return false;
+}
 
 // Launch `kernel` on the context stream; with profiling on, the launch is
 // bracketed by events and charged `bytes` algorithmic HBM bytes.
@@ -167,11 +175,12 @@ inline void launch(vd_ctx *ctx, const char *name, double bytes, dim3 grid,
                    Args &&...args) {
     if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
     hipEvent_t ev0 = nullptr;
-    if (ctx->prof) prof_begin(ctx, &ev0);
+    const bool timed = ctx->prof && prof_selected(ctx, name);
+    if (timed) prof_begin(ctx, &ev0);
     hipLaunchKernelGGL(kernel, grid, block, shmem, ctx->stream,
                        std::forward<Args>(args)...);
     VD_HIP(hipGetLastError());
-    if (ctx->prof) prof_end(ctx, name, bytes, ev0);
+    if (timed) prof_end(ctx, name, bytes, ev0);
 }
 
 void sync(vd_ctx *ctx);
