@@ -302,24 +302,37 @@ __global__ __launch_bounds__(JX_BLOCK) void k_join_expand(
             rrow[j] = s_low[k] + (uint32_t)(o - s_off[k]);
         }
     }
-    // Key columns from the left rows.
-    int oc_i = 0;
-    for (int c = 0; c < oc.nl; ++c, ++oc_i) {
-        const uint32_t *src = oc.l[c];
-        uint32_t *dst = oc.o[oc_i];
+    // Output columns [L columns, R columns after the keys], gathered CB
+    // columns at a time: the loads of a batch are all issued before its
+    // stores (the output never aliases the inputs).
+    constexpr int CB = 4;
+    const int ncl = oc.nl, nco = oc.nl + oc.nr - NK;
+    for (int c0 = 0; c0 < nco; c0 += CB) {
+        uint32_t v[CB][ITEMS];
 #pragma unroll
-        for (int j = 0; j < ITEMS; ++j) {
-            const int q = j * JX_BLOCK + threadIdx.x;
-            if (q < nout) dst[b0 + q] = src[lrow[j]];
+        for (int cb = 0; cb < CB; ++cb) {
+            const int c = c0 + cb;
+            if (c < nco) {
+                const bool from_l = c < ncl;
+                const uint32_t *src = from_l ? oc.l[c] : oc.r[c - ncl + NK];
+#pragma unroll
+                for (int j = 0; j < ITEMS; ++j) {
+                    const int q = j * JX_BLOCK + threadIdx.x;
+                    v[cb][j] = q < nout ? src[from_l ? lrow[j] : rrow[j]] : 0u;
+                }
+            }
         }
-    }
-    for (int c = NK; c < oc.nr; ++c, ++oc_i) {
-        const uint32_t *src = oc.r[c];
-        uint32_t *dst = oc.o[oc_i];
 #pragma unroll
-        for (int j = 0; j < ITEMS; ++j) {
-            const int q = j * JX_BLOCK + threadIdx.x;
-            if (q < nout) dst[b0 + q] = src[rrow[j]];
+        for (int cb = 0; cb < CB; ++cb) {
+            const int c = c0 + cb;
+            if (c < nco) {
+                uint32_t *dst = oc.o[c];
+#pragma unroll
+                for (int j = 0; j < ITEMS; ++j) {
+                    const int q = j * JX_BLOCK + threadIdx.x;
+                    if (q < nout) dst[b0 + q] = v[cb][j];
+                }
+            }
         }
     }
 }
