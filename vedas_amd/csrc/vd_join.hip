@@ -118,7 +118,29 @@ __global__ __launch_bounds__(JB_BLOCK) void k_join_bounds(
     const uint64_t m = rhi - rlo;
     const uint64_t k = m <= (uint64_t)rcap ? 1 : (m + rcap - 1) / rcap;  // directory stride
     const uint64_t nd = (m + k - 1) / k;                                 // directory entries
-    for (uint64_t t = threadIdx.x; t < nd; t += JB_BLOCK) s_r[t] = KeyT<NK>::at(R, rlo + t * k);
+    // Stage the right keys in batches of SB per thread: the loads of a
+    // batch are all in flight before the first LDS store.
+    constexpr int SB = 8;
+    for (uint64_t t0 = 0; t0 < nd; t0 += (uint64_t)SB * JB_BLOCK) {
+        T v[SB];
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+            const uint64_t t = t0 + (uint64_t)u * JB_BLOCK + threadIdx.x;
+            if (t < nd) v[u] = KeyT<NK>::at(R, rlo + t * k);
+        }
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+            const uint64_t t = t0 + (uint64_t)u * JB_BLOCK + threadIdx.x;
+            if (t < nd) s_r[t] = v[u];
+        }
+    }
+    // The tile's left keys, all loads in flight together.
+    T lkey[JB_ITEMS];
+#pragma unroll
+    for (int j = 0; j < JB_ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)j * JB_BLOCK + threadIdx.x;
+        lkey[j] = i < nl ? KeyT<NK>::at(L, i) : T(0);
+    }
     __syncthreads();
     uint32_t cnt[JB_ITEMS];
 #pragma unroll
@@ -126,7 +148,7 @@ __global__ __launch_bounds__(JB_BLOCK) void k_join_bounds(
         const uint64_t i = base + (uint64_t)j * JB_BLOCK + threadIdx.x;
         uint32_t c = 0;
         if (i < nl) {
-            const T key = KeyT<NK>::at(L, i);
+            const T key = lkey[j];
             uint64_t lo, hi;
             if (k == 1) {
                 lo = dev::lower_bound(s_r, 0, m, key);
@@ -277,9 +299,28 @@ __global__ __launch_bounds__(JX_BLOCK) void k_join_expand(
     if (b1 <= b0) return;  // no outputs on this diagonal
     const uint64_t sa = a0 > 0 ? a0 - 1 : 0;
     const int m = (int)(a1 - sa);
-    for (int t = threadIdx.x; t < m; t += JX_BLOCK) {
-        s_off[t] = off(sa + t);
-        s_low[t] = lower[sa + t];
+    // Stage offsets and lower bounds of the diagonal's left rows, the loads
+    // of a batch in flight together.
+    constexpr int SB = 4;
+    for (int t0 = 0; t0 < m; t0 += SB * JX_BLOCK) {
+        uint64_t vo[SB];
+        uint32_t vl[SB];
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+            const int t = t0 + u * JX_BLOCK + threadIdx.x;
+            if (t < m) {
+                vo[u] = off(sa + t);
+                vl[u] = lower[sa + t];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+            const int t = t0 + u * JX_BLOCK + threadIdx.x;
+            if (t < m) {
+                s_off[t] = vo[u];
+                s_low[t] = vl[u];
+            }
+        }
     }
     __syncthreads();
 
