@@ -105,13 +105,33 @@ __global__ __launch_bounds__(MB_BLOCK) void k_join_bounds_multi(
     const uint64_t rlo = ranges[2 * blockIdx.x], rhi = ranges[2 * blockIdx.x + 1], m = rhi - rlo;
     const uint64_t k = m <= (uint64_t)MJ_RCAP ? 1 : (m + MJ_RCAP - 1) / MJ_RCAP;
     const uint64_t nd = (m + k - 1) / k;
-    for (uint64_t t = threadIdx.x; t < nd; t += MB_BLOCK) s_r[t] = rk[rlo + t * k];
+    // Staging and the left keys in batches of loads issued before their use.
+    constexpr int SB = 8;
+    for (uint64_t t0 = 0; t0 < nd; t0 += (uint64_t)SB * MB_BLOCK) {
+        uint32_t v[SB];
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+            const uint64_t t = t0 + (uint64_t)u * MB_BLOCK + threadIdx.x;
+            if (t < nd) v[u] = rk[rlo + t * k];
+        }
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+            const uint64_t t = t0 + (uint64_t)u * MB_BLOCK + threadIdx.x;
+            if (t < nd) s_r[t] = v[u];
+        }
+    }
+    uint32_t lkey[MB_ITEMS];
+#pragma unroll
+    for (int it = 0; it < MB_ITEMS; ++it) {
+        const uint64_t i = base + (uint64_t)it * MB_BLOCK + threadIdx.x;
+        lkey[it] = i < nl ? lk[i] : 0u;
+    }
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < MB_ITEMS; ++it) {
         const uint64_t i = base + (uint64_t)it * MB_BLOCK + threadIdx.x;
         if (i >= nl) break;
-        const uint32_t key = lk[i];
+        const uint32_t key = lkey[it];
         uint64_t lo, hi;
         if (k == 1) {
             lo = dev::lower_bound(s_r, 0, m, key);
@@ -167,10 +187,27 @@ __global__ __launch_bounds__(MJ_BLOCK) void k_join_expand_multi(
     if (b1 <= b0) return;
     const uint64_t sa = a0 > 0 ? a0 - 1 : 0;
     const int m = (int)(a1 - sa);
-    for (int t = threadIdx.x; t < m; t += MJ_BLOCK) {
-        s_off[t] = off[sa + t];
-        s_low[t] = lower[sa + t];
-        s_pair[t] = (uint16_t)find_seg(lbase, np, sa + t);
+    constexpr int SB = 4;
+    for (int t0 = 0; t0 < m; t0 += SB * MJ_BLOCK) {
+        uint64_t vo[SB];
+        uint32_t vl[SB];
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+            const int t = t0 + u * MJ_BLOCK + threadIdx.x;
+            if (t < m) {
+                vo[u] = off[sa + t];
+                vl[u] = lower[sa + t];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+            const int t = t0 + u * MJ_BLOCK + threadIdx.x;
+            if (t < m) {
+                s_off[t] = vo[u];
+                s_low[t] = vl[u];
+                s_pair[t] = (uint16_t)find_seg(lbase, np, sa + t);
+            }
+        }
     }
     __syncthreads();
     const int nout = (int)(b1 - b0);
@@ -196,23 +233,36 @@ __global__ __launch_bounds__(MJ_BLOCK) void k_join_expand_multi(
             }
         }
         const uint64_t ob = b0 - D.obase;
-        int oc = 0;
-        for (int c = 0; c < D.ncl; ++c, ++oc) {
-            const uint32_t *src = D.lc[c];
-            uint32_t *dst = D.oc[oc] + ob;
+        // Output columns [L columns, R columns after the key], CB at a time,
+        // the loads of a batch issued before its stores.
+        constexpr int CB = 4;
+        const int ncl = D.ncl, nco = D.ncl + D.ncr - 1;
+        for (int c0 = 0; c0 < nco; c0 += CB) {
+            uint32_t v[CB][ITEMS];
 #pragma unroll
-            for (int it = 0; it < ITEMS; ++it) {
-                const int q = it * MJ_BLOCK + threadIdx.x;
-                if (q < nout) dst[q] = src[lrow[it]];
+            for (int cb = 0; cb < CB; ++cb) {
+                const int c = c0 + cb;
+                if (c < nco) {
+                    const bool from_l = c < ncl;
+                    const uint32_t *src = from_l ? D.lc[c] : D.rc[c - ncl + 1];
+#pragma unroll
+                    for (int it = 0; it < ITEMS; ++it) {
+                        const int q = it * MJ_BLOCK + threadIdx.x;
+                        v[cb][it] = q < nout ? src[from_l ? lrow[it] : rrow[it]] : 0u;
+                    }
+                }
             }
-        }
-        for (int c = 1; c < D.ncr; ++c, ++oc) {
-            const uint32_t *src = D.rc[c];
-            uint32_t *dst = D.oc[oc] + ob;
 #pragma unroll
-            for (int it = 0; it < ITEMS; ++it) {
-                const int q = it * MJ_BLOCK + threadIdx.x;
-                if (q < nout) dst[q] = src[rrow[it]];
+            for (int cb = 0; cb < CB; ++cb) {
+                const int c = c0 + cb;
+                if (c < nco) {
+                    uint32_t *dst = D.oc[c] + ob;
+#pragma unroll
+                    for (int it = 0; it < ITEMS; ++it) {
+                        const int q = it * MJ_BLOCK + threadIdx.x;
+                        if (q < nout) dst[q] = v[cb][it];
+                    }
+                }
             }
         }
         return;
@@ -318,13 +368,6 @@ __global__ __launch_bounds__(256) void k_gather_sort_multi(const uint64_t *keys,
 }
 
 // ---- unique -------------------------------------------------------------------
-__device__ __forceinline__ bool differs(const SortRelDev &D, uint64_t i) {
-    if (i == 0) return true;
-    for (int c = 0; c < D.ncols; ++c)
-        if (D.in[c][i] != D.in[c][i - 1]) return true;
-    return false;
-}
-
 __global__ __launch_bounds__(MB_BLOCK) void k_unique_count_multi(const SortRelDev *S, int ns,
                                                                  const uint64_t *tbase,
                                                                  uint32_t *cnt) {
@@ -335,11 +378,8 @@ __global__ __launch_bounds__(MB_BLOCK) void k_unique_count_multi(const SortRelDe
     const int j = uniform(s_j);
     const SortRelDev &D = S[j];
     const uint64_t base = (blockIdx.x - tbase[j]) * (uint64_t)MB_TILE;
-    uint32_t c = 0;
-    for (int it = 0; it < MB_ITEMS; ++it) {
-        const uint64_t i = base + (uint64_t)it * MB_BLOCK + threadIdx.x;
-        if (i < D.n) c += differs(D, i);
-    }
+    const uint32_t c =
+        (uint32_t)__popc(dev::unique_keep_bits<MB_BLOCK, MB_ITEMS>(D.in, D.ncols, D.n, base));
     const uint32_t t = dev::block_reduce<MB_BLOCK>(c, s_tmp);
     if (threadIdx.x == 0) cnt[blockIdx.x] = t;
 }
@@ -355,24 +395,15 @@ __global__ __launch_bounds__(MB_BLOCK) void k_unique_scatter_multi(const SortRel
                                                                    const uint64_t *tbase,
                                                                    const uint64_t *off) {
     __shared__ int s_j;
-    __shared__ uint32_t s_tmp[MB_BLOCK / 64];
+    __shared__ uint32_t s_cnt[MB_ITEMS][MB_BLOCK / 64];
     if (threadIdx.x == 0) s_j = find_seg(tbase, ns, blockIdx.x);
     __syncthreads();
     const int j = uniform(s_j);
     const SortRelDev &D = S[j];
     const uint64_t base = (blockIdx.x - tbase[j]) * (uint64_t)MB_TILE;
-    // rounds of MB_BLOCK consecutive rows, compacted with a block scan each
-    uint64_t o = off[blockIdx.x] - off[tbase[j]];
-    for (int it = 0; it < MB_ITEMS; ++it) {
-        const uint64_t i = base + (uint64_t)it * MB_BLOCK + threadIdx.x;
-        const uint32_t keep = (i < D.n && differs(D, i)) ? 1u : 0u;
-        uint32_t tot;
-        const uint32_t ex = dev::block_excl_scan<MB_BLOCK>(keep, s_tmp, &tot);
-        if (keep)
-            for (int cc = 0; cc < D.ncols; ++cc) D.out[cc][o + ex] = D.in[cc][i];
-        o += tot;
-        if (base + (uint64_t)(it + 1) * MB_BLOCK >= D.n) break;  // block-uniform
-    }
+    const uint32_t keep = dev::unique_keep_bits<MB_BLOCK, MB_ITEMS>(D.in, D.ncols, D.n, base);
+    dev::unique_scatter_tile<MB_BLOCK, MB_ITEMS>(keep, D.in, D.out, D.ncols, base,
+                                                 off[blockIdx.x] - off[tbase[j]], s_cnt);
 }
 
 // ---- host drivers ---------------------------------------------------------------

@@ -384,5 +384,80 @@ __device__ __forceinline__ uint64_t wave_bound(const T *a, uint64_t lo, uint64_t
     return lo + (uint64_t)__popcll(__ballot(b));
 }
 
+// Adjacent-unique over a tile of ITEMS rounds of BLOCK consecutive rows
+// (row = base + round * BLOCK + thread).  Keep flags: bit j = the row of
+// round j exists (< n) and differs from the row before it in some column.
+// Column by column, all rounds at once, so a thread's loads are
+// independent; the previous row's value comes from the lane before (a wave
+// holds consecutive rows), and lane 0 loads it.
+template <int BLOCK, int ITEMS>
+__device__ __forceinline__ uint32_t unique_keep_bits(const uint32_t *const *in, int ncols,
+                                                     uint64_t n, uint64_t base) {
+    static_assert(ITEMS <= 32, "keep flags in one word");
+    const int lane = lane_id();
+    uint32_t diff = 0, valid = 0;
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
+        valid |= (i < n ? 1u : 0u) << j;
+        diff |= (i == 0 ? 1u : 0u) << j;
+    }
+    for (int c = 0; c < ncols; ++c) {
+        const uint32_t *col = in[c];
+        uint32_t v[ITEMS], pv[ITEMS];
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
+            v[j] = i < n ? col[i] : 0u;
+            pv[j] = (lane == 0 && i > 0 && i <= n) ? col[i - 1] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            const uint32_t up = __shfl_up(v[j], 1, WAVE);
+            const uint32_t prev = lane == 0 ? pv[j] : up;
+            diff |= (v[j] != prev ? 1u : 0u) << j;
+        }
+    }
+    return diff & valid;
+}
+
+// Writes the kept rows of the tile (flags from unique_keep_bits) to out at
+// o + (rows kept before them in row order).  One ballot per round gives
+// each wave its mask; s_cnt (ITEMS x BLOCK/64 words of LDS) carries the
+// per-(round, wave) counts across one block barrier.  All threads call it.
+template <int BLOCK, int ITEMS>
+__device__ __forceinline__ void unique_scatter_tile(uint32_t keep, const uint32_t *const *in,
+                                                    uint32_t *const *out, int ncols,
+                                                    uint64_t base, uint64_t o,
+                                                    uint32_t (*s_cnt)[BLOCK / WAVE]) {
+    constexpr int WAVES = BLOCK / WAVE;
+    const int lane = lane_id();
+    const int w = threadIdx.x / WAVE;
+    uint64_t mask[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        mask[j] = __ballot((keep >> j) & 1u);
+        if (lane == 0) s_cnt[j][w] = (uint32_t)__popcll(mask[j]);
+    }
+    __syncthreads();
+    const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        uint32_t before = 0, round_total = 0;
+#pragma unroll
+        for (int ww = 0; ww < WAVES; ++ww) {
+            const uint32_t c = s_cnt[j][ww];
+            before += ww < w ? c : 0u;
+            round_total += c;
+        }
+        if ((keep >> j) & 1u) {
+            const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
+            const uint64_t pos = o + before + (uint64_t)__popcll(mask[j] & lt);
+            for (int cc = 0; cc < ncols; ++cc) out[cc][pos] = in[cc][i];
+        }
+        o += round_total;
+    }
+}
+
 }  // namespace dev
 }  // namespace vd

@@ -111,85 +111,23 @@ __global__ __launch_bounds__(256) void k_gather_perm(
     }
 }
 
-// Keep flags of a tile's rows (row = base + round * U_BLOCK + thread):
-// bit j = row of round j exists and differs from the row before it in some
-// column.  Column by column, all rounds at once, so the loads of a thread
-// are independent; the previous row's value comes from the lane before
-// (a wave holds consecutive rows), and lane 0 loads it.
-__device__ __forceinline__ uint32_t tile_keep_bits(const Cols &cols, uint64_t n, uint64_t base) {
-    const int lane = threadIdx.x & 63;
-    uint32_t diff = 0, valid = 0;
-#pragma unroll
-    for (int j = 0; j < U_ITEMS; ++j) {
-        const uint64_t i = base + (uint64_t)j * U_BLOCK + threadIdx.x;
-        valid |= (i < n ? 1u : 0u) << j;
-        diff |= (i == 0 ? 1u : 0u) << j;
-    }
-    for (int c = 0; c < cols.n; ++c) {
-        const uint32_t *col = cols.in[c];
-        uint32_t v[U_ITEMS], pv[U_ITEMS];
-#pragma unroll
-        for (int j = 0; j < U_ITEMS; ++j) {
-            const uint64_t i = base + (uint64_t)j * U_BLOCK + threadIdx.x;
-            v[j] = i < n ? col[i] : 0u;
-            pv[j] = (lane == 0 && i > 0 && i <= n) ? col[i - 1] : 0u;
-        }
-#pragma unroll
-        for (int j = 0; j < U_ITEMS; ++j) {
-            const uint32_t up = __shfl_up(v[j], 1, 64);
-            const uint32_t prev = lane == 0 ? pv[j] : up;
-            diff |= (v[j] != prev ? 1u : 0u) << j;
-        }
-    }
-    return diff & valid;
-}
-
 __global__ __launch_bounds__(U_BLOCK) void k_unique_count(Cols cols, uint64_t n,
                                                           uint32_t *cnt) {
     __shared__ uint32_t s_tmp[U_BLOCK / 64];
     const uint64_t base = (uint64_t)blockIdx.x * U_TILE;
-    const uint32_t c = (uint32_t)__popc(tile_keep_bits(cols, n, base));
+    const uint32_t c =
+        (uint32_t)__popc(dev::unique_keep_bits<U_BLOCK, U_ITEMS>(cols.in, cols.n, n, base));
     const uint32_t t = dev::block_reduce<U_BLOCK>(c, s_tmp);
     if (threadIdx.x == 0) cnt[blockIdx.x] = t;
 }
 
-// One ballot per round gives each wave its kept-row mask; the per-(round,
-// wave) counts go through LDS once, and each kept row then lands at the
-// tile's offset + the rows kept before it in row order, so loads and stores
-// stay coalesced with a single block barrier per tile.
 __global__ __launch_bounds__(U_BLOCK) void k_unique_scatter(
     Cols cols, uint64_t n, const uint64_t *__restrict__ off) {
-    constexpr int WAVES = U_BLOCK / 64;
-    __shared__ uint32_t s_cnt[U_ITEMS][WAVES];
-    const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
+    __shared__ uint32_t s_cnt[U_ITEMS][U_BLOCK / 64];
     const uint64_t base = (uint64_t)blockIdx.x * U_TILE;
-    const uint32_t keep = tile_keep_bits(cols, n, base);
-    uint64_t mask[U_ITEMS];
-#pragma unroll
-    for (int j = 0; j < U_ITEMS; ++j) {
-        mask[j] = __ballot((keep >> j) & 1u);
-        if (lane == 0) s_cnt[j][w] = (uint32_t)__popcll(mask[j]);
-    }
-    __syncthreads();
-    const uint64_t lt = (1ull << lane) - 1ull;
-    uint64_t o = off[blockIdx.x];
-#pragma unroll
-    for (int j = 0; j < U_ITEMS; ++j) {
-        uint32_t before = 0, round_total = 0;
-#pragma unroll
-        for (int ww = 0; ww < WAVES; ++ww) {
-            const uint32_t c = s_cnt[j][ww];
-            before += ww < w ? c : 0u;
-            round_total += c;
-        }
-        if ((keep >> j) & 1u) {
-            const uint64_t i = base + (uint64_t)j * U_BLOCK + threadIdx.x;
-            const uint64_t pos = o + before + (uint64_t)__popcll(mask[j] & lt);
-            for (int cc = 0; cc < cols.n; ++cc) cols.out[cc][pos] = cols.in[cc][i];
-        }
-        o += round_total;
-    }
+    const uint32_t keep = dev::unique_keep_bits<U_BLOCK, U_ITEMS>(cols.in, cols.n, n, base);
+    dev::unique_scatter_tile<U_BLOCK, U_ITEMS>(keep, cols.in, cols.out, cols.n, base,
+                                               off[blockIdx.x], s_cnt);
 }
 
 unsigned sgrid(uint64_t n) {
