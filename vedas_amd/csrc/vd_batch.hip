@@ -401,9 +401,13 @@ __global__ __launch_bounds__(MB_BLOCK) void k_unique_scatter_multi(const SortRel
     const int j = uniform(s_j);
     const SortRelDev &D = S[j];
     const uint64_t base = (blockIdx.x - tbase[j]) * (uint64_t)MB_TILE;
-    const uint32_t keep = dev::unique_keep_bits<MB_BLOCK, MB_ITEMS>(D.in, D.ncols, D.n, base);
-    dev::unique_scatter_tile<MB_BLOCK, MB_ITEMS>(keep, D.in, D.out, D.ncols, base,
-                                                 off[blockIdx.x] - off[tbase[j]], s_cnt);
+    constexpr int KC = 6;  // columns kept in registers between the two phases
+    uint32_t vals[KC][MB_ITEMS];
+    const uint32_t keep =
+        dev::unique_keep_bits<MB_BLOCK, MB_ITEMS, KC>(D.in, D.ncols, D.n, base, vals);
+    dev::unique_scatter_tile<MB_BLOCK, MB_ITEMS, KC>(keep, D.in, D.out, D.ncols, base,
+                                                     off[blockIdx.x] - off[tbase[j]], s_cnt,
+                                                     vals);
 }
 
 // ---- host drivers ---------------------------------------------------------------

@@ -390,9 +390,12 @@ __device__ __forceinline__ uint64_t wave_bound(const T *a, uint64_t lo, uint64_t
 // Column by column, all rounds at once, so a thread's loads are
 // independent; the previous row's value comes from the lane before (a wave
 // holds consecutive rows), and lane 0 loads it.
-template <int BLOCK, int ITEMS>
+// The values of columns < KC are left in kept[][] for the scatter, so it
+// does not read them again (a tile's rows outlive the L2 between the two).
+template <int BLOCK, int ITEMS, int KC = 0>
 __device__ __forceinline__ uint32_t unique_keep_bits(const uint32_t *const *in, int ncols,
-                                                     uint64_t n, uint64_t base) {
+                                                     uint64_t n, uint64_t base,
+                                                     uint32_t (*kept)[ITEMS] = nullptr) {
     static_assert(ITEMS <= 32, "keep flags in one word");
     const int lane = lane_id();
     uint32_t diff = 0, valid = 0;
@@ -417,6 +420,12 @@ __device__ __forceinline__ uint32_t unique_keep_bits(const uint32_t *const *in, 
             const uint32_t prev = lane == 0 ? pv[j] : up;
             diff |= (v[j] != prev ? 1u : 0u) << j;
         }
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc)
+            if (kc == c) {
+#pragma unroll
+                for (int j = 0; j < ITEMS; ++j) kept[kc][j] = v[j];
+            }
     }
     return diff & valid;
 }
@@ -425,11 +434,14 @@ __device__ __forceinline__ uint32_t unique_keep_bits(const uint32_t *const *in, 
 // o + (rows kept before them in row order).  One ballot per round gives
 // each wave its mask; s_cnt (ITEMS x BLOCK/64 words of LDS) carries the
 // per-(round, wave) counts across one block barrier.  All threads call it.
-template <int BLOCK, int ITEMS>
+// Columns < KC come from vals (filled by unique_keep_bits), the others
+// are read again.
+template <int BLOCK, int ITEMS, int KC>
 __device__ __forceinline__ void unique_scatter_tile(uint32_t keep, const uint32_t *const *in,
                                                     uint32_t *const *out, int ncols,
                                                     uint64_t base, uint64_t o,
-                                                    uint32_t (*s_cnt)[BLOCK / WAVE]) {
+                                                    uint32_t (*s_cnt)[BLOCK / WAVE],
+                                                    const uint32_t (*vals)[ITEMS]) {
     constexpr int WAVES = BLOCK / WAVE;
     const int lane = lane_id();
     const int w = threadIdx.x / WAVE;
@@ -453,7 +465,10 @@ __device__ __forceinline__ void unique_scatter_tile(uint32_t keep, const uint32_
         if ((keep >> j) & 1u) {
             const uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
             const uint64_t pos = o + before + (uint64_t)__popcll(mask[j] & lt);
-            for (int cc = 0; cc < ncols; ++cc) out[cc][pos] = in[cc][i];
+#pragma unroll
+            for (int cc = 0; cc < KC; ++cc)
+                if (cc < ncols) out[cc][pos] = vals[cc][j];
+            for (int cc = KC; cc < ncols; ++cc) out[cc][pos] = in[cc][i];
         }
         o += round_total;
     }

@@ -17,6 +17,7 @@ constexpr int R_ITEMS = 16;
 constexpr int U_BLOCK = 256;
 constexpr int U_ITEMS = 8;
 constexpr int U_TILE = U_BLOCK * U_ITEMS;
+constexpr int U_KEEP_COLS = 6;  // columns the unique scatter keeps in registers
 
 struct Cols {
     const uint32_t *in[VD_MAX_COLS];
@@ -125,9 +126,11 @@ __global__ __launch_bounds__(U_BLOCK) void k_unique_scatter(
     Cols cols, uint64_t n, const uint64_t *__restrict__ off) {
     __shared__ uint32_t s_cnt[U_ITEMS][U_BLOCK / 64];
     const uint64_t base = (uint64_t)blockIdx.x * U_TILE;
-    const uint32_t keep = dev::unique_keep_bits<U_BLOCK, U_ITEMS>(cols.in, cols.n, n, base);
-    dev::unique_scatter_tile<U_BLOCK, U_ITEMS>(keep, cols.in, cols.out, cols.n, base,
-                                               off[blockIdx.x], s_cnt);
+    uint32_t vals[U_KEEP_COLS][U_ITEMS];
+    const uint32_t keep =
+        dev::unique_keep_bits<U_BLOCK, U_ITEMS, U_KEEP_COLS>(cols.in, cols.n, n, base, vals);
+    dev::unique_scatter_tile<U_BLOCK, U_ITEMS, U_KEEP_COLS>(keep, cols.in, cols.out, cols.n,
+                                                            base, off[blockIdx.x], s_cnt, vals);
 }
 
 unsigned sgrid(uint64_t n) {
