@@ -80,20 +80,54 @@ __device__ __forceinline__ uint64_t g_upper(const KeyCols &k, uint64_t lo,
     return lo;
 }
 
-// Right-row range [lo, hi) each bounds block can touch: one thread per
-// block, all searches in parallel (a serial search inside every block
+// First index q in [lo, hi) with pred(q), for pred false then true over
+// the range, searched by a group of G lanes (G divides 64; a group's lanes
+// hold the same lo/hi and run together): each round samples G evenly
+// spaced indices, so the search takes log_G of the range in dependent
+// loads instead of log_2.
+template <int G, typename Pred>
+__device__ __forceinline__ uint64_t group_bound(uint64_t lo, uint64_t hi, Pred pred) {
+    static_assert(64 % G == 0, "groups tile the wave");
+    const int lane = dev::lane_id();
+    const int g = lane % G;
+    const int shift = lane - g;
+    const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+    while (hi - lo > (uint64_t)G) {
+        const uint64_t step = (hi - lo + G - 1) / G;
+        const uint64_t q = lo + (uint64_t)(g + 1) * step - 1;  // last of segment g
+        const bool before = q < hi && !pred(q);
+        const int c = __popcll((__ballot(before) >> shift) & gmask);  // segments before the bound
+        const uint64_t qc = lo + (uint64_t)(c + 1) * step - 1;
+        hi = qc < hi ? qc + 1 : hi;
+        lo = min(lo + (uint64_t)c * step, hi);
+    }
+    const uint64_t q = lo + (uint64_t)g;
+    const bool before = q < hi && !pred(q);
+    return lo + (uint64_t)__popcll((__ballot(before) >> shift) & gmask);
+}
+
+// Right-row range [lo, hi) each bounds block can touch: a group of 16 lanes
+// per block, all searches in parallel (a serial search inside every block
 // would put ~2 log2(|R|) dependent global loads in front of its work).
+constexpr int JR_GROUP = 16;
 template <int NK>
 __global__ __launch_bounds__(256) void k_join_ranges(KeyCols L, uint64_t nl, KeyCols R,
                                                      uint64_t nr, uint64_t nblocks,
                                                      uint64_t *__restrict__ ranges) {
-    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nblocks) return;
+    using T = typename KeyT<NK>::T;
+    const uint64_t b = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / JR_GROUP;
+    if (b >= nblocks) return;  // whole groups leave together
     const uint64_t base = b * JB_TILE;
     const uint64_t last = min(base + JB_TILE, nl) - 1;
-    const uint64_t lo = g_lower<NK>(R, 0, nr, KeyT<NK>::at(L, base));
-    ranges[2 * b] = lo;
-    ranges[2 * b + 1] = g_upper<NK>(R, lo, nr, KeyT<NK>::at(L, last));
+    const T kfirst = KeyT<NK>::at(L, base), klast = KeyT<NK>::at(L, last);
+    const uint64_t lo =
+        group_bound<JR_GROUP>(0, nr, [&](uint64_t q) { return !(KeyT<NK>::at(R, q) < kfirst); });
+    const uint64_t hi =
+        group_bound<JR_GROUP>(lo, nr, [&](uint64_t q) { return klast < KeyT<NK>::at(R, q); });
+    if (threadIdx.x % JR_GROUP == 0) {
+        ranges[2 * b] = lo;
+        ranges[2 * b + 1] = hi;
+    }
 }
 
 // The right range of a tile goes to LDS whole when it has at most rcap
@@ -224,19 +258,16 @@ struct JoinOff {
 };
 
 // mp[b] = number of left rows i with i + off[i] < b * nv (clamped to the
-// end of the diagonal), for b = 0 .. nblocks.
+// end of the diagonal), for b = 0 .. nblocks: a group of lanes per b.
 __global__ __launch_bounds__(256) void k_join_partition(
     JoinOff off, uint64_t nl, uint64_t total, int nv,
     uint64_t nblocks, uint64_t *__restrict__ mp) {
-    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b > nblocks) return;
+    const uint64_t b = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / JR_GROUP;
+    if (b > nblocks) return;  // whole groups leave together
     const uint64_t diag = min(b * (uint64_t)nv, nl + total);
-    uint64_t lo = 0, hi = nl;
-    while (lo < hi) {
-        const uint64_t mid = lo + ((hi - lo) >> 1);
-        if (mid + off(mid) < diag) lo = mid + 1; else hi = mid;
-    }
-    mp[b] = lo;
+    const uint64_t lo =
+        group_bound<JR_GROUP>(0, nl, [&](uint64_t i) { return !(i + off(i) < diag); });
+    if (threadIdx.x % JR_GROUP == 0) mp[b] = lo;
 }
 
 // out = {total, first output key, last output key}: the total is the
@@ -397,8 +428,8 @@ vd_rel *join_impl(vd_ctx *ctx, const vd_rel *L, const vd_rel *R) {
     // tile + the total).
     DBuf<uint64_t> ranges(ctx, 4 * jb_tiles + 1);
     uint64_t *ttot = ranges.p + 2 * jb_tiles, *tpre = ranges.p + 3 * jb_tiles;
-    launch(ctx, "join_ranges", 0.0, dim3((unsigned)ceil_div(jb_tiles, 256)), dim3(256), 0,
-           k_join_ranges<NK>, lk, nl, rk, nr, jb_tiles, ranges.p);
+    launch(ctx, "join_ranges", 0.0, dim3((unsigned)ceil_div(jb_tiles * JR_GROUP, 256)),
+           dim3(256), 0, k_join_ranges<NK>, lk, nl, rk, nr, jb_tiles, ranges.p);
     launch(ctx, "join_bounds", 4.0 * NK * (double)(nl + nr) + 12.0 * nl,
            dim3((unsigned)jb_tiles), dim3(JB_BLOCK), 0, k_join_bounds<NK>, lk,
            nl, rk, nr, rcap, (const uint64_t *)ranges.p, lower.p, off.p, ttot);
@@ -429,7 +460,8 @@ vd_rel *join_impl(vd_ctx *ctx, const vd_rel *L, const vd_rel *R) {
         const uint64_t nblocks = ceil_div(nl + total, (uint64_t)nv);
         VD_REQUIRE(nblocks < (1ull << 31), VD_ERANGE, "join expansion grid too large");
         DBuf<uint64_t> mp(ctx, nblocks + 1);
-        launch(ctx, "join_partition", 0.0, dim3((unsigned)ceil_div(nblocks + 1, 256)),
+        launch(ctx, "join_partition", 0.0,
+               dim3((unsigned)ceil_div((nblocks + 1) * JR_GROUP, 256)),
                dim3(256), 0, k_join_partition, joff, nl, total,
                nv, nblocks, mp.p);
         const double xbytes = 4.0 * (double)total * out_cols + 12.0 * nl +

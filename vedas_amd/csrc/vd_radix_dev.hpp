@@ -16,14 +16,17 @@ constexpr int HIST_BLOCK = 256;
 constexpr int HIST_WAVES = HIST_BLOCK / 64;
 
 // LDS histogram: one copy per wave (fewer same-address atomics across
-// waves), MAX_PASSES x 256 bins each.
-struct LdsHist {
-    uint32_t h[HIST_WAVES][MAX_PASSES][RBINS];
+// waves), NP passes x 256 bins each.
+template <int NP>
+struct LdsHistN {
+    uint32_t h[HIST_WAVES][NP][RBINS];
 };
+using LdsHist = LdsHistN<MAX_PASSES>;
 
-__device__ __forceinline__ void hist_zero(LdsHist *s) {
+template <int NP>
+__device__ __forceinline__ void hist_zero(LdsHistN<NP> *s) {
     uint32_t *p = &s->h[0][0][0];
-    for (int i = threadIdx.x; i < HIST_WAVES * MAX_PASSES * RBINS; i += HIST_BLOCK)
+    for (int i = threadIdx.x; i < HIST_WAVES * NP * RBINS; i += HIST_BLOCK)
         p[i] = 0;
 }
 
@@ -32,13 +35,14 @@ __device__ __forceinline__ void hist_zero(LdsHist *s) {
 // of a wave's lanes.  When every calling lane of the wave holds the same
 // digit (the top digits of small IDs), one lane adds the lane count: LDS
 // atomics on one address from many lanes are serialised.
-__device__ __forceinline__ void hist_add(LdsHist *s, uint64_t key, int lo_bit,
+template <int NP>
+__device__ __forceinline__ void hist_add(LdsHistN<NP> *s, uint64_t key, int lo_bit,
                                          int hi_bit, int npass) {
     const int w = threadIdx.x / 64;
     const uint64_t act = __ballot(1);
     const int leader = __ffsll((unsigned long long)act) - 1;
     const bool is_leader = (int)(threadIdx.x & 63) == leader;
-    for (int p = 0; p < npass; ++p) {
+    for (int p = 0; p < npass && p < NP; ++p) {
         const int sh = lo_bit + p * RBITS;
         const int width = min(RBITS, hi_bit - sh);
         const uint32_t d = (uint32_t)(key >> sh) & ((1u << width) - 1u);
@@ -53,9 +57,10 @@ __device__ __forceinline__ void hist_add(LdsHist *s, uint64_t key, int lo_bit,
 
 // Fold the block's LDS histogram into the global one (npass x 256 words).
 // Must be preceded by a __syncthreads().
-__device__ __forceinline__ void hist_flush(LdsHist *s, uint32_t *ghist,
+template <int NP>
+__device__ __forceinline__ void hist_flush(LdsHistN<NP> *s, uint32_t *ghist,
                                            int npass) {
-    for (int i = threadIdx.x; i < npass * RBINS; i += HIST_BLOCK) {
+    for (int i = threadIdx.x; i < min(npass, NP) * RBINS; i += HIST_BLOCK) {
         const int p = i / RBINS, d = i % RBINS;
         uint32_t c = 0;
 #pragma unroll

@@ -54,7 +54,7 @@ __global__ __launch_bounds__(RB) void k_pack_c01(
     const uint32_t *__restrict__ c0, const uint32_t *__restrict__ c1,
     const uint32_t *__restrict__ vsrc, uint64_t n, uint64_t *__restrict__ keys,
     uint32_t *__restrict__ vals, int hi_bit, int npass, uint32_t *ghist) {
-    __shared__ radix::LdsHist s;
+    __shared__ radix::LdsHistN<4> s;  // the four bytes of column 0
     radix::hist_zero(&s);
     __syncthreads();
     const uint64_t base = (uint64_t)blockIdx.x * (RB * R_ITEMS);
