@@ -554,10 +554,22 @@ void join_multi(vd_ctx *ctx, const vd_rel *const *L, const vd_rel *const *R, int
 
 void sort_col0_range(vd_ctx *ctx, vd_rel *const *rels, int n);
 
+// Relations of at least this many rows are sorted one by one (sort_col0):
+// its launch and read-back costs are small against their rows, and it moves
+// the columns with the keys instead of gathering them by row id afterwards.
+constexpr uint64_t MULTI_SORT_SINGLE_ROWS = 1ull << 20;
+
 void sort_col0_multi(vd_ctx *ctx, vd_rel *const *rels, int n) {
     std::vector<vd_rel *> work;
-    for (int j = 0; j < n; ++j)
-        if (rels[j] && rels[j]->nrows > 1) work.push_back(rels[j]);
+    for (int j = 0; j < n; ++j) {
+        if (!rels[j] || rels[j]->nrows <= 1) continue;
+        if (rels[j]->nrows >= MULTI_SORT_SINGLE_ROWS) {
+            const int rc = vd_sort_col0(ctx, rels[j]);
+            if (rc != VD_OK) throw Error(rc, ctx->last_error);
+        } else {
+            work.push_back(rels[j]);
+        }
+    }
     if (!work.empty()) sort_col0_range(ctx, work.data(), (int)work.size());
 }
 

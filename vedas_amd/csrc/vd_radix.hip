@@ -224,13 +224,17 @@ __attribute__((amdgpu_waves_per_eu(OsShape<BLOCK, ITEMS, M>::WAVES_PER_EU, 8))) 
     }
     __syncthreads();
 
-    uint32_t pos[ITEMS];
+    // Staged positions (< TILE <= 65535), two per register: they stay live
+    // until the payload columns are staged.
+    uint32_t pos2[(ITEMS + 1) / 2];
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
         const uint32_t d = (uint32_t)(k[j] >> shift) & dmask;
-        pos[j] = s_lstart[d] + s_wcnt[w][d] + rk[j];
-        s_keys[pos[j]] = k[j];
+        const uint32_t p = s_lstart[d] + s_wcnt[w][d] + rk[j];
+        if (j % 2 == 0) pos2[j / 2] = p; else pos2[j / 2] |= p << 16;
+        s_keys[p] = k[j];
     }
+    auto pos = [&](int j) { return (pos2[j / 2] >> (16 * (j % 2))) & 0xFFFFu; };
     __syncthreads();
 
     // Staged position i (i = tid + BLOCK * r) goes to dst[r].  Positions at
@@ -249,7 +253,7 @@ __attribute__((amdgpu_waves_per_eu(OsShape<BLOCK, ITEMS, M>::WAVES_PER_EU, 8))) 
     for (int c = 0; c < M; ++c) {
         __syncthreads();  // the previous reads of the staging buffer are done
 #pragma unroll
-        for (int j = 0; j < ITEMS; ++j) s_v[pos[j]] = v[c][j];
+        for (int j = 0; j < ITEMS; ++j) s_v[pos(j)] = v[c][j];
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < ITEMS; ++r) {
