@@ -72,20 +72,26 @@ struct JPairDev {
     int ncl, ncr;
 };
 
-// Right-row range of every bounds tile, all tiles in parallel.
+// Right-row range of every bounds tile, all tiles in parallel: a group of
+// MR_GROUP lanes per tile searches with MR_GROUP samples per round.
+constexpr int MR_GROUP = 16;
 __global__ __launch_bounds__(256) void k_join_ranges_multi(const JPairDev *P, int np,
                                                            const uint64_t *tbase, uint64_t ntiles,
                                                            uint64_t *ranges) {
-    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= ntiles) return;
+    const uint64_t b = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / MR_GROUP;
+    if (b >= ntiles) return;  // whole groups leave together
     const int j = find_seg(tbase, np, b);
     const JPairDev &D = P[j];
     const uint32_t *lk = D.lc[0], *rk = D.rc[0];
     const uint64_t base = (b - tbase[j]) * (uint64_t)MB_TILE;
     const uint64_t last = min(base + MB_TILE, D.nl) - 1;
-    const uint64_t lo = dev::lower_bound(rk, 0, D.nr, lk[base]);
-    ranges[2 * b] = lo;
-    ranges[2 * b + 1] = dev::upper_bound(rk, lo, D.nr, lk[last]);
+    const uint32_t kfirst = lk[base], klast = lk[last];
+    const uint64_t lo = dev::group_bound<MR_GROUP>(0, D.nr, [&](uint64_t q) { return rk[q] >= kfirst; });
+    const uint64_t hi = dev::group_bound<MR_GROUP>(lo, D.nr, [&](uint64_t q) { return rk[q] > klast; });
+    if (threadIdx.x % MR_GROUP == 0) {
+        ranges[2 * b] = lo;
+        ranges[2 * b + 1] = hi;
+    }
 }
 
 // As k_join_bounds (vd_join.hip): the right range, or a directory of every
@@ -150,13 +156,15 @@ __global__ __launch_bounds__(MB_BLOCK) void k_join_bounds_multi(
 }
 
 // Output size of every pair from the global offsets.
-__global__ void k_pair_totals(const JPairDev *P, int np, const uint64_t *off, uint64_t *tot) {
+__global__ void k_pair_totals(const JPairDev *P, int np, TileOff off, uint64_t nl,
+                              uint64_t ntiles, uint64_t *tot) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= np) return;
-    tot[j] = off[P[j].lbase + P[j].nl] - off[P[j].lbase];
+    auto at = [&](uint64_t i) { return i < nl ? off(i) : off.tpre[ntiles]; };
+    tot[j] = at(P[j].lbase + P[j].nl) - at(P[j].lbase);
 }
 
-__global__ __launch_bounds__(256) void k_join_partition_multi(const uint64_t *off, uint64_t nl,
+__global__ __launch_bounds__(256) void k_join_partition_multi(TileOff off, uint64_t nl,
                                                               uint64_t total, int nv,
                                                               uint64_t nblocks, uint64_t *mp) {
     const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -165,7 +173,7 @@ __global__ __launch_bounds__(256) void k_join_partition_multi(const uint64_t *of
     uint64_t lo = 0, hi = nl;
     while (lo < hi) {
         const uint64_t mid = lo + ((hi - lo) >> 1);
-        if (mid + off[mid] < diag) lo = mid + 1; else hi = mid;
+        if (mid + off(mid) < diag) lo = mid + 1; else hi = mid;
     }
     mp[b] = lo;
 }
@@ -173,7 +181,7 @@ __global__ __launch_bounds__(256) void k_join_partition_multi(const uint64_t *of
 // NV: merge-path diagonal per block (the join_nv tunable).
 template <int MJ_NV>
 __global__ __launch_bounds__(MJ_BLOCK) void k_join_expand_multi(
-    const uint64_t *off, const uint32_t *lower, const uint64_t *mp, uint64_t nl, uint64_t total,
+    TileOff off, const uint32_t *lower, const uint64_t *mp, uint64_t nl, uint64_t total,
     const JPairDev *P, int np, const uint64_t *lbase) {
     constexpr int ITEMS = MJ_NV / MJ_BLOCK;
     __shared__ uint64_t s_off[MJ_NV + 1];
@@ -195,7 +203,7 @@ __global__ __launch_bounds__(MJ_BLOCK) void k_join_expand_multi(
         for (int u = 0; u < SB; ++u) {
             const int t = t0 + u * MJ_BLOCK + threadIdx.x;
             if (t < m) {
-                vo[u] = off[sa + t];
+                vo[u] = off(sa + t);
                 vl[u] = lower[sa + t];
             }
         }
@@ -496,22 +504,24 @@ void join_multi(vd_ctx *ctx, const vd_rel *const *L, const vd_rel *const *R, int
     DBuf<JPairDev> dp = to_device(ctx, hp);
     DBuf<uint64_t> dlbase = to_device(ctx, lbase);
     DBuf<uint32_t> lower(ctx, nl), cnt(ctx, nl);
-    DBuf<uint64_t> off(ctx, nl + 1), dtot(ctx, np);
+    const uint64_t otiles = ceil_div(nl, OFF_TILE);
+    DBuf<uint64_t> loc(ctx, nl), tpre(ctx, otiles + 1), dtot(ctx, np);
+    const TileOff off{loc.p, tpre.p};
     if (nl > 0) {
         std::vector<uint64_t> tb = tile_prefix(lrows);
         DBuf<uint64_t> dtb = to_device(ctx, tb);
         double bbytes = 0;
         for (int j = 0; j < np; ++j) bbytes += 4.0 * (hp[j].nl + hp[j].nr) + 8.0 * hp[j].nl;
         DBuf<uint64_t> ranges(ctx, 2 * tb[np]);
-        launch(ctx, "join_ranges", 0.0, dim3((unsigned)ceil_div(tb[np], 256)), dim3(256), 0,
-               k_join_ranges_multi, (const JPairDev *)dp.p, np, (const uint64_t *)dtb.p, tb[np],
+        launch(ctx, "join_ranges", 0.0, dim3((unsigned)ceil_div(tb[np] * MR_GROUP, 256)),
+               dim3(256), 0, k_join_ranges_multi, (const JPairDev *)dp.p, np, (const uint64_t *)dtb.p, tb[np],
                ranges.p);
         launch(ctx, "join_bounds", bbytes, dim3((unsigned)tb[np]), dim3(MB_BLOCK), 0,
                k_join_bounds_multi, (const JPairDev *)dp.p, np, (const uint64_t *)dtb.p,
                (const uint64_t *)ranges.p, lower.p, cnt.p);
-        scan_u32_to_u64(ctx, cnt.p, off.p, nl, off.p + nl);
+        tile_offsets_u32(ctx, cnt.p, nl, loc.p, tpre.p);
         launch(ctx, "join_totals", 0.0, dim3((unsigned)ceil_div(np, 256)), dim3(256), 0,
-               k_pair_totals, (const JPairDev *)dp.p, np, (const uint64_t *)off.p, dtot.p);
+               k_pair_totals, (const JPairDev *)dp.p, np, off, nl, otiles, dtot.p);
         download(ctx, tot.data(), dtot.p, np * sizeof(uint64_t));
     }
     uint64_t total = 0;
@@ -530,7 +540,7 @@ void join_multi(vd_ctx *ctx, const vd_rel *const *L, const vd_rel *const *R, int
             VD_REQUIRE(nblocks < (1ull << 31), VD_ERANGE, "join expansion grid too large");
             DBuf<uint64_t> mp(ctx, nblocks + 1);
             launch(ctx, "join_partition", 0.0, dim3((unsigned)ceil_div(nblocks + 1, 256)),
-                   dim3(256), 0, k_join_partition_multi, (const uint64_t *)off.p, nl, total,
+                   dim3(256), 0, k_join_partition_multi, off, nl, total,
                    nv, nblocks, mp.p);
             double xbytes = 0;
             for (int j = 0; j < np; ++j)
@@ -538,7 +548,7 @@ void join_multi(vd_ctx *ctx, const vd_rel *const *L, const vd_rel *const *R, int
                           4.0 * hp[j].nr * (hp[j].ncr - 1);
             auto go = [&](auto kern) {
                 launch(ctx, "join_expand", xbytes, dim3((unsigned)nblocks), dim3(MJ_BLOCK), 0,
-                       kern, (const uint64_t *)off.p, (const uint32_t *)lower.p,
+                       kern, off, (const uint32_t *)lower.p,
                        (const uint64_t *)mp.p, nl, total, (const JPairDev *)dp.p, np,
                        (const uint64_t *)dlbase.p);
             };

@@ -193,6 +193,22 @@ void download(vd_ctx *ctx, void *dst, const void *src, size_t bytes);
 void read_words(vd_ctx *ctx, uint64_t *dst, const uint64_t *dptr, int n);
 
 // ---- primitives ------------------------------------------------------------
+// Two-level offsets of per-row counts, in tiles of OFF_TILE rows: loc[i] =
+// exclusive scan of the counts within i's tile, tpre[t] = the counts before
+// tile t (tpre[ntiles] = the total); the offset of row i < n is
+// tpre[i / OFF_TILE] + loc[i] (TileOff).  No pass adds the two.
+constexpr int OFF_TILE = 2048;
+struct TileOff {
+    const uint64_t *loc, *tpre;
+    __device__ __forceinline__ uint64_t operator()(uint64_t i) const {
+        return tpre[i / OFF_TILE] + loc[i];
+    }
+};
+// loc and tpre (ceil(n / OFF_TILE) + 1 entries) of n counts.
+void tile_offsets_u32(vd_ctx *ctx, const uint32_t *in, uint64_t n, uint64_t *loc,
+                      uint64_t *tpre);
+// tpre from the tile totals (one block).
+void scan_tile_totals(vd_ctx *ctx, const uint64_t *tot, uint64_t ntiles, uint64_t *tpre);
 // Exclusive scan of n uint32 counts into uint64 offsets; *d_total (device)
 // receives the sum.  Arbitrary n.
 void scan_u32_to_u64(vd_ctx *ctx, const uint32_t *in, uint64_t *out,
@@ -382,6 +398,32 @@ __device__ __forceinline__ uint64_t wave_bound(const T *a, uint64_t lo, uint64_t
     const uint64_t q = lo + (uint64_t)lane;
     const bool b = q < hi && before(q);
     return lo + (uint64_t)__popcll(__ballot(b));
+}
+
+// First index q in [lo, hi) with pred(q), for pred false then true over
+// the range, searched by a group of G lanes (G divides 64; a group's lanes
+// hold the same lo/hi and run together): each round samples G evenly
+// spaced indices, so the search takes log_G of the range in dependent
+// loads instead of log_2.
+template <int G, typename Pred>
+__device__ __forceinline__ uint64_t group_bound(uint64_t lo, uint64_t hi, Pred pred) {
+    static_assert(64 % G == 0, "groups tile the wave");
+    const int lane = lane_id();
+    const int g = lane % G;
+    const int shift = lane - g;
+    const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+    while (hi - lo > (uint64_t)G) {
+        const uint64_t step = (hi - lo + G - 1) / G;
+        const uint64_t q = lo + (uint64_t)(g + 1) * step - 1;  // last of segment g
+        const bool before = q < hi && !pred(q);
+        const int c = __popcll((__ballot(before) >> shift) & gmask);  // segments before the bound
+        const uint64_t qc = lo + (uint64_t)(c + 1) * step - 1;
+        hi = qc < hi ? qc + 1 : hi;
+        lo = min(lo + (uint64_t)c * step, hi);
+    }
+    const uint64_t q = lo + (uint64_t)g;
+    const bool before = q < hi && !pred(q);
+    return lo + (uint64_t)__popcll((__ballot(before) >> shift) & gmask);
 }
 
 // Adjacent-unique over a tile of ITEMS rounds of BLOCK consecutive rows

@@ -8,7 +8,7 @@
 //                     when it is longer than the LDS capacity.  The same
 //                     kernel scans the counts of its tile: loc_i = exclusive
 //                     scan of cnt_i within the tile, plus the tile total.
-//  2. k_join_tile_scan : one block scans the tile totals into tpre (tpre[t] =
+//  2. scan_tile_totals : one block scans the tile totals into tpre (tpre[t] =
 //                     rows before tile t; tpre[ntiles] = output size, read
 //                     back once to allocate).  The output offset of left row
 //                     i is off_i = tpre[i / 2048] + loc_i (join_off below).
@@ -80,32 +80,6 @@ __device__ __forceinline__ uint64_t g_upper(const KeyCols &k, uint64_t lo,
     return lo;
 }
 
-// First index q in [lo, hi) with pred(q), for pred false then true over
-// the range, searched by a group of G lanes (G divides 64; a group's lanes
-// hold the same lo/hi and run together): each round samples G evenly
-// spaced indices, so the search takes log_G of the range in dependent
-// loads instead of log_2.
-template <int G, typename Pred>
-__device__ __forceinline__ uint64_t group_bound(uint64_t lo, uint64_t hi, Pred pred) {
-    static_assert(64 % G == 0, "groups tile the wave");
-    const int lane = dev::lane_id();
-    const int g = lane % G;
-    const int shift = lane - g;
-    const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
-    while (hi - lo > (uint64_t)G) {
-        const uint64_t step = (hi - lo + G - 1) / G;
-        const uint64_t q = lo + (uint64_t)(g + 1) * step - 1;  // last of segment g
-        const bool before = q < hi && !pred(q);
-        const int c = __popcll((__ballot(before) >> shift) & gmask);  // segments before the bound
-        const uint64_t qc = lo + (uint64_t)(c + 1) * step - 1;
-        hi = qc < hi ? qc + 1 : hi;
-        lo = min(lo + (uint64_t)c * step, hi);
-    }
-    const uint64_t q = lo + (uint64_t)g;
-    const bool before = q < hi && !pred(q);
-    return lo + (uint64_t)__popcll((__ballot(before) >> shift) & gmask);
-}
-
 // Right-row range [lo, hi) each bounds block can touch: a group of 16 lanes
 // per block, all searches in parallel (a serial search inside every block
 // would put ~2 log2(|R|) dependent global loads in front of its work).
@@ -121,9 +95,9 @@ __global__ __launch_bounds__(256) void k_join_ranges(KeyCols L, uint64_t nl, Key
     const uint64_t last = min(base + JB_TILE, nl) - 1;
     const T kfirst = KeyT<NK>::at(L, base), klast = KeyT<NK>::at(L, last);
     const uint64_t lo =
-        group_bound<JR_GROUP>(0, nr, [&](uint64_t q) { return !(KeyT<NK>::at(R, q) < kfirst); });
+        dev::group_bound<JR_GROUP>(0, nr, [&](uint64_t q) { return !(KeyT<NK>::at(R, q) < kfirst); });
     const uint64_t hi =
-        group_bound<JR_GROUP>(lo, nr, [&](uint64_t q) { return klast < KeyT<NK>::at(R, q); });
+        dev::group_bound<JR_GROUP>(lo, nr, [&](uint64_t q) { return klast < KeyT<NK>::at(R, q); });
     if (threadIdx.x % JR_GROUP == 0) {
         ranges[2 * b] = lo;
         ranges[2 * b + 1] = hi;
@@ -231,31 +205,8 @@ __global__ __launch_bounds__(JB_BLOCK) void k_join_bounds(
     }
 }
 
-// tpre[t] = sum of tile_total[0..t), t = 0..ntiles (one block, chunks of
-// 1024 tiles with a running carry).
-__global__ __launch_bounds__(1024) void k_join_tile_scan(const uint64_t *__restrict__ tile_total,
-                                                         uint64_t ntiles,
-                                                         uint64_t *__restrict__ tpre) {
-    __shared__ uint64_t s_tmp[1024 / 64];
-    uint64_t carry = 0;
-    for (uint64_t b = 0; b < ntiles; b += 1024) {
-        const uint64_t t = b + threadIdx.x;
-        const uint64_t v = t < ntiles ? tile_total[t] : 0;
-        uint64_t tot;
-        const uint64_t ex = dev::block_excl_scan<1024>(v, s_tmp, &tot);
-        if (t < ntiles) tpre[t] = carry + ex;
-        carry += tot;
-    }
-    if (threadIdx.x == 0) tpre[ntiles] = carry;
-}
-
-// Output offset of left row i (i < nl): tile prefix + offset within tile.
-struct JoinOff {
-    const uint64_t *loc, *tpre;
-    __device__ __forceinline__ uint64_t operator()(uint64_t i) const {
-        return tpre[i / JB_TILE] + loc[i];
-    }
-};
+static_assert(JB_TILE == OFF_TILE, "bounds tiles are the offset tiles");
+using JoinOff = TileOff;
 
 // mp[b] = number of left rows i with i + off[i] < b * nv (clamped to the
 // end of the diagonal), for b = 0 .. nblocks: a group of lanes per b.
@@ -266,7 +217,7 @@ __global__ __launch_bounds__(256) void k_join_partition(
     if (b > nblocks) return;  // whole groups leave together
     const uint64_t diag = min(b * (uint64_t)nv, nl + total);
     const uint64_t lo =
-        group_bound<JR_GROUP>(0, nl, [&](uint64_t i) { return !(i + off(i) < diag); });
+        dev::group_bound<JR_GROUP>(0, nl, [&](uint64_t i) { return !(i + off(i) < diag); });
     if (threadIdx.x % JR_GROUP == 0) mp[b] = lo;
 }
 
@@ -433,8 +384,7 @@ vd_rel *join_impl(vd_ctx *ctx, const vd_rel *L, const vd_rel *R) {
     launch(ctx, "join_bounds", 4.0 * NK * (double)(nl + nr) + 12.0 * nl,
            dim3((unsigned)jb_tiles), dim3(JB_BLOCK), 0, k_join_bounds<NK>, lk,
            nl, rk, nr, rcap, (const uint64_t *)ranges.p, lower.p, off.p, ttot);
-    launch(ctx, "join_tile_scan", 8.0 * (double)jb_tiles, dim3(1), dim3(1024), 0,
-           k_join_tile_scan, (const uint64_t *)ttot, jb_tiles, tpre);
+    scan_tile_totals(ctx, ttot, jb_tiles, tpre);
     const JoinOff joff{off.p, tpre};
     DBuf<uint64_t> tk(ctx, 3);
     launch(ctx, "join_total", 0.0, dim3(1), dim3(64), 0, k_join_total_keys, joff, nl, jb_tiles,

@@ -62,7 +62,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_tile_reduce(const InT *in,
 template <typename InT>
 __global__ __launch_bounds__(SCAN_BLOCK) void k_tile_scan(
     const InT *in, uint64_t *out, uint64_t n, const uint64_t *prefix,
-    uint64_t *d_total) {
+    uint64_t *d_total, uint64_t *tile_tot) {
     __shared__ uint64_t s_val[SCAN_TILE];
     __shared__ uint64_t s_tmp[SCAN_BLOCK / 64];
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
@@ -97,6 +97,25 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_tile_scan(
     }
     if (d_total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
         *d_total = pre + tot;
+    if (tile_tot && threadIdx.x == 0) tile_tot[blockIdx.x] = tot;
+}
+
+// tpre[t] = sum of tot[0..t), t = 0..ntiles (one block, chunks of 1024 with
+// a running carry).
+__global__ __launch_bounds__(1024) void k_scan_tile_totals(const uint64_t *__restrict__ tot,
+                                                           uint64_t ntiles,
+                                                           uint64_t *__restrict__ tpre) {
+    __shared__ uint64_t s_tmp[1024 / 64];
+    uint64_t carry = 0;
+    for (uint64_t b = 0; b < ntiles; b += 1024) {
+        const uint64_t t = b + threadIdx.x;
+        const uint64_t v = t < ntiles ? tot[t] : 0;
+        uint64_t sum;
+        const uint64_t ex = dev::block_excl_scan<1024>(v, s_tmp, &sum);
+        if (t < ntiles) tpre[t] = carry + ex;
+        carry += sum;
+    }
+    if (threadIdx.x == 0) tpre[ntiles] = carry;
 }
 
 template <typename InT>
@@ -111,7 +130,8 @@ void scan_impl(vd_ctx *ctx, const InT *in, uint64_t *out, uint64_t n,
     const double bytes = (double)n * (sizeof(InT) * 2 + sizeof(uint64_t));
     if (tiles == 1) {
         launch(ctx, "scan_tile", bytes, dim3(1), dim3(SCAN_BLOCK), 0,
-               k_tile_scan<InT>, in, out, n, (const uint64_t *)nullptr, d_total);
+               k_tile_scan<InT>, in, out, n, (const uint64_t *)nullptr, d_total,
+               (uint64_t *)nullptr);
         return;
     }
     DBuf<uint64_t> partial(ctx, tiles);
@@ -120,7 +140,7 @@ void scan_impl(vd_ctx *ctx, const InT *in, uint64_t *out, uint64_t n,
     scan_impl<uint64_t>(ctx, partial.p, partial.p, tiles, d_total);
     launch(ctx, "scan_tile", bytes, dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0,
            k_tile_scan<InT>, in, out, n, (const uint64_t *)partial.p,
-           (uint64_t *)nullptr);
+           (uint64_t *)nullptr, (uint64_t *)nullptr);
 }
 
 }  // namespace
@@ -133,6 +153,28 @@ void scan_u32_to_u64(vd_ctx *ctx, const uint32_t *in, uint64_t *out,
 void scan_u64(vd_ctx *ctx, const uint64_t *in, uint64_t *out, uint64_t n,
               uint64_t *d_total) {
     scan_impl<uint64_t>(ctx, in, out, n, d_total);
+}
+
+static_assert(SCAN_TILE == OFF_TILE, "tile offsets use the scan tile");
+
+void scan_tile_totals(vd_ctx *ctx, const uint64_t *tot, uint64_t ntiles, uint64_t *tpre) {
+    launch(ctx, "scan_tiles", 8.0 * (double)ntiles, dim3(1), dim3(1024), 0, k_scan_tile_totals,
+           tot, ntiles, tpre);
+}
+
+void tile_offsets_u32(vd_ctx *ctx, const uint32_t *in, uint64_t n, uint64_t *loc,
+                      uint64_t *tpre) {
+    const uint64_t tiles = ceil_div(n, OFF_TILE);
+    if (tiles == 0) {
+        fill_u64(ctx, tpre, 0, 1);
+        return;
+    }
+    VD_REQUIRE(tiles < (1ull << 31), VD_ERANGE, "scan too large");
+    DBuf<uint64_t> tot(ctx, tiles);
+    launch(ctx, "scan_tile", (double)n * 12.0, dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0,
+           k_tile_scan<uint32_t>, in, loc, n, (const uint64_t *)nullptr, (uint64_t *)nullptr,
+           tot.p);
+    scan_tile_totals(ctx, tot.p, tiles, tpre);
 }
 
 static unsigned grid_for(uint64_t n) {

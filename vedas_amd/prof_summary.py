@@ -31,7 +31,7 @@ LABELS = [
     (r"k_join_bounds_multi\b", "join_bounds"),
     (r"k_join_bounds<", "join_bounds"),
     (r"k_join_partition", "join_partition"),
-    (r"k_join_tile_scan\b", "join_tile_scan"),
+    (r"k_scan_tile_totals\b", "scan_tiles"),
     (r"k_join_total_keys\b", "join_total"),
     (r"k_join_ranges", "join_ranges"),
     (r"k_pair_totals\b", "join_totals"),
