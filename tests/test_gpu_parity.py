@@ -400,8 +400,10 @@ def test_sort_col0_every_tile_shape(ctx, items, ncols):
     from vedas_amd._native import Rel
     ctx.tune("radix_items", items)
     try:
+        # keys below 2^27 with the top byte varying take the 9-bit digit
+        # plan (3 passes); the others the 8-bit one
         for n, keymax in ((4097, 2**31), (6145, 300), (8193, 2**20), (24_577, 7),
-                          (100_003, 2**32 - 1)):
+                          (100_003, 2**32 - 1), (8193, 2**27), (70_001, 2**25 + 3)):
             print(f"sort_col0 items={items} ncols={ncols} n={n}", flush=True)
             rng = np.random.default_rng(n * 31 + ncols)
             cols = [rng.integers(0, keymax, n, dtype=np.uint64).astype(np.uint32)]
@@ -414,6 +416,26 @@ def test_sort_col0_every_tile_shape(ctx, items, ncols):
             r.free()
     finally:
         ctx.tune("radix_items", 16)
+
+
+@pytest.mark.parametrize("ncols", [2, 3, 4, 6])
+def test_sort_col0_digit_plans(ctx, ncols):
+    """Column-0 values that only vary in some 9-bit or 8-bit digits: the
+    passes skip the constant digits, in both digit plans."""
+    from vedas_amd._native import Rel
+    rng = np.random.default_rng(ncols)
+    n = 50_001
+    for col0 in (rng.integers(0, 64, n).astype(np.uint32) << 18,        # bits 18..23
+                 (rng.integers(0, 8, n).astype(np.uint32) << 24) | 5,  # bits 24..26
+                 rng.integers(1 << 24, 1 << 27, n).astype(np.uint32),
+                 np.full(n, 123456, np.uint32)):
+        cols = [col0] + [rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+                         for _ in range(ncols - 1)]
+        r = Rel.from_columns(ctx, cols)
+        r.sort_col0()
+        for g, w in zip(r.columns(), O.sort_col0(cols)):
+            assert np.array_equal(g, w)
+        r.free()
 
 
 @pytest.mark.parametrize("items", [8, 24])

@@ -251,19 +251,20 @@ uint64_t *radix_sort_u64_payload(vd_ctx *ctx, uint64_t *keys, uint64_t *tmp, uin
                                  int lo_bit, int hi_bit, uint32_t *d_hist, bool hist_ready,
                                  int m, const RadixCols &src, const RadixCols &pa,
                                  const RadixCols &pb, RadixCols *final);
-// Stable sort of n keys on the 8-bit digits at the given shifts (LSD
-// order; digits not listed must be equal in every key).  d_gofs[p] points
-// to the 256 exclusive digit offsets of pass p (device).  Payload columns
-// as for radix_sort_u64_payload.  Returns the buffer holding the sorted
-// keys (keys or tmp).
+// Stable sort of n keys on the digits of digit_bits (8 or 9) bits at the
+// given shifts (LSD order; digits not listed must be equal in every key).
+// d_gofs[p] points to the 2^digit_bits exclusive digit offsets of pass p
+// (device).  Payload columns as for radix_sort_u64_payload.  Returns the
+// buffer holding the sorted keys (keys or tmp).
 uint64_t *radix_sort_shifts(vd_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n,
                             const int *shifts, int npass, const uint32_t *const *d_gofs, int m,
                             const RadixCols &src, const RadixCols &pa, const RadixCols &pb,
-                            RadixCols *final);
-// Exclusive scans of npos 256-bin digit histograms (d_hist -> d_gofs) and
-// d_varies[i] = 0 when one bin of position i holds all n keys, else 1.
+                            RadixCols *final, int digit_bits = 8);
+// Exclusive scans of npos digit histograms of 2^digit_bits bins (d_hist ->
+// d_gofs) and d_varies[i] = 0 when one bin of position i holds all n keys,
+// else 1.
 void radix_scan_hist_flags(vd_ctx *ctx, const uint32_t *d_hist, int npos, uint64_t n,
-                           uint32_t *d_gofs, uint64_t *d_varies);
+                           uint32_t *d_gofs, uint64_t *d_varies, int digit_bits = 8);
 int radix_passes(int lo_bit, int hi_bit);
 // Device histogram buffer size (uint32 words) needed for radix_sort_u64.
 inline size_t radix_hist_words(int lo_bit, int hi_bit) {

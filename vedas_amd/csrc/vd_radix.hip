@@ -68,13 +68,14 @@ __global__ __launch_bounds__(RBINS) void k_rs_scan_hist(const uint32_t *ghist,
 // Exclusive scan of each digit position's histogram (one block per
 // position) and a flag per position: 1 when the digit varies (no bin holds
 // all n rows), 0 when every row has the same digit there.
-__global__ __launch_bounds__(RBINS) void k_rs_scan_hist_flags(const uint32_t *ghist, uint64_t n,
-                                                              uint32_t *gofs, uint64_t *varies) {
-    __shared__ uint32_t s_tmp[RBINS / 64];
-    const uint32_t v = ghist[blockIdx.x * RBINS + threadIdx.x];
+template <int BINS>
+__global__ __launch_bounds__(BINS) void k_rs_scan_hist_flags(const uint32_t *ghist, uint64_t n,
+                                                             uint32_t *gofs, uint64_t *varies) {
+    __shared__ uint32_t s_tmp[BINS / 64];
+    const uint32_t v = ghist[blockIdx.x * BINS + threadIdx.x];
     uint32_t tot;
-    const uint32_t ex = dev::block_excl_scan<RBINS>(v, s_tmp, &tot);
-    gofs[blockIdx.x * RBINS + threadIdx.x] = ex;
+    const uint32_t ex = dev::block_excl_scan<BINS>(v, s_tmp, &tot);
+    gofs[blockIdx.x * BINS + threadIdx.x] = ex;
     const int all_one_bin = __syncthreads_or((uint64_t)v == n);
     if (threadIdx.x == 0) varies[blockIdx.x] = all_one_bin ? 0 : 1;
 }
@@ -93,13 +94,17 @@ struct OsShape {
     static constexpr int WAVES_PER_EU = (BLOCK == 512 && ITEMS * (2 + M) <= 48) ? 4 : 1;
 };
 
+// Digit width of a pass: 8 bits (256 bins), or 9 bits (512 bins) where one
+// 9-bit pass replaces a nearly empty 8-bit one (keys of 25-27 bits in 3
+// passes instead of 4).  Threads 0..BINS-1 own one digit each.
+
 // M 32-bit payload columns travel with the keys (pay.in[c] -> pay.out[c]);
 // they are loaded with the keys and staged, column after column, through the
 // same LDS buffer as the keys once those are written out, so every global
 // access stays coalesced and the block keeps two-per-CU occupancy.
 // BLOCK threads (a multiple of 256: threads 0..255 own one digit each in the
 // count and look-back phases), ITEMS keys per thread.
-template <int BLOCK, int ITEMS, int M>
+template <int BLOCK, int ITEMS, int M, int DB = RBITS>
 __global__ __launch_bounds__(BLOCK)
 __attribute__((amdgpu_waves_per_eu(OsShape<BLOCK, ITEMS, M>::WAVES_PER_EU, 8))) void k_rs_onesweep(
     const uint64_t *__restrict__ in, uint64_t *__restrict__ out, RadixPayload pay,
@@ -107,6 +112,7 @@ __attribute__((amdgpu_waves_per_eu(OsShape<BLOCK, ITEMS, M>::WAVES_PER_EU, 8))) 
     uint32_t *lookback, uint32_t *tile_counter) {
     constexpr int WAVES = BLOCK / 64;
     constexpr int TILE = BLOCK * ITEMS;
+    constexpr int RBINS = 1 << DB;
     static_assert(BLOCK % RBINS == 0, "one digit per thread in the count phase");
     static_assert(M >= 0 && M <= RADIX_MAX_PAYLOAD, "payload columns");
     static_assert(TILE <= 65535, "16-bit wave counters");
@@ -154,7 +160,7 @@ __attribute__((amdgpu_waves_per_eu(OsShape<BLOCK, ITEMS, M>::WAVES_PER_EU, 8))) 
         const uint32_t d = (uint32_t)(k[j] >> shift) & dmask;
         uint64_t peers = ~0ull;
 #pragma unroll
-        for (int b = 0; b < RBITS; ++b) {
+        for (int b = 0; b < DB; ++b) {
             const int bit = (d >> b) & 1;
             const uint64_t bal = __ballot(bit);
             peers &= bit ? bal : ~bal;
@@ -268,14 +274,15 @@ __attribute__((amdgpu_waves_per_eu(OsShape<BLOCK, ITEMS, M>::WAVES_PER_EU, 8))) 
 // pa (odd passes) or pb (even passes); *final receives the last buffers.
 // Pass p sorts on the digit at shifts[p] (widths[p] bits) with the 256
 // exclusive digit offsets at gofs[p].
-template <int BLOCK, int ITEMS, int M>
+template <int BLOCK, int ITEMS, int M, int DB>
 void run_passes(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, const RadixCols &src,
                 const RadixCols &pa, const RadixCols &pb, RadixCols *final,
                 uint64_t n, const int *shifts, const int *widths, int npass,
                 const uint32_t *const *gofs) {
     constexpr uint64_t TILE = (uint64_t)BLOCK * ITEMS;
+    constexpr int BINS = 1 << DB;
     const uint64_t ntiles = ceil_div(n, TILE);
-    const uint64_t lb_words = (uint64_t)npass * ntiles * RBINS;
+    const uint64_t lb_words = (uint64_t)npass * ntiles * BINS;
     DBuf<uint32_t> lb(ctx, lb_words + npass);
     fill_u32(ctx, lb.p, 0, lb_words + npass);
     uint32_t *counters = lb.p + lb_words;
@@ -289,9 +296,9 @@ void run_passes(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, const RadixCols &src,
         }
         launch(ctx, M ? "radix_onesweep_kv" : "radix_onesweep",
                (double)n * (16.0 + 8.0 * M), dim3((unsigned)ntiles),
-               dim3(BLOCK), 0, k_rs_onesweep<BLOCK, ITEMS, M>, (const uint64_t *)a, b, pay,
+               dim3(BLOCK), 0, k_rs_onesweep<BLOCK, ITEMS, M, DB>, (const uint64_t *)a, b, pay,
                n, shifts[p], widths[p], gofs[p],
-               lb.p + (uint64_t)p * ntiles * RBINS, counters + p);
+               lb.p + (uint64_t)p * ntiles * BINS, counters + p);
         std::swap(a, b);
         cur = dstc;
     }
@@ -306,10 +313,23 @@ void run_passes(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, const RadixCols &src,
 void dispatch_passes(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, int m, const RadixCols &src,
                      const RadixCols &pa, const RadixCols &pb, RadixCols *final, uint64_t n,
                      const int *shifts, const int *widths, int npass,
-                     const uint32_t *const *g) {
+                     const uint32_t *const *g, int digit_bits = RBITS) {
     const int items = ctx->radix_items;
+    if (digit_bits == 9) {
+        // 512-thread shapes only (one digit per thread), as the default.
+#define VD_RUN9(IT, MM) \
+    run_passes<512, IT, MM, 9>(ctx, a, b, src, pa, pb, final, n, shifts, widths, npass, g)
+        switch (m) {
+            case 0: VD_RUN9(16, 0); break;
+            case 1: VD_RUN9(16, 1); break;
+            case 2: VD_RUN9(12, 2); break;
+            default: VD_RUN9(8, 3); break;
+        }
+#undef VD_RUN9
+        return;
+    }
 #define VD_RUN(BL, IT, MM) \
-    run_passes<BL, IT, MM>(ctx, a, b, src, pa, pb, final, n, shifts, widths, npass, g)
+    run_passes<BL, IT, MM, RBITS>(ctx, a, b, src, pa, pb, final, n, shifts, widths, npass, g)
     if (m > 0) {
         if (items == 8) {
             switch (m) {
@@ -413,24 +433,31 @@ static uint64_t *sort_impl(vd_ctx *ctx, uint64_t *a, uint64_t *b, int m,
 uint64_t *radix_sort_shifts(vd_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n,
                             const int *shifts, int npass, const uint32_t *const *d_gofs, int m,
                             const RadixCols &src, const RadixCols &pa, const RadixCols &pb,
-                            RadixCols *final) {
+                            RadixCols *final, int digit_bits) {
     VD_REQUIRE(m >= 0 && m <= RADIX_MAX_PAYLOAD, VD_EINVAL, "payload columns");
     VD_REQUIRE(npass >= 0 && npass <= radix::MAX_PASSES, VD_EINVAL, "radix pass count");
+    VD_REQUIRE(digit_bits == 8 || digit_bits == 9, VD_EINVAL, "radix digit width");
     if (final) *final = src;
     if (npass == 0 || n <= 1) return keys;
     VD_REQUIRE(n <= (uint64_t)LB_VAL, VD_ERANGE,
                "radix sort supports at most 2^30-1 keys per call");
     int widths[radix::MAX_PASSES];
-    for (int p = 0; p < npass; ++p) widths[p] = std::min(RBITS, 64 - shifts[p]);
+    for (int p = 0; p < npass; ++p) widths[p] = std::min(digit_bits, 64 - shifts[p]);
     uint64_t *a = keys, *b = tmp;
-    dispatch_passes(ctx, a, b, m, src, pa, pb, final, n, shifts, widths, npass, d_gofs);
+    dispatch_passes(ctx, a, b, m, src, pa, pb, final, n, shifts, widths, npass, d_gofs,
+                    digit_bits);
     return a;
 }
 
 void radix_scan_hist_flags(vd_ctx *ctx, const uint32_t *d_hist, int npos, uint64_t n,
-                           uint32_t *d_gofs, uint64_t *d_varies) {
-    launch(ctx, "radix_scan_hist", 0.0, dim3(npos), dim3(RBINS), 0, k_rs_scan_hist_flags,
-           d_hist, n, d_gofs, d_varies);
+                           uint32_t *d_gofs, uint64_t *d_varies, int digit_bits) {
+    VD_REQUIRE(digit_bits == 8 || digit_bits == 9, VD_EINVAL, "radix digit width");
+    if (digit_bits == 9)
+        launch(ctx, "radix_scan_hist", 0.0, dim3(npos), dim3(512), 0, k_rs_scan_hist_flags<512>,
+               d_hist, n, d_gofs, d_varies);
+    else
+        launch(ctx, "radix_scan_hist", 0.0, dim3(npos), dim3(RBINS), 0,
+               k_rs_scan_hist_flags<RBINS>, d_hist, n, d_gofs, d_varies);
 }
 
 }  // namespace vd

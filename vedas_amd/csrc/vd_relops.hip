@@ -48,28 +48,76 @@ __global__ __launch_bounds__(RB) void k_pack_rowid(
     radix::hist_flush(&s, ghist, npass);
 }
 
+// Counts one digit per calling lane into a wave's LDS bins; when every
+// calling lane of the wave holds the same digit (the top digits of small
+// IDs), one lane adds the lane count (LDS atomics on one address from many
+// lanes are serialised).
+__device__ __forceinline__ void count_digit(uint32_t *bins, uint32_t d) {
+    const uint64_t act = __ballot(1);
+    const int leader = __ffsll((unsigned long long)act) - 1;
+    const uint32_t d0 = __shfl(d, leader, 64);
+    if (__ballot(d == d0) == act) {
+        if ((int)(threadIdx.x & 63) == leader) atomicAdd(&bins[d0], (uint32_t)__popcll(act));
+    } else {
+        atomicAdd(&bins[d], 1u);
+    }
+}
+
+// Histograms of column 0 for the two digit plans of an IndexSwap sort:
+// its four bytes (8-bit digits) and the three 9-bit digits of bits 0..26.
+constexpr int SWAP_H8 = 4 * 256;
+constexpr int SWAP_H9 = 3 * 512;
+struct SwapHist {
+    uint32_t h8[radix::HIST_WAVES][SWAP_H8];  // one copy per wave
+    uint32_t h9[SWAP_H9];                     // one copy per block (LDS)
+};
+
 // key = col0[i] << 32 | col1[i] (no col1: 0); with vals, vals[i] = vsrc[i]
-// (vsrc null: the row id i).
+// (vsrc null: the row id i).  Counts both digit plans' histograms (g8:
+// 4 x 256, g9: 3 x 512 words) and the maximum of column 0 (*gmax).
 __global__ __launch_bounds__(RB) void k_pack_c01(
     const uint32_t *__restrict__ c0, const uint32_t *__restrict__ c1,
     const uint32_t *__restrict__ vsrc, uint64_t n, uint64_t *__restrict__ keys,
-    uint32_t *__restrict__ vals, int hi_bit, int npass, uint32_t *ghist) {
-    __shared__ radix::LdsHistN<4> s;  // the four bytes of column 0
-    radix::hist_zero(&s);
+    uint32_t *__restrict__ vals, uint32_t *g8, uint32_t *g9, uint32_t *gmax) {
+    __shared__ SwapHist s;
+    __shared__ uint32_t s_max;
+    static_assert(RB == radix::HIST_BLOCK, "one histogram copy per wave");
+    uint32_t *z = &s.h8[0][0];
+    for (int i = threadIdx.x; i < (int)(sizeof(SwapHist) / 4); i += RB) z[i] = 0;
+    if (threadIdx.x == 0) s_max = 0;
     __syncthreads();
+    const int w = threadIdx.x / 64;
     const uint64_t base = (uint64_t)blockIdx.x * (RB * R_ITEMS);
+    uint32_t mx = 0;
 #pragma unroll 4
     for (int j = 0; j < R_ITEMS; ++j) {
         const uint64_t i = base + (uint64_t)j * RB + threadIdx.x;
         if (i < n) {
-            const uint64_t key = ((uint64_t)c0[i] << 32) | (c1 ? c1[i] : 0u);
-            keys[i] = key;
+            const uint32_t v = c0[i];
+            keys[i] = ((uint64_t)v << 32) | (c1 ? c1[i] : 0u);
             if (vals) vals[i] = vsrc ? vsrc[i] : (uint32_t)i;
-            radix::hist_add(&s, key, 32, hi_bit, npass);
+            mx = max(mx, v);
+#pragma unroll
+            for (int b = 0; b < 4; ++b) count_digit(&s.h8[w][b * 256], (v >> (8 * b)) & 255u);
+#pragma unroll
+            for (int b = 0; b < 3; ++b) count_digit(&s.h9[b * 512], (v >> (9 * b)) & 511u);
         }
     }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(&s_max, mx);
     __syncthreads();
-    radix::hist_flush(&s, ghist, npass);
+    for (int i = threadIdx.x; i < SWAP_H8; i += RB) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int ww = 0; ww < radix::HIST_WAVES; ++ww) c += s.h8[ww][i];
+        if (c) atomicAdd(&g8[i], c);
+    }
+    for (int i = threadIdx.x; i < SWAP_H9; i += RB) {
+        const uint32_t c = s.h9[i];
+        if (c) atomicAdd(&g9[i], c);
+    }
+    if (threadIdx.x == 0) atomicMax(gmax, s_max);
 }
 
 // out0 = key >> 32, out1 = low word of the key; with rows, out_c =
@@ -174,25 +222,48 @@ void sort_col0(vd_ctx *ctx, vd_rel *rel) {
     const int m = nc <= 2 ? 0 : (by_row ? 1 : nc - 2);
     DBuf<uint64_t> keys(ctx, n), ktmp(ctx, n);
     DBuf<uint32_t> rows(ctx, by_row ? n : 0);
-    DBuf<uint32_t> hist(ctx, 4 * 256), gofs(ctx, 4 * 256);
-    DBuf<uint64_t> varies(ctx, 4);
-    fill_u32(ctx, hist.p, 0, 4 * 256);
+    // Histograms of both digit plans in one buffer [8-bit: 4 x 256 | 9-bit:
+    // 3 x 512], their offsets, and the words read back: "varies" flags of
+    // the four bytes and of the three 9-bit digits, then column 0's maximum
+    // (counted by the pack kernel into the low half of the last word).
+    DBuf<uint32_t> hist(ctx, SWAP_H8 + SWAP_H9), gofs(ctx, SWAP_H8 + SWAP_H9);
+    DBuf<uint64_t> flags(ctx, 8);
+    uint32_t *h8 = hist.p, *h9 = hist.p + SWAP_H8;
+    uint32_t *hmax = reinterpret_cast<uint32_t *>(flags.p + 7);
+    fill_u32(ctx, hist.p, 0, SWAP_H8 + SWAP_H9);
+    fill_u64(ctx, flags.p, 0, 8);
     launch(ctx, "sort_pack", (double)n * (8.0 + 4.0 * std::min(nc, 2) + (by_row ? 4.0 : 0.0)),
            dim3((unsigned)ceil_div(n, RB * R_ITEMS)), dim3(RB), 0, k_pack_c01,
            (const uint32_t *)rel->cols[0], (const uint32_t *)(nc >= 2 ? rel->cols[1] : nullptr),
-           (const uint32_t *)nullptr, n, keys.p, by_row ? rows.p : nullptr, 64, 4, hist.p);
-    radix_scan_hist_flags(ctx, hist.p, 4, n, gofs.p, varies.p);
-    uint64_t hv[4];
-    read_words(ctx, hv, varies.p, 4);
+           (const uint32_t *)nullptr, n, keys.p, by_row ? rows.p : nullptr, h8, h9, hmax);
+    radix_scan_hist_flags(ctx, h8, 4, n, gofs.p, flags.p, 8);
+    radix_scan_hist_flags(ctx, h9, 3, n, gofs.p + SWAP_H8, flags.p + 4, 9);
+    uint64_t hv[8];
+    read_words(ctx, hv, flags.p, 8);
+    // Passes: the digits that vary.  9-bit digits when they need fewer
+    // passes (values below 2^27 whose top byte varies: 3 passes, not 4).
+    int n8 = 0, n9 = 0;
+    for (int d = 0; d < 4; ++d) n8 += hv[d] ? 1 : 0;
+    for (int d = 0; d < 3; ++d) n9 += hv[4 + d] ? 1 : 0;
+    const bool use9 = hv[7] < (1ull << 27) && n9 < n8;
     int shifts[4];
     const uint32_t *g[4];
     int npass = 0;
-    for (int d = 0; d < 4; ++d)
-        if (hv[d]) {
-            shifts[npass] = 32 + 8 * d;
-            g[npass] = gofs.p + d * 256;
-            ++npass;
-        }
+    if (use9) {
+        for (int d = 0; d < 3; ++d)
+            if (hv[4 + d]) {
+                shifts[npass] = 32 + 9 * d;
+                g[npass] = gofs.p + SWAP_H8 + d * 512;
+                ++npass;
+            }
+    } else {
+        for (int d = 0; d < 4; ++d)
+            if (hv[d]) {
+                shifts[npass] = 32 + 8 * d;
+                g[npass] = gofs.p + d * 256;
+                ++npass;
+            }
+    }
     if (npass == 0) return;  // column 0 is constant: already in order
 
     // Payload buffers: src = the relation's columns (or the row ids); pa
@@ -218,7 +289,7 @@ void sort_col0(vd_ctx *ctx, vd_rel *rel) {
             pb.p[c] = fresh_col();
         }
         sorted = radix_sort_shifts(ctx, keys.p, ktmp.p, n, shifts, npass, g, m, src, pa, pb,
-                                   &fin);
+                                   &fin, use9 ? 9 : 8);
         for (int c = 0; c < std::min(nc, 2); ++c)
             out01[c] = rel_borrowed(rel, c) ? fresh_col() : rel->cols[c];
         if (by_row)
