@@ -22,8 +22,8 @@ import sys
 # Kernel symbol (as rocprofv3 prints it) -> launch label of the library's
 # per-launch timer (vd_prof_*), so both views name the same kernel.
 LABELS = [
-    (r"k_rs_onesweep<\d+, \d+, [1-9]>", "radix_onesweep_kv"),
-    (r"k_rs_onesweep<\d+, \d+, 0>", "radix_onesweep"),
+    (r"k_rs_onesweep<\d+, \d+, [1-9][,>]", "radix_onesweep_kv"),
+    (r"k_rs_onesweep<\d+, \d+, 0[,>]", "radix_onesweep"),
     (r"k_rs_hist\b", "radix_hist"),
     (r"k_rs_scan_hist\b|k_rs_scan_hist_flags\b", "radix_scan_hist"),
     (r"k_join_expand_multi\b", "join_expand"),
