@@ -192,9 +192,11 @@ def batch_bench(eng, texts, reps=2, budget=96 << 30):
             "results_equal": same, "kernels": prof}
 
 
-def serve_bench(eng, texts, qps, seed=1, slots=4, max_batch=64, budget=96 << 30):
+def serve_bench(eng, texts, qps, seed=1, slots=4, max_batch=128, budget=96 << 30):
     """Open-loop Poisson arrivals at `qps` over the mix, served plain (one
-    query at a time) and with continuous batching; p50/p99 latency."""
+    query at a time) and with continuous batching; p50/p99 latency.  Four
+    slots of up to 128 requests: a sweep of 2-8 slots and batches of 32-256
+    on the 500M store put 64-request batches lowest at saturation."""
     rng = np.random.default_rng(seed)
     arr = list(np.cumsum(rng.exponential(1000.0 / qps, len(texts))))
 
