@@ -351,7 +351,7 @@ __global__ __launch_bounds__(MB_BLOCK) void k_pack_multi(const SortRelDev *S, in
             const uint64_t key = ((uint64_t)j << (cbits + rbits)) |
                                  ((uint64_t)D.in[0][i] << rbits) | r;
             keys[r] = key;
-            radix::hist_add(&h, key, lo_bit, hi_bit, npass);
+            radix::hist_add<true>(&h, key, lo_bit, hi_bit, npass);
         }
     }
     __syncthreads();

@@ -190,7 +190,14 @@ __attribute__((amdgpu_waves_per_eu(OsShape<BLOCK, ITEMS, M>::WAVES_PER_EU, 8))) 
         uint32_t pub = tcount;
         if ((uint32_t)tid == dmask) pub -= (uint32_t)TILE - nvalid;  // padding
         uint32_t *mine = lookback + (uint64_t)tile * RBINS + tid;
-        if (tile == 0) {
+        // A digit no key of the whole input holds (bins above a narrow last
+        // digit, the top bins of the highest digit) is neither published
+        // nor looked up: no tile ever reads it.
+        const uint32_t gcount =
+            (tid + 1 < RBINS ? gofs[tid + 1] : (uint32_t)n) - gofs[tid];
+        if (gcount == 0) {
+            // nothing to publish: excl stays 0 and no staged key uses it
+        } else if (tile == 0) {
             lb_store(mine, LB_INC | pub);
         } else {
             lb_store(mine, LB_AGG | pub);

@@ -31,27 +31,34 @@ __device__ __forceinline__ void hist_zero(LdsHistN<NP> *s) {
 }
 
 // Count the digits of one key for passes [0, npass), digit p covering bits
-// [lo_bit + 8p, min(lo_bit + 8p + 8, hi_bit)).  May be called by a subset
-// of a wave's lanes.  When every calling lane of the wave holds the same
-// digit (the top digits of small IDs), one lane adds the lane count: LDS
-// atomics on one address from many lanes are serialised.
-template <int NP>
+// [lo_bit + 8p, min(lo_bit + 8p + 8, hi_bit)).  With UNIFORM (for keys
+// whose top digits are often equal across a wave, such as a segment id in
+// the top bits), a digit held by every calling lane of the wave is added
+// once with the lane count: LDS atomics on one address are serialised.
+// Without it, one atomic per digit (cheaper for random digits).
+template <bool UNIFORM = false, int NP>
 __device__ __forceinline__ void hist_add(LdsHistN<NP> *s, uint64_t key, int lo_bit,
                                          int hi_bit, int npass) {
     const int w = threadIdx.x / 64;
-    const uint64_t act = __ballot(1);
-    const int leader = __ffsll((unsigned long long)act) - 1;
-    const bool is_leader = (int)(threadIdx.x & 63) == leader;
+    uint64_t act = 0;
+    int leader = 0;
+    if (UNIFORM) {
+        act = __ballot(1);
+        leader = __ffsll((unsigned long long)act) - 1;
+    }
     for (int p = 0; p < npass && p < NP; ++p) {
         const int sh = lo_bit + p * RBITS;
         const int width = min(RBITS, hi_bit - sh);
         const uint32_t d = (uint32_t)(key >> sh) & ((1u << width) - 1u);
-        const uint32_t d0 = __shfl(d, leader, 64);
-        if (__ballot(d == d0) == act) {
-            if (is_leader) atomicAdd(&s->h[w][p][d0], (uint32_t)__popcll(act));
-        } else {
-            atomicAdd(&s->h[w][p][d], 1u);
+        if (UNIFORM) {
+            const uint32_t d0 = __shfl(d, leader, 64);
+            if (__ballot(d == d0) == act) {
+                if ((int)(threadIdx.x & 63) == leader)
+                    atomicAdd(&s->h[w][p][d0], (uint32_t)__popcll(act));
+                continue;
+            }
         }
+        atomicAdd(&s->h[w][p][d], 1u);
     }
 }
 
