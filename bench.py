@@ -207,9 +207,16 @@ def serve_bench(eng, texts, qps, seed=1, slots=4, max_batch=128, budget=96 << 30
                 "qps": len(recs) / (span / 1e3), "batches": len({r["batch"] for r in recs})}
 
     plain, _ = eng.serve(texts, arr, plain=True)
-    served, _ = eng.serve(texts, arr, slots=slots, max_batch=max_batch, mem_budget=budget)
+    # The served stream is short (tens of ms at saturation) and its rate
+    # varies with how the first batches form: three runs, the median kept.
+    runs = []
+    for _ in range(3):
+        served, _ = eng.serve(texts, arr, slots=slots, max_batch=max_batch, mem_budget=budget)
+        runs.append(summarise(served))
+    runs.sort(key=lambda r: r["qps"])
     out = {"offered_qps": qps, "requests": len(texts), "slots": slots, "max_batch": max_batch,
-           "mem_budget": budget, "plain": summarise(plain), "served": summarise(served)}
+           "mem_budget": budget, "plain": summarise(plain), "served": runs[1],
+           "served_qps_runs": [r["qps"] for r in runs]}
     out["qps_speedup"] = out["served"]["qps"] / out["plain"]["qps"]
     return out
 
