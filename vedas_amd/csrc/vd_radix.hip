@@ -153,21 +153,25 @@ __attribute__((amdgpu_waves_per_eu(OsShape<BLOCK, ITEMS, M>::WAVES_PER_EU, 8))) 
 
     // Rank within the wave: peers = lanes holding the same digit (digit bits
     // above `width` are zero for every lane, so their ballots change nothing).
-    const uint64_t lt = (1ull << lane) - 1ull;
+    // Per digit bit: m = 0 where the lane's bit is set, all-ones where it is
+    // clear (sign-extended bit of ~d), so the lane keeps `bal ^ m` = the
+    // lanes agreeing with it; both 32-bit halves of the mask use the same m.
     uint32_t rk[ITEMS];
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
         const uint32_t d = (uint32_t)(k[j] >> shift) & dmask;
-        uint64_t peers = ~0ull;
+        const uint32_t nd = ~d;
+        uint32_t plo = ~0u, phi = ~0u;
 #pragma unroll
         for (int b = 0; b < DB; ++b) {
-            const int bit = (d >> b) & 1;
-            const uint64_t bal = __ballot(bit);
-            peers &= bit ? bal : ~bal;
+            const uint32_t m = (uint32_t)((int32_t)(nd << (31 - b)) >> 31);
+            const uint64_t bal = __ballot(m == 0u);
+            plo &= (uint32_t)bal ^ m;
+            phi &= (uint32_t)(bal >> 32) ^ m;
         }
         const uint32_t before = s_wcnt[w][d];
-        const uint32_t r = (uint32_t)__popcll(peers & lt);
-        const uint32_t cnt = (uint32_t)__popcll(peers);
+        const uint32_t r = __builtin_amdgcn_mbcnt_hi(phi, __builtin_amdgcn_mbcnt_lo(plo, 0u));
+        const uint32_t cnt = (uint32_t)(__builtin_popcount(plo) + __builtin_popcount(phi));
         __builtin_amdgcn_wave_barrier();
         if (r == cnt - 1) s_wcnt[w][d] = (uint16_t)(before + cnt);
         __builtin_amdgcn_wave_barrier();
