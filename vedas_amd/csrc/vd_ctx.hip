@@ -97,8 +97,15 @@ void prof_end(vd_ctx *ctx, const char *name, double bytes, hipEvent_t ev0) {
     ctx->pending.push_back({name, bytes, ev0, ev1});
 }
 
-void prof_collect(vd_ctx *ctx) {
-    for (auto &p : ctx->pending) {
+void prof_collect(vd_ctx *ctx, bool done_only) {
+    size_t k = 0;
+    for (; k < ctx->pending.size(); ++k) {
+        auto &p = ctx->pending[k];
+        if (done_only) {
+            const hipError_t q = hipEventQuery(p.stop);
+            if (q == hipErrorNotReady) break;
+            VD_HIP(q);
+        }
         float ms = 0.f;
         VD_HIP(hipEventElapsedTime(&ms, p.start, p.stop));
         auto &st = ctx->prof_stats[p.name];
@@ -108,7 +115,7 @@ void prof_collect(vd_ctx *ctx) {
         ctx->event_pool.push_back(p.start);
         ctx->event_pool.push_back(p.stop);
     }
-    ctx->pending.clear();
+    ctx->pending.erase(ctx->pending.begin(), ctx->pending.begin() + (std::ptrdiff_t)k);
 }
 
 // Most waits here are short: a size read-back right after the kernel that
@@ -128,22 +135,52 @@ void sync(vd_ctx *ctx) {
             break;
         }
     }
-    if (!ctx->pending.empty()) prof_collect(ctx);
+    if (!ctx->pending.empty()) prof_collect(ctx, false);
 }
 
-uint64_t read_u64(vd_ctx *ctx, const uint64_t *dptr) {
-    VD_HIP(hipMemcpyAsync(ctx->h_pinned, dptr, sizeof(uint64_t),
-                          hipMemcpyDeviceToHost, ctx->stream));
-    sync(ctx);
-    return ctx->h_pinned[0];
+// ---- small read-backs -------------------------------------------------------------
+Readback rb_next(vd_ctx *ctx) { return Readback{ctx->d_rb, ++ctx->rb_seq}; }
+
+void rb_wait(vd_ctx *ctx, const Readback &rb, uint64_t *dst, int n) {
+    // A read-back usually follows the kernels it waits for by well under a
+    // millisecond; after kSpinMicros the thread blocks on the stream, which
+    // also surfaces a failed kernel (its sequence word never comes).
+    constexpr double kSpinMicros = 20000.0;
+    const uint64_t *word = ctx->h_rb + RB_WORDS;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t spin = 1; __atomic_load_n(word, __ATOMIC_ACQUIRE) != rb.seq; ++spin) {
+        __builtin_ia32_pause();
+        if ((spin & 1023) != 0) continue;
+        const double us = std::chrono::duration<double, std::micro>(
+                              std::chrono::steady_clock::now() - t0).count();
+        if (us > kSpinMicros) {
+            VD_HIP(hipStreamSynchronize(ctx->stream));
+            VD_REQUIRE(__atomic_load_n(word, __ATOMIC_ACQUIRE) == rb.seq, VD_EHIP,
+                       "read-back kernel did not publish its words");
+            break;
+        }
+    }
+    std::memcpy(dst, ctx->h_rb, (size_t)n * sizeof(uint64_t));
+    if (!ctx->pending.empty()) prof_collect(ctx, true);
+}
+
+__global__ __launch_bounds__(64) void k_rb_copy(const uint64_t *__restrict__ src, int n,
+                                                Readback rb) {
+    for (int i = (int)threadIdx.x; i < n; i += 64) rb.dst[i] = src[i];
+    dev::rb_publish(rb);
 }
 
 void read_words(vd_ctx *ctx, uint64_t *dst, const uint64_t *dptr, int n) {
-    VD_REQUIRE(n <= 64, VD_EINVAL, "read_words: too many words");
-    VD_HIP(hipMemcpyAsync(ctx->h_pinned, dptr, n * sizeof(uint64_t),
-                          hipMemcpyDeviceToHost, ctx->stream));
-    sync(ctx);
-    std::memcpy(dst, ctx->h_pinned, n * sizeof(uint64_t));
+    VD_REQUIRE(n >= 0 && n <= RB_WORDS, VD_EINVAL, "read_words: too many words");
+    const Readback rb = rb_next(ctx);
+    launch(ctx, "read_back", 0.0, dim3(1), dim3(64), 0, k_rb_copy, dptr, n, rb);
+    rb_wait(ctx, rb, dst, n);
+}
+
+uint64_t read_u64(vd_ctx *ctx, const uint64_t *dptr) {
+    uint64_t v = 0;
+    read_words(ctx, &v, dptr, 1);
+    return v;
 }
 
 void upload(vd_ctx *ctx, void *dst, const void *src, size_t bytes) {
@@ -169,6 +206,14 @@ void upload(vd_ctx *ctx, void *dst, const void *src, size_t bytes) {
 
 void download(vd_ctx *ctx, void *dst, const void *src, size_t bytes) {
     if (bytes == 0) return;
+    if (bytes % sizeof(uint64_t) == 0 && bytes <= RB_WORDS * sizeof(uint64_t) &&
+        reinterpret_cast<uintptr_t>(src) % alignof(uint64_t) == 0) {
+        uint64_t w[RB_WORDS];
+        const int n = (int)(bytes / sizeof(uint64_t));
+        read_words(ctx, w, static_cast<const uint64_t *>(src), n);
+        std::memcpy(dst, w, bytes);
+        return;
+    }
     VD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
     sync(ctx);
 }
@@ -242,11 +287,16 @@ int vd_ctx_create(int device, vd_ctx **out) {
         ctx->device = device;
         VD_HIP(hipSetDevice(device));
         VD_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-        VD_HIP(hipHostMalloc((void **)&ctx->h_pinned, 64 * sizeof(uint64_t),
-                             hipHostMallocDefault));
+        // Coherent: kernel stores reach host memory without a cache flush
+        // of their own, so the sequence word is seen as soon as it is written.
+        VD_HIP(hipHostMalloc((void **)&ctx->h_rb, (vd::RB_WORDS + 1) * sizeof(uint64_t),
+                             hipHostMallocCoherent | hipHostMallocMapped));
+        std::memset(ctx->h_rb, 0, (vd::RB_WORDS + 1) * sizeof(uint64_t));
+        VD_HIP(hipHostGetDevicePointer((void **)&ctx->d_rb, ctx->h_rb, 0));
     } catch (const Error &e) {
         std::fprintf(stderr, "vd_ctx_create: %s\n", e.what());
         int code = e.code;
+        if (ctx->h_rb) (void)hipHostFree(ctx->h_rb);
         if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return code;
@@ -268,7 +318,7 @@ int vd_ctx_destroy(vd_ctx *ctx) {
     if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
     for (auto &kv : ctx->free_blocks) (void)hipFree(kv.second);
     for (auto &kv : ctx->live_blocks) (void)hipFree(kv.first);
-    if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
+    if (ctx->h_rb) (void)hipHostFree(ctx->h_rb);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return VD_OK;

@@ -813,8 +813,7 @@ int vd_select(vd_ctx *ctx, const vd_index *idx, int perm, uint64_t begin,
         launch(ctx, "select_pieces", 0.0, dim3(1), dim3(64), 0, k_select_pieces,
                (const uint32_t *)pa.y, begin, end, clip, lo, hi, sk, nskip, dp.p);
         Pieces hp;
-        VD_HIP(hipMemcpyAsync(&hp, dp.p, sizeof(Pieces), hipMemcpyDeviceToHost, ctx->stream));
-        sync(ctx);
+        download(ctx, &hp, dp.p, sizeof(Pieces));
         *out = select_pieces_rel(ctx, idx, pa, cols, distinct, hp, dp.p);
     });
 }
@@ -832,9 +831,7 @@ int vd_select_key(vd_ctx *ctx, const vd_index *idx, int perm, uint32_t k1, int h
         launch(ctx, "select_pieces", 0.0, dim3(1), dim3(64), 0, k_lookup_pieces, perm_dev(pa),
                k1, has_k2, k2, clip, lo, hi, sk, nskip, dk.p);
         KeyPieces hk;
-        VD_HIP(hipMemcpyAsync(&hk, dk.p, sizeof(KeyPieces), hipMemcpyDeviceToHost,
-                              ctx->stream));
-        sync(ctx);
+        download(ctx, &hk, dk.p, sizeof(KeyPieces));
         if (begin) *begin = hk.begin;
         if (end) *end = hk.end;
         *out = select_pieces_rel(ctx, idx, pa, cols, distinct, hk.p, &dk.p->p);

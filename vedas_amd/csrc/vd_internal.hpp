@@ -83,8 +83,12 @@ struct vd_ctx {
     // than the one issuing work on the context.
     std::recursive_mutex alloc_mu;
 
-    // Pinned host staging for small read-backs.
-    uint64_t *h_pinned = nullptr;  // 64 words
+    // Read-back slot: RB_WORDS words plus a sequence word in coherent
+    // pinned host memory that kernels write directly (d_rb is its device
+    // address).  A read-back kernel stores its words, then the sequence
+    // number rb_seq of the read-back; the host waits for that number.
+    uint64_t *h_rb = nullptr, *d_rb = nullptr;
+    uint64_t rb_seq = 0;
     // Pinned staging for descriptor uploads; reused once the previous
     // upload's event has completed.
     void *h_stage = nullptr;
@@ -159,7 +163,8 @@ struct DBuf {
 hipEvent_t take_event(vd_ctx *ctx);
 void prof_begin(vd_ctx *ctx, hipEvent_t *ev);
 void prof_end(vd_ctx *ctx, const char *name, double bytes, hipEvent_t ev0);
-void prof_collect(vd_ctx *ctx);  // after a stream sync
+// After a stream sync, or (done_only) the launches whose stop event is done.
+void prof_collect(vd_ctx *ctx, bool done_only);
 inline bool prof_selected(const vd_ctx *ctx, const char *name) {
     if (ctx->prof_only.empty()) return true;
     for (const auto &l : ctx->prof_only)
@@ -184,12 +189,28 @@ inline void launch(vd_ctx *ctx, const char *name, double bytes, dim3 grid,
 }
 
 void sync(vd_ctx *ctx);
+
+// ---- small read-backs ----------------------------------------------------------
+// The host learns that a read-back is complete from the sequence word the
+// kernel writes into pinned memory, not from the stream's completion
+// signal: the word lands a few microseconds after the store, the signal
+// tens of microseconds later.  Every kernel before the read-back kernel on
+// the stream has finished when its words are visible.
+constexpr int RB_WORDS = 4096;
+struct Readback {
+    uint64_t *dst;  // RB_WORDS words, then the sequence word
+    uint64_t seq;
+};
+Readback rb_next(vd_ctx *ctx);
+// Waits for rb's sequence word, then copies n words of the slot to dst.
+void rb_wait(vd_ctx *ctx, const Readback &rb, uint64_t *dst, int n);
 uint64_t read_u64(vd_ctx *ctx, const uint64_t *dptr);          // syncs
 // Stream-ordered host -> device copy through pinned staging (the host
 // buffer may be reused as soon as this returns).
 void upload(vd_ctx *ctx, void *dst, const void *src, size_t bytes);
 // Device -> host copy of `bytes`, synchronising the stream.
 void download(vd_ctx *ctx, void *dst, const void *src, size_t bytes);
+// n <= RB_WORDS device words to the host through the read-back slot.
 void read_words(vd_ctx *ctx, uint64_t *dst, const uint64_t *dptr, int n);
 
 // ---- primitives ------------------------------------------------------------
@@ -299,6 +320,16 @@ constexpr int WAVE = 64;
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (WAVE - 1); }
 
+// Ends a read-back: called by every lane of the one wave that wrote the
+// words of rb.dst (after those stores), it makes them visible to the host,
+// then stores the sequence word.
+__device__ __forceinline__ void rb_publish(const Readback &rb) {
+    __threadfence_system();
+    if (lane_id() == 0)
+        __hip_atomic_store(rb.dst + RB_WORDS, rb.seq, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_incl_scan(T v) {
     const int lane = lane_id();
@@ -339,6 +370,37 @@ __device__ __forceinline__ T block_excl_scan(T v, T *s_tmp, T *total) {
     return base + inc - v;
 }
 
+// tpre[t] = sum of tot[0..t), t = 0..ntiles, by one block of TT_BLOCK
+// threads (s_tmp: TT_BLOCK / 64 words): a thread adds TT_PER consecutive
+// totals (independent loads), then one block scan per TT_BLOCK * TT_PER
+// totals.
+constexpr int TT_BLOCK = 1024, TT_PER = 16;
+__device__ __forceinline__ void scan_tile_totals_block(const uint64_t *__restrict__ tot,
+                                                       uint64_t ntiles,
+                                                       uint64_t *__restrict__ tpre,
+                                                       uint64_t *s_tmp) {
+    uint64_t carry = 0;
+    for (uint64_t b = 0; b < ntiles; b += (uint64_t)TT_BLOCK * TT_PER) {
+        const uint64_t t0 = b + (uint64_t)threadIdx.x * TT_PER;
+        uint64_t v[TT_PER];
+        uint64_t run = 0;
+#pragma unroll
+        for (int j = 0; j < TT_PER; ++j) {
+            v[j] = t0 + j < ntiles ? tot[t0 + j] : 0;
+            run += v[j];
+        }
+        uint64_t sum;
+        uint64_t ex = carry + block_excl_scan<TT_BLOCK>(run, s_tmp, &sum);
+#pragma unroll
+        for (int j = 0; j < TT_PER; ++j) {
+            if (t0 + j < ntiles) tpre[t0 + j] = ex;
+            ex += v[j];
+        }
+        carry += sum;
+    }
+    if (threadIdx.x == 0) tpre[ntiles] = carry;
+}
+
 template <int BLOCK, typename T>
 __device__ __forceinline__ T block_reduce(T v, T *s_tmp) {
     constexpr int NW = BLOCK / WAVE;
@@ -350,6 +412,29 @@ __device__ __forceinline__ T block_reduce(T v, T *s_tmp) {
     for (int i = 0; i < NW; ++i) tot += s_tmp[i];
     __syncthreads();
     return tot;
+}
+
+// Block-wide min and max of u32 values (every thread ends with both);
+// s_tmp must hold BLOCK/64 words.  All threads must call it.
+template <int BLOCK>
+__device__ __forceinline__ void block_min_max(uint32_t &mn, uint32_t &mx, uint64_t *s_tmp) {
+    constexpr int NW = BLOCK / WAVE;
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        mn = min(mn, (uint32_t)__shfl_xor(mn, off, WAVE));
+        mx = max(mx, (uint32_t)__shfl_xor(mx, off, WAVE));
+    }
+    if (lane_id() == 0) s_tmp[threadIdx.x / WAVE] = ((uint64_t)mx << 32) | mn;
+    __syncthreads();
+    mn = ~0u;
+    mx = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+        const uint64_t w = s_tmp[i];
+        mn = min(mn, (uint32_t)w);
+        mx = max(mx, (uint32_t)(w >> 32));
+    }
+    __syncthreads();
 }
 
 // First index in [lo, hi) with a[i] >= v (a sorted ascending).

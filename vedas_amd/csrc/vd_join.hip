@@ -8,9 +8,10 @@
 //                     when it is longer than the LDS capacity.  The same
 //                     kernel scans the counts of its tile: loc_i = exclusive
 //                     scan of cnt_i within the tile, plus the tile total.
-//  2. scan_tile_totals : one block scans the tile totals into tpre (tpre[t] =
-//                     rows before tile t; tpre[ntiles] = output size, read
-//                     back once to allocate).  The output offset of left row
+//  2. k_join_scan_total : one block scans the tile totals into tpre (tpre[t]
+//                     = rows before tile t; tpre[ntiles] = output size, read
+//                     back once to allocate, with the first and last output
+//                     keys).  The output offset of left row
 //                     i is off_i = tpre[i / 2048] + loc_i (join_off below).
 //  3. k_join_partition / k_join_expand : load-balanced expansion.  The merge
 //                     of the left rows (row i at merge position i + off_i)
@@ -112,7 +113,8 @@ template <int NK>
 __global__ __launch_bounds__(JB_BLOCK) void k_join_bounds(
     KeyCols L, uint64_t nl, KeyCols R, uint64_t nr, int rcap,
     const uint64_t *__restrict__ ranges, uint32_t *__restrict__ lower,
-    uint64_t *__restrict__ loc, uint64_t *__restrict__ tile_total) {
+    uint64_t *__restrict__ loc, uint64_t *__restrict__ tile_total,
+    uint64_t *__restrict__ tile_keys) {
     using T = typename KeyT<NK>::T;
     constexpr int SR_WORDS = JB_RCAP_BYTES / sizeof(T);
     static_assert(JB_RCAP_BYTES >= JB_TILE * 12, "counts and offsets reuse the staging buffer");
@@ -150,6 +152,28 @@ __global__ __launch_bounds__(JB_BLOCK) void k_join_bounds(
         lkey[j] = i < nl ? KeyT<NK>::at(L, i) : T(0);
     }
     __syncthreads();
+    // Every row's first staged entry >= key (t0) and > key (t1), branchless
+    // and in lockstep over the rows: the 2 * ITEMS LDS loads of a step are
+    // independent, so their latencies overlap.
+    uint32_t t0[JB_ITEMS], t1[JB_ITEMS];
+#pragma unroll
+    for (int j = 0; j < JB_ITEMS; ++j) t0[j] = t1[j] = 0;
+    if (nd > 0) {
+        for (uint32_t len = (uint32_t)nd; len > 1;) {
+            const uint32_t half = len >> 1;
+#pragma unroll
+            for (int j = 0; j < JB_ITEMS; ++j) {
+                t0[j] += s_r[t0[j] + half] < lkey[j] ? half : 0u;
+                t1[j] += lkey[j] < s_r[t1[j] + half] ? 0u : half;
+            }
+            len -= half;
+        }
+#pragma unroll
+        for (int j = 0; j < JB_ITEMS; ++j) {
+            t0[j] += s_r[t0[j]] < lkey[j] ? 1u : 0u;
+            t1[j] += lkey[j] < s_r[t1[j]] ? 0u : 1u;
+        }
+    }
     uint32_t cnt[JB_ITEMS];
 #pragma unroll
     for (int j = 0; j < JB_ITEMS; ++j) {
@@ -157,17 +181,14 @@ __global__ __launch_bounds__(JB_BLOCK) void k_join_bounds(
         uint32_t c = 0;
         if (i < nl) {
             const T key = lkey[j];
-            uint64_t lo, hi;
-            if (k == 1) {
-                lo = dev::lower_bound(s_r, 0, m, key);
-                hi = dev::upper_bound(s_r, lo, m, key);
-            } else {
-                // first directory entry >= key / > key, then the k-1 keys before it
-                const uint64_t t0 = dev::lower_bound(s_r, 0, nd, key);
-                const uint64_t t1 = dev::upper_bound(s_r, t0, nd, key);
-                lo = g_lower<NK>(R, rlo + (t0 ? (t0 - 1) * k + 1 : 0), rlo + min(t0 * k, m),
+            uint64_t lo = t0[j], hi = t1[j];
+            if (k > 1) {
+                // staged entries are every k-th key: the bounds lie within
+                // the k-1 keys before the directory entry found
+                const uint64_t d0 = t0[j], d1 = t1[j];
+                lo = g_lower<NK>(R, rlo + (d0 ? (d0 - 1) * k + 1 : 0), rlo + min(d0 * k, m),
                                  key) - rlo;
-                hi = g_upper<NK>(R, rlo + (t1 ? (t1 - 1) * k + 1 : 0), rlo + min(t1 * k, m),
+                hi = g_upper<NK>(R, rlo + (d1 ? (d1 - 1) * k + 1 : 0), rlo + min(d1 * k, m),
                                  key) - rlo;
             }
             lower[i] = (uint32_t)(rlo + lo);
@@ -175,7 +196,19 @@ __global__ __launch_bounds__(JB_BLOCK) void k_join_bounds(
         }
         cnt[j] = c;
     }
-    __syncthreads();  // the right keys are no longer read
+    // Smallest and largest first key column of the tile's rows with matches
+    // (the first and last output keys of the tile).
+    uint32_t kmn = ~0u, kmx = 0;
+#pragma unroll
+    for (int j = 0; j < JB_ITEMS; ++j) {
+        const uint32_t c0 = NK == 1 ? (uint32_t)lkey[j] : (uint32_t)((uint64_t)lkey[j] >> 32);
+        if (cnt[j]) {
+            kmn = min(kmn, c0);
+            kmx = max(kmx, c0);
+        }
+    }
+    // (its barriers also end every read of the staged right keys)
+    dev::block_min_max<JB_BLOCK>(kmn, kmx, s_tmp);
     uint64_t *s_off = reinterpret_cast<uint64_t *>(s_r);
     uint32_t *s_cnt = reinterpret_cast<uint32_t *>(s_off + JB_TILE);
 #pragma unroll
@@ -196,7 +229,10 @@ __global__ __launch_bounds__(JB_BLOCK) void k_join_bounds(
         s_off[threadIdx.x * JB_ITEMS + j] = ex;
         ex += mine[j];
     }
-    if (threadIdx.x == 0) tile_total[tile] = tot;
+    if (threadIdx.x == 0) {
+        tile_total[tile] = tot;
+        tile_keys[tile] = ((uint64_t)kmx << 32) | kmn;
+    }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < JB_ITEMS; ++j) {
@@ -221,42 +257,30 @@ __global__ __launch_bounds__(256) void k_join_partition(
     if (threadIdx.x % JR_GROUP == 0) mp[b] = lo;
 }
 
-// out = {total, first output key, last output key}: the total is the
-// last tile prefix; the first output row comes from the first left row with
-// matches (first i with off[i+1] > 0, off[nl] = total), the last from the
-// last one (last i with off[i] < total).
-// One wave (64-ary searches over the offsets).
-__global__ __launch_bounds__(64) void k_join_total_keys(JoinOff off, uint64_t nl, uint64_t ntiles,
-                                                        const uint32_t *__restrict__ lkey,
-                                                        uint64_t *out) {
-    const uint64_t total = off.tpre[ntiles];
-    auto at = [&](uint64_t i) { return i < nl ? off(i) : total; };
-    // first index q in [lo, hi) with pred(q) true, for a monotone pred
-    auto search = [&](uint64_t lo, uint64_t hi, auto pred) {
-        const int lane = threadIdx.x & 63;
-        while (hi - lo > 64) {
-            const uint64_t step = (hi - lo + 63) / 64;
-            const uint64_t q = lo + (uint64_t)(lane + 1) * step - 1;
-            const int c = __popcll(__ballot(q < hi && !pred(q)));
-            const uint64_t qc = lo + (uint64_t)(c + 1) * step - 1;
-            hi = qc < hi ? qc + 1 : hi;
-            lo = min(lo + (uint64_t)c * step, hi);
-        }
-        const uint64_t q = lo + (uint64_t)lane;
-        return lo + (uint64_t)__popcll(__ballot(q < hi && !pred(q)));
-    };
-    uint64_t first = 0, last = 0;
-    if (total > 0) {
-        const uint64_t i0 = search(0, nl, [&](uint64_t i) { return at(i + 1) > 0; });
-        const uint64_t i1 = search(0, nl, [&](uint64_t i) { return at(i) >= total; }) - 1;
-        first = lkey[i0];
-        last = lkey[i1];
+// One block: scans the tile totals into the tile prefixes, then reads back
+// {total, first output key, last output key} (the output's column 0 holds
+// the left keys of matched rows in order: the smallest and largest of the
+// tiles' matched keys).
+__global__ __launch_bounds__(dev::TT_BLOCK) void k_join_scan_total(
+    const uint64_t *__restrict__ ttot, const uint64_t *__restrict__ tkeys, uint64_t ntiles,
+    uint64_t *__restrict__ tpre, Readback rb) {
+    __shared__ uint64_t s_tmp[dev::TT_BLOCK / 64];
+    dev::scan_tile_totals_block(ttot, ntiles, tpre, s_tmp);
+    uint32_t mn = ~0u, mx = 0;
+    for (uint64_t t = threadIdx.x; t < ntiles; t += dev::TT_BLOCK) {
+        const uint64_t w = tkeys[t];
+        mn = min(mn, (uint32_t)w);
+        mx = max(mx, (uint32_t)(w >> 32));
     }
+    dev::block_min_max<dev::TT_BLOCK>(mn, mx, s_tmp);
+    if (threadIdx.x >= 64) return;
     if (threadIdx.x == 0) {
-        out[0] = total;
-        out[1] = first;
-        out[2] = last;
+        const uint64_t total = tpre[ntiles];  // written by this thread
+        rb.dst[0] = total;
+        rb.dst[1] = total ? mn : 0;
+        rb.dst[2] = total ? mx : 0;
     }
+    dev::rb_publish(rb);
 }
 
 struct OutCols {
@@ -376,21 +400,22 @@ vd_rel *join_impl(vd_ctx *ctx, const vd_rel *L, const vd_rel *R) {
     const uint64_t jb_tiles = ceil_div(nl, JB_TILE);
     const int rcap = std::min<int>(ctx->join_rcap, JB_RCAP_BYTES / (NK == 2 ? 8 : 4));
     // ranges (2 per tile), tile totals (1 per tile), tile prefixes (1 per
-    // tile + the total).
-    DBuf<uint64_t> ranges(ctx, 4 * jb_tiles + 1);
+    // tile + the total), tile key bounds (1 per tile).
+    DBuf<uint64_t> ranges(ctx, 5 * jb_tiles + 1);
     uint64_t *ttot = ranges.p + 2 * jb_tiles, *tpre = ranges.p + 3 * jb_tiles;
+    uint64_t *tkeys = ranges.p + 4 * jb_tiles + 1;
     launch(ctx, "join_ranges", 0.0, dim3((unsigned)ceil_div(jb_tiles * JR_GROUP, 256)),
            dim3(256), 0, k_join_ranges<NK>, lk, nl, rk, nr, jb_tiles, ranges.p);
     launch(ctx, "join_bounds", 4.0 * NK * (double)(nl + nr) + 12.0 * nl,
            dim3((unsigned)jb_tiles), dim3(JB_BLOCK), 0, k_join_bounds<NK>, lk,
-           nl, rk, nr, rcap, (const uint64_t *)ranges.p, lower.p, off.p, ttot);
-    scan_tile_totals(ctx, ttot, jb_tiles, tpre);
+           nl, rk, nr, rcap, (const uint64_t *)ranges.p, lower.p, off.p, ttot, tkeys);
     const JoinOff joff{off.p, tpre};
-    DBuf<uint64_t> tk(ctx, 3);
-    launch(ctx, "join_total", 0.0, dim3(1), dim3(64), 0, k_join_total_keys, joff, nl, jb_tiles,
-           (const uint32_t *)L->cols[0], tk.p);
+    const Readback rb = rb_next(ctx);
+    launch(ctx, "scan_tiles", 16.0 * (double)jb_tiles, dim3(1), dim3(dev::TT_BLOCK), 0,
+           k_join_scan_total, (const uint64_t *)ttot, (const uint64_t *)tkeys, jb_tiles, tpre,
+           rb);
     uint64_t h[3];
-    read_words(ctx, h, tk.p, 3);
+    rb_wait(ctx, rb, h, 3);
     const uint64_t total = h[0];
     VD_REQUIRE(total < (1ull << 40), VD_ERANGE, "join output too large");
 

@@ -100,22 +100,11 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_tile_scan(
     if (tile_tot && threadIdx.x == 0) tile_tot[blockIdx.x] = tot;
 }
 
-// tpre[t] = sum of tot[0..t), t = 0..ntiles (one block, chunks of 1024 with
-// a running carry).
-__global__ __launch_bounds__(1024) void k_scan_tile_totals(const uint64_t *__restrict__ tot,
-                                                           uint64_t ntiles,
-                                                           uint64_t *__restrict__ tpre) {
-    __shared__ uint64_t s_tmp[1024 / 64];
-    uint64_t carry = 0;
-    for (uint64_t b = 0; b < ntiles; b += 1024) {
-        const uint64_t t = b + threadIdx.x;
-        const uint64_t v = t < ntiles ? tot[t] : 0;
-        uint64_t sum;
-        const uint64_t ex = dev::block_excl_scan<1024>(v, s_tmp, &sum);
-        if (t < ntiles) tpre[t] = carry + ex;
-        carry += sum;
-    }
-    if (threadIdx.x == 0) tpre[ntiles] = carry;
+// tpre[t] = sum of tot[0..t), t = 0..ntiles.
+__global__ __launch_bounds__(dev::TT_BLOCK) void k_scan_tile_totals(
+    const uint64_t *__restrict__ tot, uint64_t ntiles, uint64_t *__restrict__ tpre) {
+    __shared__ uint64_t s_tmp[dev::TT_BLOCK / 64];
+    dev::scan_tile_totals_block(tot, ntiles, tpre, s_tmp);
 }
 
 template <typename InT>
@@ -158,7 +147,8 @@ void scan_u64(vd_ctx *ctx, const uint64_t *in, uint64_t *out, uint64_t n,
 static_assert(SCAN_TILE == OFF_TILE, "tile offsets use the scan tile");
 
 void scan_tile_totals(vd_ctx *ctx, const uint64_t *tot, uint64_t ntiles, uint64_t *tpre) {
-    launch(ctx, "scan_tiles", 8.0 * (double)ntiles, dim3(1), dim3(1024), 0, k_scan_tile_totals,
+    launch(ctx, "scan_tiles", 8.0 * (double)ntiles, dim3(1), dim3(dev::TT_BLOCK), 0,
+           k_scan_tile_totals,
            tot, ntiles, tpre);
 }
 
