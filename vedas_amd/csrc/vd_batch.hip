@@ -133,21 +133,22 @@ __global__ __launch_bounds__(MB_BLOCK) void k_join_bounds_multi(
         lkey[it] = i < nl ? lk[i] : 0u;
     }
     __syncthreads();
+    // first staged entry >= key (t0) and > key (t1) of every row
+    uint32_t t0[MB_ITEMS], t1[MB_ITEMS];
+    dev::lockstep_bounds<MB_ITEMS>(s_r, (uint32_t)nd, lkey, t0, t1);
 #pragma unroll
     for (int it = 0; it < MB_ITEMS; ++it) {
         const uint64_t i = base + (uint64_t)it * MB_BLOCK + threadIdx.x;
         if (i >= nl) break;
         const uint32_t key = lkey[it];
-        uint64_t lo, hi;
-        if (k == 1) {
-            lo = dev::lower_bound(s_r, 0, m, key);
-            hi = dev::upper_bound(s_r, lo, m, key);
-        } else {
-            const uint64_t t0 = dev::lower_bound(s_r, 0, nd, key);
-            const uint64_t t1 = dev::upper_bound(s_r, t0, nd, key);
-            lo = dev::lower_bound(rk, rlo + (t0 ? (t0 - 1) * k + 1 : 0), rlo + min(t0 * k, m),
+        uint64_t lo = t0[it], hi = t1[it];
+        if (k > 1) {
+            // staged entries are every k-th key: the bounds lie within the
+            // k-1 keys before the directory entry found
+            const uint64_t d0 = t0[it], d1 = t1[it];
+            lo = dev::lower_bound(rk, rlo + (d0 ? (d0 - 1) * k + 1 : 0), rlo + min(d0 * k, m),
                                   key) - rlo;
-            hi = dev::upper_bound(rk, rlo + (t1 ? (t1 - 1) * k + 1 : 0), rlo + min(t1 * k, m),
+            hi = dev::upper_bound(rk, rlo + (d1 ? (d1 - 1) * k + 1 : 0), rlo + min(d1 * k, m),
                                   key) - rlo;
         }
         lower[D.lbase + i] = (uint32_t)(rlo + lo);
@@ -224,18 +225,17 @@ __global__ __launch_bounds__(MJ_BLOCK) void k_join_expand_multi(
         // uniform, columns are written one after the other, coalesced.
         const JPairDev &D = P[uniform(s_pair[0])];
         uint32_t lrow[ITEMS], rrow[ITEMS];
+        int kk[ITEMS];  // last staged row with s_off <= output index
+        dev::lockstep_last_le<ITEMS>(s_off, m, [&](int it) {
+            return b0 + (uint64_t)(it * MJ_BLOCK + threadIdx.x);
+        }, kk);
 #pragma unroll
         for (int it = 0; it < ITEMS; ++it) {
             const int q = it * MJ_BLOCK + threadIdx.x;
             lrow[it] = rrow[it] = 0;
             if (q < nout) {
                 const uint64_t o = b0 + q;
-                int lo = 0, hi = m;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (s_off[mid] <= o) lo = mid + 1; else hi = mid;
-                }
-                const int k = lo - 1;
+                const int k = kk[it];
                 lrow[it] = (uint32_t)(sa + k - D.lbase);
                 rrow[it] = s_low[k] + (uint32_t)(o - s_off[k]);
             }

@@ -437,6 +437,47 @@ __device__ __forceinline__ void block_min_max(uint32_t &mn, uint32_t &mx, uint64
     __syncthreads();
 }
 
+// For ITEMS keys at once: lo[j] = first index in a[0, n) with a >= key[j],
+// hi[j] = first index with a > key[j] (a sorted ascending).  Branchless and
+// in lockstep over the keys, so the 2 * ITEMS loads of a step are
+// independent and their latencies overlap (n is uniform over the block).
+template <int ITEMS, typename T, typename K>
+__device__ __forceinline__ void lockstep_bounds(const T *a, uint32_t n, const K (&key)[ITEMS],
+                                                uint32_t (&lo)[ITEMS], uint32_t (&hi)[ITEMS]) {
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) lo[j] = hi[j] = 0;
+    if (n == 0) return;
+    for (uint32_t len = n; len > 1;) {
+        const uint32_t half = len >> 1;
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            lo[j] += a[lo[j] + half] < key[j] ? half : 0u;
+            hi[j] += key[j] < a[hi[j] + half] ? 0u : half;
+        }
+        len -= half;
+    }
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        lo[j] += a[lo[j]] < key[j] ? 1u : 0u;
+        hi[j] += key[j] < a[hi[j]] ? 0u : 1u;
+    }
+}
+
+// For ITEMS values v(j) at once: k[j] = last index in a[0, n) with
+// a[k] <= v(j), where a is ascending and a[0] <= every value (n >= 1).
+// Branchless and in lockstep, as lockstep_bounds.
+template <int ITEMS, typename T, typename F>
+__device__ __forceinline__ void lockstep_last_le(const T *a, int n, F v, int (&k)[ITEMS]) {
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) k[j] = 0;
+    for (int len = n; len > 1;) {
+        const int half = len >> 1;
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) k[j] += a[k[j] + half] <= v(j) ? half : 0;
+        len -= half;
+    }
+}
+
 // First index in [lo, hi) with a[i] >= v (a sorted ascending).
 template <typename T, typename V>
 __device__ __forceinline__ uint64_t lower_bound(const T *a, uint64_t lo,

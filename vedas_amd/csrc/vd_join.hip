@@ -152,28 +152,9 @@ __global__ __launch_bounds__(JB_BLOCK) void k_join_bounds(
         lkey[j] = i < nl ? KeyT<NK>::at(L, i) : T(0);
     }
     __syncthreads();
-    // Every row's first staged entry >= key (t0) and > key (t1), branchless
-    // and in lockstep over the rows: the 2 * ITEMS LDS loads of a step are
-    // independent, so their latencies overlap.
+    // Every row's first staged entry >= key (t0) and > key (t1).
     uint32_t t0[JB_ITEMS], t1[JB_ITEMS];
-#pragma unroll
-    for (int j = 0; j < JB_ITEMS; ++j) t0[j] = t1[j] = 0;
-    if (nd > 0) {
-        for (uint32_t len = (uint32_t)nd; len > 1;) {
-            const uint32_t half = len >> 1;
-#pragma unroll
-            for (int j = 0; j < JB_ITEMS; ++j) {
-                t0[j] += s_r[t0[j] + half] < lkey[j] ? half : 0u;
-                t1[j] += lkey[j] < s_r[t1[j] + half] ? 0u : half;
-            }
-            len -= half;
-        }
-#pragma unroll
-        for (int j = 0; j < JB_ITEMS; ++j) {
-            t0[j] += s_r[t0[j]] < lkey[j] ? 1u : 0u;
-            t1[j] += lkey[j] < s_r[t1[j]] ? 0u : 1u;
-        }
-    }
+    dev::lockstep_bounds<JB_ITEMS>(s_r, (uint32_t)nd, lkey, t0, t1);
     uint32_t cnt[JB_ITEMS];
 #pragma unroll
     for (int j = 0; j < JB_ITEMS; ++j) {
@@ -330,21 +311,21 @@ __global__ __launch_bounds__(JX_BLOCK) void k_join_expand(
     }
     __syncthreads();
 
+    // Output o's left row: the last staged k with s_off[k] <= o (s_off[0]
+    // <= every output of the diagonal).
     uint32_t lrow[ITEMS], rrow[ITEMS];
     const int nout = (int)(b1 - b0);
+    int kk[ITEMS];
+    dev::lockstep_last_le<ITEMS>(s_off, m, [&](int j) {
+        return b0 + (uint64_t)(j * JX_BLOCK + threadIdx.x);
+    }, kk);
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
         const int q = j * JX_BLOCK + threadIdx.x;
         lrow[j] = rrow[j] = 0;
         if (q < nout) {
             const uint64_t o = b0 + q;
-            // last k with s_off[k] <= o
-            int lo = 0, hi = m;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (s_off[mid] <= o) lo = mid + 1; else hi = mid;
-            }
-            const int k = lo - 1;
+            const int k = kk[j];
             lrow[j] = (uint32_t)(sa + k);
             rrow[j] = s_low[k] + (uint32_t)(o - s_off[k]);
         }
