@@ -49,15 +49,17 @@ __global__ __launch_bounds__(PK_BLOCK) void k_pack_xyz(
     __shared__ radix::LdsHist s;
     radix::hist_zero(&s);
     __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * (PK_BLOCK * PK_ITEMS);
+    for (uint64_t base = (uint64_t)blockIdx.x * (PK_BLOCK * PK_ITEMS); base < n;
+         base += (uint64_t)gridDim.x * (PK_BLOCK * PK_ITEMS)) {
 #pragma unroll 4
-    for (int j = 0; j < PK_ITEMS; ++j) {
-        const uint64_t i = base + (uint64_t)j * PK_BLOCK + threadIdx.x;
-        if (i < n) {
-            const uint64_t key = ((uint64_t)x[i] << sx) | ((uint64_t)y[i] << sy) |
-                                 (uint64_t)z[i];
-            keys[i] = key;
-            radix::hist_add(&s, key, 0, kbits, npass);
+        for (int j = 0; j < PK_ITEMS; ++j) {
+            const uint64_t i = base + (uint64_t)j * PK_BLOCK + threadIdx.x;
+            if (i < n) {
+                const uint64_t key = ((uint64_t)x[i] << sx) | ((uint64_t)y[i] << sy) |
+                                     (uint64_t)z[i];
+                keys[i] = key;
+                radix::hist_add(&s, key, 0, kbits, npass);
+            }
         }
     }
     __syncthreads();
@@ -188,15 +190,17 @@ __global__ __launch_bounds__(PK_BLOCK) void k_pack_split(
     __shared__ radix::LdsHist s;
     radix::hist_zero(&s);
     __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * (PK_BLOCK * PK_ITEMS);
+    for (uint64_t base = (uint64_t)blockIdx.x * (PK_BLOCK * PK_ITEMS); base < n;
+         base += (uint64_t)gridDim.x * (PK_BLOCK * PK_ITEMS)) {
 #pragma unroll 4
-    for (int j = 0; j < PK_ITEMS; ++j) {
-        const uint64_t i = base + (uint64_t)j * PK_BLOCK + threadIdx.x;
-        if (i < n) {
-            const uint64_t key = ((uint64_t)y[i] << sy) | (uint64_t)z[i];
-            keys[i] = key;
-            vals[i] = x[i];
-            radix::hist_add(&s, key, 0, kbits, npass);
+        for (int j = 0; j < PK_ITEMS; ++j) {
+            const uint64_t i = base + (uint64_t)j * PK_BLOCK + threadIdx.x;
+            if (i < n) {
+                const uint64_t key = ((uint64_t)y[i] << sy) | (uint64_t)z[i];
+                keys[i] = key;
+                vals[i] = x[i];
+                radix::hist_add(&s, key, 0, kbits, npass);
+            }
         }
     }
     __syncthreads();
@@ -210,14 +214,16 @@ __global__ __launch_bounds__(PK_BLOCK) void k_pack_xpos(
     __shared__ radix::LdsHist s;
     radix::hist_zero(&s);
     __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * (PK_BLOCK * PK_ITEMS);
+    for (uint64_t base = (uint64_t)blockIdx.x * (PK_BLOCK * PK_ITEMS); base < n;
+         base += (uint64_t)gridDim.x * (PK_BLOCK * PK_ITEMS)) {
 #pragma unroll 4
-    for (int j = 0; j < PK_ITEMS; ++j) {
-        const uint64_t i = base + (uint64_t)j * PK_BLOCK + threadIdx.x;
-        if (i < n) {
-            const uint64_t key = ((uint64_t)xs[i] << 32) | i;
-            keys[i] = key;
-            radix::hist_add(&s, key, 32, hi_bit, npass);
+        for (int j = 0; j < PK_ITEMS; ++j) {
+            const uint64_t i = base + (uint64_t)j * PK_BLOCK + threadIdx.x;
+            if (i < n) {
+                const uint64_t key = ((uint64_t)xs[i] << 32) | i;
+                keys[i] = key;
+                radix::hist_add(&s, key, 32, hi_bit, npass);
+            }
         }
     }
     __syncthreads();
@@ -476,7 +482,7 @@ void build_perm(vd_ctx *ctx, vd_index *idx, int perm, const uint32_t *cols[3],
         const int npass = radix_passes(0, kbits);
         DBuf<uint32_t> hist(ctx, radix_hist_words(0, kbits));
         fill_u32(ctx, hist.p, 0, radix_hist_words(0, kbits));
-        launch(ctx, "index_pack", (double)n * 20.0, dim3((unsigned)pk_tiles),
+        launch(ctx, "index_pack", (double)n * 20.0, dim3(radix::hist_grid(pk_tiles)),
                dim3(PK_BLOCK), 0, k_pack_xyz, cols[xi], cols[yi], cols[zi], n, sy,
                sx, keys, kbits, npass, hist.p);
         uint64_t *sorted = radix_sort_u64(ctx, keys, tmp, n, 0, kbits, hist.p, true);
@@ -488,7 +494,7 @@ void build_perm(vd_ctx *ctx, vd_index *idx, int perm, const uint32_t *cols[3],
         DBuf<uint32_t> xa(ctx, n), xb(ctx, n);
         DBuf<uint32_t> hist(ctx, radix_hist_words(0, lbits));
         fill_u32(ctx, hist.p, 0, radix_hist_words(0, lbits));
-        launch(ctx, "index_pack", (double)n * 24.0, dim3((unsigned)pk_tiles), dim3(PK_BLOCK), 0,
+        launch(ctx, "index_pack", (double)n * 24.0, dim3(radix::hist_grid(pk_tiles)), dim3(PK_BLOCK), 0,
                k_pack_split, cols[xi], cols[yi], cols[zi], n, sy, keys, xa.p, lbits,
                radix_passes(0, lbits), hist.p);
         uint64_t *lo = keys, *lo_tmp = tmp;
@@ -500,7 +506,7 @@ void build_perm(vd_ctx *ctx, vd_index *idx, int perm, const uint32_t *cols[3],
         const int npass2 = radix_passes(32, hi);
         DBuf<uint32_t> hist2(ctx, radix_hist_words(32, hi));
         fill_u32(ctx, hist2.p, 0, radix_hist_words(32, hi));
-        launch(ctx, "index_pack", (double)n * 12.0, dim3((unsigned)pk_tiles), dim3(PK_BLOCK), 0,
+        launch(ctx, "index_pack", (double)n * 12.0, dim3(radix::hist_grid(pk_tiles)), dim3(PK_BLOCK), 0,
                k_pack_xpos, (const uint32_t *)xv, n, k2.p, hi, npass2, hist2.p);
         uint64_t *s2 = radix_sort_u64(ctx, k2.p, k2t.p, n, 32, hi, hist2.p, true);
         launch(ctx, "index_gather", (double)n * 24.0, dim3(stream_grid(n)), dim3(256), 0,

@@ -13,6 +13,14 @@ constexpr int RBINS = 1 << RBITS;
 constexpr int MAX_PASSES = 8;
 // Threads of every kernel that uses the LDS histogram below.
 constexpr int HIST_BLOCK = 256;
+// Kernels that count histograms while they pack keys run at most this many
+// blocks, each striding over tiles: every block ends with one global atomic
+// per histogram bin, so a grid of one block per tile would issue tens of
+// millions of same-address atomics on large inputs.
+constexpr uint64_t HIST_GRID_MAX = 1024;
+inline unsigned hist_grid(uint64_t ntiles) {
+    return (unsigned)(ntiles < HIST_GRID_MAX ? (ntiles ? ntiles : 1) : HIST_GRID_MAX);
+}
 constexpr int HIST_WAVES = HIST_BLOCK / 64;
 
 // LDS histogram: one copy per wave (fewer same-address atomics across

@@ -33,15 +33,17 @@ __global__ __launch_bounds__(RB) void k_pack_rowid(
     __shared__ radix::LdsHist s;
     radix::hist_zero(&s);
     __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * (RB * R_ITEMS);
+    for (uint64_t base = (uint64_t)blockIdx.x * (RB * R_ITEMS); base < n;
+         base += (uint64_t)gridDim.x * (RB * R_ITEMS)) {
 #pragma unroll 4
-    for (int j = 0; j < R_ITEMS; ++j) {
-        const uint64_t i = base + (uint64_t)j * RB + threadIdx.x;
-        if (i < n) {
-            const uint32_t v = perm ? col[perm[i]] : col[i];
-            const uint64_t key = ((uint64_t)v << 32) | (uint64_t)i;
-            keys[i] = key;
-            radix::hist_add(&s, key, 32, hi_bit, npass);
+        for (int j = 0; j < R_ITEMS; ++j) {
+            const uint64_t i = base + (uint64_t)j * RB + threadIdx.x;
+            if (i < n) {
+                const uint32_t v = perm ? col[perm[i]] : col[i];
+                const uint64_t key = ((uint64_t)v << 32) | (uint64_t)i;
+                keys[i] = key;
+                radix::hist_add(&s, key, 32, hi_bit, npass);
+            }
         }
     }
     __syncthreads();
@@ -87,20 +89,22 @@ __global__ __launch_bounds__(RB) void k_pack_c01(
     if (threadIdx.x == 0) s_max = 0;
     __syncthreads();
     const int w = threadIdx.x / 64;
-    const uint64_t base = (uint64_t)blockIdx.x * (RB * R_ITEMS);
     uint32_t mx = 0;
+    for (uint64_t base = (uint64_t)blockIdx.x * (RB * R_ITEMS); base < n;
+         base += (uint64_t)gridDim.x * (RB * R_ITEMS)) {
 #pragma unroll 4
-    for (int j = 0; j < R_ITEMS; ++j) {
-        const uint64_t i = base + (uint64_t)j * RB + threadIdx.x;
-        if (i < n) {
-            const uint32_t v = c0[i];
-            keys[i] = ((uint64_t)v << 32) | (c1 ? c1[i] : 0u);
-            if (vals) vals[i] = vsrc ? vsrc[i] : (uint32_t)i;
-            mx = max(mx, v);
+        for (int j = 0; j < R_ITEMS; ++j) {
+            const uint64_t i = base + (uint64_t)j * RB + threadIdx.x;
+            if (i < n) {
+                const uint32_t v = c0[i];
+                keys[i] = ((uint64_t)v << 32) | (c1 ? c1[i] : 0u);
+                if (vals) vals[i] = vsrc ? vsrc[i] : (uint32_t)i;
+                mx = max(mx, v);
 #pragma unroll
-            for (int b = 0; b < 4; ++b) count_digit(&s.h8[w][b * 256], (v >> (8 * b)) & 255u);
+                for (int b = 0; b < 4; ++b) count_digit(&s.h8[w][b * 256], (v >> (8 * b)) & 255u);
 #pragma unroll
-            for (int b = 0; b < 3; ++b) count_digit(&s.h9[b * 512], (v >> (9 * b)) & 511u);
+                for (int b = 0; b < 3; ++b) count_digit(&s.h9[b * 512], (v >> (9 * b)) & 511u);
+            }
         }
     }
 #pragma unroll
@@ -197,7 +201,7 @@ uint64_t *sort_by_col(vd_ctx *ctx, const uint32_t *col, const uint32_t *perm,
     DBuf<uint32_t> hist(ctx, radix_hist_words(32, hi));
     fill_u32(ctx, hist.p, 0, radix_hist_words(32, hi));
     launch(ctx, "sort_pack", (double)n * (perm ? 16.0 : 12.0),
-           dim3((unsigned)ceil_div(n, RB * R_ITEMS)), dim3(RB), 0, k_pack_rowid,
+           dim3(radix::hist_grid(ceil_div(n, RB * R_ITEMS))), dim3(RB), 0, k_pack_rowid,
            col, perm, n, keys.p, hi, npass, hist.p);
     return radix_sort_u64(ctx, keys.p, tmp.p, n, 32, hi, hist.p, true);
 }
@@ -233,7 +237,7 @@ void sort_col0(vd_ctx *ctx, vd_rel *rel) {
     fill_u32(ctx, hist.p, 0, SWAP_H8 + SWAP_H9);
     fill_u64(ctx, flags.p, 0, 8);
     launch(ctx, "sort_pack", (double)n * (8.0 + 4.0 * std::min(nc, 2) + (by_row ? 4.0 : 0.0)),
-           dim3((unsigned)ceil_div(n, RB * R_ITEMS)), dim3(RB), 0, k_pack_c01,
+           dim3(radix::hist_grid(ceil_div(n, RB * R_ITEMS))), dim3(RB), 0, k_pack_c01,
            (const uint32_t *)rel->cols[0], (const uint32_t *)(nc >= 2 ? rel->cols[1] : nullptr),
            (const uint32_t *)nullptr, n, keys.p, by_row ? rows.p : nullptr, h8, h9, hmax);
     radix_scan_hist_flags(ctx, h8, 4, n, gofs.p, flags.p, 8);
