@@ -418,6 +418,31 @@ def test_sort_col0_every_tile_shape(ctx, items, ncols):
         ctx.tune("radix_items", 16)
 
 
+def test_sort_col0_views_any_alignment(ctx):
+    """IndexSwap sorts of unfiltered selects, which read the permutation in
+    place: column 0 starts at every offset modulo 16 bytes (the column
+    maximum taken before the pack reads it in 16-byte loads, with a head and
+    a tail), over 1 to thousands of rows."""
+    from vedas_amd._native import Index
+    s, p, o = rand_triples(40_000, 23)
+    idx = Index(ctx, s, p, o, ["PSO"])
+    ix = O.build_perm(s, p, o, "PSO")
+    try:
+        for lo in (1000, 1001, 1002, 1003):
+            for cnt in (1, 3, 6, 777, 9001):
+                print(f"view sort lo={lo} rows={cnt}", flush=True)
+                want = O.select(ix, lo, lo + cnt, 3)
+                r = idx.select("PSO", lo, lo + cnt, 3)
+                assert r.is_view(0)
+                r.swap_cols(0, 1)
+                r.sort_col0()
+                for g, w in zip(r.columns(), O.sort_col0([want[1], want[0]])):
+                    assert np.array_equal(g, w)
+                r.free()
+    finally:
+        idx.free()
+
+
 @pytest.mark.parametrize("ncols", [2, 3, 4, 6])
 def test_sort_col0_digit_plans(ctx, ncols):
     """Column-0 values that only vary in some 9-bit or 8-bit digits: the

@@ -240,6 +240,8 @@ void scan_u64(vd_ctx *ctx, const uint64_t *in, uint64_t *out, uint64_t n,
 void fill_u32(vd_ctx *ctx, uint32_t *p, uint32_t v, uint64_t n);
 void fill_u64(vd_ctx *ctx, uint64_t *p, uint64_t v, uint64_t n);
 uint32_t max_u32(vd_ctx *ctx, const uint32_t *p, uint64_t n);  // syncs
+// atomicMax of p[0, n) into *d_out (which the caller has set), no sync.
+void max_u32_dev(vd_ctx *ctx, const uint32_t *p, uint64_t n, uint32_t *d_out);
 void copy_d2d(vd_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 // Radix sort of 64-bit keys on bits [lo_bit, hi_bit), stable (LSD).

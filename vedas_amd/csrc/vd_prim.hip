@@ -22,14 +22,26 @@ __global__ __launch_bounds__(256) void k_fill(T *p, T v, uint64_t n) {
         p[i] = v;
 }
 
+// 16-byte loads over the aligned body; the < 4 elements before it and the
+// < 4 after it are read one by one.
 __global__ __launch_bounds__(256) void k_max_u32(const uint32_t *p, uint64_t n,
                                                  uint32_t *out) {
     __shared__ uint32_t s[4];
     uint32_t m = 0;
-    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += stride)
-        m = max(m, p[i]);
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t mis = (reinterpret_cast<uintptr_t>(p) & 15u) / 4u;
+    const uint64_t head = min<uint64_t>(n, (4u - mis) & 3u);
+    if (tid < head) m = p[tid];
+    const uint4 *q = reinterpret_cast<const uint4 *>(p + head);
+    const uint64_t nq = (n - head) / 4;
+#pragma unroll 4
+    for (uint64_t i = tid; i < nq; i += stride) {
+        const uint4 v = q[i];
+        m = max(m, max(max(v.x, v.y), max(v.z, v.w)));
+    }
+    const uint64_t t0 = head + nq * 4;
+    if (t0 + tid < n) m = max(m, p[t0 + tid]);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor(m, off, 64));
     if (dev::lane_id() == 0) s[threadIdx.x / 64] = m;
@@ -184,6 +196,11 @@ void fill_u64(vd_ctx *ctx, uint64_t *p, uint64_t v, uint64_t n) {
     if (n == 0) return;
     launch(ctx, "fill", (double)n * 8, dim3(grid_for(n)), dim3(256), 0,
            k_fill<uint64_t>, p, v, n);
+}
+
+void max_u32_dev(vd_ctx *ctx, const uint32_t *p, uint64_t n, uint32_t *d_out) {
+    launch(ctx, "max_u32", (double)n * 4, dim3(grid_for(n) > 2048 ? 2048 : grid_for(n)),
+           dim3(256), 0, k_max_u32, p, n, d_out);
 }
 
 uint32_t max_u32(vd_ctx *ctx, const uint32_t *p, uint64_t n) {

@@ -65,31 +65,45 @@ __device__ __forceinline__ void count_digit(uint32_t *bins, uint32_t d) {
     }
 }
 
-// Histograms of column 0 for the two digit plans of an IndexSwap sort:
-// its four bytes (8-bit digits) and the three 9-bit digits of bits 0..26.
+// Digit plan of an IndexSwap sort from column 0's maximum mx: digits below
+// its bit length can vary, the ones above are 0 in every row.  9-bit digits
+// (bits 0..26) when they need fewer passes than bytes, else 8-bit digits.
+struct SwapPlan {
+    bool use9;
+    int ndig;  // digits that can vary
+};
+__host__ __device__ inline SwapPlan swap_plan(uint32_t mx) {
+    int nb = 0;
+    while (nb < 32 && (mx >> nb) != 0) ++nb;
+    const int n8 = (nb + 7) / 8, n9 = (nb + 8) / 9;
+    const bool use9 = nb <= 27 && n9 < n8;
+    return SwapPlan{use9, use9 ? n9 : n8};
+}
+
+// Histograms of column 0 for the plan: 4 x 256 words (8-bit digits) or
+// 3 x 512 (9-bit digits), one copy per wave in LDS.
 constexpr int SWAP_H8 = 4 * 256;
 constexpr int SWAP_H9 = 3 * 512;
-struct SwapHist {
-    uint32_t h8[radix::HIST_WAVES][SWAP_H8];  // one copy per wave
-    uint32_t h9[SWAP_H9];                     // one copy per block (LDS)
-};
+constexpr int SWAP_HW = SWAP_H9 > SWAP_H8 ? SWAP_H9 : SWAP_H8;
 
 // key = col0[i] << 32 | col1[i] (no col1: 0); with vals, vals[i] = vsrc[i]
-// (vsrc null: the row id i).  Counts both digit plans' histograms (g8:
-// 4 x 256, g9: 3 x 512 words) and the maximum of column 0 (*gmax).
+// (vsrc null: the row id i).  Counts the histograms of the digits that can
+// vary under the plan of *gmax (column 0's maximum, computed before) into
+// g8 (8-bit plan) or g9 (9-bit plan).  The top digit counts through
+// count_digit (its values are the most concentrated), the others with plain
+// LDS atomics.
 __global__ __launch_bounds__(RB) void k_pack_c01(
     const uint32_t *__restrict__ c0, const uint32_t *__restrict__ c1,
     const uint32_t *__restrict__ vsrc, uint64_t n, uint64_t *__restrict__ keys,
-    uint32_t *__restrict__ vals, uint32_t *g8, uint32_t *g9, uint32_t *gmax) {
-    __shared__ SwapHist s;
-    __shared__ uint32_t s_max;
+    uint32_t *__restrict__ vals, uint32_t *g8, uint32_t *g9, const uint32_t *gmax) {
+    __shared__ uint32_t s_h[radix::HIST_WAVES][SWAP_HW];
     static_assert(RB == radix::HIST_BLOCK, "one histogram copy per wave");
-    uint32_t *z = &s.h8[0][0];
-    for (int i = threadIdx.x; i < (int)(sizeof(SwapHist) / 4); i += RB) z[i] = 0;
-    if (threadIdx.x == 0) s_max = 0;
+    for (int i = threadIdx.x; i < radix::HIST_WAVES * SWAP_HW; i += RB) (&s_h[0][0])[i] = 0;
+    const SwapPlan plan = swap_plan(*gmax);
+    const int W = plan.use9 ? 9 : 8, bins = 1 << W, nd = plan.ndig;
+    const uint32_t mask = (uint32_t)bins - 1u;
     __syncthreads();
-    const int w = threadIdx.x / 64;
-    uint32_t mx = 0;
+    uint32_t *h = s_h[threadIdx.x / 64];
     for (uint64_t base = (uint64_t)blockIdx.x * (RB * R_ITEMS); base < n;
          base += (uint64_t)gridDim.x * (RB * R_ITEMS)) {
 #pragma unroll 4
@@ -99,29 +113,20 @@ __global__ __launch_bounds__(RB) void k_pack_c01(
                 const uint32_t v = c0[i];
                 keys[i] = ((uint64_t)v << 32) | (c1 ? c1[i] : 0u);
                 if (vals) vals[i] = vsrc ? vsrc[i] : (uint32_t)i;
-                mx = max(mx, v);
-#pragma unroll
-                for (int b = 0; b < 4; ++b) count_digit(&s.h8[w][b * 256], (v >> (8 * b)) & 255u);
-#pragma unroll
-                for (int b = 0; b < 3; ++b) count_digit(&s.h9[b * 512], (v >> (9 * b)) & 511u);
+                for (int d = 0; d + 1 < nd; ++d)
+                    atomicAdd(&h[d * bins + ((v >> (W * d)) & mask)], 1u);
+                if (nd > 0) count_digit(&h[(nd - 1) * bins], (v >> (W * (nd - 1))) & mask);
             }
         }
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, off, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(&s_max, mx);
     __syncthreads();
-    for (int i = threadIdx.x; i < SWAP_H8; i += RB) {
+    uint32_t *g = plan.use9 ? g9 : g8;
+    for (int i = threadIdx.x; i < nd * bins; i += RB) {
         uint32_t c = 0;
 #pragma unroll
-        for (int ww = 0; ww < radix::HIST_WAVES; ++ww) c += s.h8[ww][i];
-        if (c) atomicAdd(&g8[i], c);
+        for (int ww = 0; ww < radix::HIST_WAVES; ++ww) c += s_h[ww][i];
+        if (c) atomicAdd(&g[i], c);
     }
-    for (int i = threadIdx.x; i < SWAP_H9; i += RB) {
-        const uint32_t c = s.h9[i];
-        if (c) atomicAdd(&g9[i], c);
-    }
-    if (threadIdx.x == 0) atomicMax(gmax, s_max);
 }
 
 // out0 = key >> 32, out1 = low word of the key; with rows, out_c =
@@ -226,42 +231,43 @@ void sort_col0(vd_ctx *ctx, vd_rel *rel) {
     const int m = nc <= 2 ? 0 : (by_row ? 1 : nc - 2);
     DBuf<uint64_t> keys(ctx, n), ktmp(ctx, n);
     DBuf<uint32_t> rows(ctx, by_row ? n : 0);
-    // Histograms of both digit plans in one buffer [8-bit: 4 x 256 | 9-bit:
+    // Histograms of either digit plan in one buffer [8-bit: 4 x 256 | 9-bit:
     // 3 x 512], their offsets, and the words read back: "varies" flags of
-    // the four bytes and of the three 9-bit digits, then column 0's maximum
-    // (counted by the pack kernel into the low half of the last word).
+    // the four bytes and of the three 9-bit digits (meaningful for the
+    // plan's counted digits), then column 0's maximum (in the low half of
+    // the last word, computed first: it selects the plan).
     DBuf<uint32_t> hist(ctx, SWAP_H8 + SWAP_H9), gofs(ctx, SWAP_H8 + SWAP_H9);
     DBuf<uint64_t> flags(ctx, 8);
     uint32_t *h8 = hist.p, *h9 = hist.p + SWAP_H8;
     uint32_t *hmax = reinterpret_cast<uint32_t *>(flags.p + 7);
     fill_u32(ctx, hist.p, 0, SWAP_H8 + SWAP_H9);
     fill_u64(ctx, flags.p, 0, 8);
+    max_u32_dev(ctx, rel->cols[0], n, hmax);  // the digit plan of the pack
     launch(ctx, "sort_pack", (double)n * (8.0 + 4.0 * std::min(nc, 2) + (by_row ? 4.0 : 0.0)),
            dim3(radix::hist_grid(ceil_div(n, RB * R_ITEMS))), dim3(RB), 0, k_pack_c01,
            (const uint32_t *)rel->cols[0], (const uint32_t *)(nc >= 2 ? rel->cols[1] : nullptr),
-           (const uint32_t *)nullptr, n, keys.p, by_row ? rows.p : nullptr, h8, h9, hmax);
+           (const uint32_t *)nullptr, n, keys.p, by_row ? rows.p : nullptr, h8, h9,
+           (const uint32_t *)hmax);
     radix_scan_hist_flags(ctx, h8, 4, n, gofs.p, flags.p, 8);
     radix_scan_hist_flags(ctx, h9, 3, n, gofs.p + SWAP_H8, flags.p + 4, 9);
     uint64_t hv[8];
     read_words(ctx, hv, flags.p, 8);
-    // Passes: the digits that vary.  9-bit digits when they need fewer
-    // passes (values below 2^27 whose top byte varies: 3 passes, not 4).
-    int n8 = 0, n9 = 0;
-    for (int d = 0; d < 4; ++d) n8 += hv[d] ? 1 : 0;
-    for (int d = 0; d < 3; ++d) n9 += hv[4 + d] ? 1 : 0;
-    const bool use9 = hv[7] < (1ull << 27) && n9 < n8;
+    // Passes: the digits of the plan that vary (the pack counted only the
+    // plan's digits below the maximum's bit length).
+    const SwapPlan plan = swap_plan((uint32_t)hv[7]);
+    const bool use9 = plan.use9;
     int shifts[4];
     const uint32_t *g[4];
     int npass = 0;
     if (use9) {
-        for (int d = 0; d < 3; ++d)
+        for (int d = 0; d < plan.ndig; ++d)
             if (hv[4 + d]) {
                 shifts[npass] = 32 + 9 * d;
                 g[npass] = gofs.p + SWAP_H8 + d * 512;
                 ++npass;
             }
     } else {
-        for (int d = 0; d < 4; ++d)
+        for (int d = 0; d < plan.ndig; ++d)
             if (hv[d]) {
                 shifts[npass] = 32 + 8 * d;
                 g[npass] = gofs.p + d * 256;
