@@ -149,7 +149,9 @@ void rb_wait(vd_ctx *ctx, const Readback &rb, uint64_t *dst, int n) {
     const uint64_t *word = ctx->h_rb + RB_WORDS;
     const auto t0 = std::chrono::steady_clock::now();
     for (uint32_t spin = 1; __atomic_load_n(word, __ATOMIC_ACQUIRE) != rb.seq; ++spin) {
+#if defined(__x86_64__)
         __builtin_ia32_pause();
+#endif
         if ((spin & 1023) != 0) continue;
         const double us = std::chrono::duration<double, std::micro>(
                               std::chrono::steady_clock::now() - t0).count();
