@@ -113,7 +113,7 @@ template <int NK>
 __global__ __launch_bounds__(JB_BLOCK) void k_join_bounds(
     KeyCols L, uint64_t nl, KeyCols R, uint64_t nr, int rcap,
     const uint64_t *__restrict__ ranges, uint32_t *__restrict__ lower,
-    uint64_t *__restrict__ loc, uint64_t *__restrict__ tile_total,
+    uint32_t *__restrict__ loc, uint64_t *__restrict__ tile_total,
     uint64_t *__restrict__ tile_keys) {
     using T = typename KeyT<NK>::T;
     constexpr int SR_WORDS = JB_RCAP_BYTES / sizeof(T);
@@ -218,12 +218,20 @@ __global__ __launch_bounds__(JB_BLOCK) void k_join_bounds(
 #pragma unroll
     for (int j = 0; j < JB_ITEMS; ++j) {
         const uint64_t i = base + (uint64_t)j * JB_BLOCK + threadIdx.x;
-        if (i < nl) loc[i] = s_off[j * JB_BLOCK + threadIdx.x];
+        if (i < nl) loc[i] = (uint32_t)s_off[j * JB_BLOCK + threadIdx.x];
     }
 }
 
 static_assert(JB_TILE == OFF_TILE, "bounds tiles are the offset tiles");
-using JoinOff = TileOff;
+// Output offset of left row i: the tile's prefix plus the 32-bit offset
+// within its tile (a tile's total is checked to fit 32 bits).
+struct JoinOff {
+    const uint32_t *loc;
+    const uint64_t *tpre;
+    __device__ __forceinline__ uint64_t operator()(uint64_t i) const {
+        return tpre[i / OFF_TILE] + loc[i];
+    }
+};
 
 // mp[b] = number of left rows i with i + off[i] < b * nv (clamped to the
 // end of the diagonal), for b = 0 .. nblocks: a group of lanes per b.
@@ -248,18 +256,22 @@ __global__ __launch_bounds__(dev::TT_BLOCK) void k_join_scan_total(
     __shared__ uint64_t s_tmp[dev::TT_BLOCK / 64];
     dev::scan_tile_totals_block(ttot, ntiles, tpre, s_tmp);
     uint32_t mn = ~0u, mx = 0;
+    uint64_t wide = 0;  // high halves of the tile totals
     for (uint64_t t = threadIdx.x; t < ntiles; t += dev::TT_BLOCK) {
         const uint64_t w = tkeys[t];
         mn = min(mn, (uint32_t)w);
         mx = max(mx, (uint32_t)(w >> 32));
+        wide |= ttot[t] >> 32;
     }
     dev::block_min_max<dev::TT_BLOCK>(mn, mx, s_tmp);
+    const int overflow = __syncthreads_or(wide != 0);
     if (threadIdx.x >= 64) return;
     if (threadIdx.x == 0) {
         const uint64_t total = tpre[ntiles];  // written by this thread
         rb.dst[0] = total;
         rb.dst[1] = total ? mn : 0;
         rb.dst[2] = total ? mx : 0;
+        rb.dst[3] = overflow ? 1 : 0;  // a tile's offsets do not fit 32 bits
     }
     dev::rb_publish(rb);
 }
@@ -377,7 +389,7 @@ vd_rel *join_impl(vd_ctx *ctx, const vd_rel *L, const vd_rel *R) {
     KeyCols lk{L->cols[0], NK == 2 ? L->cols[1] : nullptr};
     KeyCols rk{R->cols[0], NK == 2 ? R->cols[1] : nullptr};
     DBuf<uint32_t> lower(ctx, nl);
-    DBuf<uint64_t> off(ctx, nl);  // offsets within each bounds tile
+    DBuf<uint32_t> off(ctx, nl);  // offsets within each bounds tile
     const uint64_t jb_tiles = ceil_div(nl, JB_TILE);
     const int rcap = std::min<int>(ctx->join_rcap, JB_RCAP_BYTES / (NK == 2 ? 8 : 4));
     // ranges (2 per tile), tile totals (1 per tile), tile prefixes (1 per
@@ -387,18 +399,20 @@ vd_rel *join_impl(vd_ctx *ctx, const vd_rel *L, const vd_rel *R) {
     uint64_t *tkeys = ranges.p + 4 * jb_tiles + 1;
     launch(ctx, "join_ranges", 0.0, dim3((unsigned)ceil_div(jb_tiles * JR_GROUP, 256)),
            dim3(256), 0, k_join_ranges<NK>, lk, nl, rk, nr, jb_tiles, ranges.p);
-    launch(ctx, "join_bounds", 4.0 * NK * (double)(nl + nr) + 12.0 * nl,
+    launch(ctx, "join_bounds", 4.0 * NK * (double)(nl + nr) + 8.0 * nl,
            dim3((unsigned)jb_tiles), dim3(JB_BLOCK), 0, k_join_bounds<NK>, lk,
            nl, rk, nr, rcap, (const uint64_t *)ranges.p, lower.p, off.p, ttot, tkeys);
-    const JoinOff joff{off.p, tpre};
+    const JoinOff joff{(const uint32_t *)off.p, tpre};
     const Readback rb = rb_next(ctx);
     launch(ctx, "scan_tiles", 16.0 * (double)jb_tiles, dim3(1), dim3(dev::TT_BLOCK), 0,
            k_join_scan_total, (const uint64_t *)ttot, (const uint64_t *)tkeys, jb_tiles, tpre,
            rb);
-    uint64_t h[3];
-    rb_wait(ctx, rb, h, 3);
+    uint64_t h[4];
+    rb_wait(ctx, rb, h, 4);
     const uint64_t total = h[0];
     VD_REQUIRE(total < (1ull << 40), VD_ERANGE, "join output too large");
+    VD_REQUIRE(h[3] == 0, VD_ERANGE,
+               "join: 2048 consecutive left rows match more than 2^32-1 right rows");
 
     vd_rel *out = rel_new(ctx, out_cols, total);
     out->key_range = true;  // output column 0 = left keys of matched rows
@@ -420,7 +434,7 @@ vd_rel *join_impl(vd_ctx *ctx, const vd_rel *L, const vd_rel *R) {
                dim3((unsigned)ceil_div((nblocks + 1) * JR_GROUP, 256)),
                dim3(256), 0, k_join_partition, joff, nl, total,
                nv, nblocks, mp.p);
-        const double xbytes = 4.0 * (double)total * out_cols + 12.0 * nl +
+        const double xbytes = 4.0 * (double)total * out_cols + 8.0 * nl +
                               4.0 * (double)nl * (L->ncols) + 4.0 * (double)nr * (R->ncols - NK);
         auto go = [&](auto kern) {
             launch(ctx, "join_expand", xbytes, dim3((unsigned)nblocks), dim3(JX_BLOCK),
