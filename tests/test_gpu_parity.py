@@ -308,8 +308,46 @@ def test_join_parity(ctx, nk, shape):
     assert out.nrows == len(want[0])
     for c, (g, w) in enumerate(zip(got, want)):
         assert np.array_equal(g, w), c
+    if out.nrows:  # first and last output key, read back with the size
+        assert out.key_range() == (int(want[0][0]), int(want[0][-1]))
     for r in (gl, gr, out):
         r.free()
+
+
+@pytest.mark.parametrize("only", [None, ["join_expand", "radix_onesweep_kv"]])
+def test_operators_under_profiling(ctx, only):
+    """Per-launch timing on (all launches, or some labels): the size
+    read-backs return while timed launches are still pending, and the
+    results and the launch counts are unchanged."""
+    from vedas_amd._native import Rel
+    rng = np.random.default_rng(77)
+    L = make_rel(rng, 60_000, 3, 5000)
+    R = make_rel(rng, 50_000, 2, 5000)
+    want = O.join(L, R, 1)
+    ctx.prof(True)
+    ctx.prof_only(only)
+    ctx.prof_reset()
+    try:
+        for _ in range(2):
+            gl, gr = Rel.from_columns(ctx, L), Rel.from_columns(ctx, R)
+            out = gl.join(gr, 1)
+            out.swap_cols(0, 1)
+            out.sort_col0()
+            got = out.columns()
+            for g, w in zip(got, O.sort_col0([want[1], want[0]] + want[2:])):
+                assert np.array_equal(g, w)
+            for r in (gl, gr, out):
+                r.free()
+        ctx.sync()
+        st = ctx.prof_stats()
+        assert st["join_expand"]["launches"] == 2 and st["radix_onesweep_kv"]["launches"] >= 2
+        if only:
+            assert set(st) <= set(only)
+        else:
+            assert st["join_bounds"]["launches"] == 2 and "read_back" in st
+    finally:
+        ctx.prof(False)
+        ctx.prof_only(None)
 
 
 @pytest.mark.parametrize("nv,rcap", [(1024, 2048), (2048, 4096), (4096, 8192)])
