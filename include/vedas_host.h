@@ -120,12 +120,17 @@ int vdq_batch(vdq_engine *e, const char *const *texts, int n, vdq_result **outs,
  * peak memory fits mem_budget, and run in continuous batches on `slots`
  * concurrent streams.  plain != 0 runs each request alone through the
  * single-query path on one stream (the baseline).  recs[i] reports
- * request i; outs (may be NULL) receives the results. */
+ * request i; outs (may be NULL) receives the results.  Fill policy: while
+ * two or more batches are running, a free slot starts the next one only
+ * once min_fill requests are queued or the oldest has waited max_wait_ms
+ * (min_fill 0: start at once); otherwise it starts at once. */
 typedef struct {
     int slots;
     int max_batch;
     uint64_t mem_budget;
     int plain;
+    int min_fill;
+    double max_wait_ms;
 } vdq_serve_cfg_t;
 typedef struct {
     double arrival_ms, start_ms, done_ms;

@@ -35,12 +35,16 @@ def arrivals(n, qps, seed):
     return list(np.cumsum(rng.exponential(1000.0 / qps, n)))
 
 
-@pytest.mark.parametrize("slots,max_batch,budget", [(3, 8, 64 << 30), (2, 32, 64 << 30),
-                                                    (4, 4, 1 << 20)])
-def test_served_results_equal_alone(served, slots, max_batch, budget):
+@pytest.mark.parametrize("slots,max_batch,budget,fill", [(3, 8, 64 << 30, 0), (2, 32, 64 << 30, 0),
+                                                         (4, 4, 1 << 20, 0), (4, 16, 64 << 30, 12)])
+def test_served_results_equal_alone(served, slots, max_batch, budget, fill):
+    """fill > 0: the fill policy holds small batches back (up to 1 ms)
+    while two batches run; every request still completes, with the same
+    result."""
     e, texts, alone = served
-    recs, res = e.serve(texts, arrivals(len(texts), 2000.0, slots), slots=slots,
-                        max_batch=max_batch, mem_budget=budget, keep_results=True)
+    recs, res = e.serve(texts, arrivals(len(texts), 20000.0 if fill else 2000.0, slots),
+                        slots=slots, max_batch=max_batch, mem_budget=budget, keep_results=True,
+                        min_fill=fill, max_wait_ms=1.0 if fill else 0.0)
     got = [O.canonical(r.headers(), r.columns()) for r in res]
     for r in res:
         r.free()

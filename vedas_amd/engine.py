@@ -29,7 +29,7 @@ class BatchStats(C.Structure):
 
 class ServeCfg(C.Structure):
     _fields_ = [("slots", C.c_int), ("max_batch", C.c_int), ("mem_budget", C.c_uint64),
-                ("plain", C.c_int)]
+                ("plain", C.c_int), ("min_fill", C.c_int), ("max_wait_ms", C.c_double)]
 
 
 class ServeRec(C.Structure):
@@ -210,14 +210,15 @@ class Engine:
                 {k: getattr(st, k) for k, _ in BatchStats._fields_})
 
     def serve(self, texts, arrival_ms, slots=4, max_batch=64, mem_budget=64 << 30,
-              plain=False, keep_results=False):
-        """Serve an open-loop arrival stream; returns (records, [Result] or None)."""
+              plain=False, keep_results=False, min_fill=0, max_wait_ms=0.0):
+        """Serve an open-loop arrival stream; returns (records, [Result] or None).
+        min_fill / max_wait_ms: the fill policy (see vdq_serve_cfg_t)."""
         n = len(texts)
         arr = (C.c_char_p * max(1, n))(*[t.encode() for t in texts])
         arrival = (C.c_double * max(1, n))(*arrival_ms)
         recs = (ServeRec * max(1, n))()
         outs = (C.c_void_p * max(1, n))() if keep_results else None
-        cfg = ServeCfg(slots, max_batch, mem_budget, int(plain))
+        cfg = ServeCfg(slots, max_batch, mem_budget, int(plain), min_fill, max_wait_ms)
         _check(lib().vdq_serve(self.h, arr, arrival, n, C.byref(cfg), recs, outs))
         out = [{k: getattr(recs[i], k) for k, _ in ServeRec._fields_} for i in range(n)]
         res = [Result(C.c_void_p(outs[i])) for i in range(n)] if keep_results else None

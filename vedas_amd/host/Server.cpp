@@ -1,5 +1,6 @@
 #include "Server.h"
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -135,9 +136,20 @@ void Server::run(std::vector<ServeRequest> &reqs, std::vector<ServeRecord> &recs
         while (finished < n && failure.empty()) {
             const double t = now_ms();
             while (next < order.size() && reqs[order[next]].arrival_ms <= t) queue.push_back(order[next++]);
-            bool dispatched = false;
+            bool dispatched = false, held = false;
+            int busy = 0;
+            for (int si = 0; si < nslots; ++si) busy += slots[si].busy ? 1 : 0;
             for (int si = 0; si < nslots && !queue.empty(); ++si) {
                 if (slots[si].busy) continue;
+                // Fill policy: with two batches already on the GPU (a load
+                // that keeps it busy), a small queue waits to become a fuller
+                // batch, for up to max_wait_ms.
+                if (!cfg.plain && cfg.min_fill > 0 && busy >= 2 &&
+                    (int)queue.size() < cfg.min_fill &&
+                    t - reqs[queue.front()].arrival_ms < cfg.max_wait_ms) {
+                    held = true;
+                    break;
+                }
                 std::vector<int> batch;
                 double add = 0;
                 while (!queue.empty() && (int)batch.size() < max_batch) {
@@ -156,14 +168,20 @@ void Server::run(std::vector<ServeRequest> &reqs, std::vector<ServeRecord> &recs
                 slots[si].busy = true;
                 slots[si].batch_id = batch_id++;
                 dispatched = true;
+                ++busy;
             }
             if (dispatched) {
                 cv_work.notify_all();
                 continue;
             }
-            // wait for a completion or the next arrival
-            if (next < order.size()) {
-                const double dt = reqs[order[next]].arrival_ms - now_ms();
+            // wait for a completion, the next arrival or the end of a hold
+            double wake = next < order.size() ? reqs[order[next]].arrival_ms : -1.0;
+            if (held) {
+                const double hold_end = reqs[queue.front()].arrival_ms + cfg.max_wait_ms;
+                wake = wake < 0 ? hold_end : std::min(wake, hold_end);
+            }
+            if (wake >= 0) {
+                const double dt = wake - now_ms();
                 if (dt > 0)
                     cv_done.wait_for(lk, std::chrono::microseconds((long long)(dt * 1000.0) + 1));
             } else {

@@ -37,6 +37,11 @@ struct ServeConfig {
     uint64_t mem_budget = 64ull << 30;
     bool plain = false;
     bool keep_results = false;
+    // While two or more batches run, a free slot waits for min_fill queued
+    // requests or for the oldest to have waited max_wait_ms (min_fill 0: no
+    // waiting).
+    int min_fill = 0;
+    double max_wait_ms = 0.0;
 };
 
 struct ServeRecord {

@@ -487,6 +487,8 @@ int vdq_serve(vdq_engine *e, const char *const *texts, const double *arrival_ms,
         sc.max_batch = cfg->max_batch;
         sc.mem_budget = cfg->mem_budget;
         sc.plain = cfg->plain != 0;
+        sc.min_fill = cfg->min_fill;
+        sc.max_wait_ms = cfg->max_wait_ms;
         sc.keep_results = outs != nullptr;
         Statistics &st = e->statistics();
         Server srv(e->storage.get(), &st, sc, e->slots(sc.plain ? 1 : std::max(1, sc.slots)));
