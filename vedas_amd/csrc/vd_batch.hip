@@ -78,20 +78,21 @@ constexpr int MR_GROUP = 16;
 __global__ __launch_bounds__(256) void k_join_ranges_multi(const JPairDev *P, int np,
                                                            const uint64_t *tbase, uint64_t ntiles,
                                                            uint64_t *ranges) {
-    const uint64_t b = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / MR_GROUP;
+    const uint64_t g = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / MR_GROUP;
+    const uint64_t b = g / 2;
     if (b >= ntiles) return;  // whole groups leave together
     const int j = find_seg(tbase, np, b);
     const JPairDev &D = P[j];
     const uint32_t *lk = D.lc[0], *rk = D.rc[0];
     const uint64_t base = (b - tbase[j]) * (uint64_t)MB_TILE;
-    const uint64_t last = min(base + MB_TILE, D.nl) - 1;
-    const uint32_t kfirst = lk[base], klast = lk[last];
-    const uint64_t lo = dev::group_bound<MR_GROUP>(0, D.nr, [&](uint64_t q) { return rk[q] >= kfirst; });
-    const uint64_t hi = dev::group_bound<MR_GROUP>(lo, D.nr, [&](uint64_t q) { return rk[q] > klast; });
-    if (threadIdx.x % MR_GROUP == 0) {
-        ranges[2 * b] = lo;
-        ranges[2 * b + 1] = hi;
-    }
+    // even group: first right row >= the tile's first key; odd group: first
+    // right row > its last key (one code path, so both run at once)
+    const bool upper = g % 2 != 0;
+    const uint32_t key = lk[upper ? min(base + MB_TILE, D.nl) - 1 : base];
+    const uint64_t end = dev::group_bound<MR_GROUP>(0, D.nr, [&](uint64_t q) {
+        return upper ? rk[q] > key : rk[q] >= key;
+    });
+    if (threadIdx.x % MR_GROUP == 0) ranges[2 * b + (upper ? 1 : 0)] = end;
 }
 
 // As k_join_bounds (vd_join.hip): the right range, or a directory of every
@@ -513,7 +514,7 @@ void join_multi(vd_ctx *ctx, const vd_rel *const *L, const vd_rel *const *R, int
         double bbytes = 0;
         for (int j = 0; j < np; ++j) bbytes += 4.0 * (hp[j].nl + hp[j].nr) + 8.0 * hp[j].nl;
         DBuf<uint64_t> ranges(ctx, 2 * tb[np]);
-        launch(ctx, "join_ranges", 0.0, dim3((unsigned)ceil_div(tb[np] * MR_GROUP, 256)),
+        launch(ctx, "join_ranges", 0.0, dim3((unsigned)ceil_div(tb[np] * 2 * MR_GROUP, 256)),
                dim3(256), 0, k_join_ranges_multi, (const JPairDev *)dp.p, np, (const uint64_t *)dtb.p, tb[np],
                ranges.p);
         launch(ctx, "join_bounds", bbytes, dim3((unsigned)tb[np]), dim3(MB_BLOCK), 0,

@@ -81,28 +81,29 @@ __device__ __forceinline__ uint64_t g_upper(const KeyCols &k, uint64_t lo,
     return lo;
 }
 
-// Right-row range [lo, hi) each bounds block can touch: a group of 16 lanes
-// per block, all searches in parallel (a serial search inside every block
-// would put ~2 log2(|R|) dependent global loads in front of its work).
+// Right-row range [lo, hi) each bounds block can touch: two groups of 16
+// lanes per block, one for each end, all searches in parallel (a serial
+// search inside every block would put ~2 log2(|R|) dependent global loads
+// in front of its work).
 constexpr int JR_GROUP = 16;
 template <int NK>
 __global__ __launch_bounds__(256) void k_join_ranges(KeyCols L, uint64_t nl, KeyCols R,
                                                      uint64_t nr, uint64_t nblocks,
                                                      uint64_t *__restrict__ ranges) {
     using T = typename KeyT<NK>::T;
-    const uint64_t b = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / JR_GROUP;
+    const uint64_t g = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / JR_GROUP;
+    const uint64_t b = g / 2;
     if (b >= nblocks) return;  // whole groups leave together
     const uint64_t base = b * JB_TILE;
-    const uint64_t last = min(base + JB_TILE, nl) - 1;
-    const T kfirst = KeyT<NK>::at(L, base), klast = KeyT<NK>::at(L, last);
-    const uint64_t lo =
-        dev::group_bound<JR_GROUP>(0, nr, [&](uint64_t q) { return !(KeyT<NK>::at(R, q) < kfirst); });
-    const uint64_t hi =
-        dev::group_bound<JR_GROUP>(lo, nr, [&](uint64_t q) { return klast < KeyT<NK>::at(R, q); });
-    if (threadIdx.x % JR_GROUP == 0) {
-        ranges[2 * b] = lo;
-        ranges[2 * b + 1] = hi;
-    }
+    // even group: first right row >= the tile's first key; odd group: first
+    // right row > its last key (one code path, so both run at once)
+    const bool upper = g % 2 != 0;
+    const T key = KeyT<NK>::at(L, upper ? min(base + JB_TILE, nl) - 1 : base);
+    const uint64_t end = dev::group_bound<JR_GROUP>(0, nr, [&](uint64_t q) {
+        const T r = KeyT<NK>::at(R, q);
+        return upper ? key < r : !(r < key);
+    });
+    if (threadIdx.x % JR_GROUP == 0) ranges[2 * b + (upper ? 1 : 0)] = end;
 }
 
 // The right range of a tile goes to LDS whole when it has at most rcap
@@ -397,7 +398,7 @@ vd_rel *join_impl(vd_ctx *ctx, const vd_rel *L, const vd_rel *R) {
     DBuf<uint64_t> ranges(ctx, 5 * jb_tiles + 1);
     uint64_t *ttot = ranges.p + 2 * jb_tiles, *tpre = ranges.p + 3 * jb_tiles;
     uint64_t *tkeys = ranges.p + 4 * jb_tiles + 1;
-    launch(ctx, "join_ranges", 0.0, dim3((unsigned)ceil_div(jb_tiles * JR_GROUP, 256)),
+    launch(ctx, "join_ranges", 0.0, dim3((unsigned)ceil_div(jb_tiles * 2 * JR_GROUP, 256)),
            dim3(256), 0, k_join_ranges<NK>, lk, nl, rk, nr, jb_tiles, ranges.p);
     launch(ctx, "join_bounds", 4.0 * NK * (double)(nl + nr) + 8.0 * nl,
            dim3((unsigned)jb_tiles), dim3(JB_BLOCK), 0, k_join_bounds<NK>, lk,
