@@ -344,7 +344,7 @@ def test_operators_under_profiling(ctx, only):
         if only:
             assert set(st) <= set(only)
         else:
-            assert st["join_bounds"]["launches"] == 2 and "read_back" in st
+            assert st["join_bounds"]["launches"] == 2 and st["scan_tiles"]["launches"] == 2
     finally:
         ctx.prof(False)
         ctx.prof_only(None)

@@ -283,11 +283,6 @@ uint64_t *radix_sort_shifts(vd_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t
                             const int *shifts, int npass, const uint32_t *const *d_gofs, int m,
                             const RadixCols &src, const RadixCols &pa, const RadixCols &pb,
                             RadixCols *final, int digit_bits = 8);
-// Exclusive scans of npos digit histograms of 2^digit_bits bins (d_hist ->
-// d_gofs) and d_varies[i] = 0 when one bin of position i holds all n keys,
-// else 1.
-void radix_scan_hist_flags(vd_ctx *ctx, const uint32_t *d_hist, int npos, uint64_t n,
-                           uint32_t *d_gofs, uint64_t *d_varies, int digit_bits = 8);
 int radix_passes(int lo_bit, int hi_bit);
 // Device histogram buffer size (uint32 words) needed for radix_sort_u64.
 inline size_t radix_hist_words(int lo_bit, int hi_bit) {

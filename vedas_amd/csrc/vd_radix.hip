@@ -65,21 +65,6 @@ __global__ __launch_bounds__(RBINS) void k_rs_scan_hist(const uint32_t *ghist,
     gofs[blockIdx.x * RBINS + threadIdx.x] = ex;
 }
 
-// Exclusive scan of each digit position's histogram (one block per
-// position) and a flag per position: 1 when the digit varies (no bin holds
-// all n rows), 0 when every row has the same digit there.
-template <int BINS>
-__global__ __launch_bounds__(BINS) void k_rs_scan_hist_flags(const uint32_t *ghist, uint64_t n,
-                                                             uint32_t *gofs, uint64_t *varies) {
-    __shared__ uint32_t s_tmp[BINS / 64];
-    const uint32_t v = ghist[blockIdx.x * BINS + threadIdx.x];
-    uint32_t tot;
-    const uint32_t ex = dev::block_excl_scan<BINS>(v, s_tmp, &tot);
-    gofs[blockIdx.x * BINS + threadIdx.x] = ex;
-    const int all_one_bin = __syncthreads_or((uint64_t)v == n);
-    if (threadIdx.x == 0) varies[blockIdx.x] = all_one_bin ? 0 : 1;
-}
-
 // Look-back window: predecessor tiles read per round trip.  The published
 // values cross XCDs (agent scope), so each round trip is long; reading two
 // tiles at once halves the trips, while wider windows cost more in extra
@@ -460,15 +445,5 @@ uint64_t *radix_sort_shifts(vd_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t
     return a;
 }
 
-void radix_scan_hist_flags(vd_ctx *ctx, const uint32_t *d_hist, int npos, uint64_t n,
-                           uint32_t *d_gofs, uint64_t *d_varies, int digit_bits) {
-    VD_REQUIRE(digit_bits == 8 || digit_bits == 9, VD_EINVAL, "radix digit width");
-    if (digit_bits == 9)
-        launch(ctx, "radix_scan_hist", 0.0, dim3(npos), dim3(512), 0, k_rs_scan_hist_flags<512>,
-               d_hist, n, d_gofs, d_varies);
-    else
-        launch(ctx, "radix_scan_hist", 0.0, dim3(npos), dim3(RBINS), 0,
-               k_rs_scan_hist_flags<RBINS>, d_hist, n, d_gofs, d_varies);
-}
 
 }  // namespace vd

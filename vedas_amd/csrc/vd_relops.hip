@@ -129,6 +129,38 @@ __global__ __launch_bounds__(RB) void k_pack_c01(
     }
 }
 
+// One block: the exclusive scans of the digits the pack counted (the plan
+// of column 0's maximum *gmax) into gofs8 or gofs9, then a read-back of
+// {bit d set: digit d varies (no bin holds all n rows), the maximum}.
+constexpr int SP_BLOCK = 512;
+__global__ __launch_bounds__(SP_BLOCK) void k_swap_plan_scan(
+    const uint32_t *__restrict__ g8, const uint32_t *__restrict__ g9, const uint32_t *gmax,
+    uint64_t n, uint32_t *__restrict__ gofs8, uint32_t *__restrict__ gofs9, Readback rb) {
+    __shared__ uint32_t s_tmp[SP_BLOCK / 64];
+    const uint32_t mx = *gmax;
+    const SwapPlan plan = swap_plan(mx);
+    const int bins = plan.use9 ? 512 : 256;
+    const uint32_t *h = plan.use9 ? g9 : g8;
+    uint32_t *go = plan.use9 ? gofs9 : gofs8;
+    const int tid = (int)threadIdx.x;
+    uint64_t varies = 0;
+    for (int d = 0; d < plan.ndig; ++d) {
+        const uint32_t v = tid < bins ? h[d * bins + tid] : 0u;
+        uint32_t tot;
+        const uint32_t ex = dev::block_excl_scan<SP_BLOCK>(v, s_tmp, &tot);
+        if (tid < bins) go[d * bins + tid] = ex;
+        const int one_bin = __syncthreads_or(tid < bins && (uint64_t)v == n);
+        varies |= (one_bin ? 0ull : 1ull) << d;
+    }
+    if (tid < 64) {
+        if (tid == 0) {
+            rb.dst[0] = varies;
+            rb.dst[1] = mx;
+        }
+        dev::rb_publish(rb);
+    }
+}
+
 // out0 = key >> 32, out1 = low word of the key; with rows, out_c =
 // in_c[rows[i]] for c >= 2 (rows = the source row of each sorted key).
 __global__ __launch_bounds__(256) void k_unpack_c01(const uint64_t *__restrict__ keys,
@@ -232,48 +264,37 @@ void sort_col0(vd_ctx *ctx, vd_rel *rel) {
     DBuf<uint64_t> keys(ctx, n), ktmp(ctx, n);
     DBuf<uint32_t> rows(ctx, by_row ? n : 0);
     // Histograms of either digit plan in one buffer [8-bit: 4 x 256 | 9-bit:
-    // 3 x 512], their offsets, and the words read back: "varies" flags of
-    // the four bytes and of the three 9-bit digits (meaningful for the
-    // plan's counted digits), then column 0's maximum (in the low half of
-    // the last word, computed first: it selects the plan).
+    // 3 x 512] and their offsets; column 0's maximum, taken first: it
+    // selects the plan.
     DBuf<uint32_t> hist(ctx, SWAP_H8 + SWAP_H9), gofs(ctx, SWAP_H8 + SWAP_H9);
-    DBuf<uint64_t> flags(ctx, 8);
+    DBuf<uint32_t> dmax(ctx, 1);
     uint32_t *h8 = hist.p, *h9 = hist.p + SWAP_H8;
-    uint32_t *hmax = reinterpret_cast<uint32_t *>(flags.p + 7);
     fill_u32(ctx, hist.p, 0, SWAP_H8 + SWAP_H9);
-    fill_u64(ctx, flags.p, 0, 8);
-    max_u32_dev(ctx, rel->cols[0], n, hmax);  // the digit plan of the pack
+    fill_u32(ctx, dmax.p, 0, 1);
+    max_u32_dev(ctx, rel->cols[0], n, dmax.p);
     launch(ctx, "sort_pack", (double)n * (8.0 + 4.0 * std::min(nc, 2) + (by_row ? 4.0 : 0.0)),
            dim3(radix::hist_grid(ceil_div(n, RB * R_ITEMS))), dim3(RB), 0, k_pack_c01,
            (const uint32_t *)rel->cols[0], (const uint32_t *)(nc >= 2 ? rel->cols[1] : nullptr),
            (const uint32_t *)nullptr, n, keys.p, by_row ? rows.p : nullptr, h8, h9,
-           (const uint32_t *)hmax);
-    radix_scan_hist_flags(ctx, h8, 4, n, gofs.p, flags.p, 8);
-    radix_scan_hist_flags(ctx, h9, 3, n, gofs.p + SWAP_H8, flags.p + 4, 9);
-    uint64_t hv[8];
-    read_words(ctx, hv, flags.p, 8);
+           (const uint32_t *)dmax.p);
+    const Readback rb = rb_next(ctx);
+    launch(ctx, "radix_scan_hist", 0.0, dim3(1), dim3(SP_BLOCK), 0, k_swap_plan_scan,
+           (const uint32_t *)h8, (const uint32_t *)h9, (const uint32_t *)dmax.p, n, gofs.p,
+           gofs.p + SWAP_H8, rb);
+    uint64_t hv[2];
+    rb_wait(ctx, rb, hv, 2);
     // Passes: the digits of the plan that vary (the pack counted only the
     // plan's digits below the maximum's bit length).
-    const SwapPlan plan = swap_plan((uint32_t)hv[7]);
-    const bool use9 = plan.use9;
+    const SwapPlan plan = swap_plan((uint32_t)hv[1]);
     int shifts[4];
     const uint32_t *g[4];
     int npass = 0;
-    if (use9) {
-        for (int d = 0; d < plan.ndig; ++d)
-            if (hv[4 + d]) {
-                shifts[npass] = 32 + 9 * d;
-                g[npass] = gofs.p + SWAP_H8 + d * 512;
-                ++npass;
-            }
-    } else {
-        for (int d = 0; d < plan.ndig; ++d)
-            if (hv[d]) {
-                shifts[npass] = 32 + 8 * d;
-                g[npass] = gofs.p + d * 256;
-                ++npass;
-            }
-    }
+    for (int d = 0; d < plan.ndig; ++d)
+        if ((hv[0] >> d) & 1u) {
+            shifts[npass] = 32 + (plan.use9 ? 9 : 8) * d;
+            g[npass] = plan.use9 ? gofs.p + SWAP_H8 + d * 512 : gofs.p + d * 256;
+            ++npass;
+        }
     if (npass == 0) return;  // column 0 is constant: already in order
 
     // Payload buffers: src = the relation's columns (or the row ids); pa
@@ -299,7 +320,7 @@ void sort_col0(vd_ctx *ctx, vd_rel *rel) {
             pb.p[c] = fresh_col();
         }
         sorted = radix_sort_shifts(ctx, keys.p, ktmp.p, n, shifts, npass, g, m, src, pa, pb,
-                                   &fin, use9 ? 9 : 8);
+                                   &fin, plan.use9 ? 9 : 8);
         for (int c = 0; c < std::min(nc, 2); ++c)
             out01[c] = rel_borrowed(rel, c) ? fresh_col() : rel->cols[c];
         if (by_row)

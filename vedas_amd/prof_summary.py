@@ -25,7 +25,7 @@ LABELS = [
     (r"k_rs_onesweep<\d+, \d+, [1-9][,>]", "radix_onesweep_kv"),
     (r"k_rs_onesweep<\d+, \d+, 0[,>]", "radix_onesweep"),
     (r"k_rs_hist\b", "radix_hist"),
-    (r"k_rs_scan_hist\b|k_rs_scan_hist_flags\b", "radix_scan_hist"),
+    (r"k_rs_scan_hist\b|k_swap_plan_scan\b", "radix_scan_hist"),
     (r"k_join_expand_multi\b", "join_expand"),
     (r"k_join_expand<", "join_expand"),
     (r"k_join_bounds_multi\b", "join_bounds"),
