@@ -96,11 +96,17 @@ __global__ __launch_bounds__(256) void k_join_ranges_multi(const JPairDev *P, in
 }
 
 // As k_join_bounds (vd_join.hip): the right range, or a directory of every
-// k-th key of it, in LDS.
+// k-th key of it, in LDS; then the tile's exclusive scan of the counts.
+// Block b is tile b of the batched row space, in which every input's left
+// rows start on a tile (its rows past nl are padding with no matches):
+// lower / loc are written for all OFF_TILE rows of the tile.
+static_assert(MB_TILE == OFF_TILE, "bounds tiles are the offset tiles");
 __global__ __launch_bounds__(MB_BLOCK) void k_join_bounds_multi(
     const JPairDev *P, int np, const uint64_t *tbase, const uint64_t *ranges, uint32_t *lower,
-    uint32_t *cnt) {
+    uint32_t *loc, uint64_t *tile_total, uint64_t *wide) {
+    static_assert(MJ_RCAP * 4 >= MB_TILE * 12, "counts and offsets reuse the staging buffer");
     __shared__ uint32_t s_r[MJ_RCAP];
+    __shared__ uint64_t s_tmp[MB_BLOCK / 64];
     __shared__ int s_j;
     if (threadIdx.x == 0) s_j = find_seg(tbase, np, blockIdx.x);
     __syncthreads();
@@ -137,24 +143,60 @@ __global__ __launch_bounds__(MB_BLOCK) void k_join_bounds_multi(
     // first staged entry >= key (t0) and > key (t1) of every row
     uint32_t t0[MB_ITEMS], t1[MB_ITEMS];
     dev::lockstep_bounds<MB_ITEMS>(s_r, (uint32_t)nd, lkey, t0, t1);
+    const uint64_t row0 = (uint64_t)blockIdx.x * MB_TILE;  // tile b in the row space
+    uint32_t cnt[MB_ITEMS];
 #pragma unroll
     for (int it = 0; it < MB_ITEMS; ++it) {
         const uint64_t i = base + (uint64_t)it * MB_BLOCK + threadIdx.x;
-        if (i >= nl) break;
-        const uint32_t key = lkey[it];
-        uint64_t lo = t0[it], hi = t1[it];
-        if (k > 1) {
-            // staged entries are every k-th key: the bounds lie within the
-            // k-1 keys before the directory entry found
-            const uint64_t d0 = t0[it], d1 = t1[it];
-            lo = dev::lower_bound(rk, rlo + (d0 ? (d0 - 1) * k + 1 : 0), rlo + min(d0 * k, m),
-                                  key) - rlo;
-            hi = dev::upper_bound(rk, rlo + (d1 ? (d1 - 1) * k + 1 : 0), rlo + min(d1 * k, m),
-                                  key) - rlo;
+        uint32_t c = 0, low = 0;
+        if (i < nl) {
+            const uint32_t key = lkey[it];
+            uint64_t lo = t0[it], hi = t1[it];
+            if (k > 1) {
+                // staged entries are every k-th key: the bounds lie within
+                // the k-1 keys before the directory entry found
+                const uint64_t d0 = t0[it], d1 = t1[it];
+                lo = dev::lower_bound(rk, rlo + (d0 ? (d0 - 1) * k + 1 : 0),
+                                      rlo + min(d0 * k, m), key) - rlo;
+                hi = dev::upper_bound(rk, rlo + (d1 ? (d1 - 1) * k + 1 : 0),
+                                      rlo + min(d1 * k, m), key) - rlo;
+            }
+            low = (uint32_t)(rlo + lo);
+            c = (uint32_t)(hi - lo);
         }
-        lower[D.lbase + i] = (uint32_t)(rlo + lo);
-        cnt[D.lbase + i] = (uint32_t)(hi - lo);
+        lower[row0 + (uint64_t)it * MB_BLOCK + threadIdx.x] = low;
+        cnt[it] = c;
     }
+    __syncthreads();  // the staged right keys are no longer read
+    uint64_t *s_off = reinterpret_cast<uint64_t *>(s_r);
+    uint32_t *s_cnt = reinterpret_cast<uint32_t *>(s_off + MB_TILE);
+#pragma unroll
+    for (int it = 0; it < MB_ITEMS; ++it) s_cnt[it * MB_BLOCK + threadIdx.x] = cnt[it];
+    __syncthreads();
+    // Thread t scans rows t*ITEMS .. t*ITEMS+ITEMS-1 of the tile.
+    uint64_t run = 0;
+    uint32_t mine[MB_ITEMS];
+#pragma unroll
+    for (int it = 0; it < MB_ITEMS; ++it) {
+        mine[it] = s_cnt[threadIdx.x * MB_ITEMS + it];
+        run += mine[it];
+    }
+    uint64_t tot;
+    uint64_t ex = dev::block_excl_scan<MB_BLOCK>(run, s_tmp, &tot);
+#pragma unroll
+    for (int it = 0; it < MB_ITEMS; ++it) {
+        s_off[threadIdx.x * MB_ITEMS + it] = ex;
+        ex += mine[it];
+    }
+    if (threadIdx.x == 0) {
+        tile_total[blockIdx.x] = tot;
+        if (tot >> 32) atomicOr((unsigned long long *)wide, 1ull);  // offsets beyond 32 bits
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < MB_ITEMS; ++it)
+        loc[row0 + (uint64_t)it * MB_BLOCK + threadIdx.x] =
+            (uint32_t)s_off[it * MB_BLOCK + threadIdx.x];
 }
 
 // Output size of every pair from the global offsets.
@@ -479,51 +521,60 @@ void join_multi(vd_ctx *ctx, const vd_rel *const *L, const vd_rel *const *R, int
                 vd_rel **outs) {
     VD_REQUIRE(np <= 65535, VD_ERANGE, "at most 65535 joins per batched call");
     std::vector<JPairDev> hp(np);
-    std::vector<uint64_t> lrows(np), lbase(np + 1, 0);
+    std::vector<uint64_t> lrows(np);
     for (int j = 0; j < np; ++j) {
         VD_REQUIRE(L[j] && R[j] && L[j]->ncols >= 1 && R[j]->ncols >= 1, VD_EINVAL,
                    "bad join input");
         VD_REQUIRE(L[j]->ncols + R[j]->ncols - 1 <= VD_MAX_COLS, VD_EINVAL,
                    "join output has too many columns");
+        // an empty right side contributes no left rows (all counts 0)
+        lrows[j] = R[j]->nrows ? L[j]->nrows : 0;
+    }
+    // Row space of the batch: input j's left rows start on tile tb[j], so a
+    // bounds block (one tile) scans its own counts; rows past an input's end
+    // up to its next tile are padding with no matches.
+    const std::vector<uint64_t> tb = tile_prefix(lrows);
+    const uint64_t ntiles = tb[np];
+    const uint64_t nrow = ntiles * MB_TILE;
+    VD_REQUIRE(nrow < (1ull << 32), VD_ERANGE, "batched join inputs too large");
+    std::vector<uint64_t> lbase(np + 1);
+    for (int j = 0; j <= np; ++j) lbase[j] = tb[j] * MB_TILE;
+    for (int j = 0; j < np; ++j) {
         JPairDev d{};
         for (int c = 0; c < L[j]->ncols; ++c) d.lc[c] = L[j]->cols[c];
         for (int c = 0; c < R[j]->ncols; ++c) d.rc[c] = R[j]->cols[c];
         d.ncl = L[j]->ncols;
         d.ncr = R[j]->ncols;
-        // an empty right side contributes no left rows (all counts 0)
-        d.nl = R[j]->nrows ? L[j]->nrows : 0;
+        d.nl = lrows[j];
         d.nr = R[j]->nrows;
         d.lbase = lbase[j];
-        lrows[j] = d.nl;
-        lbase[j + 1] = lbase[j] + d.nl;
         hp[j] = d;
     }
-    const uint64_t nl = lbase[np];
-    VD_REQUIRE(nl < (1ull << 32), VD_ERANGE, "batched join inputs too large");
     for (int j = 0; j < np; ++j) outs[j] = nullptr;
-    std::vector<uint64_t> tot(np, 0);
+    std::vector<uint64_t> tot(np + 1, 0);  // + the flag of a tile beyond 32-bit offsets
     DBuf<JPairDev> dp = to_device(ctx, hp);
     DBuf<uint64_t> dlbase = to_device(ctx, lbase);
-    DBuf<uint32_t> lower(ctx, nl), cnt(ctx, nl);
-    const uint64_t otiles = ceil_div(nl, OFF_TILE);
-    DBuf<uint64_t> loc(ctx, nl), tpre(ctx, otiles + 1), dtot(ctx, np);
-    const TileOff off{loc.p, tpre.p};
-    if (nl > 0) {
-        std::vector<uint64_t> tb = tile_prefix(lrows);
+    DBuf<uint32_t> lower(ctx, nrow), loc(ctx, nrow);
+    DBuf<uint64_t> ttot(ctx, ntiles), tpre(ctx, ntiles + 1), dtot(ctx, np + 1);
+    const TileOff off{(const uint32_t *)loc.p, tpre.p};
+    if (ntiles > 0) {
         DBuf<uint64_t> dtb = to_device(ctx, tb);
         double bbytes = 0;
         for (int j = 0; j < np; ++j) bbytes += 4.0 * (hp[j].nl + hp[j].nr) + 8.0 * hp[j].nl;
-        DBuf<uint64_t> ranges(ctx, 2 * tb[np]);
-        launch(ctx, "join_ranges", 0.0, dim3((unsigned)ceil_div(tb[np] * 2 * MR_GROUP, 256)),
-               dim3(256), 0, k_join_ranges_multi, (const JPairDev *)dp.p, np, (const uint64_t *)dtb.p, tb[np],
-               ranges.p);
-        launch(ctx, "join_bounds", bbytes, dim3((unsigned)tb[np]), dim3(MB_BLOCK), 0,
+        DBuf<uint64_t> ranges(ctx, 2 * ntiles);
+        fill_u64(ctx, dtot.p + np, 0, 1);
+        launch(ctx, "join_ranges", 0.0, dim3((unsigned)ceil_div(ntiles * 2 * MR_GROUP, 256)),
+               dim3(256), 0, k_join_ranges_multi, (const JPairDev *)dp.p, np,
+               (const uint64_t *)dtb.p, ntiles, ranges.p);
+        launch(ctx, "join_bounds", bbytes, dim3((unsigned)ntiles), dim3(MB_BLOCK), 0,
                k_join_bounds_multi, (const JPairDev *)dp.p, np, (const uint64_t *)dtb.p,
-               (const uint64_t *)ranges.p, lower.p, cnt.p);
-        tile_offsets_u32(ctx, cnt.p, nl, loc.p, tpre.p);
+               (const uint64_t *)ranges.p, lower.p, loc.p, ttot.p, dtot.p + np);
+        scan_tile_totals(ctx, ttot.p, ntiles, tpre.p);
         launch(ctx, "join_totals", 0.0, dim3((unsigned)ceil_div(np, 256)), dim3(256), 0,
-               k_pair_totals, (const JPairDev *)dp.p, np, off, nl, otiles, dtot.p);
-        download(ctx, tot.data(), dtot.p, np * sizeof(uint64_t));
+               k_pair_totals, (const JPairDev *)dp.p, np, off, nrow, ntiles, dtot.p);
+        download(ctx, tot.data(), dtot.p, (np + 1) * sizeof(uint64_t));
+        VD_REQUIRE(tot[np] == 0, VD_ERANGE,
+                   "batched join: 2048 consecutive left rows match more than 2^32-1 right rows");
     }
     uint64_t total = 0;
     try {
@@ -537,11 +588,11 @@ void join_multi(vd_ctx *ctx, const vd_rel *const *L, const vd_rel *const *R, int
         if (total > 0) {
             upload(ctx, dp.p, hp.data(), np * sizeof(JPairDev));
             const int nv = ctx->join_nv;
-            const uint64_t nblocks = ceil_div(nl + total, (uint64_t)nv);
+            const uint64_t nblocks = ceil_div(nrow + total, (uint64_t)nv);
             VD_REQUIRE(nblocks < (1ull << 31), VD_ERANGE, "join expansion grid too large");
             DBuf<uint64_t> mp(ctx, nblocks + 1);
             launch(ctx, "join_partition", 0.0, dim3((unsigned)ceil_div(nblocks + 1, 256)),
-                   dim3(256), 0, k_join_partition_multi, off, nl, total,
+                   dim3(256), 0, k_join_partition_multi, off, nrow, total,
                    nv, nblocks, mp.p);
             double xbytes = 0;
             for (int j = 0; j < np; ++j)
@@ -550,7 +601,7 @@ void join_multi(vd_ctx *ctx, const vd_rel *const *L, const vd_rel *const *R, int
             auto go = [&](auto kern) {
                 launch(ctx, "join_expand", xbytes, dim3((unsigned)nblocks), dim3(MJ_BLOCK), 0,
                        kern, off, (const uint32_t *)lower.p,
-                       (const uint64_t *)mp.p, nl, total, (const JPairDev *)dp.p, np,
+                       (const uint64_t *)mp.p, nrow, total, (const JPairDev *)dp.p, np,
                        (const uint64_t *)dlbase.p);
             };
             if (nv == 1024) go(k_join_expand_multi<1024>);

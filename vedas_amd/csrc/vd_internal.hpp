@@ -215,19 +215,18 @@ void read_words(vd_ctx *ctx, uint64_t *dst, const uint64_t *dptr, int n);
 
 // ---- primitives ------------------------------------------------------------
 // Two-level offsets of per-row counts, in tiles of OFF_TILE rows: loc[i] =
-// exclusive scan of the counts within i's tile, tpre[t] = the counts before
-// tile t (tpre[ntiles] = the total); the offset of row i < n is
-// tpre[i / OFF_TILE] + loc[i] (TileOff).  No pass adds the two.
+// exclusive scan of the counts within i's tile (32 bits: a tile's total is
+// checked to fit), tpre[t] = the counts before tile t (tpre[ntiles] = the
+// total); the offset of row i < n is tpre[i / OFF_TILE] + loc[i] (TileOff).
+// No pass adds the two.
 constexpr int OFF_TILE = 2048;
 struct TileOff {
-    const uint64_t *loc, *tpre;
+    const uint32_t *loc;
+    const uint64_t *tpre;
     __device__ __forceinline__ uint64_t operator()(uint64_t i) const {
         return tpre[i / OFF_TILE] + loc[i];
     }
 };
-// loc and tpre (ceil(n / OFF_TILE) + 1 entries) of n counts.
-void tile_offsets_u32(vd_ctx *ctx, const uint32_t *in, uint64_t n, uint64_t *loc,
-                      uint64_t *tpre);
 // tpre from the tile totals (one block).
 void scan_tile_totals(vd_ctx *ctx, const uint64_t *tot, uint64_t ntiles, uint64_t *tpre);
 // Exclusive scan of n uint32 counts into uint64 offsets; *d_total (device)

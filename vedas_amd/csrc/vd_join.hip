@@ -224,15 +224,7 @@ __global__ __launch_bounds__(JB_BLOCK) void k_join_bounds(
 }
 
 static_assert(JB_TILE == OFF_TILE, "bounds tiles are the offset tiles");
-// Output offset of left row i: the tile's prefix plus the 32-bit offset
-// within its tile (a tile's total is checked to fit 32 bits).
-struct JoinOff {
-    const uint32_t *loc;
-    const uint64_t *tpre;
-    __device__ __forceinline__ uint64_t operator()(uint64_t i) const {
-        return tpre[i / OFF_TILE] + loc[i];
-    }
-};
+using JoinOff = TileOff;
 
 // mp[b] = number of left rows i with i + off[i] < b * nv (clamped to the
 // end of the diagonal), for b = 0 .. nblocks: a group of lanes per b.

@@ -164,21 +164,6 @@ void scan_tile_totals(vd_ctx *ctx, const uint64_t *tot, uint64_t ntiles, uint64_
            tot, ntiles, tpre);
 }
 
-void tile_offsets_u32(vd_ctx *ctx, const uint32_t *in, uint64_t n, uint64_t *loc,
-                      uint64_t *tpre) {
-    const uint64_t tiles = ceil_div(n, OFF_TILE);
-    if (tiles == 0) {
-        fill_u64(ctx, tpre, 0, 1);
-        return;
-    }
-    VD_REQUIRE(tiles < (1ull << 31), VD_ERANGE, "scan too large");
-    DBuf<uint64_t> tot(ctx, tiles);
-    launch(ctx, "scan_tile", (double)n * 12.0, dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0,
-           k_tile_scan<uint32_t>, in, loc, n, (const uint64_t *)nullptr, (uint64_t *)nullptr,
-           tot.p);
-    scan_tile_totals(ctx, tot.p, tiles, tpre);
-}
-
 static unsigned grid_for(uint64_t n) {
     uint64_t g = ceil_div(n, 256);
     if (g > 8192) g = 8192;
