@@ -3,14 +3,17 @@
 // Triples are drawn in chunks of 2^20; chunk k uses its own splitmix64
 // stream seeded from (seed, k), so the output does not depend on the number
 // of threads.  Zipf draws invert the CDF of w_k = k^-s over k = 1..M by
-// binary search (ID = rank, ID 1 the most frequent).  With `distinct`, the
-// first occurrence of each triple is kept and further chunks are drawn
-// until n distinct triples exist.
+// binary search inside a guide-table bucket (ID = rank, ID 1 the most
+// frequent).  With `distinct`, the output is the first n distinct triples
+// of the chunk stream, in stream order (first occurrences found per hash
+// bucket in parallel).
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <thread>
 #include <vector>
@@ -40,24 +43,65 @@ inline double unit(uint64_t &st) {
     return (double)(splitmix64(st) >> 11) * (1.0 / 9007199254740992.0);
 }
 
+unsigned worker_count(uint64_t items) {
+    unsigned nt = std::thread::hardware_concurrency();
+    if (nt == 0) nt = 1;
+    if (nt > 16) nt = 16;
+    if (nt > items) nt = (unsigned)std::max<uint64_t>(items, 1);
+    return nt;
+}
+
+template <class F>
+void run_threads(unsigned nt, F &&f) {
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt; ++t) th.emplace_back(f, t);
+    f(0u);
+    for (auto &x : th) x.join();
+}
+
 struct Zipf {
-    std::vector<double> cdf;  // cdf[k] = sum_{j<=k} w_j, normalised
-    uint32_t m = 0;
+    std::vector<double> cdf;     // cdf[k] = sum_{j<=k} w_j, normalised
+    std::vector<uint32_t> guide; // guide[j] = first k with cdf[k] > j / G
+    uint32_t m = 0, G = 0;
     bool uniform = true;
     void init(uint32_t items, double s) {
         m = items;
         uniform = (s == 0.0);
         if (uniform) return;
         cdf.resize(items);
+        // weights in parallel, then the running sum in order (the same sums
+        // whatever the thread count)
+        const unsigned nt = worker_count((items + 65535) / 65536);
+        run_threads(nt, [&](unsigned t) {
+            for (uint64_t k = (uint64_t)items * t / nt; k < (uint64_t)items * (t + 1) / nt; ++k)
+                cdf[k] = std::pow((double)(k + 1), -s);
+        });
         double acc = 0.0;
         for (uint32_t k = 0; k < items; ++k) {
-            acc += std::pow((double)(k + 1), -s);
+            acc += cdf[k];
             cdf[k] = acc;
         }
-        for (auto &c : cdf) c /= acc;
+        run_threads(nt, [&](unsigned t) {
+            for (uint64_t k = (uint64_t)items * t / nt; k < (uint64_t)items * (t + 1) / nt; ++k)
+                cdf[k] /= acc;
+        });
         cdf.back() = 1.0;
+        // Guide table: the draw for u in bucket j = floor(u G) lies in
+        // [guide[j], guide[j + 1]], so the search covers only a few CDF
+        // entries (the whole CDF is up to 400 MB at 50M entities).
+        G = std::min<uint32_t>(items, 1u << 22);
+        guide.resize((size_t)G + 1);
+        uint32_t k = 0;
+        for (uint32_t j = 0; j <= G; ++j) {
+            const double x = (double)j / (double)G;
+            while (k < m && cdf[k] <= x) ++k;
+            guide[j] = k;
+        }
     }
-    // Returns an ID in [1, m].
+    // Returns an ID in [1, m]: the first k with cdf[k] > u, plus one (the
+    // inverse CDF).  The bucket index may be off by one from rounding, so
+    // the search spans the neighbouring buckets too; the result is the
+    // same as an upper_bound over the whole CDF.
     uint32_t draw(uint64_t &st) const {
         const double u = unit(st);
         if (uniform) {
@@ -65,7 +109,12 @@ struct Zipf {
             if (k >= m) k = m - 1;
             return (uint32_t)k + 1;
         }
-        size_t k = std::upper_bound(cdf.begin(), cdf.end(), u) - cdf.begin();
+        uint64_t j = (uint64_t)(u * (double)G);
+        if (j >= G) j = G - 1;
+        const size_t lo = guide[j > 0 ? j - 1 : 0];
+        size_t hi = (size_t)guide[std::min<uint64_t>(j + 2, G)] + 1;
+        if (hi > m) hi = m;
+        size_t k = std::upper_bound(cdf.begin() + lo, cdf.begin() + hi, u) - cdf.begin();
         if (k >= m) k = m - 1;
         return (uint32_t)k + 1;
     }
@@ -105,21 +154,14 @@ struct Gen {
 void parallel_chunks(const Gen &g, uint64_t first_chunk, uint64_t n,
                      uint32_t *s, uint32_t *p, uint32_t *o) {
     const uint64_t nchunks = (n + CHUNK - 1) / CHUNK;
-    unsigned nt = std::thread::hardware_concurrency();
-    if (nt == 0) nt = 1;
-    if (nt > 16) nt = 16;
-    if (nt > nchunks) nt = (unsigned)nchunks;
-    std::vector<std::thread> th;
-    for (unsigned t = 0; t < nt; ++t) {
-        th.emplace_back([&, t]() {
-            for (uint64_t c = t; c < nchunks; c += nt) {
-                const uint64_t off = c * CHUNK;
-                const uint64_t cnt = std::min(CHUNK, n - off);
-                g.chunk(first_chunk + c, cnt, s + off, p + off, o + off);
-            }
-        });
-    }
-    for (auto &x : th) x.join();
+    const unsigned nt = worker_count(nchunks);
+    run_threads(nt, [&](unsigned t) {
+        for (uint64_t c = t; c < nchunks; c += nt) {
+            const uint64_t off = c * CHUNK;
+            const uint64_t cnt = std::min(CHUNK, n - off);
+            g.chunk(first_chunk + c, cnt, s + off, p + off, o + off);
+        }
+    });
 }
 
 inline uint64_t triple_hash(uint32_t s, uint32_t p, uint32_t o) {
@@ -130,6 +172,74 @@ inline uint64_t triple_hash(uint32_t s, uint32_t p, uint32_t o) {
     h *= 0xFF51AFD7ED558CCDull;
     h ^= h >> 33;
     return h;
+}
+
+// keep[i] = 1 for the first occurrence of each triple among candidates
+// [0, m), else 0.  Candidates are grouped into buckets by the top bits of
+// their hash (a counting sort that keeps ascending candidate order inside a
+// bucket); each bucket is then deduplicated on its own with an open-address
+// table, visiting its candidates in ascending order.  The flags equal those
+// of one sequential pass, whatever the thread count.
+void first_occurrences(const uint32_t *s, const uint32_t *p, const uint32_t *o,
+                       uint64_t m, uint8_t *keep) {
+    unsigned bbits = 6;
+    while (bbits < 16 && (m >> (bbits + 16)) > 0) ++bbits;
+    const uint64_t nb = 1ull << bbits;
+    const unsigned nt = worker_count((m + 65535) / 65536);
+    std::vector<uint64_t> cnt((size_t)nt * nb, 0);
+    auto lo_of = [&](unsigned t) { return m * t / nt; };
+    run_threads(nt, [&](unsigned t) {
+        uint64_t *c = &cnt[(size_t)t * nb];
+        for (uint64_t i = lo_of(t); i < lo_of(t + 1); ++i)
+            ++c[triple_hash(s[i], p[i], o[i]) >> (64 - bbits)];
+    });
+    std::vector<uint64_t> bstart(nb + 1, 0);
+    uint64_t acc = 0;
+    for (uint64_t b = 0; b < nb; ++b) {
+        bstart[b] = acc;
+        for (unsigned t = 0; t < nt; ++t) {
+            const uint64_t v = cnt[(size_t)t * nb + b];
+            cnt[(size_t)t * nb + b] = acc;
+            acc += v;
+        }
+    }
+    bstart[nb] = acc;
+    std::unique_ptr<uint32_t[]> order(new uint32_t[m]);  // first touched by the workers
+    run_threads(nt, [&](unsigned t) {
+        uint64_t *c = &cnt[(size_t)t * nb];
+        for (uint64_t i = lo_of(t); i < lo_of(t + 1); ++i)
+            order[c[triple_hash(s[i], p[i], o[i]) >> (64 - bbits)]++] = (uint32_t)i;
+    });
+    std::atomic<uint64_t> next{0};
+    run_threads(nt, [&](unsigned) {
+        std::vector<uint32_t> table;
+        for (;;) {
+            const uint64_t b = next.fetch_add(1);
+            if (b >= nb) break;
+            const uint64_t lo = bstart[b], hi = bstart[b + 1];
+            uint64_t cap = 16;
+            while (cap < 2 * (hi - lo)) cap <<= 1;
+            table.assign(cap, 0);  // 0 = empty, else candidate + 1
+            const uint64_t mask = cap - 1;
+            for (uint64_t k = lo; k < hi; ++k) {
+                const uint32_t i = order[k];
+                uint64_t h = triple_hash(s[i], p[i], o[i]) & mask;
+                bool dup = false;
+                for (;;) {
+                    const uint32_t e = table[h];
+                    if (e == 0) break;
+                    const uint32_t r = e - 1;
+                    if (s[r] == s[i] && p[r] == p[i] && o[r] == o[i]) {
+                        dup = true;
+                        break;
+                    }
+                    h = (h + 1) & mask;
+                }
+                keep[i] = !dup;  // every candidate is in exactly one bucket
+                if (!dup) table[h] = i + 1;
+            }
+        }
+    });
 }
 
 }  // namespace
@@ -147,39 +257,46 @@ extern "C" int vd_gen_triples(const vd_gen_params_t *prm, uint32_t *s,
             parallel_chunks(g, 0, n, s, p, o);
             return VD_OK;
         }
-        // Open addressing over row ids; 0 = empty, else row + 1.
-        uint64_t cap = 1;
-        while (cap < 2 * n) cap <<= 1;
-        if (cap >> 32) return VD_ERANGE;
-        std::vector<uint32_t> table(cap, 0);
-        const uint64_t mask = cap - 1;
-        uint64_t have = 0, next_chunk = 0;
-        std::vector<uint32_t> bs, bp, bo;
-        while (have < n) {
-            const uint64_t want = std::max<uint64_t>(n - have, CHUNK);
-            const uint64_t cnt = (want + CHUNK - 1) / CHUNK * CHUNK;
-            bs.resize(cnt); bp.resize(cnt); bo.resize(cnt);
-            parallel_chunks(g, next_chunk, cnt, bs.data(), bp.data(), bo.data());
-            next_chunk += cnt / CHUNK;
-            for (uint64_t i = 0; i < cnt && have < n; ++i) {
-                uint64_t h = triple_hash(bs[i], bp[i], bo[i]) & mask;
-                bool dup = false;
-                for (;;) {
-                    const uint32_t e = table[h];
-                    if (e == 0) break;
-                    const uint64_t r = e - 1;
-                    if (s[r] == bs[i] && p[r] == bp[i] && o[r] == bo[i]) {
-                        dup = true;
-                        break;
+        // The candidate stream is chunks 0, 1, 2, ... of the generator; the
+        // output is its first n distinct triples, in stream order.  Enough
+        // chunks are drawn for a few percent of repeats; if the stream is
+        // short of n distinct triples, more chunks are added and the flags
+        // recomputed.
+        uint64_t nchunks = (n + n / 16 + CHUNK - 1) / CHUNK + 1;
+        const uint64_t max_chunks = 64 + 4 * (n / CHUNK + 1);
+        for (;;) {
+            const uint64_t m = nchunks * CHUNK;
+            if (m >> 32) return VD_ERANGE;
+            // Uninitialised buffers: the workers touch their pages first.
+            // A longer stream redraws every chunk (chunk k is the same
+            // draws each time).
+            std::unique_ptr<uint32_t[]> bs(new uint32_t[m]), bp(new uint32_t[m]),
+                bo(new uint32_t[m]);
+            std::unique_ptr<uint8_t[]> keep(new uint8_t[m]);
+            parallel_chunks(g, 0, m, bs.get(), bp.get(), bo.get());
+            first_occurrences(bs.get(), bp.get(), bo.get(), m, keep.get());
+            // first n kept candidates, in order: per-thread counts, then copies
+            const unsigned nt = worker_count((m + 65535) / 65536);
+            std::vector<uint64_t> kc(nt + 1, 0);
+            run_threads(nt, [&](unsigned t) {
+                uint64_t c = 0;
+                for (uint64_t i = m * t / nt; i < m * (t + 1) / nt; ++i) c += keep[i];
+                kc[t + 1] = c;
+            });
+            for (unsigned t = 0; t < nt; ++t) kc[t + 1] += kc[t];
+            if (kc[nt] >= n) {
+                run_threads(nt, [&](unsigned t) {
+                    uint64_t w = kc[t];
+                    for (uint64_t i = m * t / nt; i < m * (t + 1) / nt && w < n; ++i) {
+                        if (!keep[i]) continue;
+                        s[w] = bs[i]; p[w] = bp[i]; o[w] = bo[i];
+                        ++w;
                     }
-                    h = (h + 1) & mask;
-                }
-                if (dup) continue;
-                s[have] = bs[i]; p[have] = bp[i]; o[have] = bo[i];
-                table[h] = (uint32_t)(have + 1);
-                ++have;
+                });
+                break;
             }
-            if (next_chunk > 64 + 4 * (n / CHUNK + 1)) return VD_ERANGE;  // space too small
+            if (nchunks >= max_chunks) return VD_ERANGE;  // ID space too small
+            nchunks = std::min(max_chunks, nchunks + std::max<uint64_t>(nchunks / 2, 1));
         }
     } catch (const std::bad_alloc &) {
         return VD_ENOMEM;
