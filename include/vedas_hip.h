@@ -237,6 +237,18 @@ int  vd_select_multi(vd_ctx *ctx, const vd_index *idx,
 int  vd_index_lookup_multi(vd_ctx *ctx, const vd_index *idx,
                            const vd_select_job_t *jobs, int njobs,
                            uint64_t *begin, uint64_t *end);
+/* vd_index_lookup_multi plus, for every job, the first and last value of
+ * the column after its constants over the rows found: the Y column for a
+ * job with k1 only, the Z column for a job with k1 and k2 (cols is not
+ * used).  That column is sorted inside the range, so first/last are its
+ * minimum and maximum (both 0 for an empty range).  Replaces the host
+ * binary searches and data reads of the bound pre-scan,
+ * QueryExecutor::updateDataBound / updateL2Bound (QueryExecutor.cu:
+ * 648-675, 737-765), for all patterns of a query in one launch. */
+int  vd_index_bounds_multi(vd_ctx *ctx, const vd_index *idx,
+                           const vd_select_job_t *jobs, int njobs,
+                           uint64_t *begin, uint64_t *end,
+                           uint32_t *first, uint32_t *last);
 /* outs[i] = vd_join(L[i], R[i], 1). */
 int  vd_join_multi(vd_ctx *ctx, const vd_rel *const *L, const vd_rel *const *R,
                    int npairs, vd_rel **outs);

@@ -149,6 +149,18 @@ const char *vdq_result_header(const vdq_result *r, int col);
 int vdq_result_column(const vdq_result *r, int col, uint32_t *host_out);
 int vdq_result_stats(const vdq_result *r, vdq_stats_t *out);
 int vdq_result_free(vdq_result *r);
+/* Process-wide switches of the query executor, as the reference's static
+ * QueryExecutor::ENABLE_* flags set from vdQuery's command line
+ * (QueryExecutor.h:79-84, VedasQuery.cu:37,54,110-113):
+ *   "bound_prescan"    (default 1) the join-variable bound pre-scan of
+ *                      createVariableBound (QueryExecutor.cu:441-479);
+ *   "bound_after_join" (default 1) the bound update after each join
+ *                      (JoinQueryJob.cu:388-397; vdQuery's
+ *                      -dis-bound-after-join clears it).
+ * Both only narrow intermediate relations; results do not depend on them.
+ * Set them between queries, not while another thread runs one. */
+int vdq_set_option(const char *key, int value);
+int vdq_get_option(const char *key, int *value);
 /* Message of the calling thread's last failure. */
 const char *vdq_last_error(void);
 

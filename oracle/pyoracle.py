@@ -47,6 +47,12 @@ def lib():
         l.or_sort_lex.argtypes = [C.POINTER(U32P), i, u64]
         l.or_unique_adjacent.restype = u64
         l.or_unique_adjacent.argtypes = [C.POINTER(U32P), i, u64]
+        l.or_bag_digest.restype = None
+        l.or_bag_digest.argtypes = [C.POINTER(U32P), i, u64, vp]
+        l.or_perm_digest.restype = None
+        l.or_perm_digest.argtypes = [vp, vp, u64, vp, vp, u64, i, vp]
+        l.or_check_perm.restype = i
+        l.or_check_perm.argtypes = [vp, vp, u64, vp, vp, u64, vp, vp, u64, C.POINTER(u64)]
         _lib = l
     return _lib
 
@@ -134,7 +140,73 @@ def unique_adjacent(cols):
 
 
 # ---- plan walk ----------------------------------------------------------------
-def run_plan(ix_by_perm: dict, patterns, node, stats: dict | None = None):
+FULL_BOUND = (0, 0xFFFFFFFF)
+
+
+def update_bound(bnd: dict, var: str, lo: int, hi: int) -> None:
+    """QueryExecutor::updateBoundDict (QueryExecutor.cu:583-594): intersect
+    var's bound with [lo, hi]; a disjoint interval gives [0, 0], and a lower
+    end of 0 keeps 0."""
+    cur_lo, cur_hi = bnd[var]
+    if hi < cur_lo or lo > cur_hi:
+        lo = hi = 0
+    bnd[var] = (0 if lo == 0 else max(cur_lo, lo), min(cur_hi, hi))
+
+
+def prescan_bounds(ix_by_perm: dict, patterns) -> dict:
+    """QueryExecutor::createVariableBound (QueryExecutor.cu:441-479) with
+    squeezeQueryBound1Var / 2Var, updateDataBound and updateL2Bound
+    (:527-581, 648-675, 680-765): variables occurring more than once start
+    at the full range; each pattern, in order, narrows them to the first /
+    last value of the variable's column over the pattern's index range (the
+    column after the constants, sorted there); an empty pattern sets every
+    bound to [0, 0].  For one variable the permutation with both constants
+    first is used (the first of two that exist); for two variables, the
+    object / predicate first as the reference orders them.  Permutations
+    missing from ix_by_perm give no bound."""
+    count = {}
+    for pat in patterns:
+        for t in pat:
+            if isinstance(t, str):
+                count[t] = count.get(t, 0) + 1
+    bnd = {v: FULL_BOUND for v, c in count.items() if c > 1}
+    zero = False
+    for sv, pv, ov in patterns:
+        isv = [isinstance(t, str) for t in (sv, pv, ov)]
+        if sum(isv) == 1:
+            if isv[0]:
+                alts = [(sv, "POS", pv, ov), (sv, "OPS", ov, pv)]
+            elif isv[1]:
+                alts = [(pv, "OSP", ov, sv), (pv, "SOP", sv, ov)]
+            else:
+                alts = [(ov, "PSO", pv, sv), (ov, "SPO", sv, pv)]
+            items = [a for a in alts if a[1] in ix_by_perm][:1]
+        elif sum(isv) == 2:
+            if not isv[1]:
+                items = [(ov, "POS", pv, None), (sv, "PSO", pv, None)]
+            elif not isv[0]:
+                items = [(ov, "SOP", sv, None), (pv, "SPO", sv, None)]
+            else:
+                items = [(pv, "OPS", ov, None), (sv, "OSP", ov, None)]
+            items = [a for a in items if a[1] in ix_by_perm]
+        else:
+            items = []
+        for var, perm, k1, k2 in items:
+            ix = ix_by_perm[perm]
+            b, e = lookup(ix, k1, k2)
+            if b == e:
+                zero = True
+                continue
+            col = ix["y"] if k2 is None else ix["z"]
+            if var in bnd:
+                update_bound(bnd, var, int(col[b]), int(col[e - 1]))
+    if zero:
+        bnd = {v: (0, 0) for v in bnd}
+    return bnd
+
+
+def run_plan(ix_by_perm: dict, patterns, node, stats: dict | None = None,
+             bounds: bool = False):
     """Execute a plan tree over the oracle: QueryExecutor::postorderTraversal
     (QueryExecutor.cu:106-135) with the index choice of
     create1VarSelectQueryJob / create2VarSelectQueryJob
@@ -145,11 +217,20 @@ def run_plan(ix_by_perm: dict, patterns, node, stats: dict | None = None):
     with a re-sort, output [key, left rest, right rest]).  Nodes:
     ("UPLOAD", pattern or index into patterns, var), ("JOIN", l, r, var),
     ("INDEXSWAP", child, var).  ix_by_perm: {"PSO": arrays, "POS": arrays}
-    as build_perm returns.  stats (optional) accumulates "join_rows" and
-    "swap_rows".  Returns (headers, columns)."""
+    as build_perm returns (other permutations are used by the pre-scan when
+    present).  With bounds, the join-variable bounds of the reference are
+    applied: the pre-scan (prescan_bounds), the clip of 2-variable selects
+    to their first column's bound (SelectQueryJob.cu:125-141), the empty
+    1-variable select's bound (:113-119) and the update after each join
+    (JoinQueryJob.cu:388-397, 434-437); the empty-interval filter is not
+    restated (it acts only after joins of fewer than EIF_ALPHA = 100 rows and
+    only shrinks later selects).  stats (optional) accumulates "join_rows",
+    "select_rows" and "swap_rows".  Returns (headers, columns)."""
     st = stats if stats is not None else {}
     st.setdefault("join_rows", 0)
+    st.setdefault("select_rows", 0)
     st.setdefault("swap_rows", 0)
+    bnd = prescan_bounds(ix_by_perm, patterns) if bounds else None
 
     def key_first(h, cols, var):
         i = h.index(var)
@@ -161,24 +242,34 @@ def run_plan(ix_by_perm: dict, patterns, node, stats: dict | None = None):
         st["swap_rows"] += len(cols[0])
         return h, sort_col0(cols)
 
+    def upload(nd):
+        pat = patterns[nd[1]] if isinstance(nd[1], int) else nd[1]
+        sv, pv, ov = pat
+        var = nd[2]
+        if isinstance(sv, str) and isinstance(ov, str):
+            perm, hdr = ("PSO", [sv, ov]) if sv == var else ("POS", [ov, sv])
+            ix = ix_by_perm[perm]
+            b, e = lookup(ix, pv)
+            clip = bnd[hdr[0]] if bnd is not None and hdr[0] in bnd else None
+            return hdr, select(ix, b, e, 3, clip=clip)
+        if isinstance(sv, str):
+            ix = ix_by_perm["POS"]
+            b, e = lookup(ix, pv, ov)
+            hdr = [sv]
+        else:
+            ix = ix_by_perm["PSO"]
+            b, e = lookup(ix, pv, sv)
+            hdr = [ov]
+        if bnd is not None and hdr[0] in bnd and b == e:
+            update_bound(bnd, hdr[0], 0, 0)
+        return hdr, select(ix, b, e, 1)
+
     def walk(nd):
         op = nd[0]
         if op == "UPLOAD":
-            pat = patterns[nd[1]] if isinstance(nd[1], int) else nd[1]
-            sv, pv, ov = pat
-            var = nd[2]
-            if isinstance(sv, str) and isinstance(ov, str):
-                perm, hdr = ("PSO", [sv, ov]) if sv == var else ("POS", [ov, sv])
-                ix = ix_by_perm[perm]
-                b, e = lookup(ix, pv)
-                return hdr, select(ix, b, e, 3)
-            if isinstance(sv, str):
-                ix = ix_by_perm["POS"]
-                b, e = lookup(ix, pv, ov)
-                return [sv], select(ix, b, e, 1)
-            ix = ix_by_perm["PSO"]
-            b, e = lookup(ix, pv, sv)
-            return [ov], select(ix, b, e, 1)
+            h, cols = upload(nd)
+            st["select_rows"] += len(cols[0])
+            return h, cols
         if op == "JOIN":
             a, b = walk(nd[1]), walk(nd[2])
             if len(a[1][0]) > len(b[1][0]):
@@ -187,13 +278,21 @@ def run_plan(ix_by_perm: dict, patterns, node, stats: dict | None = None):
             hb, cb = key_first(b[0], b[1], nd[3])
             out = join(ca, cb, 1)
             st["join_rows"] += len(out[0])
+            if bnd is not None and nd[3] in bnd:
+                if len(out[0]):
+                    update_bound(bnd, nd[3], int(out[0][0]), int(out[0][-1]))
+                else:
+                    update_bound(bnd, nd[3], 0, 0)
             return [nd[3]] + ha[1:] + hb[1:], out
         if op == "INDEXSWAP":
             h, cols = walk(nd[1])
             return key_first(h, cols, nd[2])
         raise ValueError(op)
 
-    return walk(node)
+    out = walk(node)
+    if bnd is not None:
+        st["bounds"] = dict(bnd)
+    return out
 
 
 # ---- naive evaluator -----------------------------------------------------
@@ -240,6 +339,66 @@ def canonical(headers, cols):
     mat = np.stack([np.asarray(cols[i], np.uint32) for i in order], axis=1)
     rows = sorted(map(tuple, mat.tolist()))
     return names, rows
+
+
+# ---- checkers for full-size results -------------------------------------------
+def bag_digest(headers, cols):
+    """Order-independent digest of a relation's rows with columns ordered by
+    variable name: (names, rows, sum of one 64-bit row hash, sum of another).
+    Equal bags of rows give equal digests; compare two relations with ==."""
+    order = sorted(range(len(headers)), key=lambda i: headers[i])
+    names = tuple(headers[i] for i in order)
+    return (names,) + _raw_digest([cols[i] for i in order])
+
+
+def _raw_digest(cols):
+    cs = [np.ascontiguousarray(c, np.uint32) for c in cols]
+    n = len(cs[0]) if cs else 0
+    out = np.zeros(2, np.uint64)
+    lib().or_bag_digest(_colptrs(cs), len(cs), n, _p(out))
+    return n, int(out[0]), int(out[1])
+
+
+def triples_digest(s, p, o):
+    """Bag digest of (s, p, o) triples, columns in that order: (rows, sum, sum)."""
+    return _raw_digest([s, p, o])
+
+
+def perm_digest(ix: dict, perm: str):
+    """Bag digest of a permutation's triples in (s, p, o) order; equals
+    triples_digest of the input exactly when the multisets are equal."""
+    out = np.zeros(2, np.uint64)
+    n = len(ix["y"])
+    l1v = np.ascontiguousarray(ix["l1v"], np.uint32)
+    l1o = np.ascontiguousarray(ix["l1o"], np.uint32)
+    lib().or_perm_digest(_p(l1v), _p(l1o), len(l1v), _p(ix["y"]), _p(ix["z"]), n,
+                         PERMS[perm], _p(out))
+    return n, int(out[0]), int(out[1])
+
+
+CHECK_PERM_FAILURES = {
+    1: "L1 empty / not empty against the row count",
+    2: "L1 offsets not strictly increasing from 0 below n",
+    3: "L1 values not strictly increasing",
+    4: "rows not in (X, Y, Z) order",
+    5: "(X, Y) run head without L2 entry, or L2 entry off a head",
+    6: "L2 value differs from the row's Y",
+    7: "L2 entries left after the last row",
+}
+
+
+def check_perm(ix: dict):
+    """Raise AssertionError unless the arrays of one permutation are sorted
+    by (X, Y, Z) with L1/L2 at exactly the X and (X, Y) run heads."""
+    a = {k: np.ascontiguousarray(ix[k], np.uint32) for k in ("l1v", "l1o", "l2v", "l2o", "y", "z")}
+    assert len(a["l1v"]) == len(a["l1o"]), "L1 values and offsets differ in length"
+    assert len(a["l2v"]) == len(a["l2o"]), "L2 values and offsets differ in length"
+    assert len(a["y"]) == len(a["z"]), "Y and Z columns differ in length"
+    where = C.c_uint64()
+    rc = lib().or_check_perm(_p(a["l1v"]), _p(a["l1o"]), len(a["l1v"]), _p(a["l2v"]),
+                             _p(a["l2o"]), len(a["l2v"]), _p(a["y"]), _p(a["z"]), len(a["y"]),
+                             C.byref(where))
+    assert rc == 0, f"permutation check {rc}: {CHECK_PERM_FAILURES.get(rc)} at {where.value}"
 
 
 def groupby_starts(keys):

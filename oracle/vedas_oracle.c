@@ -296,3 +296,110 @@ uint64_t or_unique_adjacent(uint32_t *const *cols, int ncols, uint64_t n) {
     }
     return w;
 }
+
+/* ---- checkers for results too large for a row-by-row comparison -------- *
+ * These do not restate the reference; they test invariants of its outputs
+ * (sort order, run heads, multisets) so that full-size GPU results can be
+ * checked in O(n) time and memory.  Pinned in tests/test_oracle.py against
+ * sorted() / Counter on small inputs and against planted faults. */
+
+static inline uint64_t or_fmix64(uint64_t h) {
+    h ^= h >> 33;
+    h *= 0xFF51AFD7ED558CCDull;
+    h ^= h >> 33;
+    h *= 0xC4CEB9FE1A85EC53ull;
+    h ^= h >> 33;
+    return h;
+}
+
+/* Hash of one row (values in column order). */
+static inline uint64_t or_row_hash(const uint32_t *v, int ncols) {
+    uint64_t h = 0x243F6A8885A308D3ull;
+    for (int c = 0; c < ncols; ++c)
+        h = or_fmix64(h ^ ((uint64_t)v[c] + 0x9E3779B97F4A7C15ull * (uint64_t)(c + 1)));
+    return h;
+}
+
+/* Order-independent digest of the bag of rows of a column store: the sums
+ * (mod 2^64) of two different 64-bit mixes of each row's hash.  Equal bags
+ * give equal digests; a different bag gives a different digest except with
+ * probability about 2^-64 per sum.  out[0..1] receive the sums. */
+void or_bag_digest(const uint32_t *const *cols, int ncols, uint64_t n, uint64_t *out) {
+    uint64_t a = 0, b = 0;
+    uint32_t v[32];
+    if (ncols > 32) ncols = 32;
+    for (uint64_t i = 0; i < n; ++i) {
+        for (int c = 0; c < ncols; ++c) v[c] = cols[c][i];
+        const uint64_t h = or_row_hash(v, ncols);
+        a += h;
+        b += or_fmix64(h ^ 0x5851F42D4C957F2Dull);
+    }
+    out[0] = a;
+    out[1] = b;
+}
+
+/* Bag digest of a permutation's triples in (s, p, o) column order: rows of
+ * (X from L1, Y, Z) mapped back through the permutation's column order, so
+ * that it equals or_bag_digest over the input s, p, o columns exactly when
+ * the permutation holds the same multiset of triples.  Requires a valid L1
+ * (or_check_perm). */
+void or_perm_digest(const uint32_t *l1v, const uint32_t *l1o, uint64_t n_l1,
+                    const uint32_t *y, const uint32_t *z, uint64_t n, int perm,
+                    uint64_t *out) {
+    uint64_t a = 0, b = 0;
+    uint64_t g = 0;
+    for (uint64_t r = 0; r < n; ++r) {
+        while (g + 1 < n_l1 && l1o[g + 1] <= r) ++g;
+        uint32_t t[3], v[3];
+        t[0] = l1v[g]; t[1] = y[r]; t[2] = z[r];
+        for (int k = 0; k < 3; ++k) v[PERM_COLS[perm][k]] = t[k];
+        const uint64_t h = or_row_hash(v, 3);
+        a += h;
+        b += or_fmix64(h ^ 0x5851F42D4C957F2Dull);
+    }
+    out[0] = a;
+    out[1] = b;
+}
+
+/* Invariants of one permutation's arrays (X implicit through L1):
+ *   1  L1 empty for n > 0, or not empty for n == 0
+ *   2  l1o[0] != 0, L1 offsets not strictly increasing or >= n
+ *   3  L1 values not strictly increasing
+ *   4  rows not in lexicographic (X, Y, Z) order
+ *   5  an (X, Y) run head with no L2 entry, or an L2 entry off a head
+ *   6  an L2 value differs from the row's Y
+ *   7  L2 entries left over after the last row
+ * Returns 0 when all hold, else the code of the first failure; *where
+ * receives the row or entry index. */
+int or_check_perm(const uint32_t *l1v, const uint32_t *l1o, uint64_t n_l1,
+                  const uint32_t *l2v, const uint32_t *l2o, uint64_t n_l2,
+                  const uint32_t *y, const uint32_t *z, uint64_t n, uint64_t *where) {
+    *where = 0;
+    if ((n == 0) != (n_l1 == 0)) return 1;
+    if (n == 0) return n_l2 == 0 ? 0 : 7;
+    if (l1o[0] != 0) return 2;
+    for (uint64_t i = 1; i < n_l1; ++i) {
+        *where = i;
+        if (l1o[i] <= l1o[i - 1] || l1o[i] >= n) return 2;
+        if (l1v[i] <= l1v[i - 1]) return 3;
+    }
+    uint64_t g = 0, k = 0;
+    for (uint64_t r = 0; r < n; ++r) {
+        *where = r;
+        int new_x = 0;
+        if (g + 1 < n_l1 && l1o[g + 1] == r) { ++g; new_x = 1; }
+        const int head = (r == 0) || new_x || y[r] != y[r - 1];
+        if (r > 0 && !new_x) {
+            if (y[r] < y[r - 1] || (y[r] == y[r - 1] && z[r] < z[r - 1])) return 4;
+        }
+        if (head) {
+            if (k >= n_l2 || l2o[k] != r) return 5;
+            if (l2v[k] != y[r]) return 6;
+            ++k;
+        } else if (k < n_l2 && l2o[k] == r) {
+            return 5;
+        }
+    }
+    *where = k;
+    return k == n_l2 ? 0 : 7;
+}

@@ -106,6 +106,53 @@ def test_cfg0_star_query():
     assert st["join_rows"] == len(want[1]) and 1e5 < st["join_rows"] < 2e6
 
 
+def test_bound_prescan_on_cfg0_store():
+    """The join-variable bound pre-scan (createVariableBound) on the 1M
+    store: selects shrink (objects of predicate 1 include literals, which no
+    subject of predicate 2 has), results and every intermediate size equal
+    the oracle's plan walk with the reference's bounds, and results equal
+    the walk without bounds."""
+    from vedas_amd._native import gen_triples
+    from vedas_amd.engine import Engine, get_option, query_text, set_option
+    s, p, o = gen_triples(1_000_000, seed=42)
+    lit = int(o[(p == 1) & (o >= 50_000_000)][0])
+    queries = [plans.chain_plan([1, 2, 3]), plans.chain_plan([2, 1]),
+               plans.snowflake_plan([1, 2], [3, 4]),
+               ([("?s", 1, lit), ("?s", 2, "?y"), ("?y", 3, "?z")],
+                ("JOIN", ("INDEXSWAP", ("JOIN", ("UPLOAD", 0, "?s"), ("UPLOAD", 1, "?s"), "?s"),
+                          "?y"), ("UPLOAD", 2, "?y"), "?y")),
+               ([("?a", 1, "?b"), ("?b", 2, 4_000_000_000)],
+                ("JOIN", ("UPLOAD", 0, "?b"), ("UPLOAD", 1, "?b"), "?b"))]
+    ix = {perm: O.build_perm(s, p, o, perm) for perm in ("PSO", "POS")}
+    e = Engine(0)
+    assert get_option("bound_prescan") == 1
+    shrunk = 0
+    try:
+        e.load(s, p, o, ("PSO", "POS"))
+        for pats, node in queries:
+            got = {}
+            for on in (0, 1):
+                set_option("bound_prescan", on)
+                r = e.query(query_text(pats, node))
+                got[on] = (O.bag_digest(r.headers(), r.columns()), r.stats())
+                r.free()
+            st0, st1 = {}, {}
+            h0, c0 = O.run_plan(ix, pats, node, st0)
+            h1, c1 = O.run_plan(ix, pats, node, st1, bounds=True)
+            want = O.bag_digest(h0, c0)
+            assert O.bag_digest(h1, c1) == want
+            assert got[0][0] == want and got[1][0] == want, pats
+            for k in ("select_rows", "join_rows"):
+                assert got[0][1][k] == st0[k], (pats, k)
+                assert got[1][1][k] == st1[k], (pats, k)
+            assert got[1][1]["select_rows"] <= got[0][1]["select_rows"]
+            shrunk += got[1][1]["select_rows"] < got[0][1]["select_rows"]
+    finally:
+        set_option("bound_prescan", 1)
+        e.close()
+    assert shrunk >= 3
+
+
 def test_autotune_keeps_results(small):
     """The autotuner only picks kernel shapes: chosen values are valid
     candidates and results stay the same after tuning (and in a serving

@@ -223,3 +223,178 @@ def test_planted_mistake_is_caught():
     want = nested_loop_join(L, R, 1)
     assert list(zip(*[c.tolist() for c in good])) == want
     assert list(zip(*[c.tolist() for c in bad])) != want
+
+
+# ---- checkers used for full-size GPU results --------------------------------
+def test_bag_digest_is_the_bag_of_rows():
+    """Equal multisets (Counter) <=> equal digests, on small relations with
+    duplicate rows; row order does not matter, column names order columns."""
+    from collections import Counter
+    rng = np.random.default_rng(4)
+    base = [rng.integers(0, 4, 300).astype(np.uint32) for _ in range(3)]
+    perm = rng.permutation(300)
+    shuffled = [c[perm] for c in base]
+    d0 = O.bag_digest(["?a", "?b", "?c"], base)
+    assert O.bag_digest(["?a", "?b", "?c"], shuffled) == d0
+    # the same columns under headers listed in another order
+    assert O.bag_digest(["?c", "?a", "?b"], [base[2], base[0], base[1]]) == d0
+    rows = Counter(zip(*(c.tolist() for c in base)))
+    for trial in range(200):
+        other = [c.copy() for c in base]
+        i = int(rng.integers(0, 300))
+        j = int(rng.integers(0, 3))
+        other[j][i] = (other[j][i] + 1 + trial % 3) % 5
+        same = Counter(zip(*(c.tolist() for c in other))) == rows
+        assert (O.bag_digest(["?a", "?b", "?c"], other) == d0) == same
+    # a duplicated row instead of another row (same row count)
+    dup = [np.concatenate([c[:-1], c[:1]]) for c in base]
+    assert (O.bag_digest(["?a", "?b", "?c"], dup) == d0) == (
+        Counter(zip(*(c.tolist() for c in dup))) == rows)
+    # two columns swapped under the same headers
+    assert O.bag_digest(["?a", "?b", "?c"], [base[1], base[0], base[2]]) != d0
+    assert O.bag_digest(["?a"], [np.zeros(0, np.uint32)]) == (("?a",), 0, 0, 0)
+
+
+@pytest.mark.parametrize("perm", PERM_NAMES)
+def test_check_perm_and_perm_digest(perm):
+    s, p, o = rand_triples(3000, 9)
+    ix = O.build_perm(s, p, o, perm)
+    O.check_perm(ix)
+    assert O.perm_digest(ix, perm) == O.triples_digest(s, p, o)
+    golden = np.load(os.path.join(GOLDEN, "index_small.npz"))
+    if f"{perm}_y" in golden:
+        gix = {k: golden[f"{perm}_{k}"] for k in ("l1v", "l1o", "l2v", "l2o", "y", "z")}
+        O.check_perm(gix)
+        assert O.perm_digest(gix, perm) == O.triples_digest(golden["s"], golden["p"], golden["o"])
+
+    def broken(fn):
+        b = {k: v.copy() for k, v in ix.items()}
+        fn(b)
+        return b
+
+    heads = ix["l2o"]
+    r = int(heads[len(heads) // 2])  # a row that starts an (X, Y) run
+    faults = [
+        lambda b: b["z"].__setitem__(slice(r - 1, r + 1), b["z"][r - 1:r + 1][::-1].copy())
+        if b["y"][r - 1] == b["y"][r] else b["y"].__setitem__(r, b["y"][r - 1] - 1),
+        lambda b: b.__setitem__("l2o", np.delete(b["l2o"], len(heads) // 2)),
+        lambda b: b["l2v"].__setitem__(1, b["l2v"][1] + 1),
+        lambda b: b["l1v"].__setitem__(slice(0, 2), b["l1v"][1::-1].copy()),
+        lambda b: b["l1o"].__setitem__(1, b["l1o"][1] + 1),
+        lambda b: b.__setitem__("l2o", np.append(b["l2o"], len(b["y"]) - 1)),
+        lambda b: (b.__setitem__("l2o", np.append(b["l2o"], len(b["y"]) - 1)),
+                   b.__setitem__("l2v", np.append(b["l2v"], b["y"][-1]))),
+    ]
+    for k, f in enumerate(faults):
+        b = broken(f)
+        with pytest.raises(AssertionError):
+            O.check_perm(b)
+            # a fault that keeps the invariants must change the multiset
+            assert O.perm_digest(b, perm) == O.triples_digest(s, p, o), k
+    # a changed value with the order kept: invariants hold, multiset differs
+    b = broken(lambda b: b["z"].__setitem__(-1, b["z"][-1] + 1))
+    O.check_perm(b)
+    assert O.perm_digest(b, perm) != O.triples_digest(s, p, o)
+
+
+# ---- join-variable bounds (pre-scan, clip, after-join update) ---------------
+def brute_bounds(s, p, o, patterns):
+    """Per join variable, the intersection over the patterns that hold it
+    of [min, max] of its values among the triples matching the pattern's
+    constants, straight from the triples; [0, 0] for a variable whose
+    intersection is empty, and for every variable when a pattern matches
+    nothing."""
+    cols = (s, p, o)
+    count = {}
+    for pat in patterns:
+        for t in pat:
+            if isinstance(t, str):
+                count[t] = count.get(t, 0) + 1
+    lohi = {v: [0, 0xFFFFFFFF] for v, c in count.items() if c > 1}
+    empty = False
+    for pat in patterns:
+        m = np.ones(len(s), bool)
+        for c, t in zip(cols, pat):
+            if not isinstance(t, str):
+                m &= c == t
+        if not m.any():
+            empty = True
+            continue
+        for c, t in zip(cols, pat):
+            if isinstance(t, str) and t in lohi:
+                lohi[t][0] = max(lohi[t][0], int(c[m].min()))
+                lohi[t][1] = min(lohi[t][1], int(c[m].max()))
+    if empty:
+        return {v: (0, 0) for v in lohi}
+    return {v: (lo, hi) if lo <= hi else (0, 0) for v, (lo, hi) in lohi.items()}
+
+
+def bounded_queries(s, p, o):
+    return [plans.star_plan([1, 2, 3]), plans.chain_plan([1, 2, 3]),
+            plans.snowflake_plan([1, 2], [3, 4]),
+            ([("?s", int(p[0]), int(o[0])), ("?s", 2, "?y"), (int(s[3]), 3, "?s")],
+             ("JOIN", ("JOIN", ("UPLOAD", 0, "?s"), ("UPLOAD", 1, "?s"), "?s"),
+              ("UPLOAD", 2, "?s"), "?s")),
+            # a constant no triple has: every bound collapses to [0, 0]
+            ([("?a", 1, "?b"), ("?b", 2, 999_999)],
+             ("JOIN", ("UPLOAD", 0, "?b"), ("UPLOAD", 1, "?b"), "?b"))]
+
+
+def test_prescan_bounds_equal_brute_force():
+    s, p, o = small_store()
+    # objects of predicate 1 beyond every subject: the chain's ?b narrows
+    o = o.copy()
+    o[p == 1] += 40
+    allp = ("SPO", "PSO", "POS", "OSP", "SOP", "OPS")
+    ix = {perm: O.build_perm(s, p, o, perm) for perm in allp}
+    narrowed = 0
+    for pats, _ in bounded_queries(s, p, o):
+        want = brute_bounds(s, p, o, pats)
+        assert O.prescan_bounds(ix, pats) == want, pats
+        # only the permutations the default store has
+        assert O.prescan_bounds({k: ix[k] for k in ("PSO", "POS")}, pats) == want
+        narrowed += any(b != O.FULL_BOUND for b in want.values())
+    assert narrowed >= 3
+    # the reference's updateBoundDict rule on its own
+    b = {"?v": (5, 10)}
+    O.update_bound(b, "?v", 7, 20)
+    assert b["?v"] == (7, 10)
+    O.update_bound(b, "?v", 11, 30)  # disjoint
+    assert b["?v"] == (0, 0)
+
+
+def test_run_plan_with_bounds_is_result_neutral_and_smaller():
+    s, p, o = small_store()
+    o = o.copy()
+    o[p == 1] += 40
+    ix = {perm: O.build_perm(s, p, o, perm) for perm in ("PSO", "POS")}
+    smaller = 0
+    for pats, node in bounded_queries(s, p, o):
+        st0, st1 = {}, {}
+        h0, c0 = O.run_plan(ix, pats, node, st0)
+        h1, c1 = O.run_plan(ix, pats, node, st1, bounds=True)
+        want = O.naive_bgp(s, p, o, pats)
+        assert O.canonical(h0, c0) == want
+        assert O.canonical(h1, c1) == want
+        assert st1["select_rows"] <= st0["select_rows"]
+        assert st1["join_rows"] <= st0["join_rows"]
+        smaller += st1["select_rows"] < st0["select_rows"]
+    assert smaller >= 2
+
+
+def test_too_narrow_bound_is_caught():
+    """A clip narrower than the true bound changes the result: the
+    comparison with the naive evaluator would catch such a mistake."""
+    s, p, o = small_store()
+    ix = {perm: O.build_perm(s, p, o, perm) for perm in ("PSO", "POS")}
+    pats, node = plans.chain_plan([1, 2, 3])
+    real = O.prescan_bounds
+    try:
+        def narrow(ixp, pp):
+            b = real(ixp, pp)
+            return {v: (lo, max(lo, hi - 3)) for v, (lo, hi) in b.items()}
+        O.prescan_bounds = narrow
+        h, c = O.run_plan(ix, pats, node, bounds=True)
+    finally:
+        O.prescan_bounds = real
+    assert O.canonical(h, c) != O.naive_bgp(s, p, o, pats)

@@ -105,6 +105,9 @@ EXPORTS = {
                                   C.POINTER(C.c_void_p)]),
     "vd_index_lookup_multi": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                         C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "vd_index_bounds_multi": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                        C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                        C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "vd_join_multi": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                 C.c_int, C.POINTER(C.c_void_p)]),
     "vd_sort_col0_multi": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int]),
@@ -448,6 +451,20 @@ def lookup_multi(ctx: Ctx, index: "Index", jobs) -> list:
     ctx.check(ctx.L.vd_index_lookup_multi(ctx.h, index.h, C.cast(arr, C.c_void_p), len(jobs),
                                           b, e))
     return [(int(b[i]), int(e[i])) for i in range(len(jobs))]
+
+
+def bounds_multi(ctx: Ctx, index: "Index", jobs) -> list:
+    """jobs: [(perm name, k1, k2 or None)] -> [(begin, end, first, last)]:
+    the range and the first / last value of the column after the constants
+    (Y for one constant, Z for two), one launch."""
+    arr = (SelectJob * max(1, len(jobs)))(*[
+        SelectJob(PERMS[p], k1, VD_UNBOUND if k2 is None else k2, 1) for p, k1, k2 in jobs])
+    m = max(1, len(jobs))
+    b, e = (C.c_uint64 * m)(), (C.c_uint64 * m)()
+    f, l = (C.c_uint32 * m)(), (C.c_uint32 * m)()
+    ctx.check(ctx.L.vd_index_bounds_multi(ctx.h, index.h, C.cast(arr, C.c_void_p), len(jobs),
+                                          b, e, f, l))
+    return [(int(b[i]), int(e[i]), int(f[i]), int(l[i])) for i in range(len(jobs))]
 
 
 def join_multi(ctx: Ctx, lefts, rights) -> list:

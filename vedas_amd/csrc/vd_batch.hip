@@ -61,6 +61,28 @@ __global__ __launch_bounds__(256) void k_sel_lookup_multi(const SelJobDev *jobs,
     range[2 * j + 1] = e;
 }
 
+// The lookup of k_sel_lookup_multi plus the first and last value of the
+// column after the constants over the rows found (sorted there, so its
+// minimum and maximum): out[3j] = begin, out[3j+1] = end, out[3j+2] =
+// first | last << 32 (0 for an empty range).
+struct BoundJobDev {
+    PermDev pd;
+    const uint32_t *col;
+    uint32_t k1, k2;
+};
+
+__global__ __launch_bounds__(256) void k_sel_bounds_multi(const BoundJobDev *jobs, int nj,
+                                                          uint64_t *out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nj) return;
+    const BoundJobDev &J = jobs[j];
+    uint64_t b, e;
+    lookup_one(J.pd, J.k1, J.k2, J.k2 != VD_UNBOUND, &b, &e);
+    out[3 * j] = b;
+    out[3 * j + 1] = e;
+    out[3 * j + 2] = e > b ? ((uint64_t)J.col[b] | ((uint64_t)J.col[e - 1] << 32)) : 0;
+}
+
 // ---- join --------------------------------------------------------------------
 struct JPairDev {
     const uint32_t *lc[VD_MAX_COLS];
@@ -493,6 +515,25 @@ std::vector<uint64_t> lookup_multi(vd_ctx *ctx, const vd_index *idx, const vd_se
     return hr;
 }
 
+// Ranges and column end values of many lookups: hr[3j..3j+2] as
+// k_sel_bounds_multi writes them.
+std::vector<uint64_t> bounds_multi(vd_ctx *ctx, const vd_index *idx, const vd_select_job_t *jobs,
+                                   int nj) {
+    std::vector<BoundJobDev> hj(nj);
+    for (int j = 0; j < nj; ++j) {
+        const vd_perm_arrays &pa = get_perm(idx, jobs[j].perm);
+        hj[j] = BoundJobDev{perm_dev(pa), jobs[j].k2 == VD_UNBOUND ? pa.y : pa.z, jobs[j].k1,
+                            jobs[j].k2};
+    }
+    DBuf<BoundJobDev> dj = to_device(ctx, hj);
+    DBuf<uint64_t> out(ctx, 3 * (uint64_t)nj);
+    launch(ctx, "select_bounds", 32.0 * nj, dim3((unsigned)ceil_div(nj, 256)), dim3(256), 0,
+           k_sel_bounds_multi, (const BoundJobDev *)dj.p, nj, out.p);
+    std::vector<uint64_t> hr(3 * (size_t)nj);
+    download(ctx, hr.data(), out.p, hr.size() * sizeof(uint64_t));
+    return hr;
+}
+
 void select_multi(vd_ctx *ctx, const vd_index *idx, const vd_select_job_t *jobs, int nj,
                   vd_rel **outs) {
     for (int j = 0; j < nj; ++j)
@@ -776,6 +817,23 @@ int vd_index_lookup_multi(vd_ctx *ctx, const vd_index *idx, const vd_select_job_
         for (int j = 0; j < njobs; ++j) {
             begin[j] = hr[2 * j];
             end[j] = hr[2 * j + 1];
+        }
+    });
+}
+
+int vd_index_bounds_multi(vd_ctx *ctx, const vd_index *idx, const vd_select_job_t *jobs,
+                          int njobs, uint64_t *begin, uint64_t *end, uint32_t *first,
+                          uint32_t *last) {
+    if (!ctx || njobs < 0 || (njobs && (!jobs || !begin || !end || !first || !last)))
+        return VD_EINVAL;
+    if (njobs == 0) return VD_OK;
+    VD_TRY(ctx, {
+        const std::vector<uint64_t> hr = bounds_multi(ctx, idx, jobs, njobs);
+        for (int j = 0; j < njobs; ++j) {
+            begin[j] = hr[3 * j];
+            end[j] = hr[3 * j + 1];
+            first[j] = (uint32_t)hr[3 * j + 2];
+            last[j] = (uint32_t)(hr[3 * j + 2] >> 32);
         }
     });
 }

@@ -74,6 +74,8 @@ def lib() -> C.CDLL:
             "vdq_engine_open": (i, [vp, C.c_char_p]),
             "vdq_engine_index": (vp, [vp]),
             "vdq_query": (i, [vp, C.c_char_p, C.POINTER(vp)]),
+            "vdq_set_option": (i, [C.c_char_p, i]),
+            "vdq_get_option": (i, [C.c_char_p, C.POINTER(i)]),
             "vdq_result_shape": (i, [vp, C.POINTER(i), C.POINTER(u64)]),
             "vdq_result_header": (C.c_char_p, [vp, i]),
             "vdq_result_column": (i, [vp, i, vp]),
@@ -284,6 +286,17 @@ class Dict:
         if self.h:
             lib().vdq_dict_free(self.h)
             self.h = None
+
+
+def set_option(key: str, value) -> None:
+    """Executor switch ("bound_prescan", "bound_after_join"), process-wide."""
+    _check(lib().vdq_set_option(key.encode(), int(value)))
+
+
+def get_option(key: str) -> int:
+    v = C.c_int()
+    _check(lib().vdq_get_option(key.encode(), C.byref(v)))
+    return v.value
 
 
 def plan(text: str, est_rows=None) -> str:

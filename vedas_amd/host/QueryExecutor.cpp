@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cctype>
 #include <limits>
+#include <map>
 
 #include "IndexSwapJob.h"
 #include "JoinQueryJob.h"
@@ -10,6 +11,9 @@
 bool QueryExecutor::ENABLE_FULL_INDEX = false;
 // vdQuery and vdBench switch this on (VedasQuery.cu:37, VedasBench.cu:32).
 bool QueryExecutor::ENABLE_UPDATE_BOUND_AFTER_JOIN = true;
+// The pattern pre-scan of createVariableBound (QueryExecutor.cu:463-478);
+// off gives join variables the full range (for comparing intermediates).
+bool QueryExecutor::ENABLE_BOUND_PRESCAN = true;
 
 QueryExecutor::QueryExecutor(VedasStorage *s, VedasContext *c) : vedasStorage(s), context(c) {}
 
@@ -24,8 +28,24 @@ void QueryExecutor::updateBoundDict(VariableBoundMap *bound, const std::string &
     (*bound)[variable] = std::make_pair(new_min, new_max);
 }
 
-// Join variables start with the full range (QueryExecutor.cu:441-461); the
-// pattern pre-scan that narrows them is not performed.
+// QueryExecutor::createVariableBound, QueryExecutor.cu:441-479: every
+// variable that occurs more than once starts with the full range, then each
+// pattern narrows the bounds of its join variables to the first and last
+// value the variable takes over the pattern's index range
+// (squeezeQueryBound1Var / squeezeQueryBound2Var, :527-581, 680-735, via
+// updateDataBound / updateL2Bound, :648-675, 737-765), in pattern order;
+// a pattern with no rows sets every bound to [0, 0] (:470-475).  The
+// reference reads those ranges and values from host copies of the index;
+// here one vd_index_bounds_multi launch resolves every pattern.  For a
+// pattern with one variable the range is that of both constants and the
+// value column is the third one (POS for ?s p o, PSO for s p ?o, OSP for
+// s ?p o; the reference's alternative O-/S-first indexes hold the same
+// rows in the same order).  For two variables the range is that of the
+// constant, and each variable is read from the permutation that has it
+// right after the constant, the first variable of the reference's order
+// first (object then subject for ?s p ?o, object then predicate for
+// s ?p ?o, predicate then subject for ?s ?p o).  A permutation that was
+// not built gives no bound.
 void QueryExecutor::createVariableBound(std::vector<TriplePattern> &patterns) {
     std::map<std::string, int> count;
     for (auto &tp : patterns) {
@@ -38,6 +58,74 @@ void QueryExecutor::createVariableBound(std::vector<TriplePattern> &patterns) {
         if (c.second > 1)
             variables_bound[c.first] = std::make_pair(std::numeric_limits<TYPEID>::min(),
                                                       std::numeric_limits<TYPEID>::max());
+    if (!ENABLE_BOUND_PRESCAN || variables_bound.empty()) return;
+
+    struct Item {
+        size_t pattern;
+        std::string var;  // the variable the value column holds
+    };
+    std::vector<vd_select_job_t> jobs;
+    std::vector<Item> items;
+    auto add = [&](size_t i, const std::string &var, int perm, TYPEID k1, TYPEID k2) {
+        if (!vedasStorage->hasPermutation(perm)) return false;
+        vd_select_job_t j{};
+        j.perm = perm;
+        j.k1 = k1;
+        j.k2 = k2;
+        j.cols = 0;
+        jobs.push_back(j);
+        items.push_back(Item{i, var});
+        return true;
+    };
+    for (size_t i = 0; i < patterns.size(); ++i) {
+        TriplePattern &tp = patterns[i];
+        const TYPEID s = tp.getSubjectId(), p = tp.getPredicateId(), o = tp.getObjectId();
+        if (tp.getVariableNum() == 1) {
+            if (tp.subjectIsVariable()) {
+                add(i, tp.getSubject(), VD_PERM_POS, p, o) ||
+                    add(i, tp.getSubject(), VD_PERM_OPS, o, p);
+            } else if (tp.predicateIsVariable()) {
+                add(i, tp.getPredicate(), VD_PERM_OSP, o, s) ||
+                    add(i, tp.getPredicate(), VD_PERM_SOP, s, o);
+            } else {
+                add(i, tp.getObject(), VD_PERM_PSO, p, s) ||
+                    add(i, tp.getObject(), VD_PERM_SPO, s, p);
+            }
+        } else if (tp.getVariableNum() == 2) {
+            switch (tp.getVariableBitmap()) {
+                case 5:  // ?s p ?o
+                    add(i, tp.getObject(), VD_PERM_POS, p, VD_UNBOUND);
+                    add(i, tp.getSubject(), VD_PERM_PSO, p, VD_UNBOUND);
+                    break;
+                case 3:  // s ?p ?o
+                    add(i, tp.getObject(), VD_PERM_SOP, s, VD_UNBOUND);
+                    add(i, tp.getPredicate(), VD_PERM_SPO, s, VD_UNBOUND);
+                    break;
+                case 6:  // ?s ?p o
+                    add(i, tp.getPredicate(), VD_PERM_OPS, o, VD_UNBOUND);
+                    add(i, tp.getSubject(), VD_PERM_OSP, o, VD_UNBOUND);
+                    break;
+            }
+        }
+    }
+    if (jobs.empty()) return;
+    const size_t nj = jobs.size();
+    std::vector<uint64_t> b(nj), e(nj);
+    std::vector<uint32_t> first(nj), last(nj);
+    context->check(vd_index_bounds_multi(context->raw(), vedasStorage->index(), jobs.data(),
+                                         (int)nj, b.data(), e.data(), first.data(), last.data()),
+                   "vd_index_bounds_multi");
+    bool zeroResult = false;
+    for (size_t k = 0; k < nj; ++k) {
+        if (e[k] == b[k]) {
+            zeroResult = true;
+            continue;
+        }
+        if (variables_bound.count(items[k].var) > 0)
+            updateBoundDict(&variables_bound, items[k].var, first[k], last[k]);
+    }
+    if (zeroResult)
+        for (auto &vb : variables_bound) vb.second = std::make_pair(0u, 0u);
 }
 
 void QueryExecutor::query(std::vector<TriplePattern> &patterns, const std::set<std::string> &selected,

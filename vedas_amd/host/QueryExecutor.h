@@ -1,9 +1,10 @@
 // Job factory and plan driver (reference QueryExecutor.h:38-125, the
-// single-GPU path).  The plan tree is given by the caller; the reference's
-// planner (QueryGraph) and bound pre-scan are not part of this library.
+// single-GPU path): the bound pre-scan, then the plan tree's jobs (the tree
+// comes from the caller or the QueryGraph planner).
 #ifndef VEDAS_HOST_QUERYEXECUTOR_H
 #define VEDAS_HOST_QUERYEXECUTOR_H
 
+#include <map>
 #include <set>
 #include <string>
 #include <vector>
@@ -27,6 +28,9 @@ class QueryExecutor {
                                 TYPEID lowerBound, TYPEID upperBound);
     static bool ENABLE_FULL_INDEX;
     static bool ENABLE_UPDATE_BOUND_AFTER_JOIN;
+    static bool ENABLE_BOUND_PRESCAN;
+    // Bounds of the join variables after the last query (for inspection).
+    const VariableBoundMap &bounds() const { return variables_bound; }
 
     SelectQueryJob *createSelectQueryJob(TriplePattern *pattern, std::string index_used = "");
     SelectQueryJob *create1VarSelectQueryJob(TriplePattern *pattern);

@@ -662,6 +662,32 @@ int vdq_result_free(vdq_result *r) {
     return guarded([&] { delete r; });
 }
 
+int vdq_set_option(const char *key, int value) {
+    if (!key) return fail(VD_EINVAL, "null option name");
+    const std::string k(key);
+    if (k == "bound_prescan") {
+        QueryExecutor::ENABLE_BOUND_PRESCAN = value != 0;
+    } else if (k == "bound_after_join") {
+        QueryExecutor::ENABLE_UPDATE_BOUND_AFTER_JOIN = value != 0;
+    } else {
+        return fail(VD_EINVAL, "unknown option " + k);
+    }
+    return VD_OK;
+}
+
+int vdq_get_option(const char *key, int *value) {
+    if (!key || !value) return fail(VD_EINVAL, "null argument");
+    const std::string k(key);
+    if (k == "bound_prescan") {
+        *value = QueryExecutor::ENABLE_BOUND_PRESCAN;
+    } else if (k == "bound_after_join") {
+        *value = QueryExecutor::ENABLE_UPDATE_BOUND_AFTER_JOIN;
+    } else {
+        return fail(VD_EINVAL, "unknown option " + k);
+    }
+    return VD_OK;
+}
+
 const char *vdq_last_error(void) { return g_err.c_str(); }
 
 }  // extern "C"

@@ -25,6 +25,16 @@ STORE_100M = dict(n=100_000_000, n_entities=10_000_000, subject_zipf=0.8, n_pred
 STORE_500M = dict(n=500_000_000, n_entities=50_000_000, subject_zipf=0.8, n_predicates=1024,
                   pred_zipf=1.1, literal_frac=0.4, literal_base=60_000_000,
                   n_literals=250_000_000)
+# Seeds of the benchmark workloads.  Every store is drawn with distinct
+# triples, so a query's result is its set of bindings whatever the plan.
+SEED_100M = 42            # the 100M store (index build, selects, join set, mix)
+SEED_SELECTS = 7          # the 10k range selects on it
+SEED_MIX = 3              # the 256-query batched mix on it
+SEED_500M = 142           # the 500M served store
+SEED_SERVE_MIX = 143      # its 1024-request mix
+N_SELECTS = 10_000
+N_MIX = 256
+N_SERVE = 1024
 # 1M-triple store of the CPU-runnable case
 STORE_1M = dict(n=1_000_000, n_entities=100_000, subject_zipf=0.0, n_predicates=64,
                 pred_zipf=1.1, literal_frac=0.5, literal_base=50_000_000, n_literals=500_000)
@@ -200,3 +210,48 @@ def query_mix(n_queries: int, seed: int, prm: dict, s=None, p=None, o=None,
             pats, node = _chain_query(preds)
             accept((f"chain{L}", pats, node))
     return out
+
+
+# Permutations and constant positions of the range selects: one bound
+# position (S -> SPO, P -> PSO, O -> OSP) or two ((S, P) -> SPO,
+# (P, O) -> POS, (O, S) -> OSP).
+SELECT_PERMS = ("SPO", "PSO", "POS", "OSP")
+
+
+def select_mix(nq: int, seed: int, s, p, o):
+    """Range selects with one or two constants taken from uniformly drawn
+    existing triples (50/50, positions uniform), so none is empty.  Returns
+    {perm: (k1 array, k2 array)} with 0xFFFFFFFF for "no second constant",
+    each permutation's selects in draw order."""
+    rng = np.random.default_rng(seed)
+    pick = rng.integers(0, len(s), nq)
+    two = rng.random(nq) < 0.5
+    pos = rng.integers(0, 3, nq)
+    U = 0xFFFFFFFF
+    groups = {k: ([], []) for k in SELECT_PERMS}
+    for i, t in enumerate(pick):
+        S, P, O = int(s[t]), int(p[t]), int(o[t])
+        if not two[i]:
+            perm, k1, k2 = [("SPO", S, U), ("PSO", P, U), ("OSP", O, U)][pos[i]]
+        else:
+            perm, k1, k2 = [("SPO", S, P), ("POS", P, O), ("OSP", O, S)][pos[i]]
+        groups[perm][0].append(k1)
+        groups[perm][1].append(k2)
+    return {k: (np.array(a, np.uint32), np.array(b, np.uint32)) for k, (a, b) in groups.items()
+            if a}
+
+
+def chunk_by_memory(eng, texts, budget):
+    """Consecutive groups of queries whose estimated peak HBM bytes (store
+    statistics) sum to at most `budget`."""
+    groups, cur, acc = [], [], 0.0
+    for t in texts:
+        need = eng.estimate(t)[1]
+        if cur and acc + need > budget:
+            groups.append(cur)
+            cur, acc = [], 0.0
+        cur.append(t)
+        acc += need
+    if cur:
+        groups.append(cur)
+    return groups
