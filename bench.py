@@ -1,9 +1,9 @@
 #!/usr/bin/env python3
 """Benchmark of the VEDAS query path on one MI355X.
 
-Store (synthetic, seeded): 100M triples -- 10M entities with Zipf(0.8)
-subjects, 256 predicates Zipf(1.1), objects 60% entities / 40% literals --
-indexed into SPO, PSO, POS and OSP, all resident in HBM.
+Store (synthetic, seeded): 100M distinct triples -- 10M entities with
+Zipf(0.8) subjects, 256 predicates Zipf(1.1), objects 60% entities / 40%
+literals -- indexed into SPO, PSO, POS and OSP, all resident in HBM.
 
 Headline (`value`): join output rows per second of the single-query join
 set run through the drop-in operator API (QueryExecutor -> Select / Join /
@@ -32,6 +32,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from vedas_amd import workload as W  # noqa: E402
 from vedas_amd.workload import JOIN_SET, STORE_100M, STORE_500M, query_mix  # noqa: E402
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
@@ -50,13 +51,45 @@ def index_bytes(n: int, kbits: int) -> float:
     return passes * 2 * 8 * n + 20 * n + 16 * n
 
 
+_M1 = np.uint64(0xFF51AFD7ED558CCD)
+_M2 = np.uint64(0xC4CEB9FE1A85EC53)
+
+
+def _mix(h):
+    h ^= h >> np.uint64(33)
+    h *= _M1
+    h ^= h >> np.uint64(33)
+    h *= _M2
+    h ^= h >> np.uint64(33)
+    return h
+
+
+def result_digest(headers, cols, chunk=1 << 22):
+    """Order-independent digest of a result's rows (columns ordered by
+    variable name): (names, rows, sum of a 64-bit row hash).  Used to check
+    that two runs gave the same bag of rows."""
+    order = sorted(range(len(headers)), key=lambda i: headers[i])
+    names = tuple(headers[i] for i in order)
+    n = len(cols[0]) if cols else 0
+    total = 0
+    with np.errstate(over="ignore"):
+        for a in range(0, n, chunk):
+            h = np.full(min(chunk, n - a), 0x243F6A8885A308D3, np.uint64)
+            for c, i in enumerate(order):
+                h = _mix(h ^ (cols[i][a:a + chunk].astype(np.uint64) +
+                              np.uint64((0x9E3779B97F4A7C15 * (c + 1)) & 0xFFFFFFFFFFFFFFFF)))
+            total = (total + int(h.sum(dtype=np.uint64))) & 0xFFFFFFFFFFFFFFFF
+    return names, n, total
+
+
 def kernel_table(prof):
     return {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
                 "GB/s": round(v["bytes"] / (v["ms"] / 1e3) / 1e9, 1) if v["ms"] > 0 else None}
             for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])}
 
 
-def select_bench(ctx, idx, s, p, o, nq=10_000, seed=7, max_chunk_rows=400_000_000):
+def select_bench(ctx, idx, s, p, o, nq=W.N_SELECTS, seed=W.SEED_SELECTS,
+                 max_chunk_rows=400_000_000):
     """10k range selects with one or two constants taken from uniformly drawn
     existing triples: S -> SPO, P -> PSO, O -> OSP, (S,P) -> SPO, (P,O) -> POS,
     (O,S) -> OSP.  Each permutation's queries run as batched selects whose
@@ -64,26 +97,9 @@ def select_bench(ctx, idx, s, p, o, nq=10_000, seed=7, max_chunk_rows=400_000_00
     max_chunk_rows rows."""
     import ctypes as C
     from vedas_amd import _native
-    rng = np.random.default_rng(seed)
-    pick = rng.integers(0, len(s), nq)
-    two = rng.random(nq) < 0.5
-    pos = rng.integers(0, 3, nq)
-    groups = {"SPO": ([], []), "PSO": ([], []), "POS": ([], []), "OSP": ([], [])}
-    U = _native.VD_UNBOUND
-    for i, t in enumerate(pick):
-        S, P, O = int(s[t]), int(p[t]), int(o[t])
-        if not two[i]:
-            perm, k1, k2 = [("SPO", S, U), ("PSO", P, U), ("OSP", O, U)][pos[i]]
-        else:
-            perm, k1, k2 = [("SPO", S, P), ("POS", P, O), ("OSP", O, S)][pos[i]]
-        groups[perm][0].append(k1)
-        groups[perm][1].append(k2)
+    groups = W.select_mix(nq, seed, s, p, o)
     plans_ = []
     for perm, (k1, k2) in groups.items():
-        if not k1:
-            continue
-        k1 = np.array(k1, np.uint32)
-        k2 = np.array(k2, np.uint32)
         d1, d2 = ctx.upload(k1), ctx.upload(k2)
         db, de = ctx.alloc(8 * len(k1)), ctx.alloc(8 * len(k1))
         ctx.check(ctx.L.vd_index_lookup_batch(ctx.h, idx.h, _native.PERMS[perm], C.c_void_p(d1),
@@ -125,28 +141,11 @@ def select_bench(ctx, idx, s, p, o, nq=10_000, seed=7, max_chunk_rows=400_000_00
             "rows_per_s": rows / dt, "GB/s": 16.0 * rows / dt / 1e9, "chunks": len(staged)}
 
 
-def chunk_by_memory(eng, texts, budget):
-    """Consecutive groups of queries whose estimated peak HBM bytes (store
-    statistics) sum to at most `budget`."""
-    groups, cur, acc = [], [], 0.0
-    for t in texts:
-        need = eng.estimate(t)[1]
-        if cur and acc + need > budget:
-            groups.append(cur)
-            cur, acc = [], 0.0
-        cur.append(t)
-        acc += need
-    if cur:
-        groups.append(cur)
-    return groups
-
-
 def batch_bench(eng, texts, reps=2, budget=96 << 30):
     """The same query mix one query at a time (plain drop-in path) and in
     batches (as many queries per batch as the memory estimate allows);
     results compared canonically."""
-    from oracle.pyoracle import canonical  # checker only
-    groups = chunk_by_memory(eng, texts, budget)
+    groups = W.chunk_by_memory(eng, texts, budget)
     plain_t = []
     for _ in range(reps):
         t0 = time.perf_counter()
@@ -182,7 +181,8 @@ def batch_bench(eng, texts, reps=2, budget=96 << 30):
         res, _ = eng.batch(g)
         for t, r in zip(g, res):
             a = eng.query(t)
-            same += canonical(a.headers(), a.columns()) == canonical(r.headers(), r.columns())
+            same += result_digest(a.headers(), a.columns()) == result_digest(r.headers(),
+                                                                              r.columns())
             a.free()
             r.free()
     pt, bt = min(plain_t), min(batch_t)
@@ -228,17 +228,16 @@ def serve_bench(eng, texts, qps, seed=1, slots=4, max_batch=128, budget=96 << 30
     return out
 
 
-def serve_section(store: dict, loads, n_queries: int, seed: int, check: int = 32):
-    """The served form: a fresh engine over its own store (all permutations
-    resident), the seeded query mix drawn on that store, served at each
-    offered load; then `check` requests compared canonically between the
-    served mix and the same query run alone."""
-    from oracle.pyoracle import canonical  # checker only
+def serve_section(store: dict, loads, n_queries: int, seed: int, mix_seed: int):
+    """The served form: a fresh engine over its own store (distinct triples,
+    all permutations resident), the seeded query mix drawn on that store,
+    served at each offered load; then every request's served result
+    compared (bag digest) with the same query run alone."""
     from vedas_amd import _native
     from vedas_amd.engine import Engine, query_text
     prm = {k: v for k, v in store.items() if k != "n"}
     log(f"[bench] serving store: generating {store['n']} triples")
-    s, p, o = _native.gen_triples(store["n"], seed=seed, distinct=False, **prm)
+    s, p, o = _native.gen_triples(store["n"], seed=seed, distinct=True, **prm)
     eng = Engine(0)
     ctx = _native.Ctx.wrap(eng.ctx_handle())
     ds, dp, do = ctx.upload(s), ctx.upload(p), ctx.upload(o)
@@ -250,7 +249,7 @@ def serve_section(store: dict, loads, n_queries: int, seed: int, check: int = 32
         ctx.free(d)
     kbits = int(s.max()).bit_length() + int(p.max()).bit_length() + int(o.max()).bit_length()
     log(f"[bench] serving store built in {build_ms:.0f} ms ({kbits}-bit keys)")
-    mix = query_mix(n_queries, seed + 1, store, s, p, o)
+    mix = query_mix(n_queries, mix_seed, store, s, p, o)
     del s, p, o
     texts = [query_text(pats, node) for _, pats, node in mix]
     for r in eng.batch(texts[:16])[0]:  # warm-up
@@ -268,42 +267,92 @@ def serve_section(store: dict, loads, n_queries: int, seed: int, check: int = 32
     arr = list(np.cumsum(rng.exponential(1000.0 / max(loads), len(texts))))
     _, outs = eng.serve(texts, arr, keep_results=True)
     same = 0
-    for i in range(min(check, len(texts))):
+    for i in range(len(texts)):
         a = eng.query(texts[i])
-        same += canonical(a.headers(), a.columns()) == canonical(outs[i].headers(),
-                                                                  outs[i].columns())
+        same += result_digest(a.headers(), a.columns()) == result_digest(outs[i].headers(),
+                                                                          outs[i].columns())
         a.free()
     for r in outs:
         r.free()
     eng.close()
-    return {"store": {**store, "seed": seed, "key_bits": kbits}, "build_ms": build_ms,
-            "queries": len(texts), "loads": loads_out,
-            "results_equal_alone": f"{same}/{min(check, len(texts))}"}
+    return {"store": {**store, "seed": seed, "distinct": True, "key_bits": kbits},
+            "build_ms": build_ms, "queries": len(texts), "loads": loads_out,
+            "results_equal_alone": f"{same}/{len(texts)}"}
 
 
-def cpu_baseline(idx, names, sample_seconds: float):
-    """The join set on the host: the oracle's plan walk (C restatement of
-    select / join / IndexSwap, one thread) over the PSO and POS arrays of
-    the same store (downloaded from the device build, since the index build
-    is not part of the timed step).  Queries run in order until the sample
-    reaches sample_seconds (at least one query); join rows/s over them."""
-    from oracle import pyoracle as O
+def host_info() -> dict:
+    """Cores, CPU model and NUMA layout of the host the baseline runs on."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    try:
+        with open("/proc/cpuinfo") as f:
+            models = [ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")]
+        info["cpu_model"] = models[0] if models else None
+    except OSError:
+        info["cpu_model"] = None
+    node_dir = "/sys/devices/system/node"
+    try:
+        nodes = sorted(d for d in os.listdir(node_dir) if d.startswith("node") and d[4:].isdigit())
+        info["numa_nodes"] = len(nodes)
+        info["numa_cpus"] = {d: open(os.path.join(node_dir, d, "cpulist")).read().strip()
+                             for d in nodes}
+    except OSError:
+        info["numa_nodes"] = None
+    info["OMP_NUM_THREADS"] = os.environ.get("OMP_NUM_THREADS")
+    return info
+
+
+def cpu_baseline(idx, names, gpu_results: dict, runs: int = 5):
+    """The join set on the host cores: the multi-threaded CPU port of the
+    operators (select with the bound clip, join, stable re-sort, adjacent
+    unique; bit-equal to the oracle by tests/test_cpu_port.py) walking the
+    same plans with the reference's join-variable bounds, over the PSO and
+    POS arrays of the same store (downloaded from the device build: the
+    index build is not part of the timed step).  One warm-up run, whose
+    results are checked against the GPU's (bag digest, join rows, final
+    rows), then `runs` timed runs of the whole join set; the value is from
+    the median run, the minimum is reported beside it (vdBench reports the
+    minimum, VedasBench.cu:182-189)."""
+    from oracle import cpu_port as P
+    ops = P.make_ops()
     ix = {perm: idx.arrays(perm) for perm in ("PSO", "POS")}
-    done, rows, el = [], 0, 0.0
+    parity, rows = {}, 0
     for q in names:
         pats, node = JOIN_SET[q]
         st = {}
         t0 = time.perf_counter()
-        O.run_plan(ix, pats, node, st)
-        el += time.perf_counter() - t0
+        h, cols = P.run_query(ix, pats, node, ops, st)
+        dt = time.perf_counter() - t0
+        g = gpu_results[q]
+        d = result_digest(h, cols)
+        parity[q] = {"digest_equal": d == g["digest"], "rows": d[1], "gpu_rows": g["rows"],
+                     "join_rows": st["join_rows"], "gpu_join_rows": g["join_rows"],
+                     "select_rows": st["select_rows"], "gpu_select_rows": g["select_rows"],
+                     "join_rows_equal": st["join_rows"] == g["join_rows"]}
         rows += st["join_rows"]
-        done.append(f"{q} ({st['join_rows']} join rows)")
-        log(f"[bench] cpu baseline: {q} done, {el:.1f} s so far")
-        if el >= sample_seconds:
-            break
-    return {"value": rows / max(el, 1e-9), "unit": "join rows/s", "cores": 1, "kind": "port",
-            "sample": "oracle plan walk (C restatement, 1 thread) of " + ", ".join(done) +
-                      f" on the same 100M-triple store: {rows} join rows in {el:.1f} s"}
+        del cols
+        log(f"[bench] cpu baseline warm-up: {q} {dt:.2f} s, equal to the GPU's: "
+            f"{parity[q]['digest_equal'] and parity[q]['join_rows_equal']}")
+    times = []
+    for k in range(runs):
+        t0 = time.perf_counter()
+        for q in names:
+            pats, node = JOIN_SET[q]
+            P.run_query(ix, pats, node, ops, {})
+        times.append(time.perf_counter() - t0)
+        log(f"[bench] cpu baseline run {k}: {times[-1]:.2f} s")
+    med, mn = float(np.median(times)), min(times)
+    return {"value": rows / med, "unit": "join rows/s", "cores": ops.threads, "kind": "port",
+            "sample": f"the join set ({', '.join(names)}; {rows} join rows) on the same "
+                      f"100M-triple store by the multi-threaded CPU port on {ops.threads} "
+                      f"threads; median of {runs} runs after one warm-up",
+            "runs_s": times, "median_s": med, "min_s": mn, "value_at_min": rows / mn,
+            "host": host_info(),
+            "parity_vs_gpu": parity,
+            "all_equal": all(v["digest_equal"] and v["join_rows_equal"] for v in parity.values())}
 
 
 def main():
@@ -312,16 +361,16 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--triples", type=int, default=STORE_100M["n"])
-    ap.add_argument("--seed", type=int, default=42)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--seed", type=int, default=W.SEED_100M)
+    ap.add_argument("--cpu-runs", type=int, default=5, help="timed runs of the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default kernel tunables")
     ap.add_argument("--queries", default="star4,chain4,snowflake6")
-    ap.add_argument("--selects", type=int, default=10_000)
-    ap.add_argument("--mix", type=int, default=256, help="queries in the batched mix (0: skip)")
+    ap.add_argument("--selects", type=int, default=W.N_SELECTS)
+    ap.add_argument("--mix", type=int, default=W.N_MIX, help="queries in the batched mix (0: skip)")
     ap.add_argument("--serve-qps", type=float, nargs="*", default=[2000.0, 50000.0],
                     help="offered loads of the open-loop arrival stream (none: skip)")
-    ap.add_argument("--serve-requests", type=int, default=1024,
+    ap.add_argument("--serve-requests", type=int, default=W.N_SERVE,
                     help="requests in the served arrival stream")
     ap.add_argument("--serve-store", choices=["500M", "100M"], default="500M",
                     help="store of the served form")
@@ -333,11 +382,11 @@ def main():
         sys.exit(2)
 
     from vedas_amd import _native
-    from vedas_amd.engine import Engine, query_text
+    from vedas_amd.engine import Engine, query_text, set_option
 
-    log(f"[bench] generating {args.triples} triples")
+    log(f"[bench] generating {args.triples} distinct triples")
     t = time.perf_counter()
-    s, p, o = _native.gen_triples(args.triples, seed=args.seed, distinct=False, **GEN)
+    s, p, o = _native.gen_triples(args.triples, seed=args.seed, distinct=True, **GEN)
     log(f"[bench] generated in {time.perf_counter() - t:.1f}s")
 
     eng = Engine(0)
@@ -399,16 +448,25 @@ def main():
     for w in range(args.warmup):
         log(f"[bench] warmup {w}")
         step()
-    # One more untimed step with every launch timed: the per-kernel table of
-    # a step and the dominant kernel.  The timed steps then time only the
-    # dominant kernel's launches with HIP events (each timed launch adds two
-    # events to the stream, about 1 ms per step when every launch is timed).
+    # One more untimed step with every launch timed, query by query: the
+    # per-kernel table of a step, each query's algorithmic bytes, and the
+    # dominant kernel.  The timed steps then time only the dominant kernel's
+    # launches with HIP events (each timed launch adds two events to the
+    # stream, about 1 ms per step when every launch is timed).
     ctx.sync()
     ctx.prof(True)
-    ctx.prof_reset()
-    step()
-    ctx.sync()
-    step_prof = ctx.prof_stats()
+    step_prof, q_bytes = {}, {}
+    for q in names:
+        ctx.prof_reset()
+        r = eng.query(texts[q])
+        r.free()
+        ctx.sync()
+        qp = ctx.prof_stats()
+        q_bytes[q] = sum(v["bytes"] for v in qp.values())
+        for k, v in qp.items():
+            acc = step_prof.setdefault(k, {"launches": 0, "ms": 0.0, "bytes": 0.0})
+            for f in ("launches", "ms", "bytes"):
+                acc[f] += v[f]
     dom_name = max(step_prof.items(), key=lambda kv: kv[1]["ms"])[0]
     ctx.prof_only([dom_name])
     ctx.prof_reset()
@@ -444,26 +502,43 @@ def main():
             roofline["traffic_over_alg"] = measured / alg_per_launch if alg_per_launch else None
             roofline["traffic_source"] = os.path.relpath(traffic_file, ROOT)
 
-    # The same queries planned by the query graph planner (no PLAN clause):
-    # time and result size beside the given plans (outside the timed steps).
-    # Final rows can differ between plans when the store has duplicate
-    # triples: the finalise step removes adjacent duplicates only.
-    planned = {}
+    # Outside the timed steps: each query's result digest (checked against
+    # the CPU baseline's), its intermediates without the bound pre-scan,
+    # and the same query planned by the query graph planner (no PLAN).
+    gpu_results, planned, prescan = {}, {}, {}
     for q in names:
-        pats = JOIN_SET[q][0]
-        text = query_text(pats, None)
-        r = eng.query(text)  # warm
-        r.free()
-        r = eng.query(text)
+        r = eng.query(texts[q])
         st = r.stats()
+        gpu_results[q] = {"digest": result_digest(r.headers(), r.columns()), "rows": st["rows"],
+                          "join_rows": st["join_rows"], "select_rows": st["select_rows"]}
         r.free()
-        planned[q] = {"ms": st["elapsed_ms"], "rows": st["rows"], "join_rows": st["join_rows"],
-                      "rows_given_plan": per_q[q][0]["rows"]}
+        set_option("bound_prescan", 0)
+        try:
+            r = eng.query(texts[q])
+            off = r.stats()
+            off_digest = result_digest(r.headers(), r.columns())
+            r.free()
+        finally:
+            set_option("bound_prescan", 1)
+        prescan[q] = {"select_rows": st["select_rows"], "join_rows": st["join_rows"],
+                      "select_rows_without": off["select_rows"],
+                      "join_rows_without": off["join_rows"], "ms_without": off["elapsed_ms"],
+                      "same_result": off_digest == gpu_results[q]["digest"]}
+        pq = query_text(JOIN_SET[q][0], None)
+        r = eng.query(pq)  # warm
+        r.free()
+        r = eng.query(pq)
+        pst = r.stats()
+        planned[q] = {"ms": pst["elapsed_ms"], "rows": pst["rows"], "join_rows": pst["join_rows"],
+                      "rows_given_plan": st["rows"],
+                      "same_result": result_digest(r.headers(), r.columns()) ==
+                      gpu_results[q]["digest"]}
+        r.free()
     log("[bench] planner: " + ", ".join(f"{q} {v['ms']:.1f} ms" for q, v in planned.items()))
 
     batch = serve = None
     if args.mix > 0:
-        mix = query_mix(args.mix, 3, STORE_100M, s, p, o)
+        mix = query_mix(args.mix, W.SEED_MIX, STORE_100M, s, p, o)
         mtexts = [query_text(pats, node) for _, pats, node in mix]
         log(f"[bench] batched mix of {len(mtexts)} queries")
         for r in eng.batch(mtexts[:16])[0]:  # warm-up
@@ -474,19 +549,23 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline")
-        cpu = cpu_baseline(idx_view, names, args.cpu_seconds)
+        cpu = cpu_baseline(idx_view, names, gpu_results, runs=args.cpu_runs)
 
     if args.serve_qps:
         eng.close()  # the served form runs on its own store
-        serve = serve_section(STORE_500M if args.serve_store == "500M" else STORE_100M,
-                              args.serve_qps, args.serve_requests, args.seed + 100)
+        big = args.serve_store == "500M"
+        serve = serve_section(STORE_500M if big else STORE_100M, args.serve_qps,
+                              args.serve_requests, W.SEED_500M if big else args.seed,
+                              W.SEED_SERVE_MIX)
 
     queries = {}
     for q, sts in per_q.items():
         ms = float(np.median([x["elapsed_ms"] for x in sts]))
         jr = sts[0]["join_rows"]
         queries[q] = {"ms_median": ms, "join_rows": jr, "rows": sts[0]["rows"],
-                      "join_rows_per_s": jr / (ms / 1e3), "swap_rows": sts[0]["swap_rows"]}
+                      "select_rows": sts[0]["select_rows"], "swap_rows": sts[0]["swap_rows"],
+                      "join_rows_per_s": jr / (ms / 1e3), "alg_bytes": q_bytes[q],
+                      "GB/s": q_bytes[q] / (ms / 1e3) / 1e9}
     out = {
         "metric": "join rows/s (single-query join set on the 100M-triple store)",
         "value": total_rows / dt,
@@ -494,14 +573,17 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": dt / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "u32", "data": "synthetic (seeded generator, duplicates kept)",
+        "dtype": "u32", "data": "synthetic (seeded generator, distinct triples)",
         "config": {"workload": "single-query joins (star4, chain4, snowflake6) on 100M triples",
-                   "triples": n, "entities": GEN["n_entities"], "predicates": GEN["n_predicates"],
-                   "subject_zipf": GEN["subject_zipf"], "pred_zipf": GEN["pred_zipf"],
-                   "literal_frac": GEN["literal_frac"], "queries": names, "seed": args.seed},
+                   "triples": n, "distinct": True, "entities": GEN["n_entities"],
+                   "predicates": GEN["n_predicates"], "subject_zipf": GEN["subject_zipf"],
+                   "pred_zipf": GEN["pred_zipf"], "literal_frac": GEN["literal_frac"],
+                   "queries": names, "seed": args.seed, "bound_prescan": True},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "queries": queries,
+        "queries_note": "GB/s = algorithmic operator bytes of the query / its median wall time",
+        "bound_prescan": prescan,
         "index_build": {"ms": build_s * 1e3, "permutations": list(perms), "key_bits": kb,
                         "alg_bytes": build_alg, "GB/s": build_alg / build_s / 1e9,
                         "frac": build_alg / build_s / HBM_PEAK, "keys_per_s": 4 * n / build_s,

@@ -206,7 +206,7 @@ def prescan_bounds(ix_by_perm: dict, patterns) -> dict:
 
 
 def run_plan(ix_by_perm: dict, patterns, node, stats: dict | None = None,
-             bounds: bool = False):
+             bounds: bool = False, ops=None):
     """Execute a plan tree over the oracle: QueryExecutor::postorderTraversal
     (QueryExecutor.cu:106-135) with the index choice of
     create1VarSelectQueryJob / create2VarSelectQueryJob
@@ -225,7 +225,12 @@ def run_plan(ix_by_perm: dict, patterns, node, stats: dict | None = None,
     (JoinQueryJob.cu:388-397, 434-437); the empty-interval filter is not
     restated (it acts only after joins of fewer than EIF_ALPHA = 100 rows and
     only shrinks later selects).  stats (optional) accumulates "join_rows",
-    "select_rows" and "swap_rows".  Returns (headers, columns)."""
+    "select_rows" and "swap_rows".  ops (optional) supplies other
+    implementations of select / join / sort_col0 with these functions'
+    signatures (the multi-threaded CPU port).  Returns (headers, columns)."""
+    _select = ops.select if ops is not None else select
+    _join = ops.join if ops is not None else join
+    _sort_col0 = ops.sort_col0 if ops is not None else sort_col0
     st = stats if stats is not None else {}
     st.setdefault("join_rows", 0)
     st.setdefault("select_rows", 0)
@@ -240,7 +245,7 @@ def run_plan(ix_by_perm: dict, patterns, node, stats: dict | None = None,
         h[0], h[i] = h[i], h[0]
         cols[0], cols[i] = cols[i], cols[0]
         st["swap_rows"] += len(cols[0])
-        return h, sort_col0(cols)
+        return h, _sort_col0(cols)
 
     def upload(nd):
         pat = patterns[nd[1]] if isinstance(nd[1], int) else nd[1]
@@ -251,7 +256,7 @@ def run_plan(ix_by_perm: dict, patterns, node, stats: dict | None = None,
             ix = ix_by_perm[perm]
             b, e = lookup(ix, pv)
             clip = bnd[hdr[0]] if bnd is not None and hdr[0] in bnd else None
-            return hdr, select(ix, b, e, 3, clip=clip)
+            return hdr, _select(ix, b, e, 3, clip=clip)
         if isinstance(sv, str):
             ix = ix_by_perm["POS"]
             b, e = lookup(ix, pv, ov)
@@ -262,7 +267,7 @@ def run_plan(ix_by_perm: dict, patterns, node, stats: dict | None = None,
             hdr = [ov]
         if bnd is not None and hdr[0] in bnd and b == e:
             update_bound(bnd, hdr[0], 0, 0)
-        return hdr, select(ix, b, e, 1)
+        return hdr, _select(ix, b, e, 1)
 
     def walk(nd):
         op = nd[0]
@@ -276,7 +281,7 @@ def run_plan(ix_by_perm: dict, patterns, node, stats: dict | None = None,
                 a, b = b, a
             ha, ca = key_first(a[0], a[1], nd[3])
             hb, cb = key_first(b[0], b[1], nd[3])
-            out = join(ca, cb, 1)
+            out = _join(ca, cb, 1)
             st["join_rows"] += len(out[0])
             if bnd is not None and nd[3] in bnd:
                 if len(out[0]):

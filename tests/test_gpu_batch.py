@@ -45,6 +45,27 @@ def test_join_multi(ctx, nv):
     ctx.tune("join_nv", 2048)
 
 
+def test_join_multi_many_tiny_pairs(ctx):
+    """More pairs than one chunk of the batched join (4096): 5000 joins of
+    0-6 rows each, results equal to the oracle's pair by pair."""
+    from vedas_amd._native import Rel, join_multi
+    rng = np.random.default_rng(77)
+    Ls, Rs, want = [], [], []
+    for _ in range(5000):
+        L = rand_rel(rng, int(rng.integers(0, 7)), 2, 5)
+        R = rand_rel(rng, int(rng.integers(0, 7)), 2, 5)
+        Ls.append(Rel.from_columns(ctx, L))
+        Rs.append(Rel.from_columns(ctx, R))
+        want.append(O.join(L, R, 1))
+    outs = join_multi(ctx, Ls, Rs)
+    for j, (o, w) in enumerate(zip(outs, want)):
+        assert o.nrows == len(w[0]), j
+        for g, ww in zip(o.columns(), w):
+            assert np.array_equal(g, ww), j
+    for r in Ls + Rs + outs:
+        r.free()
+
+
 def test_sort_and_unique_multi(ctx):
     from vedas_amd._native import Rel, sort_col0_multi, unique_adjacent_multi
     rng = np.random.default_rng(1)

@@ -57,6 +57,49 @@ def test_queries_equal_naive(small, shape):
     assert st["rows"] == len(want[1])
 
 
+def predicate_variable_queries(s, p, o):
+    """Patterns with a predicate variable (routes through SOP, SPO and OSP:
+    QueryExecutor.cu:813-860, 871-971), alone and joined."""
+    s0, o0 = int(s[5]), int(o[5])
+    s1 = int(s[np.flatnonzero(s != s0)[0]])
+    o1 = int(o[np.flatnonzero(p == 2)[0]])
+    return [
+        ([(s0, "?p", o0)], ("UPLOAD", 0, "?p")),                      # s ?p o  -> SOP
+        ([(s0, "?p", "?o")], ("UPLOAD", 0, "?p")),                    # s ?p ?o -> SPO
+        ([("?s", "?p", o0)], ("UPLOAD", 0, "?s")),                    # ?s ?p o -> OSP
+        ([(s0, "?p", "?o"), ("?o", 2, "?z")],                         # join on the object
+         ("JOIN", ("UPLOAD", 0, "?o"), ("UPLOAD", 1, "?o"), "?o")),
+        ([("?s", "?p", o1), ("?s", 1, "?y")],                         # join on the subject
+         ("JOIN", ("UPLOAD", 0, "?s"), ("UPLOAD", 1, "?s"), "?s")),
+        ([(s0, "?p", "?o"), (s1, "?p", "?q")],                        # join on ?p
+         ("JOIN", ("UPLOAD", 0, "?p"), ("UPLOAD", 1, "?p"), "?p")),
+        ([(s0, "?p", o0), (s1, "?p", "?q")],                          # SOP [?p] x SPO [?p, ?q]
+         ("JOIN", ("UPLOAD", 0, "?p"), ("UPLOAD", 1, "?p"), "?p")),
+        ([("?s", "?p", o1), ("?s", "?r", o0)],                        # OSP x OSP on ?s
+         ("JOIN", ("UPLOAD", 0, "?s"), ("UPLOAD", 1, "?s"), "?s")),
+    ]
+
+
+def test_predicate_variable_patterns(small):
+    """Every predicate-variable route, plain and batched, against the naive
+    evaluator over the triples (all six permutations resident)."""
+    from vedas_amd.engine import query_text
+    s, p, o, e = small
+    qs = predicate_variable_queries(s, p, o)
+    texts = [query_text(pats, node) for pats, node in qs]
+    nonempty = 0
+    for (pats, node), text in zip(qs, texts):
+        headers, cols, st = run(e, pats, node)
+        want = O.naive_bgp(s, p, o, pats)
+        assert O.canonical(headers, cols) == want, pats
+        nonempty += len(want[1]) > 0
+    assert nonempty >= 6
+    res, _ = e.batch(texts)
+    for (pats, node), r in zip(qs, res):
+        assert O.canonical(r.headers(), r.columns()) == O.naive_bgp(s, p, o, pats), pats
+        r.free()
+
+
 def test_projection_and_finalise(small):
     s, p, o, e = small
     pats, node = plans.star_plan([1, 2, 3])

@@ -1,0 +1,199 @@
+"""Full-size parity on the 100M-triple store (10M entities with Zipf(0.8)
+subjects, 256 predicates Zipf(1.1), 40% literal objects, distinct triples):
+the index build of SPO / PSO / POS / OSP, the 10k range selects, the
+single-query join set and the 256-query batched mix, each against the
+oracle -- live on the checked arrays, and against the digests the oracle
+computed from its own build of the same store.  Needs an MI355X."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+from tests import scale
+from vedas_amd import workload as W
+
+pytestmark = pytest.mark.gpu
+
+U = 0xFFFFFFFF
+
+
+@pytest.fixture(scope="module")
+def store():
+    st = scale.build_store(W.STORE_100M, W.SEED_100M)
+    yield st
+    st["eng"].close()
+
+
+@pytest.fixture(scope="module")
+def gold():
+    return scale.golden("store_100M")
+
+
+@pytest.fixture(scope="module")
+def arrays(store, gold):
+    return scale.checked_arrays(store, gold)
+
+
+@pytest.mark.timeout(900)
+def test_cfg1_index_build_100m(store, gold, arrays):
+    """Every array of the four permutations: invariants, multiset and
+    histogram checked by the fixture; here also bit-equal to the oracle's
+    own sort of the same triples."""
+    assert arrays["_same_store"], "the generator did not reproduce the oracle's store"
+    assert len(store["s"]) == W.STORE_100M["n"]
+    # the per-predicate histogram (the reference's L1 of PSO / POS)
+    for perm in ("PSO", "POS"):
+        assert np.array_equal(arrays[perm]["l1v"], np.flatnonzero(np.bincount(store["p"])))
+
+
+def brute_count(s, p, o, perm, k1, k2):
+    cols = {"S": s, "P": p, "O": o}
+    m = cols[perm[0]] == k1
+    if k2 != U:
+        m &= cols[perm[1]] == k2
+    return int(np.count_nonzero(m))
+
+
+@pytest.mark.timeout(900)
+def test_cfg1_range_selects_100m(store, gold, arrays):
+    """The 10k range selects (one or two constants from drawn triples):
+    every range equals the oracle's lookup on the checked arrays (and the
+    oracle build's, by digest); the rows of a sample of selects through
+    vd_select_batch equal the arrays' rows; a few counts equal a scan of
+    the triples."""
+    from vedas_amd import _native
+    ctx, idx = store["ctx"], store["idx"]
+    s, p, o = store["s"], store["p"], store["o"]
+    groups = W.select_mix(W.N_SELECTS, W.SEED_SELECTS, s, p, o)
+    assert sum(len(k1) for k1, _ in groups.values()) == W.N_SELECTS
+    brute = 0
+    for perm, (k1, k2) in groups.items():
+        n = len(k1)
+        d1, d2 = ctx.upload(k1), ctx.upload(k2)
+        db, de = ctx.alloc(8 * n), ctx.alloc(8 * n)
+        ctx.check(ctx.L.vd_index_lookup_batch(ctx.h, idx.h, _native.PERMS[perm], C.c_void_p(d1),
+                                              C.c_void_p(d2), n, C.c_void_p(db), C.c_void_p(de)))
+        gb, ge = ctx.download(db, n, np.uint64), ctx.download(de, n, np.uint64)
+        for d in (db, de):
+            ctx.free(d)
+        ar = arrays[perm]
+        want = np.array([O.lookup(ar, int(a), None if b == U else int(b)) for a, b in zip(k1, k2)],
+                        np.uint64).reshape(-1, 2)
+        assert np.array_equal(gb, want[:, 0]) and np.array_equal(ge, want[:, 1]), perm
+        assert (ge > gb).all(), "a select drawn from an existing triple came back empty"
+        if arrays["_same_store"]:
+            g = gold["selects"][perm]
+            assert g["count"] == n and g["rows"] == int((ge - gb).sum())
+            assert scale.sha(gb) == g["begin_sha256"] and scale.sha(ge) == g["end_sha256"]
+        # rows of the first selects (up to 20M rows) through the batched select
+        sz = (ge - gb).astype(np.int64)
+        m = max(1, int(np.searchsorted(np.cumsum(sz), 20_000_000, side="right")))
+        rows = int(sz[:m].sum())
+        doff = ctx.alloc(8 * (m + 1))
+        dy, dz = ctx.alloc(4 * max(rows, 1)), ctx.alloc(4 * max(rows, 1))
+        tot = C.c_uint64()
+        ctx.check(ctx.L.vd_select_batch(ctx.h, idx.h, _native.PERMS[perm], C.c_void_p(d1),
+                                        C.c_void_p(d2), m, C.c_void_p(doff), C.c_void_p(dy),
+                                        C.c_void_p(dz), max(rows, 1), C.byref(tot)))
+        assert tot.value == rows
+        offs = ctx.download(doff, m + 1, np.uint64)
+        Y, Z = ctx.download(dy, rows), ctx.download(dz, rows)
+        for d in (doff, dy, dz, d1, d2):
+            ctx.free(d)
+        for q in range(m):
+            b, e, a0 = int(gb[q]), int(ge[q]), int(offs[q])
+            assert np.array_equal(Y[a0:a0 + e - b], ar["y"][b:e]), (perm, q)
+            assert np.array_equal(Z[a0:a0 + e - b], ar["z"][b:e]), (perm, q)
+        scale.note(f"{perm}: {n} selects, {int(sz.sum())} rows; {m} checked row by row")
+        for q in range(0, n, max(1, n // 3))[:3]:
+            assert brute_count(s, p, o, perm, int(k1[q]), int(k2[q])) == int(sz[q]), (perm, q)
+            brute += 1
+    assert brute >= 8
+
+
+@pytest.mark.timeout(900)
+def test_cfg2_join_set_100m(store, gold, arrays):
+    """star4, chain4 and snowflake6 with their given plans and with the
+    planner's plans: results (bag digest), join rows and select rows equal
+    the oracle's plan walk with the reference's bounds, run here on the
+    checked PSO / POS arrays; with the bounds switched off (pre-scan and
+    after-join update) the intermediates equal the walk without bounds."""
+    from vedas_amd.engine import query_text, set_option
+    eng = store["eng"]
+    ix = {k: arrays[k] for k in ("PSO", "POS")}
+    for name, (pats, node) in W.JOIN_SET.items():
+        r = eng.query(query_text(pats, node))
+        got, gst = O.bag_digest(r.headers(), r.columns()), r.stats()
+        r.free()
+        ost = {}
+        h, cols = O.run_plan(ix, pats, node, ost, bounds=True)
+        want = O.bag_digest(h, cols)
+        del cols
+        scale.note(f"{name}: {gst['rows']} rows, {gst['join_rows']} join rows "
+                   f"(oracle {want[1]} / {ost['join_rows']})")
+        assert got == want, name
+        assert 1e7 <= gst["join_rows"] <= 5e8
+        for k in ("join_rows", "select_rows"):
+            assert gst[k] == ost[k], (name, k)
+        if arrays["_same_store"]:
+            g = gold["join_set"][name]
+            assert want == scale.digest_of(g)
+            assert ost["join_rows"] == g["join_rows"] and ost["select_rows"] == g["select_rows"]
+        # the planner's plan: same bindings
+        r = eng.query(query_text(pats, None))
+        assert O.bag_digest(r.headers(), r.columns()) == want, name
+        pst = r.stats()
+        r.free()
+        if arrays["_same_store"]:
+            assert pst["join_rows"] == gold["join_set"][name]["planned"]["join_rows"]
+        # without join-variable bounds
+        set_option("bound_prescan", 0)
+        set_option("bound_after_join", 0)
+        try:
+            r = eng.query(query_text(pats, node))
+            assert O.bag_digest(r.headers(), r.columns()) == want, name
+            ust = r.stats()
+            r.free()
+        finally:
+            set_option("bound_prescan", 1)
+            set_option("bound_after_join", 1)
+        assert ust["select_rows"] >= gst["select_rows"]
+        if arrays["_same_store"]:
+            assert ust["join_rows"] == gold["join_set"][name]["join_rows_unbounded"]
+            assert ust["select_rows"] == gold["join_set"][name]["select_rows_unbounded"]
+
+
+@pytest.mark.timeout(900)
+def test_cfg3_batched_mix_100m(store, gold, arrays):
+    """The 256-query mix (stars of 2-5 patterns, chains of 2-4, a quarter
+    with a bound object) run batched and one query at a time: every result
+    equals the oracle's (its digests of the whole mix, and a live plan walk
+    of a sample on the checked arrays)."""
+    from vedas_amd.engine import query_text
+    eng = store["eng"]
+    s, p, o = store["s"], store["p"], store["o"]
+    mix = W.query_mix(W.N_MIX, W.SEED_MIX, W.STORE_100M, s, p, o)
+    texts = [query_text(pats, node) for _, pats, node in mix]
+    batched = []
+    for g in W.chunk_by_memory(eng, texts, 96 << 30):
+        res, _ = eng.batch(g)
+        for r in res:
+            batched.append(O.bag_digest(r.headers(), r.columns()))
+            r.free()
+    plain = []
+    for t in texts:
+        r = eng.query(t)
+        plain.append(O.bag_digest(r.headers(), r.columns()))
+        r.free()
+    assert batched == plain
+    ix = {k: arrays[k] for k in ("PSO", "POS")}
+    for i in range(0, len(mix), 16):
+        _, pats, node = mix[i]
+        h, cols = O.run_plan(ix, pats, node, bounds=True)
+        assert O.bag_digest(h, cols) == batched[i], i
+    if arrays["_same_store"]:
+        g = gold["mix"]
+        assert [x["text"] for x in g] == texts
+        assert batched == [scale.digest_of(x) for x in g]
+    scale.note(f"mix: {len(texts)} queries, {sum(d[1] for d in batched)} result rows")

@@ -81,11 +81,17 @@ def build_host() -> str:
 
 
 def build_oracle() -> str:
+    """The oracle (test checker) and the CPU port (baseline): test and
+    benchmark infrastructure, never linked by the package."""
     out = os.path.join(ORACLE, "liboracle.so")
     tmp = out + ".tmp"
     _run(["gcc", "-O2", "-std=c11", "-fPIC", "-shared",
           os.path.join(ORACLE, "vedas_oracle.c"), "-o", tmp])
     os.replace(tmp, out)
+    port = os.path.join(ORACLE, "libcpuport.so")
+    _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-pthread",
+          os.path.join(ORACLE, "cpu_port.cpp"), "-o", port + ".tmp"])
+    os.replace(port + ".tmp", port)
     return out
 
 

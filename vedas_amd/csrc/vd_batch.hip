@@ -558,9 +558,10 @@ void select_multi(vd_ctx *ctx, const vd_index *idx, const vd_select_job_t *jobs,
     }
 }
 
-void join_multi(vd_ctx *ctx, const vd_rel *const *L, const vd_rel *const *R, int np,
-                vd_rel **outs) {
-    VD_REQUIRE(np <= 65535, VD_ERANGE, "at most 65535 joins per batched call");
+// One chunk of a batched join (at most JM_CHUNK pairs).
+void join_multi_chunk(vd_ctx *ctx, const vd_rel *const *L, const vd_rel *const *R, int np,
+                      vd_rel **outs) {
+    VD_REQUIRE(np <= 65535, VD_ERANGE, "at most 65535 joins per chunk");
     std::vector<JPairDev> hp(np);
     std::vector<uint64_t> lrows(np);
     for (int j = 0; j < np; ++j) {
@@ -568,6 +569,8 @@ void join_multi(vd_ctx *ctx, const vd_rel *const *L, const vd_rel *const *R, int
                    "bad join input");
         VD_REQUIRE(L[j]->ncols + R[j]->ncols - 1 <= VD_MAX_COLS, VD_EINVAL,
                    "join output has too many columns");
+        // right rows are addressed by 32-bit first-match indices
+        VD_REQUIRE(R[j]->nrows < (1ull << 32), VD_ERANGE, "batched join right input too large");
         // an empty right side contributes no left rows (all counts 0)
         lrows[j] = R[j]->nrows ? L[j]->nrows : 0;
     }
@@ -649,6 +652,23 @@ void join_multi(vd_ctx *ctx, const vd_rel *const *L, const vd_rel *const *R, int
             else if (nv == 4096) go(k_join_expand_multi<4096>);
             else go(k_join_expand_multi<2048>);
         }
+    } catch (...) {
+        for (int j = 0; j < np; ++j) rel_free(ctx, outs[j]), outs[j] = nullptr;
+        throw;
+    }
+}
+
+// Pairs in chunks of JM_CHUNK: every input's left rows start on a tile of
+// the chunk's row space, so the padding up to a tile (at most 2047 rows
+// per pair) stays below JM_CHUNK * 2048 rows per chunk (64 MiB of row
+// offsets), whatever the number of tiny joins in the call.
+constexpr int JM_CHUNK = 4096;
+void join_multi(vd_ctx *ctx, const vd_rel *const *L, const vd_rel *const *R, int np,
+                vd_rel **outs) {
+    for (int j = 0; j < np; ++j) outs[j] = nullptr;
+    try {
+        for (int a = 0; a < np; a += JM_CHUNK)
+            join_multi_chunk(ctx, L + a, R + a, std::min(JM_CHUNK, np - a), outs + a);
     } catch (...) {
         for (int j = 0; j < np; ++j) rel_free(ctx, outs[j]), outs[j] = nullptr;
         throw;
