@@ -190,6 +190,22 @@ int  vd_select_batch(vd_ctx *ctx, const vd_index *idx, int perm,
                      uint32_t *d_out_z, uint64_t max_rows,
                      uint64_t *total_rows);
 
+/* No counterpart in the reference.  vd_select_batch for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid it): the same arguments and results, but
+ * the per-query first rows, row counts and the row-offset scan's partial
+ * sums live in one fixed device array of the library (16 MiB per GPU, held
+ * until the process ends) instead of allocations, so at most
+ * VD_NOALLOC_MAX_QUERIES queries per call (VD_ERANGE beyond).  It calls no
+ * hipMalloc / hipFree.  Calls that overlap on one GPU share that array and
+ * race; serialise them (one stream, or one call at a time). */
+#define VD_NOALLOC_MAX_QUERIES (1u << 20)
+int  vd_select_batch_noalloc(vd_ctx *ctx, const vd_index *idx, int perm,
+                             const uint32_t *d_k1, const uint32_t *d_k2,
+                             uint64_t nq, uint64_t *d_row_offsets,
+                             uint32_t *d_out_y, uint32_t *d_out_z,
+                             uint64_t max_rows, uint64_t *total_rows);
+
 /* ---- join (JoinQueryJob) ------------------------------------------------ */
 /* Bag inner sort-merge join of L and R on their first nkeys (1 or 2)
  * columns, both sorted ascending by those columns.  For each left row in

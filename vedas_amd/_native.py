@@ -99,6 +99,9 @@ EXPORTS = {
     "vd_select_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                   C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "vd_select_batch_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                          C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "vd_join": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                           C.POINTER(C.c_void_p)]),
     "vd_select_multi": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,

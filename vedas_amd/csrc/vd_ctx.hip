@@ -288,6 +288,7 @@ int vd_ctx_create(int device, vd_ctx **out) {
         VD_REQUIRE(device >= 0 && device < ndev, VD_EINVAL, "no such device");
         ctx->device = device;
         VD_HIP(hipSetDevice(device));
+        vd::noalloc_init(device);  // this GPU's address of the select scratch
         VD_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         // Coherent: kernel stores reach host memory without a cache flush
         // of their own, so the sequence word is seen as soon as it is written.

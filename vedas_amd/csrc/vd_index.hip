@@ -17,6 +17,7 @@
 // columns are the reference's xy_data / xyz_data arrays.
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "vd_radix_dev.hpp"
@@ -611,6 +612,67 @@ vd_rel *select_pieces_rel(vd_ctx *ctx, const vd_index *idx, const vd_perm_arrays
 
 using namespace vd;
 
+// ---- batched range select (with or without allocation) ---------------------
+// Scratch of vd_select_batch_noalloc: each query's first row, then its row
+// count, then the row-offset scan's tile totals.  Every word a call reads is
+// written earlier in the same call, so nothing needs clearing between calls.
+// Each GPU holds its own copy for the life of the process; calls that
+// overlap on one GPU share it (and race).
+constexpr uint64_t NA_QUERIES = VD_NOALLOC_MAX_QUERIES;
+constexpr uint64_t NA_SCAN_WORDS = 2048;
+static __device__ __align__(256) uint64_t scratch[2 * NA_QUERIES + NA_SCAN_WORDS];
+
+namespace {
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex g_na_mu;
+uint64_t *g_na_ptr[NA_MAX_DEVICES];  // this process's address of scratch on each GPU
+
+uint64_t *scratch_ptr(int device) {
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    VD_REQUIRE(device >= 0 && device < NA_MAX_DEVICES && g_na_ptr[device], VD_EINVAL,
+               "no-allocation scratch not resolved on this device");
+    return g_na_ptr[device];
+}
+
+// Rows of nq lookups written back to back: begin / sizes hold nq words each,
+// ws (may be null: the scan allocates) the scan's tile totals.
+void select_batch_impl(vd_ctx *ctx, const vd_index *idx, int perm, const uint32_t *d_k1,
+                       const uint32_t *d_k2, uint64_t nq, uint64_t *d_row_offsets,
+                       uint32_t *d_out_y, uint32_t *d_out_z, uint64_t max_rows,
+                       uint64_t *total_rows, uint64_t *begin, uint64_t *sizes, uint64_t *ws) {
+    const vd_perm_arrays &pa = get_perm(idx, perm);
+    VD_REQUIRE(d_k1 && d_row_offsets, VD_EINVAL, "null select arrays");
+    launch(ctx, "select_lookup", (double)nq * 24.0, dim3((unsigned)ceil_div(nq, 256)), dim3(256),
+           0, k_lookup_batch, perm_dev(pa), d_k1, d_k2, nq, begin, (uint64_t *)nullptr, sizes);
+    if (ws)
+        scan_u64_ws(ctx, sizes, d_row_offsets, nq, d_row_offsets + nq, ws, NA_SCAN_WORDS);
+    else
+        scan_u64(ctx, sizes, d_row_offsets, nq, d_row_offsets + nq);
+    const uint64_t total = read_u64(ctx, d_row_offsets + nq);
+    *total_rows = total;
+    if (max_rows == 0) return;
+    VD_REQUIRE(total <= max_rows, VD_ERANGE, "select batch output exceeds max_rows");
+    VD_REQUIRE(total == 0 || d_out_y || d_out_z, VD_EINVAL, "no output columns");
+    if (total == 0) return;
+    launch(ctx, "select_gather", (double)total * 4.0 * ((d_out_y ? 2 : 0) + (d_out_z ? 2 : 0)),
+           dim3(stream_grid(total)), dim3(256), 0, k_gather_ranges,
+           (const uint64_t *)d_row_offsets, (const uint64_t *)begin, nq, total,
+           (const uint32_t *)pa.y, (const uint32_t *)pa.z, d_out_y, d_out_z);
+}
+}  // namespace
+
+namespace vd {
+void noalloc_init(int device) {
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    VD_REQUIRE(device >= 0 && device < NA_MAX_DEVICES, VD_EINVAL, "device number too large");
+    if (g_na_ptr[device]) return;
+    void *p = nullptr;
+    VD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_na_ptr[device] = static_cast<uint64_t *>(p);
+}
+}  // namespace vd
+
+
 extern "C" {
 
 int vd_index_build(vd_ctx *ctx, const uint32_t *d_s, const uint32_t *d_p,
@@ -850,39 +912,34 @@ int vd_select_batch(vd_ctx *ctx, const vd_index *idx, int perm,
                     uint32_t *d_out_z, uint64_t max_rows, uint64_t *total_rows) {
     if (!ctx || !total_rows) return VD_EINVAL;
     *total_rows = 0;
-    try {
-        const vd_perm_arrays &pa = get_perm(idx, perm);
-        VD_REQUIRE(nq == 0 || (d_k1 && d_row_offsets), VD_EINVAL,
-                   "null select arrays");
-        if (nq == 0) return VD_OK;
-        DBuf<uint64_t> begin(ctx, nq), sizes(ctx, nq);
-        launch(ctx, "select_lookup", (double)nq * 24.0,
-               dim3((unsigned)ceil_div(nq, 256)), dim3(256), 0, k_lookup_batch,
-               perm_dev(pa), d_k1, d_k2, nq, begin.p, (uint64_t *)nullptr,
-               sizes.p);
-        scan_u64(ctx, sizes.p, d_row_offsets, nq, d_row_offsets + nq);
-        const uint64_t total = read_u64(ctx, d_row_offsets + nq);
-        *total_rows = total;
-        if (max_rows == 0) return VD_OK;
-        if (total > max_rows) {
-            ctx->last_error = "select batch output exceeds max_rows";
-            return VD_ERANGE;
+    VD_TRY(ctx, {
+        if (nq == 0) {
+            get_perm(idx, perm);
+        } else {
+            DBuf<uint64_t> begin(ctx, nq), sizes(ctx, nq);
+            select_batch_impl(ctx, idx, perm, d_k1, d_k2, nq, d_row_offsets, d_out_y, d_out_z,
+                              max_rows, total_rows, begin.p, sizes.p, nullptr);
         }
-        VD_REQUIRE(total == 0 || d_out_y || d_out_z, VD_EINVAL, "no output columns");
-        launch(ctx, "select_gather",
-               (double)total * 4.0 * ((d_out_y ? 2 : 0) + (d_out_z ? 2 : 0)),
-               dim3(stream_grid(total)), dim3(256), 0, k_gather_ranges,
-               (const uint64_t *)d_row_offsets, (const uint64_t *)begin.p, nq,
-               total, (const uint32_t *)pa.y, (const uint32_t *)pa.z, d_out_y,
-               d_out_z);
-    } catch (const Error &e) {
-        ctx->last_error = e.what();
-        return e.code;
-    } catch (const std::exception &e) {
-        ctx->last_error = e.what();
-        return VD_EINVAL;
-    }
-    return VD_OK;
+    });
+}
+
+int vd_select_batch_noalloc(vd_ctx *ctx, const vd_index *idx, int perm,
+                            const uint32_t *d_k1, const uint32_t *d_k2, uint64_t nq,
+                            uint64_t *d_row_offsets, uint32_t *d_out_y,
+                            uint32_t *d_out_z, uint64_t max_rows, uint64_t *total_rows) {
+    if (!ctx || !total_rows) return VD_EINVAL;
+    *total_rows = 0;
+    VD_TRY(ctx, {
+        VD_REQUIRE(nq <= VD_NOALLOC_MAX_QUERIES, VD_ERANGE,
+                   "more queries than VD_NOALLOC_MAX_QUERIES");
+        uint64_t *sp = scratch_ptr(ctx->device);
+        if (nq == 0) {
+            get_perm(idx, perm);
+        } else {
+            select_batch_impl(ctx, idx, perm, d_k1, d_k2, nq, d_row_offsets, d_out_y, d_out_z,
+                              max_rows, total_rows, sp, sp + NA_QUERIES, sp + 2 * NA_QUERIES);
+        }
+    });
 }
 
 }  // extern "C"

@@ -236,6 +236,13 @@ void scan_u32_to_u64(vd_ctx *ctx, const uint32_t *in, uint64_t *out,
 // Exclusive scan of n uint64 in place-capable (in may equal out).
 void scan_u64(vd_ctx *ctx, const uint64_t *in, uint64_t *out, uint64_t n,
               uint64_t *d_total);
+// scan_u64 with its tile totals in the caller's workspace of ws_words words
+// (no allocation): n up to min(ws_words, 2048) * 2048 elements.
+void scan_u64_ws(vd_ctx *ctx, const uint64_t *in, uint64_t *out, uint64_t n,
+                 uint64_t *d_total, uint64_t *ws, uint64_t ws_words);
+// Resolves, once per device, this GPU's address of the no-allocation select
+// scratch (device `device` current); called by vd_ctx_create.
+void noalloc_init(int device);
 void fill_u32(vd_ctx *ctx, uint32_t *p, uint32_t v, uint64_t n);
 void fill_u64(vd_ctx *ctx, uint64_t *p, uint64_t v, uint64_t n);
 uint32_t max_u32(vd_ctx *ctx, const uint32_t *p, uint64_t n);  // syncs

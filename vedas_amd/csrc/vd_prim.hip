@@ -156,6 +156,31 @@ void scan_u64(vd_ctx *ctx, const uint64_t *in, uint64_t *out, uint64_t n,
     scan_impl<uint64_t>(ctx, in, out, n, d_total);
 }
 
+void scan_u64_ws(vd_ctx *ctx, const uint64_t *in, uint64_t *out, uint64_t n,
+                 uint64_t *d_total, uint64_t *ws, uint64_t ws_words) {
+    if (n == 0) {
+        if (d_total) fill_u64(ctx, d_total, 0, 1);
+        return;
+    }
+    const uint64_t tiles = ceil_div(n, SCAN_TILE);
+    VD_REQUIRE(tiles <= ws_words && tiles <= (uint64_t)SCAN_TILE, VD_ERANGE,
+               "scan larger than its workspace");
+    const double bytes = (double)n * 24.0;
+    if (tiles == 1) {
+        launch(ctx, "scan_tile", bytes, dim3(1), dim3(SCAN_BLOCK), 0, k_tile_scan<uint64_t>, in,
+               out, n, (const uint64_t *)nullptr, d_total, (uint64_t *)nullptr);
+        return;
+    }
+    launch(ctx, "scan_reduce", (double)n * 8.0, dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0,
+           k_tile_reduce<uint64_t>, in, n, ws);
+    launch(ctx, "scan_tile", (double)tiles * 24.0, dim3(1), dim3(SCAN_BLOCK), 0,
+           k_tile_scan<uint64_t>, (const uint64_t *)ws, ws, tiles, (const uint64_t *)nullptr,
+           d_total, (uint64_t *)nullptr);
+    launch(ctx, "scan_tile", bytes, dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0,
+           k_tile_scan<uint64_t>, in, out, n, (const uint64_t *)ws, (uint64_t *)nullptr,
+           (uint64_t *)nullptr);
+}
+
 static_assert(SCAN_TILE == OFF_TILE, "tile offsets use the scan tile");
 
 void scan_tile_totals(vd_ctx *ctx, const uint64_t *tot, uint64_t ntiles, uint64_t *tpre) {
