@@ -208,7 +208,9 @@ def serve_bench(eng, texts, qps, seed=1, slots=4, max_batch=128, budget=96 << 30
         lat = np.array([r["done_ms"] - r["arrival_ms"] for r in recs])
         span = max(r["done_ms"] for r in recs) - min(r["arrival_ms"] for r in recs)
         return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                "qps": len(recs) / (span / 1e3), "batches": len({r["batch"] for r in recs})}
+                "qps": len(recs) / (span / 1e3), "batches": len({r["batch"] for r in recs}),
+                "mem_held": int(sum(r["mem_held"] for r in recs)),
+                "est_bytes_max": float(max(r["est_bytes"] for r in recs))}
 
     # Each mode runs the stream three times and keeps the median run: the
     # served stream is short (tens of ms at saturation) and the first run of
@@ -262,6 +264,28 @@ def serve_section(store: dict, loads, n_queries: int, seed: int, mix_seed: int):
         log(f"[bench]   plain {sv['plain']['qps']:.0f} q/s p50 {sv['plain']['p50_ms']:.1f} "
             f"p99 {sv['plain']['p99_ms']:.1f} ms; served {sv['served']['qps']:.0f} q/s "
             f"p50 {sv['served']['p50_ms']:.1f} p99 {sv['served']['p99_ms']:.1f} ms")
+    # the highest load again under intermediate-memory budgets that bind:
+    # admission holds requests until the estimates of those in flight leave
+    # room (median of three runs each)
+    budget_runs = []
+    rng = np.random.default_rng(seed + 3)
+    arr = list(np.cumsum(rng.exponential(1000.0 / max(loads), len(texts))))
+    for budget in (4 << 30, 1 << 30):
+        runs = []
+        for _ in range(3):
+            recs, _ = eng.serve(texts, arr, slots=4, max_batch=128, mem_budget=budget,
+                                min_fill=48, max_wait_ms=1.0)
+            lat = np.array([r["done_ms"] - r["arrival_ms"] for r in recs])
+            span = max(r["done_ms"] for r in recs) - min(r["arrival_ms"] for r in recs)
+            runs.append({"qps": len(recs) / (span / 1e3), "p50_ms": float(np.percentile(lat, 50)),
+                         "p99_ms": float(np.percentile(lat, 99)),
+                         "mem_held": int(sum(r["mem_held"] for r in recs)),
+                         "batches": len({r["batch"] for r in recs})})
+        runs.sort(key=lambda r: r["qps"])
+        budget_runs.append({"mem_budget": budget, "offered_qps": max(loads), **runs[1]})
+        log(f"[bench]   budget {budget >> 20} MiB: {runs[1]['qps']:.0f} q/s p50 "
+            f"{runs[1]['p50_ms']:.1f} p99 {runs[1]['p99_ms']:.1f} ms, "
+            f"{runs[1]['mem_held']} requests held for memory")
     # same result alone and in the served mix
     rng = np.random.default_rng(seed + 2)
     arr = list(np.cumsum(rng.exponential(1000.0 / max(loads), len(texts))))
@@ -277,6 +301,7 @@ def serve_section(store: dict, loads, n_queries: int, seed: int, mix_seed: int):
     eng.close()
     return {"store": {**store, "seed": seed, "distinct": True, "key_bits": kbits},
             "build_ms": build_ms, "queries": len(texts), "loads": loads_out,
+            "budget_runs": budget_runs,
             "results_equal_alone": f"{same}/{len(texts)}"}
 
 

@@ -73,6 +73,10 @@ void *vd_ctx_stream(vd_ctx *ctx);             /* the hipStream_t          */
 int  vd_ctx_trim(vd_ctx *ctx);                /* release cached blocks    */
 /* Device memory in use / cached by the context's allocator, in bytes. */
 int  vd_ctx_mem(vd_ctx *ctx, uint64_t *in_use, uint64_t *cached);
+/* Highest in-use byte count since the last reset (reset != 0 restarts it
+ * from the current in-use count): the measured peak that admission
+ * estimates are compared with. */
+int  vd_ctx_mem_peak(vd_ctx *ctx, uint64_t *peak, int reset);
 /* Tunables: "radix_items" (keys per thread in a radix tile: 16 = 512
  * threads x 16 keys, the default; 8 = 512 x 8; 24 = 256 x 24, keys only),
  * "join_nv" (merge-path diagonal per join expansion block), "join_rcap"
@@ -122,6 +126,28 @@ int  vd_index_lookup(vd_ctx *ctx, const vd_index *idx, int perm, uint32_t k1,
  * statistics (VedasStorage.cu:278-288,342-355; Histogram.cu:170-231). */
 int  vd_index_stats(vd_ctx *ctx, const vd_index *idx, int perm,
                     uint64_t *d_rows, uint64_t *d_pairs);
+/* Equal-width and equal-depth histograms of a permutation's leading column
+ * (subjects for SPO / SOP, objects for OSP / OPS), as
+ * Histogram::initEqWidth / initEqDepth build them from the sorted column at
+ * index build time (Histogram.cu:170-231, VedasStorage.cu:316-317,
+ * 518-536).  Only rows whose key is below id_limit count (the reference
+ * passes LITERAL_START_ID - 1: literal objects left out; 0xFFFFFFFF keeps
+ * every key).  VD_HIST_EQUAL_WIDTH: param is the width in IDs; bucket b
+ * holds keys [first + b*param, first + (b+1)*param - 1] up to the last key
+ * (the reference drops the last key when the span is a multiple of the
+ * width).  VD_HIST_EQUAL_DEPTH: param is the depth in rows; a bucket starts
+ * at a row, ends at the row param rows later (or the last row) extended to
+ * the end of that key's run, so it holds param + 1 rows or more; lower /
+ * upper are its first and last key.  lowers / uppers / freqs are host
+ * arrays of cap entries; *nbuckets receives the bucket count (VD_ERANGE,
+ * nothing written, when it exceeds cap).  Computed from the L1 arrays on
+ * the device; no row is read. */
+#define VD_HIST_EQUAL_WIDTH 0
+#define VD_HIST_EQUAL_DEPTH 1
+int  vd_index_histogram(vd_ctx *ctx, const vd_index *idx, int perm, int kind,
+                        uint32_t param, uint32_t id_limit, uint32_t *lowers,
+                        uint32_t *uppers, uint64_t *freqs, uint64_t cap,
+                        uint64_t *nbuckets);
 /* Batched form on device arrays: d_k2 may be NULL (one constant for every
  * query) or hold VD_UNBOUND for queries with one constant. */
 int  vd_index_lookup_batch(vd_ctx *ctx, const vd_index *idx, int perm,

@@ -134,9 +134,10 @@ typedef struct {
 } vdq_serve_cfg_t;
 typedef struct {
     double arrival_ms, start_ms, done_ms;
-    double est_bytes;
+    double est_bytes;        /* admission estimate of its peak bytes      */
     uint64_t rows;
     int batch, slot;
+    int mem_held;            /* 1: waited at the queue head for memory    */
 } vdq_serve_rec_t;
 int vdq_serve(vdq_engine *e, const char *const *texts, const double *arrival_ms, int n,
               const vdq_serve_cfg_t *cfg, vdq_serve_rec_t *recs, vdq_result **outs);

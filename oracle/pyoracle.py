@@ -346,6 +346,60 @@ def canonical(headers, cols):
     return names, rows
 
 
+# ---- histograms -------------------------------------------------------------------
+def hist_eq_width(col, width: int, id_limit: int):
+    """Histogram::initEqWidth (Histogram.cu:170-199) over a sorted key
+    column: only keys below id_limit; buckets of `width` IDs from the first
+    key, the last one cut at the last key; freq = keys in the bucket.  The
+    reference stops when a bucket would start at the last key itself (its
+    loop runs while id < endId), which leaves that key with no bucket; here
+    the last key always has one.  Returns (lowers, uppers, freqs) lists."""
+    col = np.asarray(col)
+    m = int(np.searchsorted(col, id_limit, "left"))
+    if m == 0:
+        return [], [], []
+    start, end = int(col[0]), int(col[m - 1])
+    lowers, uppers = [], []
+    lo = start
+    while lo <= end:
+        lowers.append(lo)
+        uppers.append(min(lo + width - 1, end))
+        lo += width
+    freq = [0] * len(lowers)
+    k = 0
+    for v in col[:m].tolist():
+        while v > uppers[k]:
+            k += 1
+        freq[k] += 1
+    return lowers, uppers, freq
+
+
+def hist_eq_depth(col, depth: int, id_limit: int):
+    """Histogram::initEqDepth (Histogram.cu:201-231) over a sorted key
+    column: only keys below id_limit; a bucket starts at a row, its end is
+    the row `depth` rows later (or the last row), moved on to the end of
+    that key's run; lower / upper are the keys at its start and end, freq
+    its rows (depth + 1 or more).  Returns (lowers, uppers, freqs)."""
+    col = np.asarray(col)
+    m = int(np.searchsorted(col, id_limit, "left"))
+    c = col[:m].tolist()
+    lowers, uppers, freq = [], [], []
+    d = 0
+    while d < m:
+        s = d
+        d += depth
+        if d >= m:
+            d = m - 1
+        last = c[d]
+        while d < m - 1 and c[d] == c[d + 1]:
+            d += 1
+        lowers.append(c[s])
+        uppers.append(last)
+        freq.append(d - s + 1)
+        d += 1
+    return lowers, uppers, freq
+
+
 # ---- checkers for full-size results -------------------------------------------
 def bag_digest(headers, cols):
     """Order-independent digest of a relation's rows with columns ordered by

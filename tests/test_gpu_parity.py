@@ -551,3 +551,23 @@ def test_plan_small_store_naive(ctx):
             assert got == O.naive_bgp(s, p, o, pats)
     finally:
         idx.free()
+
+
+@pytest.mark.parametrize("perm", ["SPO", "OSP", "PSO"])
+def test_histograms_equal_oracle(ctx, store, perm):
+    """Equal-width and equal-depth histograms of the leading column
+    (vd_index_histogram) against the oracle's restatement of
+    Histogram::initEqWidth / initEqDepth on the sorted column, with the
+    reference's literal limit and with every key."""
+    s, p, o, idx, orc = store
+    col = {"S": s, "P": p, "O": o}[perm[0]]
+    keys = np.sort(col)
+    for limit in (50_000_000 - 1, 0xFFFFFFFF, 40_000):
+        for w in (1, 100, 7777):
+            lo, hi, fr = idx.histogram(perm, False, w, limit)
+            want = O.hist_eq_width(keys, w, limit)
+            assert (lo.tolist(), hi.tolist(), fr.tolist()) == want, (perm, limit, w)
+        for d in (1, 1000, 300_000):
+            lo, hi, fr = idx.histogram(perm, True, d, limit)
+            want = O.hist_eq_depth(keys, d, limit)
+            assert (lo.tolist(), hi.tolist(), fr.tolist()) == want, (perm, limit, d)

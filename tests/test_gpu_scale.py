@@ -169,7 +169,9 @@ def test_cfg3_batched_mix_100m(store, gold, arrays):
     """The 256-query mix (stars of 2-5 patterns, chains of 2-4, a quarter
     with a bound object) run batched and one query at a time: every result
     equals the oracle's (its digests of the whole mix, and a live plan walk
-    of a sample on the checked arrays)."""
+    of a sample on the checked arrays).  Each query's admission estimate is
+    compared with its measured peak of device memory (64 MiB floor): none
+    below half the peak, 90% within a factor of 4, the median within 1.25."""
     from vedas_amd.engine import query_text
     eng = store["eng"]
     s, p, o = store["s"], store["p"], store["o"]
@@ -181,12 +183,32 @@ def test_cfg3_batched_mix_100m(store, gold, arrays):
         for r in res:
             batched.append(O.bag_digest(r.headers(), r.columns()))
             r.free()
-    plain = []
+    # one at a time, with each query's measured peak of device memory beside
+    # its admission estimate
+    ctx = store["ctx"]
+    plain, ratios = [], []
     for t in texts:
+        _, est = eng.estimate(t)
+        ctx.sync()
+        base = ctx.mem()[0]
+        ctx.mem_peak(reset=True)
         r = eng.query(t)
+        ctx.sync()
+        meas = ctx.mem_peak() - base
         plain.append(O.bag_digest(r.headers(), r.columns()))
         r.free()
+        ratios.append((est + (64 << 20)) / (meas + (64 << 20)))
     assert batched == plain
+    scale.note("estimate / measured peak (64 MiB floor): " +
+               " ".join(f"{x:.2f}" for x in sorted(ratios)[::16]))
+    r = np.array(ratios)
+    # admission must not under-reserve: no estimate below half the peak; the
+    # estimate is an expectation over the bound constant, so a query whose
+    # constant picks cold keys is over-reserved (the safe side) -- most are
+    # close
+    assert r.min() >= 0.5, sorted(ratios)[:3]
+    assert np.mean((r >= 0.25) & (r <= 4)) >= 0.9, sorted(ratios)[-10:]
+    assert 0.8 <= np.median(r) <= 1.25
     ix = {k: arrays[k] for k in ("PSO", "POS")}
     for i in range(0, len(mix), 16):
         _, pats, node = mix[i]

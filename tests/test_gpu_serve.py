@@ -56,6 +56,8 @@ def test_served_results_equal_alone(served, slots, max_batch, budget, fill):
         assert rc["slot"] >= 0 and rc["batch"] >= 0
     assert len({rc["batch"] for rc in recs}) < len(texts) or max_batch == 1
     assert max(rc["slot"] for rc in recs) < slots
+    if budget < (1 << 30):  # a budget that binds: admission holds requests back
+        assert sum(rc["mem_held"] for rc in recs) > 0
 
 
 def test_plain_serving_and_estimates(served):
@@ -68,3 +70,31 @@ def test_plain_serving_and_estimates(served):
     assert len({rc["batch"] for rc in recs}) == len(texts)  # one request per batch
     rows, peak = e.estimate(texts[0])
     assert rows >= 0 and peak > 0
+
+
+def test_estimates_against_measured_peaks(served):
+    """The admission estimate (vdq_estimate: histogram moments, views free)
+    against the measured peak of device memory of each query of the mix run
+    alone (allocator peak above the resident store).  Reported per query;
+    every estimate must be within a factor of 8 of the measurement, both
+    taken with a 16 MiB floor (below that the pool's power-of-two size
+    classes and a cold bound constant's tiny result make the ratio
+    meaningless, and admission budgets are GiBs)."""
+    from vedas_amd._native import Ctx
+    e, texts, alone = served
+    ctx = Ctx.wrap(e.ctx_handle())
+    ratios = []
+    for t in texts:
+        _, peak = e.estimate(t)
+        ctx.sync()
+        base = ctx.mem()[0]
+        ctx.mem_peak(reset=True)
+        r = e.query(t)
+        ctx.sync()
+        meas = ctx.mem_peak() - base
+        r.free()
+        ratios.append((peak + (16 << 20)) / (meas + (16 << 20)))
+        if not 0.5 <= ratios[-1] <= 2:
+            print(f"estimate {peak:.3g} B, measured {meas} B: {t}")
+    print("estimate / measured peak per query:", " ".join(f"{x:.2f}" for x in ratios))
+    assert all(1 / 8 <= x <= 8 for x in ratios), ratios

@@ -398,3 +398,59 @@ def test_too_narrow_bound_is_caught():
     finally:
         O.prescan_bounds = real
     assert O.canonical(h, c) != O.naive_bgp(s, p, o, pats)
+
+
+# ---- equal-width / equal-depth histograms -----------------------------------
+def check_hist_width(col, width, limit, hist):
+    """Brute force: contiguous buckets of `width` IDs from the first key
+    below the limit through the last, each counting its keys."""
+    lowers, uppers, freq = (np.asarray(x, np.int64) for x in hist)
+    keys = np.asarray(col, np.int64)
+    keys = keys[keys < limit]
+    if len(keys) == 0:
+        assert len(lowers) == 0
+        return
+    assert lowers[0] == keys.min() and uppers[-1] == keys.max()
+    assert (np.diff(lowers) == width).all() and (uppers[:-1] - lowers[:-1] == width - 1).all()
+    for lo, hi, f in zip(lowers, uppers, freq):
+        assert f == np.count_nonzero((keys >= lo) & (keys <= hi))
+
+
+def check_hist_depth(col, depth, limit, hist):
+    """Buckets partition the sorted rows below the limit in order; a
+    bucket from row s ends at the last row of the key held by row
+    min(s + depth, m - 1) (found with searchsorted on the key, not by
+    walking); lower / upper are the keys at its ends."""
+    lowers, uppers, freq = (np.asarray(x, np.int64) for x in hist)
+    keys = np.asarray(col, np.int64)
+    m = int(np.searchsorted(keys, limit, "left"))
+    assert int(freq.sum()) == m
+    s = 0
+    for lo, hi, f in zip(lowers, uppers, freq):
+        e0 = min(s + depth, m - 1)
+        end = int(np.searchsorted(keys, keys[e0], "right")) - 1
+        end = min(end, m - 1)
+        assert s + f - 1 == end and lo == keys[s] and hi == keys[e0]
+        s = end + 1
+    assert s == m
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_histograms_equal_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    # Zipf-like keys (long runs at small IDs) plus "literals" above a limit
+    keys = np.sort(np.concatenate([rng.zipf(1.3, 20_000).clip(1, 5_000),
+                                   rng.integers(60_000, 70_000, 3000)]).astype(np.uint32))
+    for limit in (50_000 - 1, 0xFFFFFFFF, 1, 3):
+        for w in (1, 7, 100):
+            h = O.hist_eq_width(keys, w, limit)
+            check_hist_width(keys, w, limit, h)
+        for d in (1, 10, 1000, 50_000):
+            h = O.hist_eq_depth(keys, d, limit)
+            check_hist_depth(keys, d, limit, h)
+            if len(h[2]) > 1:
+                assert min(h[2][:-1]) >= d + 1  # the reference's depth + 1 rows
+    # the span a multiple of the width: the last key still has its bucket
+    h = O.hist_eq_width(np.array([1, 50, 101], np.uint32), 100, 1000)
+    assert h == ([1, 101], [100, 101], [2, 1])
+    assert O.hist_eq_width(np.array([], np.uint32), 10, 99) == ([], [], [])

@@ -59,6 +59,7 @@ EXPORTS = {
     "vd_ctx_stream": (C.c_void_p, [C.c_void_p]),
     "vd_ctx_trim": (C.c_int, [C.c_void_p]),
     "vd_ctx_mem": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "vd_ctx_mem_peak": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.c_int]),
     "vd_ctx_tune": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
     "vd_dev_alloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     "vd_dev_free": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -76,6 +77,9 @@ EXPORTS = {
                                   C.c_int, C.c_uint32, C.POINTER(C.c_uint64),
                                   C.POINTER(C.c_uint64)]),
     "vd_index_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "vd_index_histogram": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_uint32,
+                                     C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_uint64, C.POINTER(C.c_uint64)]),
     "vd_index_lookup_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                         C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]),
     "vd_rel_create": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),
@@ -216,6 +220,12 @@ class Ctx:
         a, b = C.c_uint64(), C.c_uint64()
         self.check(self.L.vd_ctx_mem(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def mem_peak(self, reset: bool = False) -> int:
+        """Highest in-use bytes since the last reset (reset: start again)."""
+        v = C.c_uint64()
+        self.check(self.L.vd_ctx_mem_peak(self.h, C.byref(v), int(reset)))
+        return v.value
 
     # -- device buffers
     def alloc(self, nbytes: int) -> int:
@@ -392,6 +402,24 @@ class Index:
         return {"l1v": d(v.l1_values, v.n_l1), "l1o": d(v.l1_offsets, v.n_l1),
                 "l2v": d(v.l2_values, v.n_l2), "l2o": d(v.l2_offsets, v.n_l2),
                 "y": d(v.col_y, v.n), "z": d(v.col_z, v.n)}
+
+    def histogram(self, perm: str, depth: bool, param: int, id_limit: int = 0xFFFFFFFF):
+        """Equal-width (depth=False) or equal-depth histogram of the leading
+        column: (lowers, uppers, freqs) arrays."""
+        cap = 1 << 16
+        while True:
+            lo, hi = np.empty(cap, np.uint32), np.empty(cap, np.uint32)
+            fr = np.empty(cap, np.uint64)
+            nb = C.c_uint64()
+            rc = self.ctx.L.vd_index_histogram(self.ctx.h, self.h, PERMS[perm], int(depth), param,
+                                               id_limit, _ptr(lo), _ptr(hi), _ptr(fr), cap,
+                                               C.byref(nb))
+            if rc == VD_ERANGE and nb.value > cap:
+                cap = nb.value
+                continue
+            self.ctx.check(rc)
+            k = nb.value
+            return lo[:k].copy(), hi[:k].copy(), fr[:k].copy()
 
     def lookup(self, perm: str, k1: int, k2: int | None = None):
         b, e = C.c_uint64(), C.c_uint64()

@@ -59,6 +59,7 @@ void *dalloc(vd_ctx *ctx, size_t bytes) {
     }
     ctx->live_blocks[p] = c;
     ctx->bytes_live += c;
+    if (ctx->bytes_live > ctx->bytes_peak) ctx->bytes_peak = ctx->bytes_live;
     return p;
 }
 
@@ -347,6 +348,13 @@ int vd_ctx_mem(vd_ctx *ctx, uint64_t *in_use, uint64_t *cached) {
     if (!ctx) return VD_EINVAL;
     if (in_use) *in_use = ctx->bytes_live;
     if (cached) *cached = ctx->bytes_cached;
+    return VD_OK;
+}
+
+int vd_ctx_mem_peak(vd_ctx *ctx, uint64_t *peak, int reset) {
+    if (!ctx) return VD_EINVAL;
+    if (peak) *peak = ctx->bytes_peak;
+    if (reset) ctx->bytes_peak = ctx->bytes_live;
     return VD_OK;
 }
 

@@ -269,6 +269,68 @@ __global__ __launch_bounds__(256) void k_index_stats(
     pairs[i] = dev::lower_bound(l2o, 0, n_l2, e) - dev::lower_bound(l2o, 0, n_l2, b);
 }
 
+// ---- histograms of a permutation's leading column -----------------------------
+// Only the rows whose leading key is below a limit count; the L1 arrays give
+// every key's run of rows, so no row is read.  pos(i) = first row of L1
+// entry i, or rows_in (the rows below the limit) for i == m.
+__device__ __forceinline__ uint64_t hist_pos(const uint32_t *l1o, uint64_t m, uint64_t rows_in,
+                                             uint64_t i) {
+    return i < m ? l1o[i] : rows_in;
+}
+
+// Equal width: bucket b holds keys [start + b w, min(start + (b + 1) w - 1, end)].
+__global__ __launch_bounds__(256) void k_hist_width(
+    const uint32_t *__restrict__ l1v, const uint32_t *__restrict__ l1o, uint64_t m,
+    uint64_t rows_in, uint32_t start, uint32_t end, uint32_t width, uint64_t nb,
+    uint64_t *__restrict__ freq) {
+    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const uint64_t lo = (uint64_t)start + b * width;
+    uint64_t hi = lo + width - 1;
+    if (hi > end) hi = end;
+    const uint64_t i0 = dev::lower_bound(l1v, 0, m, (uint32_t)lo);
+    const uint64_t i1 = dev::upper_bound(l1v, 0, m, (uint32_t)hi);
+    freq[b] = hist_pos(l1o, m, rows_in, i1) - hist_pos(l1o, m, rows_in, i0);
+}
+
+// Equal depth: a bucket that starts at L1 entry g ends with the entry that
+// holds row min(pos(g) + depth, rows_in - 1) (a key's run is never split);
+// nxt[g] = the entry after it, where the next bucket starts.
+__global__ __launch_bounds__(256) void k_hist_depth_next(
+    const uint32_t *__restrict__ l1o, uint64_t m, uint64_t rows_in, uint32_t depth,
+    uint32_t *__restrict__ nxt) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= m) return;
+    uint64_t e = (uint64_t)l1o[g] + depth;
+    if (e > rows_in - 1) e = rows_in - 1;
+    nxt[g] = (uint32_t)dev::upper_bound(l1o, 0, m, (uint32_t)e);
+}
+
+// Buckets of the chain of starts: lower = first key, upper = last key,
+// freq = rows.
+__global__ __launch_bounds__(256) void k_hist_depth_gather(
+    const uint32_t *__restrict__ l1v, const uint32_t *__restrict__ l1o, uint64_t m,
+    uint64_t rows_in, const uint32_t *__restrict__ nxt, const uint32_t *__restrict__ chain,
+    uint64_t nb, uint32_t *__restrict__ lowers, uint32_t *__restrict__ uppers,
+    uint64_t *__restrict__ freq) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nb) return;
+    const uint32_t g = chain[k], g2 = nxt[g];
+    lowers[k] = l1v[g];
+    uppers[k] = l1v[g2 - 1];
+    freq[k] = hist_pos(l1o, m, rows_in, g2) - l1o[g];
+}
+
+// be[0] = L1 entries with a key below *limit, be[1] = their rows.
+__global__ void k_hist_limit(const uint32_t *__restrict__ l1v, const uint32_t *__restrict__ l1o,
+                             uint64_t n_l1, uint64_t n, const uint32_t *__restrict__ limit,
+                             uint64_t *__restrict__ be) {
+    if (threadIdx.x != 0) return;
+    const uint64_t m = dev::lower_bound(l1v, 0, n_l1, *limit);
+    be[0] = m;
+    be[1] = m < n_l1 ? l1o[m] : n;
+}
+
 // Rows of query q are output rows [off[q], off[q+1]); one thread per output
 // row finds its query by binary search over off (nq + 1 entries).
 __global__ __launch_bounds__(256) void k_gather_ranges(
@@ -840,6 +902,79 @@ int vd_index_stats(vd_ctx *ctx, const vd_index *idx, int perm, uint64_t *d_rows,
                dim3((unsigned)ceil_div(pa.n_l1, 256)), dim3(256), 0, k_index_stats,
                (const uint32_t *)pa.l1o, pa.n_l1, (const uint32_t *)pa.l2o, pa.n_l2, pa.n,
                d_rows, d_pairs);
+    });
+}
+
+int vd_index_histogram(vd_ctx *ctx, const vd_index *idx, int perm, int kind, uint32_t param,
+                       uint32_t id_limit, uint32_t *lowers, uint32_t *uppers, uint64_t *freqs,
+                       uint64_t cap, uint64_t *nbuckets) {
+    if (!ctx || !nbuckets || (cap && (!lowers || !uppers || !freqs))) return VD_EINVAL;
+    *nbuckets = 0;
+    VD_TRY(ctx, {
+        VD_REQUIRE(kind == VD_HIST_EQUAL_WIDTH || kind == VD_HIST_EQUAL_DEPTH, VD_EINVAL,
+                   "unknown histogram kind");
+        VD_REQUIRE(param > 0, VD_EINVAL, "histogram width / depth must be positive");
+        const vd_perm_arrays &pa = get_perm(idx, perm);
+        // the L1 entries with a key below id_limit, and their rows
+        std::vector<uint32_t> hv;
+        uint64_t m = 0, rows_in = 0;
+        if (pa.n_l1) {
+            DBuf<uint32_t> k(ctx, 1);
+            DBuf<uint64_t> be(ctx, 2);
+            VD_HIP(hipMemcpyAsync(k.p, &id_limit, 4, hipMemcpyHostToDevice, ctx->stream));
+            // first L1 entry >= id_limit, as a one-constant lookup's begin
+            launch(ctx, "index_lookup", 24.0, dim3(1), dim3(64), 0, k_hist_limit,
+                   (const uint32_t *)pa.l1v, (const uint32_t *)pa.l1o, pa.n_l1, pa.n,
+                   (const uint32_t *)k.p, be.p);
+            uint64_t r[2];
+            read_words(ctx, r, be.p, 2);
+            m = r[0];
+            rows_in = r[1];
+        }
+        if (m == 0) return VD_OK;
+        uint32_t ends[2];
+        download(ctx, &ends[0], pa.l1v, 4);
+        download(ctx, &ends[1], pa.l1v + m - 1, 4);
+        if (kind == VD_HIST_EQUAL_WIDTH) {
+            const uint64_t nb = (uint64_t)(ends[1] - ends[0]) / param + 1;
+            *nbuckets = nb;
+            VD_REQUIRE(nb <= cap, VD_ERANGE, "more histogram buckets than cap");
+            DBuf<uint64_t> f(ctx, nb);
+            launch(ctx, "histogram", 16.0 * (double)nb, dim3((unsigned)ceil_div(nb, 256)),
+                   dim3(256), 0, k_hist_width, (const uint32_t *)pa.l1v,
+                   (const uint32_t *)pa.l1o, m, rows_in, ends[0], ends[1], param, nb, f.p);
+            download(ctx, freqs, f.p, nb * sizeof(uint64_t));
+            for (uint64_t b = 0; b < nb; ++b) {
+                const uint64_t lo = (uint64_t)ends[0] + b * param;
+                lowers[b] = (uint32_t)lo;
+                uppers[b] = (uint32_t)std::min<uint64_t>(lo + param - 1, ends[1]);
+            }
+        } else {
+            // every L1 entry's successor bucket start on the device, the chain
+            // of starts from entry 0 on the host (one step per bucket), then
+            // the buckets' keys and rows on the device
+            DBuf<uint32_t> nxt(ctx, m);
+            launch(ctx, "histogram", 8.0 * (double)m, dim3((unsigned)ceil_div(m, 256)), dim3(256),
+                   0, k_hist_depth_next, (const uint32_t *)pa.l1o, m, rows_in, param, nxt.p);
+            std::vector<uint32_t> hn(m);
+            download(ctx, hn.data(), nxt.p, m * sizeof(uint32_t));
+            std::vector<uint32_t> chain;
+            for (uint64_t g = 0; g < m; g = hn[g]) chain.push_back((uint32_t)g);
+            const uint64_t nb = chain.size();
+            *nbuckets = nb;
+            VD_REQUIRE(nb <= cap, VD_ERANGE, "more histogram buckets than cap");
+            DBuf<uint32_t> dc(ctx, nb);
+            upload(ctx, dc.p, chain.data(), nb * sizeof(uint32_t));
+            DBuf<uint32_t> lo(ctx, nb), hi(ctx, nb);
+            DBuf<uint64_t> f(ctx, nb);
+            launch(ctx, "histogram", 24.0 * (double)nb, dim3((unsigned)ceil_div(nb, 256)),
+                   dim3(256), 0, k_hist_depth_gather, (const uint32_t *)pa.l1v,
+                   (const uint32_t *)pa.l1o, m, rows_in, (const uint32_t *)nxt.p,
+                   (const uint32_t *)dc.p, nb, lo.p, hi.p, f.p);
+            download(ctx, lowers, lo.p, nb * 4);
+            download(ctx, uppers, hi.p, nb * 4);
+            download(ctx, freqs, f.p, nb * 8);
+        }
     });
 }
 

@@ -79,6 +79,7 @@ struct vd_ctx {
     std::multimap<size_t, void *> free_blocks;
     std::unordered_map<void *, size_t> live_blocks;
     uint64_t bytes_live = 0, bytes_cached = 0;
+    uint64_t bytes_peak = 0;  // highest bytes_live since the last reset
     // Guards the pool: a relation may be freed from another host thread
     // than the one issuing work on the context.
     std::recursive_mutex alloc_mu;

@@ -35,7 +35,7 @@ class ServeCfg(C.Structure):
 class ServeRec(C.Structure):
     _fields_ = [("arrival_ms", C.c_double), ("start_ms", C.c_double), ("done_ms", C.c_double),
                 ("est_bytes", C.c_double), ("rows", C.c_uint64), ("batch", C.c_int),
-                ("slot", C.c_int)]
+                ("slot", C.c_int), ("mem_held", C.c_int)]
 
 
 class TuneResult(C.Structure):

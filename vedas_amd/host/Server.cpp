@@ -58,7 +58,8 @@ void Server::run(std::vector<ServeRequest> &reqs, std::vector<ServeRecord> &recs
     std::condition_variable cv_work, cv_done;
     std::vector<Slot> slots(nslots);
     bool stop = false;
-    double in_flight = 0;
+    double in_flight = 0;   // estimated bytes of the requests on the GPU
+    int in_flight_reqs = 0; // their number (the byte sum carries rounding)
     int finished = 0;
     std::string failure;
     const auto t0 = Clock::now();
@@ -116,6 +117,8 @@ void Server::run(std::vector<ServeRequest> &reqs, std::vector<ServeRecord> &recs
                     }
                     in_flight -= need[batch[k]];
                 }
+                in_flight_reqs -= (int)batch.size();
+                if (in_flight_reqs == 0) in_flight = 0;
                 if (!err.empty() && failure.empty()) failure = err;
                 finished += (int)batch.size();
                 slots[si].batch.clear();
@@ -157,13 +160,17 @@ void Server::run(std::vector<ServeRequest> &reqs, std::vector<ServeRecord> &recs
                     const bool fits = in_flight + add + need[r] <= (double)cfg.mem_budget;
                     // nothing in flight and an empty batch: admit even an
                     // oversized request alone rather than stall
-                    if (!fits && !(batch.empty() && in_flight + add == 0)) break;
+                    if (!fits && !(batch.empty() && in_flight_reqs == 0)) {
+                        recs[r].mem_held = 1;  // waits for memory to free up
+                        break;
+                    }
                     batch.push_back(r);
                     add += need[r];
                     queue.pop_front();
                 }
                 if (batch.empty()) break;  // head does not fit yet
                 in_flight += add;
+                in_flight_reqs += (int)batch.size();
                 slots[si].batch = batch;
                 slots[si].busy = true;
                 slots[si].batch_id = batch_id++;

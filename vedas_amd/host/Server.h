@@ -49,6 +49,7 @@ struct ServeRecord {
     double est_bytes = 0;
     uint64_t rows = 0;
     int batch = -1, slot = -1;
+    int mem_held = 0;  // 1: waited at the head of the queue for memory
 };
 
 class Server {

@@ -1,25 +1,44 @@
 // Store statistics and plan size estimates.
 //
-// Statistics: per leading key of each built permutation, its row count and
-// its number of distinct (X,Y) pairs (vd_index_stats), kept on the host.
-// For the predicate-first permutations this gives, per predicate, the
-// triple count and the distinct subjects (PSO) / objects (POS) -- the role
-// of the reference's Histogram and DataMetaInfo (Histogram.cu:170-231,
-// DataMetaInfo.h:7-20), which its planner and this build's admission use.
+// Statistics, computed on the device when the store is loaded:
+//  * per leading key of PSO / POS, its row count and its number of distinct
+//    (X,Y) pairs (vd_index_stats): per predicate, the triples and the
+//    distinct subjects / objects;
+//  * equal-depth and equal-width histograms of the subject and object
+//    columns (vd_index_histogram over SPO / OSP), as the reference's
+//    Histogram builds them at index build time (Histogram.cu:170-231,
+//    VedasStorage.cu:316-317, 518-536) with its default sizes (depth 1000,
+//    width 100; VedasStorage.cu:526-527), here over every key.
 //
-// PlanEstimator: expected rows of every plan node under independence
-// (|L join R| = |L| |R| / max(d_L(v), d_R(v))) and the HBM bytes a plan
-// holds at its peak, for memory admission in the serving loop.
+// PlanEstimator: expected rows of every plan node and the HBM bytes a plan
+// holds at its peak, for memory admission in the serving loop.  Join sizes
+// come from key-distribution moments: a variable that entered a relation
+// through a patterns in which it is the subject and b in which it is the
+// object has, under independence, the distribution s(v)^a o(v)^b of its
+// value v (s, o: the subject and object key shares), so
+//   |A join B on v| = |A| |B| M(a1+a2, b1+b2) / (M(a1,b1) M(a2,b2)),
+//   M(a,b) = sum_v s(v)^a o(v)^b,
+// with M read from the equal-depth histograms as piecewise-constant shares
+// (a key holding more rows than the depth has a bucket of its own, so the
+// heavy keys that dominate these sums are exact).  Without the histograms
+// (no SPO / OSP) the estimate falls back to |A| |B| / max(d_A, d_B).
 #ifndef VEDAS_HOST_STATISTICS_H
 #define VEDAS_HOST_STATISTICS_H
 
 #include <map>
+#include <mutex>
 #include <string>
-#include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include "PlanTreeNode.h"
 #include "VedasStorage.h"
+
+struct KeyHistogram {
+    std::vector<TYPEID> lowers, uppers;
+    std::vector<uint64_t> freqs;
+    uint64_t rows() const;
+};
 
 class Statistics {
   public:
@@ -31,11 +50,25 @@ class Statistics {
     uint64_t pairs(int perm, TYPEID x) const;
     bool has(int perm) const { return keys_[perm].size() > 0 || built_[perm]; }
 
+    static constexpr unsigned kDepth = 1000;  // Histogram default depth
+    static constexpr unsigned kWidth = 100;   // Histogram default width
+    // Histograms of the subject (pos 0) and object (pos 2) columns.
+    bool hasHistograms() const { return !depth_[0].lowers.empty() && !depth_[2].lowers.empty(); }
+    const KeyHistogram &depthHistogram(int pos) const { return depth_[pos]; }
+    const KeyHistogram &widthHistogram(int pos) const { return width_[pos]; }
+    // sum over keys v of s(v)^a o(v)^b (key shares from the equal-depth
+    // histograms); 1 for a == b == 0 is not a key count but the empty sum's
+    // neutral value used by the estimator.
+    double moment(int a, int b) const;
+
   private:
     uint64_t n_ = 0;
     bool built_[VD_NUM_PERMS] = {};
     std::vector<TYPEID> keys_[VD_NUM_PERMS];
     std::vector<uint64_t> rows_[VD_NUM_PERMS], pairs_[VD_NUM_PERMS];
+    KeyHistogram depth_[3], width_[3];
+    mutable std::mutex mu_;
+    mutable std::map<std::pair<int, int>, double> moments_;
 };
 
 struct PlanEstimate {
@@ -50,14 +83,21 @@ class PlanEstimator {
     PlanEstimate estimate(const std::vector<TriplePattern> &patterns, PlanTreeNode *root) const;
 
   private:
+    struct Var {
+        double distinct = 1;
+        int a = -1, b = -1;  // subject / object occurrences (-1: unknown)
+    };
     struct Est {
         double rows = 0;
-        std::map<std::string, double> distinct;  // per variable
+        std::map<std::string, Var> vars;
         int cols = 0;
+        std::string key;    // the variable of column 0
+        bool view = false;  // rows of an index permutation (no device memory)
     };
     Est node(PlanTreeNode *n, const std::vector<TriplePattern> &pats, PlanEstimate &acc,
              double &live) const;
-    Est select(const TriplePattern &tp) const;
+    Est select(const TriplePattern &tp, const std::string &var) const;
+    void rekey(Est &e, const std::string &var, PlanEstimate &acc, double &live) const;
     const Statistics &st_;
 };
 

@@ -500,6 +500,7 @@ int vdq_serve(vdq_engine *e, const char *const *texts, const double *arrival_ms,
             recs[i].start_ms = rr[i].start_ms;
             recs[i].done_ms = rr[i].done_ms;
             recs[i].est_bytes = rr[i].est_bytes;
+            recs[i].mem_held = rr[i].mem_held;
             recs[i].rows = rr[i].rows;
             recs[i].batch = rr[i].batch;
             recs[i].slot = rr[i].slot;
