@@ -106,9 +106,19 @@ int  vd_index_build(vd_ctx *ctx, const uint32_t *d_s, const uint32_t *d_p,
  * file holds: VedasStorage::open, VedasStorage.cu:452-484): for every
  * permutation p in perm_mask, views[p] gives n, n_l1, n_l2 and the six
  * arrays with vd_index_view's meaning.  The arrays are copied (the caller
- * keeps its buffers); only sizes are checked, not the sort order. */
+ * keeps its buffers); only sizes are checked, not the sort order.  Two
+ * arrays the reference's full-index file does not hold may be left NULL
+ * and are then rebuilt on the device: col_y (each row's Y from its L2 run;
+ * the file has no Y column for SPO, SOP and OPS) or l2_values / l2_offsets
+ * (the (X, Y) run heads of the rows; the file has no L2 for OSP). */
 int  vd_index_from_arrays(vd_ctx *ctx, uint32_t perm_mask, const vd_perm_view_t *views,
                           vd_index **out);
+/* Rows of runs: out[r] = values[k] for the run k that holds row r, run k
+ * covering rows [offsets[k], offsets[k+1]) and the last run ending at n
+ * (device arrays).  Expands an L1 or L2 array into its column, e.g. the
+ * predicate of every PSO row to rebuild the triples of a .vds file. */
+int  vd_runs_expand(vd_ctx *ctx, const uint32_t *values, const uint32_t *offsets,
+                    uint64_t nruns, uint64_t n, uint32_t *out);
 int  vd_index_free(vd_ctx *ctx, vd_index *idx);
 int  vd_index_view(const vd_index *idx, int perm, vd_perm_view_t *out);
 uint32_t vd_index_perms(const vd_index *idx);   /* mask of built perms */

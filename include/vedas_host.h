@@ -69,12 +69,25 @@ int vdq_dict_triples(const vdq_dict *d, uint32_t *s, uint32_t *p, uint32_t *o);
 int vdq_dict_term(const vdq_dict *d, int is_predicate, uint32_t id, char *out, size_t cap);
 int vdq_dict_translate(const vdq_dict *d, const char *sparql, char *out, size_t cap);
 int vdq_engine_load_dict(vdq_engine *e, const vdq_dict *d, uint32_t perm_mask);
+/* The .vdd dictionary file of a store (save_dict, VedasBuild.cu:145-160;
+ * load_dict, VedasQuery.cu:213-268): per dictionary (subject/object,
+ * predicate, literal) a count line, then "id term" lines, terms in
+ * N-Triples form.  vdq_dict_open replaces the dictionary's terms (its
+ * triples are cleared).  vdq_engine_set_dict gives an engine whose store
+ * came from a .vds file (vdq_engine_open) the dictionary vdq_sparql uses
+ * (the caller keeps it alive). */
+int vdq_dict_save(const vdq_dict *d, const char *path);
+int vdq_dict_open(vdq_dict *d, const char *path);
+int vdq_engine_set_dict(vdq_engine *e, const vdq_dict *d);
 int vdq_sparql(vdq_engine *e, const char *sparql, vdq_result **out);
 int vdq_result_is_predicate(const vdq_result *r, int col);
 
-/* The .vds file of the reference's default index (PSO and POS):
- * VedasStorage::write / VedasStorage::open (VedasStorage.cu:452-515).
- * open replaces the engine's store with the file's (no sort). */
+/* The .vds store file: VedasStorage::write / VedasStorage::open
+ * (VedasStorage.cu:452-516).  save writes the default index (PSO, POS) and,
+ * when SPO, SOP, OSP and OPS are built too, the full-index arrays; open
+ * replaces the engine's store with the file's permutations (no sort; a
+ * full-index file written by the reference, whose subject L1 is empty, gets
+ * SPO and SOP by an index build over its triples). */
 int vdq_engine_save(vdq_engine *e, const char *path);
 int vdq_engine_open(vdq_engine *e, const char *path);
 vd_ctx *vdq_engine_ctx(vdq_engine *e);

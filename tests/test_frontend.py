@@ -75,3 +75,66 @@ def test_sparql_forms():
     with pytest.raises(_native.VedasError):
         d.load_text('<http://x/a> <http://x/p> <http://x/b>\n')  # no final dot
     d.close()
+
+
+def parse_vdd(path):
+    """The .vdd layout read independently: per section a count line, then
+    'id term' lines (the term is everything after the first space)."""
+    lines = open(path).read().split("\n")
+    pos, out = 0, []
+    while pos < len(lines) and lines[pos] != "":
+        n = int(lines[pos])
+        sec = {}
+        for ln in lines[pos + 1:pos + 1 + n]:
+            i, term = ln.split(" ", 1)
+            sec[term] = int(i)
+        out.append(sec)
+        pos += 1 + n
+    return out
+
+
+@pytest.mark.parametrize("ldict", [False, True])
+def test_vdd_round_trip(tmp_path, ldict):
+    """save writes the reference's .vdd layout with the loader's numbering;
+    open reads it back: every term has the same ID and SPARQL translates the
+    same way."""
+    from vedas_amd.engine import Dict
+    text = open(W3C).read()
+    so, p, lit, _ = restate(text, ldict)
+    d = Dict(ldict)
+    d.load(W3C)
+    path = str(tmp_path / "w3c.vdd")
+    d.save(path)
+    secs = parse_vdd(path)
+    assert secs[0] == so and secs[1] == p and secs[2] == lit
+    d2 = Dict(False)
+    d2.open(path)
+    for term, i in so.items():
+        assert d2.term(i) == term
+    for term, i in p.items():
+        assert d2.term(i, True) == term
+    for term, i in lit.items():
+        assert d2.term(i) == term
+    q = ("SELECT * WHERE { ?s <http://example.org/property> ?o . "
+         "?o <http://example.org/property> ?z }")
+    assert d2.translate(q) == d.translate(q)
+
+
+def test_vdd_reference_style_file(tmp_path):
+    """A file as the reference's vdBuild writes it after an ID reassignment:
+    IDs in any order and not contiguous, no literal section."""
+    from vedas_amd.engine import Dict
+    path = tmp_path / "ref.vdd"
+    path.write_text("3\n7 <http://x/b>\n2 <http://x/a>\n5 _:n1\n"
+                    "2\n2 <http://x/q>\n1 <http://x/p>\n")
+    d = Dict(False)
+    d.open(str(path))
+    assert d.term(2) == "<http://x/a>" and d.term(7) == "<http://x/b>" and d.term(5) == "_:n1"
+    assert d.term(1, True) == "<http://x/p>"
+    with pytest.raises(Exception):
+        d.term(3)  # a gap in the IDs
+    assert "7" in d.translate("SELECT * WHERE { <http://x/b> <http://x/q> ?o }")
+    for bad in ("x\n", "2\n1 <a>\n", "1\nfoo <a>\n0\n"):
+        path.write_text(bad)
+        with pytest.raises(Exception):
+            d.open(str(path))

@@ -69,6 +69,8 @@ EXPORTS = {
     "vd_index_build": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p)]),
     "vd_index_free": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "vd_runs_expand": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64,
+                                 C.c_void_p]),
     "vd_index_from_arrays": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(PermView),
                                        C.POINTER(C.c_void_p)]),
     "vd_index_view": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(PermView)]),

@@ -269,6 +269,56 @@ __global__ __launch_bounds__(256) void k_index_stats(
     pairs[i] = dev::lower_bound(l2o, 0, n_l2, e) - dev::lower_bound(l2o, 0, n_l2, b);
 }
 
+// ---- runs (a permutation's L1 / L2 arrays) and rows ---------------------------------
+// out[r] = values[k] for the run k holding row r: run k covers rows
+// [offsets[k], offsets[k + 1]) (the last run ends at n).  A thread expands
+// RUN_ROWS consecutive rows, searching only for its first.
+constexpr int RUN_ROWS = 16;
+__global__ __launch_bounds__(256) void k_runs_expand(const uint32_t *__restrict__ values,
+                                                     const uint32_t *__restrict__ offsets,
+                                                     uint64_t nruns, uint64_t n,
+                                                     uint32_t *__restrict__ out) {
+    const uint64_t r0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * RUN_ROWS;
+    if (r0 >= n) return;
+    uint64_t k = dev::upper_bound(offsets, 0, nruns, (uint32_t)r0);
+    k = k > 0 ? k - 1 : 0;  // offsets[0] is 0 in a valid array
+    uint64_t next = k + 1 < nruns ? offsets[k + 1] : n;
+    const uint64_t r1 = r0 + RUN_ROWS < n ? r0 + RUN_ROWS : n;
+    for (uint64_t r = r0; r < r1; ++r) {
+        while (r >= next) {
+            ++k;
+            next = k + 1 < nruns ? offsets[k + 1] : n;
+        }
+        out[r] = values[k];
+    }
+}
+
+// head[r] = 1 where row r starts an (X, Y) run: an L1 offset or a change of Y.
+__global__ __launch_bounds__(256) void k_mark_l1(const uint32_t *__restrict__ l1o, uint64_t n_l1,
+                                                 uint64_t n, uint32_t *__restrict__ head) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_l1 && l1o[i] < n) head[l1o[i]] = 1;
+}
+
+__global__ __launch_bounds__(256) void k_mark_y(const uint32_t *__restrict__ y, uint64_t n,
+                                                uint32_t *__restrict__ head) {
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    if (r == 0 || y[r] != y[r - 1]) head[r] = 1;
+}
+
+// L2 entries at the heads: pos = exclusive scan of head.
+__global__ __launch_bounds__(256) void k_scatter_heads(const uint32_t *__restrict__ head,
+                                                       const uint64_t *__restrict__ pos,
+                                                       const uint32_t *__restrict__ y, uint64_t n,
+                                                       uint32_t *__restrict__ l2v,
+                                                       uint32_t *__restrict__ l2o) {
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n || !head[r]) return;
+    l2v[pos[r]] = y[r];
+    l2o[pos[r]] = (uint32_t)r;
+}
+
 // ---- histograms of a permutation's leading column -----------------------------
 // Only the rows whose leading key is below a limit count; the L1 arrays give
 // every key's run of rows, so no row is read.  pos(i) = first row of L1
@@ -795,6 +845,45 @@ int vd_index_build(vd_ctx *ctx, const uint32_t *d_s, const uint32_t *d_p,
     return VD_OK;
 }
 
+namespace {
+
+// Rows of runs: out[r] = values of the run holding row r.
+void runs_expand(vd_ctx *ctx, const uint32_t *values, const uint32_t *offsets, uint64_t nruns,
+                 uint64_t n, uint32_t *out) {
+    if (n == 0) return;
+    VD_REQUIRE(nruns > 0, VD_EINVAL, "rows without runs");
+    launch(ctx, "runs_expand", 8.0 * (double)n,
+           dim3((unsigned)ceil_div(ceil_div(n, (uint64_t)RUN_ROWS), 256)), dim3(256), 0,
+           k_runs_expand, values, offsets, nruns, n, out);
+}
+
+// The L2 of rows with the given L1 and Y column: (X, Y) run heads.
+void l2_from_rows(vd_ctx *ctx, const uint32_t *l1o, uint64_t n_l1, const uint32_t *y, uint64_t n,
+                  uint32_t **l2v, uint32_t **l2o, uint64_t *n_l2) {
+    DBuf<uint32_t> head(ctx, n);
+    DBuf<uint64_t> pos(ctx, n + 1);
+    fill_u32(ctx, head.p, 0, n);
+    launch(ctx, "l2_heads", 4.0 * (double)n_l1, dim3((unsigned)ceil_div(n_l1, 256)), dim3(256), 0,
+           k_mark_l1, l1o, n_l1, n, head.p);
+    launch(ctx, "l2_heads", 8.0 * (double)n, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0,
+           k_mark_y, y, n, head.p);
+    scan_u32_to_u64(ctx, head.p, pos.p, n, pos.p + n);
+    const uint64_t m = read_u64(ctx, pos.p + n);
+    *l2v = (uint32_t *)dalloc(ctx, (m ? m : 1) * 4);
+    *l2o = (uint32_t *)dalloc(ctx, (m ? m : 1) * 4);
+    launch(ctx, "l2_heads", 16.0 * (double)n, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0,
+           k_scatter_heads, (const uint32_t *)head.p, (const uint64_t *)pos.p, y, n, *l2v, *l2o);
+    *n_l2 = m;
+}
+
+}  // namespace
+
+int vd_runs_expand(vd_ctx *ctx, const uint32_t *values, const uint32_t *offsets, uint64_t nruns,
+                   uint64_t n, uint32_t *out) {
+    if (!ctx || (n && (!values || !offsets || !out))) return VD_EINVAL;
+    VD_TRY(ctx, runs_expand(ctx, values, offsets, nruns, n, out));
+}
+
 int vd_index_from_arrays(vd_ctx *ctx, uint32_t perm_mask, const vd_perm_view_t *views,
                          vd_index **out) {
     if (!ctx || !out || !views) return VD_EINVAL;
@@ -811,10 +900,12 @@ int vd_index_from_arrays(vd_ctx *ctx, uint32_t perm_mask, const vd_perm_view_t *
             if (!(perm_mask & VD_PERM_BIT(p))) continue;
             const vd_perm_view_t &v = views[p];
             VD_REQUIRE(first || v.n == idx->n, VD_EINVAL, "permutations of different sizes");
-            VD_REQUIRE(v.n < (1ull << 32) && v.n_l1 <= v.n && v.n_l2 <= v.n && v.n_l1 <= v.n_l2,
+            const bool have_l2 = v.l2_values && v.l2_offsets;
+            VD_REQUIRE(v.n < (1ull << 32) && v.n_l1 <= v.n &&
+                           (!have_l2 || (v.n_l2 <= v.n && v.n_l1 <= v.n_l2)),
                        VD_EINVAL, "inconsistent permutation sizes");
-            VD_REQUIRE(v.n == 0 || (v.col_y && v.col_z && v.l1_values && v.l1_offsets &&
-                                    v.l2_values && v.l2_offsets),
+            VD_REQUIRE(v.n == 0 || (v.col_z && v.l1_values && v.l1_offsets &&
+                                    (v.col_y || have_l2)),
                        VD_EINVAL, "null permutation array");
             first = false;
             idx->n = v.n;
@@ -826,13 +917,28 @@ int vd_index_from_arrays(vd_ctx *ctx, uint32_t perm_mask, const vd_perm_view_t *
             };
             pa.n = v.n;
             pa.n_l1 = v.n_l1;
-            pa.n_l2 = v.n_l2;
-            pa.y = copy(v.col_y, v.n);
             pa.z = copy(v.col_z, v.n);
             pa.l1v = copy(v.l1_values, v.n_l1);
             pa.l1o = copy(v.l1_offsets, v.n_l1);
-            pa.l2v = copy(v.l2_values, v.n_l2);
-            pa.l2o = copy(v.l2_offsets, v.n_l2);
+            if (have_l2) {
+                pa.n_l2 = v.n_l2;
+                pa.l2v = copy(v.l2_values, v.n_l2);
+                pa.l2o = copy(v.l2_offsets, v.n_l2);
+            }
+            if (v.col_y) {
+                pa.y = copy(v.col_y, v.n);
+            } else {  // Y of each row from its (X, Y) run
+                pa.y = (uint32_t *)dalloc(ctx, (v.n ? v.n : 1) * 4);
+                runs_expand(ctx, pa.l2v, pa.l2o, pa.n_l2, v.n, pa.y);
+            }
+            if (!have_l2) {
+                if (v.n) {
+                    l2_from_rows(ctx, pa.l1o, pa.n_l1, pa.y, v.n, &pa.l2v, &pa.l2o, &pa.n_l2);
+                } else {
+                    pa.l2v = copy(nullptr, 0);
+                    pa.l2o = copy(nullptr, 0);
+                }
+            }
             pa.built = true;
         }
         idx->hold = make_hold(ctx, idx->perm);

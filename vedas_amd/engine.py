@@ -68,6 +68,9 @@ def lib() -> C.CDLL:
             "vdq_dict_term": (i, [vp, i, C.c_uint32, C.c_char_p, C.c_size_t]),
             "vdq_dict_translate": (i, [vp, C.c_char_p, C.c_char_p, C.c_size_t]),
             "vdq_engine_load_dict": (i, [vp, vp, C.c_uint32]),
+            "vdq_dict_save": (i, [vp, C.c_char_p]),
+            "vdq_dict_open": (i, [vp, C.c_char_p]),
+            "vdq_engine_set_dict": (i, [vp, vp]),
             "vdq_sparql": (i, [vp, C.c_char_p, C.POINTER(vp)]),
             "vdq_result_is_predicate": (i, [vp, i]),
             "vdq_engine_save": (i, [vp, C.c_char_p]),
@@ -177,6 +180,11 @@ class Engine:
         _check(lib().vdq_engine_load_dict(self.h, d.h, mask))
         self._dict = d
 
+    def set_dict(self, d: "Dict") -> None:
+        """The dictionary of a store opened from a .vds file (for sparql)."""
+        _check(lib().vdq_engine_set_dict(self.h, d.h))
+        self._dict = d
+
     def sparql(self, text: str) -> "Result":
         h = C.c_void_p()
         _check(lib().vdq_sparql(self.h, text.encode(), C.byref(h)))
@@ -265,6 +273,15 @@ class Dict:
         _check(lib().vdq_dict_load_ntriples_text(self.h, text.encode(), C.byref(n)))
         self.n = n.value
         return self.n
+
+    def save(self, path: str) -> None:
+        """Write the .vdd dictionary file."""
+        _check(lib().vdq_dict_save(self.h, path.encode()))
+
+    def open(self, path: str) -> None:
+        """Replace the terms with a .vdd file's (no triples afterwards)."""
+        _check(lib().vdq_dict_open(self.h, path.encode()))
+        self.n = 0
 
     def triples(self):
         import numpy as np

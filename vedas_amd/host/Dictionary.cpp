@@ -54,6 +54,77 @@ std::string Dictionary::term(TYPEID id, bool is_predicate) const {
     return (id >= 1 && id <= so_rev_.size()) ? so_rev_[id - 1] : std::string();
 }
 
+// ---- .vdd file ----------------------------------------------------------------
+namespace {
+
+void save_section(std::ofstream &out, const std::vector<std::string> &rev, TYPEID first) {
+    size_t count = 0;
+    for (auto &t : rev) count += !t.empty();
+    out << count << "\n";
+    for (size_t i = 0; i < rev.size(); ++i)
+        if (!rev[i].empty()) out << (first + (TYPEID)i) << " " << rev[i] << "\n";
+}
+
+// One section: the count line, then "id term" lines; an ID at or above
+// `first` goes to rev[id - first].  Returns false at the end of the file
+// before the count (an optional section).
+bool load_section(std::ifstream &in, std::unordered_map<std::string, TYPEID> &map,
+                  std::vector<std::string> &rev, TYPEID first, bool optional, const char *what) {
+    std::string line;
+    if (!std::getline(in, line)) {
+        if (optional) return false;
+        throw VedasError(VD_EINVAL, std::string(".vdd: missing ") + what + " count");
+    }
+    size_t count = 0;
+    try {
+        count = std::stoull(line);
+    } catch (const std::exception &) {
+        throw VedasError(VD_EINVAL, std::string(".vdd: bad ") + what + " count: " + line);
+    }
+    for (size_t k = 0; k < count; ++k) {
+        if (!std::getline(in, line))
+            throw VedasError(VD_EINVAL, std::string(".vdd: ") + what + " section truncated");
+        const size_t sp = line.find(' ');
+        unsigned long long id = 0;
+        try {
+            id = std::stoull(line.substr(0, sp));
+        } catch (const std::exception &) {
+            throw VedasError(VD_EINVAL, std::string(".vdd: bad ") + what + " line: " + line);
+        }
+        if (sp == std::string::npos || id < first || id > 0xFFFFFFFFull ||
+            id - first > (64ull << 20) + count)
+            throw VedasError(VD_EINVAL, std::string(".vdd: bad ") + what + " line: " + line);
+        const std::string term = line.substr(sp + 1);
+        const size_t slot = (size_t)(id - first);
+        if (slot >= rev.size()) rev.resize(slot + 1);
+        rev[slot] = term;
+        map[term] = (TYPEID)id;
+    }
+    return true;
+}
+
+}  // namespace
+
+void Dictionary::save(const char *path) const {
+    std::ofstream out(path, std::ios::out | std::ios::trunc);
+    if (!out) throw VedasError(VD_EINVAL, std::string("cannot open ") + path);
+    save_section(out, so_rev_, 1);
+    save_section(out, p_rev_, 1);
+    save_section(out, l_rev_, (TYPEID)LITERAL_START_ID);
+    if (!out) throw VedasError(VD_EINVAL, std::string("write failed: ") + path);
+}
+
+void Dictionary::open(const char *path) {
+    std::ifstream in(path);
+    if (!in) throw VedasError(VD_EINVAL, std::string("cannot open ") + path);
+    Dictionary d(false);
+    load_section(in, d.so_map_, d.so_rev_, 1, false, "subject/object");
+    load_section(in, d.p_map_, d.p_rev_, 1, false, "predicate");
+    load_section(in, d.l_map_, d.l_rev_, (TYPEID)LITERAL_START_ID, true, "literal");
+    d.ldict_ = !d.l_map_.empty() || ldict_;
+    *this = std::move(d);
+}
+
 // ---- N-Triples ---------------------------------------------------------------
 namespace {
 

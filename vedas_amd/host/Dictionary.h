@@ -35,6 +35,15 @@ class Dictionary {
     // dictionary); empty when the ID is unknown.
     std::string term(TYPEID id, bool is_predicate) const;
 
+    // The .vdd dictionary file (save_dict, VedasBuild.cu:145-160; read by
+    // load_dict, VedasQuery.cu:213-268): for the subject/object, predicate
+    // and literal dictionaries in turn, a line with the term count, then one
+    // "id term" line per term (here in ID order).  open replaces the terms
+    // (the literal dictionary is on when the file has literal terms) and
+    // clears the triples; the literal section may be missing.
+    void save(const char *path) const;
+    void open(const char *path);
+
     // Parse an N-Triples file and number its terms; the triples are appended.
     void loadNTriples(const char *path);
     void loadNTriplesText(const std::string &text);

@@ -25,13 +25,15 @@ class VedasStorage {
     VedasStorage(VedasContext *ctx, const uint32_t *d_s, const uint32_t *d_p,
                  const uint32_t *d_o, uint64_t n, uint32_t perm_mask);
     // A store read from a .vds file (VedasStorage::open, VedasStorage.cu:
-    // 452-484): the PSO and POS permutations of the reference's default
-    // index, uploaded as they are (no sort).
+    // 452-484): the PSO and POS permutations of the default index, and all
+    // six of a full-index file, uploaded as they are (no sort; the columns
+    // the file leaves out are rebuilt from its runs on the device).
     VedasStorage(VedasContext *ctx, const char *fname);
     ~VedasStorage();
-    // Write the default-index part of the .vds format (VedasStorage::write,
-    // VedasStorage.cu:486-515): ten arrays, each a size_t count followed by
-    // the uint32 values.  Needs the PSO and POS permutations.
+    // Write the .vds format (VedasStorage::write, VedasStorage.cu:492-516):
+    // arrays, each a size_t count followed by the uint32 values -- the ten
+    // of the default index (needs PSO and POS), then, when SPO, SOP, OSP and
+    // OPS are built too, the fifteen of the full index.
     void write(const char *fname);
     VedasStorage(const VedasStorage &) = delete;
     VedasStorage &operator=(const VedasStorage &) = delete;
@@ -42,6 +44,9 @@ class VedasStorage {
     size_t getObjectIndexSize() const;
     bool isPreload() const { return true; }
     bool hasPermutation(int perm) const { return (vd_index_perms(index_) >> perm) & 1u; }
+    // SPO, SOP, OSP and OPS built as well as PSO and POS (the reference's
+    // -full-index store): write() then adds the full-index arrays.
+    bool isFullIndex() const;
     vd_index *index() const { return index_; }
     VedasContext *context() const { return ctx_; }
     vd_perm_view_t view(int perm) const;

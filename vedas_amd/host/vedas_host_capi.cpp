@@ -343,6 +343,22 @@ int vdq_dict_translate(const vdq_dict *d, const char *sparql, char *out, size_t 
     });
 }
 
+int vdq_dict_save(const vdq_dict *d, const char *path) {
+    if (!d || !path) return fail(VD_EINVAL, "null argument");
+    return guarded([&] { d->d.save(path); });
+}
+
+int vdq_dict_open(vdq_dict *d, const char *path) {
+    if (!d || !path) return fail(VD_EINVAL, "null argument");
+    return guarded([&] { d->d.open(path); });
+}
+
+int vdq_engine_set_dict(vdq_engine *e, const vdq_dict *d) {
+    if (!e || !d) return fail(VD_EINVAL, "null argument");
+    e->dict = d;
+    return VD_OK;
+}
+
 int vdq_engine_load_dict(vdq_engine *e, const vdq_dict *d, uint32_t perm_mask) {
     if (!e || !d) return fail(VD_EINVAL, "null argument");
     const size_t n = d->d.subjects().size();
