@@ -33,7 +33,7 @@ def joinset():
     from vedas_amd.engine import Engine, query_text
     from vedas_amd.workload import JOIN_SET, STORE_100M
     prm = {k: v for k, v in STORE_100M.items() if k != "n"}
-    s, p, o = N.gen_triples(STORE_100M["n"], seed=42, distinct=False, **prm)
+    s, p, o = N.gen_triples(STORE_100M["n"], seed=42, distinct=True, **prm)
     eng = Engine(0)
     ctx = N.Ctx.wrap(eng.ctx_handle())
     _calibrate(N, ctx)
@@ -58,7 +58,7 @@ def build():
     from vedas_amd import _native as N
     from vedas_amd.workload import STORE_100M
     prm = {k: v for k, v in STORE_100M.items() if k != "n"}
-    s, p, o = N.gen_triples(STORE_100M["n"], seed=42, distinct=False, **prm)
+    s, p, o = N.gen_triples(STORE_100M["n"], seed=42, distinct=True, **prm)
     ctx = N.Ctx(0)
     _calibrate(N, ctx)
     ds, dp, do = ctx.upload(s), ctx.upload(p), ctx.upload(o)
