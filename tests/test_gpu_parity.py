@@ -286,6 +286,39 @@ def test_select_batch(ctx, store):
         ctx.free(d)
 
 
+def test_select_batch_tiny_and_empty_ranges(ctx, store):
+    """Tens of thousands of (S, P) selects of a row or two, a third of them
+    absent (empty ranges): output tiles then span more queries than the
+    gather stages in LDS, and empty ranges sit between rows."""
+    import ctypes as C
+    from vedas_amd._native import PERMS
+    s, p, o, idx, orc = store
+    rng = np.random.default_rng(19)
+    nq = 40_000
+    pick = rng.integers(0, len(s), nq)
+    k1 = s[pick].astype(np.uint32)
+    k2 = p[pick].astype(np.uint32)
+    k2[rng.random(nq) < 0.33] = 65_000  # no such predicate
+    ix = orc["SPO"]
+    rng_be = [O.lookup(ix, int(a), int(b)) for a, b in zip(k1, k2)]
+    total = int(sum(e - b for b, e in rng_be))
+    assert total < 3 * nq
+    dk1, dk2 = ctx.upload(k1), ctx.upload(k2)
+    doff = ctx.alloc(8 * (nq + 1))
+    dy, dz = ctx.alloc(4 * total), ctx.alloc(4 * total)
+    tot = C.c_uint64()
+    ctx.check(ctx.L.vd_select_batch(ctx.h, idx.h, PERMS["SPO"], C.c_void_p(dk1), C.c_void_p(dk2),
+                                    nq, C.c_void_p(doff), C.c_void_p(dy), C.c_void_p(dz), total,
+                                    C.byref(tot)))
+    assert tot.value == total
+    assert np.array_equal(ctx.download(dy, total),
+                          np.concatenate([ix["y"][b:e] for b, e in rng_be]))
+    assert np.array_equal(ctx.download(dz, total),
+                          np.concatenate([ix["z"][b:e] for b, e in rng_be]))
+    for d in (dk1, dk2, doff, dy, dz):
+        ctx.free(d)
+
+
 def make_rel(rng, n, ncols, keymax, nk=1):
     cols = [rng.integers(0, keymax, n).astype(np.uint32) for _ in range(ncols)]
     return O.sort_lex(cols) if nk == 2 else O.sort_col0(cols)

@@ -381,17 +381,72 @@ __global__ void k_hist_limit(const uint32_t *__restrict__ l1v, const uint32_t *_
     be[1] = m < n_l1 ? l1o[m] : n;
 }
 
-// Rows of query q are output rows [off[q], off[q+1]); one thread per output
-// row finds its query by binary search over off (nq + 1 entries).
-__global__ __launch_bounds__(256) void k_gather_ranges(
+// Rows of query q are output rows [off[q], off[q+1]) (nq + 1 offsets);
+// output row o of query q copies source row begin[q] + o - off[q].  A block
+// copies GR_TILE consecutive output rows.  It finds the queries of its
+// first and last row once: inside one query (a large range) the tile is a
+// plain copy of consecutive source rows; across up to GR_LDS queries their
+// offsets and first rows are staged in LDS and each row searches there;
+// across more (ranges of a few rows) each row searches the offsets in
+// memory between the two queries found.
+constexpr int GR_BLOCK = 256;
+constexpr int GR_ITEMS = 16;
+constexpr uint64_t GR_TILE = GR_BLOCK * GR_ITEMS;
+constexpr int GR_LDS = 1024;
+__global__ __launch_bounds__(GR_BLOCK) void k_gather_ranges(
     const uint64_t *__restrict__ off, const uint64_t *__restrict__ begin,
     uint64_t nq, uint64_t total, const uint32_t *__restrict__ y,
     const uint32_t *__restrict__ z, uint32_t *__restrict__ oy,
     uint32_t *__restrict__ oz) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total;
-         o += stride) {
-        const uint64_t q = dev::upper_bound(off, 0, nq + 1, o) - 1;
+    __shared__ uint64_t s_off[GR_LDS + 1];
+    __shared__ uint64_t s_beg[GR_LDS];
+    __shared__ uint64_t s_q[2];
+    const uint64_t o0 = (uint64_t)blockIdx.x * GR_TILE;
+    const uint64_t o1 = o0 + GR_TILE < total ? o0 + GR_TILE : total;
+    if (threadIdx.x < 2) {
+        const uint64_t x = threadIdx.x == 0 ? o0 : o1 - 1;
+        s_q[threadIdx.x] = dev::upper_bound(off, 0, nq + 1, x) - 1;
+    }
+    __syncthreads();
+    const uint64_t qa = s_q[0], qb = s_q[1];
+    const uint64_t nqs = qb - qa + 1;
+    if (nqs == 1) {
+        const uint64_t src0 = begin[qa] + (o0 - off[qa]);
+        uint32_t vy[GR_ITEMS], vz[GR_ITEMS];
+#pragma unroll
+        for (int k = 0; k < GR_ITEMS; ++k) {
+            const uint64_t o = o0 + (uint64_t)k * GR_BLOCK + threadIdx.x;
+            if (o < o1) {
+                if (oy) vy[k] = y[src0 + (o - o0)];
+                if (oz) vz[k] = z[src0 + (o - o0)];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < GR_ITEMS; ++k) {
+            const uint64_t o = o0 + (uint64_t)k * GR_BLOCK + threadIdx.x;
+            if (o < o1) {
+                if (oy) oy[o] = vy[k];
+                if (oz) oz[o] = vz[k];
+            }
+        }
+        return;
+    }
+    if (nqs <= (uint64_t)GR_LDS) {
+        for (uint64_t t = threadIdx.x; t <= nqs; t += GR_BLOCK) {
+            s_off[t] = off[qa + t];  // qb + 1 <= nq: off has that entry
+            if (t < nqs) s_beg[t] = begin[qa + t];
+        }
+        __syncthreads();
+        for (uint64_t o = o0 + threadIdx.x; o < o1; o += GR_BLOCK) {
+            const uint64_t k = dev::upper_bound(s_off, 0, nqs + 1, o) - 1;
+            const uint64_t src = s_beg[k] + (o - s_off[k]);
+            if (oy) oy[o] = y[src];
+            if (oz) oz[o] = z[src];
+        }
+        return;
+    }
+    for (uint64_t o = o0 + threadIdx.x; o < o1; o += GR_BLOCK) {
+        const uint64_t q = dev::upper_bound(off, qa, qb + 1, o) - 1;
         const uint64_t src = begin[q] + (o - off[q]);
         if (oy) oy[o] = y[src];
         if (oz) oz[o] = z[src];
@@ -767,7 +822,7 @@ void select_batch_impl(vd_ctx *ctx, const vd_index *idx, int perm, const uint32_
     VD_REQUIRE(total == 0 || d_out_y || d_out_z, VD_EINVAL, "no output columns");
     if (total == 0) return;
     launch(ctx, "select_gather", (double)total * 4.0 * ((d_out_y ? 2 : 0) + (d_out_z ? 2 : 0)),
-           dim3(stream_grid(total)), dim3(256), 0, k_gather_ranges,
+           dim3((unsigned)ceil_div(total, GR_TILE)), dim3(GR_BLOCK), 0, k_gather_ranges,
            (const uint64_t *)d_row_offsets, (const uint64_t *)begin, nq, total,
            (const uint32_t *)pa.y, (const uint32_t *)pa.z, d_out_y, d_out_z);
 }

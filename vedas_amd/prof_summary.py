@@ -6,10 +6,14 @@ profiles/.
   traffic  two --pmc runs (FETCH_SIZE, then WRITE_SIZE) of the same
            workload: HBM bytes per launch of each kernel, each counter
            scaled by what it reads for the calibration copy of known size
-           (the first __amd_rocclr_copyBuffer dispatch, 1 GiB).
+           (the first __amd_rocclr_copyBuffer dispatch, 1 GiB);
+  counters one --pmc run of several counters: their mean per launch for
+           each kernel whose name matches a pattern, with the LDS
+           bank-conflict rate and the share of wave cycles spent waiting.
 
   python3 -m vedas_amd.prof_summary trace STATS_CSV OUT.md "COMMAND"
   python3 -m vedas_amd.prof_summary traffic FETCH_CSV WRITE_CSV OUT.md "COMMAND"
+  python3 -m vedas_amd.prof_summary counters PMC_CSV PATTERN OUT.md "COMMAND"
 
 The traffic table is also read by bench.py (label -> bytes per launch).
 """
@@ -46,6 +50,10 @@ LABELS = [
     (r"k_select_pieces\b", "select_pieces"),
     (r"k_gather_ranges\b", "select_gather"),
     (r"k_sel_lookup_multi\b", "select_lookup"),
+    (r"k_sel_bounds_multi\b", "select_bounds"),
+    (r"k_hist_", "histogram"),
+    (r"k_runs_expand\b", "runs_expand"),
+    (r"k_mark_l1\b|k_mark_y\b|k_scatter_heads\b", "l2_heads"),
     (r"k_pack_c01\b|k_pack_rowid\b|k_pack_multi\b", "sort_pack"),
     (r"k_unpack_c01\b", "sort_unpack"),
     (r"k_gather_perm\b|k_gather_sort_multi\b", "sort_gather"),
@@ -137,11 +145,47 @@ def traffic(fetch_csv: str, write_csv: str, out: str, command: str) -> None:
     open(out, "w").write("\n".join(lines) + "\n")
 
 
+def counters(pmc_csv: str, pattern: str, out: str, command: str) -> None:
+    rows = list(csv.DictReader(open(pmc_csv)))
+    per = {}  # kernel -> dispatch -> counter -> value
+    for r in rows:
+        if not re.search(pattern, r["Kernel_Name"]):
+            continue
+        k = short(r["Kernel_Name"])
+        d = per.setdefault(k, {}).setdefault(r["Dispatch_Id"], {})
+        d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    names = sorted({c for k in per.values() for d in k.values() for c in d})
+    lines = [f"# Hardware counters per launch: {command.split(' -- ')[-1]}", "",
+             f"Command: `{command}`", "",
+             "Means over the launches of each kernel.  LDS conflict rate = "
+             "SQ_LDS_BANK_CONFLICT / (SQ_LDS_IDX_ACTIVE - SQ_LDS_BANK_CONFLICT); "
+             "wait share = SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES (cycles a wave waits for an "
+             "issued instruction's result, over its resident cycles); cycles per wave = "
+             "SQ_WAVE_CYCLES / SQ_WAVES.", "",
+             "| kernel | launches | " + " | ".join(names) +
+             " | LDS conflict rate | wait share | cycles per wave |",
+             "|---|---:|" + "---:|" * (len(names) + 3)]
+    for k, ds in sorted(per.items()):
+        n = len(ds)
+        mean = {c: sum(d.get(c, 0.0) for d in ds.values()) / n for c in names}
+        conf = mean.get("SQ_LDS_BANK_CONFLICT", 0.0)
+        act = mean.get("SQ_LDS_IDX_ACTIVE", 0.0)
+        rate = conf / (act - conf) if act > conf else float("nan")
+        wc = mean.get("SQ_WAVE_CYCLES", 0.0)
+        wait = mean.get("SQ_WAIT_INST_ANY", 0.0) / wc if wc else float("nan")
+        cpw = wc / mean["SQ_WAVES"] if mean.get("SQ_WAVES") else float("nan")
+        lines.append(f"| {k} | {n} | " + " | ".join(f"{mean[c]:.4g}" for c in names) +
+                     f" | {rate:.3f} | {wait:.3f} | {cpw:.0f} |")
+    open(out, "w").write("\n".join(lines) + "\n")
+
+
 def main(argv):
     if argv[1] == "trace":
         trace(argv[2], argv[3], argv[4])
     elif argv[1] == "traffic":
         traffic(argv[2], argv[3], argv[4], argv[5])
+    elif argv[1] == "counters":
+        counters(argv[2], argv[3], argv[4], argv[5])
     else:
         raise SystemExit(__doc__)
     return 0
