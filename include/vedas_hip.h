@@ -79,9 +79,11 @@ int  vd_ctx_mem(vd_ctx *ctx, uint64_t *in_use, uint64_t *cached);
 int  vd_ctx_mem_peak(vd_ctx *ctx, uint64_t *peak, int reset);
 /* Tunables: "radix_items" (keys per thread in a radix tile: 16 = 512
  * threads x 16 keys, the default; 8 = 512 x 8; 24 = 256 x 24, keys only),
- * "join_nv" (merge-path diagonal per join expansion block), "join_rcap"
- * (right keys staged in LDS per bounds block).  Returns VD_EINVAL for an
- * unknown key or an unsupported value. */
+ * "radix_bits" (digit width of the IndexSwap re-sort: 0 = 9-bit digits when
+ * they save a pass, else 8-bit, the default; 8 = always 8-bit; 9 = 9-bit
+ * whenever the key fits 27 bits), "join_nv" (merge-path diagonal per join
+ * expansion block), "join_rcap" (right keys staged in LDS per bounds
+ * block).  Returns VD_EINVAL for an unknown key or an unsupported value. */
 int  vd_ctx_tune(vd_ctx *ctx, const char *key, int value);
 
 /* ---- device buffers ---------------------------------------------------- */

@@ -105,11 +105,13 @@ int vdq_plan(const char *text, const uint64_t *est_rows, char *out, size_t cap);
  * fixed by ModernGPU and Thrust): times the given queries under each
  * candidate of the kernel tunables -- radix tile shape ("radix_items"),
  * merge-path diagonal ("join_nv"), LDS right-key capacity ("join_rcap"),
- * see vd_ctx_tune -- by coordinate descent, keeps the fastest and applies
- * it to the engine's contexts (serving slots included).  Results do not
- * depend on the tunables. */
+ * radix digit width of the re-sorts ("radix_bits"), see vd_ctx_tune -- by
+ * coordinate descent, keeps the fastest and applies it to the engine's
+ * contexts (serving slots included).  Results do not depend on the
+ * tunables. */
 typedef struct {
     int radix_items, join_nv, join_rcap;  /* chosen values               */
+    int radix_bits;                       /* IndexSwap digit width (0 = by rule) */
     double base_ms;                       /* queries with the defaults    */
     double tuned_ms;                      /* queries with the chosen set  */
     int evaluations;

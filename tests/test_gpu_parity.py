@@ -517,21 +517,29 @@ def test_sort_col0_views_any_alignment(ctx):
 @pytest.mark.parametrize("ncols", [2, 3, 4, 6])
 def test_sort_col0_digit_plans(ctx, ncols):
     """Column-0 values that only vary in some 9-bit or 8-bit digits: the
-    passes skip the constant digits, in both digit plans."""
+    passes skip the constant digits, in both digit plans, under every
+    radix_bits setting (the rule, always 8-bit, 9-bit up to 27 bits)."""
     from vedas_amd._native import Rel
     rng = np.random.default_rng(ncols)
     n = 50_001
-    for col0 in (rng.integers(0, 64, n).astype(np.uint32) << 18,        # bits 18..23
-                 (rng.integers(0, 8, n).astype(np.uint32) << 24) | 5,  # bits 24..26
-                 rng.integers(1 << 24, 1 << 27, n).astype(np.uint32),
-                 np.full(n, 123456, np.uint32)):
-        cols = [col0] + [rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
-                         for _ in range(ncols - 1)]
-        r = Rel.from_columns(ctx, cols)
-        r.sort_col0()
-        for g, w in zip(r.columns(), O.sort_col0(cols)):
-            assert np.array_equal(g, w)
-        r.free()
+    try:
+        for bits in (0, 8, 9):
+            ctx.tune("radix_bits", bits)
+            for col0 in (rng.integers(0, 64, n).astype(np.uint32) << 18,        # bits 18..23
+                         (rng.integers(0, 8, n).astype(np.uint32) << 24) | 5,  # bits 24..26
+                         rng.integers(1 << 24, 1 << 27, n).astype(np.uint32),
+                         rng.integers(0, 1 << 20, n).astype(np.uint32),
+                         rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32),
+                         np.full(n, 123456, np.uint32)):
+                cols = [col0] + [rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+                                 for _ in range(ncols - 1)]
+                r = Rel.from_columns(ctx, cols)
+                r.sort_col0()
+                for g, w in zip(r.columns(), O.sort_col0(cols)):
+                    assert np.array_equal(g, w), bits
+                r.free()
+    finally:
+        ctx.tune("radix_bits", 0)
 
 
 @pytest.mark.parametrize("items", [8, 24])

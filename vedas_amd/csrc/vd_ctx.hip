@@ -370,6 +370,9 @@ int vd_ctx_tune(vd_ctx *ctx, const char *key, int value) {
     } else if (k == "join_rcap") {
         if (value != 2048 && value != 4096 && value != 8192) return VD_EINVAL;
         ctx->join_rcap = value;
+    } else if (k == "radix_bits") {
+        if (value != 0 && value != 8 && value != 9) return VD_EINVAL;
+        ctx->radix_bits = value;
     } else {
         ctx->last_error = "unknown tunable " + k;
         return VD_EINVAL;

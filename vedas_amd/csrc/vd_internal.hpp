@@ -110,6 +110,7 @@ struct vd_ctx {
     int radix_items = 16;   // keys per thread in a radix tile
     int join_nv = 2048;     // merge-path diagonal length per expansion block
     int join_rcap = 4096;   // right keys staged in LDS per bounds block
+    int radix_bits = 0;     // IndexSwap digit width: 0 = by rule, 8, 9
 };
 
 // A relation: ncols device columns of nrows uint32 each.  A column whose bit

@@ -66,17 +66,19 @@ __device__ __forceinline__ void count_digit(uint32_t *bins, uint32_t d) {
 }
 
 // Digit plan of an IndexSwap sort from column 0's maximum mx: digits below
-// its bit length can vary, the ones above are 0 in every row.  9-bit digits
-// (bits 0..26) when they need fewer passes than bytes, else 8-bit digits.
+// its bit length can vary, the ones above are 0 in every row.  `bits` is
+// the radix_bits tunable: 0 = 9-bit digits (bits 0..26) when they need
+// fewer passes than bytes, else 8-bit; 8 = always bytes; 9 = 9-bit digits
+// whenever the key fits 27 bits.
 struct SwapPlan {
     bool use9;
     int ndig;  // digits that can vary
 };
-__host__ __device__ inline SwapPlan swap_plan(uint32_t mx) {
+__host__ __device__ inline SwapPlan swap_plan(uint32_t mx, int bits) {
     int nb = 0;
     while (nb < 32 && (mx >> nb) != 0) ++nb;
     const int n8 = (nb + 7) / 8, n9 = (nb + 8) / 9;
-    const bool use9 = nb <= 27 && n9 < n8;
+    const bool use9 = nb <= 27 && (bits == 9 || (bits == 0 && n9 < n8));
     return SwapPlan{use9, use9 ? n9 : n8};
 }
 
@@ -95,11 +97,11 @@ constexpr int SWAP_HW = SWAP_H9 > SWAP_H8 ? SWAP_H9 : SWAP_H8;
 __global__ __launch_bounds__(RB) void k_pack_c01(
     const uint32_t *__restrict__ c0, const uint32_t *__restrict__ c1,
     const uint32_t *__restrict__ vsrc, uint64_t n, uint64_t *__restrict__ keys,
-    uint32_t *__restrict__ vals, uint32_t *g8, uint32_t *g9, const uint32_t *gmax) {
+    uint32_t *__restrict__ vals, uint32_t *g8, uint32_t *g9, const uint32_t *gmax, int bits) {
     __shared__ uint32_t s_h[radix::HIST_WAVES][SWAP_HW];
     static_assert(RB == radix::HIST_BLOCK, "one histogram copy per wave");
     for (int i = threadIdx.x; i < radix::HIST_WAVES * SWAP_HW; i += RB) (&s_h[0][0])[i] = 0;
-    const SwapPlan plan = swap_plan(*gmax);
+    const SwapPlan plan = swap_plan(*gmax, bits);
     const int W = plan.use9 ? 9 : 8, bins = 1 << W, nd = plan.ndig;
     const uint32_t mask = (uint32_t)bins - 1u;
     __syncthreads();
@@ -135,10 +137,11 @@ __global__ __launch_bounds__(RB) void k_pack_c01(
 constexpr int SP_BLOCK = 512;
 __global__ __launch_bounds__(SP_BLOCK) void k_swap_plan_scan(
     const uint32_t *__restrict__ g8, const uint32_t *__restrict__ g9, const uint32_t *gmax,
-    uint64_t n, uint32_t *__restrict__ gofs8, uint32_t *__restrict__ gofs9, Readback rb) {
+    uint64_t n, uint32_t *__restrict__ gofs8, uint32_t *__restrict__ gofs9, Readback rb,
+    int bits) {
     __shared__ uint32_t s_tmp[SP_BLOCK / 64];
     const uint32_t mx = *gmax;
-    const SwapPlan plan = swap_plan(mx);
+    const SwapPlan plan = swap_plan(mx, bits);
     const int bins = plan.use9 ? 512 : 256;
     const uint32_t *h = plan.use9 ? g9 : g8;
     uint32_t *go = plan.use9 ? gofs9 : gofs8;
@@ -276,16 +279,16 @@ void sort_col0(vd_ctx *ctx, vd_rel *rel) {
            dim3(radix::hist_grid(ceil_div(n, RB * R_ITEMS))), dim3(RB), 0, k_pack_c01,
            (const uint32_t *)rel->cols[0], (const uint32_t *)(nc >= 2 ? rel->cols[1] : nullptr),
            (const uint32_t *)nullptr, n, keys.p, by_row ? rows.p : nullptr, h8, h9,
-           (const uint32_t *)dmax.p);
+           (const uint32_t *)dmax.p, ctx->radix_bits);
     const Readback rb = rb_next(ctx);
     launch(ctx, "radix_scan_hist", 0.0, dim3(1), dim3(SP_BLOCK), 0, k_swap_plan_scan,
            (const uint32_t *)h8, (const uint32_t *)h9, (const uint32_t *)dmax.p, n, gofs.p,
-           gofs.p + SWAP_H8, rb);
+           gofs.p + SWAP_H8, rb, ctx->radix_bits);
     uint64_t hv[2];
     rb_wait(ctx, rb, hv, 2);
     // Passes: the digits of the plan that vary (the pack counted only the
     // plan's digits below the maximum's bit length).
-    const SwapPlan plan = swap_plan((uint32_t)hv[1]);
+    const SwapPlan plan = swap_plan((uint32_t)hv[1], ctx->radix_bits);
     int shifts[4];
     const uint32_t *g[4];
     int npass = 0;

@@ -40,7 +40,8 @@ class ServeRec(C.Structure):
 
 class TuneResult(C.Structure):
     _fields_ = [("radix_items", C.c_int), ("join_nv", C.c_int), ("join_rcap", C.c_int),
-                ("base_ms", C.c_double), ("tuned_ms", C.c_double), ("evaluations", C.c_int)]
+                ("radix_bits", C.c_int), ("base_ms", C.c_double), ("tuned_ms", C.c_double),
+                ("evaluations", C.c_int)]
 
 
 _lib = None

@@ -573,7 +573,8 @@ int vdq_autotune(vdq_engine *e, const char *const *texts, int n, int reps, vdq_t
         };
         std::vector<Knob> knobs = {{"radix_items", {16, 8, 24}, 16},
                                    {"join_nv", {2048, 1024, 4096}, 2048},
-                                   {"join_rcap", {4096, 2048, 8192}, 4096}};
+                                   {"join_rcap", {4096, 2048, 8192}, 4096},
+                                   {"radix_bits", {0, 8, 9}, 0}};
         for (auto &k : knobs) set(k.key, k.cur);
         run_all();  // warm-up
         const double base = run_all();
@@ -600,6 +601,7 @@ int vdq_autotune(vdq_engine *e, const char *const *texts, int n, int reps, vdq_t
         out->radix_items = knobs[0].cur;
         out->join_nv = knobs[1].cur;
         out->join_rcap = knobs[2].cur;
+        out->radix_bits = knobs[3].cur;
         out->base_ms = base;
         out->tuned_ms = best;
         out->evaluations = evals;
