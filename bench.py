@@ -43,12 +43,58 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def index_bytes(n: int, kbits: int) -> float:
-    """Algorithmic HBM bytes of one permutation build: ceil(K/8) radix
-    passes reading and writing 8-byte packed keys, one pass packing
-    (12 B read, 8 B written) and one unpacking (8 B read, 8 B written)."""
-    passes = -(-kbits // 8)
-    return passes * 2 * 8 * n + 20 * n + 16 * n
+PERM_COLS = {"SPO": (0, 1, 2), "SOP": (0, 2, 1), "PSO": (1, 0, 2), "POS": (1, 2, 0),
+             "OSP": (2, 0, 1), "OPS": (2, 1, 0)}
+
+
+def build_order(perms):
+    """(permutation, sibling it is derived from or None) in build order, as
+    the index build plans it: a permutation XYZ whose Y comes before its Z in
+    a built sibling's key is derived from that sibling (a stable sort on X),
+    the others are sorted on all bits (vd_index.hip build_packed)."""
+    order, todo = [], [q for q in PERM_COLS if q in perms]
+    while todo:
+        pick = None
+        for q in todo:
+            for b, _ in order:
+                pos = {c: k for k, c in enumerate(PERM_COLS[b])}
+                if pos[PERM_COLS[q][1]] < pos[PERM_COLS[q][2]]:
+                    pick = (q, b)
+                    break
+            if pick:
+                break
+        pick = pick or (todo[0], None)
+        order.append(pick)
+        todo.remove(pick[0])
+    return order
+
+
+def x_passes(xbits: int) -> int:
+    """Passes of a stable sort on an X of xbits bits (radix_bits 0: 9-bit
+    digits when they need fewer passes than bytes, up to 27 bits)."""
+    n8, n9 = -(-xbits // 8), -(-xbits // 9)
+    return n9 if xbits <= 27 and n9 < n8 else n8
+
+
+def index_bytes(n: int, bits, perms) -> float:
+    """Algorithmic HBM bytes of the index build when the triple fits one
+    64-bit key (bits = bit lengths of s, p, o).  A permutation sorted on all
+    K bits packs the triple (12 B read, 8 B written) and runs ceil(K/8)
+    passes (8 B read, 8 B written); a derived one repacks its sibling's
+    sorted keys (8 B + 8 B) and runs the passes of its X bits.  Every
+    permutation writes Y and Z from its sorted keys (8 B + 8 B) and reads
+    the keys once more for its L1/L2 heads (8 B; the head entries written
+    are not counted)."""
+    kbits = sum(bits)
+    assert kbits <= 64
+    total = 0.0
+    for q, src in build_order(perms):
+        if src is None:
+            total += 20 * n + -(-kbits // 8) * 16 * n
+        else:
+            total += 16 * n + x_passes(bits[PERM_COLS[q][0]]) * 16 * n
+        total += 24 * n
+    return total
 
 
 _M1 = np.uint64(0xFF51AFD7ED558CCD)
@@ -437,9 +483,9 @@ def main():
     build_s = time.perf_counter() - t0
     for d in (ds, dp, do):
         ctx.free(d)
-    kb = {name: int(s.max()).bit_length() + int(p.max()).bit_length() + int(o.max()).bit_length()
-          for name in perms}
-    build_alg = sum(index_bytes(n, kb[x]) for x in perms)
+    bits = (int(s.max()).bit_length(), int(p.max()).bit_length(), int(o.max()).bit_length())
+    kb = {name: sum(bits) for name in perms}
+    build_alg = index_bytes(n, bits, perms)
     log(f"[bench] index build {build_s * 1e3:.1f} ms")
 
     idx_view = _native.Index.wrap(ctx, eng.index_handle(), n)
@@ -612,7 +658,11 @@ def main():
         "bound_prescan": prescan,
         "index_build": {"ms": build_s * 1e3, "permutations": list(perms), "key_bits": kb,
                         "alg_bytes": build_alg, "GB/s": build_alg / build_s / 1e9,
-                        "frac": build_alg / build_s / HBM_PEAK, "keys_per_s": 4 * n / build_s,
+                        "frac": build_alg / build_s / HBM_PEAK,
+                        "keys_per_s": len(perms) * n / build_s,
+                        "order": [f"{q}<-{b}" if b else f"{q} (all bits)"
+                                  for q, b in build_order(perms)],
+                        "alg_note": "bytes of the derived-permutation build (index_bytes)",
                         "kernels": kernel_table(build_prof)},
         "autotune": tune,
         "planned_queries": planned,

@@ -542,6 +542,35 @@ def test_sort_col0_digit_plans(ctx, ncols):
         ctx.tune("radix_bits", 0)
 
 
+@pytest.mark.parametrize("perms", [["SOP", "POS", "OPS"], ["OPS", "OSP"], ["POS"], ["SOP", "PSO"]])
+def test_index_build_derivation_orders(ctx, perms):
+    """Permutation sets whose first fully sorted permutation is not SPO:
+    {SOP, POS, OPS} sorts SOP and POS in full and derives OPS from POS;
+    {OPS, OSP} derives neither from the other; a single permutation; {SOP,
+    PSO} sorts both.  Every set of all six derives five (the default set's
+    PSO / OSP from SPO and POS from OSP)."""
+    s, p, o = rand_triples(30_011, len(perms) * 7 + 1)
+    s[::5] = s[0]  # long runs of X
+    check_index(ctx, s, p, o, perms)
+
+
+@pytest.mark.parametrize("bits", [8, 9])
+def test_index_build_digit_widths(ctx, bits):
+    """Derived permutations sort on X's bits under the radix_bits setting
+    (always bytes / 9-bit digits up to 27 bits), with X of 9, 20 and 27
+    bits."""
+    ctx.tune("radix_bits", bits)
+    try:
+        rng = np.random.default_rng(bits)
+        n = 40_003
+        s = rng.integers(1, 1 << 20, n).astype(np.uint32)
+        p = rng.integers(1, 1 << 9, n).astype(np.uint32)
+        o = rng.integers(1, 1 << 27, n).astype(np.uint32)
+        check_index(ctx, s, p, o)
+    finally:
+        ctx.tune("radix_bits", 0)
+
+
 @pytest.mark.parametrize("items", [8, 24])
 def test_index_build_tile_shapes(ctx, items):
     ctx.tune("radix_items", items)

@@ -10,6 +10,12 @@
 //                        (X changes) and L2 heads ((X,Y) changes).
 //   4. scan of the block counts.
 //   5. k_compact_heads : L1 = (X, first row), L2 = (Y, first row) entries.
+// A permutation XYZ whose rows a sibling permutation already holds in an
+// order that, among rows of equal X, is (Y, Z) (Y before Z in the sibling's
+// key) is derived instead of sorted in full: its keys are repacked from the
+// sibling's sorted keys and stably sorted on X's bits only (LSD passes keep
+// the sibling's order among equal X).  Of SPO, PSO, POS and OSP only SPO is
+// sorted on all bits; PSO and OSP follow from SPO and POS from OSP.
 // The resulting arrays match the reference's createIndex1Level /
 // createIndex output (VedasStorage.cu:270-358): L1 values/offsets are the
 // distinct leading IDs and their first rows, L2 values/offsets the second
@@ -21,6 +27,7 @@
 #include <type_traits>
 
 #include "vd_radix_dev.hpp"
+#include "vd_swap_dev.hpp"
 
 #include "vd_index.hpp"
 
@@ -65,6 +72,52 @@ __global__ __launch_bounds__(PK_BLOCK) void k_pack_xyz(
     }
     __syncthreads();
     radix::hist_flush(&s, ghist, npass);
+}
+
+// Field layout of a permutation's packed key: shift and mask of the s, p
+// and o columns.
+struct PackLayout {
+    int sh[3];
+    uint32_t mask[3];
+};
+
+// Keys of a derived permutation: each sorted key of the sibling permutation
+// (layout `from`) with its fields moved to layout `to`, in the sibling's
+// order; counts the digit histograms of column xc (the new X) under the
+// plan of *gmax into g8 / g9.
+__global__ __launch_bounds__(PK_BLOCK) void k_repack(const uint64_t *__restrict__ src, uint64_t n,
+                                                     PackLayout from, PackLayout to, int xc,
+                                                     uint64_t *__restrict__ keys, uint32_t *g8,
+                                                     uint32_t *g9, const uint32_t *gmax,
+                                                     int bits) {
+    __shared__ uint32_t s_h[radix::HIST_WAVES][swap::HW];
+    for (int i = threadIdx.x; i < radix::HIST_WAVES * swap::HW; i += PK_BLOCK)
+        (&s_h[0][0])[i] = 0;
+    const swap::Plan plan = swap::plan_for(*gmax, bits);
+    __syncthreads();
+    uint32_t *h = s_h[threadIdx.x / 64];
+    for (uint64_t base = (uint64_t)blockIdx.x * (PK_BLOCK * PK_ITEMS); base < n;
+         base += (uint64_t)gridDim.x * (PK_BLOCK * PK_ITEMS)) {
+#pragma unroll 4
+        for (int j = 0; j < PK_ITEMS; ++j) {
+            const uint64_t i = base + (uint64_t)j * PK_BLOCK + threadIdx.x;
+            if (i < n) {
+                const uint64_t v = src[i];
+                uint64_t key = 0;
+                uint32_t x = 0;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const uint32_t f = (uint32_t)(v >> from.sh[c]) & from.mask[c];
+                    key |= (uint64_t)f << to.sh[c];
+                    x = c == xc ? f : x;
+                }
+                keys[i] = key;
+                swap::hist_add(h, x, plan);
+            }
+        }
+    }
+    __syncthreads();
+    swap::hist_flush(&s_h[0][0], radix::HIST_WAVES, plan, g8, g9);
 }
 
 // Sorted triples, one row = (X, Y, Z).  PackedRows: the whole triple in one
@@ -136,49 +189,69 @@ __global__ __launch_bounds__(UN_BLOCK) void k_compact_heads(
     const uint64_t *__restrict__ off_l2, uint32_t *__restrict__ l1v,
     uint32_t *__restrict__ l1o, uint32_t *__restrict__ l2v,
     uint32_t *__restrict__ l2o) {
-    __shared__ uint32_t s_tmp[UN_BLOCK / 64];
+    constexpr int WAVES = UN_BLOCK / 64;
+    constexpr int SLOTS = UN_ITEMS * WAVES;
+    __shared__ uint32_t s_off[2][SLOTS];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint64_t base = (uint64_t)blockIdx.x * UN_TILE;
-    // Each thread owns UN_ITEMS consecutive rows so the block scan keeps row
-    // order.
-    bool h1[UN_ITEMS], h2[UN_ITEMS];
+    // Rows are read striped (row base + j * UN_BLOCK + threadIdx.x: every
+    // load coalesced, as in k_unpack_count).  In row order, slot (j, w) = the
+    // 64 rows of band j held by wave w comes before (j, w + 1) and (j + 1,
+    // 0): a head's rank in the tile is the heads of the slots before its own
+    // plus those of lower lanes in its slot.
+    uint64_t b1[UN_ITEMS], b2[UN_ITEMS];
     uint32_t xv[UN_ITEMS], yv[UN_ITEMS];
-    uint32_t c1 = 0, c2 = 0;
 #pragma unroll
     for (int j = 0; j < UN_ITEMS; ++j) {
-        const uint64_t i = base + (uint64_t)threadIdx.x * UN_ITEMS + j;
-        h1[j] = h2[j] = false;
+        const uint64_t i = base + (uint64_t)j * UN_BLOCK + threadIdx.x;
+        bool h1 = false, h2 = false;
         xv[j] = yv[j] = 0;
         if (i < n) {
             const Row r = rows.at(i);
             xv[j] = r.x;
             yv[j] = r.y;
             if (i == 0) {
-                h1[j] = h2[j] = true;
+                h1 = h2 = true;
             } else {
                 const Row q = rows.at(i - 1);
-                h1[j] = r.x != q.x;
-                h2[j] = r.xy != q.xy;
+                h1 = r.x != q.x;
+                h2 = r.xy != q.xy;
             }
-            c1 += h1[j];
-            c2 += h2[j];
+        }
+        b1[j] = __ballot(h1);
+        b2[j] = __ballot(h2);
+        if (lane == 0) {
+            s_off[0][j * WAVES + w] = (uint32_t)__popcll(b1[j]);
+            s_off[1][j * WAVES + w] = (uint32_t)__popcll(b2[j]);
         }
     }
-    uint32_t tot;
-    uint32_t p1 = dev::block_excl_scan<UN_BLOCK>(c1, s_tmp, &tot);
-    uint32_t p2 = dev::block_excl_scan<UN_BLOCK>(c2, s_tmp, &tot);
-    uint64_t o1 = off_l1[blockIdx.x] + p1, o2 = off_l2[blockIdx.x] + p2;
+    __syncthreads();
+    if (threadIdx.x < 2) {  // exclusive scan of the slot counts, in row order
+        uint32_t run = 0;
+        for (int k = 0; k < SLOTS; ++k) {
+            const uint32_t c = s_off[threadIdx.x][k];
+            s_off[threadIdx.x][k] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    const uint64_t t1 = off_l1[blockIdx.x], t2 = off_l2[blockIdx.x];
 #pragma unroll
     for (int j = 0; j < UN_ITEMS; ++j) {
-        const uint64_t i = base + (uint64_t)threadIdx.x * UN_ITEMS + j;
-        if (h1[j]) {
-            l1v[o1] = xv[j];
-            l1o[o1] = (uint32_t)i;
-            ++o1;
+        const uint64_t i = base + (uint64_t)j * UN_BLOCK + threadIdx.x;
+        if ((b1[j] >> lane) & 1u) {
+            const uint64_t o = t1 + s_off[0][j * WAVES + w] +
+                               __builtin_amdgcn_mbcnt_hi((uint32_t)(b1[j] >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)b1[j], 0u));
+            l1v[o] = xv[j];
+            l1o[o] = (uint32_t)i;
         }
-        if (h2[j]) {
-            l2v[o2] = yv[j];
-            l2o[o2] = (uint32_t)i;
-            ++o2;
+        if ((b2[j] >> lane) & 1u) {
+            const uint64_t o = t2 + s_off[1][j * WAVES + w] +
+                               __builtin_amdgcn_mbcnt_hi((uint32_t)(b2[j] >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)b2[j], 0u));
+            l2v[o] = yv[j];
+            l2o[o] = (uint32_t)i;
         }
     }
 }
@@ -631,56 +704,142 @@ void finish_perm(vd_ctx *ctx, vd_perm_arrays &pa, uint64_t n, const R &rows) {
            (const uint64_t *)o1.p, (const uint64_t *)o2.p, pa.l1v, pa.l1o, pa.l2v, pa.l2o);
 }
 
-void build_perm(vd_ctx *ctx, vd_index *idx, int perm, const uint32_t *cols[3],
-                const int bits[3], uint64_t *keys, uint64_t *tmp) {
+PackLayout pack_layout(int perm, const int bits[3]) {
+    const int xi = PERM_COLS[perm][0], yi = PERM_COLS[perm][1], zi = PERM_COLS[perm][2];
+    PackLayout L{};
+    L.sh[zi] = 0;
+    L.sh[yi] = bits[zi];
+    L.sh[xi] = bits[zi] + bits[yi];
+    for (int c = 0; c < 3; ++c) L.mask[c] = bits[c] >= 32 ? 0xFFFFFFFFu : ((1u << bits[c]) - 1u);
+    return L;
+}
+
+// Whether permutation p can be derived from q: among rows of equal X_p,
+// q's order is (Y_p, Z_p), i.e. Y_p comes before Z_p in q's key.
+bool derivable(int q, int p) {
+    int pos[3];
+    for (int k = 0; k < 3; ++k) pos[PERM_COLS[q][k]] = k;
+    return pos[PERM_COLS[p][1]] < pos[PERM_COLS[p][2]];
+}
+
+// Every permutation of the mask when the whole triple fits one 64-bit key
+// (bits[0] + bits[1] + bits[2] <= 64, n > 0).  Build order: a permutation
+// with a built sibling it can be derived from is derived (a stable sort on
+// its X bits), the others are sorted on all bits; a sibling's sorted keys
+// are kept until its last derived permutation is built.
+void build_packed(vd_ctx *ctx, vd_index *idx, uint32_t mask, const uint32_t *cols[3],
+                  const int bits[3]) {
+    const uint64_t n = idx->n;
+    int order[VD_NUM_PERMS], from[VD_NUM_PERMS], uses[VD_NUM_PERMS] = {};
+    bool planned[VD_NUM_PERMS] = {};
+    int nord = 0;
+    for (;;) {
+        int pick = -1, src = -1;
+        for (int p = 0; p < VD_NUM_PERMS && pick < 0; ++p) {
+            if (!(mask & VD_PERM_BIT(p)) || planned[p]) continue;
+            for (int k = 0; k < nord; ++k)
+                if (derivable(order[k], p)) {
+                    pick = p;
+                    src = order[k];
+                    break;
+                }
+        }
+        for (int p = 0; p < VD_NUM_PERMS && pick < 0; ++p)
+            if ((mask & VD_PERM_BIT(p)) && !planned[p]) pick = p;
+        if (pick < 0) break;
+        planned[pick] = true;
+        from[pick] = src;
+        if (src >= 0) ++uses[src];
+        order[nord++] = pick;
+    }
+
+    const int kbits = bits[0] + bits[1] + bits[2];
+    const uint64_t pk_tiles = ceil_div(n, PK_BLOCK * PK_ITEMS);
+    DBuf<uint64_t> held[VD_NUM_PERMS];  // sorted keys kept as a sibling
+    DBuf<uint64_t> spare;               // a free buffer of n keys
+    DBuf<uint32_t> hist(ctx, swap::HBUF), gofs(ctx, swap::HBUF), dmax(ctx, 1);
+    for (int k = 0; k < nord; ++k) {
+        const int p = order[k];
+        const PackLayout to = pack_layout(p, bits);
+        const int xi = PERM_COLS[p][0], yi = PERM_COLS[p][1];
+        DBuf<uint64_t> keys = spare.p ? std::move(spare) : DBuf<uint64_t>(ctx, n);
+        DBuf<uint64_t> tmp(ctx, n);
+        uint64_t *sorted = nullptr;
+        if (from[p] < 0) {
+            const int npass = radix_passes(0, kbits);
+            DBuf<uint32_t> h(ctx, radix_hist_words(0, kbits));
+            fill_u32(ctx, h.p, 0, radix_hist_words(0, kbits));
+            launch(ctx, "index_pack", (double)n * 20.0, dim3(radix::hist_grid(pk_tiles)),
+                   dim3(PK_BLOCK), 0, k_pack_xyz, cols[xi], cols[yi], cols[PERM_COLS[p][2]], n,
+                   to.sh[yi], to.sh[xi], keys.p, kbits, npass, h.p);
+            sorted = radix_sort_u64(ctx, keys.p, tmp.p, n, 0, kbits, h.p, true);
+        } else {
+            const int q = from[p];
+            fill_u32(ctx, hist.p, 0, swap::HBUF);
+            fill_u32(ctx, dmax.p, to.mask[xi], 1);
+            launch(ctx, "index_repack", (double)n * 16.0, dim3(radix::hist_grid(pk_tiles)),
+                   dim3(PK_BLOCK), 0, k_repack, (const uint64_t *)held[q].p, n,
+                   pack_layout(q, bits), to, xi, keys.p, hist.p, hist.p + swap::H8,
+                   (const uint32_t *)dmax.p, ctx->radix_bits);
+            int shifts[4], dbits = 8;
+            const uint32_t *g[4];
+            const int npass = swap_passes(ctx, hist.p, hist.p + swap::H8, dmax.p, n, gofs.p,
+                                          to.sh[xi], shifts, g, &dbits);
+            RadixCols none{};
+            sorted = radix_sort_shifts(ctx, keys.p, tmp.p, n, shifts, npass, g, 0, none, none,
+                                       none, nullptr, dbits);
+            if (--uses[q] == 0) {
+                if (!spare.p) spare = std::move(held[q]);
+                held[q].reset();
+            }
+        }
+        DBuf<uint64_t> &res = sorted == keys.p ? keys : tmp;
+        DBuf<uint64_t> &other = sorted == keys.p ? tmp : keys;
+        vd_perm_arrays &pa = idx->perm[p];
+        const uint32_t ymask = to.mask[yi], zmask = to.mask[PERM_COLS[p][2]];
+        finish_perm(ctx, pa, n, PackedRows{sorted, to.sh[xi], to.sh[yi], ymask, zmask});
+        pa.built = true;
+        if (uses[p] > 0) held[p] = std::move(res);
+        if (!spare.p) spare = std::move(other);
+    }
+}
+
+// One permutation when the triple needs more than 64 bits: two LSD phases,
+// (Y,Z) with X carried as a value, then a stable sort on X (key X << 32 |
+// position) and a gather of the (Y,Z) words.
+void build_perm_split(vd_ctx *ctx, vd_index *idx, int perm, const uint32_t *cols[3],
+                      const int bits[3], uint64_t *keys, uint64_t *tmp) {
     const uint64_t n = idx->n;
     vd_perm_arrays &pa = idx->perm[perm];
     const int xi = PERM_COLS[perm][0], yi = PERM_COLS[perm][1],
               zi = PERM_COLS[perm][2];
-    const int bz = std::max(bits[zi], 1), by = std::max(bits[yi], 1),
-              bx = std::max(bits[xi], 1);
-    const int kbits = bx + by + bz;
-    const int sy = bz, sx = by + bz;
+    const int bz = bits[zi], by = bits[yi], bx = bits[xi];
+    const int sy = bz;
     const uint32_t ymask = by >= 32 ? 0xFFFFFFFFu : ((1u << by) - 1u);
     const uint32_t zmask = bz >= 32 ? 0xFFFFFFFFu : ((1u << bz) - 1u);
     const uint64_t pk_tiles = ceil_div(n, PK_BLOCK * PK_ITEMS);
-
-    if (kbits <= 64) {
-        // The whole triple in one key: a key-only sort.
-        const int npass = radix_passes(0, kbits);
-        DBuf<uint32_t> hist(ctx, radix_hist_words(0, kbits));
-        fill_u32(ctx, hist.p, 0, radix_hist_words(0, kbits));
-        launch(ctx, "index_pack", (double)n * 20.0, dim3(radix::hist_grid(pk_tiles)),
-               dim3(PK_BLOCK), 0, k_pack_xyz, cols[xi], cols[yi], cols[zi], n, sy,
-               sx, keys, kbits, npass, hist.p);
-        uint64_t *sorted = radix_sort_u64(ctx, keys, tmp, n, 0, kbits, hist.p, true);
-        finish_perm(ctx, pa, n, PackedRows{sorted, sx, sy, ymask, zmask});
-    } else {
-        // Two LSD phases: (Y,Z) with X carried as a value, then a stable
-        // sort on X (key X << 32 | position) and a gather of the (Y,Z) words.
-        const int lbits = by + bz;
-        DBuf<uint32_t> xa(ctx, n), xb(ctx, n);
-        DBuf<uint32_t> hist(ctx, radix_hist_words(0, lbits));
-        fill_u32(ctx, hist.p, 0, radix_hist_words(0, lbits));
-        launch(ctx, "index_pack", (double)n * 24.0, dim3(radix::hist_grid(pk_tiles)), dim3(PK_BLOCK), 0,
-               k_pack_split, cols[xi], cols[yi], cols[zi], n, sy, keys, xa.p, lbits,
-               radix_passes(0, lbits), hist.p);
-        uint64_t *lo = keys, *lo_tmp = tmp;
-        uint32_t *xv = xa.p, *xv_tmp = xb.p;
-        radix_sort_u64_kv(ctx, lo, lo_tmp, xv, xv_tmp, n, 0, lbits, hist.p, true);
-        // lo/xv hold phase-1 order; lo_tmp / xv_tmp are free.
-        DBuf<uint64_t> k2(ctx, n), k2t(ctx, n);
-        const int hi = 32 + bx;
-        const int npass2 = radix_passes(32, hi);
-        DBuf<uint32_t> hist2(ctx, radix_hist_words(32, hi));
-        fill_u32(ctx, hist2.p, 0, radix_hist_words(32, hi));
-        launch(ctx, "index_pack", (double)n * 12.0, dim3(radix::hist_grid(pk_tiles)), dim3(PK_BLOCK), 0,
-               k_pack_xpos, (const uint32_t *)xv, n, k2.p, hi, npass2, hist2.p);
-        uint64_t *s2 = radix_sort_u64(ctx, k2.p, k2t.p, n, 32, hi, hist2.p, true);
-        launch(ctx, "index_gather", (double)n * 24.0, dim3(stream_grid(n)), dim3(256), 0,
-               k_split_gather, (const uint64_t *)s2, (const uint64_t *)lo, n, xv_tmp, lo_tmp);
-        finish_perm(ctx, pa, n, SplitRows{xv_tmp, lo_tmp, sy, ymask, zmask});
-    }
+    const int lbits = by + bz;
+    DBuf<uint32_t> xa(ctx, n), xb(ctx, n);
+    DBuf<uint32_t> hist(ctx, radix_hist_words(0, lbits));
+    fill_u32(ctx, hist.p, 0, radix_hist_words(0, lbits));
+    launch(ctx, "index_pack", (double)n * 24.0, dim3(radix::hist_grid(pk_tiles)), dim3(PK_BLOCK), 0,
+           k_pack_split, cols[xi], cols[yi], cols[zi], n, sy, keys, xa.p, lbits,
+           radix_passes(0, lbits), hist.p);
+    uint64_t *lo = keys, *lo_tmp = tmp;
+    uint32_t *xv = xa.p, *xv_tmp = xb.p;
+    radix_sort_u64_kv(ctx, lo, lo_tmp, xv, xv_tmp, n, 0, lbits, hist.p, true);
+    // lo/xv hold phase-1 order; lo_tmp / xv_tmp are free.
+    DBuf<uint64_t> k2(ctx, n), k2t(ctx, n);
+    const int hi = 32 + bx;
+    const int npass2 = radix_passes(32, hi);
+    DBuf<uint32_t> hist2(ctx, radix_hist_words(32, hi));
+    fill_u32(ctx, hist2.p, 0, radix_hist_words(32, hi));
+    launch(ctx, "index_pack", (double)n * 12.0, dim3(radix::hist_grid(pk_tiles)), dim3(PK_BLOCK), 0,
+           k_pack_xpos, (const uint32_t *)xv, n, k2.p, hi, npass2, hist2.p);
+    uint64_t *s2 = radix_sort_u64(ctx, k2.p, k2t.p, n, 32, hi, hist2.p, true);
+    launch(ctx, "index_gather", (double)n * 24.0, dim3(stream_grid(n)), dim3(256), 0,
+           k_split_gather, (const uint64_t *)s2, (const uint64_t *)lo, n, xv_tmp, lo_tmp);
+    finish_perm(ctx, pa, n, SplitRows{xv_tmp, lo_tmp, sy, ymask, zmask});
     pa.built = true;
 }
 
@@ -862,21 +1021,25 @@ int vd_index_build(vd_ctx *ctx, const uint32_t *d_s, const uint32_t *d_p,
         int bits[3] = {1, 1, 1};
         if (n > 0)
             for (int c = 0; c < 3; ++c) bits[c] = std::max(1, bits_for(max_u32(ctx, cols[c], n)));
-        DBuf<uint64_t> keys(ctx, n), tmp(ctx, n);
-        for (int p = 0; p < VD_NUM_PERMS; ++p) {
-            if (!(perm_mask & VD_PERM_BIT(p))) continue;
-            if (n == 0) {
-                vd_perm_arrays &pa = idx->perm[p];
-                pa.built = true;
-                pa.y = (uint32_t *)dalloc(ctx, 4);
-                pa.z = (uint32_t *)dalloc(ctx, 4);
-                pa.l1v = (uint32_t *)dalloc(ctx, 4);
-                pa.l1o = (uint32_t *)dalloc(ctx, 4);
-                pa.l2v = (uint32_t *)dalloc(ctx, 4);
-                pa.l2o = (uint32_t *)dalloc(ctx, 4);
-                continue;
+        if (n > 0 && bits[0] + bits[1] + bits[2] <= 64) {
+            build_packed(ctx, idx, perm_mask, cols, bits);
+        } else {
+            DBuf<uint64_t> keys(ctx, n), tmp(ctx, n);
+            for (int p = 0; p < VD_NUM_PERMS; ++p) {
+                if (!(perm_mask & VD_PERM_BIT(p))) continue;
+                if (n == 0) {
+                    vd_perm_arrays &pa = idx->perm[p];
+                    pa.built = true;
+                    pa.y = (uint32_t *)dalloc(ctx, 4);
+                    pa.z = (uint32_t *)dalloc(ctx, 4);
+                    pa.l1v = (uint32_t *)dalloc(ctx, 4);
+                    pa.l1o = (uint32_t *)dalloc(ctx, 4);
+                    pa.l2v = (uint32_t *)dalloc(ctx, 4);
+                    pa.l2o = (uint32_t *)dalloc(ctx, 4);
+                    continue;
+                }
+                build_perm_split(ctx, idx, p, cols, bits, keys.p, tmp.p);
             }
-            build_perm(ctx, idx, p, cols, bits, keys.p, tmp.p);
         }
     } catch (const Error &e) {
         ctx->last_error = e.what();

@@ -271,7 +271,7 @@ struct RadixCols {
     uint32_t *p[RADIX_MAX_PAYLOAD];
 };
 struct RadixPayload {  // kernel argument of one pass
-    const uint32_t *in[RADIX_MAX_PAYLOAD];
+    const uint32_t *in[RADIX_MAX_PAYLOAD];  // null: the row id
     uint32_t *out[RADIX_MAX_PAYLOAD];
 };
 // Same with m <= RADIX_MAX_PAYLOAD uint32 columns: the first pass reads
@@ -282,15 +282,40 @@ uint64_t *radix_sort_u64_payload(vd_ctx *ctx, uint64_t *keys, uint64_t *tmp, uin
                                  int lo_bit, int hi_bit, uint32_t *d_hist, bool hist_ready,
                                  int m, const RadixCols &src, const RadixCols &pa,
                                  const RadixCols &pb, RadixCols *final);
+// Keys held as two 32-bit columns, key = hi << 32 | lo (lo null: 0): with
+// in_hi set, the first pass reads the keys from (in_hi, in_lo) instead of
+// the key buffer; with out_hi set, the last pass writes the sorted keys'
+// halves to (out_hi, out_lo) (out_lo null: the low half is dropped) instead
+// of a key buffer.  The columns written must not alias those read when a
+// single pass does both.
+struct RadixKeyIO {
+    const uint32_t *in_hi, *in_lo;
+    uint32_t *out_hi, *out_lo;
+};
 // Stable sort of n keys on the digits of digit_bits (8 or 9) bits at the
 // given shifts (LSD order; digits not listed must be equal in every key).
 // d_gofs[p] points to the 2^digit_bits exclusive digit offsets of pass p
-// (device).  Payload columns as for radix_sort_u64_payload.  Returns the
-// buffer holding the sorted keys (keys or tmp).
+// (device).  Payload columns as for radix_sort_u64_payload; a null src
+// column is the row id (0, 1, ...).  Returns the buffer holding the sorted
+// keys (keys or tmp; meaningless when kio->out_hi is set).  With kio, a
+// first pass reading columns does not read `keys` and a last pass writing
+// columns does not write its key buffer: 1 pass needs neither buffer, 2
+// passes only tmp.
 uint64_t *radix_sort_shifts(vd_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n,
                             const int *shifts, int npass, const uint32_t *const *d_gofs, int m,
                             const RadixCols &src, const RadixCols &pa, const RadixCols &pb,
-                            RadixCols *final, int digit_bits = 8);
+                            RadixCols *final, int digit_bits = 8,
+                            const RadixKeyIO *kio = nullptr);
+// Digit plan of a stable sort on a 32-bit value held at bits [lo_bit,
+// lo_bit + 32) of its keys, from the histograms h8 / h9 (the swap::Plan of
+// the value's maximum *d_max under the radix_bits tunable, counted by the
+// producer with swap::hist_add): writes their exclusive offsets into gofs
+// (swap::HBUF words) and reads back which digits vary.  Fills shifts[] and
+// g[] (at most 4) for those digits, sets *digit_bits (8 or 9) and returns
+// the number of passes (0: the value is the same in every row).
+int swap_passes(vd_ctx *ctx, const uint32_t *h8, const uint32_t *h9, const uint32_t *d_max,
+                uint64_t n, uint32_t *gofs, int lo_bit, int *shifts, const uint32_t **g,
+                int *digit_bits);
 int radix_passes(int lo_bit, int hi_bit);
 // Device histogram buffer size (uint32 words) needed for radix_sort_u64.
 inline size_t radix_hist_words(int lo_bit, int hi_bit) {

@@ -93,7 +93,7 @@ template <int BLOCK, int ITEMS, int M, int DB = RBITS>
 __global__ __launch_bounds__(BLOCK)
 __attribute__((amdgpu_waves_per_eu(OsShape<BLOCK, ITEMS, M>::WAVES_PER_EU, 8))) void k_rs_onesweep(
     const uint64_t *__restrict__ in, uint64_t *__restrict__ out, RadixPayload pay,
-    uint64_t n, int shift, int width, const uint32_t *__restrict__ gofs,
+    RadixKeyIO kio, uint64_t n, int shift, int width, const uint32_t *__restrict__ gofs,
     uint32_t *lookback, uint32_t *tile_counter) {
     constexpr int WAVES = BLOCK / 64;
     constexpr int TILE = BLOCK * ITEMS;
@@ -125,15 +125,33 @@ __attribute__((amdgpu_waves_per_eu(OsShape<BLOCK, ITEMS, M>::WAVES_PER_EU, 8))) 
 
     // Load: position within the tile = (w * ITEMS + j) * 64 + lane, which is
     // also the ranking order below.  Missing keys are all-ones: they carry
-    // the largest digit and rank after every real key of the tile.
+    // the largest digit and rank after every real key of the tile.  Keys
+    // come from the key buffer or from two columns; a payload column
+    // without a source is the row id.
     uint64_t k[ITEMS];
     uint32_t v[M ? M : 1][ITEMS];
+    if (kio.in_hi) {
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-        const uint64_t idx = base + (uint64_t)((w * ITEMS + j) * 64 + lane);
-        k[j] = idx < n ? in[idx] : ~0ull;
+        for (int j = 0; j < ITEMS; ++j) {
+            const uint64_t idx = base + (uint64_t)((w * ITEMS + j) * 64 + lane);
+            k[j] = idx < n ? ((uint64_t)kio.in_hi[idx] << 32) | (kio.in_lo ? kio.in_lo[idx] : 0u)
+                           : ~0ull;
+        }
+    } else {
 #pragma unroll
-        for (int c = 0; c < M; ++c) v[c][j] = idx < n ? pay.in[c][idx] : 0u;
+        for (int j = 0; j < ITEMS; ++j) {
+            const uint64_t idx = base + (uint64_t)((w * ITEMS + j) * 64 + lane);
+            k[j] = idx < n ? in[idx] : ~0ull;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < M; ++c) {
+        const uint32_t *src = pay.in[c];
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            const uint64_t idx = base + (uint64_t)((w * ITEMS + j) * 64 + lane);
+            v[c][j] = idx < n ? (src ? src[idx] : (uint32_t)idx) : 0u;
+        }
     }
 
     // Rank within the wave: peers = lanes holding the same digit (digit bits
@@ -241,14 +259,29 @@ __attribute__((amdgpu_waves_per_eu(OsShape<BLOCK, ITEMS, M>::WAVES_PER_EU, 8))) 
 
     // Staged position i (i = tid + BLOCK * r) goes to dst[r].  Positions at
     // or past nvalid hold the padding keys and are not written.
+    // The sorted keys go to the key buffer or, split, to two columns.
     uint32_t dst[ITEMS];
+    if (kio.out_hi) {
 #pragma unroll
-    for (int r = 0; r < ITEMS; ++r) {
-        const uint32_t i = (uint32_t)(tid + r * BLOCK);
-        const uint64_t key = s_keys[i];
-        const uint32_t d = (uint32_t)(key >> shift) & dmask;
-        dst[r] = s_gbase[d] + i;
-        if (i < nvalid) out[dst[r]] = key;
+        for (int r = 0; r < ITEMS; ++r) {
+            const uint32_t i = (uint32_t)(tid + r * BLOCK);
+            const uint64_t key = s_keys[i];
+            const uint32_t d = (uint32_t)(key >> shift) & dmask;
+            dst[r] = s_gbase[d] + i;
+            if (i < nvalid) {
+                kio.out_hi[dst[r]] = (uint32_t)(key >> 32);
+                if (kio.out_lo) kio.out_lo[dst[r]] = (uint32_t)key;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < ITEMS; ++r) {
+            const uint32_t i = (uint32_t)(tid + r * BLOCK);
+            const uint64_t key = s_keys[i];
+            const uint32_t d = (uint32_t)(key >> shift) & dmask;
+            dst[r] = s_gbase[d] + i;
+            if (i < nvalid) out[dst[r]] = key;
+        }
     }
     uint32_t *s_v = reinterpret_cast<uint32_t *>(s_keys);
 #pragma unroll
@@ -269,12 +302,13 @@ __attribute__((amdgpu_waves_per_eu(OsShape<BLOCK, ITEMS, M>::WAVES_PER_EU, 8))) 
 // payload from src (first pass) or the buffer the previous pass wrote, into
 // pa (odd passes) or pb (even passes); *final receives the last buffers.
 // Pass p sorts on the digit at shifts[p] (widths[p] bits) with the 256
-// exclusive digit offsets at gofs[p].
+// exclusive digit offsets at gofs[p].  kio (may be null): the first pass
+// reads, the last writes, split key columns instead of a and b.
 template <int BLOCK, int ITEMS, int M, int DB>
 void run_passes(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, const RadixCols &src,
                 const RadixCols &pa, const RadixCols &pb, RadixCols *final,
                 uint64_t n, const int *shifts, const int *widths, int npass,
-                const uint32_t *const *gofs) {
+                const uint32_t *const *gofs, const RadixKeyIO *kio) {
     constexpr uint64_t TILE = (uint64_t)BLOCK * ITEMS;
     constexpr int BINS = 1 << DB;
     const uint64_t ntiles = ceil_div(n, TILE);
@@ -290,10 +324,22 @@ void run_passes(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, const RadixCols &src,
             pay.in[c] = cur.p[c];
             pay.out[c] = dstc.p[c];
         }
+        RadixKeyIO io{};
+        double kbytes = 16.0;
+        if (kio && p == 0) {
+            io.in_hi = kio->in_hi;
+            io.in_lo = kio->in_lo;
+            if (io.in_hi) kbytes -= io.in_lo ? 0.0 : 4.0;
+        }
+        if (kio && p == npass - 1) {
+            io.out_hi = kio->out_hi;
+            io.out_lo = kio->out_lo;
+            if (io.out_hi) kbytes -= io.out_lo ? 0.0 : 4.0;
+        }
         launch(ctx, M ? "radix_onesweep_kv" : "radix_onesweep",
-               (double)n * (16.0 + 8.0 * M), dim3((unsigned)ntiles),
+               (double)n * (kbytes + 8.0 * M), dim3((unsigned)ntiles),
                dim3(BLOCK), 0, k_rs_onesweep<BLOCK, ITEMS, M, DB>, (const uint64_t *)a, b, pay,
-               n, shifts[p], widths[p], gofs[p],
+               io, n, shifts[p], widths[p], gofs[p],
                lb.p + (uint64_t)p * ntiles * BINS, counters + p);
         std::swap(a, b);
         cur = dstc;
@@ -309,12 +355,13 @@ void run_passes(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, const RadixCols &src,
 void dispatch_passes(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, int m, const RadixCols &src,
                      const RadixCols &pa, const RadixCols &pb, RadixCols *final, uint64_t n,
                      const int *shifts, const int *widths, int npass,
-                     const uint32_t *const *g, int digit_bits = RBITS) {
+                     const uint32_t *const *g, int digit_bits = RBITS,
+                     const RadixKeyIO *kio = nullptr) {
     const int items = ctx->radix_items;
     if (digit_bits == 9) {
         // 512-thread shapes only (one digit per thread), as the default.
 #define VD_RUN9(IT, MM) \
-    run_passes<512, IT, MM, 9>(ctx, a, b, src, pa, pb, final, n, shifts, widths, npass, g)
+    run_passes<512, IT, MM, 9>(ctx, a, b, src, pa, pb, final, n, shifts, widths, npass, g, kio)
         switch (m) {
             case 0: VD_RUN9(16, 0); break;
             case 1: VD_RUN9(16, 1); break;
@@ -325,7 +372,8 @@ void dispatch_passes(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, int m, const Radix
         return;
     }
 #define VD_RUN(BL, IT, MM) \
-    run_passes<BL, IT, MM, RBITS>(ctx, a, b, src, pa, pb, final, n, shifts, widths, npass, g)
+    run_passes<BL, IT, MM, RBITS>(ctx, a, b, src, pa, pb, final, n, shifts, widths, npass, g, \
+                                  kio)
     if (m > 0) {
         if (items == 8) {
             switch (m) {
@@ -429,7 +477,7 @@ static uint64_t *sort_impl(vd_ctx *ctx, uint64_t *a, uint64_t *b, int m,
 uint64_t *radix_sort_shifts(vd_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n,
                             const int *shifts, int npass, const uint32_t *const *d_gofs, int m,
                             const RadixCols &src, const RadixCols &pa, const RadixCols &pb,
-                            RadixCols *final, int digit_bits) {
+                            RadixCols *final, int digit_bits, const RadixKeyIO *kio) {
     VD_REQUIRE(m >= 0 && m <= RADIX_MAX_PAYLOAD, VD_EINVAL, "payload columns");
     VD_REQUIRE(npass >= 0 && npass <= radix::MAX_PASSES, VD_EINVAL, "radix pass count");
     VD_REQUIRE(digit_bits == 8 || digit_bits == 9, VD_EINVAL, "radix digit width");
@@ -441,7 +489,7 @@ uint64_t *radix_sort_shifts(vd_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t
     for (int p = 0; p < npass; ++p) widths[p] = std::min(digit_bits, 64 - shifts[p]);
     uint64_t *a = keys, *b = tmp;
     dispatch_passes(ctx, a, b, m, src, pa, pb, final, n, shifts, widths, npass, d_gofs,
-                    digit_bits);
+                    digit_bits, kio);
     return a;
 }
 

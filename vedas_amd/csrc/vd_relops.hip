@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "vd_radix_dev.hpp"
+#include "vd_swap_dev.hpp"
 
 namespace vd {
 namespace {
@@ -50,90 +51,44 @@ __global__ __launch_bounds__(RB) void k_pack_rowid(
     radix::hist_flush(&s, ghist, npass);
 }
 
-// Counts one digit per calling lane into a wave's LDS bins; when every
-// calling lane of the wave holds the same digit (the top digits of small
-// IDs), one lane adds the lane count (LDS atomics on one address from many
-// lanes are serialised).
-__device__ __forceinline__ void count_digit(uint32_t *bins, uint32_t d) {
-    const uint64_t act = __ballot(1);
-    const int leader = __ffsll((unsigned long long)act) - 1;
-    const uint32_t d0 = __shfl(d, leader, 64);
-    if (__ballot(d == d0) == act) {
-        if ((int)(threadIdx.x & 63) == leader) atomicAdd(&bins[d0], (uint32_t)__popcll(act));
-    } else {
-        atomicAdd(&bins[d], 1u);
-    }
-}
-
-// Digit plan of an IndexSwap sort from column 0's maximum mx: digits below
-// its bit length can vary, the ones above are 0 in every row.  `bits` is
-// the radix_bits tunable: 0 = 9-bit digits (bits 0..26) when they need
-// fewer passes than bytes, else 8-bit; 8 = always bytes; 9 = 9-bit digits
-// whenever the key fits 27 bits.
-struct SwapPlan {
-    bool use9;
-    int ndig;  // digits that can vary
-};
-__host__ __device__ inline SwapPlan swap_plan(uint32_t mx, int bits) {
-    int nb = 0;
-    while (nb < 32 && (mx >> nb) != 0) ++nb;
-    const int n8 = (nb + 7) / 8, n9 = (nb + 8) / 9;
-    const bool use9 = nb <= 27 && (bits == 9 || (bits == 0 && n9 < n8));
-    return SwapPlan{use9, use9 ? n9 : n8};
-}
-
-// Histograms of column 0 for the plan: 4 x 256 words (8-bit digits) or
-// 3 x 512 (9-bit digits), one copy per wave in LDS.
-constexpr int SWAP_H8 = 4 * 256;
-constexpr int SWAP_H9 = 3 * 512;
-constexpr int SWAP_HW = SWAP_H9 > SWAP_H8 ? SWAP_H9 : SWAP_H8;
-
-// key = col0[i] << 32 | col1[i] (no col1: 0); with vals, vals[i] = vsrc[i]
-// (vsrc null: the row id i).  Counts the histograms of the digits that can
-// vary under the plan of *gmax (column 0's maximum, computed before) into
-// g8 (8-bit plan) or g9 (9-bit plan).  The top digit counts through
-// count_digit (its values are the most concentrated), the others with plain
-// LDS atomics.
-__global__ __launch_bounds__(RB) void k_pack_c01(
-    const uint32_t *__restrict__ c0, const uint32_t *__restrict__ c1,
-    const uint32_t *__restrict__ vsrc, uint64_t n, uint64_t *__restrict__ keys,
-    uint32_t *__restrict__ vals, uint32_t *g8, uint32_t *g9, const uint32_t *gmax, int bits) {
-    __shared__ uint32_t s_h[radix::HIST_WAVES][SWAP_HW];
+// Histograms of column 0's digits under the plan of *gmax (column 0's
+// maximum, computed before) into g8 (8-bit plan) or g9 (9-bit plan).
+// 16-byte loads over the aligned body (column 0 of a view starts at any
+// 4-byte offset); the < 4 values before it and the < 4 after it are counted
+// by the first threads of the grid.
+__global__ __launch_bounds__(RB) void k_swap_hist(const uint32_t *__restrict__ c0, uint64_t n,
+                                                  uint32_t *g8, uint32_t *g9,
+                                                  const uint32_t *gmax, int bits) {
+    __shared__ uint32_t s_h[radix::HIST_WAVES][swap::HW];
     static_assert(RB == radix::HIST_BLOCK, "one histogram copy per wave");
-    for (int i = threadIdx.x; i < radix::HIST_WAVES * SWAP_HW; i += RB) (&s_h[0][0])[i] = 0;
-    const SwapPlan plan = swap_plan(*gmax, bits);
-    const int W = plan.use9 ? 9 : 8, bins = 1 << W, nd = plan.ndig;
-    const uint32_t mask = (uint32_t)bins - 1u;
+    for (int i = threadIdx.x; i < radix::HIST_WAVES * swap::HW; i += RB) (&s_h[0][0])[i] = 0;
+    const swap::Plan plan = swap::plan_for(*gmax, bits);
     __syncthreads();
     uint32_t *h = s_h[threadIdx.x / 64];
-    for (uint64_t base = (uint64_t)blockIdx.x * (RB * R_ITEMS); base < n;
-         base += (uint64_t)gridDim.x * (RB * R_ITEMS)) {
-#pragma unroll 4
-        for (int j = 0; j < R_ITEMS; ++j) {
-            const uint64_t i = base + (uint64_t)j * RB + threadIdx.x;
-            if (i < n) {
-                const uint32_t v = c0[i];
-                keys[i] = ((uint64_t)v << 32) | (c1 ? c1[i] : 0u);
-                if (vals) vals[i] = vsrc ? vsrc[i] : (uint32_t)i;
-                for (int d = 0; d + 1 < nd; ++d)
-                    atomicAdd(&h[d * bins + ((v >> (W * d)) & mask)], 1u);
-                if (nd > 0) count_digit(&h[(nd - 1) * bins], (v >> (W * (nd - 1))) & mask);
-            }
-        }
+    const uint64_t tid = (uint64_t)blockIdx.x * RB + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * RB;
+    const uint64_t mis = (reinterpret_cast<uintptr_t>(c0) & 15u) / 4u;
+    const uint64_t head = min<uint64_t>(n, (4u - mis) & 3u);
+    const uint4 *q = reinterpret_cast<const uint4 *>(c0 + head);
+    const uint64_t nq = (n - head) / 4;
+    const uint64_t t0 = head + nq * 4;
+    if (tid < head) swap::hist_add(h, c0[tid], plan);
+    if (t0 + tid < n) swap::hist_add(h, c0[t0 + tid], plan);
+#pragma unroll 2
+    for (uint64_t i = tid; i < nq; i += stride) {
+        const uint4 v = q[i];
+        swap::hist_add(h, v.x, plan);
+        swap::hist_add(h, v.y, plan);
+        swap::hist_add(h, v.z, plan);
+        swap::hist_add(h, v.w, plan);
     }
     __syncthreads();
-    uint32_t *g = plan.use9 ? g9 : g8;
-    for (int i = threadIdx.x; i < nd * bins; i += RB) {
-        uint32_t c = 0;
-#pragma unroll
-        for (int ww = 0; ww < radix::HIST_WAVES; ++ww) c += s_h[ww][i];
-        if (c) atomicAdd(&g[i], c);
-    }
+    swap::hist_flush(&s_h[0][0], radix::HIST_WAVES, plan, g8, g9);
 }
 
-// One block: the exclusive scans of the digits the pack counted (the plan
-// of column 0's maximum *gmax) into gofs8 or gofs9, then a read-back of
-// {bit d set: digit d varies (no bin holds all n rows), the maximum}.
+// One block: the exclusive scans of the digits the producer counted (the
+// plan of the maximum *gmax) into gofs8 or gofs9, then a read-back of {bit
+// d set: digit d varies (no bin holds all n rows), the maximum}.
 constexpr int SP_BLOCK = 512;
 __global__ __launch_bounds__(SP_BLOCK) void k_swap_plan_scan(
     const uint32_t *__restrict__ g8, const uint32_t *__restrict__ g9, const uint32_t *gmax,
@@ -141,7 +96,7 @@ __global__ __launch_bounds__(SP_BLOCK) void k_swap_plan_scan(
     int bits) {
     __shared__ uint32_t s_tmp[SP_BLOCK / 64];
     const uint32_t mx = *gmax;
-    const SwapPlan plan = swap_plan(mx, bits);
+    const swap::Plan plan = swap::plan_for(mx, bits);
     const int bins = plan.use9 ? 512 : 256;
     const uint32_t *h = plan.use9 ? g9 : g8;
     uint32_t *go = plan.use9 ? gofs9 : gofs8;
@@ -161,24 +116,6 @@ __global__ __launch_bounds__(SP_BLOCK) void k_swap_plan_scan(
             rb.dst[1] = mx;
         }
         dev::rb_publish(rb);
-    }
-}
-
-// out0 = key >> 32, out1 = low word of the key; with rows, out_c =
-// in_c[rows[i]] for c >= 2 (rows = the source row of each sorted key).
-__global__ __launch_bounds__(256) void k_unpack_c01(const uint64_t *__restrict__ keys,
-                                                    const uint32_t *__restrict__ rows,
-                                                    uint64_t n, Cols cols) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += stride) {
-        const uint64_t k = keys[i];
-        cols.out[0][i] = (uint32_t)(k >> 32);
-        if (cols.n >= 2) cols.out[1][i] = (uint32_t)k;
-        if (rows) {
-            const uint32_t r = rows[i];
-            for (int c = 2; c < cols.n; ++c) cols.out[c][i] = cols.in[c][r];
-        }
     }
 }
 
@@ -247,15 +184,17 @@ uint64_t *sort_by_col(vd_ctx *ctx, const uint32_t *col, const uint32_t *perm,
 }
 
 // Stable sort by column 0 (the reference's sortByFirstColumn).  The sort
-// key is col0 << 32 | col1, sorted on the bytes of column 0 only (LSD
-// passes are stable, so equal keys keep their order).  The packing kernel
-// counts the digit histograms of column 0's four bytes; they are read back
-// as one flag per byte, and the bytes that are equal in every row (the top
-// bytes of small IDs) get no pass.  Columns 2..4 travel through the passes
-// as payload columns (read from the relation by the first pass); wider
-// relations carry the row id and gather columns 2.. once at the end.  Owned
-// columns are reused as buffers; borrowed (view) columns are replaced,
-// never written.
+// key is col0 << 32 | col1, sorted on the digits of column 0 only (LSD
+// passes are stable, so equal keys keep their order).  Column 0's maximum
+// picks the digit plan; one kernel counts its digits' histograms, which are
+// read back as one flag per digit, and the digits that are equal in every
+// row (the top digits of small IDs) get no pass.  The first pass reads the
+// keys from columns 0 and 1 and the last writes them back as columns, so
+// the keys exist in memory only between passes.  Columns 2..4 travel
+// through the passes as payload columns (read from the relation by the
+// first pass); wider relations carry the row id and gather columns 2.. once
+// at the end.  Owned columns are reused as buffers; borrowed (view) columns
+// are replaced, never written.
 void sort_col0(vd_ctx *ctx, vd_rel *rel) {
     const uint64_t n = rel->nrows;
     const int nc = rel->ncols;
@@ -264,44 +203,24 @@ void sort_col0(vd_ctx *ctx, vd_rel *rel) {
     VD_REQUIRE(n < (1ull << 30), VD_ERANGE, "sort supports fewer than 2^30 rows");
     const bool by_row = nc > 2 + RADIX_MAX_PAYLOAD;
     const int m = nc <= 2 ? 0 : (by_row ? 1 : nc - 2);
-    DBuf<uint64_t> keys(ctx, n), ktmp(ctx, n);
-    DBuf<uint32_t> rows(ctx, by_row ? n : 0);
-    // Histograms of either digit plan in one buffer [8-bit: 4 x 256 | 9-bit:
-    // 3 x 512] and their offsets; column 0's maximum, taken first: it
-    // selects the plan.
-    DBuf<uint32_t> hist(ctx, SWAP_H8 + SWAP_H9), gofs(ctx, SWAP_H8 + SWAP_H9);
-    DBuf<uint32_t> dmax(ctx, 1);
-    uint32_t *h8 = hist.p, *h9 = hist.p + SWAP_H8;
-    fill_u32(ctx, hist.p, 0, SWAP_H8 + SWAP_H9);
+    DBuf<uint32_t> hist(ctx, swap::HBUF), gofs(ctx, swap::HBUF), dmax(ctx, 1);
+    uint32_t *h8 = hist.p, *h9 = hist.p + swap::H8;
+    fill_u32(ctx, hist.p, 0, swap::HBUF);
     fill_u32(ctx, dmax.p, 0, 1);
     max_u32_dev(ctx, rel->cols[0], n, dmax.p);
-    launch(ctx, "sort_pack", (double)n * (8.0 + 4.0 * std::min(nc, 2) + (by_row ? 4.0 : 0.0)),
-           dim3(radix::hist_grid(ceil_div(n, RB * R_ITEMS))), dim3(RB), 0, k_pack_c01,
-           (const uint32_t *)rel->cols[0], (const uint32_t *)(nc >= 2 ? rel->cols[1] : nullptr),
-           (const uint32_t *)nullptr, n, keys.p, by_row ? rows.p : nullptr, h8, h9,
+    launch(ctx, "sort_hist", (double)n * 4.0, dim3(radix::hist_grid(ceil_div(n, RB * R_ITEMS))),
+           dim3(RB), 0, k_swap_hist, (const uint32_t *)rel->cols[0], n, h8, h9,
            (const uint32_t *)dmax.p, ctx->radix_bits);
-    const Readback rb = rb_next(ctx);
-    launch(ctx, "radix_scan_hist", 0.0, dim3(1), dim3(SP_BLOCK), 0, k_swap_plan_scan,
-           (const uint32_t *)h8, (const uint32_t *)h9, (const uint32_t *)dmax.p, n, gofs.p,
-           gofs.p + SWAP_H8, rb, ctx->radix_bits);
-    uint64_t hv[2];
-    rb_wait(ctx, rb, hv, 2);
-    // Passes: the digits of the plan that vary (the pack counted only the
-    // plan's digits below the maximum's bit length).
-    const SwapPlan plan = swap_plan((uint32_t)hv[1], ctx->radix_bits);
-    int shifts[4];
+    int shifts[4], dbits = 8;
     const uint32_t *g[4];
-    int npass = 0;
-    for (int d = 0; d < plan.ndig; ++d)
-        if ((hv[0] >> d) & 1u) {
-            shifts[npass] = 32 + (plan.use9 ? 9 : 8) * d;
-            g[npass] = plan.use9 ? gofs.p + SWAP_H8 + d * 512 : gofs.p + d * 256;
-            ++npass;
-        }
+    const int npass = swap_passes(ctx, h8, h9, dmax.p, n, gofs.p, 32, shifts, g, &dbits);
     if (npass == 0) return;  // column 0 is constant: already in order
 
-    // Payload buffers: src = the relation's columns (or the row ids); pa
-    // aliases src where the relation owns the column, pb is fresh.
+    // Key buffers between passes: none for one pass, one for two.
+    DBuf<uint64_t> keys(ctx, npass >= 3 ? n : 0), ktmp(ctx, npass >= 2 ? n : 0);
+    DBuf<uint32_t> rows(ctx, by_row ? n : 0);
+    // Payload buffers: src = the relation's columns (null for the row ids);
+    // pa aliases src where the relation owns the column, pb is fresh.
     std::vector<uint32_t *> fresh;
     auto fresh_col = [&]() {
         fresh.push_back(nullptr);
@@ -311,38 +230,40 @@ void sort_col0(vd_ctx *ctx, vd_rel *rel) {
     RadixCols src{}, pa{}, pb{}, fin{};
     uint32_t *out01[2] = {nullptr, nullptr};
     uint32_t *gathered[VD_MAX_COLS] = {};
-    uint64_t *sorted = nullptr;
     try {
         for (int c = 0; c < m; ++c) {
             if (by_row) {
-                src.p[0] = pa.p[0] = rows.p;
+                src.p[0] = nullptr;
+                pa.p[0] = rows.p;
             } else {
                 src.p[c] = rel->cols[2 + c];
                 pa.p[c] = rel_borrowed(rel, 2 + c) ? fresh_col() : rel->cols[2 + c];
             }
             pb.p[c] = fresh_col();
         }
-        sorted = radix_sort_shifts(ctx, keys.p, ktmp.p, n, shifts, npass, g, m, src, pa, pb,
-                                   &fin, plan.use9 ? 9 : 8);
+        // A single pass reads and writes columns 0 and 1 in one launch: it
+        // writes fresh ones.
         for (int c = 0; c < std::min(nc, 2); ++c)
-            out01[c] = rel_borrowed(rel, c) ? fresh_col() : rel->cols[c];
+            out01[c] = (npass == 1 || rel_borrowed(rel, c)) ? fresh_col() : rel->cols[c];
         if (by_row)
             for (int c = 2; c < nc; ++c) gathered[c] = fresh_col();
+        RadixKeyIO kio{rel->cols[0], nc >= 2 ? rel->cols[1] : nullptr, out01[0], out01[1]};
+        radix_sort_shifts(ctx, keys.p, ktmp.p, n, shifts, npass, g, m, src, pa, pb, &fin, dbits,
+                          &kio);
     } catch (...) {
         for (auto p : fresh) dfree(ctx, p);
         throw;
     }
-    Cols cols{};
-    cols.n = nc;
-    for (int c = 0; c < nc; ++c) cols.in[c] = rel->cols[c];
-    cols.out[0] = out01[0];
-    if (nc >= 2) cols.out[1] = out01[1];
-    if (by_row)
-        for (int c = 2; c < nc; ++c) cols.out[c] = gathered[c];
-    launch(ctx, "sort_unpack",
-           (double)n * (8.0 + 4.0 * std::min(nc, 2) + (by_row ? 4.0 + 8.0 * (nc - 2) : 0.0)),
-           dim3(sgrid(n)), dim3(256), 0, k_unpack_c01, (const uint64_t *)sorted,
-           (const uint32_t *)(by_row ? fin.p[0] : nullptr), n, cols);
+    if (by_row) {
+        Cols cols{};
+        cols.n = nc - 2;
+        for (int c = 2; c < nc; ++c) {
+            cols.in[c - 2] = rel->cols[c];
+            cols.out[c - 2] = gathered[c];
+        }
+        launch(ctx, "sort_gather", (double)n * (4.0 + 8.0 * (nc - 2)), dim3(sgrid(n)),
+               dim3(256), 0, k_gather_perm, (const uint32_t *)fin.p[0], n, cols);
+    }
     // Install the new columns; free the scratch that is not installed.
     auto installed = [&](uint32_t *p) {
         for (int c = 0; c < nc; ++c)
@@ -434,6 +355,30 @@ void unique_adjacent(vd_ctx *ctx, vd_rel *rel) {
 }
 
 }  // namespace
+
+int swap_passes(vd_ctx *ctx, const uint32_t *h8, const uint32_t *h9, const uint32_t *d_max,
+                uint64_t n, uint32_t *gofs, int lo_bit, int *shifts, const uint32_t **g,
+                int *digit_bits) {
+    const Readback rb = rb_next(ctx);
+    launch(ctx, "radix_scan_hist", 0.0, dim3(1), dim3(SP_BLOCK), 0, k_swap_plan_scan, h8, h9,
+           d_max, n, gofs, gofs + swap::H8, rb, ctx->radix_bits);
+    uint64_t hv[2];
+    rb_wait(ctx, rb, hv, 2);
+    // The digits of the plan that vary (the producer counted only the
+    // plan's digits below the maximum's bit length).
+    const swap::Plan plan = swap::plan_for((uint32_t)hv[1], ctx->radix_bits);
+    const int W = plan.use9 ? 9 : 8;
+    *digit_bits = W;
+    int npass = 0;
+    for (int d = 0; d < plan.ndig; ++d)
+        if ((hv[0] >> d) & 1u) {
+            shifts[npass] = lo_bit + W * d;
+            g[npass] = plan.use9 ? gofs + swap::H8 + d * 512 : gofs + d * 256;
+            ++npass;
+        }
+    return npass;
+}
+
 }  // namespace vd
 
 using namespace vd;
