@@ -188,7 +188,7 @@ int  vd_rel_drop_col(vd_ctx *ctx, vd_rel *rel, int i); /* removeColumn :729 */
 int  vd_rel_materialize(vd_ctx *ctx, vd_rel *rel);
 int  vd_rel_is_view(const vd_rel *rel, int c);
 /* First and last value of column 0 as vd_join computed them with its size
- * read-back: returns 1 and sets *lo/*hi, or 0 when unknown (another
+ * read-back: returns 1 and sets *lo and *hi, or 0 when unknown (another
  * producer, column 0 changed since, or no rows).  Replaces the two device
  * reads of the join result's ends in JoinQueryJob::join
  * (JoinQueryJob.cu:388-397). */

@@ -173,3 +173,35 @@ def test_select_multi(ctx):
             r.free()
     finally:
         idx.free()
+
+
+def directory_pair(rng, nl=6000, hot=1_000_000, tail=1_000_000, span=1_000_000):
+    """A sparse left side against a dense right side with a hot key: every
+    2048-row left tile spans the whole key range, so its right range (2M
+    rows) is staged as a directory of every k-th key (k ~ 250-500) and the
+    exact bounds come from searches in the gaps between directory entries:
+    keys equal to the hot key, keys inside and at the ends of gaps, and keys
+    absent from the right side."""
+    rk = np.concatenate([np.full(hot, span // 2), rng.integers(0, span, tail)])
+    lk = np.concatenate([rng.integers(0, span, nl - 4), [span // 2] * 2, [0, span + 7]])
+    R = O.sort_col0([rk.astype(np.uint32), rng.integers(0, 1 << 30, len(rk)).astype(np.uint32)])
+    L = O.sort_col0([lk.astype(np.uint32), rng.integers(0, 1 << 30, len(lk)).astype(np.uint32)])
+    return L, R
+
+
+def test_join_multi_directory_gaps(ctx):
+    from vedas_amd._native import Rel, join_multi
+    rng = np.random.default_rng(5)
+    Ls, Rs, want = [], [], []
+    for nl in (6000, 300, 2049):
+        L, R = directory_pair(rng, nl=nl)
+        Ls.append(Rel.from_columns(ctx, L))
+        Rs.append(Rel.from_columns(ctx, R))
+        want.append(O.join(L, R, 1))
+    outs = join_multi(ctx, Ls, Rs)
+    for j, (o, w) in enumerate(zip(outs, want)):
+        assert o.nrows == len(w[0]) and o.nrows > 2_000_000, j
+        for g, ww in zip(o.columns(), w):
+            assert np.array_equal(g, ww), j
+    for r in Ls + Rs + outs:
+        r.free()

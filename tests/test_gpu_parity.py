@@ -409,6 +409,29 @@ def test_join_tunables_and_skew(ctx, nv, rcap):
         ctx.tune("join_rcap", 4096)
 
 
+@pytest.mark.parametrize("rcap", [2048, 4096, 8192])
+def test_join_directory_gaps(ctx, rcap):
+    """Left tiles whose right range (2M rows around a hot key) exceeds the
+    LDS staging: the bounds are finished by searches in the gaps between
+    directory entries (tests/test_gpu_batch.py:directory_pair), bit-equal
+    to the oracle for every staging size."""
+    from vedas_amd._native import Rel
+    from tests.test_gpu_batch import directory_pair
+    ctx.tune("join_rcap", rcap)
+    try:
+        L, R = directory_pair(np.random.default_rng(rcap))
+        want = O.join(L, R, 1)
+        gl, gr = Rel.from_columns(ctx, L), Rel.from_columns(ctx, R)
+        out = gl.join(gr, 1)
+        assert out.nrows == len(want[0]) and out.nrows > 2_000_000
+        for g, w in zip(out.columns(), want):
+            assert np.array_equal(g, w)
+        for r in (gl, gr, out):
+            r.free()
+    finally:
+        ctx.tune("join_rcap", 4096)
+
+
 def test_join_empty_sides(ctx):
     from vedas_amd._native import Rel
     rng = np.random.default_rng(1)

@@ -166,25 +166,30 @@ __global__ __launch_bounds__(MB_BLOCK) void k_join_bounds_multi(
     uint32_t t0[MB_ITEMS], t1[MB_ITEMS];
     dev::lockstep_bounds<MB_ITEMS>(s_r, (uint32_t)nd, lkey, t0, t1);
     const uint64_t row0 = (uint64_t)blockIdx.x * MB_TILE;  // tile b in the row space
+    // bounds relative to rlo (with a directory: exact within its gaps)
+    uint32_t lo[MB_ITEMS], hi[MB_ITEMS];
+    if (k > 1) {
+        bool valid[MB_ITEMS];
+#pragma unroll
+        for (int it = 0; it < MB_ITEMS; ++it)
+            valid[it] = base + (uint64_t)it * MB_BLOCK + threadIdx.x < nl;
+        dev::lockstep_gap_bounds<MB_ITEMS>([&](uint32_t q) { return rk[rlo + q]; }, k, m, lkey,
+                                           valid, t0, t1, lo, hi);
+    } else {
+#pragma unroll
+        for (int it = 0; it < MB_ITEMS; ++it) {
+            lo[it] = t0[it];
+            hi[it] = t1[it];
+        }
+    }
     uint32_t cnt[MB_ITEMS];
 #pragma unroll
     for (int it = 0; it < MB_ITEMS; ++it) {
         const uint64_t i = base + (uint64_t)it * MB_BLOCK + threadIdx.x;
         uint32_t c = 0, low = 0;
         if (i < nl) {
-            const uint32_t key = lkey[it];
-            uint64_t lo = t0[it], hi = t1[it];
-            if (k > 1) {
-                // staged entries are every k-th key: the bounds lie within
-                // the k-1 keys before the directory entry found
-                const uint64_t d0 = t0[it], d1 = t1[it];
-                lo = dev::lower_bound(rk, rlo + (d0 ? (d0 - 1) * k + 1 : 0),
-                                      rlo + min(d0 * k, m), key) - rlo;
-                hi = dev::upper_bound(rk, rlo + (d1 ? (d1 - 1) * k + 1 : 0),
-                                      rlo + min(d1 * k, m), key) - rlo;
-            }
-            low = (uint32_t)(rlo + lo);
-            c = (uint32_t)(hi - lo);
+            low = (uint32_t)(rlo + lo[it]);
+            c = hi[it] - lo[it];
         }
         lower[row0 + (uint64_t)it * MB_BLOCK + threadIdx.x] = low;
         cnt[it] = c;

@@ -508,6 +508,64 @@ __device__ __forceinline__ void lockstep_last_le(const T *a, int n, F v, int (&k
     }
 }
 
+// Directory searches finished in global memory.  A range of m sorted keys
+// was staged as every k-th key (k > 1); for ITEMS keys, t0 / t1 are the
+// first staged entries >= / > the key.  The exact first position >= key
+// (lo) and > key (hi), relative to the range, lie within the k - 1 keys
+// before those entries: both searches of every item run branch by branch
+// in lockstep, so their 2 x ITEMS dependent loads per step overlap (a hot
+// key's gap holds thousands of keys).  at(q) reads key q of the range.
+// Items with valid[j] false are left at lo = hi = 0.  m < 2^32.
+template <int ITEMS, typename T, typename At>
+__device__ __forceinline__ void lockstep_gap_bounds(At at, uint64_t k, uint64_t m,
+                                                    const T (&key)[ITEMS],
+                                                    const bool (&valid)[ITEMS],
+                                                    const uint32_t (&t0)[ITEMS],
+                                                    const uint32_t (&t1)[ITEMS],
+                                                    uint32_t (&lo)[ITEMS],
+                                                    uint32_t (&hi)[ITEMS]) {
+    uint32_t ll[ITEMS], ul[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        const uint64_t d0 = t0[j], d1 = t1[j];
+        lo[j] = (uint32_t)(d0 ? (d0 - 1) * k + 1 : 0);
+        hi[j] = (uint32_t)(d1 ? (d1 - 1) * k + 1 : 0);
+        ll[j] = valid[j] ? (uint32_t)min(d0 * k, m) - lo[j] : 0u;
+        ul[j] = valid[j] ? (uint32_t)min(d1 * k, m) - hi[j] : 0u;
+        if (!valid[j]) lo[j] = hi[j] = 0;
+    }
+    // every step at least halves each remaining length (< k)
+    for (uint64_t left = k; left > 0; left >>= 1) {
+        T lv[ITEMS], uv[ITEMS];
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            lv[j] = ll[j] ? at(lo[j] + (ll[j] >> 1)) : T(0);
+            uv[j] = ul[j] ? at(hi[j] + (ul[j] >> 1)) : T(0);
+        }
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            if (ll[j]) {
+                const uint32_t h = ll[j] >> 1;
+                if (lv[j] < key[j]) {
+                    lo[j] += h + 1;
+                    ll[j] -= h + 1;
+                } else {
+                    ll[j] = h;
+                }
+            }
+            if (ul[j]) {
+                const uint32_t h = ul[j] >> 1;
+                if (!(key[j] < uv[j])) {
+                    hi[j] += h + 1;
+                    ul[j] -= h + 1;
+                } else {
+                    ul[j] = h;
+                }
+            }
+        }
+    }
+}
+
 // First index in [lo, hi) with a[i] >= v (a sorted ascending).
 template <typename T, typename V>
 __device__ __forceinline__ uint64_t lower_bound(const T *a, uint64_t lo,

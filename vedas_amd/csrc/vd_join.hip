@@ -59,28 +59,6 @@ struct KeyT<2> {
     }
 };
 
-template <int NK>
-__device__ __forceinline__ uint64_t g_lower(const KeyCols &k, uint64_t lo,
-                                            uint64_t hi,
-                                            typename KeyT<NK>::T v) {
-    while (lo < hi) {
-        uint64_t mid = lo + ((hi - lo) >> 1);
-        if (KeyT<NK>::at(k, mid) < v) lo = mid + 1; else hi = mid;
-    }
-    return lo;
-}
-
-template <int NK>
-__device__ __forceinline__ uint64_t g_upper(const KeyCols &k, uint64_t lo,
-                                            uint64_t hi,
-                                            typename KeyT<NK>::T v) {
-    while (lo < hi) {
-        uint64_t mid = lo + ((hi - lo) >> 1);
-        if (v < KeyT<NK>::at(k, mid)) hi = mid; else lo = mid + 1;
-    }
-    return lo;
-}
-
 // Right-row range [lo, hi) each bounds block can touch: two groups of 16
 // lanes per block, one for each end, all searches in parallel (a serial
 // search inside every block would put ~2 log2(|R|) dependent global loads
@@ -156,25 +134,31 @@ __global__ __launch_bounds__(JB_BLOCK) void k_join_bounds(
     // Every row's first staged entry >= key (t0) and > key (t1).
     uint32_t t0[JB_ITEMS], t1[JB_ITEMS];
     dev::lockstep_bounds<JB_ITEMS>(s_r, (uint32_t)nd, lkey, t0, t1);
+    // Bounds relative to rlo: the staged entries found, or with a directory
+    // (k > 1) the exact positions within the gaps before them.
+    uint32_t lo[JB_ITEMS], hi[JB_ITEMS];
+    if (k > 1) {
+        bool valid[JB_ITEMS];
+#pragma unroll
+        for (int j = 0; j < JB_ITEMS; ++j) valid[j] = base + (uint64_t)j * JB_BLOCK + threadIdx.x < nl;
+        dev::lockstep_gap_bounds<JB_ITEMS>(
+            [&](uint32_t q) { return KeyT<NK>::at(R, rlo + q); }, k, m, lkey, valid, t0, t1, lo,
+            hi);
+    } else {
+#pragma unroll
+        for (int j = 0; j < JB_ITEMS; ++j) {
+            lo[j] = t0[j];
+            hi[j] = t1[j];
+        }
+    }
     uint32_t cnt[JB_ITEMS];
 #pragma unroll
     for (int j = 0; j < JB_ITEMS; ++j) {
         const uint64_t i = base + (uint64_t)j * JB_BLOCK + threadIdx.x;
         uint32_t c = 0;
         if (i < nl) {
-            const T key = lkey[j];
-            uint64_t lo = t0[j], hi = t1[j];
-            if (k > 1) {
-                // staged entries are every k-th key: the bounds lie within
-                // the k-1 keys before the directory entry found
-                const uint64_t d0 = t0[j], d1 = t1[j];
-                lo = g_lower<NK>(R, rlo + (d0 ? (d0 - 1) * k + 1 : 0), rlo + min(d0 * k, m),
-                                 key) - rlo;
-                hi = g_upper<NK>(R, rlo + (d1 ? (d1 - 1) * k + 1 : 0), rlo + min(d1 * k, m),
-                                 key) - rlo;
-            }
-            lower[i] = (uint32_t)(rlo + lo);
-            c = (uint32_t)(hi - lo);
+            lower[i] = (uint32_t)(rlo + lo[j]);
+            c = (uint32_t)(hi[j] - lo[j]);
         }
         cnt[j] = c;
     }
