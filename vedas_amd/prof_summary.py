@@ -54,8 +54,9 @@ LABELS = [
     (r"k_hist_", "histogram"),
     (r"k_runs_expand\b", "runs_expand"),
     (r"k_mark_l1\b|k_mark_y\b|k_scatter_heads\b", "l2_heads"),
-    (r"k_pack_c01\b|k_pack_rowid\b|k_pack_multi\b", "sort_pack"),
-    (r"k_unpack_c01\b", "sort_unpack"),
+    (r"k_pack_rowid\b|k_pack_multi\b", "sort_pack"),
+    (r"k_swap_hist\b", "sort_hist"),
+    (r"k_repack\b", "index_repack"),
     (r"k_gather_perm\b|k_gather_sort_multi\b", "sort_gather"),
     (r"k_compose_perm\b", "sort_compose"),
     (r"k_max_col0_multi\b", "sort_max"),
