@@ -76,25 +76,37 @@ def x_passes(xbits: int) -> int:
     return n9 if xbits <= 27 and n9 < n8 else n8
 
 
+def index_bytes_full_sort(n: int, kbits: int, nperms: int, rec: int = 8) -> float:
+    """The scope table's unit for the index build, per permutation of n
+    records of `rec` bytes (the packed 64-bit key) sorted on a K-bit key in
+    8-bit digits: ceil(K/8) scatter passes reading and writing every record
+    plus one histogram read, B = ceil(K/8)*2*n*rec + n*rec.  It prices every
+    permutation as a full sort, whichever way it is built."""
+    return nperms * (-(-kbits // 8) * 2 * n * rec + n * rec)
+
+
 def index_bytes(n: int, bits, perms) -> float:
     """Algorithmic HBM bytes of the index build when the triple fits one
     64-bit key (bits = bit lengths of s, p, o).  A permutation sorted on all
     K bits packs the triple (12 B read, 8 B written) and runs ceil(K/8)
-    passes (8 B read, 8 B written); a derived one repacks its sibling's
-    sorted keys (8 B + 8 B) and runs the passes of its X bits.  Every
-    permutation writes Y and Z from its sorted keys (8 B + 8 B) and reads
-    the keys once more for its L1/L2 heads (8 B; the head entries written
-    are not counted)."""
+    passes (8 B read, 8 B written); a derived one runs the passes of its X
+    bits, the first reading its sibling's sorted keys, after one histogram
+    read (4 B) of each input column that is a derived X.  Every permutation
+    writes Y and Z from its sorted keys (8 B + 8 B) and reads the keys once
+    more for its L1/L2 heads (8 B; the head entries written are not
+    counted)."""
     kbits = sum(bits)
     assert kbits <= 64
     total = 0.0
+    xcols = set()
     for q, src in build_order(perms):
         if src is None:
             total += 20 * n + -(-kbits // 8) * 16 * n
         else:
-            total += 16 * n + x_passes(bits[PERM_COLS[q][0]]) * 16 * n
+            xcols.add(PERM_COLS[q][0])
+            total += x_passes(bits[PERM_COLS[q][0]]) * 16 * n
         total += 24 * n
-    return total
+    return total + 4 * n * len(xcols)
 
 
 _M1 = np.uint64(0xFF51AFD7ED558CCD)
@@ -486,6 +498,7 @@ def main():
     bits = (int(s.max()).bit_length(), int(p.max()).bit_length(), int(o.max()).bit_length())
     kb = {name: sum(bits) for name in perms}
     build_alg = index_bytes(n, bits, perms)
+    full_b = index_bytes_full_sort(n, sum(bits), len(perms))
     log(f"[bench] index build {build_s * 1e3:.1f} ms")
 
     idx_view = _native.Index.wrap(ctx, eng.index_handle(), n)
@@ -663,6 +676,13 @@ def main():
                         "order": [f"{q}<-{b}" if b else f"{q} (all bits)"
                                   for q, b in build_order(perms)],
                         "alg_note": "bytes of the derived-permutation build (index_bytes)",
+                        "full_sort_bytes": full_b, "full_sort_GB/s": full_b / build_s / 1e9,
+                        "full_sort_frac": full_b / build_s / HBM_PEAK,
+                        "one_touch_bytes": 2 * 8 * n * len(perms),
+                        "one_touch_frac": 2 * 8 * n * len(perms) / build_s / HBM_PEAK,
+                        "full_sort_note": "the scope table's unit: ceil(K/8)*2*N*R + N*R per "
+                                          "permutation, R = 8 B packed record, K = key bits; "
+                                          "one-touch = 2*N*R per permutation",
                         "kernels": kernel_table(build_prof)},
         "autotune": tune,
         "planned_queries": planned,

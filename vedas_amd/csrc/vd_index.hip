@@ -12,9 +12,11 @@
 //   5. k_compact_heads : L1 = (X, first row), L2 = (Y, first row) entries.
 // A permutation XYZ whose rows a sibling permutation already holds in an
 // order that, among rows of equal X, is (Y, Z) (Y before Z in the sibling's
-// key) is derived instead of sorted in full: its keys are repacked from the
-// sibling's sorted keys and stably sorted on X's bits only (LSD passes keep
-// the sibling's order among equal X).  Of SPO, PSO, POS and OSP only SPO is
+// key) is derived instead of sorted in full: its first radix pass reads
+// the sibling's sorted keys, moving their fields into the XYZ layout as it
+// loads them, and the passes sort on X's bits only (LSD passes keep the
+// sibling's order among equal X); X's digit histograms are counted from the
+// input column.  Of SPO, PSO, POS and OSP only SPO is
 // sorted on all bits; PSO and OSP follow from SPO and POS from OSP.
 // The resulting arrays match the reference's createIndex1Level /
 // createIndex output (VedasStorage.cu:270-358): L1 values/offsets are the
@@ -758,6 +760,19 @@ void build_packed(vd_ctx *ctx, vd_index *idx, uint32_t mask, const uint32_t *col
     DBuf<uint64_t> held[VD_NUM_PERMS];  // sorted keys kept as a sibling
     DBuf<uint64_t> spare;               // a free buffer of n keys
     DBuf<uint32_t> hist(ctx, swap::HBUF), gofs(ctx, swap::HBUF), dmax(ctx, 1);
+    // A column's digit histograms do not depend on the order of its rows:
+    // those of every derived permutation's X are counted once from the
+    // input columns (an upper bound of the maximum: the column's bit length).
+    DBuf<uint32_t> xhist[3], xmax[3];
+    for (int k = 0; k < nord; ++k) {
+        const int p = order[k], xi = PERM_COLS[p][0];
+        if (from[p] < 0 || xhist[xi].p) continue;
+        xhist[xi] = DBuf<uint32_t>(ctx, swap::HBUF);
+        xmax[xi] = DBuf<uint32_t>(ctx, 1);
+        fill_u32(ctx, xhist[xi].p, 0, swap::HBUF);
+        fill_u32(ctx, xmax[xi].p, pack_layout(p, bits).mask[xi], 1);
+        swap_hist(ctx, cols[xi], n, xhist[xi].p, xhist[xi].p + swap::H8, xmax[xi].p);
+    }
     for (int k = 0; k < nord; ++k) {
         const int p = order[k];
         const PackLayout to = pack_layout(p, bits);
@@ -774,20 +789,34 @@ void build_packed(vd_ctx *ctx, vd_index *idx, uint32_t mask, const uint32_t *col
                    to.sh[yi], to.sh[xi], keys.p, kbits, npass, h.p);
             sorted = radix_sort_u64(ctx, keys.p, tmp.p, n, 0, kbits, h.p, true);
         } else {
+            // The first pass reads the sibling's sorted keys and moves their
+            // fields into this permutation's layout as it loads them.
             const int q = from[p];
-            fill_u32(ctx, hist.p, 0, swap::HBUF);
-            fill_u32(ctx, dmax.p, to.mask[xi], 1);
-            launch(ctx, "index_repack", (double)n * 16.0, dim3(radix::hist_grid(pk_tiles)),
-                   dim3(PK_BLOCK), 0, k_repack, (const uint64_t *)held[q].p, n,
-                   pack_layout(q, bits), to, xi, keys.p, hist.p, hist.p + swap::H8,
-                   (const uint32_t *)dmax.p, ctx->radix_bits);
+            const PackLayout fl = pack_layout(q, bits);
             int shifts[4], dbits = 8;
             const uint32_t *g[4];
-            const int npass = swap_passes(ctx, hist.p, hist.p + swap::H8, dmax.p, n, gofs.p,
-                                          to.sh[xi], shifts, g, &dbits);
-            RadixCols none{};
-            sorted = radix_sort_shifts(ctx, keys.p, tmp.p, n, shifts, npass, g, 0, none, none,
-                                       none, nullptr, dbits);
+            const int npass = swap_passes(ctx, xhist[xi].p, xhist[xi].p + swap::H8, xmax[xi].p,
+                                          n, gofs.p, to.sh[xi], shifts, g, &dbits);
+            if (npass == 0 || n <= 1) {  // one X value: the sibling's order
+                fill_u32(ctx, hist.p, 0, swap::HBUF);
+                fill_u32(ctx, dmax.p, to.mask[xi], 1);
+                launch(ctx, "index_repack", (double)n * 16.0, dim3(radix::hist_grid(pk_tiles)),
+                       dim3(PK_BLOCK), 0, k_repack, (const uint64_t *)held[q].p, n, fl, to, xi,
+                       keys.p, hist.p, hist.p + swap::H8, (const uint32_t *)dmax.p,
+                       ctx->radix_bits);
+                sorted = keys.p;
+            } else {
+                RadixKeyIO kio{};
+                kio.in_k = held[q].p;
+                for (int c = 0; c < 3; ++c) {
+                    kio.rp.from_sh[c] = fl.sh[c];
+                    kio.rp.mask[c] = fl.mask[c];
+                    kio.rp.to_sh[c] = to.sh[c];
+                }
+                RadixCols none{};
+                sorted = radix_sort_shifts(ctx, keys.p, tmp.p, n, shifts, npass, g, 0, none,
+                                           none, none, nullptr, dbits, &kio);
+            }
             if (--uses[q] == 0) {
                 if (!spare.p) spare = std::move(held[q]);
                 held[q].reset();

@@ -288,9 +288,20 @@ uint64_t *radix_sort_u64_payload(vd_ctx *ctx, uint64_t *keys, uint64_t *tmp, uin
 // halves to (out_hi, out_lo) (out_lo null: the low half is dropped) instead
 // of a key buffer.  The columns written must not alias those read when a
 // single pass does both.
+// Field move of a packed key: field c (mask[c]) goes from bit from_sh[c]
+// to bit to_sh[c].
+struct KeyRepack {
+    int from_sh[3];
+    uint32_t mask[3];
+    int to_sh[3];
+};
 struct RadixKeyIO {
     const uint32_t *in_hi, *in_lo;
     uint32_t *out_hi, *out_lo;
+    // With in_k set (and in_hi not), the first pass reads its keys from
+    // in_k, each with its fields moved by rp, instead of the key buffer.
+    const uint64_t *in_k;
+    KeyRepack rp;
 };
 // Stable sort of n keys on the digits of digit_bits (8 or 9) bits at the
 // given shifts (LSD order; digits not listed must be equal in every key).
@@ -316,6 +327,11 @@ uint64_t *radix_sort_shifts(vd_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t
 int swap_passes(vd_ctx *ctx, const uint32_t *h8, const uint32_t *h9, const uint32_t *d_max,
                 uint64_t n, uint32_t *gofs, int lo_bit, int *shifts, const uint32_t **g,
                 int *digit_bits);
+// Histograms h8 / h9 (swap::HBUF words, zeroed by the caller) of the
+// digits of col[0, n) under the plan of *d_max (col's maximum or an upper
+// bound of it) and the radix_bits tunable, as swap_passes reads them.
+void swap_hist(vd_ctx *ctx, const uint32_t *col, uint64_t n, uint32_t *h8, uint32_t *h9,
+               const uint32_t *d_max);
 int radix_passes(int lo_bit, int hi_bit);
 // Device histogram buffer size (uint32 words) needed for radix_sort_u64.
 inline size_t radix_hist_words(int lo_bit, int hi_bit) {

@@ -126,7 +126,8 @@ __attribute__((amdgpu_waves_per_eu(OsShape<BLOCK, ITEMS, M>::WAVES_PER_EU, 8))) 
     // Load: position within the tile = (w * ITEMS + j) * 64 + lane, which is
     // also the ranking order below.  Missing keys are all-ones: they carry
     // the largest digit and rank after every real key of the tile.  Keys
-    // come from the key buffer or from two columns; a payload column
+    // come from the key buffer, from two columns, or from another key
+    // buffer with their fields moved; a payload column
     // without a source is the row id.
     uint64_t k[ITEMS];
     uint32_t v[M ? M : 1][ITEMS];
@@ -136,6 +137,18 @@ __attribute__((amdgpu_waves_per_eu(OsShape<BLOCK, ITEMS, M>::WAVES_PER_EU, 8))) 
             const uint64_t idx = base + (uint64_t)((w * ITEMS + j) * 64 + lane);
             k[j] = idx < n ? ((uint64_t)kio.in_hi[idx] << 32) | (kio.in_lo ? kio.in_lo[idx] : 0u)
                            : ~0ull;
+        }
+    } else if (kio.in_k) {
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            const uint64_t idx = base + (uint64_t)((w * ITEMS + j) * 64 + lane);
+            const uint64_t v = idx < n ? kio.in_k[idx] : 0ull;
+            uint64_t key = 0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                key |= (uint64_t)((uint32_t)(v >> kio.rp.from_sh[c]) & kio.rp.mask[c])
+                       << kio.rp.to_sh[c];
+            k[j] = idx < n ? key : ~0ull;
         }
     } else {
 #pragma unroll
@@ -329,6 +342,8 @@ void run_passes(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, const RadixCols &src,
         if (kio && p == 0) {
             io.in_hi = kio->in_hi;
             io.in_lo = kio->in_lo;
+            io.in_k = kio->in_k;
+            io.rp = kio->rp;
             if (io.in_hi) kbytes -= io.in_lo ? 0.0 : 4.0;
         }
         if (kio && p == npass - 1) {

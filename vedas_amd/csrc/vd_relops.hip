@@ -208,9 +208,7 @@ void sort_col0(vd_ctx *ctx, vd_rel *rel) {
     fill_u32(ctx, hist.p, 0, swap::HBUF);
     fill_u32(ctx, dmax.p, 0, 1);
     max_u32_dev(ctx, rel->cols[0], n, dmax.p);
-    launch(ctx, "sort_hist", (double)n * 4.0, dim3(radix::hist_grid(ceil_div(n, RB * R_ITEMS))),
-           dim3(RB), 0, k_swap_hist, (const uint32_t *)rel->cols[0], n, h8, h9,
-           (const uint32_t *)dmax.p, ctx->radix_bits);
+    swap_hist(ctx, rel->cols[0], n, h8, h9, dmax.p);
     int shifts[4], dbits = 8;
     const uint32_t *g[4];
     const int npass = swap_passes(ctx, h8, h9, dmax.p, n, gofs.p, 32, shifts, g, &dbits);
@@ -355,6 +353,12 @@ void unique_adjacent(vd_ctx *ctx, vd_rel *rel) {
 }
 
 }  // namespace
+
+void swap_hist(vd_ctx *ctx, const uint32_t *col, uint64_t n, uint32_t *h8, uint32_t *h9,
+               const uint32_t *d_max) {
+    launch(ctx, "sort_hist", (double)n * 4.0, dim3(radix::hist_grid(ceil_div(n, RB * R_ITEMS))),
+           dim3(RB), 0, k_swap_hist, col, n, h8, h9, d_max, ctx->radix_bits);
+}
 
 int swap_passes(vd_ctx *ctx, const uint32_t *h8, const uint32_t *h9, const uint32_t *d_max,
                 uint64_t n, uint32_t *gofs, int lo_bit, int *shifts, const uint32_t **g,
