@@ -452,6 +452,33 @@ def test_join_empty_sides(ctx):
         r.free()
 
 
+@pytest.mark.parametrize("ncols,differ", [(5, 4), (7, 6), (9, 1), (3, 2)])
+def test_unique_rows_differing_late(ctx, ncols, differ):
+    """Adjacent unique where runs of rows agree on the first `differ`
+    columns and differ (or not) only in later ones, across wave and tile
+    boundaries (the previous row's value crosses lanes, waves and tiles)."""
+    from vedas_amd._native import Rel
+    rng = np.random.default_rng(ncols * 10 + differ)
+    n = 70_001
+    run = rng.integers(1, 300, n).cumsum()
+    head = np.searchsorted(run, np.arange(n), side="right").astype(np.uint32)
+    cols = [head.copy() for _ in range(differ)]
+    cols += [rng.integers(0, 3, n).astype(np.uint32) for _ in range(ncols - differ)]
+    for j in (63, 64, 2047, 2048, 4095):  # equal rows straddling boundaries
+        for c in cols:
+            c[j] = c[j - 1]
+    cols = O.sort_lex(cols)
+    want = O.unique_adjacent(cols)
+    r = Rel.from_columns(ctx, cols)
+    try:
+        r.unique_adjacent()
+        assert r.nrows == len(want[0]) < n
+        for g, w in zip(r.columns(), want):
+            assert np.array_equal(g, w)
+    finally:
+        r.free()
+
+
 @pytest.mark.parametrize("n,ncols,keymax", [(0, 2, 5), (1, 3, 5), (1000, 1, 50), (70_001, 3, 1000),
                                             (300_000, 5, 2**31), (4096, 2, 1),
                                             (100_000, 2, 2**32 - 1), (50_000, 4, 2**20),
