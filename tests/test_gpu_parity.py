@@ -621,6 +621,64 @@ def test_index_build_digit_widths(ctx, bits):
         ctx.tune("radix_bits", 0)
 
 
+def test_value_bounds_drive_index_swaps(ctx):
+    """Selects carry the index's column maxima as value bounds, joins pass
+    them on, and an IndexSwap on a bounded column 0 takes its digit plan
+    from the bound (no maximum kernel, no read-back): with a bound far
+    above the selected values the top digits do not vary, and the sort
+    must still equal the oracle's.  A column handed out writable loses its
+    bound (the maximum is taken again)."""
+    from vedas_amd._native import Index, Rel
+    rng = np.random.default_rng(9)
+    n = 60_000
+    s = rng.integers(1, 400, n).astype(np.uint32)
+    p = rng.integers(1, 6, n).astype(np.uint32)
+    o = rng.integers(1, 3000, n).astype(np.uint32)
+    o[7] = (1 << 27) - 5  # one large object: the O bound needs 27 bits
+    s[9] = (1 << 24) - 3  # one large subject: the S bound needs 24 bits
+    idx = Index(ctx, s, p, o, ["PSO", "POS"])
+    ix = {q: O.build_perm(s, p, o, q) for q in ["PSO", "POS"]}
+    ctx.prof(True)
+    try:
+        for pred in (1, 2, 3):
+            b, e = O.lookup(ix["PSO"], pred)
+            want = O.select(ix["PSO"], b, e, 3)            # [S, O] of predicate pred
+            r = idx.select("PSO", b, e, 3)
+            r.swap_cols(0, 1)                               # O first: bound 2^27 - 5
+            ctx.prof_reset()
+            r.sort_col0()
+            st = ctx.prof_stats()
+            assert "max_u32" not in st and st["radix_onesweep"]["launches"] == 3
+            for g, w in zip(r.columns(), O.sort_col0([want[1], want[0]])):
+                assert np.array_equal(g, w)
+            r.free()
+            # a join of two selects on S, then a swap to the right side's O
+            b2, e2 = O.lookup(ix["PSO"], pred + 1)
+            want2 = O.select(ix["PSO"], b2, e2, 3)
+            jl, jr = idx.select("PSO", b, e, 3), idx.select("PSO", b2, e2, 3)
+            out = jl.join(jr, 1)                            # [S, O_l, O_r]
+            wj = O.join(want, want2, 1)
+            out.swap_cols(0, 2)
+            ctx.prof_reset()
+            out.sort_col0()
+            assert "max_u32" not in ctx.prof_stats()
+            got = out.columns()
+            for g, w in zip(got, O.sort_col0([wj[2], wj[1], wj[0]])):
+                assert np.array_equal(g, w)
+            # columns() handed out pointers: the next swap takes the maximum
+            out.swap_cols(0, 1)
+            ctx.prof_reset()
+            out.sort_col0()
+            assert ctx.prof_stats()["max_u32"]["launches"] == 1
+            for g, w in zip(out.columns(), O.sort_col0([got[1], got[0], got[2]])):
+                assert np.array_equal(g, w)
+            for x in (jl, jr, out):
+                x.free()
+    finally:
+        ctx.prof(False)
+        idx.free()
+
+
 @pytest.mark.parametrize("items", [8, 24])
 def test_index_build_tile_shapes(ctx, items):
     ctx.tune("radix_items", items)

@@ -556,6 +556,10 @@ void select_multi(vd_ctx *ctx, const vd_index *idx, const vd_select_job_t *jobs,
             if (jobs[j].cols == 3) vc[nc++] = pa.y + b;
             vc[nc++] = pa.z + b;
             outs[j] = rel_view(nc, e - b, vc, idx->hold);
+            if (pa.bounded) {
+                if (jobs[j].cols == 3) rel_set_bound(outs[j], 0, pa.ymax);
+                rel_set_bound(outs[j], nc - 1, pa.zmax);
+            }
         }
     } catch (...) {
         for (int j = 0; j < nj; ++j) rel_free(ctx, outs[j]), outs[j] = nullptr;
@@ -629,7 +633,11 @@ void join_multi_chunk(vd_ctx *ctx, const vd_rel *const *L, const vd_rel *const *
     try {
         for (int j = 0; j < np; ++j) {
             outs[j] = rel_new(ctx, L[j]->ncols + R[j]->ncols - 1, tot[j]);
-            for (int c = 0; c < outs[j]->ncols; ++c) hp[j].oc[c] = outs[j]->cols[c];
+            for (int c = 0; c < outs[j]->ncols; ++c) {
+                hp[j].oc[c] = outs[j]->cols[c];
+                const bool from_l = c < L[j]->ncols;
+                rel_copy_bound(outs[j], c, from_l ? L[j] : R[j], from_l ? c : c - L[j]->ncols + 1);
+            }
             hp[j].obase = total;  // outputs of the pairs are laid end to end
             total += tot[j];
         }

@@ -428,6 +428,7 @@ uint64_t vd_rel_nrows(const vd_rel *rel) { return rel ? rel->nrows : 0; }
 
 uint32_t *vd_rel_col(vd_rel *rel, int c) {
     if (!rel || c < 0 || c >= rel->ncols) return nullptr;
+    rel->bounded &= ~(1u << c);  // the caller may write through the pointer
     return rel->cols[c];
 }
 
@@ -435,6 +436,10 @@ int vd_rel_swap_cols(vd_rel *rel, int i, int j) {
     if (!rel || i < 0 || j < 0 || i >= rel->ncols || j >= rel->ncols)
         return VD_EINVAL;
     std::swap(rel->cols[i], rel->cols[j]);
+    std::swap(rel->colmax[i], rel->colmax[j]);
+    const uint32_t mi = (rel->bounded >> i) & 1u, mj = (rel->bounded >> j) & 1u;
+    rel->bounded &= ~((1u << i) | (1u << j));
+    rel->bounded |= (mi << j) | (mj << i);
     if (i == 0 || j == 0) rel->key_range = false;
     const uint32_t bi = (rel->borrowed >> i) & 1u, bj = (rel->borrowed >> j) & 1u;
     rel->borrowed &= ~((1u << i) | (1u << j));
@@ -447,8 +452,12 @@ int vd_rel_drop_col(vd_ctx *ctx, vd_rel *rel, int i) {
         return VD_EINVAL;
     if (!vd::rel_borrowed(rel, i)) vd::dfree(ctx, rel->cols[i]);
     if (i == 0) rel->key_range = false;
-    for (int c = i; c + 1 < rel->ncols; ++c) rel->cols[c] = rel->cols[c + 1];
+    for (int c = i; c + 1 < rel->ncols; ++c) {
+        rel->cols[c] = rel->cols[c + 1];
+        rel->colmax[c] = rel->colmax[c + 1];
+    }
     rel->cols[rel->ncols - 1] = nullptr;
+    rel->bounded = (rel->bounded & ((1u << i) - 1u)) | ((rel->bounded >> (i + 1)) << i);
     const uint32_t low = rel->borrowed & ((1u << i) - 1u);
     rel->borrowed = low | ((rel->borrowed >> (i + 1)) << i);
     rel->ncols -= 1;

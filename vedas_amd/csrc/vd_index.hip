@@ -928,6 +928,14 @@ SkipSet check_select(int cols, int clip, const vd_interval_t *skip, int nskip, i
     return sk;
 }
 
+// Value bounds of a select's columns (Y and / or Z by `cols`).
+void select_bounds(vd_rel *r, const vd_perm_arrays &pa, int cols) {
+    if (!pa.bounded) return;
+    int c = 0;
+    if (cols & 2) rel_set_bound(r, c++, pa.ymax);
+    if (cols & 1) rel_set_bound(r, c++, pa.zmax);
+}
+
 // Rows [src, src + rows) of the permutation's columns as a view.
 vd_rel *range_view(const vd_index *idx, const vd_perm_arrays &pa, int cols, uint64_t src,
                    uint64_t rows) {
@@ -935,7 +943,9 @@ vd_rel *range_view(const vd_index *idx, const vd_perm_arrays &pa, int cols, uint
     int nc = 0;
     if (cols & 2) vc[nc++] = pa.y + src;
     if (cols & 1) vc[nc++] = pa.z + src;
-    return rel_view(nc, rows, vc, idx->hold);
+    vd_rel *r = rel_view(nc, rows, vc, idx->hold);
+    select_bounds(r, pa, cols);
+    return r;
 }
 
 // The relation of computed pieces: one run of rows is a view, several are
@@ -947,6 +957,7 @@ vd_rel *select_pieces_rel(vd_ctx *ctx, const vd_index *idx, const vd_perm_arrays
         return range_view(idx, pa, cols, hp.np == 1 ? hp.src[0] : 0, rows);
     const int ocols = cols == 3 ? 2 : 1;
     vd_rel *r = rel_new(ctx, ocols, rows);
+    select_bounds(r, pa, cols);
     try {
         launch(ctx, "select_copy", (double)rows * 8.0 * ocols, dim3(stream_grid(rows)),
                dim3(256), 0, k_copy_pieces, dp, (const uint32_t *)pa.y, (const uint32_t *)pa.z,
@@ -1037,6 +1048,7 @@ int vd_index_build(vd_ctx *ctx, const uint32_t *d_s, const uint32_t *d_p,
     *out = nullptr;
     vd_index *idx = new (std::nothrow) vd_index();
     if (!idx) return VD_ENOMEM;
+    uint32_t maxv[3] = {0, 0, 0};  // the columns' maxima
     try {
         VD_REQUIRE(perm_mask != 0 && perm_mask < (1u << VD_NUM_PERMS), VD_EINVAL,
                    "empty or invalid permutation mask");
@@ -1049,7 +1061,10 @@ int vd_index_build(vd_ctx *ctx, const uint32_t *d_s, const uint32_t *d_p,
         const uint32_t *cols[3] = {d_s, d_p, d_o};
         int bits[3] = {1, 1, 1};
         if (n > 0)
-            for (int c = 0; c < 3; ++c) bits[c] = std::max(1, bits_for(max_u32(ctx, cols[c], n)));
+            for (int c = 0; c < 3; ++c) {
+                maxv[c] = max_u32(ctx, cols[c], n);
+                bits[c] = std::max(1, bits_for(maxv[c]));
+            }
         if (n > 0 && bits[0] + bits[1] + bits[2] <= 64) {
             build_packed(ctx, idx, perm_mask, cols, bits);
         } else {
@@ -1080,6 +1095,13 @@ int vd_index_build(vd_ctx *ctx, const uint32_t *d_s, const uint32_t *d_p,
         for (auto &pa : idx->perm) free_perm(ctx, pa);
         delete idx;
         return VD_EINVAL;
+    }
+    for (int p = 0; p < VD_NUM_PERMS; ++p) {
+        vd_perm_arrays &pa = idx->perm[p];
+        if (!pa.built) continue;
+        pa.bounded = true;
+        pa.ymax = maxv[PERM_COLS[p][1]];
+        pa.zmax = maxv[PERM_COLS[p][2]];
     }
     try {
         idx->hold = make_hold(ctx, idx->perm);

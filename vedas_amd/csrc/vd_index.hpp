@@ -8,6 +8,9 @@ struct vd_perm_arrays {
     uint64_t n = 0, n_l1 = 0, n_l2 = 0;
     uint32_t *l1v = nullptr, *l1o = nullptr, *l2v = nullptr, *l2o = nullptr;
     uint32_t *y = nullptr, *z = nullptr;
+    // Maxima of the Y and Z columns (the input columns' maxima), when known.
+    bool bounded = false;
+    uint32_t ymax = 0, zmax = 0;
 };
 
 struct vd_index {

@@ -127,6 +127,13 @@ struct vd_rel {
     // relation knew them (a join); cleared when column 0 changes.
     bool key_range = false;
     uint32_t key_lo = 0, key_hi = 0;
+    // Value bounds: with bit c of `bounded` set, every value of column c is
+    // <= colmax[c].  Selects take them from the index's column maxima and
+    // joins from their inputs; they follow columns through swaps, drops and
+    // the operators that only reorder or drop rows, and a column handed out
+    // writable (vd_rel_col) loses its bound.
+    uint32_t bounded = 0;
+    uint32_t colmax[VD_MAX_COLS] = {};
 };
 
 namespace vd {
@@ -327,6 +334,11 @@ uint64_t *radix_sort_shifts(vd_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t
 int swap_passes(vd_ctx *ctx, const uint32_t *h8, const uint32_t *h9, const uint32_t *d_max,
                 uint64_t n, uint32_t *gofs, int lo_bit, int *shifts, const uint32_t **g,
                 int *digit_bits);
+// swap_passes for a value bound mx (the value stored at d_max): the scans
+// run without a read-back and every digit of mx's plan gets a pass.
+int swap_passes_bounded(vd_ctx *ctx, const uint32_t *h8, const uint32_t *h9,
+                        const uint32_t *d_max, uint32_t mx, uint64_t n, uint32_t *gofs,
+                        int lo_bit, int *shifts, const uint32_t **g, int *digit_bits);
 // Histograms h8 / h9 (swap::HBUF words, zeroed by the caller) of the
 // digits of col[0, n) under the plan of *d_max (col's maximum or an upper
 // bound of it) and the radix_bits tunable, as swap_passes reads them.
@@ -355,6 +367,17 @@ void rel_free(vd_ctx *ctx, vd_rel *r);
 // freeing the old column if the relation owned it.
 void rel_set_col(vd_ctx *ctx, vd_rel *r, int c, uint32_t *fresh);
 inline bool rel_borrowed(const vd_rel *r, int c) { return (r->borrowed >> c) & 1u; }
+inline void rel_set_bound(vd_rel *r, int c, uint32_t mx) {
+    r->bounded |= 1u << c;
+    r->colmax[c] = mx;
+}
+// Column d of `to` takes the bound (or its absence) of column s of `from`.
+inline void rel_copy_bound(vd_rel *to, int d, const vd_rel *from, int s) {
+    if ((from->bounded >> s) & 1u)
+        rel_set_bound(to, d, from->colmax[s]);
+    else
+        to->bounded &= ~(1u << d);
+}
 
 }  // namespace vd
 

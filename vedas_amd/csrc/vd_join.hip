@@ -392,6 +392,9 @@ vd_rel *join_impl(vd_ctx *ctx, const vd_rel *L, const vd_rel *R) {
                "join: 2048 consecutive left rows match more than 2^32-1 right rows");
 
     vd_rel *out = rel_new(ctx, out_cols, total);
+    // columns [L's, R's after the keys]: their values come from those columns
+    for (int c = 0; c < out_cols; ++c)
+        rel_copy_bound(out, c, c < L->ncols ? L : R, c < L->ncols ? c : c - L->ncols + NK);
     out->key_range = true;  // output column 0 = left keys of matched rows
     out->key_lo = (uint32_t)h[1];
     out->key_hi = (uint32_t)h[2];

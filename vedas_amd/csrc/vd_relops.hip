@@ -87,13 +87,14 @@ __global__ __launch_bounds__(RB) void k_swap_hist(const uint32_t *__restrict__ c
 }
 
 // One block: the exclusive scans of the digits the producer counted (the
-// plan of the maximum *gmax) into gofs8 or gofs9, then a read-back of {bit
-// d set: digit d varies (no bin holds all n rows), the maximum}.
+// plan of the maximum *gmax) into gofs8 or gofs9, then (publish) a
+// read-back of {bit d set: digit d varies (no bin holds all n rows), the
+// maximum}.
 constexpr int SP_BLOCK = 512;
 __global__ __launch_bounds__(SP_BLOCK) void k_swap_plan_scan(
     const uint32_t *__restrict__ g8, const uint32_t *__restrict__ g9, const uint32_t *gmax,
     uint64_t n, uint32_t *__restrict__ gofs8, uint32_t *__restrict__ gofs9, Readback rb,
-    int bits) {
+    int bits, int publish) {
     __shared__ uint32_t s_tmp[SP_BLOCK / 64];
     const uint32_t mx = *gmax;
     const swap::Plan plan = swap::plan_for(mx, bits);
@@ -110,7 +111,7 @@ __global__ __launch_bounds__(SP_BLOCK) void k_swap_plan_scan(
         const int one_bin = __syncthreads_or(tid < bins && (uint64_t)v == n);
         varies |= (one_bin ? 0ull : 1ull) << d;
     }
-    if (tid < 64) {
+    if (publish && tid < 64) {
         if (tid == 0) {
             rb.dst[0] = varies;
             rb.dst[1] = mx;
@@ -206,12 +207,22 @@ void sort_col0(vd_ctx *ctx, vd_rel *rel) {
     DBuf<uint32_t> hist(ctx, swap::HBUF), gofs(ctx, swap::HBUF), dmax(ctx, 1);
     uint32_t *h8 = hist.p, *h9 = hist.p + swap::H8;
     fill_u32(ctx, hist.p, 0, swap::HBUF);
-    fill_u32(ctx, dmax.p, 0, 1);
-    max_u32_dev(ctx, rel->cols[0], n, dmax.p);
-    swap_hist(ctx, rel->cols[0], n, h8, h9, dmax.p);
-    int shifts[4], dbits = 8;
+    int shifts[4], dbits = 8, npass;
     const uint32_t *g[4];
-    const int npass = swap_passes(ctx, h8, h9, dmax.p, n, gofs.p, 32, shifts, g, &dbits);
+    if (rel->bounded & 1u) {
+        // The value bound fixes the plan: no maximum and no read-back; every
+        // digit below its bit length gets a pass (one that no row changes
+        // leaves the order as it is).
+        fill_u32(ctx, dmax.p, rel->colmax[0], 1);
+        swap_hist(ctx, rel->cols[0], n, h8, h9, dmax.p);
+        npass = swap_passes_bounded(ctx, h8, h9, dmax.p, rel->colmax[0], n, gofs.p, 32, shifts,
+                                    g, &dbits);
+    } else {
+        fill_u32(ctx, dmax.p, 0, 1);
+        max_u32_dev(ctx, rel->cols[0], n, dmax.p);
+        swap_hist(ctx, rel->cols[0], n, h8, h9, dmax.p);
+        npass = swap_passes(ctx, h8, h9, dmax.p, n, gofs.p, 32, shifts, g, &dbits);
+    }
     if (npass == 0) return;  // column 0 is constant: already in order
 
     // Key buffers between passes: none for one pass, one for two.
@@ -354,6 +365,21 @@ void unique_adjacent(vd_ctx *ctx, vd_rel *rel) {
 
 }  // namespace
 
+int swap_passes_bounded(vd_ctx *ctx, const uint32_t *h8, const uint32_t *h9,
+                        const uint32_t *d_max, uint32_t mx, uint64_t n, uint32_t *gofs,
+                        int lo_bit, int *shifts, const uint32_t **g, int *digit_bits) {
+    launch(ctx, "radix_scan_hist", 0.0, dim3(1), dim3(SP_BLOCK), 0, k_swap_plan_scan, h8, h9,
+           d_max, n, gofs, gofs + swap::H8, Readback{}, ctx->radix_bits, 0);
+    const swap::Plan plan = swap::plan_for(mx, ctx->radix_bits);
+    const int W = plan.use9 ? 9 : 8;
+    *digit_bits = W;
+    for (int d = 0; d < plan.ndig; ++d) {
+        shifts[d] = lo_bit + W * d;
+        g[d] = plan.use9 ? gofs + swap::H8 + d * 512 : gofs + d * 256;
+    }
+    return plan.ndig;
+}
+
 void swap_hist(vd_ctx *ctx, const uint32_t *col, uint64_t n, uint32_t *h8, uint32_t *h9,
                const uint32_t *d_max) {
     launch(ctx, "sort_hist", (double)n * 4.0, dim3(radix::hist_grid(ceil_div(n, RB * R_ITEMS))),
@@ -365,7 +391,7 @@ int swap_passes(vd_ctx *ctx, const uint32_t *h8, const uint32_t *h9, const uint3
                 int *digit_bits) {
     const Readback rb = rb_next(ctx);
     launch(ctx, "radix_scan_hist", 0.0, dim3(1), dim3(SP_BLOCK), 0, k_swap_plan_scan, h8, h9,
-           d_max, n, gofs, gofs + swap::H8, rb, ctx->radix_bits);
+           d_max, n, gofs, gofs + swap::H8, rb, ctx->radix_bits, 1);
     uint64_t hv[2];
     rb_wait(ctx, rb, hv, 2);
     // The digits of the plan that vary (the producer counted only the
