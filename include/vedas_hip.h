@@ -169,7 +169,11 @@ int  vd_index_lookup_batch(vd_ctx *ctx, const vd_index *idx, int perm,
 /* ---- relations (FullRelationIR) ---------------------------------------- */
 /* Replaces FullRelationIR(columnNum, relationSize) and its column accessors
  * (FullRelationIR.cu:19-41, 524-592, 715-745).  Columns are separate device
- * arrays of nrows uint32; column 0 is the sort key. */
+ * arrays of nrows uint32; column 0 is the sort key.  Relations produced by
+ * vd_select / vd_select_multi / vd_join / vd_join_multi carry an upper bound
+ * of each column's values (from the index's column maxima), which
+ * vd_sort_col0 uses as its digit plan instead of measuring the maximum;
+ * vd_rel_col hands the column out writable and drops that bound. */
 int  vd_rel_create(vd_ctx *ctx, int ncols, uint64_t nrows, vd_rel **out);
 int  vd_rel_destroy(vd_ctx *ctx, vd_rel *rel);
 int  vd_rel_ncols(const vd_rel *rel);
