@@ -529,6 +529,7 @@ def main():
         tune = eng.autotune([texts[q] for q in names], reps=2)
         log(f"[bench] autotune: radix_items {tune['radix_items']} join_nv {tune['join_nv']} "
             f"join_rcap {tune['join_rcap']} radix_bits {tune['radix_bits']} "
+            f"radix_group {tune['radix_group']} "
             f"({tune['base_ms']:.2f} -> {tune['tuned_ms']:.2f} ms)")
     for w in range(args.warmup):
         log(f"[bench] warmup {w}")

@@ -81,7 +81,9 @@ int  vd_ctx_mem_peak(vd_ctx *ctx, uint64_t *peak, int reset);
  * threads x 16 keys, the default; 8 = 512 x 8; 24 = 256 x 24, keys only),
  * "radix_bits" (digit width of the IndexSwap re-sort: 0 = 9-bit digits when
  * they save a pass, else 8-bit, the default; 8 = always 8-bit; 9 = 9-bit
- * whenever the key fits 27 bits), "join_nv" (merge-path diagonal per join
+ * whenever the key fits 27 bits), "radix_group" (radix tiles an XCD takes
+ * as one group of consecutive tiles: 8, the default, or 4, 16, 32; 0 = one
+ * global tile order), "join_nv" (merge-path diagonal per join
  * expansion block), "join_rcap" (right keys staged in LDS per bounds
  * block).  Returns VD_EINVAL for an unknown key or an unsupported value. */
 int  vd_ctx_tune(vd_ctx *ctx, const char *key, int value);
@@ -258,6 +260,26 @@ int  vd_select_batch_noalloc(vd_ctx *ctx, const vd_index *idx, int perm,
  * Input relations are not modified. */
 int  vd_join(vd_ctx *ctx, const vd_rel *L, const vd_rel *R, int nkeys,
              vd_rel **out);
+
+/* Join driven by a relation in output order (no single counterpart in the
+ * reference: it yields what JoinQueryJob::join followed by
+ * IndexSwapJob::startJob, JoinQueryJob.cu:262-380 + IndexSwapJob.cu:28-95,
+ * yield when D is the re-indexed copy of one input, e.g. the POS range of a
+ * pattern whose PSO range was selected).  For every row of D, in D's
+ * order, the rows of Y whose column 0 equals D's column dkey, in Y's order.
+ * Y is sorted ascending on column 0; D may be in any order.  D rows whose
+ * key is outside [key_lo, key_hi] match nothing.  Output column c is Y's
+ * column src[c] (src[c] >= 0) or D's column -1 - src[c].  *match_lo /
+ * *match_hi receive the smallest and largest key that matched.
+ * Returns VD_OK with *out == NULL when the driven form does not apply: the
+ * key values (bounded by key_hi and the columns' value bounds) exceed
+ * 2^28 - 1 (the run table of Y holds 8 bytes per key value), or, with
+ * check_dups, two adjacent equal D rows match more than one row each (the
+ * join-then-sort order would interleave them differently). */
+int  vd_join_driven(vd_ctx *ctx, const vd_rel *D, int dkey, const vd_rel *Y,
+                    uint32_t key_lo, uint32_t key_hi, int nout, const int *src,
+                    int check_dups, vd_rel **out, uint32_t *match_lo,
+                    uint32_t *match_hi);
 
 /* ---- re-sort and finalise (FullRelationIR / IndexSwapJob / QueryPlan) --- */
 /* Stable sort of all rows by column 0 only.  Replaces

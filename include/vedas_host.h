@@ -37,6 +37,8 @@ typedef struct {
     double elapsed_ms;      /* host wall time of the query              */
     double join_ms, select_ms, swap_ms;
     int jobs;               /* operators executed                       */
+    int driven_joins;       /* joins that delivered their IndexSwap's order
+                               (those swaps re-sort nothing)             */
 } vdq_stats_t;
 
 int vdq_engine_create(int device, vdq_engine **out);
@@ -105,7 +107,8 @@ int vdq_plan(const char *text, const uint64_t *est_rows, char *out, size_t cap);
  * fixed by ModernGPU and Thrust): times the given queries under each
  * candidate of the kernel tunables -- radix tile shape ("radix_items"),
  * merge-path diagonal ("join_nv"), LDS right-key capacity ("join_rcap"),
- * radix digit width of the re-sorts ("radix_bits"), see vd_ctx_tune -- by
+ * radix digit width of the re-sorts ("radix_bits"), radix tiles per XCD
+ * group ("radix_group"), see vd_ctx_tune -- by
  * coordinate descent, keeps the fastest and applies it to the engine's
  * contexts (serving slots included).  Results do not depend on the
  * tunables. */
@@ -115,6 +118,7 @@ typedef struct {
     double base_ms;                       /* queries with the defaults    */
     double tuned_ms;                      /* queries with the chosen set  */
     int evaluations;
+    int radix_group;                      /* radix tiles per XCD group (0 = one order) */
 } vdq_tune_t;
 int vdq_autotune(vdq_engine *e, const char *const *texts, int n, int reps, vdq_tune_t *out);
 

@@ -215,6 +215,7 @@ def test_autotune_keeps_results(small):
     assert tr["join_nv"] in (1024, 2048, 4096)
     assert tr["join_rcap"] in (2048, 4096, 8192)
     assert tr["radix_bits"] in (0, 8, 9)
+    assert tr["radix_group"] in (0, 8, 16)
     assert tr["evaluations"] >= 9 and tr["tuned_ms"] <= tr["base_ms"] + 1e-9
     for t, want in zip(texts, before):
         r = e.query(t)

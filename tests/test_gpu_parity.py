@@ -539,6 +539,35 @@ def test_sort_col0_every_tile_shape(ctx, items, ncols):
         ctx.tune("radix_items", 16)
 
 
+@pytest.mark.parametrize("group", [0, 4, 16, 32])
+def test_radix_tile_groups(ctx, group):
+    """Tiles taken per XCD in groups of consecutive tiles (8 counters, a grid
+    rounded up to whole groups per XCD, blocks past the last tile exiting):
+    IndexSwap sorts with 0-3 payload columns and an index build, at sizes
+    that end inside a group, on a group boundary and past a tile, against
+    the oracle.  The default (8) runs in every other test."""
+    from vedas_amd._native import Rel
+    ctx.tune("radix_group", group)
+    try:
+        for n, ncols in ((1, 2), (6145, 4), (8192 * 8 * 3 + 1, 2), (300_007, 3),
+                         (500_001, 5), (6144 * 37 + 5, 4)):
+            print(f"radix_group={group} n={n} ncols={ncols}", flush=True)
+            rng = np.random.default_rng(n + group)
+            cols = [rng.integers(0, 1 << 27, n, dtype=np.uint64).astype(np.uint32)]
+            cols[0][::3] = 77  # duplicate keys: stability shows
+            cols += [rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+                     for _ in range(ncols - 1)]
+            r = Rel.from_columns(ctx, cols)
+            r.sort_col0()
+            for g, w in zip(r.columns(), O.sort_col0(cols)):
+                assert np.array_equal(g, w)
+            r.free()
+        s, p, o = rand_triples(250_003, 40 + group)
+        check_index(ctx, s, p, o, ["SPO", "PSO", "OSP"])
+    finally:
+        ctx.tune("radix_group", 8)
+
+
 def test_sort_col0_views_any_alignment(ctx):
     """IndexSwap sorts of unfiltered selects, which read the permutation in
     place: column 0 starts at every offset modulo 16 bytes (the column

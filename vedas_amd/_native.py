@@ -110,6 +110,10 @@ EXPORTS = {
                                           C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "vd_join": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                           C.POINTER(C.c_void_p)]),
+    "vd_join_driven": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_uint32,
+                                 C.c_uint32, C.c_int, C.POINTER(C.c_int), C.c_int,
+                                 C.POINTER(C.c_void_p), C.POINTER(C.c_uint32),
+                                 C.POINTER(C.c_uint32)]),
     "vd_select_multi": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                   C.POINTER(C.c_void_p)]),
     "vd_index_lookup_multi": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
@@ -345,6 +349,23 @@ class Rel:
         h = C.c_void_p()
         self.ctx.check(self.ctx.L.vd_join(self.ctx.h, self.h, other.h, nkeys, C.byref(h)))
         return Rel(self.ctx, h)
+
+    def join_driven(self, dkey: int, other: "Rel", src, key_lo: int = 0,
+                    key_hi: int = 0xFFFFFFFF, check_dups: bool = False):
+        """For every row of self (the driver), in order, the rows of `other`
+        (sorted on column 0) whose key equals self's column dkey.  Output
+        column c is other's column src[c] (>= 0) or self's column -1 - src[c].
+        Returns (Rel, smallest matched key, largest matched key), or None
+        when the driven form does not apply."""
+        h = C.c_void_p()
+        arr = (C.c_int * len(src))(*src)
+        lo, hi = C.c_uint32(), C.c_uint32()
+        self.ctx.check(self.ctx.L.vd_join_driven(self.ctx.h, self.h, dkey, other.h, key_lo,
+                                                 key_hi, len(src), arr, int(check_dups),
+                                                 C.byref(h), C.byref(lo), C.byref(hi)))
+        if not h.value:
+            return None
+        return Rel(self.ctx, h), lo.value, hi.value
 
     def free(self) -> None:
         if self.h:

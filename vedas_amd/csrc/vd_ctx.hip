@@ -373,6 +373,10 @@ int vd_ctx_tune(vd_ctx *ctx, const char *key, int value) {
     } else if (k == "radix_bits") {
         if (value != 0 && value != 8 && value != 9) return VD_EINVAL;
         ctx->radix_bits = value;
+    } else if (k == "radix_group") {
+        if (value != 0 && value != 4 && value != 8 && value != 16 && value != 32)
+            return VD_EINVAL;
+        ctx->radix_group = value;
     } else {
         ctx->last_error = "unknown tunable " + k;
         return VD_EINVAL;

@@ -111,6 +111,7 @@ struct vd_ctx {
     int join_nv = 2048;     // merge-path diagonal length per expansion block
     int join_rcap = 4096;   // right keys staged in LDS per bounds block
     int radix_bits = 0;     // IndexSwap digit width: 0 = by rule, 8, 9
+    int radix_group = 8;    // radix tiles per XCD group: 0 = one global order
 };
 
 // A relation: ncols device columns of nrows uint32 each.  A column whose bit

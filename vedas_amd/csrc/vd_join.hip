@@ -229,7 +229,7 @@ __global__ __launch_bounds__(256) void k_join_partition(
 // tiles' matched keys).
 __global__ __launch_bounds__(dev::TT_BLOCK) void k_join_scan_total(
     const uint64_t *__restrict__ ttot, const uint64_t *__restrict__ tkeys, uint64_t ntiles,
-    uint64_t *__restrict__ tpre, Readback rb) {
+    uint64_t *__restrict__ tpre, Readback rb, const uint32_t *__restrict__ flag) {
     __shared__ uint64_t s_tmp[dev::TT_BLOCK / 64];
     dev::scan_tile_totals_block(ttot, ntiles, tpre, s_tmp);
     uint32_t mn = ~0u, mx = 0;
@@ -248,19 +248,23 @@ __global__ __launch_bounds__(dev::TT_BLOCK) void k_join_scan_total(
         rb.dst[0] = total;
         rb.dst[1] = total ? mn : 0;
         rb.dst[2] = total ? mx : 0;
-        rb.dst[3] = overflow ? 1 : 0;  // a tile's offsets do not fit 32 bits
+        // bit 0: a tile's offsets do not fit 32 bits; bit 1: the caller's
+        // flag word (driven joins: repeated driver rows) is set
+        rb.dst[3] = (overflow ? 1u : 0u) | (flag && *flag ? 2u : 0u);
     }
     dev::rb_publish(rb);
 }
 
+// Output column c is gathered from src[c] at the output's left (driving)
+// row, or at its matched right row when bit c of from_r is set.
 struct OutCols {
-    const uint32_t *l[VD_MAX_COLS];
-    const uint32_t *r[VD_MAX_COLS];
+    const uint32_t *src[VD_MAX_COLS];
     uint32_t *o[VD_MAX_COLS];
-    int nl, nr;  // total columns of L and R
+    uint32_t from_r;
+    int n;
 };
 
-template <int NK, int NV>
+template <int NV>
 __global__ __launch_bounds__(JX_BLOCK) void k_join_expand(
     JoinOff off, const uint32_t *__restrict__ lower,
     const uint64_t *__restrict__ mp, uint64_t nl, uint64_t total, OutCols oc) {
@@ -319,19 +323,19 @@ __global__ __launch_bounds__(JX_BLOCK) void k_join_expand(
             rrow[j] = s_low[k] + (uint32_t)(o - s_off[k]);
         }
     }
-    // Output columns [L columns, R columns after the keys], gathered CB
-    // columns at a time: the loads of a batch are all issued before its
-    // stores (the output never aliases the inputs).
+    // Output columns gathered CB columns at a time: the loads of a batch
+    // are all issued before its stores (the output never aliases the
+    // inputs).
     constexpr int CB = 4;
-    const int ncl = oc.nl, nco = oc.nl + oc.nr - NK;
+    const int nco = oc.n;
     for (int c0 = 0; c0 < nco; c0 += CB) {
         uint32_t v[CB][ITEMS];
 #pragma unroll
         for (int cb = 0; cb < CB; ++cb) {
             const int c = c0 + cb;
             if (c < nco) {
-                const bool from_l = c < ncl;
-                const uint32_t *src = from_l ? oc.l[c] : oc.r[c - ncl + NK];
+                const bool from_l = ((oc.from_r >> c) & 1u) == 0;
+                const uint32_t *src = oc.src[c];
 #pragma unroll
                 for (int j = 0; j < ITEMS; ++j) {
                     const int q = j * JX_BLOCK + threadIdx.x;
@@ -351,6 +355,113 @@ __global__ __launch_bounds__(JX_BLOCK) void k_join_expand(
                 }
             }
         }
+    }
+}
+
+
+// ---- joins driven by a relation in output order ------------------------------
+//
+// vd_join_driven: for every row of D, in D's order, the rows of Y (sorted on
+// column 0) whose key equals D's key column, in Y's order.  Y's runs come
+// from a dense table over the key values (first row, end row per key), so
+// D may be in any order; the counts, offsets, partition and expansion are
+// the sort-merge join's.
+
+// tab[k] = {first row, end row} of key k's run in a sorted key column, for
+// the keys <= kmax the column holds (the table is zero-filled first, so an
+// absent key has an empty run).
+__global__ __launch_bounds__(256) void k_run_table(const uint32_t *__restrict__ key, uint64_t n,
+                                                   uint32_t kmax, uint2 *__restrict__ tab) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t k = key[i];
+        if (k > kmax) continue;
+        if (i == 0 || key[i - 1] != k) tab[k].x = (uint32_t)i;
+        if (i + 1 == n || key[i + 1] != k) tab[k].y = (uint32_t)(i + 1);
+    }
+}
+
+// Driver columns the count kernel reads: the key column, and (check > 0)
+// every column, to flag a row equal to the row before it that matches more
+// than one row.
+struct DriveCols {
+    const uint32_t *key;
+    const uint32_t *all[VD_MAX_COLS];
+    int check;  // columns compared for repeated rows (0: no check)
+};
+
+// Per driver row: first matched row and match count (keys outside [klo,
+// khi] match nothing; khi <= the table's last key), then the tile's scan
+// of the counts (32-bit tile-local offsets + tile total) and the smallest
+// and largest key with matches, as k_join_bounds writes them.
+__global__ __launch_bounds__(JB_BLOCK) void k_drive_bounds(
+    DriveCols D, uint64_t nd, const uint2 *__restrict__ tab, uint32_t klo, uint32_t khi,
+    uint32_t *__restrict__ lower, uint32_t *__restrict__ loc, uint64_t *__restrict__ tile_total,
+    uint64_t *__restrict__ tile_keys, uint32_t *__restrict__ flag) {
+    __shared__ uint64_t s_off[JB_TILE];
+    __shared__ uint32_t s_cnt[JB_TILE];
+    __shared__ uint64_t s_tmp[JB_BLOCK / 64];
+    const uint64_t tile = blockIdx.x;
+    const uint64_t base = tile * JB_TILE;
+    uint32_t key[JB_ITEMS];
+#pragma unroll
+    for (int j = 0; j < JB_ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)j * JB_BLOCK + threadIdx.x;
+        key[j] = i < nd ? D.key[i] : 0u;
+    }
+    uint2 run[JB_ITEMS];
+#pragma unroll
+    for (int j = 0; j < JB_ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)j * JB_BLOCK + threadIdx.x;
+        const bool in = i < nd && key[j] >= klo && key[j] <= khi;
+        run[j] = in ? tab[key[j]] : make_uint2(0u, 0u);
+    }
+    uint32_t cnt[JB_ITEMS], kmn = ~0u, kmx = 0;
+    bool repeated = false;
+#pragma unroll
+    for (int j = 0; j < JB_ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)j * JB_BLOCK + threadIdx.x;
+        cnt[j] = run[j].y - run[j].x;
+        if (i < nd) lower[i] = run[j].x;
+        if (cnt[j]) {
+            kmn = min(kmn, key[j]);
+            kmx = max(kmx, key[j]);
+        }
+        if (D.check && cnt[j] > 1 && i > 0) {
+            bool same = true;
+            for (int c = 0; c < D.check; ++c) same = same && D.all[c][i] == D.all[c][i - 1];
+            repeated = repeated || same;
+        }
+    }
+    if (repeated) atomicOr(flag, 1u);
+    dev::block_min_max<JB_BLOCK>(kmn, kmx, s_tmp);
+#pragma unroll
+    for (int j = 0; j < JB_ITEMS; ++j) s_cnt[j * JB_BLOCK + threadIdx.x] = cnt[j];
+    __syncthreads();
+    // Thread t scans rows t*ITEMS .. t*ITEMS+ITEMS-1 of the tile.
+    uint64_t sum = 0;
+    uint32_t mine[JB_ITEMS];
+#pragma unroll
+    for (int j = 0; j < JB_ITEMS; ++j) {
+        mine[j] = s_cnt[threadIdx.x * JB_ITEMS + j];
+        sum += mine[j];
+    }
+    uint64_t tot;
+    uint64_t ex = dev::block_excl_scan<JB_BLOCK>(sum, s_tmp, &tot);
+#pragma unroll
+    for (int j = 0; j < JB_ITEMS; ++j) {
+        s_off[threadIdx.x * JB_ITEMS + j] = ex;
+        ex += mine[j];
+    }
+    if (threadIdx.x == 0) {
+        tile_total[tile] = tot;
+        tile_keys[tile] = ((uint64_t)kmx << 32) | kmn;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < JB_ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)j * JB_BLOCK + threadIdx.x;
+        if (i < nd) loc[i] = (uint32_t)s_off[j * JB_BLOCK + threadIdx.x];
     }
 }
 
@@ -383,7 +494,7 @@ vd_rel *join_impl(vd_ctx *ctx, const vd_rel *L, const vd_rel *R) {
     const Readback rb = rb_next(ctx);
     launch(ctx, "scan_tiles", 16.0 * (double)jb_tiles, dim3(1), dim3(dev::TT_BLOCK), 0,
            k_join_scan_total, (const uint64_t *)ttot, (const uint64_t *)tkeys, jb_tiles, tpre,
-           rb);
+           rb, (const uint32_t *)nullptr);
     uint64_t h[4];
     rb_wait(ctx, rb, h, 4);
     const uint64_t total = h[0];
@@ -400,12 +511,15 @@ vd_rel *join_impl(vd_ctx *ctx, const vd_rel *L, const vd_rel *R) {
     out->key_hi = (uint32_t)h[2];
     if (total == 0) return out;
     try {
+        // columns [L's, R's after the keys]
         OutCols oc{};
-        for (int c = 0; c < L->ncols; ++c) oc.l[c] = L->cols[c];
-        for (int c = 0; c < R->ncols; ++c) oc.r[c] = R->cols[c];
-        for (int c = 0; c < out_cols; ++c) oc.o[c] = out->cols[c];
-        oc.nl = L->ncols;
-        oc.nr = R->ncols;
+        for (int c = 0; c < out_cols; ++c) {
+            const bool from_l = c < L->ncols;
+            oc.src[c] = from_l ? L->cols[c] : R->cols[c - L->ncols + NK];
+            if (!from_l) oc.from_r |= 1u << c;
+            oc.o[c] = out->cols[c];
+        }
+        oc.n = out_cols;
         const int nv = ctx->join_nv;
         const uint64_t nblocks = ceil_div(nl + total, (uint64_t)nv);
         VD_REQUIRE(nblocks < (1ull << 31), VD_ERANGE, "join expansion grid too large");
@@ -421,9 +535,118 @@ vd_rel *join_impl(vd_ctx *ctx, const vd_rel *L, const vd_rel *R) {
                    0, kern, joff, (const uint32_t *)lower.p,
                    (const uint64_t *)mp.p, nl, total, oc);
         };
-        if (nv == 1024) go(k_join_expand<NK, 1024>);
-        else if (nv == 4096) go(k_join_expand<NK, 4096>);
-        else go(k_join_expand<NK, 2048>);
+        if (nv == 1024) go(k_join_expand<1024>);
+        else if (nv == 4096) go(k_join_expand<4096>);
+        else go(k_join_expand<2048>);
+    } catch (...) {
+        rel_free(ctx, out);
+        throw;
+    }
+    return out;
+}
+
+
+// Largest key value a driven join's run table covers (8 bytes per value).
+constexpr uint32_t DRIVE_MAX_KEY = (1u << 28) - 1;
+
+// vd_join_driven (see the C ABI).  Returns nullptr, with nothing left on the
+// stream, when the driven form does not apply: a key bound too large for
+// the table, or (check_dups) a repeated driver row matching several rows.
+vd_rel *join_driven(vd_ctx *ctx, const vd_rel *D, int dkey, const vd_rel *Y, uint32_t klo,
+                    uint32_t khi, int nout, const int *src, int check_dups,
+                    uint32_t *key_lo, uint32_t *key_hi) {
+    VD_REQUIRE(dkey >= 0 && dkey < D->ncols, VD_EINVAL, "driver key column out of range");
+    VD_REQUIRE(nout >= 1 && nout <= VD_MAX_COLS, VD_EINVAL, "driven join output columns");
+    for (int c = 0; c < nout; ++c)
+        VD_REQUIRE(src[c] >= 0 ? src[c] < Y->ncols : -1 - src[c] < D->ncols, VD_EINVAL,
+                   "driven join column source out of range");
+    const uint64_t nd = D->nrows, ny = Y->nrows;
+    VD_REQUIRE(nd < (1ull << 32) && ny < (1ull << 32), VD_ERANGE,
+               "join inputs limited to 2^32-1 rows");
+    // Keys above a known bound of either key column match nothing.
+    uint32_t kmax = khi;
+    if (Y->bounded & 1u) kmax = std::min(kmax, Y->colmax[0]);
+    if ((D->bounded >> dkey) & 1u) kmax = std::min(kmax, D->colmax[dkey]);
+    if (kmax > DRIVE_MAX_KEY) return nullptr;
+    *key_lo = *key_hi = 0;
+    auto new_out = [&](uint64_t rows) {
+        vd_rel *o = rel_new(ctx, nout, rows);
+        for (int c = 0; c < nout; ++c) {
+            if (src[c] >= 0) rel_copy_bound(o, c, Y, src[c]);
+            else rel_copy_bound(o, c, D, -1 - src[c]);
+        }
+        return o;
+    };
+    if (nd == 0 || ny == 0 || klo > kmax) return new_out(0);
+
+    DBuf<uint2> tab(ctx, (uint64_t)kmax + 1);
+    fill_u32(ctx, reinterpret_cast<uint32_t *>(tab.p), 0, 2 * ((uint64_t)kmax + 1));
+    launch(ctx, "run_table", 4.0 * (double)ny, dim3((unsigned)std::min<uint64_t>(
+               ceil_div(ny, 256), 16384)), dim3(256), 0, k_run_table,
+           (const uint32_t *)Y->cols[0], ny, kmax, tab.p);
+    DBuf<uint32_t> lower(ctx, nd), off(ctx, nd);
+    const uint64_t jb_tiles = ceil_div(nd, JB_TILE);
+    // tile totals, tile prefixes (+ the total), tile key bounds, flag word
+    DBuf<uint64_t> tt(ctx, 3 * jb_tiles + 2);
+    uint64_t *ttot = tt.p, *tpre = tt.p + jb_tiles, *tkeys = tt.p + 2 * jb_tiles + 1;
+    uint32_t *flag = reinterpret_cast<uint32_t *>(tt.p + 3 * jb_tiles + 1);
+    fill_u32(ctx, flag, 0, 1);
+    DriveCols dc{};
+    dc.key = D->cols[dkey];
+    if (check_dups) {
+        dc.check = D->ncols;
+        for (int c = 0; c < D->ncols; ++c) dc.all[c] = D->cols[c];
+    }
+    launch(ctx, "drive_bounds", 4.0 * (double)nd * (1 + (check_dups ? D->ncols : 0)) +
+                                    16.0 * (double)nd,
+           dim3((unsigned)jb_tiles), dim3(JB_BLOCK), 0, k_drive_bounds, dc, nd,
+           (const uint2 *)tab.p, klo, kmax, lower.p, off.p, ttot, tkeys, flag);
+    const JoinOff joff{(const uint32_t *)off.p, tpre};
+    const Readback rb = rb_next(ctx);
+    launch(ctx, "scan_tiles", 16.0 * (double)jb_tiles, dim3(1), dim3(dev::TT_BLOCK), 0,
+           k_join_scan_total, (const uint64_t *)ttot, (const uint64_t *)tkeys, jb_tiles, tpre,
+           rb, (const uint32_t *)flag);
+    uint64_t h[4];
+    rb_wait(ctx, rb, h, 4);
+    if (h[3] & 2u) return nullptr;  // a repeated driver row: the order would differ
+    const uint64_t total = h[0];
+    VD_REQUIRE(total < (1ull << 40), VD_ERANGE, "join output too large");
+    VD_REQUIRE((h[3] & 1u) == 0, VD_ERANGE,
+               "join: 2048 consecutive driver rows match more than 2^32-1 rows");
+    *key_lo = (uint32_t)h[1];
+    *key_hi = (uint32_t)h[2];
+    vd_rel *out = new_out(total);
+    if (total == 0) return out;
+    try {
+        OutCols oc{};
+        for (int c = 0; c < nout; ++c) {
+            if (src[c] >= 0) {
+                oc.src[c] = Y->cols[src[c]];
+                oc.from_r |= 1u << c;
+            } else {
+                oc.src[c] = D->cols[-1 - src[c]];
+            }
+            oc.o[c] = out->cols[c];
+        }
+        oc.n = nout;
+        const int nv = ctx->join_nv;
+        const uint64_t nblocks = ceil_div(nd + total, (uint64_t)nv);
+        VD_REQUIRE(nblocks < (1ull << 31), VD_ERANGE, "join expansion grid too large");
+        DBuf<uint64_t> mp(ctx, nblocks + 1);
+        launch(ctx, "join_partition", 0.0,
+               dim3((unsigned)ceil_div((nblocks + 1) * JR_GROUP, 256)), dim3(256), 0,
+               k_join_partition, joff, nd, total, nv, nblocks, mp.p);
+        int nfd = 0, nfy = 0;
+        for (int c = 0; c < nout; ++c) (src[c] >= 0 ? nfy : nfd) += 1;
+        const double xbytes = 4.0 * (double)total * nout + 8.0 * nd + 4.0 * (double)nd * nfd +
+                              4.0 * (double)ny * nfy;
+        auto go = [&](auto kern) {
+            launch(ctx, "join_expand", xbytes, dim3((unsigned)nblocks), dim3(JX_BLOCK), 0, kern,
+                   joff, (const uint32_t *)lower.p, (const uint64_t *)mp.p, nd, total, oc);
+        };
+        if (nv == 1024) go(k_join_expand<1024>);
+        else if (nv == 4096) go(k_join_expand<4096>);
+        else go(k_join_expand<2048>);
     } catch (...) {
         rel_free(ctx, out);
         throw;
@@ -446,5 +669,19 @@ extern "C" int vd_join(vd_ctx *ctx, const vd_rel *L, const vd_rel *R, int nkeys,
         VD_REQUIRE(L->ncols >= nkeys && R->ncols >= nkeys, VD_EINVAL,
                    "relation has fewer columns than keys");
         *out = nkeys == 1 ? join_impl<1>(ctx, L, R) : join_impl<2>(ctx, L, R);
+    });
+}
+
+extern "C" int vd_join_driven(vd_ctx *ctx, const vd_rel *D, int dkey, const vd_rel *Y,
+                              uint32_t key_lo, uint32_t key_hi, int nout, const int *src,
+                              int check_dups, vd_rel **out, uint32_t *match_lo,
+                              uint32_t *match_hi) {
+    if (!ctx || !out || !match_lo || !match_hi) return VD_EINVAL;
+    *out = nullptr;
+    VD_TRY(ctx, {
+        VD_REQUIRE(D && Y && src, VD_EINVAL, "null argument");
+        VD_REQUIRE(Y->ncols >= 1, VD_EINVAL, "relation without columns");
+        *out = join_driven(ctx, D, dkey, Y, key_lo, key_hi, nout, src, check_dups != 0,
+                           match_lo, match_hi);
     });
 }

@@ -14,11 +14,20 @@
 // (fused into the producer of the keys where possible).
 //
 // Look-back termination: tile t only waits on tiles < t.  Tile ids come
-// from the atomic counter as blocks start, so every tile < t belongs to a
+// from an atomic counter as blocks start, so every tile < t belongs to a
 // block that is already resident, and a block publishes its aggregate
 // before it waits on anyone.  Tile 0 publishes an inclusive value without
 // waiting.  Hence every wait ends.  The look-back reads a window of
 // predecessors per round trip and stops at the first unpublished one.
+//
+// With tiles grouped per XCD (the radix_group tunable) each of the 8
+// counters hands out its own tiles in increasing order as the blocks of
+// that counter start.  Let u be the lowest tile no block has taken yet, on
+// counter v.  The resident blocks of counter v all hold tiles below u, so
+// they wait only on tiles below u, all of which are taken: the lowest
+// unfinished taken tile waits on nothing unfinished and completes, and so
+// on, until a block of counter v finishes and the next block of counter v
+// takes u.  Every wait still ends.
 
 #include "vd_radix_dev.hpp"
 
@@ -94,7 +103,7 @@ __global__ __launch_bounds__(BLOCK)
 __attribute__((amdgpu_waves_per_eu(OsShape<BLOCK, ITEMS, M>::WAVES_PER_EU, 8))) void k_rs_onesweep(
     const uint64_t *__restrict__ in, uint64_t *__restrict__ out, RadixPayload pay,
     RadixKeyIO kio, uint64_t n, int shift, int width, const uint32_t *__restrict__ gofs,
-    uint32_t *lookback, uint32_t *tile_counter) {
+    uint32_t *lookback, uint32_t *tile_counter, uint32_t group) {
     constexpr int WAVES = BLOCK / 64;
     constexpr int TILE = BLOCK * ITEMS;
     constexpr int RBINS = 1 << DB;
@@ -114,12 +123,27 @@ __attribute__((amdgpu_waves_per_eu(OsShape<BLOCK, ITEMS, M>::WAVES_PER_EU, 8))) 
 
     for (int i = tid; i < WAVES * RBINS / 2; i += BLOCK)
         reinterpret_cast<uint32_t *>(&s_wcnt[0][0])[i] = 0;
-    if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
+    if (tid == 0) {
+        if (group == 0) {
+            s_tile = atomicAdd(tile_counter, 1u);
+        } else {
+            // Block b runs on XCD b mod 8 (workgroups are dispatched to the
+            // XCDs round-robin).  Each XCD takes its own tiles from its own
+            // counter, `group` consecutive tiles at a time: groups g = v,
+            // v + 8, v + 16, ... belong to XCD v.  Neighbouring tiles of a
+            // group then run together on one XCD, whose L2 merges the
+            // partial cache lines they write at the ends of each digit run.
+            const uint32_t v = blockIdx.x & 7u;
+            const uint32_t m = atomicAdd(tile_counter + v, 1u);
+            s_tile = ((m / group) * 8u + v) * group + m % group;
+        }
+    }
     __syncthreads();
 
     const uint32_t tile = s_tile;
     const uint64_t base = (uint64_t)tile * TILE;
-    const uint64_t left = n - base;  // > 0: grid has exactly ceil(n/TILE) tiles
+    if (base >= n) return;  // a grouped grid rounds up to whole groups per XCD
+    const uint64_t left = n - base;
     const uint32_t nvalid = left < (uint64_t)TILE ? (uint32_t)left : (uint32_t)TILE;
     const uint32_t dmask = (1u << width) - 1u;
 
@@ -326,9 +350,13 @@ void run_passes(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, const RadixCols &src,
     constexpr int BINS = 1 << DB;
     const uint64_t ntiles = ceil_div(n, TILE);
     const uint64_t lb_words = (uint64_t)npass * ntiles * BINS;
-    DBuf<uint32_t> lb(ctx, lb_words + npass);
-    fill_u32(ctx, lb.p, 0, lb_words + npass);
+    DBuf<uint32_t> lb(ctx, lb_words + 8 * npass);
+    fill_u32(ctx, lb.p, 0, lb_words + 8 * npass);
     uint32_t *counters = lb.p + lb_words;
+    // Grouped tile order: 8 counters per pass, and a grid of whole groups
+    // per XCD (blocks past the last tile exit at once).
+    const uint32_t group = (uint32_t)ctx->radix_group;
+    const uint64_t grid = group ? 8 * ceil_div(ceil_div(ntiles, group), 8) * group : ntiles;
     RadixCols cur = src;
     for (int p = 0; p < npass; ++p) {
         const RadixCols &dstc = (p % 2 == 0) ? pb : pa;
@@ -352,10 +380,10 @@ void run_passes(vd_ctx *ctx, uint64_t *&a, uint64_t *&b, const RadixCols &src,
             if (io.out_hi) kbytes -= io.out_lo ? 0.0 : 4.0;
         }
         launch(ctx, M ? "radix_onesweep_kv" : "radix_onesweep",
-               (double)n * (kbytes + 8.0 * M), dim3((unsigned)ntiles),
+               (double)n * (kbytes + 8.0 * M), dim3((unsigned)grid),
                dim3(BLOCK), 0, k_rs_onesweep<BLOCK, ITEMS, M, DB>, (const uint64_t *)a, b, pay,
                io, n, shifts[p], widths[p], gofs[p],
-               lb.p + (uint64_t)p * ntiles * BINS, counters + p);
+               lb.p + (uint64_t)p * ntiles * BINS, counters + 8 * p, group);
         std::swap(a, b);
         cur = dstc;
     }

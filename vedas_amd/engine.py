@@ -18,7 +18,8 @@ class Stats(C.Structure):
     _fields_ = [("rows", C.c_uint64), ("join_rows", C.c_uint64),
                 ("select_rows", C.c_uint64), ("swap_rows", C.c_uint64),
                 ("elapsed_ms", C.c_double), ("join_ms", C.c_double),
-                ("select_ms", C.c_double), ("swap_ms", C.c_double), ("jobs", C.c_int)]
+                ("select_ms", C.c_double), ("swap_ms", C.c_double), ("jobs", C.c_int),
+                ("driven_joins", C.c_int)]
 
 
 class BatchStats(C.Structure):
@@ -41,7 +42,7 @@ class ServeRec(C.Structure):
 class TuneResult(C.Structure):
     _fields_ = [("radix_items", C.c_int), ("join_nv", C.c_int), ("join_rcap", C.c_int),
                 ("radix_bits", C.c_int), ("base_ms", C.c_double), ("tuned_ms", C.c_double),
-                ("evaluations", C.c_int)]
+                ("evaluations", C.c_int), ("radix_group", C.c_int)]
 
 
 _lib = None

@@ -85,6 +85,7 @@ void FullRelationIR::setRelation(size_t i, size_t offset, TYPEID_HOST_VEC::itera
     const size_t n = eit - bit;
     if (n == 0) return;
     if (offset + n > size()) throw VedasError(VD_EINVAL, "setRelation out of range");
+    if (i == 0) sorted0_ = false;
     ctx_->check(vd_rel_materialize(ctx_->raw(), rel_), "vd_rel_materialize");
     ctx_->check(vd_copy_h2d(ctx_->raw(), getRelationRawPointer(i) + offset, &*bit,
                             n * sizeof(TYPEID)), "vd_copy_h2d");
@@ -100,18 +101,20 @@ void FullRelationIR::setRelation(size_t i, size_t offset, const TYPEID *dbegin,
     const size_t n = dend - dbegin;
     if (n == 0) return;
     if (offset + n > size()) throw VedasError(VD_EINVAL, "setRelation out of range");
+    if (i == 0) sorted0_ = false;
     ctx_->check(vd_rel_materialize(ctx_->raw(), rel_), "vd_rel_materialize");
     ctx_->check(vd_copy_d2d(ctx_->raw(), getRelationRawPointer(i) + offset, dbegin,
                             n * sizeof(TYPEID)), "vd_copy_d2d");
 }
 
 void FullRelationIR::sortByFirstColumn() {
-    if (!rel_) return;
+    if (!rel_ || sorted0_) return;
     ctx_->check(vd_sort_col0(ctx_->raw(), rel_), "vd_sort_col0");
 }
 
 void FullRelationIR::swapColumn(size_t i, size_t j) {
     if (i == j) return;
+    if (i == 0 || j == 0) sorted0_ = false;
     ctx_->check(vd_rel_swap_cols(rel_, (int)i, (int)j), "vd_rel_swap_cols");
     std::swap(headers[i], headers[j]);
     bool t = is_predicates[i];
@@ -122,6 +125,7 @@ void FullRelationIR::swapColumn(size_t i, size_t j) {
 }
 
 void FullRelationIR::removeColumn(size_t i) {
+    if (i == 0) sorted0_ = false;
     if (getColumnNum() == 1) {
         vd_rel_destroy(ctx_->raw(), rel_);
         rel_ = nullptr;

@@ -52,8 +52,14 @@ class FullRelationIR : public IR {
     TYPEID_HOST_VEC download(size_t i) const;  // one column to the host
     vd_rel *raw() const { return rel_; }
     VedasContext *context() const { return ctx_; }
+    // The producer delivered the rows already stably sorted on column 0 (a
+    // join driven in IndexSwap order): sortByFirstColumn has nothing to do
+    // until column 0 changes.
+    void markSortedOnColumn0() { sorted0_ = true; }
+    bool sortedOnColumn0() const { return sorted0_; }
 
   private:
+    bool sorted0_ = false;
     VedasContext *ctx_ = nullptr;
     vd_rel *rel_ = nullptr;
     std::vector<bool> is_predicates;

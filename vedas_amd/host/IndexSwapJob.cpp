@@ -23,11 +23,13 @@ int IndexSwapJob::startJob(int gpuId) {
     auto *ir = dynamic_cast<FullRelationIR *>(beforeJob->getIR());
     if (!ir) throw VedasError(VD_EINVAL, "index swap on a job without a relation");
     const size_t col = ir->getColumnId(swapVar);
+    // A driven join already delivered this order (column 0 = swapVar).
+    const bool done = col == 0 && ir->sortedOnColumn0();
     ir->swapColumn(0, col);
     ir->sortByFirstColumn();
     context->exe_log.emplace_back(gpuId, SWAP_OP, ir->getHeader(0) + " <-> " + ir->getHeader(col),
                                   ir->size(), ir->getColumnNum());
-    context->swap_rows += ir->size();
+    if (!done) context->swap_rows += ir->size();
     context->swap_index_ns += std::chrono::duration_cast<std::chrono::nanoseconds>(
         std::chrono::steady_clock::now() - t0).count();
     if (ir->size() > 2 && ir->size() < EIF_ALPHA && ei_dict) {

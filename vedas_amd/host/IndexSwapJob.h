@@ -19,6 +19,8 @@ class IndexSwapJob : public QueryJob {
     int startJob(int gpuId) override;
     void print() const override;
     std::string jobTypeName() const override;
+    QueryJob *childJob() const { return beforeJob; }
+    const std::string &swapVariable() const { return swapVar; }
 
   private:
     VedasContext *context;

@@ -1,9 +1,12 @@
 #include "JoinQueryJob.h"
 
+#include <algorithm>
 #include <chrono>
 #include <iostream>
+#include <vector>
 
 #include "QueryExecutor.h"
+#include "SelectQueryJob.h"
 
 JoinQueryJob::JoinQueryJob(QueryJob *leftJob, QueryJob *rightJob, std::string joinVariable,
                            VedasContext *context, VariableBoundMap *variables_bound,
@@ -115,17 +118,28 @@ IR *JoinQueryJob::join(FullRelationIR *lir, FullRelationIR *rir) {
         std::chrono::steady_clock::now() - swap_start).count();
 
     const size_t nk = joinVars.size();
-    vd_rel *out = nullptr;
     if (lir->getColumnNum() == 0 || rir->getColumnNum() == 0)
         throw VedasError(VD_EINVAL, "join operand without columns");
-    context->check(vd_join(context->raw(), lir->raw(), rir->raw(), (int)nk, &out), "vd_join");
-    FullRelationIR *newIr = new FullRelationIR(context, out);
-    size_t h = 0;
-    for (size_t i = 0; i < nk; ++i) newIr->setHeader(h++, lir->getHeader(i), lir->getIsPredicate(i));
-    for (size_t i = nk; i < lir->getColumnNum(); ++i)
-        newIr->setHeader(h++, lir->getHeader(i), lir->getIsPredicate(i));
-    for (size_t i = nk; i < rir->getColumnNum(); ++i)
-        newIr->setHeader(h++, rir->getHeader(i), rir->getIsPredicate(i));
+    // The output re-sorted by the next IndexSwap's variable, made directly
+    // when one input is a select whose other permutation is resident.
+    TYPEID dlo = 0, dhi = 0;
+    FullRelationIR *newIr = nullptr;
+    if (join1Var && QueryExecutor::ENABLE_DRIVEN_SWAP && !swapHint.empty() &&
+        swapHint != joinVars[0])
+        newIr = drivenJoin(lir, rir, &dlo, &dhi);
+    const bool driven = newIr != nullptr;
+    if (!driven) {
+        vd_rel *out = nullptr;
+        context->check(vd_join(context->raw(), lir->raw(), rir->raw(), (int)nk, &out), "vd_join");
+        newIr = new FullRelationIR(context, out);
+        size_t h = 0;
+        for (size_t i = 0; i < nk; ++i)
+            newIr->setHeader(h++, lir->getHeader(i), lir->getIsPredicate(i));
+        for (size_t i = nk; i < lir->getColumnNum(); ++i)
+            newIr->setHeader(h++, lir->getHeader(i), lir->getIsPredicate(i));
+        for (size_t i = nk; i < rir->getColumnNum(); ++i)
+            newIr->setHeader(h++, rir->getHeader(i), rir->getIsPredicate(i));
+    }
     const size_t result_size = newIr->size();
     context->join_rows += result_size;
     context->exe_log.emplace_back(gpuId, JOIN_OP, lir->getHeaders("") + " x " + rir->getHeaders(""),
@@ -135,9 +149,10 @@ IR *JoinQueryJob::join(FullRelationIR *lir, FullRelationIR *rir) {
     // only, the result is not affected.
     if (QueryExecutor::ENABLE_UPDATE_BOUND_AFTER_JOIN && variables_bound) {
         if (result_size > 0) {
-            TYPEID ends[2];
-            // vd_join returns the ends of column 0 with its size read-back.
-            if (!vd_rel_key_range(out, &ends[0], &ends[1])) {
+            TYPEID ends[2] = {dlo, dhi};
+            // vd_join returns the ends of column 0 with its size read-back
+            // (a driven join the smallest and largest matched key).
+            if (!driven && !vd_rel_key_range(newIr->raw(), &ends[0], &ends[1])) {
                 context->check(vd_copy_d2h(context->raw(), &ends[0],
                                            newIr->getRelationRawPointer(0), 4), "vd_copy_d2h");
                 context->check(vd_copy_d2h(context->raw(), &ends[1],
@@ -148,7 +163,11 @@ IR *JoinQueryJob::join(FullRelationIR *lir, FullRelationIR *rir) {
             if (!lastJoinForVar && result_size > 2 && result_size < EIF_ALPHA && ei_dict) {
                 auto e0 = std::chrono::steady_clock::now();
                 size_t b[4];
-                eifFromSortedColumn(newIr->download(0), b);
+                // the join-ordered key column: column 0 of a plain join, the
+                // key column sorted on the host after a driven one
+                TYPEID_HOST_VEC keys = newIr->download(driven ? newIr->getColumnId(joinVariable) : 0);
+                if (driven) std::sort(keys.begin(), keys.end());
+                eifFromSortedColumn(keys, b);
                 context->eif_count++;
                 ei_dict->updateBound(joinVariable, b[0], b[1], b[2], b[3]);
                 context->eif_ns += std::chrono::duration_cast<std::chrono::nanoseconds>(
@@ -159,6 +178,70 @@ IR *JoinQueryJob::join(FullRelationIR *lir, FullRelationIR *rir) {
         }
     }
     return newIr;
+}
+
+// The join followed by the stable re-sort of its output on swapHint, in one
+// device step: the input X holding swapHint is a select (its columns
+// [key, swapHint]); the same pattern's rows in the other permutation (the
+// alternate view D, columns [swapHint, key], sorted on swapHint then key)
+// drive vd_join_driven against the other input Y.  Every D row emits its
+// key's run of Y rows, so the output comes in (swapHint, key, Y order)
+// order, which is the join's left-major order re-sorted stably on
+// swapHint: with X on the left each D row is one left row; with X on the
+// right each D row is one right row, distinct from its neighbours (D rows
+// that repeat and match several left rows would interleave differently:
+// vd_join_driven then declines).  The output columns are the join's
+// [key, left rest, right rest] with column 0 and swapHint's exchanged, as
+// IndexSwapJob leaves them.  Null when the form does not apply.
+FullRelationIR *JoinQueryJob::drivenJoin(FullRelationIR *lir, FullRelationIR *rir,
+                                         TYPEID *match_lo, TYPEID *match_hi) {
+    auto selectOf = [&](FullRelationIR *ir) -> SelectQueryJob * {
+        for (QueryJob *j : {leftJob, rightJob})
+            if (j && j->getIR() == ir) return dynamic_cast<SelectQueryJob *>(j);
+        return nullptr;
+    };
+    const std::string &key = joinVariable, &v = swapHint;
+    auto holds = [&](FullRelationIR *ir) {
+        return ir->getColumnNum() == 2 && ir->getHeader(0) == key && ir->getHeader(1) == v;
+    };
+    FullRelationIR *X = nullptr, *Y = nullptr;
+    if (holds(lir)) { X = lir; Y = rir; }
+    else if (holds(rir)) { X = rir; Y = lir; }
+    else return nullptr;
+    for (size_t i = 0; i < Y->getColumnNum(); ++i)
+        if (Y->getHeader(i) == v) return nullptr;
+    SelectQueryJob *sel = selectOf(X);
+    if (!sel || Y->getHeader(0) != key) return nullptr;
+    TYPEID clo = 0, chi = 0;
+    vd_rel *D = sel->alternateView(&clo, &chi);
+    if (!D) return nullptr;
+    // The join's output headers, then IndexSwapJob's exchange of column 0
+    // with swapHint's column.
+    std::vector<std::pair<std::string, bool>> cols;
+    for (size_t i = 0; i < lir->getColumnNum(); ++i)
+        cols.emplace_back(lir->getHeader(i), lir->getIsPredicate(i));
+    for (size_t i = 1; i < rir->getColumnNum(); ++i)
+        cols.emplace_back(rir->getHeader(i), rir->getIsPredicate(i));
+    size_t iv = 0;
+    while (cols[iv].first != v) ++iv;
+    std::swap(cols[0], cols[iv]);
+    std::vector<int> src;
+    for (auto &c : cols) {
+        if (c.first == v) src.push_back(-1);        // D column 0
+        else if (c.first == key) src.push_back(-2);  // D column 1
+        else src.push_back((int)Y->getColumnId(c.first));
+    }
+    vd_rel *out = nullptr;
+    const int rc = vd_join_driven(context->raw(), D, 1, Y->raw(), clo, chi, (int)src.size(),
+                                  src.data(), X == rir ? 1 : 0, &out, match_lo, match_hi);
+    vd_rel_destroy(context->raw(), D);
+    context->check(rc, "vd_join_driven");
+    if (!out) return nullptr;
+    auto *ir = new FullRelationIR(context, out);
+    for (size_t i = 0; i < cols.size(); ++i) ir->setHeader(i, cols[i].first, cols[i].second);
+    ir->markSortedOnColumn0();
+    context->driven_joins += 1;
+    return ir;
 }
 
 void JoinQueryJob::setOperands(QueryJob *l, QueryJob *r) {

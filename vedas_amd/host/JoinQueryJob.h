@@ -26,11 +26,18 @@ class JoinQueryJob : public QueryJob {
     unsigned getLeftIRSize() const;
     unsigned getRightIRSize() const;
     std::string getJoinVariable() const;
+    // The variable an IndexSwapJob re-sorts this join's output by next
+    // (QueryPlan::execute sets it): the join may then deliver its output in
+    // that order directly (drivenJoin).
+    void setSwapHint(const std::string &v) { swapHint = v; }
     VariableBoundMap *variables_bound{nullptr};
     EmptyIntervalDict *ei_dict{nullptr};
 
   private:
+    FullRelationIR *drivenJoin(FullRelationIR *lir, FullRelationIR *rir, TYPEID *match_lo,
+                               TYPEID *match_hi);
     std::string joinVariable;
+    std::string swapHint;
     VedasContext *context;
     QueryJob *leftJob{nullptr};
     QueryJob *rightJob{nullptr};

@@ -14,6 +14,10 @@ bool QueryExecutor::ENABLE_UPDATE_BOUND_AFTER_JOIN = true;
 // The pattern pre-scan of createVariableBound (QueryExecutor.cu:463-478);
 // off gives join variables the full range (for comparing intermediates).
 bool QueryExecutor::ENABLE_BOUND_PRESCAN = true;
+// A join followed by an IndexSwap delivers the swapped order itself where
+// one input is a select with its other permutation resident (no
+// counterpart in the reference; off runs the join and the re-sort).
+bool QueryExecutor::ENABLE_DRIVEN_SWAP = true;
 
 QueryExecutor::QueryExecutor(VedasStorage *s, VedasContext *c) : vedasStorage(s), context(c) {}
 

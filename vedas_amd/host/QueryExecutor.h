@@ -29,6 +29,7 @@ class QueryExecutor {
     static bool ENABLE_FULL_INDEX;
     static bool ENABLE_UPDATE_BOUND_AFTER_JOIN;
     static bool ENABLE_BOUND_PRESCAN;
+    static bool ENABLE_DRIVEN_SWAP;
     // Bounds of the join variables after the last query (for inspection).
     const VariableBoundMap &bounds() const { return variables_bound; }
 

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "FullRelationIR.h"
+#include "IndexSwapJob.h"
 #include "JoinQueryJob.h"
 #include "SelectQueryJob.h"
 
@@ -51,6 +52,12 @@ static void resolveSelectRanges(const std::vector<QueryJob *> &jobs) {
 // QueryPlan::execute, single-GPU branch (QueryPlan.cu:160-205).
 void QueryPlan::execute(SparqlResult &result) {
     resolveSelectRanges(jobs);
+    // A join whose output an IndexSwapJob re-sorts next learns the swap
+    // variable (it may deliver that order itself).
+    for (auto *job : jobs)
+        if (auto *sw = dynamic_cast<IndexSwapJob *>(job))
+            if (auto *jj = dynamic_cast<JoinQueryJob *>(sw->childJob()))
+                jj->setSwapHint(sw->swapVariable());
     for (auto &v : select_variables) query_variable_counter[v] = ULONG_MAX;
     for (auto *job : jobs) {
         if (auto *jj = dynamic_cast<JoinQueryJob *>(job)) jj->setQueryVarCounter(&query_variable_counter);

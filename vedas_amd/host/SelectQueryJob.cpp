@@ -81,6 +81,10 @@ int SelectQueryJob::startJob(int gpuId) {
             clip = (lo == std::numeric_limits<TYPEID>::min() &&
                     hi == std::numeric_limits<TYPEID>::max()) ? 0 : 1;
         }
+        if (clip) {
+            clip_lo = lo;
+            clip_hi = hi;
+        }
         if (!is_second_var_used) {
             select(2, clip, lo, hi, nullptr, 0, 1);
             ir = new FullRelationIR(context, rel);
@@ -89,6 +93,7 @@ int SelectQueryJob::startJob(int gpuId) {
             std::vector<vd_interval_t> skip;
             if (ei_dict) skip = ei_dict->skipIntervals(variables[0]);
             select(3, clip, lo, hi, skip.empty() ? nullptr : skip.data(), (int)skip.size(), 0);
+            alt_ok = skip.empty();
             ir = new FullRelationIR(context, rel);
             ir->setHeader(0, variables[0], is_predicates[0]);
             ir->setHeader(1, variables[1], is_predicates[1]);
@@ -96,6 +101,8 @@ int SelectQueryJob::startJob(int gpuId) {
         context->exe_log.emplace_back(gpuId, UPLOAD_OP, ir->getHeaders(""), ir->size(),
                                       ir->getColumnNum());
     }
+    sel_begin = b;
+    sel_end = e;
     intermediateResult = ir;
     context->select_rows += ir->size();
     const long long ns = std::chrono::duration_cast<std::chrono::nanoseconds>(
@@ -118,6 +125,23 @@ vd_select_job_t SelectQueryJob::lookupJob() const {
 }
 
 vd_index *SelectQueryJob::indexHandle() const { return index.storage->index(); }
+
+vd_rel *SelectQueryJob::alternateView(TYPEID *lo, TYPEID *hi) const {
+    if (!alt_ok) return nullptr;
+    static const int other[6] = {VD_PERM_SOP, VD_PERM_SPO, VD_PERM_POS,
+                                 VD_PERM_PSO, VD_PERM_OPS, VD_PERM_OSP};
+    const int alt = other[index.perm];
+    if (!index.storage->hasPermutation(alt)) return nullptr;
+    // Both permutations sort on the same X first: the constant's rows are
+    // the same row range in either.
+    vd_rel *v = nullptr;
+    context->check(vd_select(context->raw(), index.storage->index(), alt, sel_begin, sel_end, 3,
+                             0, 0, 0, nullptr, 0, 0, &v),
+                   "vd_select");
+    *lo = clip_lo;
+    *hi = clip_hi;
+    return v;
+}
 
 void SelectQueryJob::setRange(uint64_t begin, uint64_t end) {
     has_range = true;

@@ -48,8 +48,19 @@ class SelectQueryJob : public QueryJob {
     // Row range of the lookup, resolved ahead of time (QueryPlan::execute
     // batches the lookups of a plan): startJob then only filters it.
     void setRange(uint64_t begin, uint64_t end);
+    // After startJob, for a 2-variable select whose rows were only clipped
+    // on its first variable: the same constant's range in the permutation
+    // with Y and Z exchanged (PSO <-> POS, SPO <-> SOP, OSP <-> OPS), as a
+    // view with columns [second variable, first variable], unclipped, and
+    // the clip of the first variable ([0, 2^32-1] without one).  Null when
+    // that permutation is not built or rows were dropped otherwise (EIF
+    // skips, a distinct first column).
+    vd_rel *alternateView(TYPEID *clip_lo, TYPEID *clip_hi) const;
 
   private:
+    bool alt_ok = false;  // startJob kept every row inside the clip
+    uint64_t sel_begin = 0, sel_end = 0;  // the constant's row range
+    TYPEID clip_lo = 0, clip_hi = 0xFFFFFFFFu;
     bool has_range = false;
     uint64_t range_begin = 0, range_end = 0;
     IndexRef index;

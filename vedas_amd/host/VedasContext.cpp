@@ -27,5 +27,6 @@ void VedasContext::resetStats() {
     upload_ns = join_ns = swap_index_ns = eliminate_duplicate_ns = eif_ns = 0;
     eif_count = 0;
     join_rows = select_rows = swap_rows = 0;
+    driven_joins = 0;
     exe_log.clear();
 }

@@ -33,6 +33,7 @@ class VedasContext {
            eliminate_duplicate_ns = 0, eif_ns = 0;
     int eif_count = 0;
     uint64_t join_rows = 0, select_rows = 0, swap_rows = 0;
+    uint64_t driven_joins = 0;  // joins that delivered their IndexSwap order
     std::vector<ExecuteLogRecord> exe_log;
     void resetStats();
 

@@ -241,6 +241,7 @@ void fill_stats(vdq_result *res, VedasContext *ctx) {
     res->stats.select_ms = ctx->upload_ns / 1e6;
     res->stats.swap_ms = ctx->swap_index_ns / 1e6;
     res->stats.jobs = (int)ctx->exe_log.size();
+    res->stats.driven_joins = (int)ctx->driven_joins;
 }
 
 }  // namespace
@@ -574,7 +575,8 @@ int vdq_autotune(vdq_engine *e, const char *const *texts, int n, int reps, vdq_t
         std::vector<Knob> knobs = {{"radix_items", {16, 8, 24}, 16},
                                    {"join_nv", {2048, 1024, 4096}, 2048},
                                    {"join_rcap", {4096, 2048, 8192}, 4096},
-                                   {"radix_bits", {0, 8, 9}, 0}};
+                                   {"radix_bits", {0, 8, 9}, 0},
+                                   {"radix_group", {8, 0, 16}, 8}};
         for (auto &k : knobs) set(k.key, k.cur);
         run_all();  // warm-up
         const double base = run_all();
@@ -602,6 +604,7 @@ int vdq_autotune(vdq_engine *e, const char *const *texts, int n, int reps, vdq_t
         out->join_nv = knobs[1].cur;
         out->join_rcap = knobs[2].cur;
         out->radix_bits = knobs[3].cur;
+        out->radix_group = knobs[4].cur;
         out->base_ms = base;
         out->tuned_ms = best;
         out->evaluations = evals;
@@ -688,6 +691,8 @@ int vdq_set_option(const char *key, int value) {
         QueryExecutor::ENABLE_BOUND_PRESCAN = value != 0;
     } else if (k == "bound_after_join") {
         QueryExecutor::ENABLE_UPDATE_BOUND_AFTER_JOIN = value != 0;
+    } else if (k == "driven_swap") {
+        QueryExecutor::ENABLE_DRIVEN_SWAP = value != 0;
     } else {
         return fail(VD_EINVAL, "unknown option " + k);
     }
@@ -701,6 +706,8 @@ int vdq_get_option(const char *key, int *value) {
         *value = QueryExecutor::ENABLE_BOUND_PRESCAN;
     } else if (k == "bound_after_join") {
         *value = QueryExecutor::ENABLE_UPDATE_BOUND_AFTER_JOIN;
+    } else if (k == "driven_swap") {
+        *value = QueryExecutor::ENABLE_DRIVEN_SWAP;
     } else {
         return fail(VD_EINVAL, "unknown option " + k);
     }
