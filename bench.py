@@ -274,8 +274,12 @@ def serve_bench(eng, texts, qps, seed=1, slots=4, max_batch=128, budget=96 << 30
     # served stream is short (tens of ms at saturation) and the first run of
     # a mode pays its warm-up (the slot contexts' pools, for example).
     def median_run(**kw):
-        runs = sorted((summarise(eng.serve(texts, arr, **kw)[0]) for _ in range(3)),
-                      key=lambda r: r["qps"])
+        runs = []
+        for k in range(3):
+            runs.append(summarise(eng.serve(texts, arr, **kw)[0]))
+            log(f"[bench]     {'plain' if kw.get('plain') else 'served'} run {k}: "
+                f"{runs[-1]['qps']:.0f} q/s")
+        runs.sort(key=lambda r: r["qps"])
         return runs[1], [r["qps"] for r in runs]
     plain, plain_runs = median_run(plain=True)
     served, served_runs = median_run(slots=slots, max_batch=max_batch, mem_budget=budget,
@@ -622,6 +626,37 @@ def main():
         r.free()
     log("[bench] planner: " + ", ".join(f"{q} {v['ms']:.1f} ms" for q, v in planned.items()))
 
+    # The same steps with driven joins off: every IndexSwap after a join
+    # re-sorts the join's output (the reference's order of operations).
+    set_option("driven_swap", 0)
+    try:
+        step()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            step()
+        ctx.sync()
+        plain_step_ms = (time.perf_counter() - t0) / 3 * 1e3
+        plain_q = {}
+        for q in names:
+            r = eng.query(texts[q])
+            st0 = r.stats()
+            plain_q[q] = {"swap_rows": st0["swap_rows"], "join_rows": st0["join_rows"],
+                          "same_result": result_digest(r.headers(), r.columns()) ==
+                          gpu_results[q]["digest"]}
+            r.free()
+    finally:
+        set_option("driven_swap", 1)
+    driven = {"ms_per_step_with": dt / args.steps * 1e3, "ms_per_step_without": plain_step_ms,
+              "queries": {q: {"driven_joins": per_q[q][0]["driven_joins"],
+                              "swap_rows_with": per_q[q][0]["swap_rows"],
+                              "swap_rows_without": plain_q[q]["swap_rows"],
+                              "same_join_rows": plain_q[q]["join_rows"] ==
+                              per_q[q][0]["join_rows"],
+                              "same_result": plain_q[q]["same_result"]} for q in names}}
+    log(f"[bench] driven joins: {driven['ms_per_step_with']:.2f} ms per step with, "
+        f"{plain_step_ms:.2f} ms without")
+
     batch = serve = None
     if args.mix > 0:
         mix = query_mix(args.mix, W.SEED_MIX, STORE_100M, s, p, o)
@@ -664,12 +699,14 @@ def main():
                    "triples": n, "distinct": True, "entities": GEN["n_entities"],
                    "predicates": GEN["n_predicates"], "subject_zipf": GEN["subject_zipf"],
                    "pred_zipf": GEN["pred_zipf"], "literal_frac": GEN["literal_frac"],
-                   "queries": names, "seed": args.seed, "bound_prescan": True},
+                   "queries": names, "seed": args.seed, "bound_prescan": True,
+                   "driven_joins": True},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "queries": queries,
         "queries_note": "GB/s = algorithmic operator bytes of the query / its median wall time",
         "bound_prescan": prescan,
+        "driven_joins": driven,
         "index_build": {"ms": build_s * 1e3, "permutations": list(perms), "key_bits": kb,
                         "alg_bytes": build_alg, "GB/s": build_alg / build_s / 1e9,
                         "frac": build_alg / build_s / HBM_PEAK,

@@ -83,7 +83,9 @@ int  vd_ctx_mem_peak(vd_ctx *ctx, uint64_t *peak, int reset);
  * they save a pass, else 8-bit, the default; 8 = always 8-bit; 9 = 9-bit
  * whenever the key fits 27 bits), "radix_group" (radix tiles an XCD takes
  * as one group of consecutive tiles: 8, the default, or 4, 16, 32; 0 = one
- * global tile order), "join_nv" (merge-path diagonal per join
+ * global tile order), "drive_min_rows" (smallest input rows, both inputs
+ * together, of a driven join: 2^20 by default), "join_nv" (merge-path
+ * diagonal per join
  * expansion block), "join_rcap" (right keys staged in LDS per bounds
  * block).  Returns VD_EINVAL for an unknown key or an unsupported value. */
 int  vd_ctx_tune(vd_ctx *ctx, const char *key, int value);
@@ -271,15 +273,30 @@ int  vd_join(vd_ctx *ctx, const vd_rel *L, const vd_rel *R, int nkeys,
  * key is outside [key_lo, key_hi] match nothing.  Output column c is Y's
  * column src[c] (src[c] >= 0) or D's column -1 - src[c].  *match_lo /
  * *match_hi receive the smallest and largest key that matched.
- * Returns VD_OK with *out == NULL when the driven form does not apply: the
- * key values (bounded by key_hi and the columns' value bounds) exceed
- * 2^28 - 1 (the run table of Y holds 8 bytes per key value), or, with
- * check_dups, two adjacent equal D rows match more than one row each (the
- * join-then-sort order would interleave them differently). */
+ * Returns VD_OK with *out == NULL when the driven form does not apply: D
+ * and Y hold fewer rows together than the "drive_min_rows" tunable (2^20
+ * by default), or the key values (bounded by
+ * key_hi and the columns' value bounds) exceed 2^28 - 1 or twice the rows
+ * of D and Y (the run table of Y is zero-filled over every key value, 8
+ * bytes each), or, with check_dups, two adjacent equal D rows match more
+ * than one row each (the join-then-sort order would interleave them
+ * differently). */
 int  vd_join_driven(vd_ctx *ctx, const vd_rel *D, int dkey, const vd_rel *Y,
                     uint32_t key_lo, uint32_t key_hi, int nout, const int *src,
                     int check_dups, vd_rel **out, uint32_t *match_lo,
                     uint32_t *match_hi);
+/* 1 when vd_join_driven(D, dkey, Y, .., key_hi, ..) takes the driven form
+ * (its size rules, which do not depend on row order), else 0: a caller
+ * about to re-sort D for it asks first. */
+int  vd_join_driven_applies(const vd_ctx *ctx, const vd_rel *D, int dkey,
+                            const vd_rel *Y, uint32_t key_hi);
+/* vd_join(L, R, nkeys), unless its output would have more than max_rows
+ * rows: then *out = NULL and nothing is built (the count phase of
+ * JoinQueryJob::join, JoinQueryJob.cu:262-269, has run).  *rows receives
+ * the output size either way.  For a caller that takes another form for
+ * large outputs (the driven join after re-sorting its left input). */
+int  vd_join_capped(vd_ctx *ctx, const vd_rel *L, const vd_rel *R, int nkeys,
+                    uint64_t max_rows, vd_rel **out, uint64_t *rows);
 
 /* ---- re-sort and finalise (FullRelationIR / IndexSwapJob / QueryPlan) --- */
 /* Stable sort of all rows by column 0 only.  Replaces

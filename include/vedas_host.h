@@ -161,7 +161,10 @@ typedef struct {
 int vdq_serve(vdq_engine *e, const char *const *texts, const double *arrival_ms, int n,
               const vdq_serve_cfg_t *cfg, vdq_serve_rec_t *recs, vdq_result **outs);
 /* Estimated final rows and peak HBM bytes of a query from the store
- * statistics (what serving admission uses). */
+ * statistics, run alone through vdq_query (its joins followed by an
+ * IndexSwap may be driven joins, whose run tables are counted).  Serving
+ * admission uses the same estimate, without the driven joins' tables when
+ * requests run batched. */
 int vdq_estimate(vdq_engine *e, const char *text, double *rows, double *peak_bytes);
 
 int vdq_result_shape(const vdq_result *r, int *ncols, uint64_t *nrows);

@@ -15,6 +15,7 @@ pytestmark = pytest.mark.gpu
 def ctx():
     from vedas_amd._native import Ctx
     c = Ctx(0)
+    c.tune("drive_min_rows", 0)  # the stores here are small
     yield c
     c.close()
 
@@ -127,8 +128,8 @@ def test_driven_declines(ctx):
     d = Rel.from_columns(ctx, [np.array([7, 7, 8, 9], np.uint32),
                                np.array([5, 5, 2, 1], np.uint32)])
     # host-made columns carry no value bound: key_hi bounds the run table
-    assert d.join_driven(1, y, [-1, 1], key_hi=100, check_dups=True) is None
-    out, lo, hi = d.join_driven(1, y, [-1, 1, -2], key_hi=100)
+    assert d.join_driven(1, y, [-1, 1], key_hi=9, check_dups=True) is None
+    out, lo, hi = d.join_driven(1, y, [-1, 1, -2], key_hi=9)
     got = out.columns()
     assert got[0].tolist() == [7, 7, 7, 7, 7, 7, 8, 9, 9]
     assert got[1].tolist() == [50, 51, 52, 50, 51, 52, 20, 10, 11]
@@ -141,6 +142,12 @@ def test_driven_declines(ctx):
     out.free()
     big = Rel.from_columns(ctx, [np.array([1 << 29], np.uint32)])
     assert big.join_driven(0, y, [-1, 1]) is None  # no bound at all: too many key values
+    assert d.join_driven(1, y, [-1, 1], key_hi=100) is None  # 101 key values for 10 rows
+    ctx.tune("drive_min_rows", 11)
+    try:
+        assert d.join_driven(1, y, [-1, 1], key_hi=9) is None  # 10 rows < 11
+    finally:
+        ctx.tune("drive_min_rows", 0)
     for x in (y, d, big):
         x.free()
 
@@ -163,8 +170,10 @@ def test_plans_with_and_without_driven_swaps(shape):
     else:
         pats, node = plans.snowflake_plan([1, 2], [3, 1])
     text = query_text(pats, node)
+    from vedas_amd._native import Ctx
     e = Engine(0)
     try:
+        Ctx.wrap(e.ctx_handle()).tune("drive_min_rows", 0)  # a small store
         e.load(s, p, o, ("PSO", "POS"))
         got = {}
         for on in (1, 0):
@@ -187,3 +196,66 @@ def test_plans_with_and_without_driven_swaps(shape):
     rel = ex.run(node)
     assert O.bag_digest(h1, c1) == O.bag_digest(rel.headers, ex.columns(rel))
     assert get_option("driven_swap") == 1
+
+
+def test_plan_with_resorted_left_input():
+    """A join whose left input is an intermediate holding the next swap
+    variable, with an output several times that input: the input is
+    re-sorted on the swap variable and drives (vd_join_size decides), and
+    the plan's result is identical to the plain path's and equals the
+    oracle's plan walk."""
+    from tests import plans
+    from vedas_amd._native import Ctx
+    from vedas_amd.engine import Engine, query_text, set_option
+    s, p, o = distinct_triples(120_000, 21, ns=4000, npred=4, no=6000, hot_s=997, hot_o=1009)
+    keep = (p != 4) | (np.arange(len(p)) % 20 == 0)  # predicate 4 rare
+    s, p, o = s[keep], p[keep], o[keep]
+    x, y, a, z, b = "?x", "?y", "?a", "?z", "?b"
+    pats = [(x, 4, y), (x, 2, a), (z, 3, x), (y, 1, b)]
+    inner = ("JOIN", ("JOIN", ("UPLOAD", pats[0], x), ("UPLOAD", pats[1], x), x),
+             ("UPLOAD", pats[2], x), x)
+    node = ("JOIN", ("INDEXSWAP", inner, y), ("UPLOAD", pats[3], y), y)
+    text = query_text(pats, node)
+    e = Engine(0)
+    try:
+        Ctx.wrap(e.ctx_handle()).tune("drive_min_rows", 0)
+        e.load(s, p, o, ("PSO", "POS"))
+        got = {}
+        for on in (1, 0):
+            set_option("driven_swap", on)
+            r = e.query(text)
+            got[on] = (r.headers(), r.columns(), r.stats())
+            r.free()
+    finally:
+        set_option("driven_swap", 1)
+        e.close()
+    (h1, c1, st1), (h0, c0, st0) = got[1], got[0]
+    print(f"rows {st1['rows']} join rows {st1['join_rows']} driven {st1['driven_joins']} "
+          f"swap rows {st1['swap_rows']} / {st0['swap_rows']}", flush=True)
+    assert st1["driven_joins"] == 1 and st1["swap_rows"] < st0["swap_rows"]
+    assert h1 == h0
+    for u, w in zip(c1, c0):
+        assert np.array_equal(u, w)
+    ex = plans.OracleExecutor(s, p, o)
+    rel = ex.run(node)
+    assert O.bag_digest(h1, c1) == O.bag_digest(rel.headers, ex.columns(rel))
+
+
+def test_join_capped(ctx, store):
+    """vd_join_capped: the join when its output fits max_rows (equal to
+    vd_join's and the oracle's), nothing but the size beyond."""
+    idx, ix = store
+    b1, e1 = O.lookup(ix["POS"], 1)
+    b2, e2 = O.lookup(ix["PSO"], 2)
+    want = O.join(O.select(ix["POS"], b1, e1, 3), O.select(ix["PSO"], b2, e2, 3), 1)
+    n = len(want[0])
+    L = idx.select("POS", b1, e1, 3)
+    R = idx.select("PSO", b2, e2, 3)
+    out, rows = L.join_capped(R, n - 1)
+    assert out is None and rows == n
+    out, rows = L.join_capped(R, n)
+    assert rows == n
+    check_equal(out.columns(), want)
+    assert L.join_capped(R, 0)[1] == n
+    for x in (L, R, out):
+        x.free()

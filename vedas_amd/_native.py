@@ -110,6 +110,10 @@ EXPORTS = {
                                           C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "vd_join": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                           C.POINTER(C.c_void_p)]),
+    "vd_join_driven_applies": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                         C.c_uint32]),
+    "vd_join_capped": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_uint64,
+                                 C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]),
     "vd_join_driven": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_uint32,
                                  C.c_uint32, C.c_int, C.POINTER(C.c_int), C.c_int,
                                  C.POINTER(C.c_void_p), C.POINTER(C.c_uint32),
@@ -349,6 +353,14 @@ class Rel:
         h = C.c_void_p()
         self.ctx.check(self.ctx.L.vd_join(self.ctx.h, self.h, other.h, nkeys, C.byref(h)))
         return Rel(self.ctx, h)
+
+    def join_capped(self, other: "Rel", max_rows: int, nkeys: int = 1):
+        """(join, output rows), or (None, output rows) when the output would
+        have more than max_rows rows (nothing is built then)."""
+        h, n = C.c_void_p(), C.c_uint64()
+        self.ctx.check(self.ctx.L.vd_join_capped(self.ctx.h, self.h, other.h, nkeys, max_rows,
+                                                 C.byref(h), C.byref(n)))
+        return (Rel(self.ctx, h) if h.value else None), n.value
 
     def join_driven(self, dkey: int, other: "Rel", src, key_lo: int = 0,
                     key_hi: int = 0xFFFFFFFF, check_dups: bool = False):

@@ -82,19 +82,14 @@ hipEvent_t take_event(vd_ctx *ctx) {
         ctx->event_pool.pop_back();
         return e;
     }
+    // Timing events only: no system-scope release or acquire around them
+    // (the host never reads device memory through them).
     hipEvent_t e;
-    VD_HIP(hipEventCreate(&e));
+    VD_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     return e;
 }
 
-void prof_begin(vd_ctx *ctx, hipEvent_t *ev) {
-    *ev = take_event(ctx);
-    VD_HIP(hipEventRecord(*ev, ctx->stream));
-}
-
-void prof_end(vd_ctx *ctx, const char *name, double bytes, hipEvent_t ev0) {
-    hipEvent_t ev1 = take_event(ctx);
-    VD_HIP(hipEventRecord(ev1, ctx->stream));
+void prof_push(vd_ctx *ctx, const char *name, double bytes, hipEvent_t ev0, hipEvent_t ev1) {
     ctx->pending.push_back({name, bytes, ev0, ev1});
 }
 
@@ -373,6 +368,9 @@ int vd_ctx_tune(vd_ctx *ctx, const char *key, int value) {
     } else if (k == "radix_bits") {
         if (value != 0 && value != 8 && value != 9) return VD_EINVAL;
         ctx->radix_bits = value;
+    } else if (k == "drive_min_rows") {
+        if (value < 0) return VD_EINVAL;
+        ctx->drive_min_rows = value;
     } else if (k == "radix_group") {
         if (value != 0 && value != 4 && value != 8 && value != 16 && value != 32)
             return VD_EINVAL;

@@ -1,6 +1,7 @@
 // Internal definitions shared by the HIP translation units of libvedas_hip.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -112,6 +113,7 @@ struct vd_ctx {
     int join_rcap = 4096;   // right keys staged in LDS per bounds block
     int radix_bits = 0;     // IndexSwap digit width: 0 = by rule, 8, 9
     int radix_group = 8;    // radix tiles per XCD group: 0 = one global order
+    int drive_min_rows = 1 << 20;  // smallest inputs of a driven join
 };
 
 // A relation: ncols device columns of nrows uint32 each.  A column whose bit
@@ -171,8 +173,8 @@ struct DBuf {
 
 // ---- launches --------------------------------------------------------------
 hipEvent_t take_event(vd_ctx *ctx);
-void prof_begin(vd_ctx *ctx, hipEvent_t *ev);
-void prof_end(vd_ctx *ctx, const char *name, double bytes, hipEvent_t ev0);
+// A launch timed by events its own dispatch recorded.
+void prof_push(vd_ctx *ctx, const char *name, double bytes, hipEvent_t ev0, hipEvent_t ev1);
 // After a stream sync, or (done_only) the launches whose stop event is done.
 void prof_collect(vd_ctx *ctx, bool done_only);
 inline bool prof_selected(const vd_ctx *ctx, const char *name) {
@@ -182,20 +184,25 @@ inline bool prof_selected(const vd_ctx *ctx, const char *name) {
     return false;
 }
 
-// Launch `kernel` on the context stream; with profiling on, the launch is
-// bracketed by events and charged `bytes` algorithmic HBM bytes.
+// Launch `kernel` on the context stream; with profiling on, the launch
+// carries start and stop events (recorded by the kernel's own dispatch, no
+// marker packets around it) and is charged `bytes` algorithmic HBM bytes.
 template <typename... KArgs, typename... Args>
 inline void launch(vd_ctx *ctx, const char *name, double bytes, dim3 grid,
                    dim3 block, size_t shmem, void (*kernel)(KArgs...),
                    Args &&...args) {
     if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
-    hipEvent_t ev0 = nullptr;
-    const bool timed = ctx->prof && prof_selected(ctx, name);
-    if (timed) prof_begin(ctx, &ev0);
+    if (ctx->prof && prof_selected(ctx, name)) {
+        hipEvent_t ev0 = take_event(ctx), ev1 = take_event(ctx);
+        hipExtLaunchKernelGGL(kernel, grid, block, (uint32_t)shmem, ctx->stream, ev0, ev1, 0u,
+                              std::forward<Args>(args)...);
+        VD_HIP(hipGetLastError());
+        prof_push(ctx, name, bytes, ev0, ev1);
+        return;
+    }
     hipLaunchKernelGGL(kernel, grid, block, shmem, ctx->stream,
                        std::forward<Args>(args)...);
     VD_HIP(hipGetLastError());
-    if (timed) prof_end(ctx, name, bytes, ev0);
 }
 
 void sync(vd_ctx *ctx);

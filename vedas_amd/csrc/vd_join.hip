@@ -369,15 +369,31 @@ __global__ __launch_bounds__(JX_BLOCK) void k_join_expand(
 
 // tab[k] = {first row, end row} of key k's run in a sorted key column, for
 // the keys <= kmax the column holds (the table is zero-filled first, so an
-// absent key has an empty run).
+// absent key has an empty run).  A run's first row looks RT_WIN rows ahead:
+// a shorter run gets its entry in one 8-byte store; a longer one gets its
+// end from its last row, which sees the first row more than RT_WIN back.
+constexpr int RT_WIN = 8;
 __global__ __launch_bounds__(256) void k_run_table(const uint32_t *__restrict__ key, uint64_t n,
                                                    uint32_t kmax, uint2 *__restrict__ tab) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const uint32_t k = key[i];
         if (k > kmax) continue;
-        if (i == 0 || key[i - 1] != k) tab[k].x = (uint32_t)i;
-        if (i + 1 == n || key[i + 1] != k) tab[k].y = (uint32_t)(i + 1);
+        if (i == 0 || key[i - 1] != k) {
+            // the next RT_WIN keys, loaded together (the column is sorted:
+            // the run ends at the first one that differs)
+            uint32_t nx[RT_WIN];
+#pragma unroll
+            for (int w = 0; w < RT_WIN; ++w) nx[w] = i + 1 + w < n ? key[i + 1 + w] : ~k;
+            uint64_t e = i + 1;
+#pragma unroll
+            for (int w = 0; w < RT_WIN; ++w) e += (nx[w] == k && e == i + 1 + w) ? 1 : 0;
+            if (e <= i + RT_WIN) tab[k] = make_uint2((uint32_t)i, (uint32_t)e);
+            else tab[k].x = (uint32_t)i;  // longer than RT_WIN rows
+        } else if ((i + 1 == n || key[i + 1] != k) && i >= (uint64_t)RT_WIN &&
+                   key[i - RT_WIN] == k) {
+            tab[k].y = (uint32_t)(i + 1);  // the last row of a run of > RT_WIN rows
+        }
     }
 }
 
@@ -465,50 +481,80 @@ __global__ __launch_bounds__(JB_BLOCK) void k_drive_bounds(
     }
 }
 
-template <int NK>
-vd_rel *join_impl(vd_ctx *ctx, const vd_rel *L, const vd_rel *R) {
-    const uint64_t nl = L->nrows, nr = R->nrows;
-    const int out_cols = L->ncols + R->ncols - NK;
-    VD_REQUIRE(out_cols <= VD_MAX_COLS, VD_EINVAL, "join output has too many columns");
-    if (nl == 0 || nr == 0) return rel_new(ctx, out_cols, 0);
-    VD_REQUIRE(nl < (1ull << 32) && nr < (1ull << 32), VD_ERANGE,
-               "join inputs limited to 2^32-1 rows");
+// The count phase of a join: per left row its first match (lower) and
+// tile-local offset (off), per tile the prefix (tpre, in `ranges`), and the
+// read-back {total, first key, last key}.
+struct JoinCount {
+    DBuf<uint32_t> lower, off;
+    DBuf<uint64_t> ranges;
+    uint64_t *tpre = nullptr;
+    uint64_t total = 0;
+    uint32_t key_lo = 0, key_hi = 0;
+};
 
+template <int NK>
+void join_count(vd_ctx *ctx, const vd_rel *L, const vd_rel *R, JoinCount &jc) {
+    const uint64_t nl = L->nrows, nr = R->nrows;
     KeyCols lk{L->cols[0], NK == 2 ? L->cols[1] : nullptr};
     KeyCols rk{R->cols[0], NK == 2 ? R->cols[1] : nullptr};
-    DBuf<uint32_t> lower(ctx, nl);
-    DBuf<uint32_t> off(ctx, nl);  // offsets within each bounds tile
+    jc.lower = DBuf<uint32_t>(ctx, nl);
+    jc.off = DBuf<uint32_t>(ctx, nl);  // offsets within each bounds tile
     const uint64_t jb_tiles = ceil_div(nl, JB_TILE);
     const int rcap = std::min<int>(ctx->join_rcap, JB_RCAP_BYTES / (NK == 2 ? 8 : 4));
     // ranges (2 per tile), tile totals (1 per tile), tile prefixes (1 per
     // tile + the total), tile key bounds (1 per tile).
-    DBuf<uint64_t> ranges(ctx, 5 * jb_tiles + 1);
-    uint64_t *ttot = ranges.p + 2 * jb_tiles, *tpre = ranges.p + 3 * jb_tiles;
-    uint64_t *tkeys = ranges.p + 4 * jb_tiles + 1;
+    jc.ranges = DBuf<uint64_t>(ctx, 5 * jb_tiles + 1);
+    uint64_t *ranges = jc.ranges.p;
+    uint64_t *ttot = ranges + 2 * jb_tiles, *tpre = ranges + 3 * jb_tiles;
+    uint64_t *tkeys = ranges + 4 * jb_tiles + 1;
+    jc.tpre = tpre;
     launch(ctx, "join_ranges", 0.0, dim3((unsigned)ceil_div(jb_tiles * 2 * JR_GROUP, 256)),
-           dim3(256), 0, k_join_ranges<NK>, lk, nl, rk, nr, jb_tiles, ranges.p);
+           dim3(256), 0, k_join_ranges<NK>, lk, nl, rk, nr, jb_tiles, ranges);
     launch(ctx, "join_bounds", 4.0 * NK * (double)(nl + nr) + 8.0 * nl,
            dim3((unsigned)jb_tiles), dim3(JB_BLOCK), 0, k_join_bounds<NK>, lk,
-           nl, rk, nr, rcap, (const uint64_t *)ranges.p, lower.p, off.p, ttot, tkeys);
-    const JoinOff joff{(const uint32_t *)off.p, tpre};
+           nl, rk, nr, rcap, (const uint64_t *)ranges, jc.lower.p, jc.off.p, ttot, tkeys);
     const Readback rb = rb_next(ctx);
     launch(ctx, "scan_tiles", 16.0 * (double)jb_tiles, dim3(1), dim3(dev::TT_BLOCK), 0,
            k_join_scan_total, (const uint64_t *)ttot, (const uint64_t *)tkeys, jb_tiles, tpre,
            rb, (const uint32_t *)nullptr);
     uint64_t h[4];
     rb_wait(ctx, rb, h, 4);
-    const uint64_t total = h[0];
-    VD_REQUIRE(total < (1ull << 40), VD_ERANGE, "join output too large");
+    jc.total = h[0];
+    jc.key_lo = (uint32_t)h[1];
+    jc.key_hi = (uint32_t)h[2];
+    VD_REQUIRE(jc.total < (1ull << 40), VD_ERANGE, "join output too large");
     VD_REQUIRE(h[3] == 0, VD_ERANGE,
                "join: 2048 consecutive left rows match more than 2^32-1 right rows");
+}
+
+
+// The join; with max_rows, nullptr (and *rows = the output size) when the
+// output would have more rows than that.
+template <int NK>
+vd_rel *join_impl(vd_ctx *ctx, const vd_rel *L, const vd_rel *R,
+                  uint64_t max_rows = ~0ull, uint64_t *rows = nullptr) {
+    const uint64_t nl = L->nrows, nr = R->nrows;
+    const int out_cols = L->ncols + R->ncols - NK;
+    VD_REQUIRE(out_cols <= VD_MAX_COLS, VD_EINVAL, "join output has too many columns");
+    if (rows) *rows = 0;
+    if (nl == 0 || nr == 0) return rel_new(ctx, out_cols, 0);
+    VD_REQUIRE(nl < (1ull << 32) && nr < (1ull << 32), VD_ERANGE,
+               "join inputs limited to 2^32-1 rows");
+    JoinCount jc;
+    join_count<NK>(ctx, L, R, jc);
+    const uint64_t total = jc.total;
+    if (rows) *rows = total;
+    if (total > max_rows) return nullptr;
+    const JoinOff joff{(const uint32_t *)jc.off.p, jc.tpre};
+    DBuf<uint32_t> &lower = jc.lower;
 
     vd_rel *out = rel_new(ctx, out_cols, total);
     // columns [L's, R's after the keys]: their values come from those columns
     for (int c = 0; c < out_cols; ++c)
         rel_copy_bound(out, c, c < L->ncols ? L : R, c < L->ncols ? c : c - L->ncols + NK);
     out->key_range = true;  // output column 0 = left keys of matched rows
-    out->key_lo = (uint32_t)h[1];
-    out->key_hi = (uint32_t)h[2];
+    out->key_lo = jc.key_lo;
+    out->key_hi = jc.key_hi;
     if (total == 0) return out;
     try {
         // columns [L's, R's after the keys]
@@ -548,10 +594,31 @@ vd_rel *join_impl(vd_ctx *ctx, const vd_rel *L, const vd_rel *R) {
 
 // Largest key value a driven join's run table covers (8 bytes per value).
 constexpr uint32_t DRIVE_MAX_KEY = (1u << 28) - 1;
+// The driven form pays for a zero-filled table of every key value up to the
+// bound and one random table read per driver row; it replaces a re-sort of
+// the join output.  It is taken for inputs of at least ctx->drive_min_rows
+// rows together (2^20 by default) whose key bound is at most
+// DRIVE_KEYS_PER_ROW times that.
+constexpr uint64_t DRIVE_KEYS_PER_ROW = 2;
+
+// Whether a driven join of D (key column dkey) and Y applies, and the
+// largest key its run table covers: keys above khi or a known bound of
+// either key column match nothing.
+bool drive_applies(const vd_ctx *ctx, const vd_rel *D, int dkey, const vd_rel *Y, uint32_t khi,
+                   uint32_t *kmax_out) {
+    const uint64_t nd = D->nrows, ny = Y->nrows;
+    uint32_t kmax = khi;
+    if (Y->bounded & 1u) kmax = std::min(kmax, Y->colmax[0]);
+    if ((D->bounded >> dkey) & 1u) kmax = std::min(kmax, D->colmax[dkey]);
+    *kmax_out = kmax;
+    return kmax <= DRIVE_MAX_KEY && nd + ny >= (uint64_t)ctx->drive_min_rows &&
+           (uint64_t)kmax + 1 <= DRIVE_KEYS_PER_ROW * (nd + ny);
+}
 
 // vd_join_driven (see the C ABI).  Returns nullptr, with nothing left on the
-// stream, when the driven form does not apply: a key bound too large for
-// the table, or (check_dups) a repeated driver row matching several rows.
+// stream, when the driven form does not apply: inputs too small or a key
+// bound too large for the run table, or (check_dups) a repeated driver row
+// matching several rows.
 vd_rel *join_driven(vd_ctx *ctx, const vd_rel *D, int dkey, const vd_rel *Y, uint32_t klo,
                     uint32_t khi, int nout, const int *src, int check_dups,
                     uint32_t *key_lo, uint32_t *key_hi) {
@@ -563,11 +630,8 @@ vd_rel *join_driven(vd_ctx *ctx, const vd_rel *D, int dkey, const vd_rel *Y, uin
     const uint64_t nd = D->nrows, ny = Y->nrows;
     VD_REQUIRE(nd < (1ull << 32) && ny < (1ull << 32), VD_ERANGE,
                "join inputs limited to 2^32-1 rows");
-    // Keys above a known bound of either key column match nothing.
-    uint32_t kmax = khi;
-    if (Y->bounded & 1u) kmax = std::min(kmax, Y->colmax[0]);
-    if ((D->bounded >> dkey) & 1u) kmax = std::min(kmax, D->colmax[dkey]);
-    if (kmax > DRIVE_MAX_KEY) return nullptr;
+    uint32_t kmax = 0;
+    if (!drive_applies(ctx, D, dkey, Y, khi, &kmax)) return nullptr;
     *key_lo = *key_hi = 0;
     auto new_out = [&](uint64_t rows) {
         vd_rel *o = rel_new(ctx, nout, rows);
@@ -684,4 +748,26 @@ extern "C" int vd_join_driven(vd_ctx *ctx, const vd_rel *D, int dkey, const vd_r
         *out = join_driven(ctx, D, dkey, Y, key_lo, key_hi, nout, src, check_dups != 0,
                            match_lo, match_hi);
     });
+}
+
+extern "C" int vd_join_capped(vd_ctx *ctx, const vd_rel *L, const vd_rel *R, int nkeys,
+                              uint64_t max_rows, vd_rel **out, uint64_t *rows) {
+    if (!ctx || !out || !rows) return VD_EINVAL;
+    *out = nullptr;
+    *rows = 0;
+    VD_TRY(ctx, {
+        VD_REQUIRE(L && R, VD_EINVAL, "null relation");
+        VD_REQUIRE(nkeys == 1 || nkeys == 2, VD_EINVAL, "nkeys must be 1 or 2");
+        VD_REQUIRE(L->ncols >= nkeys && R->ncols >= nkeys, VD_EINVAL,
+                   "relation has fewer columns than keys");
+        *out = nkeys == 1 ? join_impl<1>(ctx, L, R, max_rows, rows)
+                          : join_impl<2>(ctx, L, R, max_rows, rows);
+    });
+}
+
+extern "C" int vd_join_driven_applies(const vd_ctx *ctx, const vd_rel *D, int dkey,
+                                      const vd_rel *Y, uint32_t key_hi) {
+    if (!ctx || !D || !Y || dkey < 0 || dkey >= D->ncols || Y->ncols < 1) return 0;
+    uint32_t kmax = 0;
+    return vd::drive_applies(ctx, D, dkey, Y, key_hi, &kmax) ? 1 : 0;
 }

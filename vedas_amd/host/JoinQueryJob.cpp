@@ -124,13 +124,15 @@ IR *JoinQueryJob::join(FullRelationIR *lir, FullRelationIR *rir) {
     // when one input is a select whose other permutation is resident.
     TYPEID dlo = 0, dhi = 0;
     FullRelationIR *newIr = nullptr;
+    vd_rel *out = nullptr;  // a plain join the driven attempt already built
     if (join1Var && QueryExecutor::ENABLE_DRIVEN_SWAP && !swapHint.empty() &&
         swapHint != joinVars[0])
-        newIr = drivenJoin(lir, rir, &dlo, &dhi);
+        newIr = drivenJoin(lir, rir, &dlo, &dhi, &out);
     const bool driven = newIr != nullptr;
     if (!driven) {
-        vd_rel *out = nullptr;
-        context->check(vd_join(context->raw(), lir->raw(), rir->raw(), (int)nk, &out), "vd_join");
+        if (!out)
+            context->check(vd_join(context->raw(), lir->raw(), rir->raw(), (int)nk, &out),
+                           "vd_join");
         newIr = new FullRelationIR(context, out);
         size_t h = 0;
         for (size_t i = 0; i < nk; ++i)
@@ -181,20 +183,27 @@ IR *JoinQueryJob::join(FullRelationIR *lir, FullRelationIR *rir) {
 }
 
 // The join followed by the stable re-sort of its output on swapHint, in one
-// device step: the input X holding swapHint is a select (its columns
-// [key, swapHint]); the same pattern's rows in the other permutation (the
-// alternate view D, columns [swapHint, key], sorted on swapHint then key)
-// drive vd_join_driven against the other input Y.  Every D row emits its
-// key's run of Y rows, so the output comes in (swapHint, key, Y order)
-// order, which is the join's left-major order re-sorted stably on
-// swapHint: with X on the left each D row is one left row; with X on the
-// right each D row is one right row, distinct from its neighbours (D rows
-// that repeat and match several left rows would interleave differently:
-// vd_join_driven then declines).  The output columns are the join's
-// [key, left rest, right rest] with column 0 and swapHint's exchanged, as
-// IndexSwapJob leaves them.  Null when the form does not apply.
+// device step (vd_join_driven): rows of the input X holding swapHint, in
+// swapHint order, each emit their key's run of the other input Y.
+//  * X a select (columns [key, swapHint]): the same pattern's rows in the
+//    other permutation (the alternate view D, columns [swapHint, key],
+//    sorted on swapHint then key) drive, with no sort at all.
+//  * X the left input, any relation: X itself is re-sorted on swapHint and
+//    drives, when the join's output has at least twice X's rows, so
+//    re-sorting X costs less than re-sorting the output: the join's count
+//    phase runs first, and a smaller output is built as the plain join's
+//    (returned in *plain_out, with null).
+// The output comes in (swapHint, key, Y order) order, which is the join's
+// left-major order re-sorted stably on swapHint: with X on the left each D
+// row is one left row, in left order within equal swapHint values; with X
+// on the right each D row is one right row, distinct from its neighbours
+// (D rows that repeat and match several left rows would interleave
+// differently: vd_join_driven then declines).  The output columns are the
+// join's [key, left rest, right rest] with column 0 and swapHint's
+// exchanged, as IndexSwapJob leaves them.  Null when no form applies.
 FullRelationIR *JoinQueryJob::drivenJoin(FullRelationIR *lir, FullRelationIR *rir,
-                                         TYPEID *match_lo, TYPEID *match_hi) {
+                                         TYPEID *match_lo, TYPEID *match_hi,
+                                         vd_rel **plain_out) {
     auto selectOf = [&](FullRelationIR *ir) -> SelectQueryJob * {
         for (QueryJob *j : {leftJob, rightJob})
             if (j && j->getIR() == ir) return dynamic_cast<SelectQueryJob *>(j);
@@ -204,17 +213,12 @@ FullRelationIR *JoinQueryJob::drivenJoin(FullRelationIR *lir, FullRelationIR *ri
     auto holds = [&](FullRelationIR *ir) {
         return ir->getColumnNum() == 2 && ir->getHeader(0) == key && ir->getHeader(1) == v;
     };
-    FullRelationIR *X = nullptr, *Y = nullptr;
-    if (holds(lir)) { X = lir; Y = rir; }
-    else if (holds(rir)) { X = rir; Y = lir; }
-    else return nullptr;
-    for (size_t i = 0; i < Y->getColumnNum(); ++i)
-        if (Y->getHeader(i) == v) return nullptr;
-    SelectQueryJob *sel = selectOf(X);
-    if (!sel || Y->getHeader(0) != key) return nullptr;
-    TYPEID clo = 0, chi = 0;
-    vd_rel *D = sel->alternateView(&clo, &chi);
-    if (!D) return nullptr;
+    auto has = [&](FullRelationIR *ir, const std::string &h) {
+        for (size_t i = 0; i < ir->getColumnNum(); ++i)
+            if (ir->getHeader(i) == h) return true;
+        return false;
+    };
+    if (lir->getHeader(0) != key || rir->getHeader(0) != key) return nullptr;
     // The join's output headers, then IndexSwapJob's exchange of column 0
     // with swapHint's column.
     std::vector<std::pair<std::string, bool>> cols;
@@ -223,19 +227,58 @@ FullRelationIR *JoinQueryJob::drivenJoin(FullRelationIR *lir, FullRelationIR *ri
     for (size_t i = 1; i < rir->getColumnNum(); ++i)
         cols.emplace_back(rir->getHeader(i), rir->getIsPredicate(i));
     size_t iv = 0;
-    while (cols[iv].first != v) ++iv;
+    while (iv < cols.size() && cols[iv].first != v) ++iv;
+    if (iv == cols.size()) return nullptr;
     std::swap(cols[0], cols[iv]);
-    std::vector<int> src;
-    for (auto &c : cols) {
-        if (c.first == v) src.push_back(-1);        // D column 0
-        else if (c.first == key) src.push_back(-2);  // D column 1
-        else src.push_back((int)Y->getColumnId(c.first));
-    }
+    // Output column sources: D's columns (-1 - c) or Y's (c).
+    auto sources = [&](FullRelationIR *D, FullRelationIR *Y) {
+        std::vector<int> src;
+        for (auto &c : cols)
+            src.push_back(has(D, c.first) ? -1 - (int)D->getColumnId(c.first)
+                                          : (int)Y->getColumnId(c.first));
+        return src;
+    };
     vd_rel *out = nullptr;
-    const int rc = vd_join_driven(context->raw(), D, 1, Y->raw(), clo, chi, (int)src.size(),
-                                  src.data(), X == rir ? 1 : 0, &out, match_lo, match_hi);
-    vd_rel_destroy(context->raw(), D);
-    context->check(rc, "vd_join_driven");
+    FullRelationIR *X = nullptr;
+    if (holds(lir) && selectOf(lir)) X = lir;
+    else if (holds(rir) && selectOf(rir)) X = rir;
+    vd_rel *D = nullptr;
+    TYPEID clo = 0, chi = 0;
+    if (X) D = selectOf(X)->alternateView(&clo, &chi);
+    if (D) {
+        FullRelationIR *Y = X == lir ? rir : lir;
+        if (has(Y, v)) {
+            vd_rel_destroy(context->raw(), D);
+            return nullptr;
+        }
+        // D's columns are [swapHint, key]
+        std::vector<int> src;
+        for (auto &c : cols)
+            src.push_back(c.first == v ? -1 : c.first == key ? -2 : (int)Y->getColumnId(c.first));
+        const int rc = vd_join_driven(context->raw(), D, 1, Y->raw(), clo, chi, (int)src.size(),
+                                      src.data(), X == rir ? 1 : 0, &out, match_lo, match_hi);
+        vd_rel_destroy(context->raw(), D);
+        context->check(rc, "vd_join_driven");
+    } else if (has(lir, v) && !has(rir, v)) {
+        if (!vd_join_driven_applies(context->raw(), lir->raw(), 0, rir->raw(), 0xFFFFFFFFu))
+            return nullptr;
+        // The join itself when its output is small (its count phase tells).
+        uint64_t total = 0;
+        context->check(vd_join_capped(context->raw(), lir->raw(), rir->raw(), 1,
+                                      2 * (uint64_t)lir->size() - 1, plain_out, &total),
+                       "vd_join_capped");
+        if (*plain_out) return nullptr;
+        lir->swapColumn(0, lir->getColumnId(v));
+        lir->sortByFirstColumn();
+        context->swap_rows += lir->size();
+        const std::vector<int> src = sources(lir, rir);
+        context->check(vd_join_driven(context->raw(), lir->raw(), (int)lir->getColumnId(key),
+                                      rir->raw(), 0, 0xFFFFFFFFu, (int)src.size(), src.data(), 0,
+                                      &out, match_lo, match_hi),
+                       "vd_join_driven");
+        // the size rules were checked before the re-sort; nothing else declines
+        if (!out) throw VedasError(VD_EINVAL, "driven join declined after its input was re-sorted");
+    }
     if (!out) return nullptr;
     auto *ir = new FullRelationIR(context, out);
     for (size_t i = 0; i < cols.size(); ++i) ir->setHeader(i, cols[i].first, cols[i].second);

@@ -35,7 +35,7 @@ class JoinQueryJob : public QueryJob {
 
   private:
     FullRelationIR *drivenJoin(FullRelationIR *lir, FullRelationIR *rir, TYPEID *match_lo,
-                               TYPEID *match_hi);
+                               TYPEID *match_hi, vd_rel **plain_out);
     std::string joinVariable;
     std::string swapHint;
     VedasContext *context;

@@ -42,7 +42,7 @@ void Server::run(std::vector<ServeRequest> &reqs, std::vector<ServeRecord> &recs
     if ((int)slot_ctx.size() < nslots) throw VedasError(VD_EINVAL, "too few slot contexts");
 
     // Memory estimate of every request (admission).
-    PlanEstimator est(*stats);
+    PlanEstimator est(*stats, cfg.plain);
     std::vector<double> need(n);
     for (int i = 0; i < n; ++i) {
         need[i] = est.estimate(reqs[i].patterns, reqs[i].root).peak_bytes;

@@ -198,14 +198,15 @@ void PlanEstimator::rekey(Est &e, const std::string &var, PlanEstimate &acc,
 }
 
 PlanEstimator::Est PlanEstimator::node(PlanTreeNode *n, const std::vector<TriplePattern> &pats,
-                                       PlanEstimate &acc, double &live) const {
+                                       PlanEstimate &acc, double &live,
+                                       bool swapped_next) const {
     if (n->op == UPLOAD) {
         Est e = select(*n->tp, n->var);
         acc.work_rows += e.rows;  // a view: no device memory
         return e;
     }
     if (n->op == INDEXSWAP) {
-        Est e = node(n->child1, pats, acc, live);
+        Est e = node(n->child1, pats, acc, live, true);
         rekey(e, n->var, acc, live);
         return e;
     }
@@ -240,8 +241,13 @@ PlanEstimator::Est PlanEstimator::node(PlanTreeNode *n, const std::vector<Triple
     ve.distinct = std::min(va.distinct, vb.distinct);
     e.vars[v] = ve;
     e.key = v;
-    // first-match and offset arrays of the left rows, the output columns
-    const double temps = a.rows * 8.0;
+    // first-match and offset arrays of the left rows, the output columns;
+    // a join an IndexSwap follows may be driven (the driver's first-match
+    // and offset arrays and a run table of at most 16 bytes per input row,
+    // for inputs of 2^20 rows or more)
+    double temps = a.rows * 8.0;
+    if (driven_ && swapped_next && a.rows + b.rows >= (double)(1u << 20))
+        temps = std::max(temps, 8.0 * std::max(a.rows, b.rows) + 16.0 * (a.rows + b.rows));
     const double out = e.rows * e.cols * 4.0;
     acc.peak_bytes = std::max(acc.peak_bytes, live + temps + out);
     for (auto *x : {&a, &b})

@@ -79,7 +79,11 @@ struct PlanEstimate {
 
 class PlanEstimator {
   public:
-    explicit PlanEstimator(const Statistics &st) : st_(st) {}
+    // driven: the plan runs on the one-query path (QueryPlan), whose joins
+    // followed by an IndexSwap may be driven joins; the batched executor
+    // has none.
+    explicit PlanEstimator(const Statistics &st, bool driven = true)
+        : st_(st), driven_(driven) {}
     PlanEstimate estimate(const std::vector<TriplePattern> &patterns, PlanTreeNode *root) const;
 
   private:
@@ -94,11 +98,13 @@ class PlanEstimator {
         std::string key;    // the variable of column 0
         bool view = false;  // rows of an index permutation (no device memory)
     };
+    // swapped_next: an IndexSwap re-sorts this node's output next
     Est node(PlanTreeNode *n, const std::vector<TriplePattern> &pats, PlanEstimate &acc,
-             double &live) const;
+             double &live, bool swapped_next = false) const;
     Est select(const TriplePattern &tp, const std::string &var) const;
     void rekey(Est &e, const std::string &var, PlanEstimate &acc, double &live) const;
     const Statistics &st_;
+    bool driven_;
 };
 
 #endif

@@ -53,6 +53,8 @@ LABELS = [
     (r"k_sel_bounds_multi\b", "select_bounds"),
     (r"k_hist_", "histogram"),
     (r"k_runs_expand\b", "runs_expand"),
+    (r"k_run_table\b", "run_table"),
+    (r"k_drive_bounds\b", "drive_bounds"),
     (r"k_mark_l1\b|k_mark_y\b|k_scatter_heads\b", "l2_heads"),
     (r"k_pack_rowid\b|k_pack_multi\b", "sort_pack"),
     (r"k_swap_hist\b", "sort_hist"),
