@@ -262,6 +262,13 @@ int  vd_select_batch_noalloc(vd_ctx *ctx, const vd_index *idx, int perm,
  * Input relations are not modified. */
 int  vd_join(vd_ctx *ctx, const vd_rel *L, const vd_rel *R, int nkeys,
              vd_rel **out);
+/* 1 when the rows are known to be pairwise distinct, else 0.  Selects of
+ * an index built without repeated triples return distinct rows (the build
+ * counts repeated rows of its sorted keys); joins of distinct inputs and
+ * the operators that only reorder or drop rows keep the property; dropping
+ * a column or vd_rel_col clears it.  vd_unique_adjacent (and _multi) leave
+ * such a relation as it is without reading it. */
+int  vd_rel_distinct(const vd_rel *rel);
 
 /* Join driven by a relation in output order (no single counterpart in the
  * reference: it yields what JoinQueryJob::join followed by

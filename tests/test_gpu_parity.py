@@ -778,3 +778,50 @@ def test_histograms_equal_oracle(ctx, store, perm):
             lo, hi, fr = idx.histogram(perm, True, d, limit)
             want = O.hist_eq_depth(keys, d, limit)
             assert (lo.tolist(), hi.tolist(), fr.tolist()) == want, (perm, limit, d)
+
+
+@pytest.mark.parametrize("repeat", [False, True])
+def test_distinct_rows_skip_the_unique(ctx, repeat):
+    """An index of distinct triples gives selects whose rows are known to be
+    distinct, joins of them keep the property, and the adjacent unique then
+    launches nothing; one repeated triple (counted by the build) leaves
+    every select unmarked and the unique runs.  Dropping a column clears
+    the mark.  Results equal the oracle's either way."""
+    from vedas_amd._native import Index
+    s, p, o = rand_triples(50_000, 77)
+    t = np.unique(np.stack([s, p, o], 1), axis=0)
+    s, p, o = (np.ascontiguousarray(t[:, i]) for i in range(3))
+    if repeat:
+        s, p, o = (np.append(a, a[123]) for a in (s, p, o))
+    idx = Index(ctx, s, p, o, ["PSO", "POS"])
+    ix = {q: O.build_perm(s, p, o, q) for q in ("PSO", "POS")}
+    ctx.prof(True)
+    try:
+        b1, e1 = O.lookup(ix["POS"], 2)
+        b2, e2 = O.lookup(ix["PSO"], 3)
+        L = idx.select("POS", b1, e1, 3)
+        R = idx.select("PSO", b2, e2, 3)
+        assert L.distinct == R.distinct == (not repeat)
+        out = L.join(R, 1)
+        assert out.distinct == (not repeat)
+        want = O.join(O.select(ix["POS"], b1, e1, 3), O.select(ix["PSO"], b2, e2, 3), 1)
+        ctx.prof_reset()
+        out.unique_adjacent()
+        launched = "unique_count" in ctx.prof_stats()
+        assert launched == repeat
+        for g, w in zip(out.columns(), O.unique_adjacent(want)):
+            assert np.array_equal(g, w)
+        out.free()
+        out = L.join(R, 1)
+        out.drop_col(2)
+        assert not out.distinct
+        ctx.prof_reset()
+        out.unique_adjacent()
+        assert "unique_count" in ctx.prof_stats()
+        for g, w in zip(out.columns(), O.unique_adjacent(want[:2])):
+            assert np.array_equal(g, w)
+        for x in (L, R, out):
+            x.free()
+    finally:
+        ctx.prof(False)
+        idx.free()

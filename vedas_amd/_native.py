@@ -110,6 +110,7 @@ EXPORTS = {
                                           C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "vd_join": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                           C.POINTER(C.c_void_p)]),
+    "vd_rel_distinct": (C.c_int, [C.c_void_p]),
     "vd_join_driven_applies": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                          C.c_uint32]),
     "vd_join_capped": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_uint64,
@@ -325,6 +326,11 @@ class Rel:
         if self.ctx.L.vd_rel_key_range(self.h, C.byref(lo), C.byref(hi)):
             return lo.value, hi.value
         return None
+
+    @property
+    def distinct(self) -> bool:
+        """The rows are known to be pairwise distinct."""
+        return bool(self.ctx.L.vd_rel_distinct(self.h))
 
     def is_view(self, i: int) -> bool:
         """Column i reads index storage in place (read-only)."""

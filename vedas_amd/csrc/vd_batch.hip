@@ -556,6 +556,7 @@ void select_multi(vd_ctx *ctx, const vd_index *idx, const vd_select_job_t *jobs,
             if (jobs[j].cols == 3) vc[nc++] = pa.y + b;
             vc[nc++] = pa.z + b;
             outs[j] = rel_view(nc, e - b, vc, idx->hold);
+            outs[j]->distinct = idx->distinct();
             if (pa.bounded) {
                 if (jobs[j].cols == 3) rel_set_bound(outs[j], 0, pa.ymax);
                 rel_set_bound(outs[j], nc - 1, pa.zmax);
@@ -633,6 +634,7 @@ void join_multi_chunk(vd_ctx *ctx, const vd_rel *const *L, const vd_rel *const *
     try {
         for (int j = 0; j < np; ++j) {
             outs[j] = rel_new(ctx, L[j]->ncols + R[j]->ncols - 1, tot[j]);
+            outs[j]->distinct = L[j]->distinct && R[j]->distinct;
             for (int c = 0; c < outs[j]->ncols; ++c) {
                 hp[j].oc[c] = outs[j]->cols[c];
                 const bool from_l = c < L[j]->ncols;
@@ -775,8 +777,10 @@ void sort_col0_range(vd_ctx *ctx, vd_rel *const *rels, int n) {
 
 void unique_multi(vd_ctx *ctx, vd_rel *const *rels_in, int n_in) {
     std::vector<vd_rel *> rels;
+    // (rows known to be distinct have no adjacent duplicates)
     for (int j = 0; j < n_in; ++j)
-        if (rels_in[j] && rels_in[j]->nrows > 1) rels.push_back(rels_in[j]);
+        if (rels_in[j] && rels_in[j]->nrows > 1 && !rels_in[j]->distinct)
+            rels.push_back(rels_in[j]);
     const int n = (int)rels.size();
     if (n == 0) return;
     std::vector<SortRelDev> hs(n);

@@ -431,6 +431,7 @@ uint64_t vd_rel_nrows(const vd_rel *rel) { return rel ? rel->nrows : 0; }
 uint32_t *vd_rel_col(vd_rel *rel, int c) {
     if (!rel || c < 0 || c >= rel->ncols) return nullptr;
     rel->bounded &= ~(1u << c);  // the caller may write through the pointer
+    rel->distinct = false;
     return rel->cols[c];
 }
 
@@ -454,6 +455,7 @@ int vd_rel_drop_col(vd_ctx *ctx, vd_rel *rel, int i) {
         return VD_EINVAL;
     if (!vd::rel_borrowed(rel, i)) vd::dfree(ctx, rel->cols[i]);
     if (i == 0) rel->key_range = false;
+    rel->distinct = false;  // rows may agree on the columns left
     for (int c = i; c + 1 < rel->ncols; ++c) {
         rel->cols[c] = rel->cols[c + 1];
         rel->colmax[c] = rel->colmax[c + 1];
@@ -486,6 +488,8 @@ int vd_rel_key_range(const vd_rel *rel, uint32_t *lo, uint32_t *hi) {
     if (hi) *hi = rel->key_hi;
     return 1;
 }
+
+int vd_rel_distinct(const vd_rel *rel) { return rel && rel->distinct ? 1 : 0; }
 
 int vd_rel_is_view(const vd_rel *rel, int c) {
     if (!rel || c < 0 || c >= rel->ncols) return 0;

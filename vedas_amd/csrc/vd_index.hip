@@ -153,14 +153,14 @@ struct SplitRows {
 };
 
 // Columns Y and Z; per-block counts of L1 heads (X changes) and L2 heads
-// ((X,Y) changes).
+// ((X,Y) changes); rows equal to the row before them added to *dups.
 template <typename R>
 __global__ __launch_bounds__(UN_BLOCK) void k_unpack_count(
     R rows, uint64_t n, uint32_t *__restrict__ ycol, uint32_t *__restrict__ zcol,
-    uint32_t *__restrict__ cnt_l1, uint32_t *__restrict__ cnt_l2) {
+    uint32_t *__restrict__ cnt_l1, uint32_t *__restrict__ cnt_l2, uint64_t *dups) {
     __shared__ uint32_t s_tmp[UN_BLOCK / 64];
     const uint64_t base = (uint64_t)blockIdx.x * UN_TILE;
-    uint32_t c1 = 0, c2 = 0;
+    uint32_t c1 = 0, c2 = 0, c3 = 0;
 #pragma unroll
     for (int j = 0; j < UN_ITEMS; ++j) {
         const uint64_t i = base + (uint64_t)j * UN_BLOCK + threadIdx.x;
@@ -174,14 +174,17 @@ __global__ __launch_bounds__(UN_BLOCK) void k_unpack_count(
                 const Row q = rows.at(i - 1);
                 c1 += r.x != q.x;
                 c2 += r.xy != q.xy;
+                c3 += r.xy == q.xy && r.z == q.z;
             }
         }
     }
     const uint32_t t1 = dev::block_reduce<UN_BLOCK>(c1, s_tmp);
     const uint32_t t2 = dev::block_reduce<UN_BLOCK>(c2, s_tmp);
+    const uint32_t t3 = dev::block_reduce<UN_BLOCK>(c3, s_tmp);
     if (threadIdx.x == 0) {
         cnt_l1[blockIdx.x] = t1;
         cnt_l2[blockIdx.x] = t2;
+        if (t3) atomicAdd((unsigned long long *)dups, (unsigned long long)t3);
     }
 }
 
@@ -684,19 +687,22 @@ void finish_perm(vd_ctx *ctx, vd_perm_arrays &pa, uint64_t n, const R &rows) {
     DBuf<uint32_t> c1(ctx, tiles), c2(ctx, tiles);
     DBuf<uint64_t> o1(ctx, tiles + 1), o2(ctx, tiles + 1);
     constexpr bool split = std::is_same<R, SplitRows>::value;
-    launch(ctx, "index_unpack", (double)n * (split ? 20.0 : 16.0), dim3((unsigned)tiles),
-           dim3(UN_BLOCK), 0, k_unpack_count<R>, rows, n, pa.y, pa.z, c1.p, c2.p);
-    scan_u32_to_u64(ctx, c1.p, o1.p, tiles, o1.p + tiles);
-    scan_u32_to_u64(ctx, c2.p, o2.p, tiles, o2.p + tiles);
-    uint64_t tot[2];
+    uint64_t tot[3];
     {
-        DBuf<uint64_t> t(ctx, 2);
+        DBuf<uint64_t> t(ctx, 3);  // L1 heads, L2 heads, repeated rows
+        fill_u64(ctx, t.p + 2, 0, 1);
+        launch(ctx, "index_unpack", (double)n * (split ? 20.0 : 16.0), dim3((unsigned)tiles),
+               dim3(UN_BLOCK), 0, k_unpack_count<R>, rows, n, pa.y, pa.z, c1.p, c2.p, t.p + 2);
+        scan_u32_to_u64(ctx, c1.p, o1.p, tiles, o1.p + tiles);
+        scan_u32_to_u64(ctx, c2.p, o2.p, tiles, o2.p + tiles);
         copy_d2d(ctx, t.p, o1.p + tiles, 8);
         copy_d2d(ctx, t.p + 1, o2.p + tiles, 8);
-        read_words(ctx, tot, t.p, 2);
+        read_words(ctx, tot, t.p, 3);
     }
     pa.n_l1 = tot[0];
     pa.n_l2 = tot[1];
+    pa.dups_known = true;
+    pa.dups = tot[2];
     pa.l1v = (uint32_t *)dalloc(ctx, pa.n_l1 * 4);
     pa.l1o = (uint32_t *)dalloc(ctx, pa.n_l1 * 4);
     pa.l2v = (uint32_t *)dalloc(ctx, pa.n_l2 * 4);
@@ -945,6 +951,8 @@ vd_rel *range_view(const vd_index *idx, const vd_perm_arrays &pa, int cols, uint
     if (cols & 1) vc[nc++] = pa.z + src;
     vd_rel *r = rel_view(nc, rows, vc, idx->hold);
     select_bounds(r, pa, cols);
+    // rows of distinct triples with X (and, for Z alone, Y) fixed
+    r->distinct = cols != 2 && idx->distinct();
     return r;
 }
 
@@ -962,10 +970,13 @@ vd_rel *select_pieces_rel(vd_ctx *ctx, const vd_index *idx, const vd_perm_arrays
         launch(ctx, "select_copy", (double)rows * 8.0 * ocols, dim3(stream_grid(rows)),
                dim3(256), 0, k_copy_pieces, dp, (const uint32_t *)pa.y, (const uint32_t *)pa.z,
                r->cols[0], ocols == 2 ? r->cols[1] : (uint32_t *)nullptr, cols);
+        r->distinct = cols != 2 && idx->distinct();
         if (distinct && rows > 1) {
             int rc = vd_unique_adjacent(ctx, r);
             if (rc != VD_OK) throw Error(rc, ctx->last_error);
         }
+        // Y alone, sorted, after the adjacent unique: every value once
+        if (distinct) r->distinct = true;
     } catch (...) {
         rel_free(ctx, r);
         throw;

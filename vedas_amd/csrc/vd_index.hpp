@@ -11,6 +11,9 @@ struct vd_perm_arrays {
     // Maxima of the Y and Z columns (the input columns' maxima), when known.
     bool bounded = false;
     uint32_t ymax = 0, zmax = 0;
+    // Rows equal to the row before them (repeated triples), when counted.
+    bool dups_known = false;
+    uint64_t dups = 0;
 };
 
 struct vd_index {
@@ -22,6 +25,13 @@ struct vd_index {
     // ranges of them without a copy share it, so the arrays outlive
     // vd_index_free until the last such relation is destroyed.
     std::shared_ptr<void> hold;
+    // Every triple occurs once (a built permutation counted no repeated
+    // row): selects then return rows that are pairwise distinct.
+    bool distinct() const {
+        for (const auto &pa : perm)
+            if (pa.built && pa.dups_known) return pa.dups == 0;
+        return false;
+    }
 };
 
 namespace vd {

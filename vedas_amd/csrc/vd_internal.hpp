@@ -137,6 +137,11 @@ struct vd_rel {
     // writable (vd_rel_col) loses its bound.
     uint32_t bounded = 0;
     uint32_t colmax[VD_MAX_COLS] = {};
+    // The rows are pairwise distinct: selects of an index without repeated
+    // triples, joins of distinct inputs (each output row is one pair of
+    // input rows), and anything that only reorders or drops rows of such a
+    // relation.  Dropping a column or handing one out writable clears it.
+    bool distinct = false;
 };
 
 namespace vd {

@@ -553,6 +553,7 @@ vd_rel *join_impl(vd_ctx *ctx, const vd_rel *L, const vd_rel *R,
     for (int c = 0; c < out_cols; ++c)
         rel_copy_bound(out, c, c < L->ncols ? L : R, c < L->ncols ? c : c - L->ncols + NK);
     out->key_range = true;  // output column 0 = left keys of matched rows
+    out->distinct = L->distinct && R->distinct;  // one row per pair of input rows
     out->key_lo = jc.key_lo;
     out->key_hi = jc.key_hi;
     if (total == 0) return out;
@@ -635,6 +636,17 @@ vd_rel *join_driven(vd_ctx *ctx, const vd_rel *D, int dkey, const vd_rel *Y, uin
     *key_lo = *key_hi = 0;
     auto new_out = [&](uint64_t rows) {
         vd_rel *o = rel_new(ctx, nout, rows);
+        // One row per (D row, Y row) pair: distinct when both inputs are and
+        // the output holds every column of D and every column of Y but its
+        // key (equal to D's key in every output row).
+        uint32_t dcov = 0, ycov = 0;
+        for (int c = 0; c < nout; ++c) {
+            if (src[c] >= 0) ycov |= 1u << src[c];
+            else dcov |= 1u << (-1 - src[c]);
+        }
+        const uint32_t dall = (D->ncols >= 32 ? ~0u : (1u << D->ncols) - 1u);
+        const uint32_t yall = (Y->ncols >= 32 ? ~0u : (1u << Y->ncols) - 1u) & ~1u;
+        o->distinct = D->distinct && Y->distinct && dcov == dall && (ycov | 1u) == (yall | 1u);
         for (int c = 0; c < nout; ++c) {
             if (src[c] >= 0) rel_copy_bound(o, c, Y, src[c]);
             else rel_copy_bound(o, c, D, -1 - src[c]);

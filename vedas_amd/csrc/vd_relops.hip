@@ -332,7 +332,7 @@ void sort_lex(vd_ctx *ctx, vd_rel *rel) {
 
 void unique_adjacent(vd_ctx *ctx, vd_rel *rel) {
     const uint64_t n = rel->nrows;
-    if (n <= 1) return;
+    if (n <= 1 || rel->distinct) return;  // distinct rows: nothing adjacent repeats
     Cols cols{};
     cols.n = rel->ncols;
     for (int c = 0; c < rel->ncols; ++c) cols.in[c] = rel->cols[c];
