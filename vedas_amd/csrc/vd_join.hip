@@ -231,15 +231,29 @@ __global__ __launch_bounds__(dev::TT_BLOCK) void k_join_scan_total(
     const uint64_t *__restrict__ ttot, const uint64_t *__restrict__ tkeys, uint64_t ntiles,
     uint64_t *__restrict__ tpre, Readback rb, const uint32_t *__restrict__ flag) {
     __shared__ uint64_t s_tmp[dev::TT_BLOCK / 64];
-    dev::scan_tile_totals_block(ttot, ntiles, tpre, s_tmp);
+    // Key bounds and the high halves of the totals first, from coalesced
+    // loads all in flight together (one round trip for up to TT_BLOCK *
+    // TT_PER tiles); the scan then reads the totals from the cache.
+    constexpr int PER = dev::TT_PER;
     uint32_t mn = ~0u, mx = 0;
     uint64_t wide = 0;  // high halves of the tile totals
-    for (uint64_t t = threadIdx.x; t < ntiles; t += dev::TT_BLOCK) {
-        const uint64_t w = tkeys[t];
-        mn = min(mn, (uint32_t)w);
-        mx = max(mx, (uint32_t)(w >> 32));
-        wide |= ttot[t] >> 32;
+    for (uint64_t b = 0; b < ntiles; b += (uint64_t)dev::TT_BLOCK * PER) {
+        uint64_t v[PER], w[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const uint64_t t = b + (uint64_t)j * dev::TT_BLOCK + threadIdx.x;
+            const bool in = t < ntiles;
+            v[j] = in ? ttot[t] : 0;
+            w[j] = in ? tkeys[t] : 0xFFFFFFFFull;  // (max 0, min ~0): no keys
+        }
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            wide |= v[j] >> 32;
+            mn = min(mn, (uint32_t)w[j]);
+            mx = max(mx, (uint32_t)(w[j] >> 32));
+        }
     }
+    dev::scan_tile_totals_block(ttot, ntiles, tpre, s_tmp);
     dev::block_min_max<dev::TT_BLOCK>(mn, mx, s_tmp);
     const int overflow = __syncthreads_or(wide != 0);
     if (threadIdx.x >= 64) return;
@@ -372,27 +386,39 @@ __global__ __launch_bounds__(JX_BLOCK) void k_join_expand(
 // absent key has an empty run).  A run's first row looks RT_WIN rows ahead:
 // a shorter run gets its entry in one 8-byte store; a longer one gets its
 // end from its last row, which sees the first row more than RT_WIN back.
+// A thread takes RT_ITEMS rows, a block RT_ITEMS * 256 consecutive rows,
+// and every load of a phase is issued before any is used (a grid-stride
+// loop would put two dependent round trips per row in each thread's way).
 constexpr int RT_WIN = 8;
+constexpr int RT_ITEMS = 4;
 __global__ __launch_bounds__(256) void k_run_table(const uint32_t *__restrict__ key, uint64_t n,
                                                    uint32_t kmax, uint2 *__restrict__ tab) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const uint32_t k = key[i];
-        if (k > kmax) continue;
-        if (i == 0 || key[i - 1] != k) {
+    const uint64_t base = (uint64_t)blockIdx.x * (256 * RT_ITEMS) + threadIdx.x;
+    uint32_t k[RT_ITEMS], prv[RT_ITEMS], nxt[RT_ITEMS];
+#pragma unroll
+    for (int j = 0; j < RT_ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)j * 256;
+        k[j] = i < n ? key[i] : 0u;
+        prv[j] = i < n && i > 0 ? key[i - 1] : ~k[j];
+        nxt[j] = i + 1 < n ? key[i + 1] : ~k[j];
+    }
+#pragma unroll
+    for (int j = 0; j < RT_ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)j * 256;
+        if (i >= n || k[j] > kmax) continue;
+        if (prv[j] != k[j]) {
             // the next RT_WIN keys, loaded together (the column is sorted:
             // the run ends at the first one that differs)
             uint32_t nx[RT_WIN];
 #pragma unroll
-            for (int w = 0; w < RT_WIN; ++w) nx[w] = i + 1 + w < n ? key[i + 1 + w] : ~k;
+            for (int w = 0; w < RT_WIN; ++w) nx[w] = i + 1 + w < n ? key[i + 1 + w] : ~k[j];
             uint64_t e = i + 1;
 #pragma unroll
-            for (int w = 0; w < RT_WIN; ++w) e += (nx[w] == k && e == i + 1 + w) ? 1 : 0;
-            if (e <= i + RT_WIN) tab[k] = make_uint2((uint32_t)i, (uint32_t)e);
-            else tab[k].x = (uint32_t)i;  // longer than RT_WIN rows
-        } else if ((i + 1 == n || key[i + 1] != k) && i >= (uint64_t)RT_WIN &&
-                   key[i - RT_WIN] == k) {
-            tab[k].y = (uint32_t)(i + 1);  // the last row of a run of > RT_WIN rows
+            for (int w = 0; w < RT_WIN; ++w) e += (nx[w] == k[j] && e == i + 1 + w) ? 1 : 0;
+            if (e <= i + RT_WIN) tab[k[j]] = make_uint2((uint32_t)i, (uint32_t)e);
+            else tab[k[j]].x = (uint32_t)i;  // longer than RT_WIN rows
+        } else if (nxt[j] != k[j] && i >= (uint64_t)RT_WIN && key[i - RT_WIN] == k[j]) {
+            tab[k[j]].y = (uint32_t)(i + 1);  // the last row of a run of > RT_WIN rows
         }
     }
 }
@@ -657,9 +683,8 @@ vd_rel *join_driven(vd_ctx *ctx, const vd_rel *D, int dkey, const vd_rel *Y, uin
 
     DBuf<uint2> tab(ctx, (uint64_t)kmax + 1);
     fill_u32(ctx, reinterpret_cast<uint32_t *>(tab.p), 0, 2 * ((uint64_t)kmax + 1));
-    launch(ctx, "run_table", 4.0 * (double)ny, dim3((unsigned)std::min<uint64_t>(
-               ceil_div(ny, 256), 16384)), dim3(256), 0, k_run_table,
-           (const uint32_t *)Y->cols[0], ny, kmax, tab.p);
+    launch(ctx, "run_table", 4.0 * (double)ny, dim3((unsigned)ceil_div(ny, 256 * RT_ITEMS)),
+           dim3(256), 0, k_run_table, (const uint32_t *)Y->cols[0], ny, kmax, tab.p);
     DBuf<uint32_t> lower(ctx, nd), off(ctx, nd);
     const uint64_t jb_tiles = ceil_div(nd, JB_TILE);
     // tile totals, tile prefixes (+ the total), tile key bounds, flag word
