@@ -233,21 +233,42 @@ def batch_bench(eng, texts, reps=2, budget=96 << 30):
         else:
             prof = kernel_table(ctx.prof_stats())
             ctx.prof(False)
-    # equality of results, checked outside the timed runs
-    same = 0
-    for g in groups:
-        res, _ = eng.batch(g)
-        for t, r in zip(g, res):
-            a = eng.query(t)
-            same += result_digest(a.headers(), a.columns()) == result_digest(r.headers(),
-                                                                              r.columns())
-            a.free()
-            r.free()
-    pt, bt = min(plain_t), min(batch_t)
+    # The same batches captured as HIP graphs (vdq_graph_create) and
+    # replayed: every kernel of each batch again, no host work between them.
+    graphs = [eng.graph(g) for g in groups]
+    graph_t = []
+    try:
+        for _ in range(reps + 1):
+            t0 = time.perf_counter()
+            for gr in graphs:
+                gr.replay()
+            graph_t.append(time.perf_counter() - t0)
+        # equality of results, checked outside the timed runs: every query
+        # alone, batched and from the graph's last replay
+        same = same_graph = 0
+        for g, gr in zip(groups, graphs):
+            res, _ = eng.batch(g)
+            for i, (t, r) in enumerate(zip(g, res)):
+                a = eng.query(t)
+                d = result_digest(r.headers(), r.columns())
+                same += result_digest(a.headers(), a.columns()) == d
+                gres = gr.result(i)
+                same_graph += result_digest(gres.headers(), gres.columns()) == d
+                a.free()
+                r.free()
+    finally:
+        for gr in graphs:
+            gr.free()
+    pt, bt, gt = min(plain_t), min(batch_t), min(graph_t[1:])
     return {"queries": len(texts), "batches": len(groups), "plain_ms": pt * 1e3,
             "batched_ms": bt * 1e3, "plain_qps": len(texts) / pt, "batched_qps": len(texts) / bt,
-            "speedup": pt / bt, "rounds": rounds, "calls": calls, "join_rows": join_rows,
-            "results_equal": same, "kernels": prof}
+            "speedup": pt / bt, "graph_ms": gt * 1e3, "graph_qps": len(texts) / gt,
+            "graph_speedup_vs_batched": bt / gt, "graph_speedup_vs_plain": pt / gt,
+            "graph_note": "each batch captured once as a HIP graph (two runs: one logs the "
+                          "read-back sizes, one is captured), then replayed: the same kernels "
+                          "over the same store, no host work between them",
+            "rounds": rounds, "calls": calls, "join_rows": join_rows,
+            "results_equal": same, "graph_results_equal": same_graph, "kernels": prof}
 
 
 def serve_bench(eng, texts, qps, seed=1, slots=4, max_batch=128, budget=96 << 30,
@@ -665,7 +686,8 @@ def main():
         for r in eng.batch(mtexts[:16])[0]:  # warm-up
             r.free()
         batch = batch_bench(eng, mtexts)
-        log(f"[bench] mix: plain {batch['plain_qps']:.0f} q/s, batched {batch['batched_qps']:.0f} q/s")
+        log(f"[bench] mix: plain {batch['plain_qps']:.0f} q/s, batched {batch['batched_qps']:.0f} q/s, "
+            f"graph replay {batch['graph_qps']:.0f} q/s")
 
     cpu = None
     if not args.no_cpu:

@@ -37,6 +37,7 @@ extern "C" {
 typedef struct vd_ctx   vd_ctx;    /* device + stream + allocator + profiler */
 typedef struct vd_rel   vd_rel;    /* column-store relation of uint32 IDs    */
 typedef struct vd_index vd_index;  /* HBM-resident triple permutations       */
+typedef struct vd_graph vd_graph;  /* a captured run, replayable as one launch */
 
 /* Permutations.  XYZ = sort order; X is implicit through the L1 index, Y and
  * Z are stored as columns (the reference's xy_data / xyz_data arrays,
@@ -89,6 +90,28 @@ int  vd_ctx_mem_peak(vd_ctx *ctx, uint64_t *peak, int reset);
  * expansion block), "join_rcap" (right keys staged in LDS per bounds
  * block).  Returns VD_EINVAL for an unknown key or an unsupported value. */
 int  vd_ctx_tune(vd_ctx *ctx, const char *key, int value);
+
+/* ---- graph capture (no counterpart in the reference) ---------------------
+ * A run of calls on a context can be captured into one HIP graph and
+ * relaunched without host work between its kernels.  Sizes the host reads
+ * back decide allocations and launch shapes, so a run is captured in two
+ * steps: run it once with vd_ctx_record(ctx, 1) (every read-back word is
+ * logged), then again between vd_ctx_capture_begin and vd_ctx_capture_end
+ * (nothing executes: read-backs return the logged words, in order, and the
+ * kernels and copies go into the graph).  The second run must make the same
+ * calls on the same data.  Device memory the capture allocates or frees
+ * belongs to the graph until vd_graph_free (relations made during the
+ * capture read and hold it; destroying them frees nothing); host data
+ * uploaded during the capture is copied into buffers of the graph.
+ * vd_copy_h2d / vd_copy_d2h are refused during a capture.  Every launch of
+ * the graph redoes the captured work and rewrites those relations' columns.
+ * vd_ctx_capture_end with out == NULL discards the capture (use it when the
+ * captured run fails).  Free every graph of a context before the context. */
+int  vd_ctx_record(vd_ctx *ctx, int on);
+int  vd_ctx_capture_begin(vd_ctx *ctx);
+int  vd_ctx_capture_end(vd_ctx *ctx, vd_graph **out);
+int  vd_graph_launch(vd_ctx *ctx, vd_graph *g);
+int  vd_graph_free(vd_ctx *ctx, vd_graph *g);
 
 /* ---- device buffers ---------------------------------------------------- */
 int  vd_dev_alloc(vd_ctx *ctx, size_t bytes, void **dptr);

@@ -134,6 +134,24 @@ typedef struct {
 } vdq_batch_stats_t;
 int vdq_batch(vdq_engine *e, const char *const *texts, int n, vdq_result **outs,
               vdq_batch_stats_t *stats);
+
+/* A batch captured as one HIP graph (no counterpart in the reference,
+ * whose QueryPlan::execute issues every operator from the host,
+ * QueryPlan.cu:160-205): vdq_batch's work is run once with every size the
+ * host reads back logged, run again under stream capture with those sizes
+ * (vd_ctx_record / vd_ctx_capture_*), and the graph is launched once to
+ * fill the results.  vdq_graph_replay relaunches it and waits: the same
+ * kernels over the same store with no host work between them, the results
+ * rewritten in place.  For a batch that is run again and again (a fixed
+ * set of queries over a store that does not change); a replay after the
+ * store changed is wrong.  vdq_graph_result gives result i, owned by the
+ * graph (do not free it; valid until vdq_graph_free).  The graph holds the
+ * batch's device memory; free it before its engine. */
+typedef struct vdq_graph vdq_graph;
+int vdq_graph_create(vdq_engine *e, const char *const *texts, int n, vdq_graph **out);
+int vdq_graph_replay(vdq_graph *g);
+vdq_result *vdq_graph_result(vdq_graph *g, int i);
+int vdq_graph_free(vdq_graph *g);
 /* Serving (no counterpart in the reference): requests arrive open loop at
  * arrival_ms (relative to the call), are admitted while their estimated
  * peak memory fits mem_budget, and run in continuous batches on `slots`

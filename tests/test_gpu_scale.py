@@ -167,7 +167,8 @@ def test_cfg2_join_set_100m(store, gold, arrays):
 @pytest.mark.timeout(900)
 def test_cfg3_batched_mix_100m(store, gold, arrays):
     """The 256-query mix (stars of 2-5 patterns, chains of 2-4, a quarter
-    with a bound object) run batched and one query at a time: every result
+    with a bound object) run batched, batched from a captured HIP graph
+    replayed, and one query at a time: every result
     equals the oracle's (its digests of the whole mix, and a live plan walk
     of a sample on the checked arrays).  Each query's admission estimate is
     compared with its measured peak of device memory (64 MiB floor): none
@@ -177,12 +178,23 @@ def test_cfg3_batched_mix_100m(store, gold, arrays):
     s, p, o = store["s"], store["p"], store["o"]
     mix = W.query_mix(W.N_MIX, W.SEED_MIX, W.STORE_100M, s, p, o)
     texts = [query_text(pats, node) for _, pats, node in mix]
-    batched = []
+    batched, graphed = [], []
     for g in W.chunk_by_memory(eng, texts, 96 << 30):
         res, _ = eng.batch(g)
         for r in res:
             batched.append(O.bag_digest(r.headers(), r.columns()))
             r.free()
+        # the same batch captured as a HIP graph, replayed twice
+        gr = eng.graph(g)
+        try:
+            gr.replay()
+            gr.replay()
+            for i in range(len(g)):
+                r = gr.result(i)
+                graphed.append(O.bag_digest(r.headers(), r.columns()))
+        finally:
+            gr.free()
+    assert graphed == batched
     # one at a time, with each query's measured peak of device memory beside
     # its admission estimate
     ctx = store["ctx"]

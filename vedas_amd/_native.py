@@ -111,6 +111,11 @@ EXPORTS = {
     "vd_join": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                           C.POINTER(C.c_void_p)]),
     "vd_rel_distinct": (C.c_int, [C.c_void_p]),
+    "vd_ctx_record": (C.c_int, [C.c_void_p, C.c_int]),
+    "vd_ctx_capture_begin": (C.c_int, [C.c_void_p]),
+    "vd_ctx_capture_end": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "vd_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "vd_graph_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "vd_join_driven_applies": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                          C.c_uint32]),
     "vd_join_capped": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_uint64,
@@ -223,6 +228,25 @@ class Ctx:
 
     def sync(self) -> None:
         self.check(self.L.vd_ctx_sync(self.h))
+
+    def record(self, on: bool) -> None:
+        """Log every read-back word of the following calls (vd_ctx_record)."""
+        self.check(self.L.vd_ctx_record(self.h, int(on)))
+
+    def capture_begin(self) -> None:
+        self.check(self.L.vd_ctx_capture_begin(self.h))
+
+    def capture_end(self, keep: bool = True):
+        """The captured calls as a graph handle (or None with keep=False)."""
+        g = C.c_void_p()
+        self.check(self.L.vd_ctx_capture_end(self.h, C.byref(g) if keep else None))
+        return g if keep else None
+
+    def graph_launch(self, g) -> None:
+        self.check(self.L.vd_graph_launch(self.h, g))
+
+    def graph_free(self, g) -> None:
+        self.check(self.L.vd_graph_free(self.h, g))
 
     def tune(self, key: str, value: int) -> None:
         self.check(self.L.vd_ctx_tune(self.h, key.encode(), value))

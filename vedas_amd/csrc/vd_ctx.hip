@@ -36,6 +36,16 @@ static void release_cached(vd_ctx *ctx) {
 void *dalloc(vd_ctx *ctx, size_t bytes) {
     std::lock_guard<std::recursive_mutex> lk(ctx->alloc_mu);
     const size_t c = size_class(bytes);
+    if (ctx->capturing) {
+        // a block the capture freed before: stream order within the graph
+        auto ci = ctx->cap_free.find(c);
+        if (ci != ctx->cap_free.end()) {
+            void *q = ci->second;
+            ctx->cap_free.erase(ci);
+            ctx->live_blocks[q] = c;
+            return q;
+        }
+    }
     auto it = ctx->free_blocks.find(c);
     void *p = nullptr;
     if (it != ctx->free_blocks.end()) {
@@ -60,6 +70,7 @@ void *dalloc(vd_ctx *ctx, size_t bytes) {
     ctx->live_blocks[p] = c;
     ctx->bytes_live += c;
     if (ctx->bytes_live > ctx->bytes_peak) ctx->bytes_peak = ctx->bytes_live;
+    if (ctx->capturing) ctx->cap_blocks.emplace_back(p, c);
     return p;
 }
 
@@ -70,6 +81,16 @@ void dfree(vd_ctx *ctx, void *p) {
     if (it == ctx->live_blocks.end()) return;  // not ours: ignore
     const size_t c = it->second;
     ctx->live_blocks.erase(it);
+    if (ctx->capturing) {
+        // The graph may read the block at any replay: it stays out of the
+        // pool.  A block the capture allocated may serve a later
+        // allocation of the same capture (stream order holds in the graph).
+        bool mine = false;
+        for (const auto &b : ctx->cap_blocks) mine = mine || b.first == p;
+        if (mine) ctx->cap_free.emplace(c, p);
+        else ctx->cap_blocks.emplace_back(p, c);
+        return;
+    }
     ctx->bytes_live -= c;
     ctx->free_blocks.emplace(c, p);
     ctx->bytes_cached += c;
@@ -118,6 +139,7 @@ void prof_collect(vd_ctx *ctx, bool done_only) {
 // computes it.  Polling the stream for a while before the blocking wait
 // keeps the host thread's wake-up latency out of each of them.
 void sync(vd_ctx *ctx) {
+    if (ctx->capturing) return;  // nothing runs until the graph is launched
     constexpr double kSpinMicros = 200.0;
     const auto t0 = std::chrono::steady_clock::now();
     for (;;) {
@@ -137,7 +159,19 @@ void sync(vd_ctx *ctx) {
 // ---- small read-backs -------------------------------------------------------------
 Readback rb_next(vd_ctx *ctx) { return Readback{ctx->d_rb, ++ctx->rb_seq}; }
 
+// Words a captured run reads back: the recorded run's, in the same order.
+static void rb_replay(vd_ctx *ctx, uint64_t *dst, size_t n) {
+    VD_REQUIRE(ctx->rb_pos + n <= ctx->rb_log.size(), VD_EINVAL,
+               "captured run reads back more than the recorded run did");
+    std::memcpy(dst, ctx->rb_log.data() + ctx->rb_pos, n * sizeof(uint64_t));
+    ctx->rb_pos += n;
+}
+
 void rb_wait(vd_ctx *ctx, const Readback &rb, uint64_t *dst, int n) {
+    if (ctx->rb_mode == 2) {
+        rb_replay(ctx, dst, (size_t)n);
+        return;
+    }
     // A read-back usually follows the kernels it waits for by well under a
     // millisecond; after kSpinMicros the thread blocks on the stream, which
     // also surfaces a failed kernel (its sequence word never comes).
@@ -159,6 +193,7 @@ void rb_wait(vd_ctx *ctx, const Readback &rb, uint64_t *dst, int n) {
         }
     }
     std::memcpy(dst, ctx->h_rb, (size_t)n * sizeof(uint64_t));
+    if (ctx->rb_mode == 1) ctx->rb_log.insert(ctx->rb_log.end(), dst, dst + n);
     if (!ctx->pending.empty()) prof_collect(ctx, true);
 }
 
@@ -183,6 +218,15 @@ uint64_t read_u64(vd_ctx *ctx, const uint64_t *dptr) {
 
 void upload(vd_ctx *ctx, void *dst, const void *src, size_t bytes) {
     if (bytes == 0) return;
+    if (ctx->capturing) {
+        // the graph copies from a host buffer of its own at every replay
+        void *h = nullptr;
+        VD_HIP(hipHostMalloc(&h, bytes, hipHostMallocDefault));
+        ctx->cap_host.push_back(h);
+        std::memcpy(h, src, bytes);
+        VD_HIP(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, ctx->stream));
+        return;
+    }
     if (ctx->stage_pending) {
         VD_HIP(hipEventSynchronize(ctx->stage_done));
         ctx->stage_pending = false;
@@ -212,8 +256,20 @@ void download(vd_ctx *ctx, void *dst, const void *src, size_t bytes) {
         std::memcpy(dst, w, bytes);
         return;
     }
+    const size_t nw = (bytes + sizeof(uint64_t) - 1) / sizeof(uint64_t);
+    if (ctx->rb_mode == 2) {
+        std::vector<uint64_t> w(nw);
+        rb_replay(ctx, w.data(), nw);
+        std::memcpy(dst, w.data(), bytes);
+        return;
+    }
     VD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
     sync(ctx);
+    if (ctx->rb_mode == 1) {
+        std::vector<uint64_t> w(nw, 0);
+        std::memcpy(w.data(), dst, bytes);
+        ctx->rb_log.insert(ctx->rb_log.end(), w.begin(), w.end());
+    }
 }
 
 void copy_d2d(vd_ctx *ctx, void *dst, const void *src, size_t bytes) {
@@ -327,6 +383,112 @@ const char *vd_ctx_last_error(const vd_ctx *ctx) {
     return ctx ? ctx->last_error.c_str() : "null context";
 }
 
+// A captured run: the instantiated graph and the memory it reads and writes
+// (device blocks from the context's pool, pinned host sources of copies).
+struct vd_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<std::pair<void *, size_t>> blocks;
+    std::vector<void *> host;
+};
+
+int vd_ctx_record(vd_ctx *ctx, int on) {
+    if (!ctx) return VD_EINVAL;
+    if (ctx->capturing) return VD_EINVAL;
+    ctx->rb_mode = on ? 1 : 0;
+    if (on) ctx->rb_log.clear();
+    return VD_OK;
+}
+
+int vd_ctx_capture_begin(vd_ctx *ctx) {
+    if (!ctx) return VD_EINVAL;
+    VD_TRY(ctx, {
+        VD_REQUIRE(!ctx->capturing, VD_EINVAL, "a capture is already open");
+        vd::sync(ctx);
+        VD_HIP(hipSetDevice(ctx->device));
+        VD_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+        ctx->capturing = true;
+        ctx->rb_mode = 2;
+        ctx->rb_pos = 0;
+        ctx->prof_before_capture = ctx->prof;
+        ctx->prof = false;
+        ctx->cap_blocks.clear();
+        ctx->cap_free.clear();
+        ctx->cap_host.clear();
+    });
+}
+
+int vd_ctx_capture_end(vd_ctx *ctx, vd_graph **out) {
+    if (!ctx) return VD_EINVAL;
+    if (out) *out = nullptr;
+    if (!ctx->capturing) return VD_EINVAL;
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    ctx->capturing = false;
+    ctx->prof = ctx->prof_before_capture;
+    const bool all_read = ctx->rb_pos == ctx->rb_log.size();
+    ctx->rb_mode = 0;
+    auto *g = new vd_graph();
+    g->graph = graph;
+    {
+        std::lock_guard<std::recursive_mutex> lk(ctx->alloc_mu);
+        for (const auto &b : ctx->cap_blocks) {
+            ctx->live_blocks.erase(b.first);  // the graph owns it now
+            ctx->bytes_live -= b.second;
+        }
+        g->blocks.swap(ctx->cap_blocks);
+        ctx->cap_free.clear();
+    }
+    g->host.swap(ctx->cap_host);
+    int rc = VD_OK;
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->last_error = std::string("stream capture failed: ") + hipGetErrorString(e);
+        rc = VD_EHIP;
+    } else if (!all_read) {
+        ctx->last_error = "captured run read back fewer words than the recorded run";
+        rc = VD_EINVAL;
+    } else if (out) {
+        const hipError_t ie = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->last_error = std::string("graph instantiation failed: ") + hipGetErrorString(ie);
+            rc = VD_EHIP;
+        }
+    }
+    if (rc != VD_OK || !out) {
+        vd_graph_free(ctx, g);
+        return rc;
+    }
+    *out = g;
+    return VD_OK;
+}
+
+int vd_graph_launch(vd_ctx *ctx, vd_graph *g) {
+    if (!ctx || !g || !g->exec) return VD_EINVAL;
+    VD_TRY(ctx, {
+        VD_REQUIRE(!ctx->capturing, VD_EINVAL, "graph launch during a capture");
+        VD_HIP(hipGraphLaunch(g->exec, ctx->stream));
+    });
+}
+
+int vd_graph_free(vd_ctx *ctx, vd_graph *g) {
+    if (!ctx || !g) return VD_EINVAL;
+    if (!ctx->capturing) (void)hipStreamSynchronize(ctx->stream);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    {
+        std::lock_guard<std::recursive_mutex> lk(ctx->alloc_mu);
+        for (const auto &b : g->blocks) {
+            ctx->free_blocks.emplace(b.second, b.first);
+            ctx->bytes_cached += b.second;
+        }
+    }
+    for (void *h : g->host) (void)hipHostFree(h);
+    delete g;
+    return VD_OK;
+}
+
 int vd_ctx_sync(vd_ctx *ctx) {
     if (!ctx) return VD_EINVAL;
     VD_TRY(ctx, vd::sync(ctx));
@@ -396,6 +558,7 @@ int vd_copy_h2d(vd_ctx *ctx, void *dst, const void *src, size_t bytes) {
     if (!ctx) return VD_EINVAL;
     if (bytes == 0) return VD_OK;
     VD_TRY(ctx, {
+        VD_REQUIRE(!ctx->capturing, VD_EINVAL, "host copies are not captured");
         VD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
         vd::sync(ctx);
     });
@@ -405,6 +568,7 @@ int vd_copy_d2h(vd_ctx *ctx, void *dst, const void *src, size_t bytes) {
     if (!ctx) return VD_EINVAL;
     if (bytes == 0) return VD_OK;
     VD_TRY(ctx, {
+        VD_REQUIRE(!ctx->capturing, VD_EINVAL, "host copies are not captured");
         VD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
         vd::sync(ctx);
     });

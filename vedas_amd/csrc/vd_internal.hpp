@@ -98,6 +98,21 @@ struct vd_ctx {
     hipEvent_t stage_done = nullptr;
     bool stage_pending = false;
 
+    // Graph recording (vd_ctx_record, vd_ctx_capture_begin/end): a recorded
+    // run logs every word the host reads back; the same run captured into
+    // a HIP graph gets those words from the log instead of waiting (nothing
+    // executes during a capture).
+    int rb_mode = 0;  // 0: live, 1: live and logged, 2: answered from rb_log
+    std::vector<uint64_t> rb_log;
+    size_t rb_pos = 0;
+    bool capturing = false;
+    bool prof_before_capture = false;
+    // Device blocks a capture allocates or frees (its graph owns them) and
+    // the capture's own free list; pinned host sources of its uploads.
+    std::vector<std::pair<void *, size_t>> cap_blocks;
+    std::multimap<size_t, void *> cap_free;
+    std::vector<void *> cap_host;
+
     // Profiling (of the launches whose label is in prof_only, or of all
     // launches when it is empty).
     bool prof = false;

@@ -94,6 +94,10 @@ def lib() -> C.CDLL:
             "vdq_estimate": (i, [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
             "vdq_autotune": (i, [vp, C.POINTER(C.c_char_p), i, i, C.POINTER(TuneResult)]),
             "vdq_plan": (i, [C.c_char_p, C.POINTER(u64), C.c_char_p, C.c_size_t]),
+            "vdq_graph_create": (i, [vp, C.POINTER(C.c_char_p), i, C.POINTER(vp)]),
+            "vdq_graph_replay": (i, [vp]),
+            "vdq_graph_result": (vp, [vp, i]),
+            "vdq_graph_free": (i, [vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(l, name)
@@ -148,6 +152,40 @@ class Result:
     def free(self):
         if self.h:
             lib().vdq_result_free(self.h)
+            self.h = None
+
+
+class GraphResult(Result):
+    """A result owned by a Graph: rewritten by every replay, freed with it."""
+
+    def free(self):
+        self.h = None
+
+
+class Graph:
+    """A batch captured as one HIP graph (vdq_graph_create): replay() reruns
+    every kernel of the batch over the same store as one launch and
+    rewrites the results in place."""
+
+    def __init__(self, engine: "Engine", texts):
+        n = len(texts)
+        arr = (C.c_char_p * max(1, n))(*[t.encode() for t in texts])
+        h = C.c_void_p()
+        _check(lib().vdq_graph_create(engine.h, arr, n, C.byref(h)))
+        self.h, self.n = h, n
+
+    def replay(self) -> None:
+        _check(lib().vdq_graph_replay(self.h))
+
+    def result(self, i: int) -> GraphResult:
+        p = lib().vdq_graph_result(self.h, i)
+        if not p:
+            raise IndexError(i)
+        return GraphResult(C.c_void_p(p))
+
+    def free(self) -> None:
+        if self.h:
+            _check(lib().vdq_graph_free(self.h))
             self.h = None
 
 
@@ -220,6 +258,10 @@ class Engine:
         _check(lib().vdq_batch(self.h, arr, n, outs, C.byref(st)))
         return ([Result(C.c_void_p(outs[i])) for i in range(n)],
                 {k: getattr(st, k) for k, _ in BatchStats._fields_})
+
+    def graph(self, texts) -> Graph:
+        """The batch of `texts` captured as one replayable HIP graph."""
+        return Graph(self, texts)
 
     def serve(self, texts, arrival_ms, slots=4, max_batch=64, mem_budget=64 << 30,
               plain=False, keep_results=False, min_fill=0, max_wait_ms=0.0):

@@ -476,6 +476,95 @@ int vdq_batch(vdq_engine *e, const char *const *texts, int n, vdq_result **outs,
     });
 }
 
+// A batch captured as one HIP graph over the engine's context.
+struct vdq_graph {
+    vdq_engine *e = nullptr;
+    vd_graph *g = nullptr;
+    std::vector<std::unique_ptr<vdq_result>> results;  // rewritten by every launch
+    ~vdq_graph() {
+        results.clear();  // relations first: their columns are the graph's
+        if (g) vd_graph_free(e->ctx->raw(), g);
+    }
+};
+
+int vdq_graph_create(vdq_engine *e, const char *const *texts, int n, vdq_graph **out) {
+    if (!e || !out || n < 0 || (n && !texts)) return fail(VD_EINVAL, "null argument");
+    if (!e->storage) return fail(VD_EINVAL, "engine has no store loaded");
+    *out = nullptr;
+    return guarded([&] {
+        std::vector<std::unique_ptr<ParsedQuery>> pqs;
+        std::vector<BatchQuery> bq(n);
+        for (int i = 0; i < n; ++i) {
+            pqs.emplace_back(new ParsedQuery());
+            parse_query(texts[i], *pqs.back());
+            ensure_plan(e, *pqs.back());
+            bq[i].patterns = &pqs.back()->pats;
+            bq[i].selected = pqs.back()->selected;
+            bq[i].root = pqs.back()->root;
+        }
+        VedasContext *ctx = e->ctx.get();
+        vd_ctx *c = ctx->raw();
+        // 1. the batch once, every read-back logged
+        ctx->check(vd_ctx_record(c, 1), "vd_ctx_record");
+        try {
+            std::vector<SparqlResult> first;
+            BatchExecutor bx(e->storage.get(), ctx);
+            bx.run(bq, first, nullptr);
+            ctx->sync();
+        } catch (...) {
+            vd_ctx_record(c, 0);
+            throw;
+        }
+        ctx->check(vd_ctx_record(c, 0), "vd_ctx_record");
+        // 2. the same batch captured (read-backs answered from the log)
+        std::unique_ptr<vdq_graph> g(new vdq_graph());
+        g->e = e;
+        std::vector<SparqlResult> results;
+        ctx->check(vd_ctx_capture_begin(c), "vd_ctx_capture_begin");
+        try {
+            BatchExecutor bx(e->storage.get(), ctx);
+            bx.run(bq, results, nullptr);
+        } catch (...) {
+            results.clear();
+            vd_ctx_capture_end(c, nullptr);
+            throw;
+        }
+        const int rc = vd_ctx_capture_end(c, &g->g);
+        if (rc != VD_OK) {
+            results.clear();
+            ctx->check(rc, "vd_ctx_capture_end");
+        }
+        for (int i = 0; i < n; ++i) {
+            g->results.emplace_back(new vdq_result());
+            g->results.back()->result.setResult(results[i].release());
+            fill_stats(g->results.back().get(), nullptr);
+        }
+        // 3. the first launch computes the results
+        ctx->check(vd_graph_launch(c, g->g), "vd_graph_launch");
+        ctx->sync();
+        *out = g.release();
+    });
+}
+
+int vdq_graph_replay(vdq_graph *g) {
+    if (!g || !g->g) return fail(VD_EINVAL, "null graph");
+    return guarded([&] {
+        VedasContext *ctx = g->e->ctx.get();
+        ctx->check(vd_graph_launch(ctx->raw(), g->g), "vd_graph_launch");
+        ctx->sync();
+    });
+}
+
+vdq_result *vdq_graph_result(vdq_graph *g, int i) {
+    if (!g || i < 0 || i >= (int)g->results.size()) return nullptr;
+    return g->results[i].get();
+}
+
+int vdq_graph_free(vdq_graph *g) {
+    if (!g) return fail(VD_EINVAL, "null graph");
+    return guarded([&] { delete g; });
+}
+
 int vdq_serve(vdq_engine *e, const char *const *texts, const double *arrival_ms, int n,
               const vdq_serve_cfg_t *cfg, vdq_serve_rec_t *recs, vdq_result **outs) {
     if (!e || n < 0 || !cfg || (n && (!texts || !arrival_ms || !recs)))
