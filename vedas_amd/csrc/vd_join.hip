@@ -283,8 +283,15 @@ __global__ __launch_bounds__(JX_BLOCK) void k_join_expand(
     JoinOff off, const uint32_t *__restrict__ lower,
     const uint64_t *__restrict__ mp, uint64_t nl, uint64_t total, OutCols oc) {
     constexpr int ITEMS = NV / JX_BLOCK;
-    __shared__ uint64_t s_off[NV + 1];
+    static_assert(NV < 65536, "offsets within a diagonal in 16 bits");
+    // Offsets of the diagonal's left rows relative to its first output b0
+    // (rows after the first start at or after b0, so they fit 16 bits; the
+    // first row, which may start earlier, is staged as 0 and its exact
+    // offset kept in s_off0), and the rows' first right rows.  12 KB for
+    // NV = 2048, so LDS allows 12 blocks per CU.
+    __shared__ uint16_t s_rel[NV + 2];
     __shared__ uint32_t s_low[NV + 1];
+    __shared__ uint64_t s_off0;
     const uint64_t blk = blockIdx.x;
     const uint64_t d0 = blk * NV;
     const uint64_t d1 = min(d0 + NV, nl + total);
@@ -293,54 +300,66 @@ __global__ __launch_bounds__(JX_BLOCK) void k_join_expand(
     if (b1 <= b0) return;  // no outputs on this diagonal
     const uint64_t sa = a0 > 0 ? a0 - 1 : 0;
     const int m = (int)(a1 - sa);
-    // Stage offsets and lower bounds of the diagonal's left rows, the loads
-    // of a batch in flight together.
+    // Stage offsets and lower bounds of the diagonal's left rows.  The
+    // loads of a batch are unconditional (indices clamped to the last row)
+    // so all of them are in flight before the first is used.
     constexpr int SB = 4;
     for (int t0 = 0; t0 < m; t0 += SB * JX_BLOCK) {
-        uint64_t vo[SB];
-        uint32_t vl[SB];
+        uint64_t vpre[SB];
+        uint32_t vloc[SB], vl[SB];
 #pragma unroll
         for (int u = 0; u < SB; ++u) {
-            const int t = t0 + u * JX_BLOCK + threadIdx.x;
-            if (t < m) {
-                vo[u] = off(sa + t);
-                vl[u] = lower[sa + t];
-            }
+            const uint64_t i = sa + (uint64_t)min(t0 + u * JX_BLOCK + (int)threadIdx.x, m - 1);
+            vloc[u] = off.loc[i];
+            vpre[u] = off.tpre[i / OFF_TILE];
+            vl[u] = lower[i];
         }
 #pragma unroll
         for (int u = 0; u < SB; ++u) {
             const int t = t0 + u * JX_BLOCK + threadIdx.x;
             if (t < m) {
-                s_off[t] = vo[u];
+                const uint64_t vo = vpre[u] + vloc[u];
+                s_rel[t] = t == 0 ? (uint16_t)0 : (uint16_t)(vo - b0);
                 s_low[t] = vl[u];
+                if (t == 0) s_off0 = vo;
             }
         }
     }
     __syncthreads();
 
-    // Output o's left row: the last staged k with s_off[k] <= o (s_off[0]
-    // <= every output of the diagonal).
-    uint32_t lrow[ITEMS], rrow[ITEMS];
+    // Output b0 + q's left row: the last staged k with s_rel[k] <= q
+    // (s_rel[0] = 0 <= every q).
+    uint32_t rrow[ITEMS];
     const int nout = (int)(b1 - b0);
     int kk[ITEMS];
-    dev::lockstep_last_le<ITEMS>(s_off, m, [&](int j) {
-        return b0 + (uint64_t)(j * JX_BLOCK + threadIdx.x);
-    }, kk);
+    dev::lockstep_last_le<ITEMS>(s_rel, m, [&](int j) { return j * JX_BLOCK + (int)threadIdx.x; },
+                                 kk);
+    // Every item's rows, with no branch: k is a valid staged index for
+    // every item, and an item past the diagonal's outputs takes row 0 of
+    // both inputs (it stores nothing), so the LDS reads and the gathers of
+    // a batch are all in flight together.
+    uint32_t low[ITEMS], rel[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        low[j] = s_low[kk[j]];
+        rel[j] = s_rel[kk[j]];
+    }
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
         const int q = j * JX_BLOCK + threadIdx.x;
-        lrow[j] = rrow[j] = 0;
-        if (q < nout) {
-            const uint64_t o = b0 + q;
-            const int k = kk[j];
-            lrow[j] = (uint32_t)(sa + k);
-            rrow[j] = s_low[k] + (uint32_t)(o - s_off[k]);
-        }
+        const int k = kk[j];
+        const bool in = q < nout;
+        const uint32_t r = k == 0 ? low[j] + (uint32_t)(b0 + (uint64_t)q - s_off0)
+                                  : low[j] + (uint32_t)(q - (int)rel[j]);
+        rrow[j] = in ? r : 0u;
+        if (!in) kk[j] = -1;  // past the outputs: left row 0
     }
     // Output columns gathered CB columns at a time: the loads of a batch
     // are all issued before its stores (the output never aliases the
-    // inputs).
-    constexpr int CB = 4;
+    // inputs).  One column per batch: fewer registers, more waves per CU
+    // (measured: 1, 2, 4 columns per batch give 1.61 / 1.68 / 1.78 ms of
+    // expansion per join-set step).
+    constexpr int CB = 1;
     const int nco = oc.n;
     for (int c0 = 0; c0 < nco; c0 += CB) {
         uint32_t v[CB][ITEMS];
@@ -352,8 +371,8 @@ __global__ __launch_bounds__(JX_BLOCK) void k_join_expand(
                 const uint32_t *src = oc.src[c];
 #pragma unroll
                 for (int j = 0; j < ITEMS; ++j) {
-                    const int q = j * JX_BLOCK + threadIdx.x;
-                    v[cb][j] = q < nout ? src[from_l ? lrow[j] : rrow[j]] : 0u;
+                    const uint32_t lr = kk[j] < 0 ? 0u : (uint32_t)(sa + kk[j]);
+                    v[cb][j] = src[from_l ? lr : rrow[j]];
                 }
             }
         }
