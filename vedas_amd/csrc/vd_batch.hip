@@ -255,9 +255,14 @@ __global__ __launch_bounds__(MJ_BLOCK) void k_join_expand_multi(
     TileOff off, const uint32_t *lower, const uint64_t *mp, uint64_t nl, uint64_t total,
     const JPairDev *P, int np, const uint64_t *lbase) {
     constexpr int ITEMS = MJ_NV / MJ_BLOCK;
-    __shared__ uint64_t s_off[MJ_NV + 1];
+    static_assert(MJ_NV < 65536, "offsets within a diagonal in 16 bits");
+    // Offsets relative to the block's first output b0 (the first staged
+    // row, which may start earlier, is staged as 0 with its exact offset in
+    // s_off0), as the single join's expansion stages them.
+    __shared__ uint16_t s_rel[MJ_NV + 2];
     __shared__ uint32_t s_low[MJ_NV + 1];
     __shared__ uint16_t s_pair[MJ_NV + 1];  // pair of each staged left row
+    __shared__ uint64_t s_off0;
     const uint64_t blk = blockIdx.x;
     const uint64_t d0 = blk * MJ_NV;
     const uint64_t d1 = min(d0 + MJ_NV, nl + total);
@@ -266,54 +271,60 @@ __global__ __launch_bounds__(MJ_BLOCK) void k_join_expand_multi(
     if (b1 <= b0) return;
     const uint64_t sa = a0 > 0 ? a0 - 1 : 0;
     const int m = (int)(a1 - sa);
+    // Staging loads unconditional (indices clamped to the last row), so all
+    // of a batch are in flight together.
     constexpr int SB = 4;
     for (int t0 = 0; t0 < m; t0 += SB * MJ_BLOCK) {
-        uint64_t vo[SB];
-        uint32_t vl[SB];
+        uint64_t vpre[SB];
+        uint32_t vloc[SB], vl[SB];
 #pragma unroll
         for (int u = 0; u < SB; ++u) {
-            const int t = t0 + u * MJ_BLOCK + threadIdx.x;
-            if (t < m) {
-                vo[u] = off(sa + t);
-                vl[u] = lower[sa + t];
-            }
+            const uint64_t i = sa + (uint64_t)min(t0 + u * MJ_BLOCK + (int)threadIdx.x, m - 1);
+            vloc[u] = off.loc[i];
+            vpre[u] = off.tpre[i / OFF_TILE];
+            vl[u] = lower[i];
         }
 #pragma unroll
         for (int u = 0; u < SB; ++u) {
             const int t = t0 + u * MJ_BLOCK + threadIdx.x;
             if (t < m) {
-                s_off[t] = vo[u];
+                const uint64_t vo = vpre[u] + vloc[u];
+                s_rel[t] = t == 0 ? (uint16_t)0 : (uint16_t)(vo - b0);
                 s_low[t] = vl[u];
                 s_pair[t] = (uint16_t)find_seg(lbase, np, sa + t);
+                if (t == 0) s_off0 = vo;
             }
         }
     }
     __syncthreads();
     const int nout = (int)(b1 - b0);
+    // Right row of output b0 + q, whose left row is staged row k.
+    auto right_row = [&](int k, int q) -> uint32_t {
+        return k == 0 ? s_low[0] + (uint32_t)(b0 + (uint64_t)q - s_off0)
+                      : s_low[k] + (uint32_t)(q - (int)s_rel[k]);
+    };
     if (s_pair[0] == s_pair[m - 1]) {
         // One join for the whole block (the common case): its descriptor is
         // uniform, columns are written one after the other, coalesced.
         const JPairDev &D = P[uniform(s_pair[0])];
-        uint32_t lrow[ITEMS], rrow[ITEMS];
-        int kk[ITEMS];  // last staged row with s_off <= output index
-        dev::lockstep_last_le<ITEMS>(s_off, m, [&](int it) {
-            return b0 + (uint64_t)(it * MJ_BLOCK + threadIdx.x);
-        }, kk);
+        uint32_t rrow[ITEMS];
+        int kk[ITEMS];  // last staged row with s_rel <= q
+        dev::lockstep_last_le<ITEMS>(s_rel, m,
+                                     [&](int it) { return it * MJ_BLOCK + (int)threadIdx.x; }, kk);
+        // every item's rows without branches; past the outputs: row 0 of
+        // the pair's inputs (nothing is stored for them)
 #pragma unroll
         for (int it = 0; it < ITEMS; ++it) {
             const int q = it * MJ_BLOCK + threadIdx.x;
-            lrow[it] = rrow[it] = 0;
-            if (q < nout) {
-                const uint64_t o = b0 + q;
-                const int k = kk[it];
-                lrow[it] = (uint32_t)(sa + k - D.lbase);
-                rrow[it] = s_low[k] + (uint32_t)(o - s_off[k]);
-            }
+            const bool in = q < nout;
+            const uint32_t r = right_row(kk[it], q);
+            rrow[it] = in ? r : 0u;
+            if (!in) kk[it] = -1;
         }
         const uint64_t ob = b0 - D.obase;
-        // Output columns [L columns, R columns after the key], CB at a time,
-        // the loads of a batch issued before its stores.
-        constexpr int CB = 4;
+        // Output columns [L columns, R columns after the key], one at a time
+        // (fewer registers, more waves per CU, as in the single join).
+        constexpr int CB = 1;
         const int ncl = D.ncl, nco = D.ncl + D.ncr - 1;
         for (int c0 = 0; c0 < nco; c0 += CB) {
             uint32_t v[CB][ITEMS];
@@ -325,8 +336,8 @@ __global__ __launch_bounds__(MJ_BLOCK) void k_join_expand_multi(
                     const uint32_t *src = from_l ? D.lc[c] : D.rc[c - ncl + 1];
 #pragma unroll
                     for (int it = 0; it < ITEMS; ++it) {
-                        const int q = it * MJ_BLOCK + threadIdx.x;
-                        v[cb][it] = q < nout ? src[from_l ? lrow[it] : rrow[it]] : 0u;
+                        const uint32_t lr = kk[it] < 0 ? 0u : (uint32_t)(sa + kk[it] - D.lbase);
+                        v[cb][it] = src[from_l ? lr : rrow[it]];
                     }
                 }
             }
@@ -353,13 +364,13 @@ __global__ __launch_bounds__(MJ_BLOCK) void k_join_expand_multi(
         int lo = 0, hi = m;
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
-            if (s_off[mid] <= o) lo = mid + 1; else hi = mid;
+            if ((int)s_rel[mid] <= q) lo = mid + 1; else hi = mid;
         }
         const int k = lo - 1;
         const uint64_t grow = sa + k;  // global left row
         const JPairDev &D = P[s_pair[k]];
         const uint64_t lrow = grow - D.lbase;
-        const uint64_t rrow = s_low[k] + (o - s_off[k]);
+        const uint64_t rrow = right_row(k, q);
         const uint64_t loc = o - D.obase;
         int oc = 0;
         for (int c = 0; c < D.ncl; ++c) D.oc[oc++][loc] = D.lc[c][lrow];
