@@ -402,42 +402,82 @@ __global__ __launch_bounds__(JX_BLOCK) void k_join_expand(
 
 // tab[k] = {first row, end row} of key k's run in a sorted key column, for
 // the keys <= kmax the column holds (the table is zero-filled first, so an
-// absent key has an empty run).  A run's first row looks RT_WIN rows ahead:
-// a shorter run gets its entry in one 8-byte store; a longer one gets its
-// end from its last row, which sees the first row more than RT_WIN back.
-// A thread takes RT_ITEMS rows, a block RT_ITEMS * 256 consecutive rows,
-// and every load of a phase is issued before any is used (a grid-stride
-// loop would put two dependent round trips per row in each thread's way).
-constexpr int RT_WIN = 8;
-constexpr int RT_ITEMS = 4;
-__global__ __launch_bounds__(256) void k_run_table(const uint32_t *__restrict__ key, uint64_t n,
-                                                   uint32_t kmax, uint2 *__restrict__ tab) {
-    const uint64_t base = (uint64_t)blockIdx.x * (256 * RT_ITEMS) + threadIdx.x;
-    uint32_t k[RT_ITEMS], prv[RT_ITEMS], nxt[RT_ITEMS];
-#pragma unroll
-    for (int j = 0; j < RT_ITEMS; ++j) {
-        const uint64_t i = base + (uint64_t)j * 256;
-        k[j] = i < n ? key[i] : 0u;
-        prv[j] = i < n && i > 0 ? key[i - 1] : ~k[j];
-        nxt[j] = i + 1 < n ? key[i + 1] : ~k[j];
+// absent key has an empty run).  A block takes RT_ROWS consecutive rows,
+// stages their keys in LDS and writes every table entry of its key span
+// (key[r0 - 1], key[r1 - 1]] in key order, absent keys included, so its
+// writes cover whole cache lines: a key's entry comes from two searches in
+// the staged keys.  The span's last key may run on past the block: its
+// entry gets its first row here and its end from the block holding the
+// run's last row (which starts no entry of its own for it).  A block whose
+// span is longer than RT_SPAN_MAX keys (sparse keys) writes only the
+// entries of the keys it holds, on the zero fill.
+constexpr int RT_BLOCK = 256;
+constexpr int RT_ROWS = 1024;
+constexpr uint32_t RT_SPAN_MAX = 16 * RT_ROWS;
+__global__ __launch_bounds__(RT_BLOCK) void k_run_table(const uint32_t *__restrict__ key,
+                                                        uint64_t n, uint32_t kmax,
+                                                        uint2 *__restrict__ tab) {
+    __shared__ uint32_t s_k[RT_ROWS];
+    const uint64_t r0 = (uint64_t)blockIdx.x * RT_ROWS;
+    const int cnt = (int)min<uint64_t>(RT_ROWS, n - r0);
+    for (int t = threadIdx.x; t < RT_ROWS; t += RT_BLOCK) s_k[t] = t < cnt ? key[r0 + t] : ~0u;
+    // the key before the block and the one after it
+    const uint32_t kprev = r0 > 0 ? key[r0 - 1] : 0u;
+    const bool has_prev = r0 > 0;
+    const uint32_t knext = r0 + cnt < n ? key[r0 + cnt] : ~0u;
+    const bool has_next = r0 + cnt < n;
+    __syncthreads();
+    const uint32_t kfirst = s_k[0], klast = s_k[cnt - 1];
+    // keys of this block's span: (kprev, min(klast, kmax)]; the first
+    // block's span starts at its first key
+    const uint64_t lo = has_prev ? (uint64_t)kprev + 1 : (uint64_t)kfirst;
+    const uint64_t hi = min<uint64_t>(klast, kmax);
+    // A run that starts before the block and ends inside it: its end.
+    if (has_prev && kprev == kfirst && kfirst <= kmax && threadIdx.x == 0) {
+        int e = 0;  // first staged key past the run
+        int len = cnt;
+        while (len > 0) {
+            const int half = len >> 1;
+            if (s_k[e + half] <= kfirst) { e += half + 1; len -= half + 1; }
+            else len = half;
+        }
+        if (e < cnt || !has_next || knext != kfirst) tab[kfirst].y = (uint32_t)(r0 + e);
     }
-#pragma unroll
-    for (int j = 0; j < RT_ITEMS; ++j) {
-        const uint64_t i = base + (uint64_t)j * 256;
-        if (i >= n || k[j] > kmax) continue;
-        if (prv[j] != k[j]) {
-            // the next RT_WIN keys, loaded together (the column is sorted:
-            // the run ends at the first one that differs)
-            uint32_t nx[RT_WIN];
-#pragma unroll
-            for (int w = 0; w < RT_WIN; ++w) nx[w] = i + 1 + w < n ? key[i + 1 + w] : ~k[j];
-            uint64_t e = i + 1;
-#pragma unroll
-            for (int w = 0; w < RT_WIN; ++w) e += (nx[w] == k[j] && e == i + 1 + w) ? 1 : 0;
-            if (e <= i + RT_WIN) tab[k[j]] = make_uint2((uint32_t)i, (uint32_t)e);
-            else tab[k[j]].x = (uint32_t)i;  // longer than RT_WIN rows
-        } else if (nxt[j] != k[j] && i >= (uint64_t)RT_WIN && key[i - RT_WIN] == k[j]) {
-            tab[k[j]].y = (uint32_t)(i + 1);  // the last row of a run of > RT_WIN rows
+    if (lo > hi) return;
+    auto lower = [&](uint32_t q) {  // first staged index with s_k >= q
+        int p = 0, len = cnt;
+        while (len > 0) {
+            const int half = len >> 1;
+            if (s_k[p + half] < q) { p += half + 1; len -= half + 1; }
+            else len = half;
+        }
+        return p;
+    };
+    if (hi - lo + 1 <= RT_SPAN_MAX) {
+        for (uint64_t q = lo + threadIdx.x; q <= hi; q += RT_BLOCK) {
+            const int a = lower((uint32_t)q);
+            uint2 ent = make_uint2(0u, 0u);
+            if (a < cnt && s_k[a] == (uint32_t)q) {
+                const int b = q < 0xFFFFFFFFull ? lower((uint32_t)q + 1) : cnt;
+                ent.x = (uint32_t)(r0 + a);
+                ent.y = (uint32_t)(r0 + b);
+                // the run goes on in the next block: that block writes the end
+                if (b == cnt && has_next && knext == (uint32_t)q) {
+                    tab[q].x = ent.x;
+                    continue;
+                }
+            }
+            tab[q] = ent;
+        }
+    } else {
+        // sparse keys: the entries of the keys held, on the zero fill
+        for (int t = threadIdx.x; t < cnt; t += RT_BLOCK) {
+            const uint32_t q = s_k[t];
+            if ((uint64_t)q < lo || q > hi) continue;
+            if (t > 0 && s_k[t - 1] == q) continue;  // not the run's first row here
+            const int b = q < 0xFFFFFFFFu ? lower(q + 1) : cnt;
+            if (b == cnt && has_next && knext == q) tab[q].x = (uint32_t)(r0 + t);
+            else tab[q] = make_uint2((uint32_t)(r0 + t), (uint32_t)(r0 + b));
         }
     }
 }
@@ -702,8 +742,8 @@ vd_rel *join_driven(vd_ctx *ctx, const vd_rel *D, int dkey, const vd_rel *Y, uin
 
     DBuf<uint2> tab(ctx, (uint64_t)kmax + 1);
     fill_u32(ctx, reinterpret_cast<uint32_t *>(tab.p), 0, 2 * ((uint64_t)kmax + 1));
-    launch(ctx, "run_table", 4.0 * (double)ny, dim3((unsigned)ceil_div(ny, 256 * RT_ITEMS)),
-           dim3(256), 0, k_run_table, (const uint32_t *)Y->cols[0], ny, kmax, tab.p);
+    launch(ctx, "run_table", 4.0 * (double)ny, dim3((unsigned)ceil_div(ny, RT_ROWS)),
+           dim3(RT_BLOCK), 0, k_run_table, (const uint32_t *)Y->cols[0], ny, kmax, tab.p);
     DBuf<uint32_t> lower(ctx, nd), off(ctx, nd);
     const uint64_t jb_tiles = ceil_div(nd, JB_TILE);
     // tile totals, tile prefixes (+ the total), tile key bounds, flag word
