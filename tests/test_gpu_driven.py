@@ -259,3 +259,40 @@ def test_join_capped(ctx, store):
     assert L.join_capped(R, 0)[1] == n
     for x in (L, R, out):
         x.free()
+
+
+@pytest.mark.parametrize("ny_cols", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("repeat", [False, True])
+def test_driven_record_gather(ctx, ny_cols, repeat):
+    """Y with 1 to 5 gathered columns (records of 2 or 4 words for 2 to 4 of
+    them, one of them taken twice when `repeat`; plain columns for 1 and
+    5): the output equals the oracle's join of D's key with Y followed by
+    the stable sort on D's first column."""
+    from vedas_amd._native import Rel
+    rng = np.random.default_rng(100 + ny_cols)
+    ny, nd, nk = 40_000, 30_000, 20_000
+    ykey = np.sort(rng.integers(0, nk, ny)).astype(np.uint32)
+    ycols = [ykey] + [rng.integers(0, 1 << 32, ny, dtype=np.uint64).astype(np.uint32)
+                      for _ in range(ny_cols)]
+    # D: (v, key) sorted on v then key, distinct rows
+    dv = rng.integers(0, 5000, nd).astype(np.uint32)
+    dk = rng.integers(0, nk, nd).astype(np.uint32)
+    dd = np.unique(np.stack([dv, dk], 1), axis=0)
+    dcols = [np.ascontiguousarray(dd[:, 0]), np.ascontiguousarray(dd[:, 1])]
+    src = [-1, -2] + list(range(1, ny_cols + 1))
+    if repeat:
+        src.append(1)
+    # the oracle: join D's (key, v) with Y on the key, then the output's
+    # columns [v, key, Y...] sorted stably on v (the join is key-major)
+    jw = O.join(O.sort_col0([dcols[1], dcols[0]]), ycols, 1)   # [key, v, y1..]
+    cols = {-1: jw[1], -2: jw[0]}
+    for c in range(1, ny_cols + 1):
+        cols[c] = jw[1 + c]
+    want = O.sort_col0([cols[c] for c in src])
+    D = Rel.from_columns(ctx, dcols)
+    Y = Rel.from_columns(ctx, ycols)
+    print(f"record gather ny_cols={ny_cols} repeat={repeat}", flush=True)
+    out, lo, hi = D.join_driven(1, Y, src, key_hi=nk - 1)
+    check_equal(out.columns(), want)
+    for x in (D, Y, out):
+        x.free()

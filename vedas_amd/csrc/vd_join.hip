@@ -270,15 +270,54 @@ __global__ __launch_bounds__(dev::TT_BLOCK) void k_join_scan_total(
 }
 
 // Output column c is gathered from src[c] at the output's left (driving)
-// row, or at its matched right row when bit c of from_r is set.
+// row, or at its matched right row when bit c of from_r is set.  With
+// records (the expansion's AW > 0), the right rows' columns come from
+// rec, AW words per right row, column c in word slot[c]: one load per
+// output row for all of them.
 struct OutCols {
     const uint32_t *src[VD_MAX_COLS];
     uint32_t *o[VD_MAX_COLS];
     uint32_t from_r;
     int n;
+    const uint32_t *rec;
+    int slot[VD_MAX_COLS];
 };
 
-template <int NV>
+template <int AW>
+struct RecT;
+template <>
+struct RecT<2> {
+    using T = uint2;
+    static __device__ __forceinline__ uint32_t word(const T &v, int s) { return s == 0 ? v.x : v.y; }
+};
+template <>
+struct RecT<4> {
+    using T = uint4;
+    static __device__ __forceinline__ uint32_t word(const T &v, int s) {
+        return s == 0 ? v.x : s == 1 ? v.y : s == 2 ? v.z : v.w;
+    }
+};
+
+// rec[i] = (c[0][i], .., c[nc - 1][i], 0 ..) as AW-word records.
+struct RecCols {
+    const uint32_t *c[4];
+    int nc;
+};
+template <int AW>
+__global__ __launch_bounds__(256) void k_interleave(RecCols rc, uint64_t n,
+                                                    typename RecT<AW>::T *__restrict__ rec) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    uint32_t w[AW];
+#pragma unroll
+    for (int s = 0; s < AW; ++s) w[s] = s < rc.nc ? rc.c[s][i] : 0u;
+    typename RecT<AW>::T v;
+    if constexpr (AW == 2) v = make_uint2(w[0], w[1]);
+    else v = make_uint4(w[0], w[1], w[2], w[3]);
+    rec[i] = v;
+}
+
+template <int NV, int AW = 0>
 __global__ __launch_bounds__(JX_BLOCK) void k_join_expand(
     JoinOff off, const uint32_t *__restrict__ lower,
     const uint64_t *__restrict__ mp, uint64_t nl, uint64_t total, OutCols oc) {
@@ -361,11 +400,30 @@ __global__ __launch_bounds__(JX_BLOCK) void k_join_expand(
     // expansion per join-set step).
     constexpr int CB = 1;
     const int nco = oc.n;
+    if constexpr (AW > 0) {
+        // every right-row column from one record load per output row
+        using R = RecT<AW>;
+        typename R::T rv[ITEMS];
+        const typename R::T *rec = reinterpret_cast<const typename R::T *>(oc.rec);
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) rv[j] = rec[rrow[j]];
+        for (int c = 0; c < nco; ++c) {
+            if (((oc.from_r >> c) & 1u) == 0) continue;
+            const int sl = oc.slot[c];
+            uint32_t *dst = oc.o[c];
+#pragma unroll
+            for (int j = 0; j < ITEMS; ++j) {
+                const int q = j * JX_BLOCK + threadIdx.x;
+                if (q < nout) dst[b0 + q] = R::word(rv[j], sl);
+            }
+        }
+    }
     for (int c0 = 0; c0 < nco; c0 += CB) {
         uint32_t v[CB][ITEMS];
 #pragma unroll
         for (int cb = 0; cb < CB; ++cb) {
             const int c = c0 + cb;
+            if (AW > 0 && ((oc.from_r >> c) & 1u)) continue;  // written from the records
             if (c < nco) {
                 const bool from_l = ((oc.from_r >> c) & 1u) == 0;
                 const uint32_t *src = oc.src[c];
@@ -379,6 +437,7 @@ __global__ __launch_bounds__(JX_BLOCK) void k_join_expand(
 #pragma unroll
         for (int cb = 0; cb < CB; ++cb) {
             const int c = c0 + cb;
+            if (AW > 0 && ((oc.from_r >> c) & 1u)) continue;
             if (c < nco) {
                 uint32_t *dst = oc.o[c];
 #pragma unroll
@@ -779,16 +838,42 @@ vd_rel *join_driven(vd_ctx *ctx, const vd_rel *D, int dkey, const vd_rel *Y, uin
     if (total == 0) return out;
     try {
         OutCols oc{};
+        // Y's columns the output takes, each once
+        int ycol[VD_MAX_COLS], nyc = 0;
         for (int c = 0; c < nout; ++c) {
             if (src[c] >= 0) {
                 oc.src[c] = Y->cols[src[c]];
                 oc.from_r |= 1u << c;
+                int k = 0;
+                while (k < nyc && ycol[k] != src[c]) ++k;
+                if (k == nyc) ycol[nyc++] = src[c];
+                oc.slot[c] = k;
             } else {
                 oc.src[c] = D->cols[-1 - src[c]];
             }
             oc.o[c] = out->cols[c];
         }
         oc.n = nout;
+        // Y's runs sit at random rows, so each of its columns costs a cache
+        // line per run: two to four of them go into one array of records
+        // first (one line per run) when the lines saved, one per column
+        // after the first for each driver row, outweigh the record pass.
+        const int aw = nyc == 2 ? 2 : (nyc == 3 || nyc == 4) ? 4 : 0;
+        DBuf<uint32_t> rec;
+        if (aw && 64.0 * (nyc - 1) * (double)nd > 4.0 * (double)ny * (nyc + aw)) {
+            rec = DBuf<uint32_t>(ctx, ny * (uint64_t)aw);
+            RecCols rc{};
+            for (int k = 0; k < nyc; ++k) rc.c[k] = Y->cols[ycol[k]];
+            rc.nc = nyc;
+            const dim3 g((unsigned)ceil_div(ny, 256));
+            if (aw == 2)
+                launch(ctx, "interleave", 4.0 * (double)ny * (nyc + aw), g, dim3(256), 0,
+                       k_interleave<2>, rc, ny, reinterpret_cast<uint2 *>(rec.p));
+            else
+                launch(ctx, "interleave", 4.0 * (double)ny * (nyc + aw), g, dim3(256), 0,
+                       k_interleave<4>, rc, ny, reinterpret_cast<uint4 *>(rec.p));
+            oc.rec = rec.p;
+        }
         const int nv = ctx->join_nv;
         const uint64_t nblocks = ceil_div(nd + total, (uint64_t)nv);
         VD_REQUIRE(nblocks < (1ull << 31), VD_ERANGE, "join expansion grid too large");
@@ -804,9 +889,19 @@ vd_rel *join_driven(vd_ctx *ctx, const vd_rel *D, int dkey, const vd_rel *Y, uin
             launch(ctx, "join_expand", xbytes, dim3((unsigned)nblocks), dim3(JX_BLOCK), 0, kern,
                    joff, (const uint32_t *)lower.p, (const uint64_t *)mp.p, nd, total, oc);
         };
-        if (nv == 1024) go(k_join_expand<1024>);
-        else if (nv == 4096) go(k_join_expand<4096>);
-        else go(k_join_expand<2048>);
+        if (!rec.p) {
+            if (nv == 1024) go(k_join_expand<1024>);
+            else if (nv == 4096) go(k_join_expand<4096>);
+            else go(k_join_expand<2048>);
+        } else if (aw == 2) {
+            if (nv == 1024) go(k_join_expand<1024, 2>);
+            else if (nv == 4096) go(k_join_expand<4096, 2>);
+            else go(k_join_expand<2048, 2>);
+        } else {
+            if (nv == 1024) go(k_join_expand<1024, 4>);
+            else if (nv == 4096) go(k_join_expand<4096, 4>);
+            else go(k_join_expand<2048, 4>);
+        }
     } catch (...) {
         rel_free(ctx, out);
         throw;

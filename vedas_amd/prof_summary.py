@@ -38,6 +38,7 @@ LABELS = [
     (r"k_scan_tile_totals\b", "scan_tiles"),
     (r"k_join_scan_total\b", "scan_tiles"),
     (r"k_rb_copy\b", "read_back"),
+    (r"k_interleave<", "interleave"),
     (r"k_join_ranges", "join_ranges"),
     (r"k_pair_totals\b", "join_totals"),
     (r"k_pack_xyz\b|k_pack_split\b|k_pack_xpos\b", "index_pack"),
