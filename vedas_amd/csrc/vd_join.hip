@@ -223,6 +223,51 @@ __global__ __launch_bounds__(256) void k_join_partition(
     if (threadIdx.x % JR_GROUP == 0) mp[b] = lo;
 }
 
+// The same cuts made before the host reads the output size back: the size
+// comes from *total_p (tpre[ntiles], written by the scan kernel before this
+// one on the stream), the groups stride over b = 0 .. min(nblocks, cap - 1),
+// so the cuts are ready when the host launches the expansion.
+__global__ __launch_bounds__(256) void k_join_partition_ahead(
+    JoinOff off, uint64_t nl, const uint64_t *__restrict__ total_p, int nv, uint64_t cap,
+    uint64_t *__restrict__ mp) {
+    const uint64_t total = *total_p;
+    const uint64_t last = min((nl + total + (uint64_t)nv - 1) / (uint64_t)nv, cap - 1);
+    const uint64_t ngroups = (uint64_t)gridDim.x * blockDim.x / JR_GROUP;
+    for (uint64_t b = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / JR_GROUP; b <= last;
+         b += ngroups) {
+        const uint64_t diag = min(b * (uint64_t)nv, nl + total);
+        const uint64_t lo =
+            dev::group_bound<JR_GROUP>(0, nl, [&](uint64_t i) { return !(i + off(i) < diag); });
+        if (threadIdx.x % JR_GROUP == 0) mp[b] = lo;
+    }
+}
+
+// Cuts made ahead of the size read-back for outputs up to 32 times their
+// left rows (a larger output gets the partition kernel after it), by groups
+// enough for outputs up to 8 times the left rows in one pass.
+inline uint64_t cut_capacity(uint64_t nl, int nv) { return ceil_div(33 * nl, (uint64_t)nv) + 2; }
+
+void partition_ahead(vd_ctx *ctx, const JoinOff &joff, uint64_t nl, const uint64_t *total_p,
+                     int nv, DBuf<uint64_t> &mp, uint64_t &cap) {
+    cap = cut_capacity(nl, nv);
+    mp = DBuf<uint64_t>(ctx, cap);
+    const uint64_t groups = std::min(cap, ceil_div(9 * nl, (uint64_t)nv) + 2);
+    launch(ctx, "join_partition", 0.0, dim3((unsigned)ceil_div(groups * JR_GROUP, 256)),
+           dim3(256), 0, k_join_partition_ahead, joff, nl, total_p, nv, cap, mp.p);
+}
+
+// The expansion's cuts: those made ahead when they cover the output, else
+// the partition kernel's into `spare`.
+const uint64_t *join_cuts(vd_ctx *ctx, const JoinOff &joff, uint64_t nl, uint64_t total, int nv,
+                          const DBuf<uint64_t> &mp, uint64_t cap, DBuf<uint64_t> &spare) {
+    const uint64_t nblocks = ceil_div(nl + total, (uint64_t)nv);
+    if (nblocks + 1 <= cap) return mp.p;
+    spare = DBuf<uint64_t>(ctx, nblocks + 1);
+    launch(ctx, "join_partition", 0.0, dim3((unsigned)ceil_div((nblocks + 1) * JR_GROUP, 256)),
+           dim3(256), 0, k_join_partition, joff, nl, total, nv, nblocks, spare.p);
+    return spare.p;
+}
+
 // One block: scans the tile totals into the tile prefixes, then reads back
 // {total, first output key, last output key} (the output's column 0 holds
 // the left keys of matched rows in order: the smallest and largest of the
@@ -630,10 +675,11 @@ __global__ __launch_bounds__(JB_BLOCK) void k_drive_bounds(
 // read-back {total, first key, last key}.
 struct JoinCount {
     DBuf<uint32_t> lower, off;
-    DBuf<uint64_t> ranges;
+    DBuf<uint64_t> ranges, mp;  // mp: the cuts made ahead (mp_cap entries, for nv)
     uint64_t *tpre = nullptr;
-    uint64_t total = 0;
+    uint64_t total = 0, mp_cap = 0;
     uint32_t key_lo = 0, key_hi = 0;
+    int nv = 0;
 };
 
 template <int NK>
@@ -661,6 +707,9 @@ void join_count(vd_ctx *ctx, const vd_rel *L, const vd_rel *R, JoinCount &jc) {
     launch(ctx, "scan_tiles", 16.0 * (double)jb_tiles, dim3(1), dim3(dev::TT_BLOCK), 0,
            k_join_scan_total, (const uint64_t *)ttot, (const uint64_t *)tkeys, jb_tiles, tpre,
            rb, (const uint32_t *)nullptr);
+    jc.nv = ctx->join_nv;
+    partition_ahead(ctx, JoinOff{(const uint32_t *)jc.off.p, tpre}, nl, tpre + jb_tiles, jc.nv,
+                    jc.mp, jc.mp_cap);
     uint64_t h[4];
     rb_wait(ctx, rb, h, 4);
     jc.total = h[0];
@@ -711,20 +760,16 @@ vd_rel *join_impl(vd_ctx *ctx, const vd_rel *L, const vd_rel *R,
             oc.o[c] = out->cols[c];
         }
         oc.n = out_cols;
-        const int nv = ctx->join_nv;
+        const int nv = jc.nv;
         const uint64_t nblocks = ceil_div(nl + total, (uint64_t)nv);
         VD_REQUIRE(nblocks < (1ull << 31), VD_ERANGE, "join expansion grid too large");
-        DBuf<uint64_t> mp(ctx, nblocks + 1);
-        launch(ctx, "join_partition", 0.0,
-               dim3((unsigned)ceil_div((nblocks + 1) * JR_GROUP, 256)),
-               dim3(256), 0, k_join_partition, joff, nl, total,
-               nv, nblocks, mp.p);
+        DBuf<uint64_t> spare;
+        const uint64_t *mp = join_cuts(ctx, joff, nl, total, nv, jc.mp, jc.mp_cap, spare);
         const double xbytes = 4.0 * (double)total * out_cols + 8.0 * nl +
                               4.0 * (double)nl * (L->ncols) + 4.0 * (double)nr * (R->ncols - NK);
         auto go = [&](auto kern) {
             launch(ctx, "join_expand", xbytes, dim3((unsigned)nblocks), dim3(JX_BLOCK),
-                   0, kern, joff, (const uint32_t *)lower.p,
-                   (const uint64_t *)mp.p, nl, total, oc);
+                   0, kern, joff, (const uint32_t *)lower.p, mp, nl, total, oc);
         };
         if (nv == 1024) go(k_join_expand<1024>);
         else if (nv == 4096) go(k_join_expand<4096>);
@@ -825,6 +870,10 @@ vd_rel *join_driven(vd_ctx *ctx, const vd_rel *D, int dkey, const vd_rel *Y, uin
     launch(ctx, "scan_tiles", 16.0 * (double)jb_tiles, dim3(1), dim3(dev::TT_BLOCK), 0,
            k_join_scan_total, (const uint64_t *)ttot, (const uint64_t *)tkeys, jb_tiles, tpre,
            rb, (const uint32_t *)flag);
+    const int nv = ctx->join_nv;
+    DBuf<uint64_t> mp;
+    uint64_t mp_cap = 0;
+    partition_ahead(ctx, joff, nd, tpre + jb_tiles, nv, mp, mp_cap);
     uint64_t h[4];
     rb_wait(ctx, rb, h, 4);
     if (h[3] & 2u) return nullptr;  // a repeated driver row: the order would differ
@@ -874,20 +923,17 @@ vd_rel *join_driven(vd_ctx *ctx, const vd_rel *D, int dkey, const vd_rel *Y, uin
                        k_interleave<4>, rc, ny, reinterpret_cast<uint4 *>(rec.p));
             oc.rec = rec.p;
         }
-        const int nv = ctx->join_nv;
         const uint64_t nblocks = ceil_div(nd + total, (uint64_t)nv);
         VD_REQUIRE(nblocks < (1ull << 31), VD_ERANGE, "join expansion grid too large");
-        DBuf<uint64_t> mp(ctx, nblocks + 1);
-        launch(ctx, "join_partition", 0.0,
-               dim3((unsigned)ceil_div((nblocks + 1) * JR_GROUP, 256)), dim3(256), 0,
-               k_join_partition, joff, nd, total, nv, nblocks, mp.p);
+        DBuf<uint64_t> spare;
+        const uint64_t *cuts = join_cuts(ctx, joff, nd, total, nv, mp, mp_cap, spare);
         int nfd = 0, nfy = 0;
         for (int c = 0; c < nout; ++c) (src[c] >= 0 ? nfy : nfd) += 1;
         const double xbytes = 4.0 * (double)total * nout + 8.0 * nd + 4.0 * (double)nd * nfd +
                               4.0 * (double)ny * nfy;
         auto go = [&](auto kern) {
             launch(ctx, "join_expand", xbytes, dim3((unsigned)nblocks), dim3(JX_BLOCK), 0, kern,
-                   joff, (const uint32_t *)lower.p, (const uint64_t *)mp.p, nd, total, oc);
+                   joff, (const uint32_t *)lower.p, cuts, nd, total, oc);
         };
         if (!rec.p) {
             if (nv == 1024) go(k_join_expand<1024>);
