@@ -490,7 +490,7 @@ def main():
         sys.exit(2)
 
     from vedas_amd import _native
-    from vedas_amd.engine import Engine, query_text, set_option
+    from vedas_amd.engine import Engine, get_option, query_text, set_option
 
     log(f"[bench] generating {args.triples} distinct triples")
     t = time.perf_counter()
@@ -647,36 +647,53 @@ def main():
         r.free()
     log("[bench] planner: " + ", ".join(f"{q} {v['ms']:.1f} ms" for q, v in planned.items()))
 
-    # The same steps with driven joins off: every IndexSwap after a join
-    # re-sorts the join's output (the reference's order of operations).
-    set_option("driven_swap", 0)
-    try:
-        step()
-        ctx.sync()
-        t0 = time.perf_counter()
-        for _ in range(3):
+    # The same steps with the plan rewrites off, one at a time and both
+    # (every IndexSwap then re-sorts its whole input, the reference's order
+    # of operations): step time, rows re-sorted, identical results.
+    def steps_with(opts):
+        old = {k: get_option(k) for k in opts}
+        for k, v in opts.items():
+            set_option(k, v)
+        try:
             step()
-        ctx.sync()
-        plain_step_ms = (time.perf_counter() - t0) / 3 * 1e3
-        plain_q = {}
-        for q in names:
-            r = eng.query(texts[q])
-            st0 = r.stats()
-            plain_q[q] = {"swap_rows": st0["swap_rows"], "join_rows": st0["join_rows"],
-                          "same_result": result_digest(r.headers(), r.columns()) ==
-                          gpu_results[q]["digest"]}
-            r.free()
-    finally:
-        set_option("driven_swap", 1)
-    driven = {"ms_per_step_with": dt / args.steps * 1e3, "ms_per_step_without": plain_step_ms,
-              "queries": {q: {"driven_joins": per_q[q][0]["driven_joins"],
-                              "swap_rows_with": per_q[q][0]["swap_rows"],
-                              "swap_rows_without": plain_q[q]["swap_rows"],
-                              "same_join_rows": plain_q[q]["join_rows"] ==
-                              per_q[q][0]["join_rows"],
-                              "same_result": plain_q[q]["same_result"]} for q in names}}
+            ctx.sync()
+            t0 = time.perf_counter()
+            for _ in range(3):
+                step()
+            ctx.sync()
+            ms = (time.perf_counter() - t0) / 3 * 1e3
+            qs = {}
+            for q in names:
+                r = eng.query(texts[q])
+                st0 = r.stats()
+                qs[q] = {"swap_rows": st0["swap_rows"], "join_rows": st0["join_rows"],
+                         "filtered_rows": st0["filtered_rows"],
+                         "same_result": result_digest(r.headers(), r.columns()) ==
+                         gpu_results[q]["digest"]}
+                r.free()
+        finally:
+            for k, v in old.items():
+                set_option(k, v)
+        return ms, qs
+
+    def compare(ms_without, qs):
+        return {"ms_per_step_with": dt / args.steps * 1e3, "ms_per_step_without": ms_without,
+                "queries": {q: {"driven_joins": per_q[q][0]["driven_joins"],
+                                "filtered_rows_with": per_q[q][0]["filtered_rows"],
+                                "swap_rows_with": per_q[q][0]["swap_rows"],
+                                "swap_rows_without": qs[q]["swap_rows"],
+                                "same_join_rows": qs[q]["join_rows"] == per_q[q][0]["join_rows"],
+                                "same_result": qs[q]["same_result"]} for q in names}}
+
+    driven = compare(*steps_with({"driven_swap": 0}))
+    filtered = compare(*steps_with({"filtered_swap": 0}))
+    ref_ms, ref_q = steps_with({"driven_swap": 0, "filtered_swap": 0})
+    reference_order = {"ms_per_step": ref_ms,
+                       "same_result": all(v["same_result"] for v in ref_q.values()),
+                       "swap_rows": {q: v["swap_rows"] for q, v in ref_q.items()}}
     log(f"[bench] driven joins: {driven['ms_per_step_with']:.2f} ms per step with, "
-        f"{plain_step_ms:.2f} ms without")
+        f"{driven['ms_per_step_without']:.2f} ms without; filtered IndexSwaps: "
+        f"{filtered['ms_per_step_without']:.2f} ms without; both off {ref_ms:.2f} ms")
 
     batch = serve = None
     if args.mix > 0:
@@ -707,6 +724,7 @@ def main():
         jr = sts[0]["join_rows"]
         queries[q] = {"ms_median": ms, "join_rows": jr, "rows": sts[0]["rows"],
                       "select_rows": sts[0]["select_rows"], "swap_rows": sts[0]["swap_rows"],
+                      "filtered_rows": sts[0]["filtered_rows"],
                       "join_rows_per_s": jr / (ms / 1e3), "alg_bytes": q_bytes[q],
                       "GB/s": q_bytes[q] / (ms / 1e3) / 1e9}
     out = {
@@ -722,13 +740,18 @@ def main():
                    "predicates": GEN["n_predicates"], "subject_zipf": GEN["subject_zipf"],
                    "pred_zipf": GEN["pred_zipf"], "literal_frac": GEN["literal_frac"],
                    "queries": names, "seed": args.seed, "bound_prescan": True,
-                   "driven_joins": True},
+                   "driven_joins": True, "filtered_swaps": True},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "queries": queries,
         "queries_note": "GB/s = algorithmic operator bytes of the query / its median wall time",
         "bound_prescan": prescan,
         "driven_joins": driven,
+        "filtered_swaps": filtered,
+        "reference_order": reference_order,
+        "plan_rewrites_note": "driven joins: a join and the IndexSwap after it in one pass; "
+                              "filtered IndexSwaps: the rows a swapped relation's join cannot "
+                              "match dropped before the sort; reference_order: both off",
         "index_build": {"ms": build_s * 1e3, "permutations": list(perms), "key_bits": kb,
                         "alg_bytes": build_alg, "GB/s": build_alg / build_s / 1e9,
                         "frac": build_alg / build_s / HBM_PEAK,

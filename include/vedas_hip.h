@@ -333,6 +333,17 @@ int  vd_join_capped(vd_ctx *ctx, const vd_rel *L, const vd_rel *R, int nkeys,
  * FullRelationIR::sortByFirstColumn (FullRelationIR.cu:594-713) as used by
  * IndexSwapJob::startJob (IndexSwapJob.cu:28-95). */
 int  vd_sort_col0(vd_ctx *ctx, vd_rel *rel);
+/* The rows of X whose column xcol value occurs in column ycol of Y, in X's
+ * order (columns, bounds and the distinct mark of X).  For an IndexSwap
+ * (IndexSwapJob.cu:28-95) whose output only feeds a join on the swapped
+ * column (JoinQueryJob.cu:125-453): the join of the stably sorted X keeps
+ * exactly these rows, in the same order, so filtering first and sorting
+ * the rows kept gives the same join output.  Keys above key_hi (or a known
+ * bound of either column) match nothing; *out = NULL with *rows = ~0 when
+ * that bound exceeds 2^28 - 1 (the key bitmap's size), and *out = NULL with
+ * *rows = the kept count when it exceeds max_rows (nothing is written). */
+int  vd_semi_filter(vd_ctx *ctx, const vd_rel *X, int xcol, const vd_rel *Y, int ycol,
+                    uint32_t key_hi, uint64_t max_rows, vd_rel **out, uint64_t *rows);
 /* Lexicographic sort of all rows over all columns.  Replaces
  * FullRelationIR::sort (FullRelationIR.cu:218-336). */
 int  vd_sort_lex(vd_ctx *ctx, vd_rel *rel);

@@ -39,6 +39,8 @@ typedef struct {
     int jobs;               /* operators executed                       */
     int driven_joins;       /* joins that delivered their IndexSwap's order
                                (those swaps re-sort nothing)             */
+    uint64_t filtered_rows; /* rows an IndexSwap left to its join dropped
+                               before the sort (they match nothing)      */
 } vdq_stats_t;
 
 int vdq_engine_create(int device, vdq_engine **out);

@@ -120,6 +120,9 @@ EXPORTS = {
                                          C.c_uint32]),
     "vd_join_capped": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_uint64,
                                  C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]),
+    "vd_semi_filter": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                 C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p),
+                                 C.POINTER(C.c_uint64)]),
     "vd_join_driven": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_uint32,
                                  C.c_uint32, C.c_int, C.POINTER(C.c_int), C.c_int,
                                  C.POINTER(C.c_void_p), C.POINTER(C.c_uint32),
@@ -390,6 +393,19 @@ class Rel:
         h, n = C.c_void_p(), C.c_uint64()
         self.ctx.check(self.ctx.L.vd_join_capped(self.ctx.h, self.h, other.h, nkeys, max_rows,
                                                  C.byref(h), C.byref(n)))
+        return (Rel(self.ctx, h) if h.value else None), n.value
+
+    def semi_filter(self, col: int, other: "Rel", ocol: int = 0, key_hi: int = 0xFFFFFFFF,
+                    max_rows: int = (1 << 64) - 1):
+        """(the rows of self whose column `col` value occurs in other's column
+        `ocol`, in self's order, kept rows), or (None, kept rows) when more
+        than max_rows are kept, or (None, None) when the key bound is beyond
+        the filter's bitmap."""
+        h, n = C.c_void_p(), C.c_uint64()
+        self.ctx.check(self.ctx.L.vd_semi_filter(self.ctx.h, self.h, col, other.h, ocol, key_hi,
+                                                 max_rows, C.byref(h), C.byref(n)))
+        if not h.value and n.value == (1 << 64) - 1:
+            return None, None
         return (Rel(self.ctx, h) if h.value else None), n.value
 
     def join_driven(self, dkey: int, other: "Rel", src, key_lo: int = 0,

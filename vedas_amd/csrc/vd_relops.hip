@@ -1,5 +1,6 @@
 // Relation re-sorts and finalisation: sort by column 0 (IndexSwap),
-// lexicographic sort, and removal of adjacent duplicate rows.
+// lexicographic sort, removal of adjacent duplicate rows, and the
+// semi-join filter that may precede an IndexSwap.
 //
 // sort_col0: key = col0 << 32 | col1, radix-sorted on the bits of col0 only
 // (LSD passes are stable, so equal keys keep their original order), with a
@@ -363,6 +364,135 @@ void unique_adjacent(vd_ctx *ctx, vd_rel *rel) {
     rel->nrows = kept;
 }
 
+// ---- semi-join filter ------------------------------------------------------------
+//
+// The rows of X whose key (column xcol) occurs in column ycol of Y, in X's
+// order.  Y's keys set bits of a bitmap over the key values; X's rows are
+// kept by their bit, counted per tile, scanned, and scattered as the
+// adjacent unique scatters (one ballot per round).  A join after a stable
+// sort of X on the key keeps exactly the rows this filter keeps, in the
+// same order, so the filter may run before the sort.
+
+// Largest key value the bitmap covers (32 MB of bits).
+constexpr uint32_t SEMI_MAX_KEY = (1u << 28) - 1;
+
+// Bit k of bits (word k / 32, bit k % 32) for each value k <= kmax of col.
+// Only a row whose value differs from the row before it sets a bit (a
+// sorted column's runs of one value, hot keys included, set it once), and
+// a wave merges the bits of its lanes that fall in one word, so one lane
+// per word sets them.
+__global__ __launch_bounds__(256) void k_semi_mark(const uint32_t *__restrict__ col, uint64_t n,
+                                                   uint32_t kmax, uint32_t *__restrict__ bits) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const int lane = (int)dev::lane_id();
+    const uint32_t k = i < n ? col[i] : ~0u;
+    const uint32_t before = lane == 0 && i > 0 && i <= n ? col[i - 1] : 0u;
+    const uint32_t up = (uint32_t)__shfl_up((int)k, 1, dev::WAVE);
+    const uint32_t prev = lane == 0 ? before : up;
+    const bool in = k <= kmax;
+    const uint32_t w = in ? k >> 5 : ~0u;
+    uint32_t b = in && (i == 0 || prev != k) ? 1u << (k & 31u) : 0u;
+    // after the step of offset o, a lane holds the bits of the lanes
+    // [lane, lane + 2o) of its run of one word
+#pragma unroll
+    for (int o = 1; o < dev::WAVE; o <<= 1) {
+        const uint32_t w2 = (uint32_t)__shfl_down((int)w, o, dev::WAVE);
+        const uint32_t b2 = (uint32_t)__shfl_down((int)b, o, dev::WAVE);
+        if (lane + o < dev::WAVE && w2 == w) b |= b2;
+    }
+    const uint32_t wp = (uint32_t)__shfl_up((int)w, 1, dev::WAVE);
+    if (b && (lane == 0 || wp != w)) atomicOr(bits + w, b);
+}
+
+// Keep flags of a tile (bit j: the row of round j exists and its key's bit
+// is set), all key loads then all bitmap loads in flight together.
+__device__ __forceinline__ uint32_t semi_keep_bits(const uint32_t *__restrict__ key, uint64_t n,
+                                                   uint64_t base, uint32_t kmax,
+                                                   const uint32_t *__restrict__ bits) {
+    uint32_t k[U_ITEMS], w[U_ITEMS];
+#pragma unroll
+    for (int j = 0; j < U_ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)j * U_BLOCK + threadIdx.x;
+        k[j] = i < n ? key[i] : ~0u;  // ~0u > kmax: not kept
+    }
+#pragma unroll
+    for (int j = 0; j < U_ITEMS; ++j) w[j] = k[j] <= kmax ? bits[k[j] >> 5] : 0u;
+    uint32_t keep = 0;
+#pragma unroll
+    for (int j = 0; j < U_ITEMS; ++j) keep |= ((w[j] >> (k[j] & 31u)) & 1u) << j;
+    return keep;
+}
+
+__global__ __launch_bounds__(U_BLOCK) void k_semi_count(const uint32_t *__restrict__ key,
+                                                        uint64_t n, uint32_t kmax,
+                                                        const uint32_t *__restrict__ bits,
+                                                        uint32_t *__restrict__ cnt) {
+    __shared__ uint32_t s_tmp[U_BLOCK / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * U_TILE;
+    const uint32_t c = (uint32_t)__popc(semi_keep_bits(key, n, base, kmax, bits));
+    const uint32_t t = dev::block_reduce<U_BLOCK>(c, s_tmp);
+    if (threadIdx.x == 0) cnt[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(U_BLOCK) void k_semi_scatter(Cols cols, const uint32_t *__restrict__ key,
+                                                          uint64_t n, uint32_t kmax,
+                                                          const uint32_t *__restrict__ bits,
+                                                          const uint64_t *__restrict__ off) {
+    __shared__ uint32_t s_cnt[U_ITEMS][U_BLOCK / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * U_TILE;
+    const uint32_t keep = semi_keep_bits(key, n, base, kmax, bits);
+    dev::unique_scatter_tile<U_BLOCK, U_ITEMS, 0>(keep, cols.in, cols.out, cols.n, base,
+                                                  off[blockIdx.x], s_cnt, nullptr);
+}
+
+// vd_semi_filter (see the C ABI).
+vd_rel *semi_filter(vd_ctx *ctx, const vd_rel *X, int xcol, const vd_rel *Y, int ycol,
+                    uint32_t key_hi, uint64_t max_rows, uint64_t *rows) {
+    uint32_t kmax = key_hi;
+    if ((X->bounded >> xcol) & 1u) kmax = std::min(kmax, X->colmax[xcol]);
+    if ((Y->bounded >> ycol) & 1u) kmax = std::min(kmax, Y->colmax[ycol]);
+    *rows = ~0ull;
+    if (kmax > SEMI_MAX_KEY) return nullptr;  // no bitmap that large
+    const uint64_t n = X->nrows, ny = Y->nrows;
+    auto new_out = [&](uint64_t kept) {
+        vd_rel *o = rel_new(ctx, X->ncols, kept);
+        for (int c = 0; c < X->ncols; ++c) rel_copy_bound(o, c, X, c);
+        o->distinct = X->distinct;  // rows of X, each once
+        return o;
+    };
+    if (n == 0 || ny == 0) {
+        *rows = 0;
+        return new_out(0);
+    }
+    const uint64_t words = (uint64_t)kmax / 32 + 1;
+    DBuf<uint32_t> bits(ctx, words);
+    fill_u32(ctx, bits.p, 0, words);
+    launch(ctx, "semi_mark", 4.0 * (double)ny, dim3((unsigned)ceil_div(ny, 256)), dim3(256), 0,
+           k_semi_mark, (const uint32_t *)Y->cols[ycol], ny, kmax, bits.p);
+    const uint64_t tiles = ceil_div(n, U_TILE);
+    DBuf<uint32_t> cnt(ctx, tiles);
+    DBuf<uint64_t> off(ctx, tiles + 1);
+    const uint32_t *key = X->cols[xcol];
+    launch(ctx, "semi_count", 4.0 * (double)n, dim3((unsigned)tiles), dim3(U_BLOCK), 0,
+           k_semi_count, key, n, kmax, (const uint32_t *)bits.p, cnt.p);
+    scan_u32_to_u64(ctx, cnt.p, off.p, tiles, off.p + tiles);
+    const uint64_t kept = read_u64(ctx, off.p + tiles);
+    *rows = kept;
+    if (kept > max_rows) return nullptr;
+    vd_rel *out = new_out(kept);
+    if (kept == 0) return out;
+    Cols cols{};
+    cols.n = X->ncols;
+    for (int c = 0; c < X->ncols; ++c) {
+        cols.in[c] = X->cols[c];
+        cols.out[c] = out->cols[c];
+    }
+    launch(ctx, "semi_scatter", 4.0 * (double)n + 4.0 * X->ncols * (double)kept,
+           dim3((unsigned)tiles), dim3(U_BLOCK), 0, k_semi_scatter, cols, key, n, kmax,
+           (const uint32_t *)bits.p, (const uint64_t *)off.p);
+    return out;
+}
+
 }  // namespace
 
 int swap_passes_bounded(vd_ctx *ctx, const uint32_t *h8, const uint32_t *h9,
@@ -423,6 +553,19 @@ int vd_sort_col0(vd_ctx *ctx, vd_rel *rel) {
 int vd_sort_lex(vd_ctx *ctx, vd_rel *rel) {
     if (!ctx || !rel) return VD_EINVAL;
     VD_TRY(ctx, sort_lex(ctx, rel));
+}
+
+int vd_semi_filter(vd_ctx *ctx, const vd_rel *X, int xcol, const vd_rel *Y, int ycol,
+                   uint32_t key_hi, uint64_t max_rows, vd_rel **out, uint64_t *rows) {
+    if (!ctx || !out || !rows) return VD_EINVAL;
+    *out = nullptr;
+    *rows = 0;
+    VD_TRY(ctx, {
+        VD_REQUIRE(X && Y, VD_EINVAL, "null relation");
+        VD_REQUIRE(xcol >= 0 && xcol < X->ncols && ycol >= 0 && ycol < Y->ncols, VD_EINVAL,
+                   "semi-join filter column out of range");
+        *out = semi_filter(ctx, X, xcol, Y, ycol, key_hi, max_rows, rows);
+    });
 }
 
 int vd_unique_adjacent(vd_ctx *ctx, vd_rel *rel) {

@@ -19,7 +19,7 @@ class Stats(C.Structure):
                 ("select_rows", C.c_uint64), ("swap_rows", C.c_uint64),
                 ("elapsed_ms", C.c_double), ("join_ms", C.c_double),
                 ("select_ms", C.c_double), ("swap_ms", C.c_double), ("jobs", C.c_int),
-                ("driven_joins", C.c_int)]
+                ("driven_joins", C.c_int), ("filtered_rows", C.c_uint64)]
 
 
 class BatchStats(C.Structure):

@@ -108,8 +108,16 @@ void FullRelationIR::setRelation(size_t i, size_t offset, const TYPEID *dbegin,
 }
 
 void FullRelationIR::sortByFirstColumn() {
+    pending_ = false;
     if (!rel_ || sorted0_) return;
     ctx_->check(vd_sort_col0(ctx_->raw(), rel_), "vd_sort_col0");
+}
+
+void FullRelationIR::replaceRelation(vd_rel *rel) {
+    if (!rel || vd_rel_ncols(rel) != (int)headers.size())
+        throw VedasError(VD_EINVAL, "replacement relation has other columns");
+    if (rel_ && ctx_) vd_rel_destroy(ctx_->raw(), rel_);
+    rel_ = rel;
 }
 
 void FullRelationIR::swapColumn(size_t i, size_t j) {

@@ -57,9 +57,18 @@ class FullRelationIR : public IR {
     // until column 0 changes.
     void markSortedOnColumn0() { sorted0_ = true; }
     bool sortedOnColumn0() const { return sorted0_; }
+    // An IndexSwapJob moved its variable to column 0 and left the sort to
+    // the join that consumes the relation (which may first drop the rows
+    // the join cannot match): sortByFirstColumn clears it.
+    void markPendingSort() { pending_ = true; }
+    bool pendingSort() const { return pending_; }
+    // Takes ownership of `rel` (same columns, e.g. a subset of the rows)
+    // in place of the current relation; headers stay.
+    void replaceRelation(vd_rel *rel);
 
   private:
     bool sorted0_ = false;
+    bool pending_ = false;
     VedasContext *ctx_ = nullptr;
     vd_rel *rel_ = nullptr;
     std::vector<bool> is_predicates;

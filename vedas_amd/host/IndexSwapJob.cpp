@@ -3,6 +3,8 @@
 #include <chrono>
 #include <iostream>
 
+#include "QueryExecutor.h"
+
 IndexSwapJob::IndexSwapJob(QueryJob *beforeJob, std::string swapVar, VedasContext *context,
                            VariableBoundMap *variables_bound, EmptyIntervalDict *ei_dict)
     : context(context), variables_bound(variables_bound), swapVar(swapVar),
@@ -26,10 +28,17 @@ int IndexSwapJob::startJob(int gpuId) {
     // A driven join already delivered this order (column 0 = swapVar).
     const bool done = col == 0 && ir->sortedOnColumn0();
     ir->swapColumn(0, col);
-    ir->sortByFirstColumn();
+    // The join that consumes a large relation sorts it after dropping the
+    // rows it cannot match (JoinQueryJob::join); such a relation is too
+    // large for the empty-interval update below either way.
+    const bool defer = !done && joinParent && QueryExecutor::FILTERED_SWAP_MIN_ROWS > 0 &&
+                       ir->size() >= QueryExecutor::FILTERED_SWAP_MIN_ROWS &&
+                       ir->size() >= EIF_ALPHA;
+    if (defer) ir->markPendingSort();
+    else ir->sortByFirstColumn();
     context->exe_log.emplace_back(gpuId, SWAP_OP, ir->getHeader(0) + " <-> " + ir->getHeader(col),
                                   ir->size(), ir->getColumnNum());
-    if (!done) context->swap_rows += ir->size();
+    if (!done && !defer) context->swap_rows += ir->size();
     context->swap_index_ns += std::chrono::duration_cast<std::chrono::nanoseconds>(
         std::chrono::steady_clock::now() - t0).count();
     if (ir->size() > 2 && ir->size() < EIF_ALPHA && ei_dict) {

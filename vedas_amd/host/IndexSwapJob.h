@@ -21,8 +21,13 @@ class IndexSwapJob : public QueryJob {
     std::string jobTypeName() const override;
     QueryJob *childJob() const { return beforeJob; }
     const std::string &swapVariable() const { return swapVar; }
+    // The parent job is a join on swapVar (QueryPlan::execute sets it): a
+    // relation of at least QueryExecutor::FILTERED_SWAP_MIN_ROWS rows is
+    // left unsorted for that join, which filters it first.
+    void setJoinParent(bool v) { joinParent = v; }
 
   private:
+    bool joinParent = false;
     VedasContext *context;
     VariableBoundMap *variables_bound{nullptr};
     std::string swapVar;

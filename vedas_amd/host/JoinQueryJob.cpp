@@ -83,6 +83,28 @@ IR *JoinQueryJob::join(FullRelationIR *lir, FullRelationIR *rir) {
 
     auto swap_start = std::chrono::steady_clock::now();
     if (join1Var) {
+        // An input an IndexSwapJob left unsorted (column 0 = the join
+        // variable): its rows whose key the other input lacks cannot join,
+        // so they are dropped before the sort when that keeps at most half
+        // of them.  The stable sort keeps the kept rows' order, so the join
+        // output is the one the full sort gives.  Which input is the left
+        // one was decided above on the unfiltered sizes, as without it.
+        auto settle = [&](FullRelationIR *x, FullRelationIR *y) {
+            if (!x->pendingSort()) return;
+            vd_rel *f = nullptr;
+            uint64_t kept = 0;
+            context->check(vd_semi_filter(context->raw(), x->raw(), 0, y->raw(), 0, 0xFFFFFFFFu,
+                                          x->size() / 2, &f, &kept),
+                           "vd_semi_filter");
+            if (f) {
+                context->filtered_rows += x->size() - kept;
+                x->replaceRelation(f);
+            }
+            x->sortByFirstColumn();
+            context->swap_rows += x->size();
+        };
+        settle(lir, rir);
+        settle(rir, lir);
         // Implicit index swap when the key is not column 0 (:162-180).
         const size_t lIdx = lir->getColumnId(joinVars[0]);
         if (lIdx != 0) {

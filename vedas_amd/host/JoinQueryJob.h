@@ -30,6 +30,8 @@ class JoinQueryJob : public QueryJob {
     // (QueryPlan::execute sets it): the join may then deliver its output in
     // that order directly (drivenJoin).
     void setSwapHint(const std::string &v) { swapHint = v; }
+    QueryJob *leftOperand() const { return leftJob; }
+    QueryJob *rightOperand() const { return rightJob; }
     VariableBoundMap *variables_bound{nullptr};
     EmptyIntervalDict *ei_dict{nullptr};
 

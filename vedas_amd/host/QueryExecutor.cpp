@@ -18,6 +18,7 @@ bool QueryExecutor::ENABLE_BOUND_PRESCAN = true;
 // one input is a select with its other permutation resident (no
 // counterpart in the reference; off runs the join and the re-sort).
 bool QueryExecutor::ENABLE_DRIVEN_SWAP = true;
+uint64_t QueryExecutor::FILTERED_SWAP_MIN_ROWS = 1u << 20;
 
 QueryExecutor::QueryExecutor(VedasStorage *s, VedasContext *c) : vedasStorage(s), context(c) {}
 

@@ -30,6 +30,9 @@ class QueryExecutor {
     static bool ENABLE_UPDATE_BOUND_AFTER_JOIN;
     static bool ENABLE_BOUND_PRESCAN;
     static bool ENABLE_DRIVEN_SWAP;
+    // Smallest relation an IndexSwapJob feeding a join leaves to that join
+    // to filter and sort (0: every IndexSwapJob sorts).
+    static uint64_t FILTERED_SWAP_MIN_ROWS;
     // Bounds of the join variables after the last query (for inspection).
     const VariableBoundMap &bounds() const { return variables_bound; }
 

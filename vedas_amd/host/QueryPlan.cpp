@@ -58,6 +58,13 @@ void QueryPlan::execute(SparqlResult &result) {
         if (auto *sw = dynamic_cast<IndexSwapJob *>(job))
             if (auto *jj = dynamic_cast<JoinQueryJob *>(sw->childJob()))
                 jj->setSwapHint(sw->swapVariable());
+    // An IndexSwapJob whose output feeds a join on its variable leaves a
+    // large relation's sort to that join.
+    for (auto *job : jobs)
+        if (auto *jj = dynamic_cast<JoinQueryJob *>(job))
+            for (QueryJob *in : {jj->leftOperand(), jj->rightOperand()})
+                if (auto *sw = dynamic_cast<IndexSwapJob *>(in))
+                    sw->setJoinParent(sw->swapVariable() == jj->getJoinVariable());
     for (auto &v : select_variables) query_variable_counter[v] = ULONG_MAX;
     for (auto *job : jobs) {
         if (auto *jj = dynamic_cast<JoinQueryJob *>(job)) jj->setQueryVarCounter(&query_variable_counter);

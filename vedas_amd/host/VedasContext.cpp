@@ -28,5 +28,6 @@ void VedasContext::resetStats() {
     eif_count = 0;
     join_rows = select_rows = swap_rows = 0;
     driven_joins = 0;
+    filtered_rows = 0;
     exe_log.clear();
 }

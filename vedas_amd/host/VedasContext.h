@@ -34,6 +34,7 @@ class VedasContext {
     int eif_count = 0;
     uint64_t join_rows = 0, select_rows = 0, swap_rows = 0;
     uint64_t driven_joins = 0;  // joins that delivered their IndexSwap order
+    uint64_t filtered_rows = 0;  // rows dropped before a join's deferred sort
     std::vector<ExecuteLogRecord> exe_log;
     void resetStats();
 

@@ -242,6 +242,7 @@ void fill_stats(vdq_result *res, VedasContext *ctx) {
     res->stats.swap_ms = ctx->swap_index_ns / 1e6;
     res->stats.jobs = (int)ctx->exe_log.size();
     res->stats.driven_joins = (int)ctx->driven_joins;
+    res->stats.filtered_rows = ctx->filtered_rows;
 }
 
 }  // namespace
@@ -782,6 +783,9 @@ int vdq_set_option(const char *key, int value) {
         QueryExecutor::ENABLE_UPDATE_BOUND_AFTER_JOIN = value != 0;
     } else if (k == "driven_swap") {
         QueryExecutor::ENABLE_DRIVEN_SWAP = value != 0;
+    } else if (k == "filtered_swap") {
+        if (value < 0) return fail(VD_EINVAL, "filtered_swap takes a row count >= 0");
+        QueryExecutor::FILTERED_SWAP_MIN_ROWS = (uint64_t)value;
     } else {
         return fail(VD_EINVAL, "unknown option " + k);
     }
@@ -797,6 +801,8 @@ int vdq_get_option(const char *key, int *value) {
         *value = QueryExecutor::ENABLE_UPDATE_BOUND_AFTER_JOIN;
     } else if (k == "driven_swap") {
         *value = QueryExecutor::ENABLE_DRIVEN_SWAP;
+    } else if (k == "filtered_swap") {
+        *value = (int)std::min<uint64_t>(QueryExecutor::FILTERED_SWAP_MIN_ROWS, INT32_MAX);
     } else {
         return fail(VD_EINVAL, "unknown option " + k);
     }
