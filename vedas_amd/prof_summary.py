@@ -39,6 +39,9 @@ LABELS = [
     (r"k_join_scan_total\b", "scan_tiles"),
     (r"k_rb_copy\b", "read_back"),
     (r"k_interleave<", "interleave"),
+    (r"k_semi_mark\b", "semi_mark"),
+    (r"k_semi_count\b", "semi_count"),
+    (r"k_semi_scatter\b", "semi_scatter"),
     (r"k_join_ranges", "join_ranges"),
     (r"k_pair_totals\b", "join_totals"),
     (r"k_pack_xyz\b|k_pack_split\b|k_pack_xpos\b", "index_pack"),
