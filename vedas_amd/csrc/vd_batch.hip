@@ -22,6 +22,11 @@ constexpr int MB_ITEMS = 8;
 constexpr int MB_TILE = MB_BLOCK * MB_ITEMS;  // rows per tile
 constexpr int MJ_BLOCK = 256;
 constexpr int MJ_RCAP = 8192;  // staged right keys (32 KB)
+// The bounds kernel's blocks: 512 threads x 4 rows per MB_TILE-row tile
+// (its 32 KB of staged keys allow 4 blocks per CU: 32 waves).
+constexpr int MJB_BLOCK = 512;
+constexpr int MJB_ITEMS = 4;
+static_assert(MJB_BLOCK * MJB_ITEMS == MB_TILE, "bounds blocks cover a tile");
 
 __device__ __forceinline__ int find_seg(const uint64_t *base, int n, uint64_t x) {
     // last j in [0, n) with base[j] <= x (base[0] == 0, non-decreasing)
@@ -123,12 +128,12 @@ __global__ __launch_bounds__(256) void k_join_ranges_multi(const JPairDev *P, in
 // rows start on a tile (its rows past nl are padding with no matches):
 // lower / loc are written for all OFF_TILE rows of the tile.
 static_assert(MB_TILE == OFF_TILE, "bounds tiles are the offset tiles");
-__global__ __launch_bounds__(MB_BLOCK) void k_join_bounds_multi(
+__global__ __launch_bounds__(MJB_BLOCK) void k_join_bounds_multi(
     const JPairDev *P, int np, const uint64_t *tbase, const uint64_t *ranges, uint32_t *lower,
     uint32_t *loc, uint64_t *tile_total, uint64_t *wide) {
     static_assert(MJ_RCAP * 4 >= MB_TILE * 12, "counts and offsets reuse the staging buffer");
     __shared__ uint32_t s_r[MJ_RCAP];
-    __shared__ uint64_t s_tmp[MB_BLOCK / 64];
+    __shared__ uint64_t s_tmp[MJB_BLOCK / 64];
     __shared__ int s_j;
     if (threadIdx.x == 0) s_j = find_seg(tbase, np, blockIdx.x);
     __syncthreads();
@@ -142,77 +147,77 @@ __global__ __launch_bounds__(MB_BLOCK) void k_join_bounds_multi(
     const uint64_t nd = (m + k - 1) / k;
     // Staging and the left keys in batches of loads issued before their use.
     constexpr int SB = 8;
-    for (uint64_t t0 = 0; t0 < nd; t0 += (uint64_t)SB * MB_BLOCK) {
+    for (uint64_t t0 = 0; t0 < nd; t0 += (uint64_t)SB * MJB_BLOCK) {
         uint32_t v[SB];
 #pragma unroll
         for (int u = 0; u < SB; ++u) {
-            const uint64_t t = t0 + (uint64_t)u * MB_BLOCK + threadIdx.x;
+            const uint64_t t = t0 + (uint64_t)u * MJB_BLOCK + threadIdx.x;
             if (t < nd) v[u] = rk[rlo + t * k];
         }
 #pragma unroll
         for (int u = 0; u < SB; ++u) {
-            const uint64_t t = t0 + (uint64_t)u * MB_BLOCK + threadIdx.x;
+            const uint64_t t = t0 + (uint64_t)u * MJB_BLOCK + threadIdx.x;
             if (t < nd) s_r[t] = v[u];
         }
     }
-    uint32_t lkey[MB_ITEMS];
+    uint32_t lkey[MJB_ITEMS];
 #pragma unroll
-    for (int it = 0; it < MB_ITEMS; ++it) {
-        const uint64_t i = base + (uint64_t)it * MB_BLOCK + threadIdx.x;
+    for (int it = 0; it < MJB_ITEMS; ++it) {
+        const uint64_t i = base + (uint64_t)it * MJB_BLOCK + threadIdx.x;
         lkey[it] = i < nl ? lk[i] : 0u;
     }
     __syncthreads();
     // first staged entry >= key (t0) and > key (t1) of every row
-    uint32_t t0[MB_ITEMS], t1[MB_ITEMS];
-    dev::lockstep_bounds<MB_ITEMS>(s_r, (uint32_t)nd, lkey, t0, t1);
+    uint32_t t0[MJB_ITEMS], t1[MJB_ITEMS];
+    dev::lockstep_bounds<MJB_ITEMS>(s_r, (uint32_t)nd, lkey, t0, t1);
     const uint64_t row0 = (uint64_t)blockIdx.x * MB_TILE;  // tile b in the row space
     // bounds relative to rlo (with a directory: exact within its gaps)
-    uint32_t lo[MB_ITEMS], hi[MB_ITEMS];
+    uint32_t lo[MJB_ITEMS], hi[MJB_ITEMS];
     if (k > 1) {
-        bool valid[MB_ITEMS];
+        bool valid[MJB_ITEMS];
 #pragma unroll
-        for (int it = 0; it < MB_ITEMS; ++it)
-            valid[it] = base + (uint64_t)it * MB_BLOCK + threadIdx.x < nl;
-        dev::lockstep_gap_bounds<MB_ITEMS>([&](uint32_t q) { return rk[rlo + q]; }, k, m, lkey,
+        for (int it = 0; it < MJB_ITEMS; ++it)
+            valid[it] = base + (uint64_t)it * MJB_BLOCK + threadIdx.x < nl;
+        dev::lockstep_gap_bounds<MJB_ITEMS>([&](uint32_t q) { return rk[rlo + q]; }, k, m, lkey,
                                            valid, t0, t1, lo, hi);
     } else {
 #pragma unroll
-        for (int it = 0; it < MB_ITEMS; ++it) {
+        for (int it = 0; it < MJB_ITEMS; ++it) {
             lo[it] = t0[it];
             hi[it] = t1[it];
         }
     }
-    uint32_t cnt[MB_ITEMS];
+    uint32_t cnt[MJB_ITEMS];
 #pragma unroll
-    for (int it = 0; it < MB_ITEMS; ++it) {
-        const uint64_t i = base + (uint64_t)it * MB_BLOCK + threadIdx.x;
+    for (int it = 0; it < MJB_ITEMS; ++it) {
+        const uint64_t i = base + (uint64_t)it * MJB_BLOCK + threadIdx.x;
         uint32_t c = 0, low = 0;
         if (i < nl) {
             low = (uint32_t)(rlo + lo[it]);
             c = hi[it] - lo[it];
         }
-        lower[row0 + (uint64_t)it * MB_BLOCK + threadIdx.x] = low;
+        lower[row0 + (uint64_t)it * MJB_BLOCK + threadIdx.x] = low;
         cnt[it] = c;
     }
     __syncthreads();  // the staged right keys are no longer read
     uint64_t *s_off = reinterpret_cast<uint64_t *>(s_r);
     uint32_t *s_cnt = reinterpret_cast<uint32_t *>(s_off + MB_TILE);
 #pragma unroll
-    for (int it = 0; it < MB_ITEMS; ++it) s_cnt[it * MB_BLOCK + threadIdx.x] = cnt[it];
+    for (int it = 0; it < MJB_ITEMS; ++it) s_cnt[it * MJB_BLOCK + threadIdx.x] = cnt[it];
     __syncthreads();
     // Thread t scans rows t*ITEMS .. t*ITEMS+ITEMS-1 of the tile.
     uint64_t run = 0;
-    uint32_t mine[MB_ITEMS];
+    uint32_t mine[MJB_ITEMS];
 #pragma unroll
-    for (int it = 0; it < MB_ITEMS; ++it) {
-        mine[it] = s_cnt[threadIdx.x * MB_ITEMS + it];
+    for (int it = 0; it < MJB_ITEMS; ++it) {
+        mine[it] = s_cnt[threadIdx.x * MJB_ITEMS + it];
         run += mine[it];
     }
     uint64_t tot;
-    uint64_t ex = dev::block_excl_scan<MB_BLOCK>(run, s_tmp, &tot);
+    uint64_t ex = dev::block_excl_scan<MJB_BLOCK>(run, s_tmp, &tot);
 #pragma unroll
-    for (int it = 0; it < MB_ITEMS; ++it) {
-        s_off[threadIdx.x * MB_ITEMS + it] = ex;
+    for (int it = 0; it < MJB_ITEMS; ++it) {
+        s_off[threadIdx.x * MJB_ITEMS + it] = ex;
         ex += mine[it];
     }
     if (threadIdx.x == 0) {
@@ -221,9 +226,9 @@ __global__ __launch_bounds__(MB_BLOCK) void k_join_bounds_multi(
     }
     __syncthreads();
 #pragma unroll
-    for (int it = 0; it < MB_ITEMS; ++it)
-        loc[row0 + (uint64_t)it * MB_BLOCK + threadIdx.x] =
-            (uint32_t)s_off[it * MB_BLOCK + threadIdx.x];
+    for (int it = 0; it < MJB_ITEMS; ++it)
+        loc[row0 + (uint64_t)it * MJB_BLOCK + threadIdx.x] =
+            (uint32_t)s_off[it * MJB_BLOCK + threadIdx.x];
 }
 
 // Output size of every pair from the global offsets.
@@ -631,7 +636,7 @@ void join_multi_chunk(vd_ctx *ctx, const vd_rel *const *L, const vd_rel *const *
         launch(ctx, "join_ranges", 0.0, dim3((unsigned)ceil_div(ntiles * 2 * MR_GROUP, 256)),
                dim3(256), 0, k_join_ranges_multi, (const JPairDev *)dp.p, np,
                (const uint64_t *)dtb.p, ntiles, ranges.p);
-        launch(ctx, "join_bounds", bbytes, dim3((unsigned)ntiles), dim3(MB_BLOCK), 0,
+        launch(ctx, "join_bounds", bbytes, dim3((unsigned)ntiles), dim3(MJB_BLOCK), 0,
                k_join_bounds_multi, (const JPairDev *)dp.p, np, (const uint64_t *)dtb.p,
                (const uint64_t *)ranges.p, lower.p, loc.p, ttot.p, dtot.p + np);
         scan_tile_totals(ctx, ttot.p, ntiles, tpre.p);

@@ -32,8 +32,12 @@
 namespace vd {
 namespace {
 
-constexpr int JB_BLOCK = 256;
-constexpr int JB_ITEMS = 8;
+// 512 threads x 4 rows per 2048-row tile: the 32 KB of staged right keys
+// allow 4 blocks per CU, so 512-thread blocks keep 32 waves per CU where
+// 256-thread ones kept 16 (the searches are latency-bound: 0.47 -> 0.40 ms
+// of bounds per join-set step).
+constexpr int JB_BLOCK = 512;
+constexpr int JB_ITEMS = 4;
 constexpr int JB_TILE = JB_BLOCK * JB_ITEMS;
 constexpr int JX_BLOCK = 256;
 constexpr int JB_RCAP_BYTES = 32768;  // LDS for staged right keys
