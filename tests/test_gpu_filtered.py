@@ -130,3 +130,41 @@ def test_plans_with_and_without_filtered_swaps(shape):
     rel = ex.run(node)
     assert O.bag_digest(h1, c1) == O.bag_digest(rel.headers, ex.columns(rel))
     assert get_option("filtered_swap") == default
+
+
+def test_batch_with_and_without_filtered_swaps():
+    """The batched executor with every IndexSwap left to its join (filter
+    forced on) and with none: each query's result equals the other run's
+    and the oracle's plan walk (canonical order), and fewer rows are
+    re-sorted."""
+    from tests import plans
+    from vedas_amd.engine import Engine, get_option, query_text, set_option
+    s, p, o = distinct_triples(120_000, 37, ns=8000, npred=4, no=9000)
+    keep = (p != 4) | (np.arange(len(p)) % 25 == 0)
+    s, p, o = s[keep], p[keep], o[keep]
+    shapes = [plans.snowflake_plan([1, 2], [3, 4]), plans.chain_plan([1, 2, 3, 4]),
+              plans.chain_plan([2, 1, 4]), plans.snowflake_plan([2, 3], [4, 1])]
+    texts = [query_text(pats, node) for pats, node in shapes]
+    default = get_option("filtered_swap")
+    e = Engine(0)
+    try:
+        e.load(s, p, o, ("PSO", "POS"))
+        got, stats = {}, {}
+        for on in (1, 0):
+            set_option("filtered_swap", on)
+            res, st = e.batch(texts)
+            got[on] = [O.bag_digest(r.headers(), r.columns()) for r in res]
+            stats[on] = st
+            for r in res:
+                r.free()
+    finally:
+        set_option("filtered_swap", default)
+        e.close()
+    print(f"batched swap rows {stats[1]['swap_rows']} / {stats[0]['swap_rows']}", flush=True)
+    assert got[1] == got[0]
+    assert stats[1]["swap_rows"] < stats[0]["swap_rows"]
+    assert stats[1]["join_rows"] == stats[0]["join_rows"]
+    ex = plans.OracleExecutor(s, p, o)
+    for (pats, node), g in zip(shapes, got[1]):
+        rel = ex.run(node)
+        assert O.bag_digest(rel.headers, ex.columns(rel)) == g

@@ -4,6 +4,7 @@
 #include <functional>
 
 #include "FullRelationIR.h"
+#include "QueryExecutor.h"
 
 struct BatchExecutor::Rel {
     vd_rel *rel = nullptr;
@@ -15,6 +16,8 @@ struct BatchExecutor::Rel {
         throw VedasError(VD_EINVAL, "no column for " + v);
     }
     uint64_t rows() const { return rel ? vd_rel_nrows(rel) : 0; }
+    // moved to column 0 by an index swap, sorted by the join that takes it
+    bool pending = false;
 };
 
 struct BatchExecutor::Node {
@@ -24,6 +27,7 @@ struct BatchExecutor::Node {
     TriplePattern *tp = nullptr;
     bool done = false;
     int owner = -1;  // node holding the relation this node's result lives in
+    int parent = -1;
     Rel out;
 };
 
@@ -163,14 +167,23 @@ void BatchExecutor::joinRound(std::vector<std::pair<Query *, int>> &joins,
                               std::vector<std::pair<Query *, int>> &swaps, BatchStats &st) {
     vd_ctx *c = context->raw();
     std::vector<vd_rel *> to_sort;
-    // Index swaps: move the variable to column 0, sort below.
+    // Index swaps: move the variable to column 0, sort below.  A large
+    // relation whose parent joins on that variable is left to the join,
+    // which first drops the rows the other input cannot match (as
+    // IndexSwapJob and JoinQueryJob::join do on the one-query path).
     for (auto &qn : swaps) {
         Query *Q = qn.first;
         Node &n = Q->nodes[qn.second];
         Rel &r = Q->nodes[Q->nodes[n.c1].owner].out;
         swap_cols(this, r.rel, r.headers, r.is_pred, 0, r.col(n.var));
-        to_sort.push_back(r.rel);
-        st.swap_rows += r.rows();
+        const Node *par = n.parent >= 0 ? &Q->nodes[n.parent] : nullptr;
+        const uint64_t min_rows = QueryExecutor::FILTERED_SWAP_MIN_ROWS;
+        if (par && par->op == JOIN && par->var == n.var && min_rows > 0 && r.rows() >= min_rows) {
+            r.pending = true;
+        } else {
+            to_sort.push_back(r.rel);
+            st.swap_rows += r.rows();
+        }
         n.owner = Q->nodes[n.c1].owner;
     }
     // Joins: operand order, projection push-down, key to column 0
@@ -197,6 +210,22 @@ void BatchExecutor::joinRound(std::vector<std::pair<Query *, int>> &joins,
             const size_t k = X->col(n.var);
             if (k != 0) {
                 swap_cols(this, X->rel, X->headers, X->is_pred, 0, k);
+                resort = true;
+            }
+            if (X->pending) {
+                // keep the rows whose key the other input holds (when that
+                // is at most half of them), then sort
+                Rel *Y = X == L ? R : L;
+                vd_rel *f = nullptr;
+                uint64_t kept = 0;
+                context->check(vd_semi_filter(c, X->rel, 0, Y->rel, 0, 0xFFFFFFFFu,
+                                              X->rows() / 2, &f, &kept),
+                               "vd_semi_filter");
+                if (f) {
+                    vd_rel_destroy(c, X->rel);
+                    X->rel = f;
+                }
+                X->pending = false;
                 resort = true;
             }
             if (resort) {
@@ -272,7 +301,10 @@ void BatchExecutor::run(std::vector<BatchQuery> &queries, std::vector<SparqlResu
                 throw VedasError(VD_EINVAL, "malformed plan tree");
             Q.nodes.push_back(n);
             if (n.op == JOIN) Q.counter[n.var] += 1;  // QueryPlan::pushJob
-            return (int)Q.nodes.size() - 1;
+            const int self = (int)Q.nodes.size() - 1;
+            if (a >= 0) Q.nodes[a].parent = self;
+            if (b >= 0) Q.nodes[b].parent = self;
+            return self;
         };
         if (Q.q->root) {
             Q.root = walk(Q.q->root);

@@ -10,7 +10,10 @@
 // rules: smaller input left, projection push-down, key to column 0, output
 // [key, left rest, right rest]); the result of each query equals its
 // single-query result row for row after canonical ordering.  Bound and
-// empty-interval pruning (result-neutral) are not applied in batches.
+// empty-interval pruning (result-neutral) are not applied in batches; a
+// large relation an index swap feeds to a join on its variable is
+// filtered by that join's other input before its sort, as on the
+// one-query path.
 #ifndef VEDAS_HOST_BATCHEXECUTOR_H
 #define VEDAS_HOST_BATCHEXECUTOR_H
 
