@@ -54,16 +54,35 @@ struct SelJobDev {
     uint32_t k1, k2;
 };
 
-// range[2j], range[2j+1] = the rows [begin, end) matching job j.
+// Words of a one-block kernel read back straight from the block (each
+// thread's stores, its system-scope fence, one barrier, then the sequence
+// word), so no copy kernel runs after it.  rb.dst == nullptr: nothing.
+__device__ __forceinline__ void block_publish(const Readback &rb) {
+    if (!rb.dst) return;
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0)
+        __hip_atomic_store(rb.dst + RB_WORDS, rb.seq, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// range[2j], range[2j+1] = the rows [begin, end) matching job j (also to
+// rb.dst when the launch is one block and rb.dst is set).
 __global__ __launch_bounds__(256) void k_sel_lookup_multi(const SelJobDev *jobs, int nj,
-                                                          uint64_t *range) {
+                                                          uint64_t *range, Readback rb) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nj) return;
-    const SelJobDev &J = jobs[j];
-    uint64_t b, e;
-    lookup_one(J.pd, J.k1, J.k2, J.k2 != VD_UNBOUND, &b, &e);
-    range[2 * j] = b;
-    range[2 * j + 1] = e;
+    if (j < nj) {
+        const SelJobDev &J = jobs[j];
+        uint64_t b, e;
+        lookup_one(J.pd, J.k1, J.k2, J.k2 != VD_UNBOUND, &b, &e);
+        range[2 * j] = b;
+        range[2 * j + 1] = e;
+        if (rb.dst) {
+            rb.dst[2 * j] = b;
+            rb.dst[2 * j + 1] = e;
+        }
+    }
+    block_publish(rb);
 }
 
 // The lookup of k_sel_lookup_multi plus the first and last value of the
@@ -77,15 +96,24 @@ struct BoundJobDev {
 };
 
 __global__ __launch_bounds__(256) void k_sel_bounds_multi(const BoundJobDev *jobs, int nj,
-                                                          uint64_t *out) {
+                                                          uint64_t *out, Readback rb) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nj) return;
-    const BoundJobDev &J = jobs[j];
-    uint64_t b, e;
-    lookup_one(J.pd, J.k1, J.k2, J.k2 != VD_UNBOUND, &b, &e);
-    out[3 * j] = b;
-    out[3 * j + 1] = e;
-    out[3 * j + 2] = e > b ? ((uint64_t)J.col[b] | ((uint64_t)J.col[e - 1] << 32)) : 0;
+    if (j < nj) {
+        const BoundJobDev &J = jobs[j];
+        uint64_t b, e;
+        lookup_one(J.pd, J.k1, J.k2, J.k2 != VD_UNBOUND, &b, &e);
+        const uint64_t ends =
+            e > b ? ((uint64_t)J.col[b] | ((uint64_t)J.col[e - 1] << 32)) : 0;
+        out[3 * j] = b;
+        out[3 * j + 1] = e;
+        out[3 * j + 2] = ends;
+        if (rb.dst) {
+            rb.dst[3 * j] = b;
+            rb.dst[3 * j + 1] = e;
+            rb.dst[3 * j + 2] = ends;
+        }
+    }
+    block_publish(rb);
 }
 
 // ---- join --------------------------------------------------------------------
@@ -529,10 +557,14 @@ std::vector<uint64_t> lookup_multi(vd_ctx *ctx, const vd_index *idx, const vd_se
     }
     DBuf<SelJobDev> dj = to_device(ctx, hj);
     DBuf<uint64_t> range(ctx, 2 * (uint64_t)nj);
-    launch(ctx, "select_lookup", 24.0 * nj, dim3((unsigned)ceil_div(nj, 256)), dim3(256), 0,
-           k_sel_lookup_multi, (const SelJobDev *)dj.p, nj, range.p);
     std::vector<uint64_t> hr(2 * (size_t)nj);
-    download(ctx, hr.data(), range.p, hr.size() * sizeof(uint64_t));
+    // one block: the kernel publishes the words itself
+    const bool direct = nj <= 256 && 2 * nj <= RB_WORDS;
+    const Readback rb = direct ? rb_next(ctx) : Readback{};
+    launch(ctx, "select_lookup", 24.0 * nj, dim3((unsigned)ceil_div(nj, 256)), dim3(256), 0,
+           k_sel_lookup_multi, (const SelJobDev *)dj.p, nj, range.p, rb);
+    if (direct) rb_wait(ctx, rb, hr.data(), 2 * nj);
+    else download(ctx, hr.data(), range.p, hr.size() * sizeof(uint64_t));
     return hr;
 }
 
@@ -548,10 +580,14 @@ std::vector<uint64_t> bounds_multi(vd_ctx *ctx, const vd_index *idx, const vd_se
     }
     DBuf<BoundJobDev> dj = to_device(ctx, hj);
     DBuf<uint64_t> out(ctx, 3 * (uint64_t)nj);
-    launch(ctx, "select_bounds", 32.0 * nj, dim3((unsigned)ceil_div(nj, 256)), dim3(256), 0,
-           k_sel_bounds_multi, (const BoundJobDev *)dj.p, nj, out.p);
     std::vector<uint64_t> hr(3 * (size_t)nj);
-    download(ctx, hr.data(), out.p, hr.size() * sizeof(uint64_t));
+    // one block: the kernel publishes the words itself
+    const bool direct = nj <= 256 && 3 * nj <= RB_WORDS;
+    const Readback rb = direct ? rb_next(ctx) : Readback{};
+    launch(ctx, "select_bounds", 32.0 * nj, dim3((unsigned)ceil_div(nj, 256)), dim3(256), 0,
+           k_sel_bounds_multi, (const BoundJobDev *)dj.p, nj, out.p, rb);
+    if (direct) rb_wait(ctx, rb, hr.data(), 3 * nj);
+    else download(ctx, hr.data(), out.p, hr.size() * sizeof(uint64_t));
     return hr;
 }
 

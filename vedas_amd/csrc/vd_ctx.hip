@@ -114,6 +114,11 @@ void prof_push(vd_ctx *ctx, const char *name, double bytes, hipEvent_t ev0, hipE
     ctx->pending.push_back({name, bytes, ev0, ev1});
 }
 
+// Timed launches whose events are read only when this many are pending, or
+// at the vd_prof_* calls: reading an event pair costs host time that would
+// otherwise sit between a read-back and the next launch.
+constexpr size_t PROF_PENDING_MAX = 1024;
+
 void prof_collect(vd_ctx *ctx, bool done_only) {
     size_t k = 0;
     for (; k < ctx->pending.size(); ++k) {
@@ -153,7 +158,9 @@ void sync(vd_ctx *ctx) {
             break;
         }
     }
-    if (!ctx->pending.empty()) prof_collect(ctx, false);
+    // timed launches are read at vd_prof_* calls (off the launch path),
+    // here only when many are pending
+    if (ctx->pending.size() >= PROF_PENDING_MAX) prof_collect(ctx, false);
 }
 
 // ---- small read-backs -------------------------------------------------------------
@@ -194,7 +201,7 @@ void rb_wait(vd_ctx *ctx, const Readback &rb, uint64_t *dst, int n) {
     }
     std::memcpy(dst, ctx->h_rb, (size_t)n * sizeof(uint64_t));
     if (ctx->rb_mode == 1) ctx->rb_log.insert(ctx->rb_log.end(), dst, dst + n);
-    if (!ctx->pending.empty()) prof_collect(ctx, true);
+    if (ctx->pending.size() >= PROF_PENDING_MAX) prof_collect(ctx, true);
 }
 
 __global__ __launch_bounds__(64) void k_rb_copy(const uint64_t *__restrict__ src, int n,
@@ -688,6 +695,7 @@ int vd_prof_reset(vd_ctx *ctx) {
     if (!ctx) return VD_EINVAL;
     VD_TRY(ctx, {
         vd::sync(ctx);
+        vd::prof_collect(ctx, false);
         ctx->prof_stats.clear();
         ctx->prof_names.clear();
     });
@@ -697,6 +705,7 @@ int vd_prof_count(vd_ctx *ctx) {
     if (!ctx) return VD_EINVAL;
     try {
         vd::sync(ctx);
+        vd::prof_collect(ctx, false);
     } catch (const Error &e) {
         ctx->last_error = e.what();
         return e.code;

@@ -599,13 +599,34 @@ struct SkipSet {
     vd_interval_t s[4];
 };
 
+// Pieces as read-back words: np, src[0..MAX_PIECES), dst[0..MAX_PIECES].
+constexpr int PIECE_WORDS = 1 + MAX_PIECES + MAX_PIECES + 1;
+__device__ __forceinline__ void pieces_to_words(const Pieces &P, uint64_t *w) {
+    w[0] = (uint64_t)P.np;
+#pragma unroll
+    for (int i = 0; i < MAX_PIECES; ++i) w[1 + i] = P.src[i];
+#pragma unroll
+    for (int i = 0; i <= MAX_PIECES; ++i) w[1 + MAX_PIECES + i] = P.dst[i];
+}
+Pieces pieces_from_words(const uint64_t *w) {
+    Pieces P{};
+    P.np = (int)w[0];
+    for (int i = 0; i < MAX_PIECES; ++i) P.src[i] = w[1 + i];
+    for (int i = 0; i <= MAX_PIECES; ++i) P.dst[i] = w[1 + MAX_PIECES + i];
+    return P;
+}
+
 // One wave: pieces of a given row range.
 __global__ __launch_bounds__(64) void k_select_pieces(const uint32_t *__restrict__ ycol,
                                                       uint64_t b, uint64_t e, int clip,
                                                       uint32_t lo, uint32_t hi, SkipSet sk,
-                                                      int nskip, Pieces *out) {
+                                                      int nskip, Pieces *out, Readback rb) {
     const Pieces P = make_pieces<true>(ycol, b, e, clip, lo, hi, sk.s, nskip);
-    if (threadIdx.x == 0) *out = P;
+    if (threadIdx.x == 0) {
+        *out = P;
+        pieces_to_words(P, rb.dst);  // the host reads them back from here
+    }
+    dev::rb_publish(rb);
 }
 
 // Pieces plus the key's row range, so a lookup and a select cost one
@@ -637,7 +658,7 @@ __device__ __forceinline__ void wave_lookup(const PermDev &pd, uint32_t k1, uint
 __global__ __launch_bounds__(64) void k_lookup_pieces(PermDev pd, uint32_t k1, int has_k2,
                                                       uint32_t k2, int clip, uint32_t lo,
                                                       uint32_t hi, SkipSet sk, int nskip,
-                                                      KeyPieces *out) {
+                                                      KeyPieces *out, Readback rb) {
     uint64_t b, e;
     wave_lookup(pd, k1, k2, has_k2 != 0, &b, &e);
     const Pieces P = make_pieces<true>(pd.y, b, e, clip, lo, hi, sk.s, nskip);
@@ -645,7 +666,12 @@ __global__ __launch_bounds__(64) void k_lookup_pieces(PermDev pd, uint32_t k1, i
         out->begin = b;
         out->end = e;
         out->p = P;
+        // read back: begin, end, then the pieces
+        rb.dst[0] = b;
+        rb.dst[1] = e;
+        pieces_to_words(P, rb.dst + 2);
     }
+    dev::rb_publish(rb);
 }
 
 __global__ __launch_bounds__(256) void k_copy_pieces(
@@ -1399,10 +1425,12 @@ int vd_select(vd_ctx *ctx, const vd_index *idx, int perm, uint64_t begin,
             return VD_OK;
         }
         DBuf<Pieces> dp(ctx, 1);
+        const Readback rb = rb_next(ctx);
         launch(ctx, "select_pieces", 0.0, dim3(1), dim3(64), 0, k_select_pieces,
-               (const uint32_t *)pa.y, begin, end, clip, lo, hi, sk, nskip, dp.p);
-        Pieces hp;
-        download(ctx, &hp, dp.p, sizeof(Pieces));
+               (const uint32_t *)pa.y, begin, end, clip, lo, hi, sk, nskip, dp.p, rb);
+        uint64_t w[PIECE_WORDS];
+        rb_wait(ctx, rb, w, PIECE_WORDS);
+        const Pieces hp = pieces_from_words(w);
         *out = select_pieces_rel(ctx, idx, pa, cols, distinct, hp, dp.p);
     });
 }
@@ -1417,13 +1445,15 @@ int vd_select_key(vd_ctx *ctx, const vd_index *idx, int perm, uint32_t k1, int h
         const vd_perm_arrays &pa = get_perm(idx, perm);
         const SkipSet sk = check_select(cols, clip, skip, nskip, distinct);
         DBuf<KeyPieces> dk(ctx, 1);
+        const Readback rb = rb_next(ctx);
         launch(ctx, "select_pieces", 0.0, dim3(1), dim3(64), 0, k_lookup_pieces, perm_dev(pa),
-               k1, has_k2, k2, clip, lo, hi, sk, nskip, dk.p);
-        KeyPieces hk;
-        download(ctx, &hk, dk.p, sizeof(KeyPieces));
-        if (begin) *begin = hk.begin;
-        if (end) *end = hk.end;
-        *out = select_pieces_rel(ctx, idx, pa, cols, distinct, hk.p, &dk.p->p);
+               k1, has_k2, k2, clip, lo, hi, sk, nskip, dk.p, rb);
+        uint64_t w[2 + PIECE_WORDS];
+        rb_wait(ctx, rb, w, 2 + PIECE_WORDS);
+        if (begin) *begin = w[0];
+        if (end) *end = w[1];
+        *out = select_pieces_rel(ctx, idx, pa, cols, distinct, pieces_from_words(w + 2),
+                                 &dk.p->p);
     });
 }
 

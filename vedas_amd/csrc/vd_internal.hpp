@@ -1,7 +1,6 @@
 // Internal definitions shared by the HIP translation units of libvedas_hip.
 #pragma once
 
-#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -204,9 +203,10 @@ inline bool prof_selected(const vd_ctx *ctx, const char *name) {
     return false;
 }
 
-// Launch `kernel` on the context stream; with profiling on, the launch
-// carries start and stop events (recorded by the kernel's own dispatch, no
-// marker packets around it) and is charged `bytes` algorithmic HBM bytes.
+// Launch `kernel` on the context stream; with profiling on, start and stop
+// events are recorded on the stream around it (events without system-scope
+// fences, read at the vd_prof_* calls) and it is charged `bytes`
+// algorithmic HBM bytes.
 template <typename... KArgs, typename... Args>
 inline void launch(vd_ctx *ctx, const char *name, double bytes, dim3 grid,
                    dim3 block, size_t shmem, void (*kernel)(KArgs...),
@@ -214,9 +214,10 @@ inline void launch(vd_ctx *ctx, const char *name, double bytes, dim3 grid,
     if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
     if (ctx->prof && prof_selected(ctx, name)) {
         hipEvent_t ev0 = take_event(ctx), ev1 = take_event(ctx);
-        hipExtLaunchKernelGGL(kernel, grid, block, (uint32_t)shmem, ctx->stream, ev0, ev1, 0u,
-                              std::forward<Args>(args)...);
+        VD_HIP(hipEventRecord(ev0, ctx->stream));
+        hipLaunchKernelGGL(kernel, grid, block, shmem, ctx->stream, std::forward<Args>(args)...);
         VD_HIP(hipGetLastError());
+        VD_HIP(hipEventRecord(ev1, ctx->stream));
         prof_push(ctx, name, bytes, ev0, ev1);
         return;
     }
