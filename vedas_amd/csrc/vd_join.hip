@@ -848,17 +848,17 @@ vd_rel *join_driven(vd_ctx *ctx, const vd_rel *D, int dkey, const vd_rel *Y, uin
     };
     if (nd == 0 || ny == 0 || klo > kmax) return new_out(0);
 
-    DBuf<uint2> tab(ctx, (uint64_t)kmax + 1);
-    fill_u32(ctx, reinterpret_cast<uint32_t *>(tab.p), 0, 2 * ((uint64_t)kmax + 1));
+    // one zero fill: the run table, then the repeated-row flag word
+    DBuf<uint2> tab(ctx, (uint64_t)kmax + 2);
+    fill_u32(ctx, reinterpret_cast<uint32_t *>(tab.p), 0, 2 * ((uint64_t)kmax + 2));
+    uint32_t *flag = reinterpret_cast<uint32_t *>(tab.p + kmax + 1);
     launch(ctx, "run_table", 4.0 * (double)ny, dim3((unsigned)ceil_div(ny, RT_ROWS)),
            dim3(RT_BLOCK), 0, k_run_table, (const uint32_t *)Y->cols[0], ny, kmax, tab.p);
     DBuf<uint32_t> lower(ctx, nd), off(ctx, nd);
     const uint64_t jb_tiles = ceil_div(nd, JB_TILE);
-    // tile totals, tile prefixes (+ the total), tile key bounds, flag word
-    DBuf<uint64_t> tt(ctx, 3 * jb_tiles + 2);
+    // tile totals, tile prefixes (+ the total), tile key bounds
+    DBuf<uint64_t> tt(ctx, 3 * jb_tiles + 1);
     uint64_t *ttot = tt.p, *tpre = tt.p + jb_tiles, *tkeys = tt.p + 2 * jb_tiles + 1;
-    uint32_t *flag = reinterpret_cast<uint32_t *>(tt.p + 3 * jb_tiles + 1);
-    fill_u32(ctx, flag, 0, 1);
     DriveCols dc{};
     dc.key = D->cols[dkey];
     if (check_dups) {
