@@ -97,3 +97,29 @@ def test_generator_rejects_bad_params():
     s = np.empty(10, np.uint32)
     rc = lib().vd_gen_triples(C.byref(prm), s.ctypes.data, s.ctypes.data, s.ctypes.data)
     assert rc == -1
+
+
+def test_executor_options_round_trip():
+    """The process-wide executor switches (no device work): each option
+    reads back what was set, filtered_swap takes a row count and rejects a
+    negative one, unknown names fail."""
+    from vedas_amd.engine import get_option, set_option
+    saved = {k: get_option(k) for k in ("bound_prescan", "bound_after_join", "driven_swap",
+                                        "filtered_swap")}
+    try:
+        for k in ("bound_prescan", "bound_after_join", "driven_swap"):
+            set_option(k, 0)
+            assert get_option(k) == 0
+            set_option(k, 1)
+            assert get_option(k) == 1
+        assert saved["filtered_swap"] == 1 << 20  # the default: relations of 2^20 rows
+        set_option("filtered_swap", 12345)
+        assert get_option("filtered_swap") == 12345
+        with pytest.raises(Exception):
+            set_option("filtered_swap", -1)
+        assert get_option("filtered_swap") == 12345
+        with pytest.raises(Exception):
+            set_option("no_such_option", 1)
+    finally:
+        for k, v in saved.items():
+            set_option(k, v)
