@@ -564,6 +564,17 @@ __device__ Pieces make_pieces(const uint32_t *__restrict__ ycol, uint64_t b, uin
     if (clip) {
         if (lo > hi) {
             e = b;
+        } else if (WAVE) {
+            // both ends at once, half a wave each: the first row >= lo and
+            // the first row > hi (independent searches over [b, e))
+            const bool up = dev::lane_id() >= 32;
+            const uint64_t r = dev::group_bound<32>(b, e, [&](uint64_t q) {
+                const uint32_t x = ycol[q];
+                return up ? hi < x : !(x < lo);
+            });
+            const uint64_t nb = __shfl(r, 0), ne = __shfl(r, 32);
+            b = nb;
+            e = ne;
         } else {
             const uint64_t nb = S::lower(ycol, b, e, lo);
             e = S::upper(ycol, nb, e, hi);
