@@ -66,20 +66,27 @@ __device__ __forceinline__ void block_publish(const Readback &rb) {
                            __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// One wave per lookup job (wave-cooperative searches: about log64 of the
+// range in dependent loads).
+constexpr int LK_BLOCK = 512;
+constexpr int LK_JOBS = LK_BLOCK / 64;  // jobs per block
+
 // range[2j], range[2j+1] = the rows [begin, end) matching job j (also to
 // rb.dst when the launch is one block and rb.dst is set).
-__global__ __launch_bounds__(256) void k_sel_lookup_multi(const SelJobDev *jobs, int nj,
-                                                          uint64_t *range, Readback rb) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < nj) {
+__global__ __launch_bounds__(LK_BLOCK) void k_sel_lookup_multi(const SelJobDev *jobs, int nj,
+                                                               uint64_t *range, Readback rb) {
+    const int j = (int)((blockIdx.x * blockDim.x + threadIdx.x) / 64);
+    if (j < nj) {  // the same in every lane of the wave
         const SelJobDev &J = jobs[j];
         uint64_t b, e;
-        lookup_one(J.pd, J.k1, J.k2, J.k2 != VD_UNBOUND, &b, &e);
-        range[2 * j] = b;
-        range[2 * j + 1] = e;
-        if (rb.dst) {
-            rb.dst[2 * j] = b;
-            rb.dst[2 * j + 1] = e;
+        wave_lookup(J.pd, J.k1, J.k2, J.k2 != VD_UNBOUND, &b, &e);
+        if (dev::lane_id() == 0) {
+            range[2 * j] = b;
+            range[2 * j + 1] = e;
+            if (rb.dst) {
+                rb.dst[2 * j] = b;
+                rb.dst[2 * j + 1] = e;
+            }
         }
     }
     block_publish(rb);
@@ -95,22 +102,24 @@ struct BoundJobDev {
     uint32_t k1, k2;
 };
 
-__global__ __launch_bounds__(256) void k_sel_bounds_multi(const BoundJobDev *jobs, int nj,
-                                                          uint64_t *out, Readback rb) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < nj) {
+__global__ __launch_bounds__(LK_BLOCK) void k_sel_bounds_multi(const BoundJobDev *jobs, int nj,
+                                                               uint64_t *out, Readback rb) {
+    const int j = (int)((blockIdx.x * blockDim.x + threadIdx.x) / 64);
+    if (j < nj) {  // the same in every lane of the wave
         const BoundJobDev &J = jobs[j];
         uint64_t b, e;
-        lookup_one(J.pd, J.k1, J.k2, J.k2 != VD_UNBOUND, &b, &e);
-        const uint64_t ends =
-            e > b ? ((uint64_t)J.col[b] | ((uint64_t)J.col[e - 1] << 32)) : 0;
-        out[3 * j] = b;
-        out[3 * j + 1] = e;
-        out[3 * j + 2] = ends;
-        if (rb.dst) {
-            rb.dst[3 * j] = b;
-            rb.dst[3 * j + 1] = e;
-            rb.dst[3 * j + 2] = ends;
+        wave_lookup(J.pd, J.k1, J.k2, J.k2 != VD_UNBOUND, &b, &e);
+        if (dev::lane_id() == 0) {
+            const uint64_t ends =
+                e > b ? ((uint64_t)J.col[b] | ((uint64_t)J.col[e - 1] << 32)) : 0;
+            out[3 * j] = b;
+            out[3 * j + 1] = e;
+            out[3 * j + 2] = ends;
+            if (rb.dst) {
+                rb.dst[3 * j] = b;
+                rb.dst[3 * j + 1] = e;
+                rb.dst[3 * j + 2] = ends;
+            }
         }
     }
     block_publish(rb);
@@ -559,10 +568,11 @@ std::vector<uint64_t> lookup_multi(vd_ctx *ctx, const vd_index *idx, const vd_se
     DBuf<uint64_t> range(ctx, 2 * (uint64_t)nj);
     std::vector<uint64_t> hr(2 * (size_t)nj);
     // one block: the kernel publishes the words itself
-    const bool direct = nj <= 256 && 2 * nj <= RB_WORDS;
+    if (nj == 0) return {};
+    const bool direct = nj <= LK_JOBS;
     const Readback rb = direct ? rb_next(ctx) : Readback{};
-    launch(ctx, "select_lookup", 24.0 * nj, dim3((unsigned)ceil_div(nj, 256)), dim3(256), 0,
-           k_sel_lookup_multi, (const SelJobDev *)dj.p, nj, range.p, rb);
+    launch(ctx, "select_lookup", 24.0 * nj, dim3((unsigned)ceil_div(nj, LK_JOBS)),
+           dim3(LK_BLOCK), 0, k_sel_lookup_multi, (const SelJobDev *)dj.p, nj, range.p, rb);
     if (direct) rb_wait(ctx, rb, hr.data(), 2 * nj);
     else download(ctx, hr.data(), range.p, hr.size() * sizeof(uint64_t));
     return hr;
@@ -582,10 +592,11 @@ std::vector<uint64_t> bounds_multi(vd_ctx *ctx, const vd_index *idx, const vd_se
     DBuf<uint64_t> out(ctx, 3 * (uint64_t)nj);
     std::vector<uint64_t> hr(3 * (size_t)nj);
     // one block: the kernel publishes the words itself
-    const bool direct = nj <= 256 && 3 * nj <= RB_WORDS;
+    if (nj == 0) return {};
+    const bool direct = nj <= LK_JOBS;
     const Readback rb = direct ? rb_next(ctx) : Readback{};
-    launch(ctx, "select_bounds", 32.0 * nj, dim3((unsigned)ceil_div(nj, 256)), dim3(256), 0,
-           k_sel_bounds_multi, (const BoundJobDev *)dj.p, nj, out.p, rb);
+    launch(ctx, "select_bounds", 32.0 * nj, dim3((unsigned)ceil_div(nj, LK_JOBS)),
+           dim3(LK_BLOCK), 0, k_sel_bounds_multi, (const BoundJobDev *)dj.p, nj, out.p, rb);
     if (direct) rb_wait(ctx, rb, hr.data(), 3 * nj);
     else download(ctx, hr.data(), out.p, hr.size() * sizeof(uint64_t));
     return hr;

@@ -647,24 +647,6 @@ struct KeyPieces {
     uint64_t begin, end;
 };
 
-// The key lookup of lookup_one, by one wave.
-__device__ __forceinline__ void wave_lookup(const PermDev &pd, uint32_t k1, uint32_t k2,
-                                            bool has_k2, uint64_t *b, uint64_t *e) {
-    const uint64_t i = dev::wave_bound<false>(pd.l1v, 0, pd.n_l1, k1);
-    if (i >= pd.n_l1 || pd.l1v[i] != k1) {
-        const uint64_t pos = i < pd.n_l1 ? pd.l1o[i] : pd.n;
-        *b = *e = pos;
-        return;
-    }
-    uint64_t lo = pd.l1o[i];
-    uint64_t hi = (i + 1 < pd.n_l1) ? pd.l1o[i + 1] : pd.n;
-    if (has_k2) {
-        lo = dev::wave_bound<false>(pd.y, lo, hi, k2);
-        hi = dev::wave_bound<true>(pd.y, lo, hi, k2);
-    }
-    *b = lo;
-    *e = hi;
-}
 
 __global__ __launch_bounds__(64) void k_lookup_pieces(PermDev pd, uint32_t k1, int has_k2,
                                                       uint32_t k2, int clip, uint32_t lo,

@@ -60,6 +60,25 @@ __device__ __forceinline__ void lookup_one(const PermDev &pd, uint32_t k1,
     *e = hi;
 }
 
+// The key lookup of lookup_one, by one wave.
+__device__ __forceinline__ void wave_lookup(const PermDev &pd, uint32_t k1, uint32_t k2,
+                                            bool has_k2, uint64_t *b, uint64_t *e) {
+    const uint64_t i = dev::wave_bound<false>(pd.l1v, 0, pd.n_l1, k1);
+    if (i >= pd.n_l1 || pd.l1v[i] != k1) {
+        const uint64_t pos = i < pd.n_l1 ? pd.l1o[i] : pd.n;
+        *b = *e = pos;
+        return;
+    }
+    uint64_t lo = pd.l1o[i];
+    uint64_t hi = (i + 1 < pd.n_l1) ? pd.l1o[i + 1] : pd.n;
+    if (has_k2) {
+        lo = dev::wave_bound<false>(pd.y, lo, hi, k2);
+        hi = dev::wave_bound<true>(pd.y, lo, hi, k2);
+    }
+    *b = lo;
+    *e = hi;
+}
+
 PermDev perm_dev(const vd_perm_arrays &pa);
 const vd_perm_arrays &get_perm(const vd_index *idx, int perm);
 
